@@ -1,0 +1,818 @@
+// fx_api.cpp -- host runtime behind include/ferrix_hip.h.
+//
+// Context and mesh lifetime, the host symbolic phase (structural pattern,
+// element-to-slot contribution lists), the C entry points and the step
+// driver restated from src/solver/step/static_step.rs:47-267.
+#include "fx_internal.h"
+
+#include <chrono>
+#include <exception>
+#include <new>
+#include <numeric>
+
+using fx::DBuf;
+using fx::Error;
+
+namespace {
+
+thread_local std::string g_no_ctx_error;
+
+fx_status fail_ctx(fx_ctx *ctx, fx_status code, const std::string &msg) {
+    if (ctx)
+        ctx->err = msg;
+    else
+        g_no_ctx_error = msg;
+    return code;
+}
+
+#define FX_API_BEGIN try {
+#define FX_API_END(ctx)                                                                                  \
+    }                                                                                                    \
+    catch (const Error &e) {                                                                             \
+        return fail_ctx(ctx, e.code, e.msg);                                                             \
+    }                                                                                                    \
+    catch (const std::bad_alloc &) {                                                                     \
+        return fail_ctx(ctx, FX_ERR_OOM, "host out of memory");                                          \
+    }                                                                                                    \
+    catch (const std::exception &e) {                                                                    \
+        return fail_ctx(ctx, FX_ERR_INVALID, e.what());                                                  \
+    }                                                                                                    \
+    return FX_OK;
+
+void require(bool ok, const std::string &msg) {
+    if (!ok)
+        throw Error{FX_ERR_INVALID, msg};
+}
+
+using clk = std::chrono::steady_clock;
+double since(clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); }
+
+// ---------------------------------------------------------------------------
+// symbolic phase
+// ---------------------------------------------------------------------------
+struct Symbolic {
+    std::vector<int32_t> nb_ptr, nb_col, blk_cptr, blk_cbase, nod_cptr, nod_cbase, diag_slot;
+};
+
+// Elements are visited in the given order.  Contribution bases point into
+// the element-contribution buffers: ke_base[e] (+ (a*nn+b)*9), fe_base[e]
+// (+ 3a).
+Symbolic build_symbolic(int64_t n_nodes, int64_t n_el, const int64_t *el_ptr, const int32_t *conn,
+                        const std::vector<int64_t> &ke_base, const std::vector<int64_t> &fe_base) {
+    Symbolic S;
+    // node adjacency rows (every node keeps its own diagonal block)
+    std::vector<int64_t> cnt(n_nodes + 1, 0);
+    for (int64_t a = 0; a < n_nodes; ++a)
+        cnt[a + 1] = 1;
+    for (int64_t e = 0; e < n_el; ++e) {
+        const int64_t nn = el_ptr[e + 1] - el_ptr[e];
+        for (int64_t i = el_ptr[e]; i < el_ptr[e + 1]; ++i)
+            cnt[conn[i] + 1] += nn;
+    }
+    for (int64_t a = 0; a < n_nodes; ++a)
+        cnt[a + 1] += cnt[a];
+    std::vector<int32_t> cols(cnt[n_nodes]);
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t a = 0; a < n_nodes; ++a)
+        cols[pos[a]++] = (int32_t)a;
+    for (int64_t e = 0; e < n_el; ++e)
+        for (int64_t i = el_ptr[e]; i < el_ptr[e + 1]; ++i)
+            for (int64_t j = el_ptr[e]; j < el_ptr[e + 1]; ++j)
+                cols[pos[conn[i]]++] = conn[j];
+    S.nb_ptr.assign(n_nodes + 1, 0);
+    std::vector<int32_t> out;
+    out.reserve(cols.size() / 2 + 16);
+    for (int64_t a = 0; a < n_nodes; ++a) {
+        auto b = cols.begin() + cnt[a], e = cols.begin() + cnt[a + 1];
+        std::sort(b, e);
+        auto last = std::unique(b, e);
+        out.insert(out.end(), b, last);
+        if (out.size() > (size_t)INT32_MAX / 9)
+            throw Error{FX_ERR_INVALID, "structural pattern exceeds 2^31 nonzeros"};
+        S.nb_ptr[a + 1] = (int32_t)out.size();
+    }
+    S.nb_col = std::move(out);
+    const int64_t nblk = S.nb_col.size();
+    auto block_of = [&](int32_t ga, int32_t gb) -> int64_t {
+        auto b = S.nb_col.begin() + S.nb_ptr[ga], e = S.nb_col.begin() + S.nb_ptr[ga + 1];
+        return (int64_t)(std::lower_bound(b, e, gb) - S.nb_col.begin());
+    };
+    // block contribution lists in element order
+    std::vector<int64_t> bc(nblk + 1, 0);
+    std::vector<int32_t> blk_of_pair;
+    int64_t npairs = 0;
+    for (int64_t e = 0; e < n_el; ++e) {
+        const int64_t nn = el_ptr[e + 1] - el_ptr[e];
+        npairs += nn * nn;
+    }
+    if (npairs > INT32_MAX)
+        throw Error{FX_ERR_INVALID, "too many element contributions for 32-bit lists"};
+    blk_of_pair.resize(npairs);
+    int64_t q = 0;
+    for (int64_t e = 0; e < n_el; ++e)
+        for (int64_t i = el_ptr[e]; i < el_ptr[e + 1]; ++i)
+            for (int64_t j = el_ptr[e]; j < el_ptr[e + 1]; ++j) {
+                int64_t blk = block_of(conn[i], conn[j]);
+                blk_of_pair[q++] = (int32_t)blk;
+                bc[blk + 1]++;
+            }
+    for (int64_t b = 0; b < nblk; ++b)
+        bc[b + 1] += bc[b];
+    S.blk_cptr.assign(bc.begin(), bc.end());
+    S.blk_cbase.resize(npairs);
+    std::vector<int64_t> fill(bc.begin(), bc.end() - 1);
+    q = 0;
+    for (int64_t e = 0; e < n_el; ++e) {
+        const int64_t nn = el_ptr[e + 1] - el_ptr[e];
+        for (int64_t a = 0; a < nn; ++a)
+            for (int64_t b = 0; b < nn; ++b) {
+                int64_t v = ke_base[e] + (a * nn + b) * 9;
+                if (v > INT32_MAX)
+                    throw Error{FX_ERR_INVALID, "element contribution buffer exceeds 32-bit indexing"};
+                S.blk_cbase[fill[blk_of_pair[q++]]++] = (int32_t)v;
+            }
+    }
+    // node lists for the internal force
+    std::vector<int64_t> nc(n_nodes + 1, 0);
+    for (int64_t e = 0; e < n_el; ++e)
+        for (int64_t i = el_ptr[e]; i < el_ptr[e + 1]; ++i)
+            nc[conn[i] + 1]++;
+    for (int64_t a = 0; a < n_nodes; ++a)
+        nc[a + 1] += nc[a];
+    S.nod_cptr.assign(nc.begin(), nc.end());
+    S.nod_cbase.resize(nc[n_nodes]);
+    std::vector<int64_t> nf(nc.begin(), nc.end() - 1);
+    for (int64_t e = 0; e < n_el; ++e)
+        for (int64_t i = el_ptr[e]; i < el_ptr[e + 1]; ++i)
+            S.nod_cbase[nf[conn[i]]++] = (int32_t)(fe_base[e] + 3 * (i - el_ptr[e]));
+    // diagonal slots
+    S.diag_slot.resize(3 * n_nodes);
+    for (int64_t a = 0; a < n_nodes; ++a) {
+        const int64_t b0 = S.nb_ptr[a], deg = S.nb_ptr[a + 1] - b0;
+        const int64_t k = block_of((int32_t)a, (int32_t)a) - b0;
+        for (int p = 0; p < 3; ++p)
+            S.diag_slot[3 * a + p] = (int32_t)(9 * b0 + p * 3 * deg + 3 * k + p);
+    }
+    return S;
+}
+
+void set_device(fx_ctx *ctx) { FX_HIP(hipSetDevice(ctx->device)); }
+
+// ---------------------------------------------------------------------------
+// amplitudes (amplitude.rs:39-98)
+// ---------------------------------------------------------------------------
+double interp_series(double t, const double *times, const double *values, int64_t n, double sx, double sy) {
+    if (n == 0)
+        return 0.0;
+    const double tt = t - sx;
+    int64_t idx = std::lower_bound(times, times + n, tt) - times;  // partition_point(x < tt)
+    if (idx == 0)
+        return values[0] + sy;
+    if (idx >= n)
+        return values[n - 1] + sy;
+    const double t0 = times[idx - 1], t1 = times[idx], v0 = values[idx - 1], v1 = values[idx];
+    const double slope = (v1 - v0) / (t1 - t0);
+    return v0 + slope * (tt - t0) + sy;
+}
+
+double amp_y(const fx_amplitudes &A, int a, const fx_timer &t, int origin, int current_step) {
+    if (A.n > 0 && a > 0 && A.has_data[a]) {
+        const int64_t b = A.ptr[a], n = A.ptr[a + 1] - A.ptr[a];
+        return interp_series(A.total_time[a] ? t.global_time : t.local_time, A.times + b, A.values + b, n,
+                             A.shift_x[a], A.shift_y[a]);
+    }
+    if (current_step > origin)
+        return 1.0;
+    return t.local_time / t.local_max;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *fx_version(void) { return "ferrix_hip 0.1 (gfx950)"; }
+
+const char *fx_last_error(const fx_ctx *ctx) { return ctx ? ctx->err.c_str() : g_no_ctx_error.c_str(); }
+
+fx_status fx_ctx_create(int device, fx_ctx **out) {
+    fx_ctx *ctx = nullptr;
+    FX_API_BEGIN
+    require(out != nullptr, "fx_ctx_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        throw Error{FX_ERR_NO_DEVICE, "no HIP device available"};
+    if (device < 0 || device >= count)
+        throw Error{FX_ERR_NO_DEVICE, "device index out of range"};
+    hipDeviceProp_t prop;
+    FX_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
+        throw Error{FX_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", the kernels target gfx950"};
+    ctx = new fx_ctx();
+    ctx->device = device;
+    ctx->n_cu = prop.multiProcessorCount;
+    FX_HIP(hipSetDevice(device));
+    FX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    fx::GaussTable t4 = fx::gauss_table(4), t20 = fx::gauss_table(20);
+    auto &T = ctx->tables;
+    T.n4.upload(t4.N, ctx->stream);
+    T.dn4.upload(t4.dN, ctx->stream);
+    T.w4.upload(t4.w, ctx->stream);
+    T.n20.upload(t20.N, ctx->stream);
+    T.dn20.upload(t20.dN, ctx->stream);
+    T.w20.upload(t20.w, ctx->stream);
+    FX_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx;
+    ctx = nullptr;
+    }
+    catch (const Error &e) {
+        if (ctx) {
+            if (ctx->stream)
+                (void)hipStreamDestroy(ctx->stream);
+            delete ctx;
+        }
+        return fail_ctx(nullptr, e.code, e.msg);
+    }
+    catch (const std::exception &e) {
+        delete ctx;
+        return fail_ctx(nullptr, FX_ERR_INVALID, e.what());
+    }
+    return FX_OK;
+}
+
+void fx_ctx_destroy(fx_ctx *ctx) {
+    if (!ctx)
+        return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto *b : {&ctx->tables.n4, &ctx->tables.dn4, &ctx->tables.w4, &ctx->tables.n20, &ctx->tables.dn20,
+                    &ctx->tables.w20})
+        b->release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+fx_status fx_mesh_upload(fx_ctx *ctx, int64_t n_nodes, const double *xyz, const double *node_temp, int64_t n_el,
+                         const int64_t *el_ptr, const int32_t *conn, const int32_t *mat,
+                         const fx_material_table *mats, fx_mesh **out) {
+    if (!ctx)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "fx_mesh_upload: ctx is NULL");
+    fx_mesh *m = nullptr;
+    FX_API_BEGIN
+    require(out && xyz && node_temp && el_ptr && conn && mat && mats, "fx_mesh_upload: NULL argument");
+    *out = nullptr;
+    require(n_nodes > 0 && n_nodes <= INT32_MAX / 3, "fx_mesh_upload: n_nodes out of range");
+    require(n_el >= 0, "fx_mesh_upload: n_el < 0");
+    require(el_ptr[0] == 0, "fx_mesh_upload: el_ptr[0] != 0");
+    require(mats->n_mat >= 1, "fx_mesh_upload: no material");
+    for (int64_t e = 0; e < n_el; ++e) {
+        const int64_t nn = el_ptr[e + 1] - el_ptr[e];
+        require(nn == 4 || nn == 20, "element " + std::to_string(e) + ": only C3D4 (4 nodes) and C3D20 (20 nodes)");
+        for (int64_t i = el_ptr[e]; i < el_ptr[e + 1]; ++i)
+            require(conn[i] >= 0 && conn[i] < n_nodes, "element " + std::to_string(e) + ": node index out of range");
+        require(mat[e] >= 0 && mat[e] < mats->n_mat, "element " + std::to_string(e) + " has no material assigned.");
+    }
+    for (int k = 0; k < mats->n_mat; ++k)
+        require(mats->n_e[k] <= FX_LUT_MAX && mats->n_nu[k] <= FX_LUT_MAX && mats->n_e[k] >= 0 && mats->n_nu[k] >= 0,
+                "material LUT size out of range");
+    set_device(ctx);
+    hipStream_t s = ctx->stream;
+    m = new fx_mesh();
+    m->ctx = ctx;
+    m->n_nodes = n_nodes;
+    m->n_dof = 3 * n_nodes;
+    m->n_el = n_el;
+    // element blocks by type (order kept within a type), contribution bases
+    std::vector<int32_t> c4, c20, m4, m20;
+    std::vector<int64_t> g4, g20, ke_base(n_el), fe_base(n_el);
+    for (int64_t e = 0; e < n_el; ++e) {
+        const int64_t nn = el_ptr[e + 1] - el_ptr[e];
+        auto &c = nn == 4 ? c4 : c20;
+        c.insert(c.end(), conn + el_ptr[e], conn + el_ptr[e + 1]);
+        (nn == 4 ? m4 : m20).push_back(mat[e]);
+        (nn == 4 ? g4 : g20).push_back(e);
+    }
+    m->blk4.nn = 4;
+    m->blk4.n_el = (int64_t)g4.size();
+    m->blk20.nn = 20;
+    m->blk20.n_el = (int64_t)g20.size();
+    m->blk4.ke_offset = 0;
+    m->blk20.ke_offset = m->blk4.n_el * 144;
+    m->blk4.fe_offset = 0;
+    m->blk20.fe_offset = m->blk4.n_el * 12;
+    m->ke_total = m->blk20.ke_offset + m->blk20.n_el * 3600;
+    m->fe_total = m->blk20.fe_offset + m->blk20.n_el * 60;
+    require(m->ke_total < INT32_MAX, "mesh too large for 32-bit contribution indexing");
+    for (size_t k = 0; k < g4.size(); ++k) {
+        ke_base[g4[k]] = m->blk4.ke_offset + (int64_t)k * 144;
+        fe_base[g4[k]] = m->blk4.fe_offset + (int64_t)k * 12;
+    }
+    for (size_t k = 0; k < g20.size(); ++k) {
+        ke_base[g20[k]] = m->blk20.ke_offset + (int64_t)k * 3600;
+        fe_base[g20[k]] = m->blk20.fe_offset + (int64_t)k * 60;
+    }
+    Symbolic S = build_symbolic(n_nodes, n_el, el_ptr, conn, ke_base, fe_base);
+    m->nblk = (int64_t)S.nb_col.size();
+    m->nnz_struct = 9 * m->nblk;
+    m->h_nb_ptr = S.nb_ptr;
+    m->h_nb_col = S.nb_col;
+    // uploads
+    m->xyz.upload(xyz, 3 * n_nodes, s);
+    m->temp.upload(node_temp, n_nodes, s);
+    std::vector<fx::MatLut> lut(mats->n_mat);
+    for (int k = 0; k < mats->n_mat; ++k) {
+        fx::MatLut &L = lut[k];
+        memset(&L, 0, sizeof L);
+        L.n_e = mats->n_e[k];
+        L.n_nu = mats->n_nu[k];
+        for (int j = 0; j < FX_LUT_MAX; ++j) {
+            L.t_e[j] = mats->t_e[k * FX_LUT_MAX + j];
+            L.v_e[j] = mats->v_e[k * FX_LUT_MAX + j];
+            L.t_nu[j] = mats->t_nu[k * FX_LUT_MAX + j];
+            L.v_nu[j] = mats->v_nu[k * FX_LUT_MAX + j];
+        }
+    }
+    m->mats.upload(lut, s);
+    m->n_mat = mats->n_mat;
+    m->blk4.conn.upload(c4, s);
+    m->blk4.mat.upload(m4, s);
+    m->blk4.gpos.upload(g4, s);
+    m->blk20.conn.upload(c20, s);
+    m->blk20.mat.upload(m20, s);
+    m->blk20.gpos.upload(g20, s);
+    m->nb_ptr.upload(S.nb_ptr, s);
+    m->nb_col.upload(S.nb_col, s);
+    m->blk_cptr.upload(S.blk_cptr, s);
+    m->blk_cbase.upload(S.blk_cbase, s);
+    m->nod_cptr.upload(S.nod_cptr, s);
+    m->nod_cbase.upload(S.nod_cbase, s);
+    m->diag_slot.upload(S.diag_slot, s);
+    m->ke.alloc(std::max<int64_t>(1, m->ke_total));
+    m->fe.alloc(std::max<int64_t>(1, m->fe_total));
+    m->kval.alloc(m->nnz_struct);
+    m->keep.alloc(m->nnz_struct);
+    FX_HIP(hipMemsetAsync(m->kval.p, 0, m->kval.bytes(), s));
+    FX_HIP(hipMemsetAsync(m->keep.p, 0, m->keep.bytes(), s));
+    m->inv_diag.alloc(m->n_dof);
+    m->max_diag_bits.alloc(1);
+    m->status.alloc(1);
+    m->pcg.alloc(1);
+    m->part.alloc(3 * (size_t)(m->n_dof + 8192));
+    m->counters.alloc(8);
+    FX_HIP(hipMemsetAsync(m->counters.p, 0, m->counters.bytes(), s));
+    for (auto *b : {&m->u, &m->f_int, &m->f_ext, &m->rhs, &m->du})
+        b->alloc(m->n_dof);
+    FX_HIP(hipMemsetAsync(m->u.p, 0, m->u.bytes(), s));
+    m->red2.alloc(2);
+    fx::reset_status(m);
+    FX_HIP(hipStreamSynchronize(s));
+    m->device_bytes = 0;
+    for (size_t b : {m->xyz.bytes(), m->temp.bytes(), m->mats.bytes(), m->blk4.conn.bytes(), m->blk20.conn.bytes(),
+                     m->nb_ptr.bytes(), m->nb_col.bytes(), m->blk_cptr.bytes(), m->blk_cbase.bytes(),
+                     m->nod_cptr.bytes(), m->nod_cbase.bytes(), m->diag_slot.bytes(), m->ke.bytes(), m->fe.bytes(),
+                     m->kval.bytes(), m->keep.bytes(), m->part.bytes()})
+        m->device_bytes += (int64_t)b;
+    m->device_bytes += 13 * m->n_dof * (int64_t)sizeof(double);
+    *out = m;
+    m = nullptr;
+    }
+    catch (const Error &e) {
+        delete m;
+        return fail_ctx(ctx, e.code, e.msg);
+    }
+    catch (const std::bad_alloc &) {
+        delete m;
+        return fail_ctx(ctx, FX_ERR_OOM, "host out of memory");
+    }
+    catch (const std::exception &e) {
+        delete m;
+        return fail_ctx(ctx, FX_ERR_INVALID, e.what());
+    }
+    return FX_OK;
+}
+
+void fx_mesh_destroy(fx_mesh *m) {
+    if (!m)
+        return;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+}
+
+fx_status fx_mesh_info(const fx_mesh *m, int64_t *n_dof, int64_t *nnz_struct, int64_t *device_bytes) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "fx_mesh_info: mesh is NULL");
+    if (n_dof)
+        *n_dof = m->n_dof;
+    if (nnz_struct)
+        *nnz_struct = m->nnz_struct;
+    if (device_bytes)
+        *device_bytes = m->device_bytes;
+    return FX_OK;
+}
+
+fx_status fx_state_set_u(fx_mesh *m, const double *u) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(u != nullptr, "fx_state_set_u: u is NULL");
+    set_device(m->ctx);
+    FX_HIP(hipMemcpyAsync(m->u.p, u, m->n_dof * sizeof(double), hipMemcpyHostToDevice, m->ctx->stream));
+    FX_HIP(hipStreamSynchronize(m->ctx->stream));
+    FX_API_END(m->ctx)
+}
+
+fx_status fx_state_get_u(fx_mesh *m, double *u) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(u != nullptr, "fx_state_get_u: u is NULL");
+    set_device(m->ctx);
+    FX_HIP(hipMemcpyAsync(u, m->u.p, m->n_dof * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+    FX_HIP(hipStreamSynchronize(m->ctx->stream));
+    FX_API_END(m->ctx)
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// shared pieces of the assembly seam
+// ---------------------------------------------------------------------------
+namespace {
+
+double assemble_k(fx_mesh *m, bool nlgeom) {
+    fx::reset_status(m);
+    fx::launch_element(m, true, false, nlgeom);
+    fx::launch_assemble(m);
+    fx::check_status(m, "Assembler::assemble");
+    m->last_max_diag = fx::device_max_diag(m);
+    m->k_valid = true;
+    return m->last_max_diag;
+}
+
+void internal_force_dev(fx_mesh *m, bool nlgeom) {
+    fx::reset_status(m);
+    fx::launch_element(m, false, true, nlgeom);
+    fx::launch_gather_f(m);
+}
+
+// BC records grouped per DOF in first-appearance order, records in deck order
+struct BcGroups {
+    std::vector<int32_t> ud_dof, ud_ptr, rec_order;
+};
+
+BcGroups group_bcs(int64_t n, const int64_t *dofs, int64_t n_dof) {
+    BcGroups g;
+    std::vector<int64_t> first(n_dof, -1);
+    std::vector<std::vector<int32_t>> recs;
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t d = dofs[k];
+        if (d < 0 || d >= n_dof)
+            throw Error{FX_ERR_INVALID, "boundary condition DOF out of range"};
+        if (first[d] < 0) {
+            first[d] = (int64_t)recs.size();
+            recs.emplace_back();
+            g.ud_dof.push_back((int32_t)d);
+        }
+        recs[first[d]].push_back((int32_t)k);
+    }
+    g.ud_ptr.push_back(0);
+    for (auto &r : recs) {
+        g.rec_order.insert(g.rec_order.end(), r.begin(), r.end());
+        g.ud_ptr.push_back((int32_t)g.rec_order.size());
+    }
+    return g;
+}
+
+struct DevBc {
+    DBuf<int32_t> ud_dof, ud_ptr, rec_order;
+    DBuf<double> target;
+    int64_t n_unique = 0;
+    void upload(const BcGroups &g, hipStream_t s) {
+        ud_dof.upload(g.ud_dof, s);
+        ud_ptr.upload(g.ud_ptr, s);
+        rec_order.upload(g.rec_order, s);
+        n_unique = (int64_t)g.ud_dof.size();
+    }
+};
+
+void check_solve_opts(const fx_solve_opts &o) {
+    require(o.precision == FX_PREC_FP64 || o.precision == FX_PREC_MIXED, "solve: unknown precision");
+    require(o.stop_mode == FX_STOP_ABS_SQRT_RZ || o.stop_mode == FX_STOP_REL_R2, "solve: unknown stop mode");
+    require(o.max_iter >= 0, "solve: max_iter < 0");
+    require(o.spmv_group == 0 || o.spmv_group == 2 || o.spmv_group == 4 || o.spmv_group == 8 ||
+                o.spmv_group == 16 || o.spmv_group == 32 || o.spmv_group == 64,
+            "solve: spmv_group must be 0 or a power of two in 2..64");
+}
+
+std::string solve_error(fx_status st, const fx_solve_opts &o, const fx_solve_report &r) {
+    if (st == FX_ERR_NOT_CONVERGED)
+        return "PCG did not converge after " + std::to_string(o.max_iter) + " iterations (criterion " +
+               std::to_string(r.final_criterion) + ")";
+    return "PCG breakdown: p.Ap <= 0 or not finite after " + std::to_string(r.iterations) + " iterations";
+}
+
+}  // namespace
+
+extern "C" {
+
+void fx_solve_opts_default(fx_solve_opts *o) {
+    if (!o)
+        return;
+    memset(o, 0, sizeof *o);
+    o->precision = FX_PREC_FP64;
+    o->stop_mode = FX_STOP_ABS_SQRT_RZ;
+    o->tol = 1e-8;
+    o->max_iter = 1000;
+    o->inner_tol = 1e-3;
+    o->max_refine = 50;
+    o->spmv_group = 0;
+}
+
+fx_status fx_assemble(fx_mesh *m, int32_t nlgeom, double *max_diag_out, int64_t *nnz_out) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    set_device(m->ctx);
+    double md = assemble_k(m, nlgeom != 0);
+    if (max_diag_out)
+        *max_diag_out = md;
+    if (nnz_out)
+        *nnz_out = fx::count_kept(m);
+    FX_API_END(m->ctx)
+}
+
+fx_status fx_internal_force(fx_mesh *m, int32_t nlgeom, double *f_int_out) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    set_device(m->ctx);
+    internal_force_dev(m, nlgeom != 0);
+    fx::check_status(m, "Assembler::assemble_internal_force");
+    if (f_int_out) {
+        FX_HIP(hipMemcpyAsync(f_int_out, m->f_int.p, m->n_dof * sizeof(double), hipMemcpyDeviceToHost,
+                              m->ctx->stream));
+        FX_HIP(hipStreamSynchronize(m->ctx->stream));
+    }
+    FX_API_END(m->ctx)
+}
+
+fx_status fx_matrix_get(fx_mesh *m, int64_t *row_ptr, int32_t *col_idx, double *values) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(row_ptr && col_idx, "fx_matrix_get: NULL argument");
+    require(m->k_valid, "fx_matrix_get: no assembled matrix");
+    set_device(m->ctx);
+    std::vector<uint8_t> keep(m->nnz_struct);
+    std::vector<double> val(values ? m->nnz_struct : 0);
+    FX_HIP(hipMemcpyAsync(keep.data(), m->keep.p, keep.size(), hipMemcpyDeviceToHost, m->ctx->stream));
+    if (values)
+        FX_HIP(hipMemcpyAsync(val.data(), m->kval.p, val.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              m->ctx->stream));
+    FX_HIP(hipStreamSynchronize(m->ctx->stream));
+    int64_t out = 0;
+    row_ptr[0] = 0;
+    for (int64_t a = 0; a < m->n_nodes; ++a) {
+        const int64_t b0 = m->h_nb_ptr[a], deg = m->h_nb_ptr[a + 1] - b0;
+        for (int p = 0; p < 3; ++p) {
+            const int64_t base = 9 * b0 + p * 3 * deg;
+            for (int64_t k = 0; k < 3 * deg; ++k) {
+                if (!keep[base + k])
+                    continue;
+                col_idx[out] = 3 * m->h_nb_col[b0 + k / 3] + (int32_t)(k % 3);
+                if (values)
+                    values[out] = val[base + k];
+                ++out;
+            }
+            row_ptr[3 * a + p + 1] = out;
+        }
+    }
+    FX_API_END(m->ctx)
+}
+
+fx_status fx_apply_penalty(fx_mesh *m, int64_t n_bc, const int64_t *dofs, const double *targets,
+                           double *rhs_inout, double *penalty_out) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(m->k_valid, "fx_apply_penalty: assemble first");
+    require(n_bc == 0 || (dofs && targets), "fx_apply_penalty: NULL argument");
+    set_device(m->ctx);
+    hipStream_t s = m->ctx->stream;
+    double pen = m->last_max_diag > 0.0 ? m->last_max_diag * fx::kPenaltyScale : 0.0;
+    if (penalty_out)
+        *penalty_out = pen;
+    if (pen > 0.0 && n_bc > 0) {
+        BcGroups g = group_bcs(n_bc, dofs, m->n_dof);
+        DevBc d;
+        d.upload(g, s);
+        d.target.upload(targets, n_bc, s);
+        if (rhs_inout)
+            FX_HIP(hipMemcpyAsync(m->rhs.p, rhs_inout, m->n_dof * sizeof(double), hipMemcpyHostToDevice, s));
+        fx::launch_penalty(m, d.n_unique, d.ud_dof.p, d.ud_ptr.p, d.target.p, d.rec_order.p,
+                           rhs_inout ? m->rhs.p : nullptr, m->u.p);
+        if (rhs_inout)
+            FX_HIP(hipMemcpyAsync(rhs_inout, m->rhs.p, m->n_dof * sizeof(double), hipMemcpyDeviceToHost, s));
+        FX_HIP(hipStreamSynchronize(s));
+    }
+    FX_API_END(m->ctx)
+}
+
+fx_status fx_solve(fx_mesh *m, const double *rhs, double *x_out, const fx_solve_opts *opts,
+                   fx_solve_report *report) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(rhs && x_out, "fx_solve: NULL argument");
+    require(m->k_valid, "fx_solve: assemble first");
+    fx_solve_opts o;
+    if (opts)
+        o = *opts;
+    else
+        fx_solve_opts_default(&o);
+    check_solve_opts(o);
+    set_device(m->ctx);
+    hipStream_t s = m->ctx->stream;
+    FX_HIP(hipMemcpyAsync(m->rhs.p, rhs, m->n_dof * sizeof(double), hipMemcpyHostToDevice, s));
+    fx_solve_report r{};
+    fx_status st = fx::pcg_solve(m, m->rhs.p, m->du.p, o, &r);
+    FX_HIP(hipMemcpyAsync(x_out, m->du.p, m->n_dof * sizeof(double), hipMemcpyDeviceToHost, s));
+    FX_HIP(hipStreamSynchronize(s));
+    if (report)
+        *report = r;
+    if (st != FX_OK)
+        throw Error{st, solve_error(st, o, r)};
+    FX_API_END(m->ctx)
+}
+
+// ---------------------------------------------------------------------------
+// StaticStep::solve (static_step.rs:47-267)
+// ---------------------------------------------------------------------------
+fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_step_report *report) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(sd && timer, "fx_static_step: NULL argument");
+    require(sd->n_loads == 0 || (sd->load_dof && sd->load_val && sd->load_amp && sd->load_origin),
+            "fx_static_step: load arrays missing");
+    require(sd->n_bcs == 0 || (sd->bc_dof && sd->bc_val && sd->bc_amp && sd->bc_origin),
+            "fx_static_step: BC arrays missing");
+    const int n_amp = sd->amps.n;
+    auto amp_ok = [&](int a) { return a == 0 || (a > 0 && a < n_amp); };
+    for (int64_t k = 0; k < sd->n_loads; ++k)
+        require(sd->load_dof[k] >= 0 && sd->load_dof[k] < m->n_dof && amp_ok(sd->load_amp[k]),
+                "fx_static_step: load record out of range");
+    for (int64_t k = 0; k < sd->n_bcs; ++k)
+        require(sd->bc_dof[k] >= 0 && sd->bc_dof[k] < m->n_dof && amp_ok(sd->bc_amp[k]),
+                "fx_static_step: boundary record out of range");
+    const fx_solve_opts &opts = sd->solver == FX_SOLVER_ITERATIVE ? sd->solve_opts : sd->direct_opts;
+    check_solve_opts(opts);
+    set_device(m->ctx);
+    hipStream_t s = m->ctx->stream;
+    const int64_t n = m->n_dof;
+    const bool nlgeom = sd->nlgeom != 0;
+    const int step_id = sd->step_id;
+    auto t_start = clk::now();
+    fx_step_report rep{};
+
+    // Step::solve: timer.new_step(time_period)
+    timer->local_time = 0.0;
+    timer->local_max = sd->time_period;
+
+    // per-step constant data: BC groups, BC mask, load/target buffers
+    BcGroups g = group_bcs(sd->n_bcs, sd->bc_dof, n);
+    DevBc dbc;
+    dbc.upload(g, s);
+    std::vector<uint8_t> mask(n, 0);
+    for (int64_t k = 0; k < sd->n_bcs; ++k)
+        mask[sd->bc_dof[k]] = 1;
+    DBuf<uint8_t> dmask;
+    dmask.upload(mask, s);
+    std::vector<double> targets(sd->n_bcs);
+    dbc.target.alloc(std::max<int64_t>(1, sd->n_bcs));
+    std::vector<double> fext_h(n, 0.0);
+    std::vector<int32_t> touched;
+    std::vector<char> is_touched(n, 0);
+    for (int64_t k = 0; k < sd->n_loads; ++k)
+        if (!is_touched[sd->load_dof[k]]) {
+            is_touched[sd->load_dof[k]] = 1;
+            touched.push_back((int32_t)sd->load_dof[k]);
+        }
+    DBuf<int32_t> d_touched;
+    d_touched.upload(touched, s);
+    DBuf<double> d_tval, u_start, norms;
+    d_tval.alloc(std::max<size_t>(1, touched.size()));
+    u_start.alloc(n);
+    norms.alloc(2);
+    std::vector<double> tval(touched.size());
+    FX_HIP(hipMemcpyAsync(u_start.p, m->u.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+
+    double current_time = 0.0;
+    int64_t n_inc = 0;
+    double dt = sd->initial_time_increment;
+    while (current_time < sd->time_period) {
+        ++n_inc;
+        if (n_inc > sd->max_increments)
+            throw Error{FX_ERR_NOT_CONVERGED,
+                        "Maximum increment count of " + std::to_string(sd->max_increments) + " exceeded."};
+        if (current_time + dt > sd->time_period)
+            dt = sd->time_period - current_time;
+        timer->global_time += dt;  // timer.new_increment, never rolled back
+        timer->local_time += dt;
+        rep.increments++;
+        FX_HIP(hipMemcpyAsync(m->u.p, u_start.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        bool converged = false;
+        for (int it = 0; it < 15; ++it) {
+            rep.nr_iterations++;
+            auto t_asm = clk::now();
+            // 1. external force from amplitude-scaled loads, deck order
+            for (int32_t d : touched)
+                fext_h[d] = 0.0;
+            for (int64_t k = 0; k < sd->n_loads; ++k)
+                fext_h[sd->load_dof[k]] +=
+                    amp_y(sd->amps, sd->load_amp[k], *timer, sd->load_origin[k], step_id) * sd->load_val[k];
+            for (size_t k = 0; k < touched.size(); ++k)
+                tval[k] = fext_h[touched[k]];
+            if (!touched.empty())
+                FX_HIP(hipMemcpyAsync(d_tval.p, tval.data(), tval.size() * sizeof(double), hipMemcpyHostToDevice,
+                                      s));
+            fx::launch_fill(m, m->f_ext.p, n, 0.0);
+            fx::launch_scatter(m, (int64_t)touched.size(), d_touched.p, d_tval.p, m->f_ext.p);
+            // 2. internal force at u_curr (coordinates move under NLGEOM);
+            //    the thermal force is exactly zero (T never changes)
+            internal_force_dev(m, nlgeom);
+            // 3./4. residual and BC-masked norms
+            fx::launch_residual(m, m->f_ext.p, m->f_int.p, m->rhs.p, n);
+            fx::launch_masked_norms(m, m->rhs.p, m->f_ext.p, dmask.p, norms.p);
+            double h2[2];
+            FX_HIP(hipMemcpyAsync(h2, norms.p, sizeof h2, hipMemcpyDeviceToHost, s));
+            fx::check_status(m, "Assembler::assemble_internal_force");  // synchronizes
+            const double res_norm = std::sqrt(h2[0]), f_norm = std::sqrt(h2[1]);
+            const double rel = f_norm > 1e-6 ? res_norm / f_norm : res_norm;
+            rep.last_rel_residual = rel;
+            if (rel < 1e-3) {
+                converged = true;
+                rep.seconds_assembly += since(t_asm);
+                break;
+            }
+            // 5. tangent stiffness, 6. penalty boundary conditions
+            double md = assemble_k(m, nlgeom);
+            if (md > 0.0 && dbc.n_unique > 0) {
+                for (int64_t k = 0; k < sd->n_bcs; ++k)
+                    targets[k] = amp_y(sd->amps, sd->bc_amp[k], *timer, sd->bc_origin[k], step_id) * sd->bc_val[k];
+                FX_HIP(hipMemcpyAsync(dbc.target.p, targets.data(), targets.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, s));
+                fx::launch_penalty(m, dbc.n_unique, dbc.ud_dof.p, dbc.ud_ptr.p, dbc.target.p, dbc.rec_order.p,
+                                   m->rhs.p, m->u.p);
+            }
+            FX_HIP(hipStreamSynchronize(s));
+            rep.seconds_assembly += since(t_asm);
+            // 7. solve
+            auto t_sol = clk::now();
+            fx_solve_report sr{};
+            fx_status st = fx::pcg_solve(m, m->rhs.p, m->du.p, opts, &sr);
+            rep.seconds_solve += since(t_sol);
+            rep.linear_solves++;
+            rep.pcg_iterations += sr.iterations;
+            rep.outer_spmv += sr.outer_spmv;
+            if (st != FX_OK) {
+                if (report) {
+                    rep.seconds = since(t_start);
+                    *report = rep;
+                }
+                throw Error{st, solve_error(st, opts, sr)};
+            }
+            // 8. u_curr += du
+            fx::axpy_dev(m, n, 1.0, m->du.p, m->u.p);
+        }
+        if (converged) {
+            current_time += dt;
+            FX_HIP(hipMemcpyAsync(u_start.p, m->u.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        } else {
+            rep.cutbacks++;
+            dt /= 2.0;
+            if (dt < sd->min_time_increment) {
+                FX_HIP(hipMemcpyAsync(m->u.p, u_start.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+                FX_HIP(hipStreamSynchronize(s));
+                if (report) {
+                    rep.seconds = since(t_start);
+                    *report = rep;
+                }
+                throw Error{FX_ERR_NOT_CONVERGED, "Minimum time increment reached. Convergence failed."};
+            }
+        }
+    }
+    // solution_state.displacements = u_step_start
+    FX_HIP(hipMemcpyAsync(m->u.p, u_start.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    FX_HIP(hipStreamSynchronize(s));
+    rep.seconds = since(t_start);
+    if (report)
+        *report = rep;
+    FX_API_END(m->ctx)
+}
+
+}  // extern "C"

@@ -1,0 +1,339 @@
+// fx_assemble.hip -- deterministic global assembly and the step's vector kernels.
+//
+// Reference:
+//   triplet scatter with |v| > 1e-12 and per-element max diagonal
+//                                      src/solver/assembler.rs:119-148
+//   duplicate summation (TriMat::to_csr) src/solver/step/static_step.rs:198
+//   f_int scatter-add                  src/solver/assembler.rs:229-237
+//   residual, BC-masked norms          src/solver/step/static_step.rs:138-164
+//   penalty on K and on the residual   src/solver/step/static_step.rs:183-195
+//   Jacobi inverse diagonal            src/solver/preconditioner.rs:39-47
+//
+// Assembly is a gather, not a scatter: the host symbolic phase lists, for
+// every 3x3 node block of the structural pattern, the element contributions
+// in element order.  One wavefront owns one DOF row; each lane owns slots of
+// that row and sums the kept contributions in list order, which is the order
+// in which the reference's triplets would be summed when elements are
+// visited by ascending ID.  No floating-point atomics.  The structural
+// pattern stays the compute pattern; slots whose contributions were all
+// dropped hold 0.0 and keep = 0, so the exported CSR matches the filtered
+// one while SpMV sums are unchanged (0 * x adds nothing).
+#include "fx_internal.h"
+
+namespace fx {
+namespace {
+
+constexpr int kWave = 64;
+
+__global__ __launch_bounds__(256) void k_assemble(int64_t n_dof, const int32_t *__restrict__ nb_ptr,
+                                                  const int32_t *__restrict__ nb_col,
+                                                  const int32_t *__restrict__ blk_cptr,
+                                                  const int32_t *__restrict__ blk_cbase,
+                                                  const double *__restrict__ ke, double *__restrict__ kval,
+                                                  uint8_t *__restrict__ keep,
+                                                  unsigned long long *__restrict__ max_diag_bits) {
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= n_dof)
+        return;
+    const int a = (int)(row / 3), p = (int)(row % 3);
+    const int b0 = nb_ptr[a], deg = nb_ptr[a + 1] - b0;
+    const int64_t base = 9 * (int64_t)b0 + (int64_t)p * 3 * deg;
+    double dmax = 0.0;
+    for (int k = lane; k < 3 * deg; k += kWave) {
+        const int kb = k / 3, q = k % 3;
+        const int blk = b0 + kb;
+        const bool diag = (q == p) && (nb_col[blk] == a);
+        double s = 0.0;
+        uint8_t kept = 0;
+        const int c1 = blk_cptr[blk + 1];
+        for (int c = blk_cptr[blk]; c < c1; ++c) {
+            double v = ke[(int64_t)blk_cbase[c] + 3 * p + q];
+            if (fabs(v) > kDropTol) {
+                s += v;
+                kept = 1;
+                if (diag)
+                    dmax = fmax(dmax, fabs(v));
+            }
+        }
+        kval[base + k] = s;
+        keep[base + k] = kept;
+    }
+    if (dmax > 0.0)
+        atomicMax(max_diag_bits, (unsigned long long)__double_as_longlong(dmax));
+}
+
+// f_int[3a+p] = sum over the elements of node a, in element order.
+__global__ void k_gather_f(int64_t n_dof, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cbase,
+                           const double *__restrict__ fe, double *__restrict__ f) {
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n_dof)
+        return;
+    const int a = (int)(d / 3), p = (int)(d % 3);
+    double s = 0.0;
+    for (int c = cptr[a]; c < cptr[a + 1]; ++c)
+        s += fe[(int64_t)cbase[c] + p];
+    f[d] = s;
+}
+
+__global__ void k_jacobi(int64_t n, const int32_t *__restrict__ diag_slot, const double *__restrict__ kval,
+                         double *__restrict__ inv, float *__restrict__ inv32) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double d = kval[diag_slot[i]];
+    double v = fabs(d) > kJacobiGuard ? 1.0 / d : 1.0;
+    inv[i] = v;
+    if (inv32)
+        inv32[i] = (float)v;
+}
+
+__global__ void k_convert32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride)
+        b[i] = (float)a[i];
+}
+
+__global__ void k_residual(int64_t n, const double *__restrict__ fe, const double *__restrict__ fi,
+                           double *__restrict__ r) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        r[i] = fe[i] - fi[i];
+}
+
+__global__ void k_fill(int64_t n, double *__restrict__ d, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        d[i] = v;
+}
+
+__global__ void k_scatter(int64_t n, const int32_t *__restrict__ idx, const double *__restrict__ val,
+                          double *__restrict__ dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[idx[i]] = val[i];
+}
+
+// Penalty: one lane per distinct BC DOF; its records are applied in deck
+// order: K[d,d] += penalty once per record, rhs[d] += penalty*(t - u[d]).
+__global__ void k_penalty(int64_t n_unique, const int32_t *__restrict__ ud_dof, const int32_t *__restrict__ ud_ptr,
+                          const int32_t *__restrict__ rec_order, const double *__restrict__ rec_target,
+                          const int32_t *__restrict__ diag_slot, double penalty, double *__restrict__ kval,
+                          uint8_t *__restrict__ keep, double *__restrict__ rhs, const double *__restrict__ u) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_unique)
+        return;
+    const int d = ud_dof[j];
+    const int slot = diag_slot[d];
+    double k = kval[slot];
+    double r = rhs ? rhs[d] : 0.0;
+    const double ud = u[d];
+    for (int c = ud_ptr[j]; c < ud_ptr[j + 1]; ++c) {
+        k += penalty;
+        r += penalty * (rec_target[rec_order[c]] - ud);
+    }
+    kval[slot] = k;
+    keep[slot] = 1;
+    if (rhs)
+        rhs[d] = r;
+}
+
+// Sums of r^2 and f^2 over DOFs with mask == 0; two-level deterministic
+// reduction (fixed per-block order, then the last block sums the partials
+// in block order).
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_masked_norms(int64_t n, const double *__restrict__ r,
+                                                             const double *__restrict__ f,
+                                                             const uint8_t *__restrict__ mask,
+                                                             double *__restrict__ part, uint32_t *__restrict__ cnt,
+                                                             double *__restrict__ out) {
+    __shared__ double s0[kRedThreads / 64], s1[kRedThreads / 64];
+    __shared__ bool last;
+    double a = 0.0, b = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!mask[i]) {
+            a += r[i] * r[i];
+            b += f[i] * f[i];
+        }
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) {
+        s0[w] = a;
+        s1[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tb = 0.0;
+        for (int k = 0; k < kRedThreads / 64; ++k) {
+            ta += s0[k];
+            tb += s1[k];
+        }
+        part[2 * blockIdx.x] = ta;
+        part[2 * blockIdx.x + 1] = tb;
+        __threadfence();
+        last = atomicAdd(cnt, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double ta = 0.0, tb = 0.0;
+        for (unsigned k = 0; k < gridDim.x; ++k) {
+            ta += __hip_atomic_load(&part[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            tb += __hip_atomic_load(&part[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        out[0] = ta;
+        out[1] = tb;
+        *cnt = 0;
+    }
+}
+
+__global__ void k_count_keep(int64_t n, const uint8_t *__restrict__ keep, unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long s[4];
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        c += keep[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        c += __shfl_xor(c, off, 64);
+    if (threadIdx.x % 64 == 0)
+        s[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicAdd(out, s[0] + s[1] + s[2] + s[3]);
+}
+
+__global__ void k_axpy(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        y[i] += a * x[i];
+}
+
+inline unsigned grid1(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+void launch_assemble(fx_mesh *m) {
+    hipStream_t s = m->ctx->stream;
+    FX_HIP(hipMemsetAsync(m->max_diag_bits.p, 0, sizeof(unsigned long long), s));
+    const int threads = 256;
+    int64_t waves = m->n_dof;
+    unsigned blocks = (unsigned)((waves * kWave + threads - 1) / threads);
+    hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(threads), 0, s, m->n_dof, m->nb_ptr.p, m->nb_col.p,
+                       m->blk_cptr.p, m->blk_cbase.p, m->ke.p, m->kval.p, m->keep.p, m->max_diag_bits.p);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_gather_f(fx_mesh *m) {
+    hipLaunchKernelGGL(k_gather_f, dim3(grid1(m->n_dof, 256)), dim3(256), 0, m->ctx->stream, m->n_dof,
+                       m->nod_cptr.p, m->nod_cbase.p, m->fe.p, m->f_int.p);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_jacobi(fx_mesh *m, bool want32) {
+    hipLaunchKernelGGL(k_jacobi, dim3(grid1(m->n_dof, 256)), dim3(256), 0, m->ctx->stream, m->n_dof,
+                       m->diag_slot.p, m->kval.p, m->inv_diag.p, want32 ? m->inv_diag32.p : nullptr);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_convert32(fx_mesh *m) {
+    m->kval32.alloc(m->nnz_struct);
+    hipLaunchKernelGGL(k_convert32, dim3(2048), dim3(256), 0, m->ctx->stream, m->nnz_struct, m->kval.p,
+                       m->kval32.p);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_residual(fx_mesh *m, const double *f_ext, const double *f_int, double *r, int64_t n) {
+    hipLaunchKernelGGL(k_residual, dim3(grid1(n, 256)), dim3(256), 0, m->ctx->stream, n, f_ext, f_int, r);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_fill(fx_mesh *m, double *dst, int64_t n, double v) {
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_fill, dim3(grid1(n, 256)), dim3(256), 0, m->ctx->stream, n, dst, v);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_scatter(fx_mesh *m, int64_t n, const int32_t *idx, const double *val, double *dst) {
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_scatter, dim3(grid1(n, 256)), dim3(256), 0, m->ctx->stream, n, idx, val, dst);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_penalty(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const int32_t *ud_ptr,
+                    const double *rec_target, const int32_t *rec_order, double *rhs, const double *u) {
+    if (n_unique <= 0)
+        return;
+    double pen = m->last_max_diag * kPenaltyScale;
+    hipLaunchKernelGGL(k_penalty, dim3(grid1(n_unique, 256)), dim3(256), 0, m->ctx->stream, n_unique, ud_dof,
+                       ud_ptr, rec_order, rec_target, m->diag_slot.p, pen, m->kval.p, m->keep.p, rhs, u);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_masked_norms(fx_mesh *m, const double *r, const double *f, const uint8_t *mask, double *out2) {
+    unsigned blocks = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (m->n_dof + 1023) / 1024));
+    hipLaunchKernelGGL(k_masked_norms, dim3(blocks), dim3(kRedThreads), 0, m->ctx->stream, m->n_dof, r, f, mask,
+                       m->part.p, m->counters.p, out2);
+    FX_HIP(hipGetLastError());
+}
+
+void axpy_dev(fx_mesh *m, int64_t n, double a, const double *x, double *y) {
+    hipLaunchKernelGGL(k_axpy, dim3(grid1(n, 256)), dim3(256), 0, m->ctx->stream, n, a, x, y);
+    FX_HIP(hipGetLastError());
+}
+
+int64_t count_kept(fx_mesh *m) {
+    hipStream_t s = m->ctx->stream;
+    unsigned long long *d = reinterpret_cast<unsigned long long *>(m->red2.p);
+    FX_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_count_keep, dim3(1024), dim3(256), 0, s, m->nnz_struct, m->keep.p, d);
+    FX_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    FX_HIP(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, s));
+    FX_HIP(hipStreamSynchronize(s));
+    return (int64_t)h;
+}
+
+void reset_status(fx_mesh *m) {
+    DevStatus st{0, 0, INT64_MAX};
+    FX_HIP(hipMemcpyAsync(m->status.p, &st, sizeof st, hipMemcpyHostToDevice, m->ctx->stream));
+}
+
+void check_status(fx_mesh *m, const char *what) {
+    DevStatus st;
+    FX_HIP(hipMemcpyAsync(&st, m->status.p, sizeof st, hipMemcpyDeviceToHost, m->ctx->stream));
+    FX_HIP(hipStreamSynchronize(m->ctx->stream));
+    if (st.code != 0) {
+        std::string msg = std::string(what) + ": ";
+        if (st.code == FX_ERR_SINGULAR_JACOBIAN)
+            msg += "Singular Jacobian";
+        else
+            msg += "element has no Young's modulus or Poisson's ratio at its temperature";
+        msg += " (element position " + std::to_string(st.bad_element) + ")";
+        reset_status(m);
+        throw Error{(fx_status)st.code, msg};
+    }
+}
+
+double device_max_diag(fx_mesh *m) {
+    unsigned long long bits = 0;
+    FX_HIP(hipMemcpyAsync(&bits, m->max_diag_bits.p, sizeof bits, hipMemcpyDeviceToHost, m->ctx->stream));
+    FX_HIP(hipStreamSynchronize(m->ctx->stream));
+    double d;
+    memcpy(&d, &bits, sizeof d);
+    return d;
+}
+
+}  // namespace fx

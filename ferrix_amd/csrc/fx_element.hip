@@ -1,0 +1,402 @@
+// fx_element.hip -- element stiffness and internal force on gfx950.
+//
+// Reference: src/solver/mesh_lib/elements/element.rs:217-447 (stiffness and
+// internal force per Gauss point), :588-606 (B matrix), material.rs:36-68 and
+// :202-230 (LUT and D), c3d4.rs / c3d20.rs (rules and shape functions).
+//
+// Arithmetic contract (checked bit-for-bit against the CPU oracle):
+//  * J = dN_local * X^T summed over nodes in order from 0.0; det by cofactor
+//    expansion along row 0; J^-1 = adj(J) / det; dN_global = J^-1 dN_local.
+//  * K_el += ((B^T D) B) * w with the dense index order of the contraction;
+//    the terms skipped here are the structural zeros of B and D, which never
+//    change a sum that starts from +0.0 (a +0.0-seeded IEEE sum is never -0,
+//    and adding +/-0 to it is exact).  Likewise eps = B u, sigma = D eps and
+//    f_el += (B^T sigma) * w.
+//  * built with -ffp-contract=off: no multiply-add is fused.
+//
+// Layout of the outputs (consumed by the deterministic assembly):
+//   ke[block_offset + e*nn*nn*9 + (a*nn + b)*9 + 3p + q] = K_el(3a+p, 3b+q)
+//   fe[block_offset + e*3nn + 3a + p]                     = f_el(3a+p)
+//
+// C3D4: one lane per element (1 Gauss point, 144 outputs).
+// C3D20: one 256-lane workgroup per element; lanes 0..26 evaluate J, J^-1,
+// dN_global, w and D at their Gauss point into LDS (13 KB), then each lane
+// owns node pairs (a, b) and accumulates the 3x3 block over the 27 points.
+#include "fx_internal.h"
+
+namespace fx {
+namespace {
+
+__device__ __forceinline__ double lut(int n, const double *t, const double *v, double temp) {
+    if (n == 1 || temp <= t[0])
+        return v[0];
+    if (temp >= t[n - 1])
+        return v[n - 1];
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int mid = lo + (hi - lo) / 2;
+        if (t[mid] <= temp)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    double factor = (temp - t[lo]) / (t[hi] - t[lo]);
+    return v[lo] + (v[hi] - v[lo]) * factor;
+}
+
+// Isotropic D has three distinct values: d1 = (1-nu) f, d2 = nu f,
+// d3 = ((1-2nu)/2) f with f = E / ((1+nu)(1-2nu)).
+__device__ __forceinline__ bool d_coeffs(const MatLut &m, double temp, double &d1, double &d2, double &d3) {
+    if (m.n_e <= 0 || m.n_nu <= 0)
+        return false;
+    double e = lut(m.n_e, m.t_e, m.v_e, temp);
+    double nu = lut(m.n_nu, m.t_nu, m.v_nu, temp);
+    double factor = e / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    double c1 = 1.0 - nu, c2 = nu, c3 = (1.0 - 2.0 * nu) / 2.0;
+    d1 = c1 * factor;
+    d2 = c2 * factor;
+    d3 = c3 * factor;
+    return true;
+}
+
+__device__ __forceinline__ void fail(DevStatus *st, int code, int64_t gpos) {
+    st->code = code;  // every failing lane stores the same kind of value
+    atomicMin(reinterpret_cast<unsigned long long *>(&st->bad_element), (unsigned long long)gpos);
+}
+
+// J from dN (row-major 3 x nn) and coordinates xc (nn x 3); returns det, Jinv.
+template <int NN>
+__device__ __forceinline__ bool jac(const double *dN, const double *xc, double *Ji, double &det) {
+    double J[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < NN; ++i)
+                s += dN[r * NN + i] * xc[3 * i + c];
+            J[3 * r + c] = s;
+        }
+    double m0 = J[4] * J[8] - J[5] * J[7];
+    double m1 = J[3] * J[8] - J[5] * J[6];
+    double m2 = J[3] * J[7] - J[4] * J[6];
+    det = J[0] * m0 - J[1] * m1 + J[2] * m2;
+    if (det == 0.0)
+        return false;
+    Ji[0] = (J[4] * J[8] - J[5] * J[7]) / det;
+    Ji[1] = (J[2] * J[7] - J[1] * J[8]) / det;
+    Ji[2] = (J[1] * J[5] - J[2] * J[4]) / det;
+    Ji[3] = (J[5] * J[6] - J[3] * J[8]) / det;
+    Ji[4] = (J[0] * J[8] - J[2] * J[6]) / det;
+    Ji[5] = (J[2] * J[3] - J[0] * J[5]) / det;
+    Ji[6] = (J[3] * J[7] - J[4] * J[6]) / det;
+    Ji[7] = (J[1] * J[6] - J[0] * J[7]) / det;
+    Ji[8] = (J[0] * J[4] - J[1] * J[3]) / det;
+    return true;
+}
+
+// 3x3 block of ((B^T D) B) for node a (dxa,dya,dza) and node b, before
+// scaling by w.  T rows are the nonzero entries of row 3a+p of B^T D.
+__device__ __forceinline__ void kblock(double dxa, double dya, double dza, double dxb, double dyb, double dzb,
+                                       double d1, double d2, double d3, double *M) {
+    // p = 0: T = [dx d1, dx d2, dx d2, dy d3, 0, dz d3]
+    {
+        double t0 = 0.0 + dxa * d1, t1 = 0.0 + dxa * d2, t2 = 0.0 + dxa * d2;
+        double t3 = 0.0 + dya * d3, t5 = 0.0 + dza * d3;
+        M[0] = ((0.0 + t0 * dxb) + t3 * dyb) + t5 * dzb;
+        M[1] = (0.0 + t1 * dyb) + t3 * dxb;
+        M[2] = (0.0 + t2 * dzb) + t5 * dxb;
+    }
+    // p = 1: T = [dy d2, dy d1, dy d2, dx d3, dz d3, 0]
+    {
+        double t0 = 0.0 + dya * d2, t1 = 0.0 + dya * d1, t2 = 0.0 + dya * d2;
+        double t3 = 0.0 + dxa * d3, t4 = 0.0 + dza * d3;
+        M[3] = (0.0 + t0 * dxb) + t3 * dyb;
+        M[4] = ((0.0 + t1 * dyb) + t3 * dxb) + t4 * dzb;
+        M[5] = (0.0 + t2 * dzb) + t4 * dyb;
+    }
+    // p = 2: T = [dz d2, dz d2, dz d1, 0, dy d3, dx d3]
+    {
+        double t0 = 0.0 + dza * d2, t1 = 0.0 + dza * d2, t2 = 0.0 + dza * d1;
+        double t4 = 0.0 + dya * d3, t5 = 0.0 + dxa * d3;
+        M[6] = (0.0 + t0 * dxb) + t5 * dzb;
+        M[7] = (0.0 + t1 * dyb) + t4 * dzb;
+        M[8] = ((0.0 + t2 * dzb) + t4 * dyb) + t5 * dxb;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C3D4: one lane per element
+// ---------------------------------------------------------------------------
+template <bool DO_K, bool DO_F>
+__global__ __launch_bounds__(128) void k_c3d4(int64_t n_el, const int32_t *__restrict__ conn,
+                                              const int32_t *__restrict__ mat, const int64_t *__restrict__ gpos,
+                                              const double *__restrict__ xyz, const double *__restrict__ temp,
+                                              const double *__restrict__ u, int nlgeom,
+                                              const MatLut *__restrict__ mats, const double *__restrict__ gN,
+                                              const double *__restrict__ gdN, const double *__restrict__ gw,
+                                              double *__restrict__ ke, double *__restrict__ fe,
+                                              DevStatus *__restrict__ st) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el)
+        return;
+    double xc[12], ue[12], T[4], dN[12];
+    int nd[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        nd[i] = conn[4 * e + i];
+        T[i] = temp[nd[i]];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double x = xyz[3 * (int64_t)nd[i] + c];
+            double uu = u ? u[3 * (int64_t)nd[i] + c] : 0.0;
+            ue[3 * i + c] = uu;
+            xc[3 * i + c] = nlgeom ? x + uu : x;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+        dN[i] = gdN[i];
+    double Ji[9], det;
+    if (!jac<4>(dN, xc, Ji, det)) {
+        fail(st, FX_ERR_SINGULAR_JACOBIAN, gpos[e]);
+        return;
+    }
+    double dg[12];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                s += Ji[3 * r + k] * dN[4 * k + i];
+            dg[4 * r + i] = s;
+        }
+    double w = fabs(det) * gw[0];
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        t += gN[i] * T[i];
+    double d1, d2, d3;
+    if (!d_coeffs(mats[mat[e]], t, d1, d2, d3)) {
+        fail(st, FX_ERR_INVALID, gpos[e]);
+        return;
+    }
+    if (DO_K) {
+        double *out = ke + e * 144;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                double M[9];
+                kblock(dg[a], dg[4 + a], dg[8 + a], dg[b], dg[4 + b], dg[8 + b], d1, d2, d3, M);
+#pragma unroll
+                for (int k = 0; k < 9; ++k)
+                    out[(4 * a + b) * 9 + k] = 0.0 + M[k] * w;
+            }
+    }
+    if (DO_F) {
+        double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            double dx = dg[a], dy = dg[4 + a], dz = dg[8 + a];
+            double ux = ue[3 * a], uy = ue[3 * a + 1], uz = ue[3 * a + 2];
+            e0 += dx * ux;
+            e1 += dy * uy;
+            e2 += dz * uz;
+            e3 += dy * ux;
+            e3 += dx * uy;
+            e4 += dz * uy;
+            e4 += dy * uz;
+            e5 += dz * ux;
+            e5 += dx * uz;
+        }
+        double s0 = ((0.0 + d1 * e0) + d2 * e1) + d2 * e2;
+        double s1 = ((0.0 + d2 * e0) + d1 * e1) + d2 * e2;
+        double s2 = ((0.0 + d2 * e0) + d2 * e1) + d1 * e2;
+        double s3 = 0.0 + d3 * e3, s4 = 0.0 + d3 * e4, s5 = 0.0 + d3 * e5;
+        double *out = fe + e * 12;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            double dx = dg[a], dy = dg[4 + a], dz = dg[8 + a];
+            out[3 * a + 0] = 0.0 + (((0.0 + dx * s0) + dy * s3) + dz * s5) * w;
+            out[3 * a + 1] = 0.0 + (((0.0 + dy * s1) + dx * s3) + dz * s4) * w;
+            out[3 * a + 2] = 0.0 + (((0.0 + dz * s2) + dy * s4) + dx * s5) * w;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C3D20: one workgroup per element
+// ---------------------------------------------------------------------------
+constexpr int kC20Threads = 256;
+
+template <bool DO_K, bool DO_F>
+__global__ __launch_bounds__(kC20Threads) void k_c3d20(
+    int64_t n_el, const int32_t *__restrict__ conn, const int32_t *__restrict__ mat,
+    const int64_t *__restrict__ gpos, const double *__restrict__ xyz, const double *__restrict__ temp,
+    const double *__restrict__ u, int nlgeom, const MatLut *__restrict__ mats, const double *__restrict__ gN,
+    const double *__restrict__ gdN, const double *__restrict__ gw, double *__restrict__ ke,
+    double *__restrict__ fe, DevStatus *__restrict__ st) {
+    __shared__ double s_xc[60], s_u[60], s_T[20];
+    __shared__ double s_dg[27][60];  // [g][r*20 + i]
+    __shared__ double s_w[27], s_d[27][3], s_sig[27][6];
+    __shared__ int s_bad;
+    const int64_t e = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (tid == 0)
+        s_bad = 0;
+    if (tid < 20) {
+        int nd = conn[20 * e + tid];
+        s_T[tid] = temp[nd];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double x = xyz[3 * (int64_t)nd + c];
+            double uu = u ? u[3 * (int64_t)nd + c] : 0.0;
+            s_u[3 * tid + c] = uu;
+            s_xc[3 * tid + c] = nlgeom ? x + uu : x;
+        }
+    }
+    __syncthreads();
+    if (tid < 27) {
+        const int g = tid;
+        const double *dN = gdN + 60 * g;
+        double Ji[9], det;
+        if (!jac<20>(dN, s_xc, Ji, det)) {
+            s_bad = 1;
+            fail(st, FX_ERR_SINGULAR_JACOBIAN, gpos[e]);
+        } else {
+            for (int r = 0; r < 3; ++r)
+                for (int i = 0; i < 20; ++i) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        s += Ji[3 * r + k] * dN[20 * k + i];
+                    s_dg[g][20 * r + i] = s;
+                }
+            s_w[g] = fabs(det) * gw[g];
+            double t = 0.0;
+            for (int i = 0; i < 20; ++i)
+                t += gN[20 * g + i] * s_T[i];
+            double d1, d2, d3;
+            if (!d_coeffs(mats[mat[e]], t, d1, d2, d3)) {
+                s_bad = 1;
+                fail(st, FX_ERR_INVALID, gpos[e]);
+            }
+            s_d[g][0] = d1;
+            s_d[g][1] = d2;
+            s_d[g][2] = d3;
+        }
+    }
+    __syncthreads();
+    if (s_bad)
+        return;
+    if (DO_K) {
+        double *out = ke + e * 3600;
+        for (int pair = tid; pair < 400; pair += kC20Threads) {
+            const int a = pair / 20, b = pair % 20;
+            double acc[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                acc[k] = 0.0;
+            for (int g = 0; g < 27; ++g) {
+                double M[9];
+                kblock(s_dg[g][a], s_dg[g][20 + a], s_dg[g][40 + a], s_dg[g][b], s_dg[g][20 + b], s_dg[g][40 + b],
+                       s_d[g][0], s_d[g][1], s_d[g][2], M);
+                const double w = s_w[g];
+#pragma unroll
+                for (int k = 0; k < 9; ++k)
+                    acc[k] += M[k] * w;
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                out[pair * 9 + k] = acc[k];
+        }
+    }
+    if (DO_F) {
+        if (tid < 27) {
+            const int g = tid;
+            double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
+            for (int a = 0; a < 20; ++a) {
+                double dx = s_dg[g][a], dy = s_dg[g][20 + a], dz = s_dg[g][40 + a];
+                double ux = s_u[3 * a], uy = s_u[3 * a + 1], uz = s_u[3 * a + 2];
+                e0 += dx * ux;
+                e1 += dy * uy;
+                e2 += dz * uz;
+                e3 += dy * ux;
+                e3 += dx * uy;
+                e4 += dz * uy;
+                e4 += dy * uz;
+                e5 += dz * ux;
+                e5 += dx * uz;
+            }
+            const double d1 = s_d[g][0], d2 = s_d[g][1], d3 = s_d[g][2];
+            s_sig[g][0] = ((0.0 + d1 * e0) + d2 * e1) + d2 * e2;
+            s_sig[g][1] = ((0.0 + d2 * e0) + d1 * e1) + d2 * e2;
+            s_sig[g][2] = ((0.0 + d2 * e0) + d2 * e1) + d1 * e2;
+            s_sig[g][3] = 0.0 + d3 * e3;
+            s_sig[g][4] = 0.0 + d3 * e4;
+            s_sig[g][5] = 0.0 + d3 * e5;
+        }
+        __syncthreads();
+        if (tid < 60) {
+            const int a = tid / 3, p = tid % 3;
+            double f = 0.0;
+            for (int g = 0; g < 27; ++g) {
+                double dx = s_dg[g][a], dy = s_dg[g][20 + a], dz = s_dg[g][40 + a];
+                double s;
+                if (p == 0)
+                    s = ((0.0 + dx * s_sig[g][0]) + dy * s_sig[g][3]) + dz * s_sig[g][5];
+                else if (p == 1)
+                    s = ((0.0 + dy * s_sig[g][1]) + dx * s_sig[g][3]) + dz * s_sig[g][4];
+                else
+                    s = ((0.0 + dz * s_sig[g][2]) + dy * s_sig[g][4]) + dx * s_sig[g][5];
+                f += s * s_w[g];
+            }
+            fe[e * 60 + tid] = f;
+        }
+    }
+}
+
+template <bool K, bool F>
+void launch_types(fx_mesh *m, const double *u, int nlgeom, const double *const *tab4, const double *const *tab20) {
+    hipStream_t s = m->ctx->stream;
+    if (m->blk4.n_el) {
+        const ElemBlock &b = m->blk4;
+        int threads = 128;
+        int64_t blocks = (b.n_el + threads - 1) / threads;
+        hipLaunchKernelGGL((k_c3d4<K, F>), dim3((unsigned)blocks), dim3(threads), 0, s, b.n_el, b.conn.p, b.mat.p,
+                           b.gpos.p, m->xyz.p, m->temp.p, u, nlgeom, m->mats.p, tab4[0], tab4[1], tab4[2],
+                           m->ke.p + b.ke_offset, m->fe.p + b.fe_offset, m->status.p);
+        FX_HIP(hipGetLastError());
+    }
+    if (m->blk20.n_el) {
+        const ElemBlock &b = m->blk20;
+        hipLaunchKernelGGL((k_c3d20<K, F>), dim3((unsigned)b.n_el), dim3(kC20Threads), 0, s, b.n_el, b.conn.p,
+                           b.mat.p, b.gpos.p, m->xyz.p, m->temp.p, u, nlgeom, m->mats.p, tab20[0], tab20[1],
+                           tab20[2], m->ke.p + b.ke_offset, m->fe.p + b.fe_offset, m->status.p);
+        FX_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+void launch_element(fx_mesh *m, bool want_k, bool want_f, bool nlgeom) {
+    DevTables *t = &m->ctx->tables;
+    const double *tab4[3] = {t->n4.p, t->dn4.p, t->w4.p};
+    const double *tab20[3] = {t->n20.p, t->dn20.p, t->w20.p};
+    // the displacement enters the strain (f) always and the coordinates
+    // only under NLGEOM; for a linear K it is not read at all
+    const double *u = (want_f || nlgeom) ? m->u.p : nullptr;
+    int nl = nlgeom ? 1 : 0;
+    if (want_k && want_f)
+        launch_types<true, true>(m, u, nl, tab4, tab20);
+    else if (want_k)
+        launch_types<true, false>(m, u, nl, tab4, tab20);
+    else if (want_f)
+        launch_types<false, true>(m, u, nl, tab4, tab20);
+}
+
+}  // namespace fx

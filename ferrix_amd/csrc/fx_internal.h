@@ -1,0 +1,192 @@
+// fx_internal.h -- shared declarations of the gfx950 backend (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ferrix_hip.h"
+
+namespace fx {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+struct Error {
+    fx_status code;
+    std::string msg;
+};
+
+#define FX_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::fx::Error{e_ == hipErrorOutOfMemory ? FX_ERR_OOM : FX_ERR_HIP,                     \
+                              std::string(#call) + ": " + hipGetErrorString(e_)};                      \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffer (RAII)
+// ---------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p)
+            return;
+        release();
+        if (count == 0)
+            return;
+        FX_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count)
+            FX_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------
+// constants shared by host and device
+// ---------------------------------------------------------------------------
+constexpr int kLutMax = FX_LUT_MAX;
+constexpr double kDropTol = 1e-12;        // assembler.rs:135
+constexpr double kPenaltyScale = 1.0e6;   // static_step.rs:185
+constexpr double kJacobiGuard = 1e-9;     // preconditioner.rs:44
+
+// material LUT row on the device
+struct MatLut {
+    int32_t n_e, n_nu;
+    double t_e[kLutMax], v_e[kLutMax], t_nu[kLutMax], v_nu[kLutMax];
+};
+
+// Gauss table of one element type: dN_local (3 x nn) and N (nn) per point.
+struct GaussTable {
+    int ngp, nn;
+    std::vector<double> dN;  // ngp * 3 * nn
+    std::vector<double> N;   // ngp * nn
+    std::vector<double> w;   // ngp
+};
+GaussTable gauss_table(int nn);  // fx_shape.cpp
+
+// device-side status word written by kernels
+struct DevStatus {
+    int32_t code;        // fx_status of the first failure
+    int32_t pad;
+    int64_t bad_element; // smallest failing element position (atomicMin)
+};
+
+// element block of one type inside a mesh
+struct ElemBlock {
+    int nn = 0;
+    int64_t n_el = 0;
+    DBuf<int32_t> conn;     // n_el * nn node indices
+    DBuf<int32_t> mat;      // n_el
+    DBuf<int64_t> gpos;     // n_el global element positions (error reporting)
+    int64_t ke_offset = 0;  // first value of this block in the K contribution buffer
+    int64_t fe_offset = 0;  // first value in the f contribution buffer
+};
+
+// PCG scalars in device memory
+struct PcgScalars {
+    double rz, rr, bb, pap, alpha, beta, rz0, crit;
+    double tol;
+    int64_t iter, max_iter;
+    int32_t status, stop_mode;  // status: 0 running, 1 converged, 2 breakdown, 3 max_iter
+    uint32_t cnt_a, cnt_b;      // last-block counters
+    uint32_t pad[2];
+};
+
+// Gauss tables in device memory, one set per context (uploaded at creation).
+struct DevTables {
+    DBuf<double> n4, dn4, w4, n20, dn20, w20;
+};
+
+}  // namespace fx
+
+struct fx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    std::string err;
+    fx::DevTables tables;
+};
+
+struct fx_mesh {
+    fx_ctx *ctx = nullptr;
+    int64_t n_nodes = 0, n_dof = 0, n_el = 0;
+    // node-block structural pattern (3x3 blocks)
+    int64_t nblk = 0;           // structural node-pair blocks
+    int64_t nnz_struct = 0;     // 9 * nblk
+    std::vector<int32_t> h_nb_ptr, h_nb_col;
+    fx::DBuf<double> xyz, temp;
+    fx::DBuf<fx::MatLut> mats;
+    int n_mat = 0;
+    fx::ElemBlock blk4, blk20;
+    int64_t ke_total = 0, fe_total = 0;
+    fx::DBuf<double> ke, fe;             // element contributions
+    fx::DBuf<int32_t> nb_ptr, nb_col;    // n_nodes+1, nblk
+    fx::DBuf<int32_t> blk_cptr, blk_cbase;  // per block: contribution bases into ke
+    fx::DBuf<int32_t> nod_cptr, nod_cbase;  // per node: bases into fe
+    fx::DBuf<int32_t> diag_slot;         // per dof: slot of K[d,d]
+    fx::DBuf<double> kval;               // nnz_struct values (super-pattern)
+    fx::DBuf<float> kval32;
+    fx::DBuf<uint8_t> keep;              // 1 if the slot survives the drop rule
+    fx::DBuf<double> inv_diag;
+    fx::DBuf<float> inv_diag32;
+    fx::DBuf<unsigned long long> max_diag_bits;
+    fx::DBuf<fx::DevStatus> status;
+    fx::DBuf<fx::PcgScalars> pcg;
+    fx::DBuf<double> part;               // reduction partials
+    fx::DBuf<uint32_t> counters;
+    // vectors
+    fx::DBuf<double> u, f_int, f_ext, rhs, du, x, r, z, p0, p1, ap, tmp;
+    fx::DBuf<double> red2;               // 2 doubles: norm results
+    double last_max_diag = 0.0;
+    bool k_valid = false;
+    int spmv_group = 0;                  // autotuned lanes per node row
+    int64_t device_bytes = 0;
+};
+
+namespace fx {
+// launchers (defined in the .hip translation units)
+void launch_element(fx_mesh *m, bool want_k, bool want_f, bool nlgeom);
+void launch_assemble(fx_mesh *m);
+void launch_gather_f(fx_mesh *m);
+void launch_jacobi(fx_mesh *m, bool want32);
+void launch_convert32(fx_mesh *m);
+void check_status(fx_mesh *m, const char *what);
+void reset_status(fx_mesh *m);
+double device_max_diag(fx_mesh *m);
+// PCG (fx_pcg.hip)
+fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep);
+void spmv64(fx_mesh *m, const double *x, double *y, int group);
+void axpy_dev(fx_mesh *m, int64_t n, double a, const double *x, double *y);
+// step helpers (fx_assemble.hip)
+void launch_residual(fx_mesh *m, const double *f_ext, const double *f_int, double *r, int64_t n);
+void launch_masked_norms(fx_mesh *m, const double *r, const double *f, const uint8_t *mask, double *out2);
+void launch_penalty(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const int32_t *ud_ptr,
+                    const double *rec_target, const int32_t *rec_order, double *rhs, const double *u);
+void launch_scatter(fx_mesh *m, int64_t n, const int32_t *idx, const double *val, double *dst);
+void launch_fill(fx_mesh *m, double *dst, int64_t n, double v);
+int64_t count_kept(fx_mesh *m);
+}  // namespace fx

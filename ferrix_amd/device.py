@@ -1,0 +1,239 @@
+"""Python host layer over the C ABI: the GPU counterparts of FerriX's
+assembly, solver and step seams.
+
+* :class:`Context` -- one HIP device and stream (``fx_ctx``).
+* :class:`DeviceMesh` -- a resident mesh (``fx_mesh``): ``assemble``
+  (``Assembler::assemble``, ``src/solver/assembler.rs:37-151``),
+  ``internal_force`` (``:157-241``), ``apply_penalty``
+  (``src/solver/step/static_step.rs:183-195``), ``solve``
+  (``Solver::solve``, ``src/solver/solvers/mod.rs:18-44``) and
+  ``static_step`` (``StaticStep::solve``, ``static_step.rs:47-267``).
+* :func:`run_job` -- every step of a :class:`~ferrix_amd.model.Job` in
+  order, like ``main.rs:49-63`` and the reference's parity tests.
+
+Errors come back as the :mod:`ferrix_amd.inp` exception classes that mirror
+``FerrixError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import _lib as L
+from .inp import ConvergenceError, FerrixError, InvalidModelState, NumericalError
+from .model import Job, StepSpec, SOLVER_ITERATIVE, STOP_REL_R2
+
+
+class HipError(FerrixError):
+    pass
+
+
+class NoDeviceError(FerrixError):
+    pass
+
+
+_ERR = {
+    L.FX_ERR_INVALID: InvalidModelState,
+    L.FX_ERR_OOM: InvalidModelState,
+    L.FX_ERR_HIP: HipError,
+    L.FX_ERR_SINGULAR_JACOBIAN: NumericalError,
+    L.FX_ERR_NOT_CONVERGED: ConvergenceError,
+    L.FX_ERR_BREAKDOWN: NumericalError,
+    L.FX_ERR_NO_DEVICE: NoDeviceError,
+}
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+@dataclass
+class SolveOptions:
+    """``fx_solve_opts``; defaults are the reference's PCG settings
+    (``static_step.rs:204``: absolute sqrt(r.z) < 1e-8, 1000 iterations)."""
+
+    precision: int = L.FX_PREC_FP64
+    stop_mode: int = L.FX_STOP_ABS_SQRT_RZ
+    tol: float = 1e-8
+    max_iter: int = 1000
+    inner_tol: float = 1e-3
+    max_refine: int = 50
+    spmv_group: int = 0
+
+    def c(self) -> L.fx_solve_opts:
+        return L.fx_solve_opts(self.precision, self.stop_mode, self.tol, self.max_iter, self.inner_tol,
+                               self.max_refine, self.spmv_group)
+
+
+# The reference's DEFAULT/DIRECT solver is a sparse Cholesky; the GPU path
+# serves it with FP64 PCG to a tight relative residual.
+DIRECT_STANDIN = SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-12, max_iter=200_000)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        rc = self.lib.fx_ctx_create(device, C.byref(h))
+        if rc != L.FX_OK:
+            raise _ERR.get(rc, FerrixError)(self.lib.fx_last_error(None).decode())
+        self.h = h
+
+    def check(self, rc: int):
+        if rc != L.FX_OK:
+            msg = self.lib.fx_last_error(self.h).decode()
+            raise _ERR.get(rc, FerrixError)(msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fx_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceMesh:
+    def __init__(self, ctx: Context, mesh, node_temp: np.ndarray):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.n_dof = mesh.n_dof
+        keep = [_c(mesh.xyz, np.float64), _c(node_temp, np.float64), _c(mesh.el_ptr, np.int64),
+                _c(mesh.conn, np.int32), _c(mesh.mat, np.int32), _c(mesh.lut_n_e, np.int32),
+                _c(mesh.lut_t_e, np.float64), _c(mesh.lut_v_e, np.float64), _c(mesh.lut_n_nu, np.int32),
+                _c(mesh.lut_t_nu, np.float64), _c(mesh.lut_v_nu, np.float64)]
+        xyz, temp, el_ptr, conn, mat, ne, te, ve, nn_, tn, vn = keep
+        mt = L.fx_material_table(len(ne), _p(ne), _p(te), _p(ve), _p(nn_), _p(tn), _p(vn))
+        h = C.c_void_p()
+        ctx.check(self.lib.fx_mesh_upload(ctx.h, mesh.n_nodes, _p(xyz), _p(temp), mesh.n_el, _p(el_ptr),
+                                          _p(conn), _p(mat), C.byref(mt), C.byref(h)))
+        self.h = h
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        ctx.check(self.lib.fx_mesh_info(h, C.byref(a), C.byref(b), C.byref(c)))
+        self.nnz_struct = b.value
+        self.device_bytes = c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fx_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state --------------------------------------------------------------
+    def set_u(self, u):
+        u = _c(u, np.float64)
+        assert u.shape == (self.n_dof,)
+        self.ctx.check(self.lib.fx_state_set_u(self.h, _p(u)))
+
+    def get_u(self) -> np.ndarray:
+        u = np.zeros(self.n_dof)
+        self.ctx.check(self.lib.fx_state_get_u(self.h, _p(u)))
+        return u
+
+    # -- assembly seam -------------------------------------------------------
+    def assemble(self, nlgeom: bool = False) -> Tuple[float, int]:
+        md = C.c_double()
+        nnz = C.c_int64()
+        self.ctx.check(self.lib.fx_assemble(self.h, int(nlgeom), C.byref(md), C.byref(nnz)))
+        self.nnz = nnz.value
+        return md.value, nnz.value
+
+    def internal_force(self, nlgeom: bool = False) -> np.ndarray:
+        f = np.zeros(self.n_dof)
+        self.ctx.check(self.lib.fx_internal_force(self.h, int(nlgeom), _p(f)))
+        return f
+
+    def matrix(self):
+        """Filtered CSR (row_ptr int64, col int32, values float64)."""
+        rp = np.zeros(self.n_dof + 1, np.int64)
+        col = np.zeros(max(1, self.nnz), np.int32)
+        val = np.zeros(max(1, self.nnz), np.float64)
+        self.ctx.check(self.lib.fx_matrix_get(self.h, _p(rp), _p(col), _p(val)))
+        k = int(rp[-1])
+        return rp, col[:k], val[:k]
+
+    def apply_penalty(self, dofs, targets, rhs):
+        dofs = _c(dofs, np.int64)
+        targets = _c(targets, np.float64)
+        rhs = np.array(rhs, dtype=np.float64, copy=True)
+        pen = C.c_double()
+        self.ctx.check(self.lib.fx_apply_penalty(self.h, len(dofs), _p(dofs), _p(targets), _p(rhs), C.byref(pen)))
+        return rhs, pen.value
+
+    # -- solver seam ---------------------------------------------------------
+    def solve(self, rhs, opts: Optional[SolveOptions] = None):
+        rhs = _c(rhs, np.float64)
+        x = np.zeros(self.n_dof)
+        o = (opts or SolveOptions()).c()
+        rep = L.fx_solve_report()
+        self.ctx.check(self.lib.fx_solve(self.h, _p(rhs), _p(x), C.byref(o), C.byref(rep)))
+        return x, rep
+
+    # -- step driver ---------------------------------------------------------
+    def static_step(self, step: StepSpec, step_id: int, timer: L.fx_timer, direct_opts: SolveOptions = None,
+                    solve_opts: Optional[SolveOptions] = None) -> L.fx_step_report:
+        sd, keep = step_def(step, step_id, direct_opts or DIRECT_STANDIN, solve_opts)
+        rep = L.fx_step_report()
+        self.ctx.check(self.lib.fx_static_step(self.h, C.byref(sd), C.byref(timer), C.byref(rep)))
+        del keep
+        return rep
+
+
+def step_options(step: StepSpec, precision=L.FX_PREC_FP64) -> SolveOptions:
+    """Linear-solve options a StepSpec asks for (ITERATIVE settings)."""
+    return SolveOptions(precision=precision, stop_mode=step.stop_mode, tol=step.tol, max_iter=step.max_iter)
+
+
+def step_def(step: StepSpec, step_id: int, direct_opts: SolveOptions, solve_opts: Optional[SolveOptions]):
+    a = step.amps
+    keep = dict(
+        ld=_c(step.load_dof, np.int64), lv=_c(step.load_val, np.float64), la=_c(step.load_amp, np.int32),
+        lo=_c(step.load_origin, np.int32), bd=_c(step.bc_dof, np.int64), bv=_c(step.bc_val, np.float64),
+        ba=_c(step.bc_amp, np.int32), bo=_c(step.bc_origin, np.int32),
+        tt=_c(a.total_time, np.int32), sx=_c(a.shift_x, np.float64), sy=_c(a.shift_y, np.float64),
+        hd=_c(a.has_data, np.int32), ptr=_c(a.ptr, np.int64), times=_c(a.times or [0.0], np.float64),
+        values=_c(a.values or [0.0], np.float64))
+    amps = L.fx_amplitudes(len(a.total_time), _p(keep["tt"]), _p(keep["sx"]), _p(keep["sy"]), _p(keep["hd"]),
+                           _p(keep["ptr"]), _p(keep["times"]), _p(keep["values"]))
+    so = (solve_opts or step_options(step)).c()
+    sd = L.fx_step_def(step_id, int(step.nlgeom), int(step.solver), step.initial_time_increment, step.time_period,
+                       step.min_time_increment, step.max_time_increment, step.max_increments,
+                       len(keep["ld"]), _p(keep["ld"]), _p(keep["lv"]), _p(keep["la"]), _p(keep["lo"]),
+                       len(keep["bd"]), _p(keep["bd"]), _p(keep["bv"]), _p(keep["ba"]), _p(keep["bo"]),
+                       amps, so, direct_opts.c())
+    return sd, keep
+
+
+def run_job(ctx: Context, job: Job, precision=L.FX_PREC_FP64, direct_opts: Optional[SolveOptions] = None,
+            solve_opts: Optional[SolveOptions] = None, mesh: Optional[DeviceMesh] = None):
+    """All steps of ``job`` from u = 0; returns (u, [fx_step_report])."""
+    dm = mesh or DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.set_u(np.zeros(job.n_dof))
+    timer = L.fx_timer(0.0, 0.0, 1.0)
+    reps = []
+    for k, st in enumerate(job.steps):
+        so = solve_opts or step_options(st, precision)
+        do = direct_opts or SolveOptions(precision=precision, stop_mode=STOP_REL_R2, tol=1e-12, max_iter=200_000)
+        reps.append(dm.static_step(st, k + 1, timer, direct_opts=do, solve_opts=so))
+    return dm.get_u(), reps

@@ -1,0 +1,244 @@
+/*
+ * ferrix_hip.h -- C ABI of the MI355X (gfx950) static-step backend.
+ *
+ * The library sits behind FerriX's three seams for the static-step hot path
+ * (paths relative to the FerriX source tree):
+ *
+ *   element seam   trait FiniteElement / Element::compute_stiffness_sdv,
+ *                  compute_internal_force_sdv   (src/solver/mesh_lib/elements/element.rs:20-35, 217-447)
+ *   assembly seam  Assembler::assemble / assemble_internal_force
+ *                                               (src/solver/assembler.rs:37-44, 157-165)
+ *   solver seam    trait Solver::solve + trait Preconditioner
+ *                                               (src/solver/solvers/mod.rs:18-44,
+ *                                                src/solver/preconditioner.rs:15-26)
+ *   step driver    StaticStep::solve            (src/solver/step/static_step.rs:47-267)
+ *
+ * Conventions: plain C types only; every pointer argument is host memory
+ * unless its name ends in _dev; arrays are copied, the caller keeps
+ * ownership.  Node index = rank of the node ID in ascending order and
+ * DOF = 3*index + d (src/solver/mesh_lib/mesh.rs:35-49).  Elements are
+ * visited in the order given (the caller passes ascending element ID).
+ * A context is used by one host thread at a time.  Errors never cross the
+ * boundary as exceptions: every call returns an fx_status and
+ * fx_last_error() holds the message.
+ */
+#ifndef FERRIX_HIP_H
+#define FERRIX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; mapping onto FerrixError (src/solver/error.rs:12-74). */
+typedef enum fx_status {
+    FX_OK = 0,
+    FX_ERR_INVALID = 1,           /* InvalidModelState: bad argument or model      */
+    FX_ERR_OOM = 2,               /* InvalidModelState: device memory exhausted    */
+    FX_ERR_HIP = 3,               /* HIP runtime failure (message has the detail)  */
+    FX_ERR_SINGULAR_JACOBIAN = 4, /* NumericalError("Singular Jacobian"),
+                                     element.rs:257-259                            */
+    FX_ERR_NOT_CONVERGED = 5,     /* ConvergenceError: PCG (iterative.rs:117-120),
+                                     NR / increments (static_step.rs:73-78,254-257) */
+    FX_ERR_BREAKDOWN = 6,         /* NumericalError: p.Ap <= 0 or not finite        */
+    FX_ERR_NO_DEVICE = 7          /* no usable gfx950 device                        */
+} fx_status;
+
+typedef struct fx_ctx fx_ctx;
+typedef struct fx_mesh fx_mesh;
+
+/* ---- context ---------------------------------------------------------- */
+/* Binds HIP device `device`, creates one stream (default priority). */
+fx_status fx_ctx_create(int device, fx_ctx **out);
+void fx_ctx_destroy(fx_ctx *ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char *fx_last_error(const fx_ctx *ctx);
+/* Library version string. */
+const char *fx_version(void);
+
+/* ---- mesh ------------------------------------------------------------- */
+#define FX_LUT_MAX 16
+
+/* Temperature-dependent isotropic elasticity (material.rs:36-68, 202-230):
+ * per material a clamped piecewise-linear LUT for E and one for nu, each with
+ * n_* points (1..FX_LUT_MAX, temperatures ascending), rows of FX_LUT_MAX. */
+typedef struct fx_material_table {
+    int32_t n_mat;
+    const int32_t *n_e;
+    const double *t_e, *v_e;
+    const int32_t *n_nu;
+    const double *t_nu, *v_nu;
+} fx_material_table;
+
+/*
+ * Uploads the mesh and builds the structural pattern (host symbolic phase):
+ *   xyz        n_nodes x 3 coordinates in node-index order
+ *   node_temp  n_nodes temperatures (initial == current; the reference never
+ *              changes T after SolutionState::initialize, state.rs:47-78)
+ *   el_ptr     n_el+1 offsets into conn; 4 nodes = C3D4, 20 nodes = C3D20
+ *   conn       node indices per element (CalculiX node order)
+ *   mat        material index per element
+ * Replaces the Mesh/Project data the reference keeps in HashMaps
+ * (src/solver/mesh_lib/mesh.rs:16-30, project.rs:32-59).
+ */
+fx_status fx_mesh_upload(fx_ctx *ctx, int64_t n_nodes, const double *xyz, const double *node_temp,
+                         int64_t n_el, const int64_t *el_ptr, const int32_t *conn, const int32_t *mat,
+                         const fx_material_table *mats, fx_mesh **out);
+void fx_mesh_destroy(fx_mesh *mesh);
+
+/* n_dof = 3 n_nodes; nnz_struct = structural (pre-filter) nonzeros;
+ * device_bytes = HBM held by the mesh. */
+fx_status fx_mesh_info(const fx_mesh *mesh, int64_t *n_dof, int64_t *nnz_struct, int64_t *device_bytes);
+
+/* ---- displacement state (SolutionState::displacements, state.rs:16-27) -- */
+fx_status fx_state_set_u(fx_mesh *mesh, const double *u);
+fx_status fx_state_get_u(fx_mesh *mesh, double *u);
+
+/* ---- assembly seam ---------------------------------------------------- */
+/*
+ * Assembler::assemble (assembler.rs:37-151): element stiffness for every
+ * element with coordinates X (+ the resident u when nlgeom != 0), drop rule
+ * |v| > 1e-12 per element contribution, contributions summed per CSR slot in
+ * element order (deterministic), max |v| over kept element diagonal
+ * contributions.  K stays resident; *max_diag_out and *nnz_out (filtered
+ * nonzeros, i.e. what sprs to_csr would hold) are returned.
+ */
+fx_status fx_assemble(fx_mesh *mesh, int32_t nlgeom, double *max_diag_out, int64_t *nnz_out);
+
+/*
+ * Assembler::assemble_internal_force (assembler.rs:157-241) at the resident u;
+ * coordinates move with u when nlgeom != 0 (u_conf, static_step.rs:121-126).
+ * f_int_out (n_dof, host) may be NULL to keep the result on the device.
+ */
+fx_status fx_internal_force(fx_mesh *mesh, int32_t nlgeom, double *f_int_out);
+
+/*
+ * Filtered CSR of the resident K (the sprs CsMat the reference hands to the
+ * solver, static_step.rs:198): row_ptr (n_dof+1), col_idx (nnz), values
+ * (nnz, may be NULL).  Columns ascending per row.
+ */
+fx_status fx_matrix_get(fx_mesh *mesh, int64_t *row_ptr, int32_t *col_idx, double *values);
+
+/*
+ * Penalty boundary conditions (static_step.rs:183-195): when the last
+ * assembly's max_diag > 0, penalty = 1e6 * max_diag is added to K[d,d] once
+ * per record (duplicates add twice) and rhs[d] += penalty * (target - u[d])
+ * per record in order.  rhs_inout (n_dof) is host memory; *penalty_out
+ * receives the penalty (0 when not applied).
+ */
+fx_status fx_apply_penalty(fx_mesh *mesh, int64_t n_bc, const int64_t *dofs, const double *targets,
+                           double *rhs_inout, double *penalty_out);
+
+/* ---- solver seam ------------------------------------------------------ */
+typedef enum fx_precision {
+    FX_PREC_FP64 = 0,  /* Jacobi-PCG in FP64 (the reference's IterativeSolver)   */
+    FX_PREC_MIXED = 1  /* FP32-stored K and Jacobi inside FP64 iterative
+                          refinement; FP64 residuals and norms               */
+} fx_precision;
+
+typedef enum fx_stop {
+    FX_STOP_ABS_SQRT_RZ = 0, /* sqrt(r.z) < tol (iterative.rs:63; tol 1e-8)     */
+    FX_STOP_REL_R2 = 1       /* ||b - K x||_2 / ||b||_2 <= tol                 */
+} fx_stop;
+
+typedef struct fx_solve_opts {
+    int32_t precision;  /* fx_precision */
+    int32_t stop_mode;  /* fx_stop */
+    double tol;         /* reference: 1e-8 (static_step.rs:204) */
+    int64_t max_iter;   /* reference: 1000; iterations with the working operator */
+    double inner_tol;   /* MIXED: relative residual reduction per refinement pass */
+    int32_t max_refine; /* MIXED: refinement passes before NOT_CONVERGED */
+    int32_t spmv_group; /* lanes per node row in the SpMV (0 = autotuned) */
+} fx_solve_opts;
+
+typedef struct fx_solve_report {
+    int64_t iterations;     /* SpMVs with the working operator (PCG iterations) */
+    int64_t outer_spmv;     /* MIXED: FP64 residual evaluations */
+    int32_t refinements;    /* MIXED: refinement passes */
+    int32_t spmv_group;     /* lanes per node row used */
+    double final_criterion; /* last stopping-rule value */
+    double seconds;         /* wall time of the solve on the context stream */
+} fx_solve_report;
+
+/* Fills opts with the reference's values: FP64, sqrt(r.z) < 1e-8, 1000. */
+void fx_solve_opts_default(fx_solve_opts *opts);
+
+/*
+ * Solver::solve with the Jacobi preconditioner (solvers/mod.rs:18-44,
+ * iterative.rs:26-136, preconditioner.rs:39-58) on the resident K
+ * (after fx_assemble / fx_apply_penalty): x0 = 0, returns x (n_dof).
+ * FX_ERR_NOT_CONVERGED after max_iter iterations, like the reference.
+ */
+fx_status fx_solve(fx_mesh *mesh, const double *rhs, double *x_out, const fx_solve_opts *opts,
+                   fx_solve_report *report);
+
+/* ---- step driver (StaticStep::solve, static_step.rs:47-267) ----------- */
+/* Amplitudes (amplitude.rs:10-98); entry 0 must be the default ramp. */
+typedef struct fx_amplitudes {
+    int32_t n;
+    const int32_t *total_time; /* n: 1 = TIME=TOTAL TIME */
+    const double *shift_x;     /* n */
+    const double *shift_y;     /* n */
+    const int32_t *has_data;   /* n: 0 = default ramp */
+    const int64_t *ptr;        /* n+1 offsets into times/values */
+    const double *times;
+    const double *values;
+} fx_amplitudes;
+
+typedef enum fx_solver_kind {
+    FX_SOLVER_DEFAULT = 0,  /* the reference maps Default to DIRECT          */
+    FX_SOLVER_DIRECT = 1,   /* served by PCG with direct_opts (no GPU Cholesky) */
+    FX_SOLVER_ITERATIVE = 2 /* PCG with solve_opts                            */
+} fx_solver_kind;
+
+typedef struct fx_step_def {
+    int32_t step_id; /* 1-based, as passed to Step::solve */
+    int32_t nlgeom;
+    int32_t solver;  /* fx_solver_kind */
+    double initial_time_increment, time_period, min_time_increment, max_time_increment;
+    int64_t max_increments;
+    int64_t n_loads; /* *CLOAD records in deck order (boundary_conds.rs:10-53) */
+    const int64_t *load_dof;
+    const double *load_val;
+    const int32_t *load_amp;
+    const int32_t *load_origin;
+    int64_t n_bcs; /* *BOUNDARY records in deck order (boundary_conds.rs:57-101) */
+    const int64_t *bc_dof;
+    const double *bc_val;
+    const int32_t *bc_amp;
+    const int32_t *bc_origin;
+    fx_amplitudes amps;
+    fx_solve_opts solve_opts;  /* ITERATIVE */
+    fx_solve_opts direct_opts; /* DEFAULT / DIRECT */
+} fx_step_def;
+
+/* SolverTime (time.rs:8-63). */
+typedef struct fx_timer {
+    double global_time, local_time, local_max;
+} fx_timer;
+
+typedef struct fx_step_report {
+    int32_t increments;      /* attempted increments */
+    int32_t cutbacks;        /* increments that failed NR and halved dt */
+    int32_t nr_iterations;   /* NR iterations over all increments */
+    int32_t linear_solves;
+    int64_t pcg_iterations;  /* summed over linear solves */
+    int64_t outer_spmv;      /* MIXED residual SpMVs */
+    double last_rel_residual;
+    double seconds;
+    double seconds_assembly; /* element + assembly + penalty */
+    double seconds_solve;
+} fx_step_report;
+
+/*
+ * Runs one *STATIC step on the resident displacement (set it with
+ * fx_state_set_u first; read it back with fx_state_get_u).  Calls
+ * timer->new_step itself (Step::solve, steps.rs:28-42).
+ */
+fx_status fx_static_step(fx_mesh *mesh, const fx_step_def *step, fx_timer *timer, fx_step_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FERRIX_HIP_H */

@@ -1,0 +1,233 @@
+"""GPU parity: the HIP path against the CPU oracle on the same inputs.
+
+Bars (per the element/assembly arithmetic contract in fx_element.hip):
+* element stiffness, assembled K values, the filtered CSR pattern, f_int and
+  max_diag are compared BIT FOR BIT (integer/index work and the fixed-order
+  element arithmetic);
+* PCG solutions within a stated relative tolerance of the oracle's solve
+  (summation order of SpMV rows and dots differs);
+* full static steps against the oracle and, on the reference's decks,
+  against the CalculiX values the reference's own tests use.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from ferrix_amd import meshgen
+from ferrix_amd.device import DeviceMesh, SolveOptions, run_job
+from ferrix_amd.inp import ConvergenceError, NumericalError, Project
+from ferrix_amd.model import Job, SOLVER_ITERATIVE, STOP_REL_R2, with_solver
+from ferrix_amd import _lib as L
+
+from conftest import deck
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def small_c3d20(jitter=True, nx=6, ny=3, nz=2):
+    return meshgen.cfg2_c3d20_block(seed=7, nx=nx, ny=ny, nz=nz, jitter=jitter)
+
+
+def oracle_csr(job, u=None, bc=None):
+    csr, md, pen = O.assemble_csr(job.mesh, job.node_temp, u, bc)
+    return csr, md, pen
+
+
+def assert_same_csr(dm, csr):
+    rp, col, val = dm.matrix()
+    assert np.array_equal(rp, csr.row_ptr), "row_ptr differs"
+    assert np.array_equal(col, csr.col), "col_idx differs"
+    diff = np.flatnonzero(val != csr.val)
+    assert diff.size == 0, f"{diff.size} values differ, first at {diff[:5]}: {val[diff[:5]]} vs {csr.val[diff[:5]]}"
+
+
+@pytest.mark.parametrize("name,make", [
+    ("cfg1_c3d4", meshgen.cfg1_c3d4_cantilever),
+    ("c3d20_jitter", lambda: small_c3d20(True)),
+    ("c3d20_axis_aligned", lambda: small_c3d20(False)),
+])
+def test_assembly_bitwise(ctx, name, make):
+    say("case", name)
+    job = make()
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    md, nnz = dm.assemble(False)
+    csr, md_o, _ = oracle_csr(job)
+    assert md == md_o
+    assert nnz == csr.nnz
+    assert_same_csr(dm, csr)
+
+
+def test_assembly_nlgeom_bitwise(ctx):
+    job = small_c3d20(True)
+    rng = np.random.default_rng(3)
+    u = rng.normal(scale=0.3, size=job.n_dof)
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.set_u(u)
+    md, nnz = dm.assemble(True)
+    csr, md_o, _ = oracle_csr(job, u)
+    assert md == md_o and nnz == csr.nnz
+    assert_same_csr(dm, csr)
+    for nl in (False, True):
+        f = dm.internal_force(nl)
+        fo = O.internal_force(job.mesh, job.node_temp, u, u if nl else None)
+        assert np.array_equal(f, fo), f"f_int differs (nlgeom={nl}) max {np.max(np.abs(f - fo))}"
+
+
+def test_internal_force_bitwise_c3d4(ctx):
+    job = meshgen.cfg1_c3d4_cantilever()
+    u = np.random.default_rng(1).normal(size=job.n_dof)
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.set_u(u)
+    for nl in (False, True):
+        f = dm.internal_force(nl)
+        fo = O.internal_force(job.mesh, job.node_temp, u, u if nl else None)
+        assert np.array_equal(f, fo)
+
+
+def test_penalty_matches_oracle(ctx):
+    job = meshgen.cfg1_c3d4_cantilever()
+    st = job.steps[0]
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.assemble(False)
+    rhs = np.random.default_rng(2).normal(size=job.n_dof)
+    # duplicate a BC record: the reference adds the penalty twice
+    dofs = np.concatenate([st.bc_dof, st.bc_dof[:3]])
+    targets = np.concatenate([st.bc_val, np.full(3, 0.25)])
+    r_gpu, pen = dm.apply_penalty(dofs, targets, rhs)
+    csr, md, pen_o = oracle_csr(job, None, dofs)
+    assert pen == pen_o
+    r_o = rhs.copy()
+    for d, t in zip(dofs, targets):
+        r_o[d] += pen_o * (t - 0.0)
+    assert np.array_equal(r_gpu, r_o)
+    assert_same_csr(dm, csr)
+
+
+def test_pcg_reference_rule_cfg1(ctx):
+    """sqrt(r.z) < 1e-8, max 1000 (iterative.rs / static_step.rs:204)."""
+    job = meshgen.cfg1_c3d4_cantilever()
+    st = job.steps[0]
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.assemble(False)
+    b = np.zeros(job.n_dof)
+    np.add.at(b, st.load_dof, st.load_val)
+    b, _ = dm.apply_penalty(st.bc_dof, st.bc_val, b)
+    x, rep = dm.solve(b, SolveOptions())
+    csr, _, _ = oracle_csr(job, None, st.bc_dof)
+    xo, it_o, _, ok = O.pcg(csr, b, 1e-8, 1000, 0)
+    assert ok
+    say("iters gpu", rep.iterations, "oracle", it_o)
+    assert abs(rep.iterations - it_o) <= 2
+    xd = O.direct_solve(csr, b)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+
+
+def test_pcg_not_converged(ctx):
+    job = meshgen.cfg1_c3d4_cantilever()
+    st = job.steps[0]
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.assemble(False)
+    b = np.zeros(job.n_dof)
+    np.add.at(b, st.load_dof, st.load_val)
+    b, _ = dm.apply_penalty(st.bc_dof, st.bc_val, b)
+    with pytest.raises(ConvergenceError):
+        dm.solve(b, SolveOptions(max_iter=3))
+
+
+def test_pcg_bitwise_repeatable(ctx):
+    job = small_c3d20(True)
+    st = job.steps[0]
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.assemble(False)
+    b = np.zeros(job.n_dof)
+    np.add.at(b, st.load_dof, st.load_val)
+    b, _ = dm.apply_penalty(st.bc_dof, st.bc_val, b)
+    o = SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100000)
+    x1, r1 = dm.solve(b, o)
+    x2, r2 = dm.solve(b, o)
+    assert r1.iterations == r2.iterations
+    assert np.array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("G", [2, 4, 8, 16, 32, 64])
+def test_pcg_lane_groups(ctx, G):
+    job = small_c3d20(True)
+    st = job.steps[0]
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.assemble(False)
+    b = np.zeros(job.n_dof)
+    np.add.at(b, st.load_dof, st.load_val)
+    b, _ = dm.apply_penalty(st.bc_dof, st.bc_val, b)
+    x, rep = dm.solve(b, SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-11, max_iter=100000, spmv_group=G))
+    csr, _, _ = oracle_csr(job, None, st.bc_dof)
+    xd = O.direct_solve(csr, b)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+
+
+def test_step_cfg1_matches_oracle(ctx):
+    job = meshgen.cfg1_c3d4_cantilever()
+    u_o, logs = O.run_job(job)
+    u, reps = run_job(ctx, job)
+    say("nr", reps[0].nr_iterations, "pcg", reps[0].pcg_iterations)
+    assert reps[0].linear_solves == logs[0].linear_solves
+    assert np.linalg.norm(u - u_o) / np.linalg.norm(u_o) < 1e-8
+
+
+def test_step_c3d20_iterative_rel(ctx):
+    job = small_c3d20(True, 10, 4, 3)
+    u_o, logs = O.run_job(job)
+    u, reps = run_job(ctx, job)
+    assert reps[0].linear_solves == logs[0].linear_solves
+    assert np.linalg.norm(u - u_o) / np.linalg.norm(u_o) < 1e-8
+
+
+def test_step_mixed_precision(ctx):
+    job = small_c3d20(True, 10, 4, 3)
+    u_o, _ = O.run_job(job)
+    u, reps = run_job(ctx, job, precision=L.FX_PREC_MIXED)
+    say("mixed pcg", reps[0].pcg_iterations, "outer", reps[0].outer_spmv)
+    assert np.linalg.norm(u - u_o) / np.linalg.norm(u_o) < 1e-8
+
+
+def test_singular_jacobian(ctx):
+    job = meshgen.cfg1_c3d4_cantilever()
+    m = job.mesh
+    xyz = m.xyz.copy()
+    # collapse element 5 onto a plane: all four nodes at the first node
+    nodes = m.conn[m.el_ptr[5]:m.el_ptr[6]]
+    for n in nodes[1:]:
+        xyz[n] = xyz[nodes[0]]
+    m.xyz = xyz
+    dm = DeviceMesh(ctx, m, job.node_temp)
+    with pytest.raises(NumericalError, match="Singular Jacobian"):
+        dm.assemble(False)
+
+
+@pytest.mark.parametrize("name,node,expected,tol", [
+    ("C3D4_staticstep", 17, [(9.356107e-06, -1.313901e-04, 1.719218e-03)], 1e-6),
+    ("C3D20", 17, [(5.752112e-02, -2.497816e-04, -8.074951e-03)], 1e-6),
+    ("C3D4_2staticstep", 17, [(9.356107e-06, -1.313901e-04, 1.719218e-03),
+                             (2.603480e-04, -1.377070e-04, 1.001299e-03)], 1e-6),
+])
+def test_reference_decks(ctx, name, node, expected, tol):
+    """calculix_parity.rs known answers, run through the GPU step driver
+    (DIRECT served by FP64 PCG to ||r||/||b|| <= 1e-12)."""
+    say("deck", name)
+    p = Project.from_jobname(deck(name))
+    p.default_initial_temperature = 20.0
+    job = Job.from_project(p)
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.set_u(np.zeros(job.n_dof))
+    timer = L.fx_timer(0.0, 0.0, 1.0)
+    idx = p.get_index_for_node_id(node)
+    for k, st in enumerate(job.steps):
+        dm.static_step(st, k + 1, timer)
+        u = dm.get_u()
+        got = u[3 * idx:3 * idx + 3]
+        say("step", k + 1, got, expected[k])
+        assert np.all(np.abs(got - np.array(expected[k])) < tol)
