@@ -85,6 +85,7 @@ SIGNATURES = {
     "fx_solve_opts_default": (None, [P]),
     "fx_solve": (i32, [P, P, P, P, P]),
     "fx_static_step": (i32, [P, P, P, P]),
+    "fx_profile_spmv": (i32, [P, i32, i32, i32, P, P]),
 }
 
 _LIB = None

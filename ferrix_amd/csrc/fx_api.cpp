@@ -248,6 +248,9 @@ void fx_ctx_destroy(fx_ctx *ctx) {
         return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (fx_mesh *m : ctx->meshes)
+        delete m;
+    ctx->meshes.clear();
     for (auto *b : {&ctx->tables.n4, &ctx->tables.dn4, &ctx->tables.w4, &ctx->tables.n20, &ctx->tables.dn20,
                     &ctx->tables.w20})
         b->release();
@@ -376,6 +379,7 @@ fx_status fx_mesh_upload(fx_ctx *ctx, int64_t n_nodes, const double *xyz, const 
                      m->kval.bytes(), m->keep.bytes(), m->part.bytes()})
         m->device_bytes += (int64_t)b;
     m->device_bytes += 13 * m->n_dof * (int64_t)sizeof(double);
+    ctx->meshes.push_back(m);
     *out = m;
     m = nullptr;
     }
@@ -397,8 +401,13 @@ fx_status fx_mesh_upload(fx_ctx *ctx, int64_t n_nodes, const double *xyz, const 
 void fx_mesh_destroy(fx_mesh *m) {
     if (!m)
         return;
-    (void)hipSetDevice(m->ctx->device);
-    (void)hipStreamSynchronize(m->ctx->stream);
+    fx_ctx *ctx = m->ctx;
+    auto it = std::find(ctx->meshes.begin(), ctx->meshes.end(), m);
+    if (it == ctx->meshes.end())
+        return;  // not a live mesh of its context
+    ctx->meshes.erase(it);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     delete m;
 }
 
@@ -418,9 +427,11 @@ fx_status fx_state_set_u(fx_mesh *m, const double *u) {
     if (!m)
         return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
     FX_API_BEGIN
-    require(u != nullptr, "fx_state_set_u: u is NULL");
     set_device(m->ctx);
-    FX_HIP(hipMemcpyAsync(m->u.p, u, m->n_dof * sizeof(double), hipMemcpyHostToDevice, m->ctx->stream));
+    if (u)
+        FX_HIP(hipMemcpyAsync(m->u.p, u, m->n_dof * sizeof(double), hipMemcpyHostToDevice, m->ctx->stream));
+    else
+        fx::launch_fill(m, m->u.p, m->n_dof, 0.0);
     FX_HIP(hipStreamSynchronize(m->ctx->stream));
     FX_API_END(m->ctx)
 }
@@ -433,6 +444,18 @@ fx_status fx_state_get_u(fx_mesh *m, double *u) {
     set_device(m->ctx);
     FX_HIP(hipMemcpyAsync(u, m->u.p, m->n_dof * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
     FX_HIP(hipStreamSynchronize(m->ctx->stream));
+    FX_API_END(m->ctx)
+}
+
+fx_status fx_profile_spmv(fx_mesh *m, int32_t precision, int32_t reps, int32_t spmv_group, double *ms_per_launch,
+                          double *bytes_per_launch) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(m->k_valid, "fx_profile_spmv: assemble first");
+    require(reps > 0 && ms_per_launch && bytes_per_launch, "fx_profile_spmv: bad argument");
+    set_device(m->ctx);
+    *ms_per_launch = fx::profile_spmv(m, precision, reps, spmv_group, bytes_per_launch);
     FX_API_END(m->ctx)
 }
 

@@ -18,7 +18,7 @@
 // pattern stays the compute pattern; slots whose contributions were all
 // dropped hold 0.0 and keep = 0, so the exported CSR matches the filtered
 // one while SpMV sums are unchanged (0 * x adds nothing).
-#include "fx_internal.h"
+#include "fx_reduce.h"
 
 namespace fx {
 namespace {
@@ -144,57 +144,22 @@ __global__ void k_penalty(int64_t n_unique, const int32_t *__restrict__ ud_dof, 
 // in block order).
 constexpr int kRedThreads = 256;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 __global__ __launch_bounds__(kRedThreads) void k_masked_norms(int64_t n, const double *__restrict__ r,
                                                              const double *__restrict__ f,
                                                              const uint8_t *__restrict__ mask,
                                                              double *__restrict__ part, uint32_t *__restrict__ cnt,
                                                              double *__restrict__ out) {
-    __shared__ double s0[kRedThreads / 64], s1[kRedThreads / 64];
-    __shared__ bool last;
-    double a = 0.0, b = 0.0;
+    double v[2] = {0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (!mask[i]) {
-            a += r[i] * r[i];
-            b += f[i] * f[i];
+            v[0] += r[i] * r[i];
+            v[1] += f[i] * f[i];
         }
     }
-    a = wave_sum(a);
-    b = wave_sum(b);
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    if (l == 0) {
-        s0[w] = a;
-        s1[w] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double ta = 0.0, tb = 0.0;
-        for (int k = 0; k < kRedThreads / 64; ++k) {
-            ta += s0[k];
-            tb += s1[k];
-        }
-        part[2 * blockIdx.x] = ta;
-        part[2 * blockIdx.x + 1] = tb;
-        __threadfence();
-        last = atomicAdd(cnt, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double ta = 0.0, tb = 0.0;
-        for (unsigned k = 0; k < gridDim.x; ++k) {
-            ta += __hip_atomic_load(&part[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            tb += __hip_atomic_load(&part[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        out[0] = ta;
-        out[1] = tb;
-        *cnt = 0;
+    double s[2];
+    if (grid_sum<2>(v, part, cnt, s)) {
+        out[0] = s[0];
+        out[1] = s[1];
     }
 }
 

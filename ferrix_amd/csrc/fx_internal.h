@@ -12,6 +12,8 @@
 
 #include "../../include/ferrix_hip.h"
 
+struct fx_mesh;
+
 namespace fx {
 
 // ---------------------------------------------------------------------------
@@ -106,14 +108,26 @@ struct ElemBlock {
     int64_t fe_offset = 0;  // first value in the f contribution buffer
 };
 
-// PCG scalars in device memory
+// PCG state in device memory.  Each field is written by block 0 of one
+// kernel kind only and read by the other kind (rz is double-buffered by
+// iteration parity), so no kernel reads a field it may be writing.
 struct PcgScalars {
-    double rz, rr, bb, pap, alpha, beta, rz0, crit;
     double tol;
-    int64_t iter, max_iter;
-    int32_t status, stop_mode;  // status: 0 running, 1 converged, 2 breakdown, 3 max_iter
-    uint32_t cnt_a, cnt_b;      // last-block counters
-    uint32_t pad[2];
+    int64_t max_iter;
+    int32_t stop_mode;
+    int32_t status;  // 0 running, 1 converged, 2 breakdown, 3 max_iter
+    int64_t it;      // next iteration index (written by k_update)
+    int64_t it_k1;   // current iteration index (written by k_spmv_p)
+    double rz[2];    // r.z at iteration parity (k_spmv_p)
+    double rr, bb, crit;  // last r.r, b.b, stop-rule value (k_spmv_p)
+    double pap, alpha;    // last p.Ap and alpha (k_update)
+};
+
+// One captured batch of PCG iterations.
+struct PcgGraph {
+    int vt, group, chunk;
+    const void *val;
+    hipGraphExec_t exec;
 };
 
 // Gauss tables in device memory, one set per context (uploaded at creation).
@@ -129,6 +143,7 @@ struct fx_ctx {
     int n_cu = 0;
     std::string err;
     fx::DevTables tables;
+    std::vector<fx_mesh *> meshes;  // live meshes, destroyed with the context
 };
 
 struct fx_mesh {
@@ -165,6 +180,11 @@ struct fx_mesh {
     bool k_valid = false;
     int spmv_group = 0;                  // autotuned lanes per node row
     int64_t device_bytes = 0;
+    std::vector<fx::PcgGraph> graphs;    // captured iteration batches
+    ~fx_mesh() {
+        for (auto &g : graphs)
+            (void)hipGraphExecDestroy(g.exec);
+    }
 };
 
 namespace fx {
@@ -180,6 +200,7 @@ double device_max_diag(fx_mesh *m);
 // PCG (fx_pcg.hip)
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep);
 void spmv64(fx_mesh *m, const double *x, double *y, int group);
+double profile_spmv(fx_mesh *m, int precision, int reps, int group, double *bytes);
 void axpy_dev(fx_mesh *m, int64_t n, double a, const double *x, double *y);
 // step helpers (fx_assemble.hip)
 void launch_residual(fx_mesh *m, const double *f_ext, const double *f_int, double *r, int64_t n);

@@ -3,121 +3,81 @@
 // Reference: src/solver/solvers/iterative.rs:26-136 (PCG, CSR SpMV) and
 // src/solver/preconditioner.rs:39-58 (Jacobi).  Same recurrences, same
 // per-entry arithmetic (x += a p, r -= a Ap, z = r * inv_diag,
-// p = z + beta p, beta = rz_new / rz_old), same stopping rule evaluated at
-// the top of every iteration, x0 = 0.  Dot products and row sums are
-// reduced in a fixed order (lane groups, then waves, then blocks in index
-// order), so a solve is bitwise repeatable.
+// p = z + beta p, beta = rz_new / rz_old), the stopping rule evaluated at the
+// top of every iteration i < max_iter, x0 = 0.
 //
-// One iteration = two kernels, no host round trip:
-//   k_spmv_p : p_i = z + beta p_{i-1} formed on the fly for every gathered
-//              column (ping-pong p buffers), Ap = K p_i, p_i . Ap;
-//              the last block computes alpha.
-//   k_update : x, r, z updates, r.z and r.r; the last block computes beta,
-//              advances the iteration count and evaluates the stop rule.
-// Kernels read the iteration state from device memory, so the host issues
-// batches of iterations (captured once in a hipGraph) and polls the status
-// word between batches; iterations after convergence return at once.
+// One iteration i = two kernels and no host round trip:
+//   k_spmv_p(i) : reduces the r.z / r.r partials of the previous update,
+//                 evaluates the stop rule, forms p_i = z + beta p_{i-1} on the
+//                 fly for every gathered column (ping-pong p buffers),
+//                 Ap = K p_i and the p_i . Ap block partials;
+//   k_update(i) : reduces the p.Ap partials, alpha = rz / pAp, updates x, r,
+//                 z and writes the r.z / r.r block partials.
+// No block waits for another inside a kernel: each kernel writes per-block
+// partial sums and EVERY block of the next kernel reduces all of them in the
+// same fixed order (so all blocks agree bitwise and results repeat run to
+// run).  Kernel boundaries give the cross-XCD coherence, so there are no
+// device-scope fences or atomics in the iteration.  Block 0 records the
+// scalars the host polls; each field is written by one kernel kind and read
+// only by the other one (rz is double-buffered by iteration parity).
 //
-// SpMV layout: the structural pattern is made of 3x3 node blocks.  Values of
-// node row a are stored row-major per DOF row (the scalar CSR order), so
-// block k of row a holds rows p=0..2 at val[9*nb_ptr[a] + p*3*deg + 3k + q].
-// A group of G lanes owns one node row; a lane reads one column index per
-// 9 values and gathers 3 consecutive entries of z and p.  Workgroups are
+// SpMV layout: the structural pattern is made of 3x3 node blocks; values of
+// node row a are stored per DOF row (scalar CSR order): row 3a+p, column
+// slot j (block j/3, component j%3) at val[9*nb_ptr[a] + p*3*deg + j].
+// A group of G lanes owns node row a and walks the 3*deg column slots: lane
+// j loads the three rows' values at j (coalesced across lanes), one column
+// index per 3 slots and one entry of z and of p_{i-1}.  Workgroups are
 // renumbered so that each XCD (round-robin dispatch) sweeps a contiguous
-// range of node rows, which keeps gathered columns in its own L2.
-#include "fx_internal.h"
+// range of node rows and its L2 keeps the gathered columns.
+#include "fx_reduce.h"
 
 #include <chrono>
+#include <cstdlib>
 
 namespace fx {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kUnroll = 4;  // column slots per lane with loads in flight
 
 __device__ __forceinline__ unsigned xcd_swizzle(unsigned id, unsigned nb) {
     const unsigned q = nb / 8, r = nb % 8, x = id % 8, i = id / 8;
     return x * q + (x < r ? x : r) + i;
 }
 
+// Sum of part[NV*b + k] over b < nparts in a fixed order; the result is
+// returned to every thread of the block.
 template <int NV>
-__device__ __forceinline__ void block_reduce_to_part(double (&v)[NV], double *part) {
-    __shared__ double s[kBlock / 64][NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double t = v[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-            t += __shfl_xor(t, off, 64);
-        v[k] = t;
-    }
-    const int w = threadIdx.x / 64;
-    if (threadIdx.x % 64 == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            s[w][k] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double t = 0.0;
-#pragma unroll
-            for (int j = 0; j < kBlock / 64; ++j)
-                t += s[j][k];
-            part[NV * blockIdx.x + k] = t;
-        }
-    }
-}
-
-// Returns true in thread 0 of the last block to arrive; that thread then
-// sums the partials in block order into out[].
-template <int NV>
-__device__ __forceinline__ bool last_block_sum(const double *part, uint32_t *cnt, double (&out)[NV]) {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(cnt, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last || threadIdx.x != 0)
-        return false;
-    __threadfence();
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ part, int nparts, double (&out)[NV]) {
+    __shared__ double bcast[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
         out[k] = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b)
+    for (int b = threadIdx.x; b < nparts; b += kBlock)
 #pragma unroll
         for (int k = 0; k < NV; ++k)
-            out[k] += __hip_atomic_load(&part[NV * b + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *cnt = 0;
-    return true;
+            out[k] += part[NV * b + k];
+    block_sum<NV>(out);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            bcast[k] = out[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        out[k] = bcast[k];
 }
 
-__device__ __forceinline__ void stop_rule(PcgScalars *sc) {
-    double crit;
-    bool conv;
-    if (sc->stop_mode == FX_STOP_ABS_SQRT_RZ) {
-        crit = sqrt(sc->rz);
-        conv = crit < sc->tol;
-    } else {
-        crit = sc->bb > 0.0 ? sqrt(sc->rr) / sqrt(sc->bb) : 0.0;
-        conv = crit <= sc->tol;
-    }
-    sc->crit = crit;
-    if (conv)
-        sc->status = 1;
-    else if (sc->iter >= sc->max_iter)
-        sc->status = 3;
-}
-
+// x = 0, r = b, z = r inv, p_{-1} = z; partials (r.z, r.r, b.b) -> part2.
 __global__ __launch_bounds__(kBlock) void k_init(int64_t n, const double *__restrict__ b, double *__restrict__ x,
                                                  double *__restrict__ r, double *__restrict__ z,
                                                  double *__restrict__ p0, const double *__restrict__ inv,
-                                                 PcgScalars *__restrict__ sc, double *__restrict__ part) {
+                                                 double *__restrict__ part2) {
     double v[3] = {0.0, 0.0, 0.0};
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
-        double bi = b[i];
-        double zi = bi * inv[i];
+        const double bi = b[i];
+        const double zi = bi * inv[i];
         x[i] = 0.0;
         r[i] = bi;
         z[i] = zi;
@@ -126,17 +86,11 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t n, const double *__rest
         v[1] = bi * bi;
         v[2] = bi * bi;
     }
-    block_reduce_to_part<3>(v, part);
-    double s[3];
-    if (last_block_sum<3>(part, &sc->cnt_b, s)) {
-        sc->rz = s[0];
-        sc->rz0 = s[0];
-        sc->rr = s[1];
-        sc->bb = s[2];
-        sc->beta = 0.0;
-        sc->iter = 0;
-        sc->status = 0;
-        stop_rule(sc);
+    block_sum<3>(v);
+    if (threadIdx.x == 0) {
+        part2[3 * blockIdx.x] = v[0];
+        part2[3 * blockIdx.x + 1] = v[1];
+        part2[3 * blockIdx.x + 2] = v[2];
     }
 }
 
@@ -145,36 +99,82 @@ __global__ __launch_bounds__(kBlock) void k_spmv_p(int64_t n_nodes, const int32_
                                                    const int32_t *__restrict__ nb_col, const VT *__restrict__ val,
                                                    const double *__restrict__ z, double *__restrict__ pa,
                                                    double *__restrict__ pb, double *__restrict__ ap,
-                                                   PcgScalars *__restrict__ sc, double *__restrict__ part) {
+                                                   PcgScalars *__restrict__ sc, const double *__restrict__ part2,
+                                                   int nparts2, double *__restrict__ part1) {
     if (sc->status != 0)
         return;
-    const double beta = sc->beta;
-    const int64_t it = sc->iter;
+    const int64_t it = sc->it;
+    // r.z, r.r (and b.b at i = 0) after the previous update
+    double red[3];
+    reduce_partials<3>(part2, nparts2, red);
+    const double rz = red[0], rr = red[1];
+    const double bb = it == 0 ? red[2] : sc->bb;
+    double crit;
+    bool conv;
+    if (sc->stop_mode == FX_STOP_ABS_SQRT_RZ) {
+        crit = sqrt(rz);
+        conv = crit < sc->tol;
+    } else {
+        crit = bb > 0.0 ? sqrt(rr) / sqrt(bb) : 0.0;
+        conv = crit <= sc->tol;
+    }
+    const bool stop_max = it >= sc->max_iter;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->rz[it & 1] = rz;
+        sc->rr = rr;
+        sc->crit = crit;
+        sc->it_k1 = it;
+        if (it == 0)
+            sc->bb = bb;
+        if (stop_max)
+            sc->status = 3;
+        else if (conv)
+            sc->status = 1;
+    }
+    if (stop_max || conv)
+        return;
+    const double beta = it == 0 ? 0.0 : rz / sc->rz[(it - 1) & 1];
     const double *pold = (it & 1) ? pb : pa;
     double *pnew = (it & 1) ? pa : pb;
+
     const unsigned bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int64_t a = (int64_t)bid * (kBlock / G) + threadIdx.x / G;
     const int lane = threadIdx.x % G;
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     if (a < n_nodes) {
-        const int b0 = nb_ptr[a], deg = nb_ptr[a + 1] - b0;
-        const VT *v = val + 9 * (int64_t)b0;
-        const int stride = 3 * deg;
-        for (int k = lane; k < deg; k += G) {
-            const int64_t c = 3 * (int64_t)nb_col[b0 + k];
-            const double x0 = z[c] + beta * pold[c];
-            const double x1 = z[c + 1] + beta * pold[c + 1];
-            const double x2 = z[c + 2] + beta * pold[c + 2];
-            const VT *w = v + 3 * k;
-            acc0 += (double)w[0] * x0;
-            acc0 += (double)w[1] * x1;
-            acc0 += (double)w[2] * x2;
-            acc1 += (double)w[stride] * x0;
-            acc1 += (double)w[stride + 1] * x1;
-            acc1 += (double)w[stride + 2] * x2;
-            acc2 += (double)w[2 * stride] * x0;
-            acc2 += (double)w[2 * stride + 1] * x1;
-            acc2 += (double)w[2 * stride + 2] * x2;
+        const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
+        const VT *v0 = val + 9 * (int64_t)b0;
+        // kUnroll slots per lane in flight; the accumulation order is slot
+        // j, j+G, j+2G, ... exactly as in a plain strided loop
+        for (int j0 = lane; j0 < ncol; j0 += kUnroll * G) {
+            int64_t c[kUnroll];
+            double w0[kUnroll], w1[kUnroll], w2[kUnroll], zz[kUnroll], pp[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const int j = j0 + u * G;
+                const int kb = j / 3;
+                c[u] = j < ncol ? 3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb) : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const int j = j0 + u * G;
+                if (c[u] >= 0) {
+                    w0[u] = (double)v0[j];
+                    w1[u] = (double)v0[ncol + j];
+                    w2[u] = (double)v0[2 * ncol + j];
+                    zz[u] = z[c[u]];
+                    pp[u] = pold[c[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (c[u] >= 0) {
+                    const double xj = zz[u] + beta * pp[u];
+                    acc0 += w0[u] * xj;
+                    acc1 += w1[u] * xj;
+                    acc2 += w2[u] * xj;
+                }
+            }
         }
     }
 #pragma unroll
@@ -197,30 +197,34 @@ __global__ __launch_bounds__(kBlock) void k_spmv_p(int64_t n_nodes, const int32_
         ap[d + 2] = acc2;
         loc[0] = ((0.0 + q0 * acc0) + q1 * acc1) + q2 * acc2;
     }
-    block_reduce_to_part<1>(loc, part);
-    double s[1];
-    if (last_block_sum<1>(part, &sc->cnt_a, s)) {
-        const double pap = s[0];
-        const double alpha = sc->rz / pap;
-        sc->pap = pap;
-        sc->alpha = alpha;
-        if (!(pap > 0.0) || !isfinite(alpha))
-            sc->status = 2;
-    }
+    block_sum<1>(loc);
+    if (threadIdx.x == 0)
+        part1[blockIdx.x] = loc[0];
 }
 
 __global__ __launch_bounds__(kBlock) void k_update(int64_t n, double *__restrict__ x, double *__restrict__ r,
                                                    double *__restrict__ z, const double *__restrict__ pa,
                                                    const double *__restrict__ pb, const double *__restrict__ ap,
                                                    const double *__restrict__ inv, PcgScalars *__restrict__ sc,
-                                                   double *__restrict__ part) {
+                                                   const double *__restrict__ part1, int nparts1,
+                                                   double *__restrict__ part2) {
     if (sc->status != 0)
         return;
-    const double alpha = sc->alpha;
-    const int64_t it = sc->iter;
-    const double *p = (it & 1) ? pa : pb;  // p_i was written to the "new" buffer
-    double v[2] = {0.0, 0.0};
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t it = sc->it_k1;
+    double red[1];
+    reduce_partials<1>(part1, nparts1, red);
+    const double pap = red[0];
+    const double alpha = sc->rz[it & 1] / pap;
+    if (!(pap > 0.0) || !isfinite(alpha)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->status = 2;
+            sc->pap = pap;
+        }
+        return;
+    }
+    const double *p = (it & 1) ? pa : pb;  // p_i is in the "new" buffer of iteration i
+    double v[3] = {0.0, 0.0, 0.0};
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
         const double xi = x[i] + alpha * p[i];
         const double ri = r[i] - alpha * ap[i];
@@ -231,18 +235,20 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, double *__restrict
         v[0] = ri * zi;
         v[1] = ri * ri;
     }
-    block_reduce_to_part<2>(v, part);
-    double s[2];
-    if (last_block_sum<2>(part, &sc->cnt_b, s)) {
-        sc->beta = s[0] / sc->rz;
-        sc->rz = s[0];
-        sc->rr = s[1];
-        sc->iter = it + 1;
-        stop_rule(sc);
+    block_sum<3>(v);
+    if (threadIdx.x == 0) {
+        part2[3 * blockIdx.x] = v[0];
+        part2[3 * blockIdx.x + 1] = v[1];
+        part2[3 * blockIdx.x + 2] = 0.0;
+        if (blockIdx.x == 0) {
+            sc->it = it + 1;
+            sc->alpha = alpha;
+            sc->pap = pap;
+        }
     }
 }
 
-// y = K x (plain SpMV, FP64 values), used for true residuals.
+// y = K x (FP64 values), same column-sliced layout; used for true residuals.
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_nodes, const int32_t *__restrict__ nb_ptr,
                                                  const int32_t *__restrict__ nb_col, const double *__restrict__ val,
@@ -252,22 +258,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_nodes, const int32_t 
     const int lane = threadIdx.x % G;
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     if (a < n_nodes) {
-        const int b0 = nb_ptr[a], deg = nb_ptr[a + 1] - b0;
-        const double *v = val + 9 * (int64_t)b0;
-        const int stride = 3 * deg;
-        for (int k = lane; k < deg; k += G) {
-            const int64_t c = 3 * (int64_t)nb_col[b0 + k];
-            const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
-            const double *w = v + 3 * k;
-            acc0 += w[0] * x0;
-            acc0 += w[1] * x1;
-            acc0 += w[2] * x2;
-            acc1 += w[stride] * x0;
-            acc1 += w[stride + 1] * x1;
-            acc1 += w[stride + 2] * x2;
-            acc2 += w[2 * stride] * x0;
-            acc2 += w[2 * stride + 1] * x1;
-            acc2 += w[2 * stride + 2] * x2;
+        const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
+        const double *v0 = val + 9 * (int64_t)b0;
+        for (int j = lane; j < ncol; j += G) {
+            const int kb = j / 3;
+            const double xj = x[3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb)];
+            acc0 += v0[j] * xj;
+            acc1 += v0[ncol + j] * xj;
+            acc2 += v0[2 * ncol + j] * xj;
         }
     }
 #pragma unroll
@@ -283,11 +281,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_nodes, const int32_t 
     }
 }
 
-// r = b - K x ; returns via part/last block: r.r, b.b, r.(inv r)
+// r = b - K x ; block partials of (r.r, b.b, r.(inv r)) -> part.
 __global__ __launch_bounds__(kBlock) void k_true_residual(int64_t n, const double *__restrict__ b,
                                                           const double *__restrict__ kx, const double *__restrict__ inv,
-                                                          double *__restrict__ r, double *__restrict__ part,
-                                                          uint32_t *__restrict__ cnt, double *__restrict__ out) {
+                                                          double *__restrict__ r, double *__restrict__ part) {
     double v[3] = {0.0, 0.0, 0.0};
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
@@ -297,13 +294,23 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(int64_t n, const doubl
         v[1] = b[i] * b[i];
         v[2] = ri * (ri * inv[i]);
     }
-    block_reduce_to_part<3>(v, part);
-    double s[3];
-    if (last_block_sum<3>(part, cnt, s)) {
-        out[0] = s[0];
-        out[1] = s[1];
-        out[2] = s[2];
-    }
+    block_sum<3>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            part[3 * blockIdx.x + k] = v[k];
+}
+
+// One block: fixed-order sum of NV-wide partials into out (device).
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restrict__ part, int nparts,
+                                                         double *__restrict__ out) {
+    double red[NV];
+    reduce_partials<NV>(part, nparts, red);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            out[k] = red[k];
 }
 
 __global__ void k_add(int64_t n, const double *__restrict__ d, double *__restrict__ x) {
@@ -314,15 +321,20 @@ __global__ void k_add(int64_t n, const double *__restrict__ d, double *__restric
 
 inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// partial buffers: part1 (one per SpMV block) then part2 (3 per update block)
+inline double *part1_of(fx_mesh *m) { return m->part.p; }
+inline double *part2_of(fx_mesh *m) { return m->part.p + (m->n_nodes + 64); }
+
 template <int G, typename VT>
 void launch_spmv_p(fx_mesh *m, const VT *val, hipStream_t s) {
-    unsigned nb = nblocks(m->n_nodes, kBlock / G);
+    const unsigned nb = nblocks(m->n_nodes, kBlock / G);
     hipLaunchKernelGGL((k_spmv_p<G, VT>), dim3(nb), dim3(kBlock), 0, s, m->n_nodes, m->nb_ptr.p, m->nb_col.p, val,
-                       m->z.p, m->p0.p, m->p1.p, m->ap.p, m->pcg.p, m->part.p);
+                       m->z.p, m->p0.p, m->p1.p, m->ap.p, m->pcg.p, part2_of(m), (int)nblocks(m->n_dof, kBlock),
+                       part1_of(m));
 }
 
 template <typename VT>
-void launch_iteration(fx_mesh *m, const VT *val, int G, hipStream_t s) {
+void launch_spmv_any(fx_mesh *m, const VT *val, int G, hipStream_t s) {
     switch (G) {
     case 2: launch_spmv_p<2, VT>(m, val, s); break;
     case 4: launch_spmv_p<4, VT>(m, val, s); break;
@@ -331,8 +343,14 @@ void launch_iteration(fx_mesh *m, const VT *val, int G, hipStream_t s) {
     case 32: launch_spmv_p<32, VT>(m, val, s); break;
     default: launch_spmv_p<64, VT>(m, val, s); break;
     }
+}
+
+template <typename VT>
+void launch_iteration(fx_mesh *m, const VT *val, int G, hipStream_t s) {
+    launch_spmv_any<VT>(m, val, G, s);
     hipLaunchKernelGGL(k_update, dim3(nblocks(m->n_dof, kBlock)), dim3(kBlock), 0, s, m->n_dof, m->x.p, m->r.p,
-                       m->z.p, m->p0.p, m->p1.p, m->ap.p, m->inv_diag.p, m->pcg.p, m->part.p);
+                       m->z.p, m->p0.p, m->p1.p, m->ap.p, m->inv_diag.p, m->pcg.p, part1_of(m),
+                       (int)nblocks(m->n_nodes, kBlock / G), part2_of(m));
 }
 
 template <int G>
@@ -341,27 +359,16 @@ void launch_spmv_g(fx_mesh *m, const double *x, double *y, hipStream_t s) {
                        m->nb_ptr.p, m->nb_col.p, m->kval.p, x, y);
 }
 
-// Average node-row degree -> lanes per node row.
-int default_group(fx_mesh *m) {
-    double deg = (double)m->nblk / (double)std::max<int64_t>(1, m->n_nodes);
-    if (deg <= 6)
-        return 2;
-    if (deg <= 14)
-        return 4;
-    if (deg <= 28)
-        return 8;
-    if (deg <= 56)
-        return 16;
-    return 32;
-}
-
-struct PcgGraph {
-    hipGraphExec_t exec = nullptr;
-    int G = 0, chunk = 0;
-    const void *val = nullptr;
-};
-
 }  // namespace
+
+// Average columns per DOF row -> lanes per node row (about 4 slots a lane).
+int default_group(fx_mesh *m) {
+    const double ncol = 3.0 * (double)m->nblk / (double)std::max<int64_t>(1, m->n_nodes);
+    int G = 2;
+    while (G < 64 && 2 * G <= ncol / 4.0)
+        G *= 2;
+    return G;
+}
 
 void spmv64(fx_mesh *m, const double *x, double *y, int G) {
     hipStream_t s = m->ctx->stream;
@@ -376,11 +383,12 @@ void spmv64(fx_mesh *m, const double *x, double *y, int G) {
     FX_HIP(hipGetLastError());
 }
 
-// Host loop of one PCG solve on the resident operator `val` (FP64 or FP32
-// values) with rhs b_dev; result in m->x.  Returns the device status.
-template <typename VT>
-static int run_pcg(fx_mesh *m, const VT *val, const double *b_dev, double tol, int64_t max_iter, int stop_mode,
-                   int G, PcgScalars *out) {
+static void ensure_vectors(fx_mesh *m) {
+    for (auto *b : {&m->x, &m->r, &m->z, &m->p0, &m->p1, &m->ap, &m->tmp})
+        b->alloc(m->n_dof);
+}
+
+static void pcg_start(fx_mesh *m, const double *b_dev, double tol, int64_t max_iter, int stop_mode) {
     hipStream_t s = m->ctx->stream;
     PcgScalars init{};
     init.tol = tol;
@@ -388,33 +396,137 @@ static int run_pcg(fx_mesh *m, const VT *val, const double *b_dev, double tol, i
     init.stop_mode = stop_mode;
     FX_HIP(hipMemcpyAsync(m->pcg.p, &init, sizeof init, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_init, dim3(nblocks(m->n_dof, kBlock)), dim3(kBlock), 0, s, m->n_dof, b_dev, m->x.p, m->r.p,
-                       m->z.p, m->p0.p, m->inv_diag.p, m->pcg.p, m->part.p);
+                       m->z.p, m->p0.p, m->inv_diag.p, part2_of(m));
     FX_HIP(hipGetLastError());
-    // batches of iterations; the batch grows so that long solves poll rarely
-    int chunk = 8;
+}
+
+// Times `reps` back-to-back launches of the PCG SpMV kernel (the same
+// kernel the solve runs, on the context stream) between two HIP events.
+// The PCG state comes from an init pass on b = 1 with tol 0, so every launch
+// does the full work (the stop rule never fires).
+double profile_spmv(fx_mesh *m, int precision, int reps, int G, double *bytes) {
+    hipStream_t s = m->ctx->stream;
+    const int64_t n = m->n_dof;
+    if (G <= 0)
+        G = m->spmv_group > 0 ? m->spmv_group : default_group(m);
+    ensure_vectors(m);
+    launch_jacobi(m, false);
+    if (precision == FX_PREC_MIXED)
+        launch_convert32(m);
+    launch_fill(m, m->tmp.p, n, 1.0);
+    pcg_start(m, m->tmp.p, 0.0, INT64_MAX, FX_STOP_ABS_SQRT_RZ);
+    hipEvent_t e0, e1;
+    FX_HIP(hipEventCreate(&e0));
+    FX_HIP(hipEventCreate(&e1));
+    auto one = [&]() {
+        if (precision == FX_PREC_MIXED)
+            launch_spmv_any<float>(m, m->kval32.p, G, s);
+        else
+            launch_spmv_any<double>(m, m->kval.p, G, s);
+    };
+    for (int k = 0; k < 3; ++k)
+        one();
+    FX_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; ++k)
+        one();
+    FX_HIP(hipEventRecord(e1, s));
+    FX_HIP(hipGetLastError());
+    FX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const double vs = precision == FX_PREC_MIXED ? 4.0 : 8.0;
+    // values + one column index per 3x3 block + node-row pointers
+    // + z and p gathered (each at least once) + p_new and Ap written
+    *bytes = vs * (double)m->nnz_struct + 4.0 * (double)m->nblk + 4.0 * (double)(m->n_nodes + 1) +
+             32.0 * (double)n;
+    return (double)ms / reps;
+}
+
+// Iterations are issued in batches (a captured hipGraph per batch size) and
+// the status block is read after each batch.  The next batch is the largest
+// power of two not above the iterations still predicted from the observed
+// convergence rate, so few launches run past convergence (they return at
+// once).  FX_PCG_GRAPH=0 issues plain launches instead of graphs.
+static bool use_graphs() {
+    const char *e = std::getenv("FX_PCG_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+template <typename VT>
+static hipGraphExec_t batch_graph(fx_mesh *m, const VT *val, int G, int chunk) {
+    const int vt = sizeof(VT) == 4 ? 1 : 0;
+    for (auto &g : m->graphs)
+        if (g.vt == vt && g.group == G && g.chunk == chunk && g.val == (const void *)val)
+            return g.exec;
+    hipStream_t s = m->ctx->stream;
+    hipGraph_t graph;
+    FX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < chunk; ++k)
+        launch_iteration<VT>(m, val, G, s);
+    FX_HIP(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec;
+    FX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    m->graphs.push_back({vt, G, chunk, (const void *)val, exec});
+    return exec;
+}
+
+// One PCG solve on the resident operator `val` with rhs b_dev; x in m->x.
+// Returns the final status (1 converged, 2 breakdown, 3 max_iter).
+template <typename VT>
+static int run_pcg(fx_mesh *m, const VT *val, const double *b_dev, double tol, int64_t max_iter, int stop_mode,
+                   int G, PcgScalars *out) {
+    hipStream_t s = m->ctx->stream;
+    pcg_start(m, b_dev, tol, max_iter, stop_mode);
+    const bool graphs = use_graphs();
     PcgScalars h{};
+    int chunk = 8;
+    double crit_prev = -1.0;
+    int64_t it_prev = 0;
     for (;;) {
-        for (int k = 0; k < chunk; ++k)
-            launch_iteration<VT>(m, val, G, s);
-        FX_HIP(hipGetLastError());
+        if (graphs) {
+            FX_HIP(hipGraphLaunch(batch_graph<VT>(m, val, G, chunk), s));
+        } else {
+            for (int k = 0; k < chunk; ++k)
+                launch_iteration<VT>(m, val, G, s);
+            FX_HIP(hipGetLastError());
+        }
         FX_HIP(hipMemcpyAsync(&h, m->pcg.p, sizeof h, hipMemcpyDeviceToHost, s));
         FX_HIP(hipStreamSynchronize(s));
         if (h.status != 0)
             break;
-        if (chunk < 64)
+        // predicted iterations left from the mean log-reduction per iteration
+        double left = 64.0;
+        const int64_t did = h.it_k1 - it_prev;
+        if (crit_prev > 0.0 && did > 0 && h.crit > 0.0 && h.crit < crit_prev) {
+            const double rate = (std::log(crit_prev) - std::log(h.crit)) / (double)did;
+            left = (std::log(h.crit) - std::log(tol > 0.0 ? tol : 1e-300)) / rate;
+        }
+        left = std::min<double>(left, (double)(max_iter - h.it + 1));
+        chunk = 1;
+        while (chunk * 2 <= left && chunk < 64)
             chunk *= 2;
+        crit_prev = h.crit;
+        it_prev = h.it_k1;
     }
     *out = h;
     return h.status;
+}
+
+static int64_t iterations_done(const PcgScalars &h) {
+    // the stop rule fires in k_spmv_p of iteration it_k1 (it_k1 SpMVs done);
+    // a breakdown fires in k_update after that iteration's SpMV
+    return h.status == 2 ? h.it_k1 + 1 : h.it_k1;
 }
 
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep) {
     hipStream_t s = m->ctx->stream;
     auto t0 = std::chrono::steady_clock::now();
     const int64_t n = m->n_dof;
-    int G = o.spmv_group > 0 ? o.spmv_group : (m->spmv_group > 0 ? m->spmv_group : default_group(m));
-    for (auto *b : {&m->x, &m->r, &m->z, &m->p0, &m->p1, &m->ap, &m->tmp})
-        b->alloc(n);
+    const int G = o.spmv_group > 0 ? o.spmv_group : (m->spmv_group > 0 ? m->spmv_group : default_group(m));
+    ensure_vectors(m);
     const bool mixed = o.precision == FX_PREC_MIXED;
     launch_jacobi(m, false);
     fx_solve_report R{};
@@ -422,8 +534,8 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
     fx_status st = FX_OK;
     if (!mixed) {
         PcgScalars h;
-        int code = run_pcg<double>(m, m->kval.p, b_dev, o.tol, o.max_iter, o.stop_mode, G, &h);
-        R.iterations = h.iter;
+        const int code = run_pcg<double>(m, m->kval.p, b_dev, o.tol, o.max_iter, o.stop_mode, G, &h);
+        R.iterations = iterations_done(h);
         R.final_criterion = h.crit;
         if (code == 2)
             st = FX_ERR_BREAKDOWN;
@@ -432,25 +544,26 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
         if (x_dev != m->x.p)
             FX_HIP(hipMemcpyAsync(x_dev, m->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
-        // FP64 iterative refinement around an FP32-stored operator:
-        //   x = 0, r = b;  repeat: solve K32 d = r to inner_tol (PCG with FP64
-        //   vectors), x += d, r = b - K64 x (FP64), stop on the outer rule.
+        // FP64 iterative refinement around the FP32-stored operator:
+        //   x = 0;  repeat: r = b - K64 x (FP64), stop on the outer rule,
+        //   solve K32 d = r to a relative reduction inner_tol (PCG with FP64
+        //   vectors), x += d.
         launch_convert32(m);
         DBuf<double> xacc, rtrue, kx, out3;
         xacc.alloc(n);
         rtrue.alloc(n);
         kx.alloc(n);
         out3.alloc(3);
-        FX_HIP(hipMemsetAsync(xacc.p, 0, n * sizeof(double), s));
-        FX_HIP(hipMemcpyAsync(rtrue.p, b_dev, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        launch_fill(m, xacc.p, n, 0.0);
         double bb = -1.0;
         int64_t budget = o.max_iter;
         int passes = 0;
+        const unsigned nb = nblocks(n, kBlock);
         for (;;) {
-            // outer criterion on the true residual
             spmv64(m, xacc.p, kx.p, G);
-            hipLaunchKernelGGL(k_true_residual, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, s, n, b_dev, kx.p,
-                               m->inv_diag.p, rtrue.p, m->part.p, m->counters.p, out3.p);
+            hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, n, b_dev, kx.p, m->inv_diag.p, rtrue.p,
+                               m->tmp.p);
+            hipLaunchKernelGGL(k_sum_partials<3>, dim3(1), dim3(kBlock), 0, s, m->tmp.p, (int)nb, out3.p);
             FX_HIP(hipGetLastError());
             double h3[3];
             FX_HIP(hipMemcpyAsync(h3, out3.p, sizeof h3, hipMemcpyDeviceToHost, s));
@@ -474,11 +587,11 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
                 st = FX_ERR_NOT_CONVERGED;
                 break;
             }
-            // inner solve to a relative reduction inner_tol of this residual
             PcgScalars h;
-            int code = run_pcg<float>(m, m->kval32.p, rtrue.p, o.inner_tol, budget, FX_STOP_REL_R2, G, &h);
-            R.iterations += h.iter;
-            budget -= h.iter;
+            const int code = run_pcg<float>(m, m->kval32.p, rtrue.p, o.inner_tol, budget, FX_STOP_REL_R2, G, &h);
+            const int64_t done = iterations_done(h);
+            R.iterations += done;
+            budget -= done;
             ++passes;
             if (code == 2) {
                 st = FX_ERR_BREAKDOWN;
@@ -489,7 +602,6 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
         }
         R.refinements = passes;
         FX_HIP(hipMemcpyAsync(x_dev, xacc.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        FX_HIP(hipStreamSynchronize(s));
     }
     FX_HIP(hipStreamSynchronize(s));
     R.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -17,6 +17,7 @@ Errors come back as the :mod:`ferrix_amd.inp` exception classes that mirror
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 from typing import List, Optional, Tuple
 
@@ -85,6 +86,7 @@ class Context:
         if rc != L.FX_OK:
             raise _ERR.get(rc, FerrixError)(self.lib.fx_last_error(None).decode())
         self.h = h
+        self._meshes = weakref.WeakSet()
 
     def check(self, rc: int):
         if rc != L.FX_OK:
@@ -93,6 +95,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for m in list(self._meshes):
+                m.close()
             self.lib.fx_ctx_destroy(self.h)
             self.h = None
 
@@ -124,15 +128,16 @@ class DeviceMesh:
         ctx.check(self.lib.fx_mesh_upload(ctx.h, mesh.n_nodes, _p(xyz), _p(temp), mesh.n_el, _p(el_ptr),
                                           _p(conn), _p(mat), C.byref(mt), C.byref(h)))
         self.h = h
+        ctx._meshes.add(self)
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         ctx.check(self.lib.fx_mesh_info(h, C.byref(a), C.byref(b), C.byref(c)))
         self.nnz_struct = b.value
         self.device_bytes = c.value
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             self.lib.fx_mesh_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -145,6 +150,15 @@ class DeviceMesh:
         u = _c(u, np.float64)
         assert u.shape == (self.n_dof,)
         self.ctx.check(self.lib.fx_state_set_u(self.h, _p(u)))
+
+    def zero_u(self):
+        self.ctx.check(self.lib.fx_state_set_u(self.h, None))
+
+    def profile_spmv(self, precision: int, reps: int = 50, group: int = 0):
+        """(ms per launch, algorithmic bytes per launch) of the PCG SpMV."""
+        ms, by = C.c_double(), C.c_double()
+        self.ctx.check(self.lib.fx_profile_spmv(self.h, precision, reps, group, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def get_u(self) -> np.ndarray:
         u = np.zeros(self.n_dof)

@@ -51,6 +51,8 @@ typedef struct fx_mesh fx_mesh;
 /* ---- context ---------------------------------------------------------- */
 /* Binds HIP device `device`, creates one stream (default priority). */
 fx_status fx_ctx_create(int device, fx_ctx **out);
+/* Destroys the context and every mesh still alive on it (their handles
+ * become invalid). */
 void fx_ctx_destroy(fx_ctx *ctx);
 /* Message of the last failing call on this context ("" if none). */
 const char *fx_last_error(const fx_ctx *ctx);
@@ -92,6 +94,7 @@ void fx_mesh_destroy(fx_mesh *mesh);
 fx_status fx_mesh_info(const fx_mesh *mesh, int64_t *n_dof, int64_t *nnz_struct, int64_t *device_bytes);
 
 /* ---- displacement state (SolutionState::displacements, state.rs:16-27) -- */
+/* u == NULL sets the resident displacement to zero without a host copy. */
 fx_status fx_state_set_u(fx_mesh *mesh, const double *u);
 fx_status fx_state_get_u(fx_mesh *mesh, double *u);
 
@@ -237,6 +240,16 @@ typedef struct fx_step_report {
  * timer->new_step itself (Step::solve, steps.rs:28-42).
  */
 fx_status fx_static_step(fx_mesh *mesh, const fx_step_def *step, fx_timer *timer, fx_step_report *report);
+
+/* ---- instrumentation (no counterpart in the reference) -------------------
+ * Times `reps` back-to-back launches of the PCG SpMV kernel on the resident
+ * K (FP64 values, or FP32 values for FX_PREC_MIXED) between two HIP events on
+ * the context stream.  *ms_per_launch: mean launch duration; *bytes_per_launch:
+ * algorithmic bytes of one launch (values + one column index per 3x3 block +
+ * node-row pointers + two gathered and two written FP64 vectors).  Used by
+ * the benchmark's roofline figure. */
+fx_status fx_profile_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_t spmv_group,
+                          double *ms_per_launch, double *bytes_per_launch);
 
 #ifdef __cplusplus
 }

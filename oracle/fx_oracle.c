@@ -449,6 +449,16 @@ int fxo_element_thermal(int nn, const double *X, const double *Ti, const double 
     return FXO_OK;
 }
 
+/* Python-friendly wrapper around fxo_element_thermal. */
+int fxo_element_thermal_flat(int nn, const double *X, const double *Ti, const double *Tc, const int32_t *n_e,
+                             const double *t_e, const double *v_e, const int32_t *n_nu, const double *t_nu,
+                             const double *v_nu, int m, int n_a, const double *t_a, const double *v_a, double t_ref,
+                             double *f)
+{
+    mat_table mt = {n_e, t_e, v_e, n_nu, t_nu, v_nu};
+    return fxo_element_thermal(nn, X, Ti, Tc, &mt, m, n_a, t_a, v_a, t_ref, f);
+}
+
 /* Python-friendly wrapper around fxo_element with flat material arrays. */
 int fxo_element_flat(int nn, const double *X, const double *ucfg, const double *uel, const double *Tn,
                      const int32_t *n_e, const double *t_e, const double *v_e, const int32_t *n_nu,

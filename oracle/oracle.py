@@ -52,7 +52,7 @@ def lib():
         P = C.c_void_p
         i64, i32, dbl = C.c_int64, C.c_int, C.c_double
         L.fxo_element_flat.argtypes = [i32, P, P, P, P, P, P, P, P, P, P, i32, P, P]
-        L.fxo_element_thermal.argtypes = [i32, P, P, P, P, i32, i32, P, P, dbl, P]
+        L.fxo_element_thermal_flat.argtypes = [i32, P, P, P, P, P, P, P, P, P, i32, i32, P, P, dbl, P]
         L.fxo_assemble.argtypes = [i64, P, P, P, P, P, P, P, P, P, P, P, P, dbl, P, P, P, P, P, P]
         L.fxo_internal_force.argtypes = [i64, P, P, P, P, P, P, P, P, P, P, P, P, P, i64, P, P]
         L.fxo_triplets_to_csr.argtypes = [i64, i64, P, P, P, P, P, P, P]
@@ -136,6 +136,21 @@ def element(mesh, X, Tn, m=0, ucfg=None, uel=None, want_k=True, want_f=False):
     if rc != OK:
         raise OracleError(rc, "Singular Jacobian" if rc == ERR_SINGULAR else f"element rc={rc}")
     return K, f
+
+
+def thermal_force(mesh, X, Ti, Tc, alpha_t, alpha_v, t_ref, m=0):
+    """element.rs:453-520 for one element with an expansion LUT."""
+    nn = X.shape[0]
+    f = np.zeros(3 * nn)
+    ta = np.ascontiguousarray(alpha_t, np.float64)
+    va = np.ascontiguousarray(alpha_v, np.float64)
+    rc = lib().fxo_element_thermal_flat(nn, _p(np.ascontiguousarray(X, np.float64)),
+                                        _p(np.ascontiguousarray(Ti, np.float64)),
+                                        _p(np.ascontiguousarray(Tc, np.float64)),
+                                        *[_p(a) for a in _mat_args(mesh)], m, len(ta), _p(ta), _p(va), t_ref, _p(f))
+    if rc != OK:
+        raise OracleError(rc, "thermal force failed")
+    return f
 
 
 # ---------------------------------------------------------------------------

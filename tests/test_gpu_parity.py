@@ -231,3 +231,27 @@ def test_reference_decks(ctx, name, node, expected, tol):
         got = u[3 * idx:3 * idx + 3]
         say("step", k + 1, got, expected[k])
         assert np.all(np.abs(got - np.array(expected[k])) < tol)
+
+
+def test_pcg_graph_replays_match_eager(ctx, monkeypatch):
+    """Captured iteration batches (hipGraph) against plain launches: several
+    solves in a row, eager work in between, bitwise equal results."""
+    job = small_c3d20(True)
+    st = job.steps[0]
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.assemble(False)
+    b = np.zeros(job.n_dof)
+    np.add.at(b, st.load_dof, st.load_val)
+    b, _ = dm.apply_penalty(st.bc_dof, st.bc_val, b)
+    o = SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100000)
+    monkeypatch.setenv("FX_PCG_GRAPH", "0")
+    x_eager, r_eager = dm.solve(b, o)
+    monkeypatch.setenv("FX_PCG_GRAPH", "1")
+    # power-of-two scalings of b scale every CG quantity exactly
+    for scale in (1.0, 2.0, 0.5, 4.0):
+        dm.internal_force(False)  # eager work between replays
+        x, r = dm.solve(b * scale, o)
+        assert r.iterations == r_eager.iterations
+        assert np.array_equal(x, x_eager * scale)
+    x, r = dm.solve(b, o)
+    assert np.array_equal(x, x_eager)
