@@ -52,6 +52,23 @@ def device_sync():
         raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
 
 
+def calib_copy(mib, reps=5):
+    """hipMemcpyDtoD of `mib` MiB, `reps` times: a copy of known size whose
+    FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpyDtoD(b, a, ctypes.c_size_t(n)) == 0
+    device_sync()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"# calib copy {mib} MiB x {reps}", flush=True)
+
+
 def cpu_baseline(job, fp64_iters, threads=4, sample_iters=200):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -92,7 +109,12 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--sweep-groups", action="store_true", help="print SpMV time per lane-group size")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="only run device-to-device copies of this many MiB (counter calibration) and exit")
     a = ap.parse_args()
+    if a.calib_copy_mb > 0:
+        calib_copy(a.calib_copy_mb)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
               "(the batched and serving workloads shard by independent mesh, see the design notes).",

@@ -86,6 +86,13 @@ SIGNATURES = {
     "fx_solve": (i32, [P, P, P, P, P]),
     "fx_static_step": (i32, [P, P, P, P]),
     "fx_profile_spmv": (i32, [P, i32, i32, i32, P, P]),
+    "fx_batch_create": (i32, [P, P, P]),
+    "fx_batch_destroy": (None, [P]),
+    "fx_batch_add": (i32, [P, P, i32, P, P]),
+    "fx_batch_run": (i32, [P, i64, P]),
+    "fx_batch_poll": (i32, [P, i32, P, P]),
+    "fx_batch_get": (i32, [P, i32, P]),
+    "fx_batch_remove": (i32, [P, i32]),
 }
 
 _LIB = None

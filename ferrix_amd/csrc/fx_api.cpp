@@ -248,6 +248,9 @@ void fx_ctx_destroy(fx_ctx *ctx) {
         return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (fx_batch *b : ctx->batches)
+        fx::batch_free(b);
+    ctx->batches.clear();
     for (fx_mesh *m : ctx->meshes)
         delete m;
     ctx->meshes.clear();
@@ -408,6 +411,8 @@ void fx_mesh_destroy(fx_mesh *m) {
     ctx->meshes.erase(it);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (fx_batch *b : ctx->batches)
+        fx::batch_drop_mesh(b, m);
     delete m;
 }
 

@@ -1,0 +1,982 @@
+// fx_batch.hip -- many independent systems (meshes x load cases) advanced by
+// one set of kernel launches: the engine of the batched workload and of the
+// serving loop.
+//
+// A system ("slot") is one resident mesh's K (after assembly and penalty)
+// with R <= 4 right-hand sides, the load cases that share that K.  Vectors of
+// a slot are interleaved by right-hand side (v[dof*R + r]), so one pass over
+// K serves all its load cases.  Every workgroup of a launch works for exactly
+// one slot (the block tables hold (slot, local block) pairs) and block
+// partial sums are kept per slot, so a slot's arithmetic -- and its result --
+// is the same whichever other slots share the launch.  Per right-hand side the
+// recurrences and the stop rule are those of fx_pcg.hip (the reference's
+// Jacobi-PCG, src/solver/solvers/iterative.rs:26-121).
+//
+// Mixed precision: FP64 iterative refinement per slot around the FP32-stored
+// K.  The inner PCG (FP32 values, FP64 vectors) reduces the current residual
+// by inner_tol; the correction is added to the FP64 solution and an FP64
+// residual pass decides, per right-hand side, done or another inner solve.
+// Slots move between phases independently, and new slots join between chunks
+// of iterations (continuous batching).
+#include "fx_reduce.h"
+
+#include <chrono>
+#include <memory>
+
+namespace fx {
+
+constexpr int kRmax = 4;
+constexpr int kBB = 256;  // threads per block in the batch kernels
+
+struct BRhs {
+    double rz[2], rr, bb, crit, pap, alpha, tol;
+    int64_t it, it_k1, max_iter;
+    int32_t status;  // 0 running, 1 converged, 2 breakdown, 3 max_iter, 4 idle
+    int32_t stop_mode;
+};
+
+struct BSys {
+    const int32_t *nb_ptr, *nb_col;
+    const void *val;                    // double (FP64 slots) or float (mixed inner)
+    const double *val64, *inv;
+    double *b;                          // inner right-hand sides
+    const double *btrue;                // true right-hand sides (mixed)
+    double *x, *r, *z, *p0, *p1, *ap;  // inner PCG vectors
+    double *xacc;                       // FP64 solution of the refinement (mixed)
+    double *part1, *part2;
+    int64_t n_nodes, n_dof;
+    int32_t R, nparts1, nparts2, want;  // want: right-hand sides of an outer pass (bit mask)
+    BRhs rhs[kRmax];
+};
+
+namespace {
+
+// sums nv <= NV values over nparts blocks (layout part[b*nv + k]); every
+// thread receives the result; fixed order
+template <int NV>
+__device__ __forceinline__ void bsum_partials(const double *part, int nparts, int nv, double (&out)[NV]) {
+    __shared__ double bc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        out[k] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBB)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            if (k < nv)
+                out[k] += part[(size_t)b * nv + k];
+    block_sum<NV>(out);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            bc[k] = out[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        out[k] = bc[k];
+}
+
+// Inner-solve start for the right-hand sides with status 0:
+// x = 0, r = b, z = r inv, p0 = z; partials (r.z, r.r, b.b) -> part2.
+__global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    BSys &S = sys[t.x];
+    const int lb = t.y, R = S.R;
+    double v[3 * kRmax];
+#pragma unroll
+    for (int k = 0; k < 3 * kRmax; ++k)
+        v[k] = 0.0;
+    const int64_t i = (int64_t)lb * kBB + threadIdx.x;
+    if (i < S.n_dof) {
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r) {
+            if (r >= R || S.rhs[r].status != 0)  // set by the host before this launch
+                continue;
+            const int64_t q = i * R + r;
+            const double bi = S.b[q];
+            const double zi = bi * S.inv[i];
+            S.x[q] = 0.0;
+            S.r[q] = bi;
+            S.z[q] = zi;
+            S.p0[q] = zi;
+            v[3 * r] = bi * zi;
+            v[3 * r + 1] = bi * bi;
+            v[3 * r + 2] = bi * bi;
+        }
+    }
+    block_sum<3 * kRmax>(v);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < R; ++r)
+            for (int k = 0; k < 3; ++k)
+                S.part2[((size_t)lb * R + r) * 3 + k] = v[3 * r + k];
+}
+
+template <int G, typename VT>
+__global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    BSys &S = sys[t.x];
+    const int lb = t.y, R = S.R;
+    __shared__ BRhs snap[kRmax];
+    if (threadIdx.x < kRmax && (int)threadIdx.x < R)
+        snap[threadIdx.x] = S.rhs[threadIdx.x];
+    __syncthreads();  // one block-wide view of the right-hand side states
+    bool any = false;
+    for (int r = 0; r < R; ++r)
+        any |= snap[r].status == 0;
+    if (!any)
+        return;
+    double red[3 * kRmax];
+    bsum_partials<3 * kRmax>(S.part2, S.nparts2, 3 * R, red);
+    bool act[kRmax];
+    double beta[kRmax];
+    int64_t itv = 0;
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r) {
+        act[r] = false;
+        beta[r] = 0.0;
+        if (r >= R || snap[r].status != 0)
+            continue;
+        const BRhs &h = snap[r];
+        const double rz = red[3 * r], rr = red[3 * r + 1];
+        const int64_t it = h.it;
+        const double bb = it == 0 ? red[3 * r + 2] : h.bb;
+        double crit;
+        bool conv;
+        if (h.stop_mode == FX_STOP_ABS_SQRT_RZ) {
+            crit = sqrt(rz);
+            conv = crit < h.tol;
+        } else {
+            crit = bb > 0.0 ? sqrt(rr) / sqrt(bb) : 0.0;
+            conv = crit <= h.tol;
+        }
+        const bool stop_max = it >= h.max_iter;
+        act[r] = !(conv || stop_max);
+        beta[r] = it == 0 ? 0.0 : rz / h.rz[(it - 1) & 1];
+        itv = it;
+        if (lb == 0 && threadIdx.x == 0) {
+            BRhs &hw = S.rhs[r];
+            hw.rz[it & 1] = rz;
+            hw.rr = rr;
+            hw.crit = crit;
+            hw.it_k1 = it;
+            if (it == 0)
+                hw.bb = bb;
+            if (stop_max)
+                hw.status = 3;
+            else if (conv)
+                hw.status = 1;
+        }
+    }
+    bool anyact = false;
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r)
+        anyact |= act[r];
+    if (!anyact)
+        return;
+    // the running right-hand sides of a slot started together and share it
+    const double *pold = (itv & 1) ? S.p1 : S.p0;
+    double *pnew = (itv & 1) ? S.p0 : S.p1;
+    const VT *val = (const VT *)S.val;
+
+    const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    double acc[3][kRmax];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r)
+            acc[p][r] = 0.0;
+    if (a < S.n_nodes) {
+        const int b0 = S.nb_ptr[a], ncol = 3 * (S.nb_ptr[a + 1] - b0);
+        const VT *v0 = val + 9 * (int64_t)b0;
+        for (int j = lane; j < ncol; j += G) {
+            const int kb = j / 3;
+            const int64_t c = (3 * (int64_t)S.nb_col[b0 + kb] + (j - 3 * kb)) * R;
+            const double w0 = (double)v0[j], w1 = (double)v0[ncol + j], w2 = (double)v0[2 * ncol + j];
+#pragma unroll
+            for (int r = 0; r < kRmax; ++r) {
+                if (!act[r])
+                    continue;
+                const double xj = S.z[c + r] + beta[r] * pold[c + r];
+                acc[0][r] += w0 * xj;
+                acc[1][r] += w1 * xj;
+                acc[2][r] += w2 * xj;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int r = 0; r < kRmax; ++r)
+                acc[p][r] += __shfl_xor(acc[p][r], off, G);
+    double loc[kRmax];
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r)
+        loc[r] = 0.0;
+    if (lane == 0 && a < S.n_nodes) {
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r) {
+            if (!act[r])
+                continue;
+            double s = 0.0;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                const int64_t q = (3 * a + p) * R + r;
+                const double qn = S.z[q] + beta[r] * pold[q];
+                pnew[q] = qn;
+                S.ap[q] = acc[p][r];
+                s += qn * acc[p][r];
+            }
+            loc[r] = s;
+        }
+    }
+    block_sum<kRmax>(loc);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < R; ++r)
+            S.part1[(size_t)lb * R + r] = loc[r];
+}
+
+__global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    BSys &S = sys[t.x];
+    const int lb = t.y, R = S.R;
+    __shared__ BRhs snap[kRmax];
+    if (threadIdx.x < kRmax && (int)threadIdx.x < R)
+        snap[threadIdx.x] = S.rhs[threadIdx.x];
+    __syncthreads();
+    bool any = false;
+    for (int r = 0; r < R; ++r)
+        any |= snap[r].status == 0;
+    if (!any)
+        return;
+    double red[kRmax];
+    bsum_partials<kRmax>(S.part1, S.nparts1, R, red);
+    bool act[kRmax];
+    double alpha[kRmax];
+    int64_t itv = 0;
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r) {
+        act[r] = false;
+        alpha[r] = 0.0;
+        if (r >= R || snap[r].status != 0)
+            continue;
+        const BRhs &h = snap[r];
+        const double pap = red[r];
+        const double al = h.rz[h.it_k1 & 1] / pap;
+        const bool bad = !(pap > 0.0) || !isfinite(al);
+        act[r] = !bad;
+        alpha[r] = al;
+        itv = h.it_k1;
+        if (lb == 0 && threadIdx.x == 0) {
+            BRhs &hw = S.rhs[r];
+            hw.pap = pap;
+            hw.alpha = al;
+            if (bad)
+                hw.status = 2;
+        }
+    }
+    const double *p = (itv & 1) ? S.p0 : S.p1;  // p_i sits in the "new" buffer of iteration i
+    double v[2 * kRmax];
+#pragma unroll
+    for (int k = 0; k < 2 * kRmax; ++k)
+        v[k] = 0.0;
+    const int64_t i = (int64_t)lb * kBB + threadIdx.x;
+    if (i < S.n_dof) {
+        const double inv = S.inv[i];
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r) {
+            if (!act[r])
+                continue;
+            const int64_t q = i * R + r;
+            const double xi = S.x[q] + alpha[r] * p[q];
+            const double ri = S.r[q] - alpha[r] * S.ap[q];
+            const double zi = ri * inv;
+            S.x[q] = xi;
+            S.r[q] = ri;
+            S.z[q] = zi;
+            v[2 * r] = ri * zi;
+            v[2 * r + 1] = ri * ri;
+        }
+    }
+    block_sum<2 * kRmax>(v);
+    if (threadIdx.x == 0) {
+        for (int r = 0; r < R; ++r) {
+            S.part2[((size_t)lb * R + r) * 3] = v[2 * r];
+            S.part2[((size_t)lb * R + r) * 3 + 1] = v[2 * r + 1];
+            S.part2[((size_t)lb * R + r) * 3 + 2] = 0.0;
+        }
+        if (lb == 0)
+            for (int r = 0; r < R; ++r)
+                if (act[r])
+                    S.rhs[r].it = snap[r].it_k1 + 1;
+    }
+}
+
+// Mixed refinement, step 1: xacc += x for the right-hand sides in S.want.
+__global__ __launch_bounds__(kBB) void kb_accumulate(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    BSys &S = sys[t.x];
+    const int64_t i = (int64_t)t.y * kBB + threadIdx.x;
+    if (i >= S.n_dof)
+        return;
+    for (int r = 0; r < S.R; ++r)
+        if ((S.want >> r) & 1) {
+            const int64_t q = i * S.R + r;
+            S.xacc[q] += S.x[q];
+        }
+}
+
+// Mixed refinement, step 2: b = btrue - K64 xacc (the next inner
+// right-hand side) for the right-hand sides in S.want, with block partials
+// (r.r, b.b, r.(inv r)) -> part1.
+template <int G>
+__global__ __launch_bounds__(kBB) void kb_outer(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    BSys &S = sys[t.x];
+    const int lb = t.y, R = S.R, w = S.want;
+    const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    double acc[3][kRmax];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r)
+            acc[p][r] = 0.0;
+    if (a < S.n_nodes) {
+        const int b0 = S.nb_ptr[a], ncol = 3 * (S.nb_ptr[a + 1] - b0);
+        const double *v0 = S.val64 + 9 * (int64_t)b0;
+        for (int j = lane; j < ncol; j += G) {
+            const int kb = j / 3;
+            const int64_t c = (3 * (int64_t)S.nb_col[b0 + kb] + (j - 3 * kb)) * R;
+            const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
+#pragma unroll
+            for (int r = 0; r < kRmax; ++r) {
+                if (r >= R || !((w >> r) & 1))
+                    continue;
+                const double xj = S.xacc[c + r];
+                acc[0][r] += w0 * xj;
+                acc[1][r] += w1 * xj;
+                acc[2][r] += w2 * xj;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int r = 0; r < kRmax; ++r)
+                acc[p][r] += __shfl_xor(acc[p][r], off, G);
+    double v[3 * kRmax];
+#pragma unroll
+    for (int k = 0; k < 3 * kRmax; ++k)
+        v[k] = 0.0;
+    if (lane == 0 && a < S.n_nodes) {
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r) {
+            if (r >= R || !((w >> r) & 1))
+                continue;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                const int64_t d = 3 * a + p;
+                const int64_t q = d * R + r;
+                const double bt = S.btrue[q];
+                const double ri = bt - acc[p][r];
+                S.b[q] = ri;
+                v[3 * r] += ri * ri;
+                v[3 * r + 1] += bt * bt;
+                v[3 * r + 2] += ri * (ri * S.inv[d]);
+            }
+        }
+    }
+    block_sum<3 * kRmax>(v);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < R; ++r)
+            for (int k = 0; k < 3; ++k)
+                S.part1[((size_t)lb * R + r) * 3 + k] = v[3 * r + k];
+}
+
+// one block per listed slot: fixed-order sums of the outer partials
+__global__ __launch_bounds__(kBB) void kb_outer_sum(BSys *__restrict__ sys, const int32_t *__restrict__ list,
+                                                    double *__restrict__ out) {
+    BSys &S = sys[list[blockIdx.x]];
+    double red[3 * kRmax];
+    bsum_partials<3 * kRmax>(S.part1, S.nparts1, 3 * S.R, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3 * kRmax; ++k)
+            out[(size_t)blockIdx.x * 3 * kRmax + k] = k < 3 * S.R ? red[k] : 0.0;
+}
+
+inline int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
+
+template <typename VT>
+void launch_bspmv(int G, unsigned nb, hipStream_t s, BSys *sys, const int2 *tab) {
+    switch (G) {
+    case 2: hipLaunchKernelGGL((kb_spmv_p<2, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 4: hipLaunchKernelGGL((kb_spmv_p<4, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 8: hipLaunchKernelGGL((kb_spmv_p<8, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 16: hipLaunchKernelGGL((kb_spmv_p<16, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 32: hipLaunchKernelGGL((kb_spmv_p<32, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    default: hipLaunchKernelGGL((kb_spmv_p<64, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    }
+}
+
+void launch_bouter(int G, unsigned nb, hipStream_t s, BSys *sys, const int2 *tab) {
+    switch (G) {
+    case 2: hipLaunchKernelGGL((kb_outer<2>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 4: hipLaunchKernelGGL((kb_outer<4>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 8: hipLaunchKernelGGL((kb_outer<8>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 16: hipLaunchKernelGGL((kb_outer<16>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 32: hipLaunchKernelGGL((kb_outer<32>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    default: hipLaunchKernelGGL((kb_outer<64>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    }
+}
+
+}  // namespace
+}  // namespace fx
+
+// ===========================================================================
+// host side
+// ===========================================================================
+struct fx_batch {
+    fx_ctx *ctx = nullptr;
+    fx_solve_opts opts{};
+    int G = 0;
+    struct Slot {
+        bool used = false;
+        fx_mesh *mesh = nullptr;
+        int R = 0;
+        int phase = 0;  // 0 inner running, 2 finished
+        int passes = 0;
+        fx_status err = FX_OK;
+        int64_t iters = 0, outer = 0, budget = 0;
+        double crit[fx::kRmax] = {0, 0, 0, 0};
+        bool done[fx::kRmax] = {false, false, false, false};
+        fx::DBuf<double> b, rin, x, r, z, p0, p1, ap, xacc, part1, part2;
+        std::chrono::steady_clock::time_point t0;
+        double seconds = 0.0;
+        int64_t bytes = 0;
+    };
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::vector<fx::BSys> h_sys;
+    fx::DBuf<fx::BSys> d_sys;
+    fx::DBuf<int2> d_tab1, d_tab2, d_tab3, d_tab4;
+    fx::DBuf<int32_t> d_list;
+    fx::DBuf<double> d_out;
+    int n1 = 0, n2 = 0;
+    bool tables_dirty = true;
+    int64_t resident_bytes = 0;
+};
+
+namespace {
+
+using fx::BRhs;
+using fx::BSys;
+using fx::kBB;
+using fx::kRmax;
+
+bool is_mixed(const fx_batch *B) { return B->opts.precision == FX_PREC_MIXED; }
+
+void start_inner(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
+    // right-hand sides not done start an inner solve from x = 0
+    for (int r = 0; r < sl.R; ++r) {
+        BRhs &h = S.rhs[r];
+        h = BRhs{};
+        if (sl.done[r]) {
+            h.status = 4;
+            continue;
+        }
+        h.status = 0;
+        if (is_mixed(B)) {
+            h.tol = B->opts.inner_tol;
+            h.stop_mode = FX_STOP_REL_R2;
+            h.max_iter = std::max<int64_t>(sl.budget, 0);
+        } else {
+            h.tol = B->opts.tol;
+            h.stop_mode = B->opts.stop_mode;
+            h.max_iter = B->opts.max_iter;
+        }
+    }
+    for (int r = sl.R; r < kRmax; ++r)
+        S.rhs[r].status = 4;
+}
+
+void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
+    fx_mesh *m = sl.mesh;
+    S.nb_ptr = m->nb_ptr.p;
+    S.nb_col = m->nb_col.p;
+    S.val = is_mixed(B) ? (const void *)m->kval32.p : (const void *)m->kval.p;
+    S.val64 = m->kval.p;
+    S.inv = m->inv_diag.p;
+    S.b = is_mixed(B) ? sl.rin.p : sl.b.p;
+    S.btrue = sl.b.p;
+    S.x = sl.x.p;
+    S.r = sl.r.p;
+    S.z = sl.z.p;
+    S.p0 = sl.p0.p;
+    S.p1 = sl.p1.p;
+    S.ap = sl.ap.p;
+    S.xacc = sl.xacc.p;
+    S.part1 = sl.part1.p;
+    S.part2 = sl.part2.p;
+    S.n_nodes = m->n_nodes;
+    S.n_dof = m->n_dof;
+    S.R = sl.R;
+    S.nparts1 = fx::nblk(m->n_nodes, kBB / B->G);
+    S.nparts2 = fx::nblk(m->n_dof, kBB);
+    S.want = 0;
+}
+
+// block tables of the running slots: tab1 SpMV blocks, tab2 vector blocks
+void build_tables(fx_batch *B) {
+    std::vector<int2> t1, t2;
+    for (size_t k = 0; k < B->slots.size(); ++k) {
+        auto &sl = *B->slots[k];
+        if (!sl.used || sl.phase != 0)
+            continue;
+        const BSys &S = B->h_sys[k];
+        for (int b = 0; b < S.nparts1; ++b)
+            t1.push_back(make_int2((int)k, b));
+        for (int b = 0; b < S.nparts2; ++b)
+            t2.push_back(make_int2((int)k, b));
+    }
+    // workgroups are dispatched round-robin over the 8 XCDs: permute the
+    // SpMV table so that each XCD sweeps a contiguous range of row groups
+    {
+        const unsigned nb = (unsigned)t1.size(), q = nb / 8, rr = nb % 8;
+        std::vector<int2> perm(nb);
+        for (unsigned id = 0; id < nb; ++id) {
+            const unsigned x = id % 8, i = id / 8;
+            perm[id] = t1[x * q + (x < rr ? x : rr) + i];
+        }
+        t1.swap(perm);
+    }
+    B->n1 = (int)t1.size();
+    B->n2 = (int)t2.size();
+    hipStream_t s = B->ctx->stream;
+    if (B->n1) {
+        B->d_tab1.alloc(std::max<size_t>(B->d_tab1.n, t1.size()));
+        FX_HIP(hipMemcpyAsync(B->d_tab1.p, t1.data(), t1.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    }
+    if (B->n2) {
+        B->d_tab2.alloc(std::max<size_t>(B->d_tab2.n, t2.size()));
+        FX_HIP(hipMemcpyAsync(B->d_tab2.p, t2.data(), t2.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    }
+    B->tables_dirty = false;
+}
+
+// grows the device system table keeping its contents (running slots' state
+// lives there)
+void reserve_sys(fx_batch *B, size_t n) {
+    if (B->d_sys.p && B->d_sys.n >= n)
+        return;
+    const size_t cap = std::max<size_t>({n, 2 * B->d_sys.n, 16});
+    BSys *p = nullptr;
+    FX_HIP(hipMalloc(&p, cap * sizeof(BSys)));
+    if (B->d_sys.p) {
+        FX_HIP(hipMemcpyAsync(p, B->d_sys.p, B->d_sys.n * sizeof(BSys), hipMemcpyDeviceToDevice, B->ctx->stream));
+        FX_HIP(hipStreamSynchronize(B->ctx->stream));
+        B->d_sys.release();
+    }
+    B->d_sys.p = p;
+    B->d_sys.n = cap;
+}
+
+void upload_sys(fx_batch *B) {
+    reserve_sys(B, B->h_sys.size());
+    FX_HIP(hipMemcpyAsync(B->d_sys.p, B->h_sys.data(), B->h_sys.size() * sizeof(BSys), hipMemcpyHostToDevice,
+                          B->ctx->stream));
+}
+
+void download_sys(fx_batch *B) {
+    FX_HIP(hipMemcpyAsync(B->h_sys.data(), B->d_sys.p, B->h_sys.size() * sizeof(BSys), hipMemcpyDeviceToHost,
+                          B->ctx->stream));
+    FX_HIP(hipStreamSynchronize(B->ctx->stream));
+}
+
+void launch_init(fx_batch *B) {
+    if (B->n2)
+        hipLaunchKernelGGL(fx::kb_init, dim3(B->n2), dim3(kBB), 0, B->ctx->stream, B->d_sys.p, B->d_tab2.p);
+}
+
+void launch_iterations(fx_batch *B, int n) {
+    hipStream_t s = B->ctx->stream;
+    for (int k = 0; k < n; ++k) {
+        if (is_mixed(B))
+            fx::launch_bspmv<float>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
+        else
+            fx::launch_bspmv<double>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
+        hipLaunchKernelGGL(fx::kb_update, dim3(B->n2), dim3(kBB), 0, s, B->d_sys.p, B->d_tab2.p);
+    }
+    FX_HIP(hipGetLastError());
+}
+
+// After a chunk: per slot, see whether its running inner solves finished;
+// FP64 slots finish, mixed slots take an outer refinement pass.  Returns
+// true when the set of running slots changed.
+bool advance_phases(fx_batch *B) {
+    download_sys(B);
+    bool changed = false;
+    std::vector<int> outer;
+    for (size_t k = 0; k < B->slots.size(); ++k) {
+        auto &sl = *B->slots[k];
+        if (!sl.used || sl.phase != 0)
+            continue;
+        BSys &S = B->h_sys[k];
+        bool running = false;
+        int64_t it = 0;
+        for (int r = 0; r < sl.R; ++r) {
+            if (S.rhs[r].status == 0)
+                running = true;
+            if (S.rhs[r].status != 4)
+                it = std::max(it, S.rhs[r].status == 2 ? S.rhs[r].it_k1 + 1 : S.rhs[r].it_k1);
+        }
+        if (running)
+            continue;
+        sl.iters += it;
+        sl.budget -= it;
+        sl.passes++;
+        int want = 0;
+        for (int r = 0; r < sl.R; ++r) {
+            const BRhs &h = S.rhs[r];
+            if (h.status == 4)
+                continue;
+            if (h.status == 2)
+                sl.err = FX_ERR_BREAKDOWN;
+            else if (h.status == 3)
+                sl.err = FX_ERR_NOT_CONVERGED;
+            else if (!is_mixed(B)) {
+                sl.done[r] = true;
+                sl.crit[r] = h.crit;
+            } else
+                want |= 1 << r;
+        }
+        if (!is_mixed(B) || sl.err != FX_OK) {
+            sl.phase = 2;
+            sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
+            changed = true;
+            continue;
+        }
+        S.want = want;
+        outer.push_back((int)k);
+    }
+    if (outer.empty())
+        return changed;
+    // outer pass for the slots whose inner solves finished
+    hipStream_t s = B->ctx->stream;
+    std::vector<int2> t3, t4;
+    for (int k : outer) {
+        const BSys &S = B->h_sys[k];
+        for (int b = 0; b < S.nparts2; ++b)
+            t4.push_back(make_int2(k, b));
+        for (int b = 0; b < S.nparts1; ++b)
+            t3.push_back(make_int2(k, b));
+    }
+    upload_sys(B);  // want masks
+    B->d_tab3.alloc(std::max<size_t>(B->d_tab3.n, t3.size()));
+    B->d_tab4.alloc(std::max<size_t>(B->d_tab4.n, t4.size()));
+    B->d_list.alloc(std::max<size_t>(B->d_list.n, outer.size()));
+    B->d_out.alloc(std::max<size_t>(B->d_out.n, outer.size() * 3 * kRmax));
+    FX_HIP(hipMemcpyAsync(B->d_tab3.p, t3.data(), t3.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    FX_HIP(hipMemcpyAsync(B->d_tab4.p, t4.data(), t4.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    FX_HIP(hipMemcpyAsync(B->d_list.p, outer.data(), outer.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(fx::kb_accumulate, dim3((unsigned)t4.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_tab4.p);
+    fx::launch_bouter(B->G, (unsigned)t3.size(), s, B->d_sys.p, B->d_tab3.p);
+    hipLaunchKernelGGL(fx::kb_outer_sum, dim3((unsigned)outer.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_list.p,
+                       B->d_out.p);
+    FX_HIP(hipGetLastError());
+    std::vector<double> norms(outer.size() * 3 * kRmax);
+    FX_HIP(hipMemcpyAsync(norms.data(), B->d_out.p, norms.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    FX_HIP(hipStreamSynchronize(s));
+    std::vector<int> restart;
+    for (size_t j = 0; j < outer.size(); ++j) {
+        const int k = outer[j];
+        auto &sl = *B->slots[k];
+        BSys &S = B->h_sys[k];
+        sl.outer++;
+        bool all = true;
+        for (int r = 0; r < sl.R; ++r) {
+            if (!((S.want >> r) & 1))
+                continue;
+            const double rr = norms[j * 3 * kRmax + 3 * r], bb = norms[j * 3 * kRmax + 3 * r + 1];
+            const double rzv = norms[j * 3 * kRmax + 3 * r + 2];
+            double crit;
+            bool conv;
+            if (B->opts.stop_mode == FX_STOP_ABS_SQRT_RZ) {
+                crit = std::sqrt(rzv);
+                conv = crit < B->opts.tol;
+            } else {
+                crit = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : 0.0;
+                conv = crit <= B->opts.tol;
+            }
+            sl.crit[r] = crit;
+            if (conv)
+                sl.done[r] = true;
+            else
+                all = false;
+        }
+        S.want = 0;
+        if (all) {
+            sl.phase = 2;
+            sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
+            changed = true;
+        } else if (sl.passes >= B->opts.max_refine || sl.budget <= 0) {
+            sl.err = FX_ERR_NOT_CONVERGED;
+            sl.phase = 2;
+            sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
+            changed = true;
+        } else {
+            start_inner(B, sl, S);  // next refinement pass
+            restart.push_back(k);
+            changed = true;
+        }
+    }
+    upload_sys(B);
+    if (!restart.empty()) {
+        std::vector<int2> t;
+        for (int k : restart)
+            for (int b = 0; b < B->h_sys[k].nparts2; ++b)
+                t.push_back(make_int2(k, b));
+        B->d_tab4.alloc(std::max<size_t>(B->d_tab4.n, t.size()));
+        FX_HIP(hipMemcpyAsync(B->d_tab4.p, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(fx::kb_init, dim3((unsigned)t.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_tab4.p);
+        FX_HIP(hipGetLastError());
+    }
+    return changed;
+}
+
+}  // namespace
+
+namespace fx {
+void batch_free(fx_batch *b) { delete b; }
+
+void batch_drop_mesh(fx_batch *B, fx_mesh *m) {
+    for (size_t k = 0; k < B->slots.size(); ++k)
+        if (B->slots[k] && B->slots[k]->used && B->slots[k]->mesh == m) {
+            if (B->slots[k]->phase == 0)
+                B->tables_dirty = true;
+            B->h_sys[k] = BSys{};
+            B->slots[k].reset(new fx_batch::Slot());
+        }
+}
+}  // namespace fx
+
+extern "C" {
+
+fx_status fx_batch_create(fx_ctx *ctx, const fx_solve_opts *opts, fx_batch **out) {
+    if (!ctx || !out)
+        return FX_ERR_INVALID;
+    try {
+        auto *B = new fx_batch();
+        B->ctx = ctx;
+        if (opts)
+            B->opts = *opts;
+        else
+            fx_solve_opts_default(&B->opts);
+        if (B->opts.precision != FX_PREC_FP64 && B->opts.precision != FX_PREC_MIXED) {
+            delete B;
+            ctx->err = "fx_batch_create: unknown precision";
+            return FX_ERR_INVALID;
+        }
+        const int g = B->opts.spmv_group;
+        if (!(g == 0 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64)) {
+            delete B;
+            ctx->err = "fx_batch_create: spmv_group must be 0 or a power of two in 2..64";
+            return FX_ERR_INVALID;
+        }
+        // 0: chosen from the first system admitted (as fx_solve chooses it)
+        B->G = g;
+        ctx->batches.push_back(B);
+        *out = B;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return FX_ERR_OOM;
+    }
+    return FX_OK;
+}
+
+void fx_batch_destroy(fx_batch *B) {
+    if (!B)
+        return;
+    fx_ctx *ctx = B->ctx;
+    auto it = std::find(ctx->batches.begin(), ctx->batches.end(), B);
+    if (it == ctx->batches.end())
+        return;  // not a live batch of its context
+    ctx->batches.erase(it);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete B;
+}
+
+fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs, int32_t *slot_out) {
+    if (!B || !m || !rhs || !slot_out)
+        return FX_ERR_INVALID;
+    fx_ctx *ctx = B->ctx;
+    try {
+        if (m->ctx != ctx)
+            throw fx::Error{FX_ERR_INVALID, "fx_batch_add: mesh belongs to another context"};
+        if (n_rhs < 1 || n_rhs > kRmax)
+            throw fx::Error{FX_ERR_INVALID, "fx_batch_add: 1 to 4 right-hand sides per system"};
+        if (!m->k_valid)
+            throw fx::Error{FX_ERR_INVALID, "fx_batch_add: assemble the mesh first"};
+        FX_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        if (B->G == 0)
+            B->G = m->spmv_group > 0 ? m->spmv_group : fx::default_group(m);
+        int k = 0;
+        while (k < (int)B->slots.size() && B->slots[k] && B->slots[k]->used)
+            ++k;
+        if (k == (int)B->slots.size())
+            B->slots.emplace_back(new fx_batch::Slot());
+        else
+            B->slots[k].reset(new fx_batch::Slot());
+        auto &sl = *B->slots[k];
+        sl.mesh = m;
+        sl.R = n_rhs;
+        const int64_t n = m->n_dof, nv = n * n_rhs;
+        // interleave the right-hand sides: host layout is rhs-major
+        std::vector<double> inter(nv);
+        for (int r = 0; r < n_rhs; ++r)
+            for (int64_t i = 0; i < n; ++i)
+                inter[i * n_rhs + r] = rhs[(int64_t)r * n + i];
+        sl.b.upload(inter, s);
+        for (auto *v : {&sl.x, &sl.r, &sl.z, &sl.p0, &sl.p1, &sl.ap})
+            v->alloc(nv);
+        if (is_mixed(B)) {
+            sl.rin.upload(inter, s);  // x = 0: the first residual is b
+            sl.xacc.alloc(nv);
+            fx::launch_fill(m, sl.xacc.p, nv, 0.0);
+            fx::launch_convert32(m);  // K may have been reassembled since the last conversion
+        }
+        fx::launch_jacobi(m, false);
+        const int G = B->G;
+        sl.part1.alloc((size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 3);
+        sl.part2.alloc((size_t)std::max(fx::nblk(n, kBB), 1) * n_rhs * 3);
+        sl.budget = B->opts.max_iter;
+        sl.used = true;
+        sl.phase = 0;
+        sl.t0 = std::chrono::steady_clock::now();
+        sl.bytes = (int64_t)(nv * 8 * (is_mixed(B) ? 9 : 7));
+        if ((int)B->h_sys.size() < (int)B->slots.size())
+            B->h_sys.resize(B->slots.size());
+        BSys &S = B->h_sys[k];
+        S = BSys{};
+        fill_sys(B, sl, S);
+        start_inner(B, sl, S);
+        // running slots' device state is newer than the host copy: upload
+        // this entry only
+        reserve_sys(B, B->h_sys.size());
+        FX_HIP(hipMemcpyAsync(B->d_sys.p + k, &S, sizeof(BSys), hipMemcpyHostToDevice, s));
+        // start this slot's inner solve now (only its blocks)
+        std::vector<int2> t;
+        for (int b = 0; b < S.nparts2; ++b)
+            t.push_back(make_int2(k, b));
+        fx::DBuf<int2> dt;
+        dt.upload(t, s);
+        hipLaunchKernelGGL(fx::kb_init, dim3((unsigned)t.size()), dim3(kBB), 0, s, B->d_sys.p, dt.p);
+        FX_HIP(hipGetLastError());
+        FX_HIP(hipStreamSynchronize(s));
+        B->tables_dirty = true;
+        *slot_out = k;
+    } catch (const fx::Error &e) {
+        ctx->err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return FX_ERR_OOM;
+    }
+    return FX_OK;
+}
+
+fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
+    if (!B)
+        return FX_ERR_INVALID;
+    fx_ctx *ctx = B->ctx;
+    try {
+        FX_HIP(hipSetDevice(ctx->device));
+        int64_t done_iters = 0;
+        for (;;) {
+            if (B->tables_dirty)
+                build_tables(B);
+            if (B->n1 == 0)
+                break;
+            int chunk = 16;
+            if (max_iters > 0)
+                chunk = (int)std::min<int64_t>(chunk, max_iters - done_iters);
+            launch_iterations(B, chunk);
+            done_iters += chunk;
+            if (advance_phases(B))
+                B->tables_dirty = true;
+            if (max_iters > 0 && done_iters >= max_iters)
+                break;
+        }
+        if (n_running_out) {
+            int c = 0;
+            for (auto &sl : B->slots)
+                c += sl && sl->used && sl->phase == 0;
+            *n_running_out = c;
+        }
+    } catch (const fx::Error &e) {
+        ctx->err = e.msg;
+        return e.code;
+    }
+    return FX_OK;
+}
+
+fx_status fx_batch_poll(fx_batch *B, int32_t slot, int32_t *state, fx_solve_report *rep) {
+    if (!B || slot < 0 || slot >= (int)B->slots.size() || !B->slots[slot] || !B->slots[slot]->used)
+        return FX_ERR_INVALID;
+    auto &sl = *B->slots[slot];
+    if (state)
+        *state = sl.phase == 2 ? (sl.err == FX_OK ? 1 : -(int)sl.err) : 0;
+    if (rep) {
+        *rep = fx_solve_report{};
+        rep->iterations = sl.iters;
+        rep->outer_spmv = sl.outer;
+        rep->refinements = is_mixed(B) ? sl.passes : 0;
+        rep->spmv_group = B->G;
+        double c = 0.0;
+        for (int r = 0; r < sl.R; ++r)
+            c = std::max(c, sl.crit[r]);
+        rep->final_criterion = c;
+        rep->seconds = sl.seconds;
+    }
+    return FX_OK;
+}
+
+fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
+    if (!B || !x || slot < 0 || slot >= (int)B->slots.size() || !B->slots[slot] || !B->slots[slot]->used)
+        return FX_ERR_INVALID;
+    auto &sl = *B->slots[slot];
+    try {
+        const int64_t n = sl.mesh->n_dof;
+        std::vector<double> inter(n * sl.R);
+        const double *src = is_mixed(B) ? sl.xacc.p : sl.x.p;
+        FX_HIP(hipMemcpyAsync(inter.data(), src, inter.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              B->ctx->stream));
+        FX_HIP(hipStreamSynchronize(B->ctx->stream));
+        for (int r = 0; r < sl.R; ++r)
+            for (int64_t i = 0; i < n; ++i)
+                x[(int64_t)r * n + i] = inter[i * sl.R + r];
+    } catch (const fx::Error &e) {
+        B->ctx->err = e.msg;
+        return e.code;
+    }
+    return FX_OK;
+}
+
+fx_status fx_batch_remove(fx_batch *B, int32_t slot) {
+    if (!B || slot < 0 || slot >= (int)B->slots.size() || !B->slots[slot] || !B->slots[slot]->used)
+        return FX_ERR_INVALID;
+    (void)hipStreamSynchronize(B->ctx->stream);
+    const bool was_running = B->slots[slot]->phase == 0;
+    if (slot < (int)B->h_sys.size())
+        B->h_sys[slot] = BSys{};
+    if (was_running)
+        B->tables_dirty = true;
+    B->slots[slot].reset(new fx_batch::Slot());
+    return FX_OK;
+}
+
+}  // extern "C"

@@ -143,7 +143,8 @@ struct fx_ctx {
     int n_cu = 0;
     std::string err;
     fx::DevTables tables;
-    std::vector<fx_mesh *> meshes;  // live meshes, destroyed with the context
+    std::vector<fx_mesh *> meshes;    // live meshes, destroyed with the context
+    std::vector<fx_batch *> batches;  // live batches, destroyed with the context
 };
 
 struct fx_mesh {
@@ -201,6 +202,9 @@ double device_max_diag(fx_mesh *m);
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep);
 void spmv64(fx_mesh *m, const double *x, double *y, int group);
 double profile_spmv(fx_mesh *m, int precision, int reps, int group, double *bytes);
+int default_group(fx_mesh *m);  // lanes per node row for the SpMV
+void batch_free(fx_batch *b);                 // deletes a batch (its context's stream is idle)
+void batch_drop_mesh(fx_batch *b, fx_mesh *m);  // removes the slots that use m
 void axpy_dev(fx_mesh *m, int64_t n, double a, const double *x, double *y);
 // step helpers (fx_assemble.hip)
 void launch_residual(fx_mesh *m, const double *f_ext, const double *f_int, double *r, int64_t n);

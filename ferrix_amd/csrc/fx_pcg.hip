@@ -38,7 +38,7 @@ namespace fx {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kUnroll = 4;  // column slots per lane with loads in flight
+constexpr int kUnroll = 2;  // column slots per lane with loads in flight
 
 __device__ __forceinline__ unsigned xcd_swizzle(unsigned id, unsigned nb) {
     const unsigned q = nb / 8, r = nb % 8, x = id % 8, i = id / 8;
@@ -95,13 +95,17 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t n, const double *__rest
 }
 
 template <int G, typename VT>
-__global__ __launch_bounds__(kBlock) void k_spmv_p(int64_t n_nodes, const int32_t *__restrict__ nb_ptr,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_spmv_p(int64_t n_nodes, const int32_t *__restrict__ nb_ptr,
                                                    const int32_t *__restrict__ nb_col, const VT *__restrict__ val,
                                                    const double *__restrict__ z, double *__restrict__ pa,
                                                    double *__restrict__ pb, double *__restrict__ ap,
                                                    PcgScalars *__restrict__ sc, const double *__restrict__ part2,
                                                    int nparts2, double *__restrict__ part1) {
-    if (sc->status != 0)
+    __shared__ int32_t s_status;
+    if (threadIdx.x == 0)
+        s_status = sc->status;
+    __syncthreads();  // one block-wide view (block 0 may update it below)
+    if (s_status != 0)
         return;
     const int64_t it = sc->it;
     // r.z, r.r (and b.b at i = 0) after the previous update
@@ -147,13 +151,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_p(int64_t n_nodes, const int32_
         // kUnroll slots per lane in flight; the accumulation order is slot
         // j, j+G, j+2G, ... exactly as in a plain strided loop
         for (int j0 = lane; j0 < ncol; j0 += kUnroll * G) {
-            int64_t c[kUnroll];
+            int c[kUnroll];  // DOF index (n_dof < 2^31)
             double w0[kUnroll], w1[kUnroll], w2[kUnroll], zz[kUnroll], pp[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 const int j = j0 + u * G;
                 const int kb = j / 3;
-                c[u] = j < ncol ? 3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb) : -1;
+                c[u] = j < ncol ? 3 * nb_col[b0 + kb] + (j - 3 * kb) : -1;
             }
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_p(int64_t n_nodes, const int32_
     }
     block_sum<1>(loc);
     if (threadIdx.x == 0)
-        part1[blockIdx.x] = loc[0];
+        part1[bid] = loc[0];  // indexed by row group, independent of dispatch order
 }
 
 __global__ __launch_bounds__(kBlock) void k_update(int64_t n, double *__restrict__ x, double *__restrict__ r,
@@ -208,7 +212,11 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, double *__restrict
                                                    const double *__restrict__ inv, PcgScalars *__restrict__ sc,
                                                    const double *__restrict__ part1, int nparts1,
                                                    double *__restrict__ part2) {
-    if (sc->status != 0)
+    __shared__ int32_t s_status;
+    if (threadIdx.x == 0)
+        s_status = sc->status;
+    __syncthreads();
+    if (s_status != 0)
         return;
     const int64_t it = sc->it_k1;
     double red[1];

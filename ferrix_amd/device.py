@@ -8,6 +8,9 @@ assembly, solver and step seams.
   (``src/solver/step/static_step.rs:183-195``), ``solve``
   (``Solver::solve``, ``src/solver/solvers/mod.rs:18-44``) and
   ``static_step`` (``StaticStep::solve``, ``static_step.rs:47-267``).
+* :class:`Batch` -- many resident systems (meshes x up to 4 load cases)
+  solved by one set of kernel launches, with systems joining while others
+  run (``fx_batch``).
 * :func:`run_job` -- every step of a :class:`~ferrix_amd.model.Job` in
   order, like ``main.rs:49-63`` and the reference's parity tests.
 
@@ -87,6 +90,7 @@ class Context:
             raise _ERR.get(rc, FerrixError)(self.lib.fx_last_error(None).decode())
         self.h = h
         self._meshes = weakref.WeakSet()
+        self._batches = weakref.WeakSet()
 
     def check(self, rc: int):
         if rc != L.FX_OK:
@@ -95,6 +99,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for b in list(self._batches):
+                b.close()
             for m in list(self._meshes):
                 m.close()
             self.lib.fx_ctx_destroy(self.h)
@@ -212,6 +218,76 @@ class DeviceMesh:
         self.ctx.check(self.lib.fx_static_step(self.h, C.byref(sd), C.byref(timer), C.byref(rep)))
         del keep
         return rep
+
+
+class Batch:
+    """Batched PCG over resident systems (``fx_batch``).  Per load case the
+    result equals :meth:`DeviceMesh.solve` with the same options (bitwise
+    for FP64)."""
+
+    def __init__(self, ctx: Context, opts: Optional[SolveOptions] = None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        o = (opts or SolveOptions()).c()
+        h = C.c_void_p()
+        ctx.check(self.lib.fx_batch_create(ctx.h, C.byref(o), C.byref(h)))
+        self.h = h
+        self._n = {}
+        ctx._batches.add(self)
+
+    def add(self, mesh: DeviceMesh, rhs) -> int:
+        """Admits mesh's resident K with rhs of shape (n_dof,) or
+        (n_rhs, n_dof); returns the slot."""
+        rhs = _c(np.atleast_2d(rhs), np.float64)
+        assert rhs.shape[1] == mesh.n_dof
+        slot = C.c_int32()
+        self.ctx.check(self.lib.fx_batch_add(self.h, mesh.h, rhs.shape[0], _p(rhs), C.byref(slot)))
+        self._n[slot.value] = rhs.shape
+        return slot.value
+
+    def run(self, max_iters: int = 0) -> int:
+        """Advances the running systems; returns how many still run."""
+        n = C.c_int32()
+        self.ctx.check(self.lib.fx_batch_run(self.h, max_iters, C.byref(n)))
+        return n.value
+
+    def poll(self, slot: int):
+        """(state, fx_solve_report): state 0 running, 1 converged,
+        -fx_status on an error."""
+        st = C.c_int32()
+        rep = L.fx_solve_report()
+        self.ctx.check(self.lib.fx_batch_poll(self.h, slot, C.byref(st), C.byref(rep)))
+        return st.value, rep
+
+    def get(self, slot: int) -> np.ndarray:
+        x = np.zeros(self._n[slot])
+        self.ctx.check(self.lib.fx_batch_get(self.h, slot, _p(x)))
+        return x
+
+    def result(self, slot: int) -> np.ndarray:
+        """Solutions of a finished slot; raises like DeviceMesh.solve when
+        its solve stopped on an error."""
+        st, rep = self.poll(slot)
+        if st < 0:
+            raise _ERR.get(-st, FerrixError)(f"batch slot {slot}: solve stopped with status {-st}")
+        if st == 0:
+            raise InvalidModelState(f"batch slot {slot} is still running")
+        return self.get(slot)
+
+    def remove(self, slot: int):
+        self.ctx.check(self.lib.fx_batch_remove(self.h, slot))
+        self._n.pop(slot, None)
+
+    def close(self):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
+            self.lib.fx_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def step_options(step: StepSpec, precision=L.FX_PREC_FP64) -> SolveOptions:

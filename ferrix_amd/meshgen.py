@@ -279,7 +279,7 @@ def to_inp(job: Job) -> str:
     nn = int(m.el_ptr[1] - m.el_ptr[0])
     out = ["*HEADING", "synthetic", "*NODE"]
     for nid, (x, y, z) in zip(m.node_ids, m.xyz):
-        out.append(f"{nid}, {x!r}, {y!r}, {z!r}")
+        out.append(f"{int(nid)}, {float(x)!r}, {float(y)!r}, {float(z)!r}")
     out.append(f"*ELEMENT, TYPE={'C3D4' if nn == 4 else 'C3D20'}, ELSET=EALL")
     for e in range(m.n_el):
         ids = [int(m.node_ids[c]) for c in m.conn[m.el_ptr[e]:m.el_ptr[e + 1]]]
@@ -290,20 +290,20 @@ def to_inp(job: Job) -> str:
         out.append(line)
     out.append("*MATERIAL, NAME=MAT")
     out.append("*ELASTIC")
-    out.append(f"{m.lut_v_e[0, 0]!r}, {m.lut_v_nu[0, 0]!r}")
+    out.append(f"{float(m.lut_v_e[0, 0])!r}, {float(m.lut_v_nu[0, 0])!r}")
     out.append("*SOLID SECTION, ELSET=EALL, MATERIAL=MAT")
     st = job.steps[0]
     out.append("*STEP, NLGEOM" if st.nlgeom else "*STEP")
     out.append("*STATIC")
-    out.append(f"{st.initial_time_increment!r}, {st.time_period!r}, {st.min_time_increment!r}, "
-               f"{st.max_time_increment!r}")
+    out.append(f"{float(st.initial_time_increment)!r}, {float(st.time_period)!r}, "
+               f"{float(st.min_time_increment)!r}, {float(st.max_time_increment)!r}")
     # the reference applies only the first *BOUNDARY data line, so one
     # keyword per BC record
     for d, v in zip(st.bc_dof, st.bc_val):
         out.append("*BOUNDARY")
-        out.append(f"{int(m.node_ids[d // 3])}, {d % 3 + 1}, {d % 3 + 1}, {v!r}")
+        out.append(f"{int(m.node_ids[d // 3])}, {d % 3 + 1}, {d % 3 + 1}, {float(v)!r}")
     out.append("*CLOAD")
     for d, v in zip(st.load_dof, st.load_val):
-        out.append(f"{int(m.node_ids[d // 3])}, {d % 3 + 1}, {v!r}")
+        out.append(f"{int(m.node_ids[d // 3])}, {d % 3 + 1}, {float(v)!r}")
     out.append("*END STEP")
     return "\n".join(out) + "\n"

@@ -251,6 +251,46 @@ fx_status fx_static_step(fx_mesh *mesh, const fx_step_def *step, fx_timer *timer
 fx_status fx_profile_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_t spmv_group,
                           double *ms_per_launch, double *bytes_per_launch);
 
+/* ---- batched solves ------------------------------------------------------
+ * The reference runs one job at a time: one Solver::solve per step on one K
+ * (solvers/mod.rs:18-44, static_step.rs:200-228).  A batch replaces N such
+ * calls on N resident meshes -- each with 1..4 load cases sharing its K --
+ * by one set of kernel launches over all of them.  Per load case the
+ * arithmetic, stopping rule and result are those of fx_solve with the same
+ * options: for FX_PREC_FP64 a batched solution is bitwise equal to fx_solve's,
+ * whatever else shares the batch.  Systems may join while others run
+ * (continuous batching); a mesh must not be reassembled while it is in a
+ * batch.  All calls run on the context stream.
+ */
+typedef struct fx_batch fx_batch;
+
+/* opts: precision FX_PREC_FP64 or FX_PREC_MIXED (NULL = reference values). */
+fx_status fx_batch_create(fx_ctx *ctx, const fx_solve_opts *opts, fx_batch **out);
+void fx_batch_destroy(fx_batch *batch);
+
+/* Admits one system: the mesh's resident K (after fx_assemble and
+ * fx_apply_penalty) with n_rhs (1..4) right-hand sides, rhs[r*n_dof + i].
+ * Its solve starts from x = 0 on the next fx_batch_run.  *slot identifies it
+ * until fx_batch_remove. */
+fx_status fx_batch_add(fx_batch *batch, fx_mesh *mesh, int32_t n_rhs, const double *rhs, int32_t *slot);
+
+/* Advances every running system by up to max_iters PCG iterations
+ * (max_iters <= 0: until none is running).  *n_running: systems still
+ * running on return. */
+fx_status fx_batch_run(fx_batch *batch, int64_t max_iters, int32_t *n_running);
+
+/* *state: 0 running, 1 converged, -fx_status when it stopped on an error
+ * (FX_ERR_NOT_CONVERGED, FX_ERR_BREAKDOWN).  report as for fx_solve
+ * (final_criterion: the largest over the load cases; seconds: from
+ * fx_batch_add to completion). */
+fx_status fx_batch_poll(fx_batch *batch, int32_t slot, int32_t *state, fx_solve_report *report);
+
+/* Copies the solutions, x[r*n_dof + i]. */
+fx_status fx_batch_get(fx_batch *batch, int32_t slot, double *x);
+
+/* Frees the slot for reuse (a running system is dropped). */
+fx_status fx_batch_remove(fx_batch *batch, int32_t slot);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,198 @@
+"""Batched solves (fx_batch): many resident systems, up to 4 load cases each,
+advanced by one set of kernel launches.
+
+Bars:
+* FP64: every batched load case is BIT FOR BIT the fx_solve result of the
+  same system and options (same lane group), alone or in any mix, admitted
+  at any time;
+* mixed precision: bit for bit alone vs in a mix (a slot's arithmetic does
+  not depend on its neighbours), and within 1e-8 relative of the direct
+  solve of the oracle's matrix;
+* the reference's stop rule and error behaviour (not converged) per slot.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from ferrix_amd import meshgen
+from ferrix_amd import _lib as L
+from ferrix_amd.device import Batch, DeviceMesh, SolveOptions
+from ferrix_amd.inp import ConvergenceError
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def system(ctx, jobs):
+    """Resident K (assembled, penalty applied) of jobs sharing one mesh and
+    BC set, with one right-hand side per job."""
+    j0 = jobs[0]
+    dm = DeviceMesh(ctx, j0.mesh, j0.node_temp)
+    dm.assemble(False)
+    rhs = []
+    for j in jobs:
+        st = j.steps[0]
+        b = np.zeros(j.n_dof)
+        np.add.at(b, st.load_dof, st.load_val)
+        rhs.append(b)
+    st = j0.steps[0]
+    out = []
+    pen = None
+    for b in rhs:
+        if pen is None:
+            bp, pen = dm.apply_penalty(st.bc_dof, st.bc_val, b)
+        else:  # K already carries the penalty: the right-hand side only (u = 0)
+            bp = b.copy()
+            for d, t in zip(st.bc_dof, st.bc_val):
+                bp[d] += pen * t
+        out.append(bp)
+    return dm, np.array(out)
+
+
+def population(ctx, specs):
+    return [system(ctx, meshgen.cfg4_mesh(s)) for s in specs]
+
+
+def small_specs(n, seed=11):
+    return meshgen.cfg4_batch_specs(n_meshes=n, seed=seed, dof_lo=300, dof_hi=6000)
+
+
+REL = SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000, spmv_group=8)
+
+
+def test_fp64_batch_equals_solve_bitwise(ctx):
+    sys = population(ctx, small_specs(6))
+    bt = Batch(ctx, REL)
+    slots = [bt.add(dm, b) for dm, b in sys]
+    assert bt.run() == 0
+    for (dm, b), s in zip(sys, slots):
+        st, rep = bt.poll(s)
+        assert st == 1
+        x = bt.get(s)
+        for r in range(b.shape[0]):
+            xs, rs = dm.solve(b[r], REL)
+            say("slot", s, "rhs", r, "n_dof", dm.n_dof, "iters", rs.iterations)
+            assert np.array_equal(x[r], xs), f"slot {s} rhs {r}: max diff {np.max(np.abs(x[r] - xs))}"
+
+
+def test_fp64_reference_rule_batch(ctx):
+    """Default options: sqrt(r.z) < 1e-8, 1000 iterations (the reference's
+    PCG); the slot's iteration count is its slowest load case's."""
+    sys = population(ctx, small_specs(3, seed=5))
+    bt = Batch(ctx, SolveOptions(spmv_group=8))
+    slots = [bt.add(dm, b) for dm, b in sys]
+    bt.run()
+    for (dm, b), s in zip(sys, slots):
+        st, rep = bt.poll(s)
+        if st < 0:
+            assert -st == L.FX_ERR_NOT_CONVERGED
+            continue
+        its = []
+        x = bt.get(s)
+        for r in range(b.shape[0]):
+            xs, rs = dm.solve(b[r], SolveOptions(spmv_group=8))
+            its.append(rs.iterations)
+            assert np.array_equal(x[r], xs)
+        assert rep.iterations == max(its)
+
+
+def test_continuous_admission_and_removal(ctx):
+    """Systems admitted while others run, one removed mid-solve and its slot
+    reused: every finished result equals the system solved alone."""
+    sys = population(ctx, small_specs(5, seed=23))
+    solo = []
+    for dm, b in sys:
+        bt = Batch(ctx, REL)
+        s = bt.add(dm, b)
+        bt.run()
+        solo.append(bt.result(s))
+        bt.close()
+    bt = Batch(ctx, REL)
+    s0 = bt.add(*sys[0])
+    s1 = bt.add(*sys[1])
+    bt.run(max_iters=20)
+    s2 = bt.add(*sys[2])
+    bt.run(max_iters=13)
+    bt.remove(s1)  # dropped while running
+    s3 = bt.add(*sys[3])
+    assert s3 == s1  # the slot is reused
+    bt.run(max_iters=7)
+    s4 = bt.add(*sys[4])
+    assert bt.run() == 0
+    for k, s in [(0, s0), (2, s2), (3, s3), (4, s4)]:
+        assert np.array_equal(bt.result(s), solo[k]), f"system {k}"
+
+
+@pytest.mark.parametrize("spmv_group", [4, 32])
+def test_mixed_batch_alone_vs_mix(ctx, spmv_group):
+    opts = SolveOptions(precision=L.FX_PREC_MIXED, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000,
+                        inner_tol=1e-3, spmv_group=spmv_group)
+    specs = small_specs(4, seed=31)
+    sys = population(ctx, specs)
+    solo = []
+    for dm, b in sys:
+        bt = Batch(ctx, opts)
+        s = bt.add(dm, b)
+        bt.run()
+        solo.append(bt.result(s))
+        bt.close()
+    bt = Batch(ctx, opts)
+    slots = [bt.add(dm, b) for dm, b in sys[:2]]
+    bt.run(max_iters=9)
+    slots += [bt.add(dm, b) for dm, b in sys[2:]]
+    bt.run()
+    for k, (s, spec) in enumerate(zip(slots, specs)):
+        st, rep = bt.poll(s)
+        assert st == 1 and rep.refinements >= 2
+        x = bt.get(s)
+        assert np.array_equal(x, solo[k]), f"system {k}"
+        # against the direct solve of the oracle's penalised matrix
+        jobs = meshgen.cfg4_mesh(spec)
+        csr, _, _ = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
+        b = sys[k][1]
+        for r in range(b.shape[0]):
+            xd = O.direct_solve(csr, b[r])
+            err = np.linalg.norm(x[r] - xd) / np.linalg.norm(xd)
+            say("system", k, "rhs", r, "rel err", err, "inner iters", rep.iterations, "passes", rep.refinements)
+            assert err < 1e-8
+
+
+def test_mixed_batch_matches_fp64_solve(ctx):
+    specs = small_specs(2, seed=41)
+    sys = population(ctx, specs)
+    mixed = SolveOptions(precision=L.FX_PREC_MIXED, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000)
+    bt = Batch(ctx, mixed)
+    slots = [bt.add(dm, b) for dm, b in sys]
+    bt.run()
+    for (dm, b), s in zip(sys, slots):
+        x = bt.result(s)
+        for r in range(b.shape[0]):
+            x64, _ = dm.solve(b[r], SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-12, max_iter=100_000))
+            assert np.linalg.norm(x[r] - x64) / np.linalg.norm(x64) < 1e-8
+
+
+def test_batch_not_converged(ctx):
+    sys = population(ctx, small_specs(2, seed=3))
+    bt = Batch(ctx, SolveOptions(max_iter=3, spmv_group=8))
+    slots = [bt.add(dm, b) for dm, b in sys]
+    bt.run()
+    for s in slots:
+        st, rep = bt.poll(s)
+        assert st == -L.FX_ERR_NOT_CONVERGED
+        assert rep.iterations == 3
+        with pytest.raises(ConvergenceError):
+            bt.result(s)
+
+
+def test_mesh_destroyed_while_batched(ctx):
+    sys = population(ctx, small_specs(2, seed=9))
+    bt = Batch(ctx, REL)
+    s0 = bt.add(*sys[0])
+    bt.add(*sys[1])
+    bt.run(max_iters=5)
+    sys[1][0].close()  # drops its slot from the batch
+    assert bt.run() == 0
+    assert bt.poll(s0)[0] == 1
