@@ -1,30 +1,42 @@
-"""Benchmark: DOF-solves/s and p50/p99 step latency of the static-step hot path
-on one MI355X, mixed precision vs FP64 PCG.
+"""Benchmark: DOF-solves/s and p50/p99 latency of the static-step hot path on
+one MI355X, mixed-precision batched PCG vs plain FP64 PCG, at relative
+displacement error <= 1e-8.
 
-Workload (default): linear *STATIC step on a 400x100x100 mm block meshed
-with 40x10x10 C3D20 bricks (18,821 nodes, 56,463 DOF), coordinates jittered
-by +/-0.5 mm (SplitMix64 seed 2), E=70000, nu=0.33, x=0 clamped, -1000 N tip
-load; Jacobi-PCG to ||r||/||b|| <= 1e-10.  One step = element stiffness +
-internal force + deterministic assembly + penalty BCs + PCG solve + the
-convergence check at the new displacement (the reference's linear step:
-two f_int passes, one K pass, one solve).  Mesh, loads and BCs are resident
-in HBM before timing; every timed step starts from u = 0 on the device.
+Default workload ("batch"): the batched-solve population -- 512 independent
+C3D4 Kuhn-tet box meshes (SplitMix64 seed 4; sizes log-uniform 2k-200k DOF,
+aspect 1-10, E log-uniform 1e3-3e5 MPa, nu 0.2-0.45, x=0 clamped) with 1-4
+load cases each (100 N in a random direction on the x=max face): 23.15 M DOF,
+54.27 M DOF-solves per pass.  One step = the whole population: linear
+assembly + penalty BCs of every mesh and a Jacobi-PCG solve of every load
+case to ||b - K x||/||b|| <= 1e-10.  Meshes are resident in HBM before
+timing; every step reassembles from u = 0.
 
-Arms: "fp64" (the reference's Jacobi-PCG, all FP64) and "mixed" (FP32-stored
-K inside FP64 iterative refinement).  value = DOF-solves/s of the mixed arm
-(n_dof x linear solves / wall seconds over the K timed steps); both arms'
-numbers, the U error against the FP64 CPU oracle's solution and p50/p99
-per-step latency are in the JSON line.
+Arms (identical work and stopping rule):
+* "fp64_serial": the reference's pattern on the GPU -- one FP64 fx_solve per
+  load case, system after system;
+* "mixed_batched" (headline): continuous batching (fx_batch) with FP32-stored
+  K inside FP64 iterative refinement, load cases of a system sharing each
+  pass over K, systems admitted under an operator working-set budget;
+* "fp64_batched" (optional): the same batching in FP64.
+value = DOF-solves/s of the headline arm (sum of n_dof x load cases / wall
+seconds over the K timed steps).  p50/p99: per-system time from the start of
+the step to its solution on the host.  Accuracy: every solution's relative
+L2 error against the fp64_serial solution, and a sample of systems against
+the CPU oracle's direct solve.
 
-roofline: the dominant kernel is the PCG SpMV (k_spmv_p).  Its mean launch
-duration is measured with HIP events on the library's stream
-(fx_profile_spmv); achieved = algorithmic bytes per launch / duration.
+roofline: the dominant kernel is the batched SpMV kb_spmv_p<G,float>; the
+first SpMV of every 4th iteration chunk of the timed run is bracketed by HIP
+events on the library's stream; achieved = sampled algorithmic bytes /
+sampled duration.
 
 cpu_baseline: the oracle (C restatement of the reference CPU path: serial
 element loop + triplets + CSR, Jacobi-PCG with a row-parallel SpMV on 4
-OpenMP threads, the reference's default rayon pool size), timed on a bounded
-sample (full assembly + 200 PCG iterations) and extrapolated to the
-iterations the FP64 solve needs.
+OpenMP threads, the reference's default pool size) on a bounded sample of
+the same population, DOF-solves/s over the sample.
+
+"--workload single": the C3D20 block (40x10x10, 56,463 DOF), one linear
+static step per timed step, FP64 vs mixed, with the same roofline and
+baseline figures for its single-system SpMV.
 """
 import argparse
 import ctypes
@@ -99,28 +111,8 @@ def cpu_baseline(job, fp64_iters, threads=4, sample_iters=200):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--arms", default="fp64,mixed")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--spmv-reps", type=int, default=100)
-    ap.add_argument("--inner-tol", type=float, default=1e-3)
-    ap.add_argument("--sweep-groups", action="store_true", help="print SpMV time per lane-group size")
-    ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="only run device-to-device copies of this many MiB (counter calibration) and exit")
-    a = ap.parse_args()
-    if a.calib_copy_mb > 0:
-        calib_copy(a.calib_copy_mb)
-        return
-    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
-              "(the batched and serving workloads shard by independent mesh, see the design notes).",
-              file=sys.stderr)
-        sys.exit(2)
-
+def run_single(a):
+    """Single-system workload: the C3D20 block, one linear static step."""
     from ferrix_amd import _lib as L
     from ferrix_amd import meshgen
     from ferrix_amd.device import Context, DeviceMesh, SolveOptions, step_def
@@ -228,6 +220,221 @@ def main():
         "cpu_baseline": {k: v for k, v in cpu.items() if k != "seconds_per_step"} if cpu else None,
     }
     print(json.dumps(out), flush=True)
+
+
+def population(n_meshes):
+    from ferrix_amd import meshgen
+    from ferrix_amd.batch import System
+
+    specs = meshgen.cfg4_batch_specs(n_meshes=n_meshes)
+    return specs, [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
+
+
+def cpu_baseline_batch(specs, iters=None, threads=4, sample_iters=50):
+    """Oracle on a bounded sample of the population: every 32nd system (16
+    systems spanning the size range).  Per system: serial assembly + CSR
+    timed in full; Jacobi-PCG timed over `sample_iters` iterations on
+    `threads` OpenMP threads and extrapolated to the iterations each load
+    case needs (`iters`: key -> per-load-case counts of the FP64 solve, same
+    algorithm and stopping rule); without counts the solves run in full."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from ferrix_amd import meshgen
+
+    secs, dofs, names, t_all = 0.0, 0, [], time.perf_counter()
+    for k in range(0, len(specs), 32):
+        sp = specs[k]
+        jobs = meshgen.cfg4_mesh(sp)
+        st = jobs[0].steps[0]
+        t0 = time.perf_counter()
+        r, c, v, md = O.assemble_triplets(jobs[0].mesh, jobs[0].node_temp)
+        bd = np.asarray(st.bc_dof, np.int32)
+        K = O.triplets_to_csr(jobs[0].n_dof, np.concatenate([r, bd]), np.concatenate([c, bd]),
+                              np.concatenate([v, np.full(len(bd), md * 1e6)]))
+        secs += time.perf_counter() - t0
+        for q, j in enumerate(jobs):
+            b = np.zeros(j.n_dof)
+            np.add.at(b, j.steps[0].load_dof, j.steps[0].load_val)
+            if iters is not None and k in iters:
+                t_it = O.lib().fxo_pcg_time_iters(K.n, O._p(K.row_ptr), O._p(K.col), O._p(K.val), O._p(b),
+                                                  sample_iters, threads) / sample_iters
+                secs += t_it * iters[k][q]
+            else:
+                t0 = time.perf_counter()
+                O.pcg(K, b, 1e-10, 100000, 1, threads)
+                secs += time.perf_counter() - t0
+            dofs += j.n_dof
+        names.append(f"{sp['n_dof']}x{len(jobs)}")
+    how = (f"PCG timed over {sample_iters} iterations per load case and extrapolated to the FP64 solve's iteration "
+           f"count" if iters is not None else "PCG run in full")
+    return {"value": dofs / secs, "unit": "DOF-solves/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle C restatement on the host over 16 systems of the population (every 32nd: "
+                       f"{', '.join(names)} DOF x load cases): serial assembly + triplets->CSR timed in full, "
+                       f"Jacobi-PCG to rel 1e-10 on {threads} OpenMP threads, {how}; {dofs} DOF-solves in "
+                       f"{secs:.1f} s of modelled CPU time ({time.perf_counter() - t_all:.1f} s measured)")}
+
+
+def run_batch(a):
+    from ferrix_amd import _lib as L
+    from ferrix_amd.batch import solve_batched, solve_serial, upload
+    from ferrix_amd.device import Context, SolveOptions
+
+    specs, systems = population(a.meshes)
+    n_dof = sum(s.n_dof for s in systems)
+    n_solves = sum(s.n_dof * s.n_rhs for s in systems)
+    print(f"# workload batch: {len(systems)} C3D4 meshes, {n_dof} DOF, {n_solves} DOF-solves per step", flush=True)
+    ctx = Context(0)
+    t = time.perf_counter()
+    upload(ctx, systems)
+    print(f"# upload + symbolic {time.perf_counter() - t:.1f} s", flush=True)
+    fp64 = SolveOptions(precision=L.FX_PREC_FP64, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000)
+    fp64_b = SolveOptions(precision=L.FX_PREC_FP64, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000,
+                          spmv_group=a.spmv_group)
+    mixed = SolveOptions(precision=L.FX_PREC_MIXED, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000,
+                         inner_tol=a.inner_tol, max_refine=100, spmv_group=a.spmv_group)
+    ws = int(a.ws_mb * (1 << 20))
+    arms, results = {}, {}
+    for arm in a.arms.split(","):
+        if arm == "fp64_serial":
+            step = lambda: solve_serial(systems, fp64)
+        elif arm == "mixed_batched":
+            step = lambda: solve_batched(ctx, systems, mixed, ws_bytes=ws, sample_every=4)
+        elif arm == "fp64_batched":
+            step = lambda: solve_batched(ctx, systems, fp64_b, ws_bytes=ws, sample_every=4)
+        else:
+            raise SystemExit(f"unknown arm {arm}")
+        for _ in range(a.warmup):
+            step()
+        device_sync()
+        t0 = time.perf_counter()
+        runs = []
+        for _ in range(a.steps):
+            runs.append(step())
+        device_sync()
+        wall = time.perf_counter() - t0
+        last = runs[-1]
+        lat = np.concatenate([np.array(list(r.latency.values())) for r in runs])
+        res = {
+            "dof_solves_per_s": n_solves * a.steps / wall,
+            "ms_per_step": 1e3 * wall / a.steps,
+            "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "failures": len(last.failures),
+            "pcg_iterations_total": int(sum(last.iterations.values())),
+        }
+        if last.spmv_stats:
+            launches, ns, ms, by = last.spmv_stats
+            res["spmv"] = {"launches": launches, "sampled": ns, "us_per_sampled_launch": 1e3 * ms / max(ns, 1),
+                           "achieved_GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None}
+        arms[arm] = res
+        results[arm] = last
+        print(f"# {arm}: {json.dumps(res)}", flush=True)
+
+    # accuracy: every solution against the FP64 serial one; a sample against the oracle
+    ref = results.get("fp64_serial")
+    if ref is not None:
+        for arm, r in results.items():
+            if arm == "fp64_serial":
+                continue
+            errs = [float(np.max(np.linalg.norm(r.x[k] - ref.x[k], axis=1) / np.linalg.norm(ref.x[k], axis=1)))
+                    for k in ref.x if k in r.x]
+            arms[arm]["u_rel_err_vs_fp64_max"] = float(max(errs)) if errs else None
+    if not a.no_oracle_check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        small = sorted(systems, key=lambda s: s.n_dof)[:a.oracle_sample]
+        for arm, r in results.items():
+            e = []
+            for s in small:
+                if s.key not in r.x:
+                    continue
+                st = s.jobs[0].steps[0]
+                csr, _, _ = O.assemble_csr(s.jobs[0].mesh, s.jobs[0].node_temp, None, st.bc_dof)
+                for k in range(s.n_rhs):
+                    xd = O.direct_solve(csr, s.rhs[k])
+                    e.append(np.linalg.norm(r.x[s.key][k] - xd) / np.linalg.norm(xd))
+            arms[arm]["u_rel_err_vs_oracle_sample_max"] = float(max(e)) if e else None
+    head = "mixed_batched" if "mixed_batched" in arms else next(iter(arms))
+    roof = None
+    st_ = arms[head].get("spmv")
+    if st_ and st_["achieved_GBps"]:
+        roof = {"bound": "hbm", "achieved": round(st_["achieved_GBps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(st_["achieved_GBps"] / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": "kb_spmv_p<G,float>" if head == "mixed_batched" else "kb_spmv_p<G,double>",
+                "us_per_launch": round(st_["us_per_sampled_launch"], 2), "sampled_launches": st_["sampled"]}
+    print(f"# roofline {json.dumps(roof)}", flush=True)
+    cpu = None
+    if not a.no_cpu_baseline:
+        ref = results.get("fp64_serial")
+        cpu = cpu_baseline_batch(specs, ref.iters_rhs if ref is not None else None)
+    if cpu:
+        print(f"# cpu {json.dumps(cpu)}", flush=True)
+    h = arms[head]
+    out = {
+        "metric": "DOF-solves/s + p50/p99 latency, mixed vs FP64 PCG at rel err <=1e-8",
+        "value": round(h["dof_solves_per_s"], 1),
+        "unit": "DOF-solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(h["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-stored K / fp64" if head == "mixed_batched" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batched_linear_static_c3d4_{len(systems)}_meshes_2k-200k_dof_1-4_load_cases_rel1e-10",
+                   "n_meshes": len(systems), "n_dof": n_dof, "dof_solves_per_step": n_solves,
+                   "working_set_mb": a.ws_mb},
+        "p50_ms": round(h["p50_ms"], 3),
+        "p99_ms": round(h["p99_ms"], 3),
+        "arms": arms,
+        "mixed_over_fp64": (round(arms["mixed_batched"]["dof_solves_per_s"] / arms["fp64_serial"]["dof_solves_per_s"],
+                                  3) if "mixed_batched" in arms and "fp64_serial" in arms else None),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["batch", "single"], default="batch")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--arms", default=None)
+    ap.add_argument("--meshes", type=int, default=512)
+    ap.add_argument("--ws-mb", type=float, default=0.0, help="operator working-set budget of the batch (0: all)")
+    ap.add_argument("--spmv-group", type=int, default=0, help="lanes per node row of the batched SpMV (0: auto)")
+    ap.add_argument("--oracle-sample", type=int, default=4)
+    ap.add_argument("--no-oracle-check", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=100)
+    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--sweep-groups", action="store_true", help="print SpMV time per lane-group size")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="only run device-to-device copies of this many MiB (counter calibration) and exit")
+    a = ap.parse_args()
+    if a.calib_copy_mb > 0:
+        calib_copy(a.calib_copy_mb)
+        return
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
+              "(the batched and serving workloads shard by independent mesh, see the design notes).",
+              file=sys.stderr)
+        sys.exit(2)
+    if a.workload == "single":
+        a.steps = 10 if a.steps is None else a.steps
+        a.warmup = 2 if a.warmup is None else a.warmup
+        a.arms = a.arms or "fp64,mixed"
+        run_single(a)
+    else:
+        a.steps = 1 if a.steps is None else a.steps
+        a.warmup = 1 if a.warmup is None else a.warmup
+        a.arms = a.arms or "fp64_serial,mixed_batched"
+        run_batch(a)
 
 
 if __name__ == "__main__":

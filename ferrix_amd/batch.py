@@ -1,0 +1,158 @@
+"""Host driver for many independent linear static systems on one GPU.
+
+A *system* is one mesh's resident K (after assembly and the penalty BCs,
+``static_step.rs:101-198``) with 1-4 load cases that share it.  Two ways to
+solve a population of systems:
+
+* :func:`solve_serial` -- one :meth:`DeviceMesh.solve` per load case, one
+  after the other: the reference's pattern (one ``Solver::solve`` per step,
+  ``solvers/mod.rs:18-44``) on the GPU, the FP64 comparison arm;
+* :func:`solve_batched` -- continuous batching on one :class:`Batch`:
+  systems are admitted while the running set's resident operator bytes stay
+  under a working-set budget, advanced together by shared kernel launches,
+  and retired as they finish, which frees room for the next ones.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import _lib as L
+from .device import Batch, DeviceMesh, SolveOptions
+from .model import Job
+
+
+@dataclass
+class System:
+    """A resident mesh plus its load cases (jobs sharing mesh and BCs)."""
+
+    key: int
+    jobs: List[Job]
+    dm: Optional[DeviceMesh] = None
+    rhs: Optional[np.ndarray] = None  # (n_rhs, n_dof) after the penalty
+    penalty: float = 0.0
+
+    @property
+    def n_dof(self) -> int:
+        return self.jobs[0].n_dof
+
+    @property
+    def n_rhs(self) -> int:
+        return len(self.jobs)
+
+
+def upload(ctx, systems: List[System]):
+    for s in systems:
+        if s.dm is None:
+            s.dm = DeviceMesh(ctx, s.jobs[0].mesh, s.jobs[0].node_temp)
+
+
+def assemble(s: System):
+    """Linear K at u = 0, penalty BCs of the first load case's step on K and
+    every load case's right-hand side (all load cases share the BC set).
+    The first right-hand side goes through fx_apply_penalty; the others get
+    the same per-record update rhs[d] += penalty * (target - 0)."""
+    s.dm.assemble(False)
+    st0 = s.jobs[0].steps[0]
+    rows = []
+    for k, j in enumerate(s.jobs):
+        st = j.steps[0]
+        b = np.zeros(j.n_dof)
+        np.add.at(b, st.load_dof, st.load_val)
+        if k == 0:
+            b, s.penalty = s.dm.apply_penalty(st0.bc_dof, st0.bc_val, b)
+        elif s.penalty > 0.0:
+            for d, t in zip(st0.bc_dof, st0.bc_val):
+                b[d] += s.penalty * (t - 0.0)
+        rows.append(b)
+    s.rhs = np.array(rows)
+
+
+@dataclass
+class PopulationResult:
+    x: Dict[int, np.ndarray] = field(default_factory=dict)     # key -> (n_rhs, n_dof)
+    seconds: float = 0.0                                         # wall, assembly + solve
+    latency: Dict[int, float] = field(default_factory=dict)     # key -> s from start to result on host
+    iterations: Dict[int, int] = field(default_factory=dict)      # key -> slowest load case
+    iters_rhs: Dict[int, List[int]] = field(default_factory=dict)  # serial: key -> per load case
+    failures: Dict[int, str] = field(default_factory=dict)
+    spmv_stats: tuple = ()  # batched: (launches, samples, sampled ms, sampled bytes)
+
+    def dof_solves(self, systems: List[System]) -> int:
+        return sum(s.n_dof * s.n_rhs for s in systems if s.key in self.x)
+
+
+def solve_serial(systems: List[System], opts: SolveOptions) -> PopulationResult:
+    """Assemble and solve each load case with fx_solve, one at a time."""
+    res = PopulationResult()
+    t0 = time.perf_counter()
+    for s in systems:
+        assemble(s)
+        xs, its = [], []
+        for b in s.rhs:
+            x, rep = s.dm.solve(b, opts)
+            xs.append(x)
+            its.append(rep.iterations)
+        res.x[s.key] = np.array(xs)
+        res.iterations[s.key] = max(its)
+        res.iters_rhs[s.key] = its
+        res.latency[s.key] = time.perf_counter() - t0
+    res.seconds = time.perf_counter() - t0
+    return res
+
+
+def operator_bytes(s: System, opts: SolveOptions) -> int:
+    """Resident bytes one SpMV pass of this system streams: K values in the
+    working precision plus one 4-byte column index per 3x3 block."""
+    vs = 4 if opts.precision == L.FX_PREC_MIXED else 8
+    return s.dm.nnz_struct * vs + (s.dm.nnz_struct // 9) * 4
+
+
+def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int = 0, chunk_iters: int = 32,
+                  sample_every: int = 0) -> PopulationResult:
+    """Continuous batching over `systems` in order.  ws_bytes > 0 caps the
+    summed operator bytes of the running systems (at least one always runs);
+    0 admits everything at once."""
+    res = PopulationResult()
+    bt = Batch(ctx, opts)
+    if sample_every:
+        bt.sample_spmv(sample_every)
+    running: Dict[int, System] = {}   # slot -> system
+    ws = 0
+    queue = list(systems)
+    t0 = time.perf_counter()
+    qi = 0
+    try:
+        while qi < len(queue) or running:
+            while qi < len(queue):
+                s = queue[qi]
+                need = operator_bytes(s, opts)
+                if running and ws_bytes > 0 and ws + need > ws_bytes:
+                    break
+                assemble(s)
+                slot = bt.add(s.dm, s.rhs)
+                running[slot] = s
+                ws += need
+                qi += 1
+            bt.run(max_iters=chunk_iters)
+            for slot in list(running):
+                st, rep = bt.poll(slot)
+                if st == 0:
+                    continue
+                s = running.pop(slot)
+                ws -= operator_bytes(s, opts)
+                if st == 1:
+                    res.x[s.key] = bt.get(slot)
+                else:
+                    res.failures[s.key] = f"status {-st}"
+                res.iterations[s.key] = rep.iterations
+                res.latency[s.key] = time.perf_counter() - t0
+                bt.remove(slot)
+        res.seconds = time.perf_counter() - t0
+        res.spmv_stats = bt.spmv_stats()
+    finally:
+        bt.close()
+    return res

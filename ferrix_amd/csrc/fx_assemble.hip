@@ -88,11 +88,35 @@ __global__ void k_jacobi(int64_t n, const int32_t *__restrict__ diag_slot, const
         inv32[i] = (float)v;
 }
 
-__global__ void k_convert32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride)
-        b[i] = (float)a[i];
+// FP32 copy of K plus, per node, the 3x3 block C_a[p][q] = sum over the
+// node row's blocks b of (K_ab - fl32(K_ab))[p][q]: the working operator
+// fl32(K) + blockdiag(C) has exactly K's node-row block sums, so the rounding
+// error annihilates rigid translations and stays small on the smooth modes
+// that PCG resolves last.  One thread per DOF row, columns in slot order.
+__global__ void k_convert32(int64_t n_dof, const int32_t *__restrict__ nb_ptr, const double *__restrict__ a,
+                            float *__restrict__ b, float *__restrict__ corr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_dof)
+        return;
+    const int64_t node = i / 3;
+    const int p = (int)(i % 3);
+    const int64_t b0 = nb_ptr[node];
+    const int deg = nb_ptr[node + 1] - (int)b0;
+    const int64_t base = 9 * b0 + (int64_t)p * 3 * deg;
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    for (int k = 0; k < deg; ++k) {
+        const double v0 = a[base + 3 * k], v1 = a[base + 3 * k + 1], v2 = a[base + 3 * k + 2];
+        const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
+        b[base + 3 * k] = f0;
+        b[base + 3 * k + 1] = f1;
+        b[base + 3 * k + 2] = f2;
+        c0 += v0 - (double)f0;
+        c1 += v1 - (double)f1;
+        c2 += v2 - (double)f2;
+    }
+    corr[9 * node + 3 * p] = (float)c0;
+    corr[9 * node + 3 * p + 1] = (float)c1;
+    corr[9 * node + 3 * p + 2] = (float)c2;
 }
 
 __global__ void k_residual(int64_t n, const double *__restrict__ fe, const double *__restrict__ fi,
@@ -213,8 +237,9 @@ void launch_jacobi(fx_mesh *m, bool want32) {
 
 void launch_convert32(fx_mesh *m) {
     m->kval32.alloc(m->nnz_struct);
-    hipLaunchKernelGGL(k_convert32, dim3(2048), dim3(256), 0, m->ctx->stream, m->nnz_struct, m->kval.p,
-                       m->kval32.p);
+    m->kcorr32.alloc(9 * m->n_nodes);
+    hipLaunchKernelGGL(k_convert32, dim3(grid1(m->n_dof, 256)), dim3(256), 0, m->ctx->stream, m->n_dof,
+                       m->nb_ptr.p, m->kval.p, m->kval32.p, m->kcorr32.p);
     FX_HIP(hipGetLastError());
 }
 

@@ -29,7 +29,7 @@ constexpr int kRmax = 4;
 constexpr int kBB = 256;  // threads per block in the batch kernels
 
 struct BRhs {
-    double rz[2], rr, bb, crit, pap, alpha, tol;
+    double rz[2], rr, bb, crit, pap, alpha, beta, tol;
     int64_t it, it_k1, max_iter;
     int32_t status;  // 0 running, 1 converged, 2 breakdown, 3 max_iter, 4 idle
     int32_t stop_mode;
@@ -39,23 +39,34 @@ struct BSys {
     const int32_t *nb_ptr, *nb_col;
     const void *val;                    // double (FP64 slots) or float (mixed inner)
     const double *val64, *inv;
+    const float *kc;                    // FP32 operator: per-node 3x3 row-sum corrections
     double *b;                          // inner right-hand sides
     const double *btrue;                // true right-hand sides (mixed)
     double *x, *r, *z, *p0, *p1, *ap;  // inner PCG vectors
     double *xacc;                       // FP64 solution of the refinement (mixed)
     double *part1, *part2;
     int64_t n_nodes, n_dof;
+    int64_t it;                         // iteration index of the running right-hand sides
     int32_t R, nparts1, nparts2, want;  // want: right-hand sides of an outer pass (bit mask)
+    int32_t go;                         // right-hand sides the next SpMV / update work on (bit mask)
     BRhs rhs[kRmax];
 };
 
 namespace {
 
-// sums nv <= NV values over nparts blocks (layout part[b*nv + k]); every
-// thread receives the result; fixed order
+// Per-iteration kernel sequence of a batch: kb_spmv_p, kb_red1, kb_update,
+// kb_red2.  The vector kernels write block partial sums; one block per
+// system (kb_red1 / kb_red2) sums them in a fixed order and takes the
+// scalar decisions (alpha, beta, stop rule), so every block of the next
+// vector kernel reads a handful of scalars instead of re-summing all
+// partials.  The sums have exactly the association of fx_pcg.hip's
+// reduce_partials (thread-strided, then block_sum), so FP64 results are
+// bitwise those of fx_solve.
+
+// sums nv <= NV values over nparts blocks (layout part[b*nv + k]); valid in
+// thread 0; fixed order
 template <int NV>
 __device__ __forceinline__ void bsum_partials(const double *part, int nparts, int nv, double (&out)[NV]) {
-    __shared__ double bc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
         out[k] = 0.0;
@@ -65,14 +76,6 @@ __device__ __forceinline__ void bsum_partials(const double *part, int nparts, in
             if (k < nv)
                 out[k] += part[(size_t)b * nv + k];
     block_sum<NV>(out);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            bc[k] = out[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-        out[k] = bc[k];
 }
 
 // Inner-solve start for the right-hand sides with status 0:
@@ -110,32 +113,27 @@ __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int
                 S.part2[((size_t)lb * R + r) * 3 + k] = v[3 * r + k];
 }
 
-template <int G, typename VT>
-__global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    BSys &S = sys[t.x];
-    const int lb = t.y, R = S.R;
-    __shared__ BRhs snap[kRmax];
-    if (threadIdx.x < kRmax && (int)threadIdx.x < R)
-        snap[threadIdx.x] = S.rhs[threadIdx.x];
-    __syncthreads();  // one block-wide view of the right-hand side states
+// One block per listed system, after kb_init or kb_update: r.z, r.r (and
+// b.b at iteration 0) per right-hand side, the stop rule of fx_pcg.hip's
+// k_spmv_p, beta, and the mask of right-hand sides the next SpMV works on.
+__global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int32_t *__restrict__ list) {
+    BSys &S = sys[list[blockIdx.x]];
+    const int R = S.R;
     bool any = false;
     for (int r = 0; r < R; ++r)
-        any |= snap[r].status == 0;
-    if (!any)
+        any |= S.rhs[r].status == 0;
+    if (!any)  // uniform: only this block's thread 0 writes these states, in earlier launches
         return;
     double red[3 * kRmax];
     bsum_partials<3 * kRmax>(S.part2, S.nparts2, 3 * R, red);
-    bool act[kRmax];
-    double beta[kRmax];
-    int64_t itv = 0;
-#pragma unroll
-    for (int r = 0; r < kRmax; ++r) {
-        act[r] = false;
-        beta[r] = 0.0;
-        if (r >= R || snap[r].status != 0)
+    if (threadIdx.x != 0)
+        return;
+    int go = 0;
+    int64_t itv = S.it;
+    for (int r = 0; r < R; ++r) {
+        BRhs &h = S.rhs[r];
+        if (h.status != 0)
             continue;
-        const BRhs &h = snap[r];
         const double rz = red[3 * r], rr = red[3 * r + 1];
         const int64_t it = h.it;
         const double bb = it == 0 ? red[3 * r + 2] : h.bb;
@@ -149,30 +147,41 @@ __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const i
             conv = crit <= h.tol;
         }
         const bool stop_max = it >= h.max_iter;
-        act[r] = !(conv || stop_max);
-        beta[r] = it == 0 ? 0.0 : rz / h.rz[(it - 1) & 1];
-        itv = it;
-        if (lb == 0 && threadIdx.x == 0) {
-            BRhs &hw = S.rhs[r];
-            hw.rz[it & 1] = rz;
-            hw.rr = rr;
-            hw.crit = crit;
-            hw.it_k1 = it;
-            if (it == 0)
-                hw.bb = bb;
-            if (stop_max)
-                hw.status = 3;
-            else if (conv)
-                hw.status = 1;
-        }
+        h.beta = it == 0 ? 0.0 : rz / h.rz[(it - 1) & 1];
+        h.rz[it & 1] = rz;
+        h.rr = rr;
+        h.crit = crit;
+        h.it_k1 = it;
+        if (it == 0)
+            h.bb = bb;
+        if (stop_max)
+            h.status = 3;
+        else if (conv)
+            h.status = 1;
+        else
+            go |= 1 << r;
+        itv = it;  // the running right-hand sides of a system share it
     }
-    bool anyact = false;
-#pragma unroll
-    for (int r = 0; r < kRmax; ++r)
-        anyact |= act[r];
-    if (!anyact)
+    S.it = itv;
+    S.go = go;
+}
+
+template <int G, typename VT>
+__global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
         return;
-    // the running right-hand sides of a slot started together and share it
+    const int lb = t.y, R = S.R;
+    bool act[kRmax];
+    double beta[kRmax];
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r) {
+        act[r] = (go >> r) & 1;
+        beta[r] = act[r] ? S.rhs[r].beta : 0.0;
+    }
+    const int64_t itv = S.it;
     const double *pold = (itv & 1) ? S.p1 : S.p0;
     double *pnew = (itv & 1) ? S.p0 : S.p1;
     const VT *val = (const VT *)S.val;
@@ -188,18 +197,40 @@ __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const i
     if (a < S.n_nodes) {
         const int b0 = S.nb_ptr[a], ncol = 3 * (S.nb_ptr[a + 1] - b0);
         const VT *v0 = val + 9 * (int64_t)b0;
-        for (int j = lane; j < ncol; j += G) {
-            const int kb = j / 3;
-            const int64_t c = (3 * (int64_t)S.nb_col[b0 + kb] + (j - 3 * kb)) * R;
-            const double w0 = (double)v0[j], w1 = (double)v0[ncol + j], w2 = (double)v0[2 * ncol + j];
+        const double *z = S.z;
+        // two slots per lane in flight; accumulation order j, j+G, j+2G, ...
+        // as in a plain strided loop
+        for (int j0 = lane; j0 < ncol; j0 += 2 * G) {
+            int c[2];
+            double w[2][3];
 #pragma unroll
-            for (int r = 0; r < kRmax; ++r) {
-                if (!act[r])
+            for (int u = 0; u < 2; ++u) {
+                const int j = j0 + u * G;
+                const int kb = j / 3;
+                c[u] = j < ncol ? (3 * S.nb_col[b0 + kb] + (j - 3 * kb)) * R : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int j = j0 + u * G;
+                if (c[u] >= 0) {
+                    w[u][0] = (double)v0[j];
+                    w[u][1] = (double)v0[ncol + j];
+                    w[u][2] = (double)v0[2 * ncol + j];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (c[u] < 0)
                     continue;
-                const double xj = S.z[c + r] + beta[r] * pold[c + r];
-                acc[0][r] += w0 * xj;
-                acc[1][r] += w1 * xj;
-                acc[2][r] += w2 * xj;
+#pragma unroll
+                for (int r = 0; r < kRmax; ++r) {
+                    if (!act[r])
+                        continue;
+                    const double xj = z[c[u] + r] + beta[r] * pold[c[u] + r];
+                    acc[0][r] += w[u][0] * xj;
+                    acc[1][r] += w[u][1] * xj;
+                    acc[2][r] += w[u][2] * xj;
+                }
             }
         }
     }
@@ -219,14 +250,25 @@ __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const i
         for (int r = 0; r < kRmax; ++r) {
             if (!act[r])
                 continue;
-            double s = 0.0;
+            double qv[3];
 #pragma unroll
             for (int p = 0; p < 3; ++p) {
                 const int64_t q = (3 * a + p) * R + r;
-                const double qn = S.z[q] + beta[r] * pold[q];
-                pnew[q] = qn;
-                S.ap[q] = acc[p][r];
-                s += qn * acc[p][r];
+                qv[p] = S.z[q] + beta[r] * pold[q];
+                pnew[q] = qv[p];
+            }
+            if constexpr (sizeof(VT) == 4) {
+                const float *c = S.kc + 9 * a;
+#pragma unroll
+                for (int p = 0; p < 3; ++p)
+                    acc[p][r] += ((double)c[3 * p] * qv[0] + (double)c[3 * p + 1] * qv[1]) +
+                                 (double)c[3 * p + 2] * qv[2];
+            }
+            double s = 0.0;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                S.ap[(3 * a + p) * R + r] = acc[p][r];
+                s += qv[p] * acc[p][r];
             }
             loc[r] = s;
         }
@@ -237,46 +279,49 @@ __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const i
             S.part1[(size_t)lb * R + r] = loc[r];
 }
 
-__global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    BSys &S = sys[t.x];
-    const int lb = t.y, R = S.R;
-    __shared__ BRhs snap[kRmax];
-    if (threadIdx.x < kRmax && (int)threadIdx.x < R)
-        snap[threadIdx.x] = S.rhs[threadIdx.x];
-    __syncthreads();
-    bool any = false;
-    for (int r = 0; r < R; ++r)
-        any |= snap[r].status == 0;
-    if (!any)
+// One block per listed system, after kb_spmv_p: p.Ap, alpha, breakdown
+// (k_update's rule in fx_pcg.hip) and the mask the update works on.
+__global__ __launch_bounds__(kBB) void kb_red1(BSys *__restrict__ sys, const int32_t *__restrict__ list) {
+    BSys &S = sys[list[blockIdx.x]];
+    const int go = S.go;
+    if (!go)
         return;
+    const int R = S.R;
     double red[kRmax];
     bsum_partials<kRmax>(S.part1, S.nparts1, R, red);
-    bool act[kRmax];
-    double alpha[kRmax];
-    int64_t itv = 0;
-#pragma unroll
-    for (int r = 0; r < kRmax; ++r) {
-        act[r] = false;
-        alpha[r] = 0.0;
-        if (r >= R || snap[r].status != 0)
+    if (threadIdx.x != 0)
+        return;
+    int go2 = 0;
+    for (int r = 0; r < R; ++r) {
+        if (!((go >> r) & 1))
             continue;
-        const BRhs &h = snap[r];
+        BRhs &h = S.rhs[r];
         const double pap = red[r];
         const double al = h.rz[h.it_k1 & 1] / pap;
-        const bool bad = !(pap > 0.0) || !isfinite(al);
-        act[r] = !bad;
-        alpha[r] = al;
-        itv = h.it_k1;
-        if (lb == 0 && threadIdx.x == 0) {
-            BRhs &hw = S.rhs[r];
-            hw.pap = pap;
-            hw.alpha = al;
-            if (bad)
-                hw.status = 2;
+        h.pap = pap;
+        h.alpha = al;
+        if (!(pap > 0.0) || !isfinite(al)) {
+            h.status = 2;
+        } else {
+            h.it = h.it_k1 + 1;
+            go2 |= 1 << r;
         }
     }
-    const double *p = (itv & 1) ? S.p0 : S.p1;  // p_i sits in the "new" buffer of iteration i
+    S.go = go2;
+}
+
+__global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    const int lb = t.y, R = S.R;
+    double alpha[kRmax];
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r)
+        alpha[r] = ((go >> r) & 1) ? S.rhs[r].alpha : 0.0;
+    const double *p = (S.it & 1) ? S.p0 : S.p1;  // p_i sits in the "new" buffer of iteration i
     double v[2 * kRmax];
 #pragma unroll
     for (int k = 0; k < 2 * kRmax; ++k)
@@ -286,7 +331,7 @@ __global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const i
         const double inv = S.inv[i];
 #pragma unroll
         for (int r = 0; r < kRmax; ++r) {
-            if (!act[r])
+            if (!((go >> r) & 1))
                 continue;
             const int64_t q = i * R + r;
             const double xi = S.x[q] + alpha[r] * p[q];
@@ -300,17 +345,12 @@ __global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const i
         }
     }
     block_sum<2 * kRmax>(v);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
         for (int r = 0; r < R; ++r) {
             S.part2[((size_t)lb * R + r) * 3] = v[2 * r];
             S.part2[((size_t)lb * R + r) * 3 + 1] = v[2 * r + 1];
             S.part2[((size_t)lb * R + r) * 3 + 2] = 0.0;
         }
-        if (lb == 0)
-            for (int r = 0; r < R; ++r)
-                if (act[r])
-                    S.rhs[r].it = snap[r].it_k1 + 1;
-    }
 }
 
 // Mixed refinement, step 1: xacc += x for the right-hand sides in S.want.
@@ -452,6 +492,7 @@ struct fx_batch {
         fx_status err = FX_OK;
         int64_t iters = 0, outer = 0, budget = 0;
         double crit[fx::kRmax] = {0, 0, 0, 0};
+        double prev[fx::kRmax] = {HUGE_VAL, HUGE_VAL, HUGE_VAL, HUGE_VAL};  // last outer criterion
         bool done[fx::kRmax] = {false, false, false, false};
         fx::DBuf<double> b, rin, x, r, z, p0, p1, ap, xacc, part1, part2;
         std::chrono::steady_clock::time_point t0;
@@ -462,11 +503,25 @@ struct fx_batch {
     std::vector<fx::BSys> h_sys;
     fx::DBuf<fx::BSys> d_sys;
     fx::DBuf<int2> d_tab1, d_tab2, d_tab3, d_tab4;
-    fx::DBuf<int32_t> d_list;
+    fx::DBuf<int32_t> d_list, d_list2, d_run;
     fx::DBuf<double> d_out;
-    int n1 = 0, n2 = 0;
+    std::vector<int2> h_tab_start;
+    int n1 = 0, n2 = 0, nrun = 0;
     bool tables_dirty = true;
     int64_t resident_bytes = 0;
+    // SpMV launch sampling (instrumentation): the first SpMV of every
+    // sample_every-th chunk is bracketed by two HIP events on the stream
+    int sample_every = 0;
+    int64_t chunks = 0, spmv_launches = 0, samples = 0;
+    double sampled_ms = 0.0, sampled_bytes = 0.0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool ev_pending = false;
+    double ev_bytes = 0.0;
+    ~fx_batch() {
+        for (auto &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
@@ -508,6 +563,7 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.nb_col = m->nb_col.p;
     S.val = is_mixed(B) ? (const void *)m->kval32.p : (const void *)m->kval.p;
     S.val64 = m->kval.p;
+    S.kc = is_mixed(B) ? m->kcorr32.p : nullptr;
     S.inv = m->inv_diag.p;
     S.b = is_mixed(B) ? sl.rin.p : sl.b.p;
     S.btrue = sl.b.p;
@@ -554,7 +610,17 @@ void build_tables(fx_batch *B) {
     }
     B->n1 = (int)t1.size();
     B->n2 = (int)t2.size();
+    std::vector<int> run;
+    for (size_t k = 0; k < B->slots.size(); ++k)
+        if (B->slots[k]->used && B->slots[k]->phase == 0)
+            run.push_back((int)k);
+    B->nrun = (int)run.size();
     hipStream_t s = B->ctx->stream;
+    if (B->nrun) {
+        B->d_run.alloc(std::max<size_t>(B->d_run.n, run.size()));
+        FX_HIP(hipMemcpyAsync(B->d_run.p, run.data(), run.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        FX_HIP(hipStreamSynchronize(s));
+    }
     if (B->n1) {
         B->d_tab1.alloc(std::max<size_t>(B->d_tab1.n, t1.size()));
         FX_HIP(hipMemcpyAsync(B->d_tab1.p, t1.data(), t1.size() * sizeof(int2), hipMemcpyHostToDevice, s));
@@ -595,21 +661,90 @@ void download_sys(fx_batch *B) {
     FX_HIP(hipStreamSynchronize(B->ctx->stream));
 }
 
-void launch_init(fx_batch *B) {
-    if (B->n2)
-        hipLaunchKernelGGL(fx::kb_init, dim3(B->n2), dim3(kBB), 0, B->ctx->stream, B->d_sys.p, B->d_tab2.p);
+// Algorithmic bytes of one SpMV launch over the running slots, from the
+// host copy of their states: K values (FP32 or FP64), one column index per
+// 3x3 block, node-row pointers, and per active right-hand side z and p_old
+// gathered (8 B each per DOF at least once) plus p_new and Ap written.
+double spmv_bytes(const fx_batch *B) {
+    const double vs = is_mixed(B) ? 4.0 : 8.0;
+    double by = 0.0;
+    for (size_t k = 0; k < B->slots.size(); ++k) {
+        const auto &sl = *B->slots[k];
+        if (!sl.used || sl.phase != 0)
+            continue;
+        const BSys &S = B->h_sys[k];
+        int act = 0;
+        for (int r = 0; r < S.R; ++r)
+            act += S.rhs[r].status == 0;
+        if (!act)
+            continue;
+        const fx_mesh *m = sl.mesh;
+        by += vs * (double)m->nnz_struct + 4.0 * (double)m->nblk + 4.0 * (double)(m->n_nodes + 1) +
+              32.0 * (double)act * (double)m->n_dof + (is_mixed(B) ? 36.0 * (double)m->n_nodes : 0.0);
+    }
+    return by;
+}
+
+// kb_init over the vector blocks of `slots`, then kb_red2 on them: their
+// inner solves start with the next iteration chunk.
+void start_slots(fx_batch *B, const std::vector<int> &slots) {
+    if (slots.empty())
+        return;
+    hipStream_t s = B->ctx->stream;
+    std::vector<int2> &t = B->h_tab_start;
+    t.clear();
+    for (int k : slots)
+        for (int b = 0; b < B->h_sys[k].nparts2; ++b)
+            t.push_back(make_int2(k, b));
+    B->d_tab4.alloc(std::max<size_t>(B->d_tab4.n, t.size()));
+    B->d_list2.alloc(std::max<size_t>(B->d_list2.n, slots.size()));
+    FX_HIP(hipMemcpyAsync(B->d_tab4.p, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    FX_HIP(hipMemcpyAsync(B->d_list2.p, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(fx::kb_init, dim3((unsigned)t.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_tab4.p);
+    hipLaunchKernelGGL(fx::kb_red2, dim3((unsigned)slots.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_list2.p);
+    FX_HIP(hipGetLastError());
+    FX_HIP(hipStreamSynchronize(s));  // the host tables are reused
 }
 
 void launch_iterations(fx_batch *B, int n) {
     hipStream_t s = B->ctx->stream;
+    const bool sample = B->sample_every > 0 && n > 0 && B->chunks % B->sample_every == 0;
+    B->chunks++;
+    if (sample) {
+        for (auto &e : B->ev)
+            if (!e)
+                FX_HIP(hipEventCreate(&e));
+        B->ev_bytes = spmv_bytes(B);
+    }
     for (int k = 0; k < n; ++k) {
+        if (sample && k == 0)
+            FX_HIP(hipEventRecord(B->ev[0], s));
         if (is_mixed(B))
             fx::launch_bspmv<float>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
         else
             fx::launch_bspmv<double>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
+        if (sample && k == 0) {
+            FX_HIP(hipEventRecord(B->ev[1], s));
+            B->ev_pending = true;
+        }
+        hipLaunchKernelGGL(fx::kb_red1, dim3(B->nrun), dim3(kBB), 0, s, B->d_sys.p, B->d_run.p);
         hipLaunchKernelGGL(fx::kb_update, dim3(B->n2), dim3(kBB), 0, s, B->d_sys.p, B->d_tab2.p);
+        hipLaunchKernelGGL(fx::kb_red2, dim3(B->nrun), dim3(kBB), 0, s, B->d_sys.p, B->d_run.p);
     }
+    B->spmv_launches += n;
     FX_HIP(hipGetLastError());
+}
+
+// after the stream has been synchronized
+void collect_sample(fx_batch *B) {
+    if (!B->ev_pending)
+        return;
+    float ms = 0.f;
+    FX_HIP(hipEventElapsedTime(&ms, B->ev[0], B->ev[1]));
+    B->sampled_ms += ms;
+    B->sampled_bytes += B->ev_bytes;
+    B->samples++;
+    B->ev_pending = false;
 }
 
 // After a chunk: per slot, see whether its running inner solves finished;
@@ -617,6 +752,7 @@ void launch_iterations(fx_batch *B, int n) {
 // true when the set of running slots changed.
 bool advance_phases(fx_batch *B) {
     download_sys(B);
+    collect_sample(B);
     bool changed = false;
     std::vector<int> outer;
     for (size_t k = 0; k < B->slots.size(); ++k) {
@@ -710,6 +846,8 @@ bool advance_phases(fx_batch *B) {
                 crit = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : 0.0;
                 conv = crit <= B->opts.tol;
             }
+            conv = conv || fx::at_rounding_floor(crit, sl.prev[r], B->opts.tol);
+            sl.prev[r] = crit;
             sl.crit[r] = crit;
             if (conv)
                 sl.done[r] = true;
@@ -733,16 +871,7 @@ bool advance_phases(fx_batch *B) {
         }
     }
     upload_sys(B);
-    if (!restart.empty()) {
-        std::vector<int2> t;
-        for (int k : restart)
-            for (int b = 0; b < B->h_sys[k].nparts2; ++b)
-                t.push_back(make_int2(k, b));
-        B->d_tab4.alloc(std::max<size_t>(B->d_tab4.n, t.size()));
-        FX_HIP(hipMemcpyAsync(B->d_tab4.p, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(fx::kb_init, dim3((unsigned)t.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_tab4.p);
-        FX_HIP(hipGetLastError());
-    }
+    start_slots(B, restart);
     return changed;
 }
 
@@ -820,6 +949,8 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
             throw fx::Error{FX_ERR_INVALID, "fx_batch_add: 1 to 4 right-hand sides per system"};
         if (!m->k_valid)
             throw fx::Error{FX_ERR_INVALID, "fx_batch_add: assemble the mesh first"};
+        if (m->n_dof * n_rhs >= ((int64_t)1 << 31))
+            throw fx::Error{FX_ERR_INVALID, "fx_batch_add: n_dof x n_rhs must stay below 2^31"};
         FX_HIP(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         if (B->G == 0)
@@ -868,15 +999,7 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         // this entry only
         reserve_sys(B, B->h_sys.size());
         FX_HIP(hipMemcpyAsync(B->d_sys.p + k, &S, sizeof(BSys), hipMemcpyHostToDevice, s));
-        // start this slot's inner solve now (only its blocks)
-        std::vector<int2> t;
-        for (int b = 0; b < S.nparts2; ++b)
-            t.push_back(make_int2(k, b));
-        fx::DBuf<int2> dt;
-        dt.upload(t, s);
-        hipLaunchKernelGGL(fx::kb_init, dim3((unsigned)t.size()), dim3(kBB), 0, s, B->d_sys.p, dt.p);
-        FX_HIP(hipGetLastError());
-        FX_HIP(hipStreamSynchronize(s));
+        start_slots(B, std::vector<int>{k});  // start this slot's inner solve now
         B->tables_dirty = true;
         *slot_out = k;
     } catch (const fx::Error &e) {
@@ -920,6 +1043,9 @@ fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
     } catch (const fx::Error &e) {
         ctx->err = e.msg;
         return e.code;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return FX_ERR_OOM;
     }
     return FX_OK;
 }
@@ -963,6 +1089,28 @@ fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
         B->ctx->err = e.msg;
         return e.code;
     }
+    return FX_OK;
+}
+
+fx_status fx_batch_sample_spmv(fx_batch *B, int32_t every) {
+    if (!B || every < 0)
+        return FX_ERR_INVALID;
+    B->sample_every = every;
+    return FX_OK;
+}
+
+fx_status fx_batch_spmv_stats(fx_batch *B, int64_t *launches, int64_t *samples, double *ms_sampled,
+                              double *bytes_sampled) {
+    if (!B)
+        return FX_ERR_INVALID;
+    if (launches)
+        *launches = B->spmv_launches;
+    if (samples)
+        *samples = B->samples;
+    if (ms_sampled)
+        *ms_sampled = B->sampled_ms;
+    if (bytes_sampled)
+        *bytes_sampled = B->sampled_bytes;
     return FX_OK;
 }
 

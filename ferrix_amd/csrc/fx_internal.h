@@ -165,7 +165,8 @@ struct fx_mesh {
     fx::DBuf<int32_t> nod_cptr, nod_cbase;  // per node: bases into fe
     fx::DBuf<int32_t> diag_slot;         // per dof: slot of K[d,d]
     fx::DBuf<double> kval;               // nnz_struct values (super-pattern)
-    fx::DBuf<float> kval32;
+    fx::DBuf<float> kval32;              // FP32 operator of the mixed solve ...
+    fx::DBuf<float> kcorr32;             // ... and its per-node 3x3 row-sum corrections
     fx::DBuf<uint8_t> keep;              // 1 if the slot survives the drop rule
     fx::DBuf<double> inv_diag;
     fx::DBuf<float> inv_diag32;
@@ -203,6 +204,7 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
 void spmv64(fx_mesh *m, const double *x, double *y, int group);
 double profile_spmv(fx_mesh *m, int precision, int reps, int group, double *bytes);
 int default_group(fx_mesh *m);  // lanes per node row for the SpMV
+bool at_rounding_floor(double crit, double crit_prev, double tol);  // mixed outer stop
 void batch_free(fx_batch *b);                 // deletes a batch (its context's stream is idle)
 void batch_drop_mesh(fx_batch *b, fx_mesh *m);  // removes the slots that use m
 void axpy_dev(fx_mesh *m, int64_t n, double a, const double *x, double *y);

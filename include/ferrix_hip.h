@@ -291,6 +291,17 @@ fx_status fx_batch_get(fx_batch *batch, int32_t slot, double *x);
 /* Frees the slot for reuse (a running system is dropped). */
 fx_status fx_batch_remove(fx_batch *batch, int32_t slot);
 
+/* Instrumentation (no counterpart in the reference): from now on, bracket
+ * the first SpMV launch of every `every`-th iteration chunk with two HIP
+ * events on the context stream (0 = off). */
+fx_status fx_batch_sample_spmv(fx_batch *batch, int32_t every);
+
+/* SpMV launches issued so far; sampled launches, their summed duration (ms)
+ * and summed algorithmic bytes (as for fx_profile_spmv, per running system,
+ * with the right-hand sides still active at the chunk start). */
+fx_status fx_batch_spmv_stats(fx_batch *batch, int64_t *launches, int64_t *samples, double *ms_sampled,
+                              double *bytes_sampled);
+
 #ifdef __cplusplus
 }
 #endif
