@@ -274,6 +274,20 @@ def cpu_baseline_batch(specs, iters=None, threads=4, sample_iters=50):
                        f"{secs:.1f} s of modelled CPU time ({time.perf_counter() - t_all:.1f} s measured)")}
 
 
+def counter_traffic(name, bytes_per_launch):
+    """HBM bytes per launch from the committed rocprofv3 counter summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel):
+    its measured traffic/algorithmic ratio times this run's algorithmic
+    bytes per launch; None when no summary is committed."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_over_algorithmic:"):
+            return round(float(line.split(":", 1)[1]) * bytes_per_launch)
+    return None
+
+
 def run_batch(a):
     from ferrix_amd import _lib as L
     from ferrix_amd.batch import solve_batched, solve_serial, upload
@@ -298,9 +312,9 @@ def run_batch(a):
         if arm == "fp64_serial":
             step = lambda: solve_serial(systems, fp64)
         elif arm == "mixed_batched":
-            step = lambda: solve_batched(ctx, systems, mixed, ws_bytes=ws, sample_every=4)
+            step = lambda: solve_batched(ctx, systems, mixed, ws_bytes=ws, time_spmv=True)
         elif arm == "fp64_batched":
-            step = lambda: solve_batched(ctx, systems, fp64_b, ws_bytes=ws, sample_every=4)
+            step = lambda: solve_batched(ctx, systems, fp64_b, ws_bytes=ws, time_spmv=True)
         else:
             raise SystemExit(f"unknown arm {arm}")
         for _ in range(a.warmup):
@@ -323,9 +337,10 @@ def run_batch(a):
             "pcg_iterations_total": int(sum(last.iterations.values())),
         }
         if last.spmv_stats:
-            launches, ns, ms, by = last.spmv_stats
-            res["spmv"] = {"launches": launches, "sampled": ns, "us_per_sampled_launch": 1e3 * ms / max(ns, 1),
-                           "achieved_GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None}
+            launches, nt, ms, by = last.spmv_stats
+            res["spmv"] = {"launches": launches, "timed": nt, "us_per_launch": 1e3 * ms / max(nt, 1),
+                           "bytes_per_launch": by / max(launches, 1),
+                           "achieved_GBps": by / (ms * 1e-3) / 1e9 if ms > 0 and nt == launches else None}
         arms[arm] = res
         results[arm] = last
         print(f"# {arm}: {json.dumps(res)}", flush=True)
@@ -360,9 +375,12 @@ def run_batch(a):
     st_ = arms[head].get("spmv")
     if st_ and st_["achieved_GBps"]:
         roof = {"bound": "hbm", "achieved": round(st_["achieved_GBps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(st_["achieved_GBps"] / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "kb_spmv_p<G,float>" if head == "mixed_batched" else "kb_spmv_p<G,double>",
-                "us_per_launch": round(st_["us_per_sampled_launch"], 2), "sampled_launches": st_["sampled"]}
+                "frac": round(st_["achieved_GBps"] / HBM_PEAK_GBPS, 4),
+                "traffic": (counter_traffic("kb_spmv_sell_float_batch512_counters.md", st_["bytes_per_launch"])
+                            if head == "mixed_batched" and a.meshes == 512 else None),
+                "kernel": "kb_spmv_sell<float>" if head == "mixed_batched" else "kb_spmv_sell<double>",
+                "us_per_launch": round(st_["us_per_launch"], 2), "bytes_per_launch": round(st_["bytes_per_launch"]),
+                "launches": st_["launches"]}
     print(f"# roofline {json.dumps(roof)}", flush=True)
     cpu = None
     if not a.no_cpu_baseline:

@@ -93,7 +93,7 @@ SIGNATURES = {
     "fx_batch_poll": (i32, [P, i32, P, P]),
     "fx_batch_get": (i32, [P, i32, P]),
     "fx_batch_remove": (i32, [P, i32]),
-    "fx_batch_sample_spmv": (i32, [P, i32]),
+    "fx_batch_time_spmv": (i32, [P, i32]),
     "fx_batch_spmv_stats": (i32, [P, P, P, P, P]),
 }
 

@@ -79,7 +79,7 @@ class PopulationResult:
     iterations: Dict[int, int] = field(default_factory=dict)      # key -> slowest load case
     iters_rhs: Dict[int, List[int]] = field(default_factory=dict)  # serial: key -> per load case
     failures: Dict[int, str] = field(default_factory=dict)
-    spmv_stats: tuple = ()  # batched: (launches, samples, sampled ms, sampled bytes)
+    spmv_stats: tuple = ()  # batched: (launches, timed launches, timed ms, algorithmic bytes of all launches)
 
     def dof_solves(self, systems: List[System]) -> int:
         return sum(s.n_dof * s.n_rhs for s in systems if s.key in self.x)
@@ -112,14 +112,14 @@ def operator_bytes(s: System, opts: SolveOptions) -> int:
 
 
 def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int = 0, chunk_iters: int = 32,
-                  sample_every: int = 0) -> PopulationResult:
+                  time_spmv: bool = False) -> PopulationResult:
     """Continuous batching over `systems` in order.  ws_bytes > 0 caps the
     summed operator bytes of the running systems (at least one always runs);
     0 admits everything at once."""
     res = PopulationResult()
     bt = Batch(ctx, opts)
-    if sample_every:
-        bt.sample_spmv(sample_every)
+    if time_spmv:
+        bt.time_spmv(True)
     running: Dict[int, System] = {}   # slot -> system
     ws = 0
     queue = list(systems)

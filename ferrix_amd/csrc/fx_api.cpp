@@ -6,6 +6,8 @@
 #include "fx_internal.h"
 
 #include <chrono>
+#include <climits>
+#include <cstdlib>
 #include <exception>
 #include <new>
 #include <numeric>
@@ -57,6 +59,43 @@ struct Symbolic {
 // Elements are visited in the given order.  Contribution bases point into
 // the element-contribution buffers: ke_base[e] (+ (a*nn+b)*9), fe_base[e]
 // (+ 3a).
+// Sliced-ELL structure (fx_sell.h) for the batched solves of meshes whose
+// node rows are short (mean <= 32 block columns: C3D4): one thread per node
+// row streams the operator with fully coalesced loads, which pays when many
+// systems fill the GPU.  A single solve (fx_solve) keeps the lane-group SpMV
+// on the node-block layout, which has the shorter per-thread dependency
+// chains and wins when one system has to hide its own latency; longer rows
+// (C3D20) use lane groups everywhere.  FX_SELL=0 disables it.
+void build_sell(fx_mesh *m, hipStream_t s) {
+    const char *e = std::getenv("FX_SELL");
+    m->sell = m->n_nodes > 0 && (double)m->nblk <= 32.0 * (double)m->n_nodes && !(e && e[0] == '0');
+    if (!m->sell)
+        return;
+    const int64_t n = m->n_nodes, ns = (n + fx::kSliceRows - 1) / fx::kSliceRows;
+    const std::vector<int32_t> &bp = m->h_nb_ptr, &bc = m->h_nb_col;
+    std::vector<int32_t> slp(ns + 1, 0);
+    for (int64_t sl = 0; sl < ns; ++sl) {
+        int32_t w = 0;
+        for (int64_t a = sl * fx::kSliceRows; a < std::min(n, (sl + 1) * fx::kSliceRows); ++a)
+            w = std::max(w, bp[a + 1] - bp[a]);
+        if ((int64_t)slp[sl] + w > INT32_MAX)
+            throw Error{FX_ERR_INVALID, "mesh too large for the sliced-ELL operator"};
+        slp[sl + 1] = slp[sl] + w;
+    }
+    std::vector<int32_t> sc((size_t)slp[ns] * fx::kSliceRows);
+    for (int64_t sl = 0; sl < ns; ++sl)
+        for (int32_t k = 0; k < slp[sl + 1] - slp[sl]; ++k)
+            for (int lane = 0; lane < fx::kSliceRows; ++lane) {
+                const int64_t a = sl * fx::kSliceRows + lane;
+                int32_t c = 0;
+                if (a < n)
+                    c = k < bp[a + 1] - bp[a] ? bc[bp[a] + k] : (int32_t)a;
+                sc[((size_t)slp[sl] + k) * fx::kSliceRows + lane] = c;
+            }
+    m->sl_ptr.upload(slp, s);
+    m->scol.upload(sc, s);
+}
+
 Symbolic build_symbolic(int64_t n_nodes, int64_t n_el, const int64_t *el_ptr, const int32_t *conn,
                         const std::vector<int64_t> &ke_base, const std::vector<int64_t> &fe_base) {
     Symbolic S;
@@ -351,6 +390,7 @@ fx_status fx_mesh_upload(fx_ctx *ctx, int64_t n_nodes, const double *xyz, const 
     m->blk20.gpos.upload(g20, s);
     m->nb_ptr.upload(S.nb_ptr, s);
     m->nb_col.upload(S.nb_col, s);
+    build_sell(m, s);
     m->blk_cptr.upload(S.blk_cptr, s);
     m->blk_cbase.upload(S.blk_cbase, s);
     m->nod_cptr.upload(S.nod_cptr, s);

@@ -119,6 +119,35 @@ __global__ void k_convert32(int64_t n_dof, const int32_t *__restrict__ nb_ptr, c
     corr[9 * node + 3 * p + 2] = (float)c2;
 }
 
+// Sliced-ELL copy (fx_sell.h) of the node-block values: one 64-thread block
+// per slice, one thread per node row; padding is written as zeros.
+template <typename ST, typename VT>
+__global__ void k_sell_convert(int64_t n_nodes, const int32_t *__restrict__ nb_ptr, const int32_t *__restrict__ sl_ptr,
+                               const ST *__restrict__ src, VT *__restrict__ dst) {
+    const int64_t sl = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t a = sl * kSliceRows + lane;
+    const int k0 = sl_ptr[sl], w = sl_ptr[sl + 1] - k0;
+    int64_t b0 = 0;
+    int deg = 0;
+    if (a < n_nodes) {
+        b0 = nb_ptr[a];
+        deg = nb_ptr[a + 1] - (int)b0;
+    }
+    for (int k = 0; k < w; ++k) {
+        const size_t kk = (size_t)k0 + k;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                VT v = (VT)0;
+                if (k < deg)
+                    v = (VT)src[9 * b0 + (int64_t)p * 3 * deg + 3 * k + q];
+                dst[(kk * 9 + 3 * p + q) * kSliceRows + lane] = v;
+            }
+    }
+}
+
 __global__ void k_residual(int64_t n, const double *__restrict__ fe, const double *__restrict__ fi,
                            double *__restrict__ r) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -240,6 +269,24 @@ void launch_convert32(fx_mesh *m) {
     m->kcorr32.alloc(9 * m->n_nodes);
     hipLaunchKernelGGL(k_convert32, dim3(grid1(m->n_dof, 256)), dim3(256), 0, m->ctx->stream, m->n_dof,
                        m->nb_ptr.p, m->kval.p, m->kval32.p, m->kcorr32.p);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_sell(fx_mesh *m, bool fp32) {
+    if (!m->sell)
+        return;
+    const int64_t ns = (m->n_nodes + kSliceRows - 1) / kSliceRows;
+    const size_t slots = (size_t)m->scol.n * 9;  // block columns x 64 lanes x 9 entries
+    hipStream_t s = m->ctx->stream;
+    if (fp32) {
+        m->sval32.alloc(slots);
+        hipLaunchKernelGGL((k_sell_convert<float, float>), dim3((unsigned)ns), dim3(kSliceRows), 0, s, m->n_nodes,
+                           m->nb_ptr.p, m->sl_ptr.p, m->kval32.p, m->sval32.p);
+    } else {
+        m->sval64.alloc(slots);
+        hipLaunchKernelGGL((k_sell_convert<double, double>), dim3((unsigned)ns), dim3(kSliceRows), 0, s,
+                           m->n_nodes, m->nb_ptr.p, m->sl_ptr.p, m->kval.p, m->sval64.p);
+    }
     FX_HIP(hipGetLastError());
 }
 

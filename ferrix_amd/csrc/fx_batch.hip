@@ -19,6 +19,7 @@
 // Slots move between phases independently, and new slots join between chunks
 // of iterations (continuous batching).
 #include "fx_reduce.h"
+#include "fx_sell.h"
 
 #include <chrono>
 #include <memory>
@@ -40,6 +41,9 @@ struct BSys {
     const void *val;                    // double (FP64 slots) or float (mixed inner)
     const double *val64, *inv;
     const float *kc;                    // FP32 operator: per-node 3x3 row-sum corrections
+    const int32_t *sl_ptr, *scol;       // sliced-ELL operator (fx_sell.h) when sell != 0
+    const void *sval;
+    int32_t sell;
     double *b;                          // inner right-hand sides
     const double *btrue;                // true right-hand sides (mixed)
     double *x, *r, *z, *p0, *p1, *ap;  // inner PCG vectors
@@ -48,6 +52,9 @@ struct BSys {
     int64_t n_nodes, n_dof;
     int64_t it;                         // iteration index of the running right-hand sides
     int32_t R, nparts1, nparts2, want;  // want: right-hand sides of an outer pass (bit mask)
+    int32_t nouter;                     // row groups (partials) of the FP64 outer residual pass
+    double *bytes_acc;                  // instrumentation: algorithmic bytes of the SpMV launches issued
+    double spmv_bytes0, spmv_bytes_rhs; // per launch: operator part, per active right-hand side
     int32_t go;                         // right-hand sides the next SpMV / update work on (bit mask)
     BRhs rhs[kRmax];
 };
@@ -164,40 +171,78 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
     }
     S.it = itv;
     S.go = go;
+    if (go)  // the next SpMV launch works on this system (integer-valued: exact in any order)
+        atomicAdd(S.bytes_acc, S.spmv_bytes0 + (double)__popc(go) * S.spmv_bytes_rhs);
 }
 
-template <int G, typename VT>
-__global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    const BSys &S = sys[t.x];
-    const int go = S.go;
-    if (!go)
-        return;
-    const int lb = t.y, R = S.R;
-    bool act[kRmax];
-    double beta[kRmax];
-#pragma unroll
-    for (int r = 0; r < kRmax; ++r) {
-        act[r] = (go >> r) & 1;
-        beta[r] = act[r] ? S.rhs[r].beta : 0.0;
-    }
-    const int64_t itv = S.it;
-    const double *pold = (itv & 1) ? S.p1 : S.p0;
-    double *pnew = (itv & 1) ? S.p0 : S.p1;
-    const VT *val = (const VT *)S.val;
-
-    const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
-    const int lane = threadIdx.x % G;
-    double acc[3][kRmax];
+// Tail of a batched SpMV row group: p_new = z + beta p_old at the node's
+// DOFs, the FP32 operator's correction block, Ap, and p.Ap block partials
+// (right-hand sides outside `go` are computed but not stored).
+template <typename VT, int RR>
+__device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, const double (&beta)[RR],
+                                           gptr<const double> z, gptr<const double> pold, gptr<double> pnew,
+                                           double (&acc)[3][RR], double (&loc)[kRmax]) {
+    gptr<double> ap = gp(S.ap);
+    double qv[3][RR];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
-        for (int r = 0; r < kRmax; ++r)
+        for (int r = 0; r < RR; ++r) {
+            const int64_t q = (3 * a + p) * RR + r;
+            qv[p][r] = z[q] + beta[r] * pold[q];
+        }
+    if constexpr (sizeof(VT) == 4) {
+        gptr<const float> c = gp(S.kc) + 9 * a;
+        float cc[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            cc[e] = c[e];
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+                acc[p][r] += ((double)cc[3 * p] * qv[0][r] + (double)cc[3 * p + 1] * qv[1][r]) +
+                             (double)cc[3 * p + 2] * qv[2][r];
+    }
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        if (!((go >> r) & 1))
+            continue;
+        double sum = 0.0;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const int64_t q = (3 * a + p) * RR + r;
+            pnew[q] = qv[p][r];
+            ap[q] = acc[p][r];
+            sum += qv[p][r] * acc[p][r];
+        }
+        loc[r] = sum;
+    }
+}
+
+template <int G, typename VT, int RR>
+__device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, double (&loc)[kRmax]) {
+    double beta[RR];
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+        beta[r] = ((go >> r) & 1) ? S.rhs[r].beta : 0.0;
+    const int64_t itv = S.it;
+    gptr<const double> z = gp((const double *)S.z);
+    gptr<const double> pold = gp((const double *)((itv & 1) ? S.p1 : S.p0));
+    gptr<double> pnew = gp((itv & 1) ? S.p0 : S.p1);
+    gptr<const VT> val = gp((const VT *)S.val);
+    gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
+    const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    double acc[3][RR];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
             acc[p][r] = 0.0;
     if (a < S.n_nodes) {
-        const int b0 = S.nb_ptr[a], ncol = 3 * (S.nb_ptr[a + 1] - b0);
-        const VT *v0 = val + 9 * (int64_t)b0;
-        const double *z = S.z;
+        const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
+        gptr<const VT> v0 = val + 9 * (int64_t)b0;
         // two slots per lane in flight; accumulation order j, j+G, j+2G, ...
         // as in a plain strided loop
         for (int j0 = lane; j0 < ncol; j0 += 2 * G) {
@@ -207,7 +252,7 @@ __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const i
             for (int u = 0; u < 2; ++u) {
                 const int j = j0 + u * G;
                 const int kb = j / 3;
-                c[u] = j < ncol ? (3 * S.nb_col[b0 + kb] + (j - 3 * kb)) * R : -1;
+                c[u] = j < ncol ? (3 * nb_col[b0 + kb] + (j - 3 * kb)) * RR : -1;
             }
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -222,14 +267,15 @@ __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const i
             for (int u = 0; u < 2; ++u) {
                 if (c[u] < 0)
                     continue;
+                double xj[RR];
 #pragma unroll
-                for (int r = 0; r < kRmax; ++r) {
-                    if (!act[r])
-                        continue;
-                    const double xj = z[c[u] + r] + beta[r] * pold[c[u] + r];
-                    acc[0][r] += w[u][0] * xj;
-                    acc[1][r] += w[u][1] * xj;
-                    acc[2][r] += w[u][2] * xj;
+                for (int r = 0; r < RR; ++r)
+                    xj[r] = z[c[u] + r] + beta[r] * pold[c[u] + r];
+#pragma unroll
+                for (int r = 0; r < RR; ++r) {
+                    acc[0][r] += w[u][0] * xj[r];
+                    acc[1][r] += w[u][1] * xj[r];
+                    acc[2][r] += w[u][2] * xj[r];
                 }
             }
         }
@@ -239,44 +285,78 @@ __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const i
 #pragma unroll
         for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int r = 0; r < kRmax; ++r)
+            for (int r = 0; r < RR; ++r)
                 acc[p][r] += __shfl_xor(acc[p][r], off, G);
-    double loc[kRmax];
+    if (lane == 0 && a < S.n_nodes)
+        bspmv_tail<VT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
+}
+
+template <typename VT, int RR>
+__device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double (&loc)[kRmax]) {
+    double beta[RR];
 #pragma unroll
-    for (int r = 0; r < kRmax; ++r)
-        loc[r] = 0.0;
-    if (lane == 0 && a < S.n_nodes) {
+    for (int r = 0; r < RR; ++r)
+        beta[r] = ((go >> r) & 1) ? S.rhs[r].beta : 0.0;
+    const int64_t itv = S.it;
+    gptr<const double> z = gp((const double *)S.z);
+    gptr<const double> pold = gp((const double *)((itv & 1) ? S.p1 : S.p0));
+    gptr<double> pnew = gp((itv & 1) ? S.p0 : S.p1);
+    const int64_t a = (int64_t)lb * kBB + threadIdx.x;
+    double acc[3][RR];
 #pragma unroll
-        for (int r = 0; r < kRmax; ++r) {
-            if (!act[r])
-                continue;
-            double qv[3];
+    for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                const int64_t q = (3 * a + p) * R + r;
-                qv[p] = S.z[q] + beta[r] * pold[q];
-                pnew[q] = qv[p];
-            }
-            if constexpr (sizeof(VT) == 4) {
-                const float *c = S.kc + 9 * a;
-#pragma unroll
-                for (int p = 0; p < 3; ++p)
-                    acc[p][r] += ((double)c[3 * p] * qv[0] + (double)c[3 * p + 1] * qv[1]) +
-                                 (double)c[3 * p + 2] * qv[2];
-            }
-            double s = 0.0;
-#pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                S.ap[(3 * a + p) * R + r] = acc[p][r];
-                s += qv[p] * acc[p][r];
-            }
-            loc[r] = s;
-        }
+        for (int r = 0; r < RR; ++r)
+            acc[p][r] = 0.0;
+    if (a < S.n_nodes) {
+        sell_node_rows<VT, RR>(gp(S.nb_ptr), gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold, beta,
+                               acc);
+        bspmv_tail<VT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
+    }
+}
+
+// Batched SpMV + p update for every (system, row group) of the table: lane
+// groups on the node-block layout (kb_spmv_p) or one thread per node row on
+// the sliced-ELL operator (kb_spmv_sell); the right-hand side count of the
+// block's system selects a straight-line instance.
+template <int G, typename VT>
+__global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    double loc[kRmax] = {0.0, 0.0, 0.0, 0.0};
+    switch (S.R) {
+    case 1: bspmv_lanes<G, VT, 1>(S, t.y, go, loc); break;
+    case 2: bspmv_lanes<G, VT, 2>(S, t.y, go, loc); break;
+    case 3: bspmv_lanes<G, VT, 3>(S, t.y, go, loc); break;
+    default: bspmv_lanes<G, VT, 4>(S, t.y, go, loc); break;
     }
     block_sum<kRmax>(loc);
     if (threadIdx.x == 0)
-        for (int r = 0; r < R; ++r)
-            S.part1[(size_t)lb * R + r] = loc[r];
+        for (int r = 0; r < S.R; ++r)
+            S.part1[(size_t)t.y * S.R + r] = loc[r];
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBB) void kb_spmv_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    double loc[kRmax] = {0.0, 0.0, 0.0, 0.0};
+    switch (S.R) {
+    case 1: bspmv_sell<VT, 1>(S, t.y, go, loc); break;
+    case 2: bspmv_sell<VT, 2>(S, t.y, go, loc); break;
+    case 3: bspmv_sell<VT, 3>(S, t.y, go, loc); break;
+    default: bspmv_sell<VT, 4>(S, t.y, go, loc); break;
+    }
+    block_sum<kRmax>(loc);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < S.R; ++r)
+            S.part1[(size_t)t.y * S.R + r] = loc[r];
 }
 
 // One block per listed system, after kb_spmv_p: p.Ap, alpha, breakdown
@@ -310,46 +390,66 @@ __global__ __launch_bounds__(kBB) void kb_red1(BSys *__restrict__ sys, const int
     S.go = go2;
 }
 
+template <int RR>
+__device__ __forceinline__ void bupdate(const BSys &S, int lb, int go, double (&v)[2 * kRmax]) {
+    double alpha[RR];
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+        alpha[r] = ((go >> r) & 1) ? S.rhs[r].alpha : 0.0;
+    gptr<const double> p = gp((const double *)((S.it & 1) ? S.p0 : S.p1));  // p_i: the "new" buffer of iteration i
+    gptr<double> x = gp(S.x), rv = gp(S.r), z = gp(S.z);
+    gptr<const double> ap = gp((const double *)S.ap);
+    const int64_t i = (int64_t)lb * kBB + threadIdx.x;
+    if (i >= S.n_dof)
+        return;
+    const double inv = gp(S.inv)[i];
+    double xo[RR], ro[RR], po[RR], apo[RR];
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        const int64_t q = i * RR + r;
+        xo[r] = x[q];
+        ro[r] = rv[q];
+        po[r] = p[q];
+        apo[r] = ap[q];
+    }
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        if (!((go >> r) & 1))
+            continue;
+        const int64_t q = i * RR + r;
+        const double xi = xo[r] + alpha[r] * po[r];
+        const double ri = ro[r] - alpha[r] * apo[r];
+        const double zi = ri * inv;
+        x[q] = xi;
+        rv[q] = ri;
+        z[q] = zi;
+        v[2 * r] = ri * zi;
+        v[2 * r + 1] = ri * ri;
+    }
+}
+
 __global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
     const int go = S.go;
     if (!go)
         return;
-    const int lb = t.y, R = S.R;
-    double alpha[kRmax];
-#pragma unroll
-    for (int r = 0; r < kRmax; ++r)
-        alpha[r] = ((go >> r) & 1) ? S.rhs[r].alpha : 0.0;
-    const double *p = (S.it & 1) ? S.p0 : S.p1;  // p_i sits in the "new" buffer of iteration i
     double v[2 * kRmax];
 #pragma unroll
     for (int k = 0; k < 2 * kRmax; ++k)
         v[k] = 0.0;
-    const int64_t i = (int64_t)lb * kBB + threadIdx.x;
-    if (i < S.n_dof) {
-        const double inv = S.inv[i];
-#pragma unroll
-        for (int r = 0; r < kRmax; ++r) {
-            if (!((go >> r) & 1))
-                continue;
-            const int64_t q = i * R + r;
-            const double xi = S.x[q] + alpha[r] * p[q];
-            const double ri = S.r[q] - alpha[r] * S.ap[q];
-            const double zi = ri * inv;
-            S.x[q] = xi;
-            S.r[q] = ri;
-            S.z[q] = zi;
-            v[2 * r] = ri * zi;
-            v[2 * r + 1] = ri * ri;
-        }
+    switch (S.R) {
+    case 1: bupdate<1>(S, t.y, go, v); break;
+    case 2: bupdate<2>(S, t.y, go, v); break;
+    case 3: bupdate<3>(S, t.y, go, v); break;
+    default: bupdate<4>(S, t.y, go, v); break;
     }
     block_sum<2 * kRmax>(v);
     if (threadIdx.x == 0)
-        for (int r = 0; r < R; ++r) {
-            S.part2[((size_t)lb * R + r) * 3] = v[2 * r];
-            S.part2[((size_t)lb * R + r) * 3 + 1] = v[2 * r + 1];
-            S.part2[((size_t)lb * R + r) * 3 + 2] = 0.0;
+        for (int r = 0; r < S.R; ++r) {
+            S.part2[((size_t)t.y * S.R + r) * 3] = v[2 * r];
+            S.part2[((size_t)t.y * S.R + r) * 3 + 1] = v[2 * r + 1];
+            S.part2[((size_t)t.y * S.R + r) * 3 + 2] = 0.0;
         }
 }
 
@@ -442,7 +542,7 @@ __global__ __launch_bounds__(kBB) void kb_outer_sum(BSys *__restrict__ sys, cons
                                                     double *__restrict__ out) {
     BSys &S = sys[list[blockIdx.x]];
     double red[3 * kRmax];
-    bsum_partials<3 * kRmax>(S.part1, S.nparts1, 3 * S.R, red);
+    bsum_partials<3 * kRmax>(S.part1, S.nouter, 3 * S.R, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < 3 * kRmax; ++k)
             out[(size_t)blockIdx.x * 3 * kRmax + k] = k < 3 * S.R ? red[k] : 0.0;
@@ -502,21 +602,22 @@ struct fx_batch {
     std::vector<std::unique_ptr<Slot>> slots;
     std::vector<fx::BSys> h_sys;
     fx::DBuf<fx::BSys> d_sys;
-    fx::DBuf<int2> d_tab1, d_tab2, d_tab3, d_tab4;
+    fx::DBuf<int2> d_tab1, d_tab1s, d_tab2, d_tab3, d_tab4;
     fx::DBuf<int32_t> d_list, d_list2, d_run;
     fx::DBuf<double> d_out;
     std::vector<int2> h_tab_start;
-    int n1 = 0, n2 = 0, nrun = 0;
+    int n1 = 0, n1s = 0, n2 = 0, nrun = 0;
     bool tables_dirty = true;
     int64_t resident_bytes = 0;
-    // SpMV launch sampling (instrumentation): the first SpMV of every
-    // sample_every-th chunk is bracketed by two HIP events on the stream
-    int sample_every = 0;
-    int64_t chunks = 0, spmv_launches = 0, samples = 0;
-    double sampled_ms = 0.0, sampled_bytes = 0.0;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    bool ev_pending = false;
-    double ev_bytes = 0.0;
+    // SpMV timing (instrumentation): when on, every SpMV launch is bracketed
+    // by two HIP events on the stream; the device counter sums the launches'
+    // algorithmic bytes (kb_red2)
+    bool timing = false;
+    int64_t spmv_launches = 0, timed = 0;
+    double timed_ms = 0.0;
+    std::vector<hipEvent_t> ev;  // 2 per iteration of a chunk
+    int ev_used = 0;
+    fx::DBuf<double> d_bytes;
     ~fx_batch() {
         for (auto &e : ev)
             if (e)
@@ -564,6 +665,10 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.val = is_mixed(B) ? (const void *)m->kval32.p : (const void *)m->kval.p;
     S.val64 = m->kval.p;
     S.kc = is_mixed(B) ? m->kcorr32.p : nullptr;
+    S.sell = m->sell ? 1 : 0;
+    S.sl_ptr = m->sl_ptr.p;
+    S.scol = m->scol.p;
+    S.sval = is_mixed(B) ? (const void *)m->sval32.p : (const void *)m->sval64.p;
     S.inv = m->inv_diag.p;
     S.b = is_mixed(B) ? sl.rin.p : sl.b.p;
     S.btrue = sl.b.p;
@@ -579,56 +684,70 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.n_nodes = m->n_nodes;
     S.n_dof = m->n_dof;
     S.R = sl.R;
-    S.nparts1 = fx::nblk(m->n_nodes, kBB / B->G);
+    S.nparts1 = m->sell ? fx::nblk(m->n_nodes, kBB) : fx::nblk(m->n_nodes, kBB / B->G);
+    S.nouter = fx::nblk(m->n_nodes, kBB / B->G);  // kb_outer: lane groups on the FP64 node-block values
+    // SpMV algorithmic bytes: values in the working precision, one column
+    // index per 3x3 block, node-row pointers (+ the FP32 operator's 3x3
+    // correction per node); per right-hand side z and p_old gathered once
+    // and p_new, Ap written
+    S.bytes_acc = B->d_bytes.p;
+    S.spmv_bytes0 = (is_mixed(B) ? 4.0 : 8.0) * (double)m->nnz_struct + 4.0 * (double)m->nblk +
+                    4.0 * (double)(m->n_nodes + 1) + (is_mixed(B) ? 36.0 * (double)m->n_nodes : 0.0);
+    S.spmv_bytes_rhs = 32.0 * (double)m->n_dof;
     S.nparts2 = fx::nblk(m->n_dof, kBB);
     S.want = 0;
 }
 
-// block tables of the running slots: tab1 SpMV blocks, tab2 vector blocks
+// workgroups are dispatched round-robin over the 8 XCDs: permute an SpMV
+// table so that each XCD sweeps a contiguous range of row groups
+void xcd_permute(std::vector<int2> &t) {
+    const unsigned nb = (unsigned)t.size(), q = nb / 8, rr = nb % 8;
+    std::vector<int2> perm(nb);
+    for (unsigned id = 0; id < nb; ++id) {
+        const unsigned x = id % 8, i = id / 8;
+        perm[id] = t[x * q + (x < rr ? x : rr) + i];
+    }
+    t.swap(perm);
+}
+
+void upload_table(fx::DBuf<int2> &d, const std::vector<int2> &t, hipStream_t s) {
+    if (t.empty())
+        return;
+    d.alloc(std::max<size_t>(d.n, t.size()));
+    FX_HIP(hipMemcpyAsync(d.p, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+}
+
+// block tables of the running slots: tab1 / tab1s SpMV blocks (lane-group
+// and sliced-ELL operators), tab2 vector blocks, run the running slots
 void build_tables(fx_batch *B) {
-    std::vector<int2> t1, t2;
+    std::vector<int2> t1, t1s, t2;
+    std::vector<int> run;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         auto &sl = *B->slots[k];
         if (!sl.used || sl.phase != 0)
             continue;
+        run.push_back((int)k);
         const BSys &S = B->h_sys[k];
         for (int b = 0; b < S.nparts1; ++b)
-            t1.push_back(make_int2((int)k, b));
+            (S.sell ? t1s : t1).push_back(make_int2((int)k, b));
         for (int b = 0; b < S.nparts2; ++b)
             t2.push_back(make_int2((int)k, b));
     }
-    // workgroups are dispatched round-robin over the 8 XCDs: permute the
-    // SpMV table so that each XCD sweeps a contiguous range of row groups
-    {
-        const unsigned nb = (unsigned)t1.size(), q = nb / 8, rr = nb % 8;
-        std::vector<int2> perm(nb);
-        for (unsigned id = 0; id < nb; ++id) {
-            const unsigned x = id % 8, i = id / 8;
-            perm[id] = t1[x * q + (x < rr ? x : rr) + i];
-        }
-        t1.swap(perm);
-    }
+    xcd_permute(t1);
+    xcd_permute(t1s);
     B->n1 = (int)t1.size();
+    B->n1s = (int)t1s.size();
     B->n2 = (int)t2.size();
-    std::vector<int> run;
-    for (size_t k = 0; k < B->slots.size(); ++k)
-        if (B->slots[k]->used && B->slots[k]->phase == 0)
-            run.push_back((int)k);
     B->nrun = (int)run.size();
     hipStream_t s = B->ctx->stream;
     if (B->nrun) {
         B->d_run.alloc(std::max<size_t>(B->d_run.n, run.size()));
         FX_HIP(hipMemcpyAsync(B->d_run.p, run.data(), run.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        FX_HIP(hipStreamSynchronize(s));
     }
-    if (B->n1) {
-        B->d_tab1.alloc(std::max<size_t>(B->d_tab1.n, t1.size()));
-        FX_HIP(hipMemcpyAsync(B->d_tab1.p, t1.data(), t1.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-    }
-    if (B->n2) {
-        B->d_tab2.alloc(std::max<size_t>(B->d_tab2.n, t2.size()));
-        FX_HIP(hipMemcpyAsync(B->d_tab2.p, t2.data(), t2.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-    }
+    upload_table(B->d_tab1, t1, s);
+    upload_table(B->d_tab1s, t1s, s);
+    upload_table(B->d_tab2, t2, s);
+    FX_HIP(hipStreamSynchronize(s));  // the host tables are temporaries
     B->tables_dirty = false;
 }
 
@@ -661,30 +780,6 @@ void download_sys(fx_batch *B) {
     FX_HIP(hipStreamSynchronize(B->ctx->stream));
 }
 
-// Algorithmic bytes of one SpMV launch over the running slots, from the
-// host copy of their states: K values (FP32 or FP64), one column index per
-// 3x3 block, node-row pointers, and per active right-hand side z and p_old
-// gathered (8 B each per DOF at least once) plus p_new and Ap written.
-double spmv_bytes(const fx_batch *B) {
-    const double vs = is_mixed(B) ? 4.0 : 8.0;
-    double by = 0.0;
-    for (size_t k = 0; k < B->slots.size(); ++k) {
-        const auto &sl = *B->slots[k];
-        if (!sl.used || sl.phase != 0)
-            continue;
-        const BSys &S = B->h_sys[k];
-        int act = 0;
-        for (int r = 0; r < S.R; ++r)
-            act += S.rhs[r].status == 0;
-        if (!act)
-            continue;
-        const fx_mesh *m = sl.mesh;
-        by += vs * (double)m->nnz_struct + 4.0 * (double)m->nblk + 4.0 * (double)(m->n_nodes + 1) +
-              32.0 * (double)act * (double)m->n_dof + (is_mixed(B) ? 36.0 * (double)m->n_nodes : 0.0);
-    }
-    return by;
-}
-
 // kb_init over the vector blocks of `slots`, then kb_red2 on them: their
 // inner solves start with the next iteration chunk.
 void start_slots(fx_batch *B, const std::vector<int> &slots) {
@@ -708,43 +803,48 @@ void start_slots(fx_batch *B, const std::vector<int> &slots) {
 
 void launch_iterations(fx_batch *B, int n) {
     hipStream_t s = B->ctx->stream;
-    const bool sample = B->sample_every > 0 && n > 0 && B->chunks % B->sample_every == 0;
-    B->chunks++;
-    if (sample) {
-        for (auto &e : B->ev)
-            if (!e)
-                FX_HIP(hipEventCreate(&e));
-        B->ev_bytes = spmv_bytes(B);
+    if (B->timing && (int)B->ev.size() < 2 * n) {
+        const size_t old = B->ev.size();
+        B->ev.resize(2 * n, nullptr);
+        for (size_t k = old; k < B->ev.size(); ++k)
+            FX_HIP(hipEventCreate(&B->ev[k]));
     }
     for (int k = 0; k < n; ++k) {
-        if (sample && k == 0)
-            FX_HIP(hipEventRecord(B->ev[0], s));
-        if (is_mixed(B))
-            fx::launch_bspmv<float>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
-        else
-            fx::launch_bspmv<double>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
-        if (sample && k == 0) {
-            FX_HIP(hipEventRecord(B->ev[1], s));
-            B->ev_pending = true;
+        if (B->timing)
+            FX_HIP(hipEventRecord(B->ev[2 * k], s));
+        if (is_mixed(B)) {
+            if (B->n1)
+                fx::launch_bspmv<float>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
+            if (B->n1s)
+                hipLaunchKernelGGL(fx::kb_spmv_sell<float>, dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p, B->d_tab1s.p);
+        } else {
+            if (B->n1)
+                fx::launch_bspmv<double>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
+            if (B->n1s)
+                hipLaunchKernelGGL(fx::kb_spmv_sell<double>, dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p,
+                                   B->d_tab1s.p);
         }
+        if (B->timing)
+            FX_HIP(hipEventRecord(B->ev[2 * k + 1], s));
         hipLaunchKernelGGL(fx::kb_red1, dim3(B->nrun), dim3(kBB), 0, s, B->d_sys.p, B->d_run.p);
         hipLaunchKernelGGL(fx::kb_update, dim3(B->n2), dim3(kBB), 0, s, B->d_sys.p, B->d_tab2.p);
         hipLaunchKernelGGL(fx::kb_red2, dim3(B->nrun), dim3(kBB), 0, s, B->d_sys.p, B->d_run.p);
     }
+    if (B->timing)
+        B->ev_used = 2 * n;
     B->spmv_launches += n;
     FX_HIP(hipGetLastError());
 }
 
 // after the stream has been synchronized
 void collect_sample(fx_batch *B) {
-    if (!B->ev_pending)
-        return;
-    float ms = 0.f;
-    FX_HIP(hipEventElapsedTime(&ms, B->ev[0], B->ev[1]));
-    B->sampled_ms += ms;
-    B->sampled_bytes += B->ev_bytes;
-    B->samples++;
-    B->ev_pending = false;
+    for (int k = 0; k + 1 < B->ev_used; k += 2) {
+        float ms = 0.f;
+        FX_HIP(hipEventElapsedTime(&ms, B->ev[k], B->ev[k + 1]));
+        B->timed_ms += ms;
+        B->timed++;
+    }
+    B->ev_used = 0;
 }
 
 // After a chunk: per slot, see whether its running inner solves finished;
@@ -806,7 +906,7 @@ bool advance_phases(fx_batch *B) {
         const BSys &S = B->h_sys[k];
         for (int b = 0; b < S.nparts2; ++b)
             t4.push_back(make_int2(k, b));
-        for (int b = 0; b < S.nparts1; ++b)
+        for (int b = 0; b < S.nouter; ++b)
             t3.push_back(make_int2(k, b));
     }
     upload_sys(B);  // want masks
@@ -896,8 +996,10 @@ extern "C" {
 fx_status fx_batch_create(fx_ctx *ctx, const fx_solve_opts *opts, fx_batch **out) {
     if (!ctx || !out)
         return FX_ERR_INVALID;
+    fx_batch *B = nullptr;
     try {
-        auto *B = new fx_batch();
+        FX_HIP(hipSetDevice(ctx->device));
+        B = new fx_batch();
         B->ctx = ctx;
         if (opts)
             B->opts = *opts;
@@ -916,9 +1018,17 @@ fx_status fx_batch_create(fx_ctx *ctx, const fx_solve_opts *opts, fx_batch **out
         }
         // 0: chosen from the first system admitted (as fx_solve chooses it)
         B->G = g;
+        B->d_bytes.alloc(1);
+        FX_HIP(hipMemsetAsync(B->d_bytes.p, 0, sizeof(double), ctx->stream));
+        FX_HIP(hipStreamSynchronize(ctx->stream));
         ctx->batches.push_back(B);
         *out = B;
+    } catch (const fx::Error &e) {
+        delete B;
+        ctx->err = e.msg;
+        return e.code;
     } catch (const std::exception &e) {
+        delete B;
         ctx->err = e.what();
         return FX_ERR_OOM;
     }
@@ -980,9 +1090,10 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
             fx::launch_fill(m, sl.xacc.p, nv, 0.0);
             fx::launch_convert32(m);  // K may have been reassembled since the last conversion
         }
+        fx::launch_sell(m, is_mixed(B));
         fx::launch_jacobi(m, false);
         const int G = B->G;
-        sl.part1.alloc((size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 3);
+        sl.part1.alloc((size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 3);  // >= SpMV and outer groups
         sl.part2.alloc((size_t)std::max(fx::nblk(n, kBB), 1) * n_rhs * 3);
         sl.budget = B->opts.max_iter;
         sl.used = true;
@@ -1022,7 +1133,7 @@ fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
         for (;;) {
             if (B->tables_dirty)
                 build_tables(B);
-            if (B->n1 == 0)
+            if (B->nrun == 0)
                 break;
             int chunk = 16;
             if (max_iters > 0)
@@ -1092,25 +1203,33 @@ fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
     return FX_OK;
 }
 
-fx_status fx_batch_sample_spmv(fx_batch *B, int32_t every) {
-    if (!B || every < 0)
+fx_status fx_batch_time_spmv(fx_batch *B, int32_t on) {
+    if (!B)
         return FX_ERR_INVALID;
-    B->sample_every = every;
+    B->timing = on != 0;
     return FX_OK;
 }
 
-fx_status fx_batch_spmv_stats(fx_batch *B, int64_t *launches, int64_t *samples, double *ms_sampled,
-                              double *bytes_sampled) {
+fx_status fx_batch_spmv_stats(fx_batch *B, int64_t *launches, int64_t *timed, double *ms_timed,
+                              double *bytes_total) {
     if (!B)
         return FX_ERR_INVALID;
+    double by = 0.0;
+    try {
+        FX_HIP(hipMemcpyAsync(&by, B->d_bytes.p, sizeof(double), hipMemcpyDeviceToHost, B->ctx->stream));
+        FX_HIP(hipStreamSynchronize(B->ctx->stream));
+    } catch (const fx::Error &e) {
+        B->ctx->err = e.msg;
+        return e.code;
+    }
     if (launches)
         *launches = B->spmv_launches;
-    if (samples)
-        *samples = B->samples;
-    if (ms_sampled)
-        *ms_sampled = B->sampled_ms;
-    if (bytes_sampled)
-        *bytes_sampled = B->sampled_bytes;
+    if (timed)
+        *timed = B->timed;
+    if (ms_timed)
+        *ms_timed = B->timed_ms;
+    if (bytes_total)
+        *bytes_total = by;
     return FX_OK;
 }
 

@@ -15,6 +15,10 @@
 struct fx_mesh;
 
 namespace fx {
+constexpr int kSliceRows = 64;  // node rows per slice of the sliced-ELL operator (one wavefront)
+}
+
+namespace fx {
 
 // ---------------------------------------------------------------------------
 // errors
@@ -167,6 +171,12 @@ struct fx_mesh {
     fx::DBuf<double> kval;               // nnz_struct values (super-pattern)
     fx::DBuf<float> kval32;              // FP32 operator of the mixed solve ...
     fx::DBuf<float> kcorr32;             // ... and its per-node 3x3 row-sum corrections
+    // sliced-ELL copy of the operator for the solve (meshes with short node
+    // rows, see fx_sell.h): slice pointers, column nodes, values
+    bool sell = false;
+    fx::DBuf<int32_t> sl_ptr, scol;
+    fx::DBuf<double> sval64;
+    fx::DBuf<float> sval32;
     fx::DBuf<uint8_t> keep;              // 1 if the slot survives the drop rule
     fx::DBuf<double> inv_diag;
     fx::DBuf<float> inv_diag32;
@@ -196,6 +206,7 @@ void launch_assemble(fx_mesh *m);
 void launch_gather_f(fx_mesh *m);
 void launch_jacobi(fx_mesh *m, bool want32);
 void launch_convert32(fx_mesh *m);
+void launch_sell(fx_mesh *m, bool fp32);  // sliced-ELL copy of kval (FP64) or kval32 (FP32)
 void check_status(fx_mesh *m, const char *what);
 void reset_status(fx_mesh *m);
 double device_max_diag(fx_mesh *m);

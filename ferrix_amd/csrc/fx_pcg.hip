@@ -94,20 +94,17 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t n, const double *__rest
     }
 }
 
-template <int G, typename VT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_spmv_p(int64_t n_nodes, const int32_t *__restrict__ nb_ptr,
-                                                   const int32_t *__restrict__ nb_col, const VT *__restrict__ val,
-                                                   const double *__restrict__ z, double *__restrict__ pa,
-                                                   double *__restrict__ pb, double *__restrict__ ap,
-                                                   PcgScalars *__restrict__ sc, const double *__restrict__ part2,
-                                                   int nparts2, double *__restrict__ part1,
-                                                   const float *__restrict__ kc) {
+// Head of an SpMV launch: the block-uniform PCG status, r.z and r.r (and
+// b.b at iteration 0) from the previous update's partials, the stop rule
+// (block 0 records it), beta.  Returns false when the solve has stopped.
+__device__ __forceinline__ bool pcg_head(PcgScalars *__restrict__ sc, const double *__restrict__ part2, int nparts2,
+                                         int64_t &it_out, double &beta_out) {
     __shared__ int32_t s_status;
     if (threadIdx.x == 0)
         s_status = sc->status;
     __syncthreads();  // one block-wide view (block 0 may update it below)
     if (s_status != 0)
-        return;
+        return false;
     const int64_t it = sc->it;
     // r.z, r.r (and b.b at i = 0) after the previous update
     double red[3];
@@ -137,8 +134,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
             sc->status = 1;
     }
     if (stop_max || conv)
-        return;
+        return false;
     const double beta = it == 0 ? 0.0 : rz / sc->rz[(it - 1) & 1];
+    it_out = it;
+    beta_out = beta;
+    return true;
+}
+
+template <int G, typename VT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_spmv_p(int64_t n_nodes, const int32_t *__restrict__ nb_ptr,
+                                                   const int32_t *__restrict__ nb_col, const VT *__restrict__ val,
+                                                   const double *__restrict__ z, double *__restrict__ pa,
+                                                   double *__restrict__ pb, double *__restrict__ ap,
+                                                   PcgScalars *__restrict__ sc, const double *__restrict__ part2,
+                                                   int nparts2, double *__restrict__ part1,
+                                                   const float *__restrict__ kc) {
+    int64_t it;
+    double beta;
+    if (!pcg_head(sc, part2, nparts2, it, beta))
+        return;
     const double *pold = (it & 1) ? pb : pa;
     double *pnew = (it & 1) ? pa : pb;
 
@@ -349,6 +363,9 @@ void launch_spmv_p(fx_mesh *m, const VT *val, hipStream_t s) {
                        part1_of(m), sizeof(VT) == 4 ? m->kcorr32.p : nullptr);
 }
 
+// row groups of the SpMV (= its partial count)
+inline int64_t spmv_groups(const fx_mesh *m, int G) { return nblocks(m->n_nodes, kBlock / G); }
+
 template <typename VT>
 void launch_spmv_any(fx_mesh *m, const VT *val, int G, hipStream_t s) {
     switch (G) {
@@ -366,7 +383,7 @@ void launch_iteration(fx_mesh *m, const VT *val, int G, hipStream_t s) {
     launch_spmv_any<VT>(m, val, G, s);
     hipLaunchKernelGGL(k_update, dim3(nblocks(m->n_dof, kBlock)), dim3(kBlock), 0, s, m->n_dof, m->x.p, m->r.p,
                        m->z.p, m->p0.p, m->p1.p, m->ap.p, m->inv_diag.p, m->pcg.p, part1_of(m),
-                       (int)nblocks(m->n_nodes, kBlock / G), part2_of(m));
+                       (int)spmv_groups(m, G), part2_of(m));
 }
 
 template <int G>

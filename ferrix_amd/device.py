@@ -278,12 +278,12 @@ class Batch:
         self.ctx.check(self.lib.fx_batch_remove(self.h, slot))
         self._n.pop(slot, None)
 
-    def sample_spmv(self, every: int):
-        """Time the first SpMV launch of every `every`-th chunk (0 = off)."""
-        self.ctx.check(self.lib.fx_batch_sample_spmv(self.h, every))
+    def time_spmv(self, on: bool = True):
+        """Bracket every later SpMV launch with HIP events."""
+        self.ctx.check(self.lib.fx_batch_time_spmv(self.h, int(on)))
 
     def spmv_stats(self):
-        """(launches, samples, sampled ms, sampled algorithmic bytes)."""
+        """(launches, timed launches, timed ms, algorithmic bytes of all launches)."""
         a, b = C.c_int64(), C.c_int64()
         c, d = C.c_double(), C.c_double()
         self.ctx.check(self.lib.fx_batch_spmv_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))

@@ -256,9 +256,12 @@ fx_status fx_profile_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_
  * (solvers/mod.rs:18-44, static_step.rs:200-228).  A batch replaces N such
  * calls on N resident meshes -- each with 1..4 load cases sharing its K --
  * by one set of kernel launches over all of them.  Per load case the
- * arithmetic, stopping rule and result are those of fx_solve with the same
- * options: for FX_PREC_FP64 a batched solution is bitwise equal to fx_solve's,
- * whatever else shares the batch.  Systems may join while others run
+ * recurrences and stopping rule are those of fx_solve with the same options.
+ * Meshes with short node rows (C3D4) run a sliced-ELL SpMV in a batch, whose
+ * row sums associate differently from fx_solve's lane-group SpMV: results
+ * agree to rounding.  Meshes on the lane-group SpMV (C3D20, or uploaded with
+ * FX_SELL=0) give FP64 solutions bitwise equal to fx_solve's.  Either way a
+ * system's result does not depend on what else shares the batch.  Systems may join while others run
  * (continuous batching); a mesh must not be reassembled while it is in a
  * batch.  All calls run on the context stream.
  */
@@ -291,16 +294,19 @@ fx_status fx_batch_get(fx_batch *batch, int32_t slot, double *x);
 /* Frees the slot for reuse (a running system is dropped). */
 fx_status fx_batch_remove(fx_batch *batch, int32_t slot);
 
-/* Instrumentation (no counterpart in the reference): from now on, bracket
- * the first SpMV launch of every `every`-th iteration chunk with two HIP
- * events on the context stream (0 = off). */
-fx_status fx_batch_sample_spmv(fx_batch *batch, int32_t every);
+/* Instrumentation (no counterpart in the reference): on != 0 brackets every
+ * SpMV launch of later fx_batch_run calls with two HIP events on the
+ * context stream. */
+fx_status fx_batch_time_spmv(fx_batch *batch, int32_t on);
 
-/* SpMV launches issued so far; sampled launches, their summed duration (ms)
- * and summed algorithmic bytes (as for fx_profile_spmv, per running system,
- * with the right-hand sides still active at the chunk start). */
-fx_status fx_batch_spmv_stats(fx_batch *batch, int64_t *launches, int64_t *samples, double *ms_sampled,
-                              double *bytes_sampled);
+/* SpMV launches issued so far, launches timed and their summed duration
+ * (ms), and the summed algorithmic bytes of all launches: per system whose
+ * load cases are running, the operator values in the working precision, one
+ * 4-byte column index per 3x3 block, node-row pointers (and the FP32
+ * operator's 3x3 correction per node), plus 32 bytes per DOF and running
+ * load case (z and p_old gathered once, p_new and Ap written). */
+fx_status fx_batch_spmv_stats(fx_batch *batch, int64_t *launches, int64_t *timed, double *ms_timed,
+                              double *bytes_total);
 
 #ifdef __cplusplus
 }

@@ -2,14 +2,19 @@
 advanced by one set of kernel launches.
 
 Bars:
-* FP64: every batched load case is BIT FOR BIT the fx_solve result of the
-  same system and options (same lane group), alone or in any mix, admitted
-  at any time;
+* FP64 on the lane-group operator (the single-solve SpMV): every batched
+  load case is BIT FOR BIT the fx_solve result of the same system and
+  options (same lane group);
+* FP64 on the sliced-ELL operator (short-row meshes in a batch): within
+  1e-9 relative of fx_solve (different summation order in the SpMV rows);
+* any precision: bit for bit alone vs in a mix, admitted at any time;
 * mixed precision: bit for bit alone vs in a mix (a slot's arithmetic does
   not depend on its neighbours), and within 1e-8 relative of the direct
   solve of the oracle's matrix;
 * the reference's stop rule and error behaviour (not converged) per slot.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -52,8 +57,18 @@ def system(ctx, jobs):
     return dm, np.array(out)
 
 
-def population(ctx, specs):
-    return [system(ctx, meshgen.cfg4_mesh(s)) for s in specs]
+def population(ctx, specs, sell=True):
+    """Resident systems; sell=False uploads them without the sliced-ELL
+    operator (FX_SELL=0), so batches run the single-solve SpMV."""
+    old = os.environ.get("FX_SELL")
+    os.environ["FX_SELL"] = "1" if sell else "0"
+    try:
+        return [system(ctx, meshgen.cfg4_mesh(s)) for s in specs]
+    finally:
+        if old is None:
+            del os.environ["FX_SELL"]
+        else:
+            os.environ["FX_SELL"] = old
 
 
 def small_specs(n, seed=11):
@@ -64,7 +79,7 @@ REL = SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000, spmv
 
 
 def test_fp64_batch_equals_solve_bitwise(ctx):
-    sys = population(ctx, small_specs(6))
+    sys = population(ctx, small_specs(6), sell=False)
     bt = Batch(ctx, REL)
     slots = [bt.add(dm, b) for dm, b in sys]
     assert bt.run() == 0
@@ -81,7 +96,7 @@ def test_fp64_batch_equals_solve_bitwise(ctx):
 def test_fp64_reference_rule_batch(ctx):
     """Default options: sqrt(r.z) < 1e-8, 1000 iterations (the reference's
     PCG); the slot's iteration count is its slowest load case's."""
-    sys = population(ctx, small_specs(3, seed=5))
+    sys = population(ctx, small_specs(3, seed=5), sell=False)
     bt = Batch(ctx, SolveOptions(spmv_group=8))
     slots = [bt.add(dm, b) for dm, b in sys]
     bt.run()
@@ -97,6 +112,24 @@ def test_fp64_reference_rule_batch(ctx):
             its.append(rs.iterations)
             assert np.array_equal(x[r], xs)
         assert rep.iterations == max(its)
+
+
+def test_fp64_batch_sell_matches_solve(ctx):
+    sys = population(ctx, small_specs(6, seed=13), sell=True)
+    bt = Batch(ctx, REL)
+    slots = [bt.add(dm, b) for dm, b in sys]
+    assert bt.run() == 0
+    for (dm, b), s in zip(sys, slots):
+        x = bt.result(s)
+        _, rep = bt.poll(s)
+        its = []
+        for r in range(b.shape[0]):
+            xs, rs = dm.solve(b[r], REL)
+            its.append(rs.iterations)
+            err = np.linalg.norm(x[r] - xs) / np.linalg.norm(xs)
+            say("slot", s, "rhs", r, "n_dof", dm.n_dof, "rel diff", err)
+            assert err < 1e-9
+        assert abs(rep.iterations - max(its)) <= max(2, max(its) // 50)
 
 
 def test_continuous_admission_and_removal(ctx):
