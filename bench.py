@@ -416,10 +416,72 @@ def run_batch(a):
     print(json.dumps(out), flush=True)
 
 
+def run_serve(a):
+    """Serving stream: Poisson arrivals of linear C3D4, NLGEOM C3D4 and C3D20
+    block requests (70/20/10), continuous batching vs the same stream served
+    one request at a time with plain FP64 PCG."""
+    from ferrix_amd import _lib as L
+    from ferrix_amd.device import Context
+    from ferrix_amd.serving import make_requests, serve_batched, serve_serial
+
+    prec = L.FX_PREC_MIXED if a.serve_precision == "mixed" else L.FX_PREC_FP64
+    ctx = Context(0)
+    warm = make_requests(4, rate=0.0, seed=a.seed + 1000)
+    serve_batched(ctx, warm, prec)
+    serve_serial(ctx, warm[:2])
+    t = time.perf_counter()
+    sat = make_requests(a.requests, rate=0.0, seed=a.seed)
+    print(f"# serve: {len(sat)} requests generated in {time.perf_counter() - t:.1f} s", flush=True)
+    r_sat = serve_batched(ctx, sat, prec)
+    capacity = len(sat) / r_sat.makespan
+    rate = a.rate if a.rate > 0 else 0.7 * capacity
+    print(f"# serve capacity (all at t=0): {capacity:.2f} req/s, {r_sat.dof_solves(sat) / r_sat.makespan:.0f} "
+          f"DOF-solves/s; arrival rate {rate:.2f} req/s", flush=True)
+    reqs = make_requests(a.requests, rate=rate, seed=a.seed)
+    rb = serve_batched(ctx, reqs, prec)
+    print(f"# serve batched: makespan {rb.makespan:.1f} s, failures {len(rb.failures)}", flush=True)
+    rs = serve_serial(ctx, reqs)
+    print(f"# serve serial fp64: makespan {rs.makespan:.1f} s, failures {len(rs.failures)}", flush=True)
+    err = 0.0
+    for k, x in rb.x.items():
+        if k in rs.x:
+            e = np.linalg.norm(x - rs.x[k], axis=1) / np.maximum(np.linalg.norm(rs.x[k], axis=1), 1e-300)
+            err = max(err, float(e.max()))
+
+    def summary(r):
+        lat = np.array(list(r.latency.values()))
+        return {"dof_solves_per_s": r.dof_solves(reqs) / r.makespan, "makespan_s": r.makespan,
+                "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+                "failures": len(r.failures)}
+
+    sb, ss = summary(rb), summary(rs)
+    kinds = {k: sum(1 for q in reqs if q.kind == k) for k in ("linear", "nlgeom", "c3d20")}
+    out = {
+        "metric": "DOF-solves/s + p50/p99 latency, mixed vs FP64 PCG at rel err <=1e-8",
+        "value": round(sb["dof_solves_per_s"], 1),
+        "unit": "DOF-solves/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(1e3 * rb.makespan, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-stored K / fp64" if prec == L.FX_PREC_MIXED else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"serving_{len(reqs)}_requests_poisson_70lin_20nlgeom_10c3d20",
+                   "requests": kinds, "rate_per_s": rate, "capacity_per_s": capacity},
+        "p50_ms": round(sb["p50_ms"], 3), "p99_ms": round(sb["p99_ms"], 3),
+        "arms": {"batched": sb, "serial_fp64": ss},
+        "u_rel_err_vs_serial_fp64_max": err,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["batch", "single"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "single", "serve"], default="batch")
+    ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 0.7 x capacity)")
+    ap.add_argument("--seed", type=int, default=5, help="serve: request stream seed")
+    ap.add_argument("--serve-precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--arms", default=None)
@@ -443,7 +505,9 @@ def main():
               "(the batched and serving workloads shard by independent mesh, see the design notes).",
               file=sys.stderr)
         sys.exit(2)
-    if a.workload == "single":
+    if a.workload == "serve":
+        run_serve(a)
+    elif a.workload == "single":
         a.steps = 10 if a.steps is None else a.steps
         a.warmup = 2 if a.warmup is None else a.warmup
         a.arms = a.arms or "fp64,mixed"

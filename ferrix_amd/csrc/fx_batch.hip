@@ -53,6 +53,7 @@ struct BSys {
     int64_t it;                         // iteration index of the running right-hand sides
     int32_t R, nparts1, nparts2, want;  // want: right-hand sides of an outer pass (bit mask)
     int32_t nouter;                     // row groups (partials) of the FP64 outer residual pass
+    int32_t G;                          // lanes per node row of the lane-group SpMV of this system
     double *bytes_acc;                  // instrumentation: algorithmic bytes of the SpMV launches issued
     double spmv_bytes0, spmv_bytes_rhs; // per launch: operator part, per active right-hand side
     int32_t go;                         // right-hand sides the next SpMV / update work on (bit mask)
@@ -469,35 +470,40 @@ __global__ __launch_bounds__(kBB) void kb_accumulate(BSys *__restrict__ sys, con
 
 // Mixed refinement, step 2: b = btrue - K64 xacc (the next inner
 // right-hand side) for the right-hand sides in S.want, with block partials
-// (r.r, b.b, r.(inv r)) -> part1.
+// (r.r, b.b, r.(inv r), (|K||x|).(|K||x|)) -> part1 (4 per right-hand side).
 template <int G>
 __global__ __launch_bounds__(kBB) void kb_outer(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
-    BSys &S = sys[t.x];
+    const BSys &S = sys[t.x];
     const int lb = t.y, R = S.R, w = S.want;
+    gptr<const double> xacc = gp((const double *)S.xacc), val = gp(S.val64);
+    gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
     const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
     const int lane = threadIdx.x % G;
-    double acc[3][kRmax];
+    double acc[3][kRmax], mag[3][kRmax];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int r = 0; r < kRmax; ++r)
-            acc[p][r] = 0.0;
+            acc[p][r] = mag[p][r] = 0.0;
     if (a < S.n_nodes) {
-        const int b0 = S.nb_ptr[a], ncol = 3 * (S.nb_ptr[a + 1] - b0);
-        const double *v0 = S.val64 + 9 * (int64_t)b0;
+        const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
+        gptr<const double> v0 = val + 9 * (int64_t)b0;
         for (int j = lane; j < ncol; j += G) {
             const int kb = j / 3;
-            const int64_t c = (3 * (int64_t)S.nb_col[b0 + kb] + (j - 3 * kb)) * R;
+            const int64_t c = (3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb)) * R;
             const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
 #pragma unroll
             for (int r = 0; r < kRmax; ++r) {
                 if (r >= R || !((w >> r) & 1))
                     continue;
-                const double xj = S.xacc[c + r];
+                const double xj = xacc[c + r];
                 acc[0][r] += w0 * xj;
                 acc[1][r] += w1 * xj;
                 acc[2][r] += w2 * xj;
+                mag[0][r] += fabs(w0 * xj);
+                mag[1][r] += fabs(w1 * xj);
+                mag[2][r] += fabs(w2 * xj);
             }
         }
     }
@@ -506,11 +512,13 @@ __global__ __launch_bounds__(kBB) void kb_outer(BSys *__restrict__ sys, const in
 #pragma unroll
         for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int r = 0; r < kRmax; ++r)
+            for (int r = 0; r < kRmax; ++r) {
                 acc[p][r] += __shfl_xor(acc[p][r], off, G);
-    double v[3 * kRmax];
+                mag[p][r] += __shfl_xor(mag[p][r], off, G);
+            }
+    double v[4 * kRmax];
 #pragma unroll
-    for (int k = 0; k < 3 * kRmax; ++k)
+    for (int k = 0; k < 4 * kRmax; ++k)
         v[k] = 0.0;
     if (lane == 0 && a < S.n_nodes) {
 #pragma unroll
@@ -524,28 +532,29 @@ __global__ __launch_bounds__(kBB) void kb_outer(BSys *__restrict__ sys, const in
                 const double bt = S.btrue[q];
                 const double ri = bt - acc[p][r];
                 S.b[q] = ri;
-                v[3 * r] += ri * ri;
-                v[3 * r + 1] += bt * bt;
-                v[3 * r + 2] += ri * (ri * S.inv[d]);
+                v[4 * r] += ri * ri;
+                v[4 * r + 1] += bt * bt;
+                v[4 * r + 2] += ri * (ri * S.inv[d]);
+                v[4 * r + 3] += mag[p][r] * mag[p][r];
             }
         }
     }
-    block_sum<3 * kRmax>(v);
+    block_sum<4 * kRmax>(v);
     if (threadIdx.x == 0)
         for (int r = 0; r < R; ++r)
-            for (int k = 0; k < 3; ++k)
-                S.part1[((size_t)lb * R + r) * 3 + k] = v[3 * r + k];
+            for (int k = 0; k < 4; ++k)
+                S.part1[((size_t)lb * R + r) * 4 + k] = v[4 * r + k];
 }
 
 // one block per listed slot: fixed-order sums of the outer partials
 __global__ __launch_bounds__(kBB) void kb_outer_sum(BSys *__restrict__ sys, const int32_t *__restrict__ list,
                                                     double *__restrict__ out) {
     BSys &S = sys[list[blockIdx.x]];
-    double red[3 * kRmax];
-    bsum_partials<3 * kRmax>(S.part1, S.nouter, 3 * S.R, red);
+    double red[4 * kRmax];
+    bsum_partials<4 * kRmax>(S.part1, S.nouter, 4 * S.R, red);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 3 * kRmax; ++k)
-            out[(size_t)blockIdx.x * 3 * kRmax + k] = k < 3 * S.R ? red[k] : 0.0;
+        for (int k = 0; k < 4 * kRmax; ++k)
+            out[(size_t)blockIdx.x * 4 * kRmax + k] = k < 4 * S.R ? red[k] : 0.0;
 }
 
 inline int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
@@ -582,11 +591,12 @@ void launch_bouter(int G, unsigned nb, hipStream_t s, BSys *sys, const int2 *tab
 struct fx_batch {
     fx_ctx *ctx = nullptr;
     fx_solve_opts opts{};
-    int G = 0;
+    int G = 0;  // lane group forced by the options (0: each system's own)
     struct Slot {
         bool used = false;
         fx_mesh *mesh = nullptr;
         int R = 0;
+        int G = 8;      // lanes per node row (lane-group SpMV, outer pass)
         int phase = 0;  // 0 inner running, 2 finished
         int passes = 0;
         fx_status err = FX_OK;
@@ -602,11 +612,13 @@ struct fx_batch {
     std::vector<std::unique_ptr<Slot>> slots;
     std::vector<fx::BSys> h_sys;
     fx::DBuf<fx::BSys> d_sys;
-    fx::DBuf<int2> d_tab1, d_tab1s, d_tab2, d_tab3, d_tab4;
+    static constexpr int kGroups = 6;  // lane groups 2, 4, ..., 64
+    fx::DBuf<int2> d_tab1g[kGroups], d_tab1s, d_tab2, d_tab3, d_tab4;
+    int n1g[kGroups] = {0, 0, 0, 0, 0, 0};
     fx::DBuf<int32_t> d_list, d_list2, d_run;
     fx::DBuf<double> d_out;
     std::vector<int2> h_tab_start;
-    int n1 = 0, n1s = 0, n2 = 0, nrun = 0;
+    int n1s = 0, n2 = 0, nrun = 0;
     bool tables_dirty = true;
     int64_t resident_bytes = 0;
     // SpMV timing (instrumentation): when on, every SpMV launch is bracketed
@@ -684,8 +696,9 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.n_nodes = m->n_nodes;
     S.n_dof = m->n_dof;
     S.R = sl.R;
-    S.nparts1 = m->sell ? fx::nblk(m->n_nodes, kBB) : fx::nblk(m->n_nodes, kBB / B->G);
-    S.nouter = fx::nblk(m->n_nodes, kBB / B->G);  // kb_outer: lane groups on the FP64 node-block values
+    S.G = sl.G;
+    S.nparts1 = m->sell ? fx::nblk(m->n_nodes, kBB) : fx::nblk(m->n_nodes, kBB / sl.G);
+    S.nouter = fx::nblk(m->n_nodes, kBB / sl.G);  // kb_outer: lane groups on the FP64 node-block values
     // SpMV algorithmic bytes: values in the working precision, one column
     // index per 3x3 block, node-row pointers (+ the FP32 operator's 3x3
     // correction per node); per right-hand side z and p_old gathered once
@@ -717,10 +730,17 @@ void upload_table(fx::DBuf<int2> &d, const std::vector<int2> &t, hipStream_t s) 
     FX_HIP(hipMemcpyAsync(d.p, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice, s));
 }
 
-// block tables of the running slots: tab1 / tab1s SpMV blocks (lane-group
+inline int group_index(int G) {  // 2 -> 0, 4 -> 1, ..., 64 -> 5
+    int i = 0;
+    while ((2 << i) < G)
+        ++i;
+    return i;
+}
+
+// block tables of the running slots: tab1g / tab1s SpMV blocks (lane-group
 // and sliced-ELL operators), tab2 vector blocks, run the running slots
 void build_tables(fx_batch *B) {
-    std::vector<int2> t1, t1s, t2;
+    std::vector<int2> t1g[fx_batch::kGroups], t1s, t2;
     std::vector<int> run;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         auto &sl = *B->slots[k];
@@ -729,13 +749,16 @@ void build_tables(fx_batch *B) {
         run.push_back((int)k);
         const BSys &S = B->h_sys[k];
         for (int b = 0; b < S.nparts1; ++b)
-            (S.sell ? t1s : t1).push_back(make_int2((int)k, b));
+            (S.sell ? t1s : t1g[group_index(S.G)]).push_back(make_int2((int)k, b));
         for (int b = 0; b < S.nparts2; ++b)
             t2.push_back(make_int2((int)k, b));
     }
-    xcd_permute(t1);
+    for (int g = 0; g < fx_batch::kGroups; ++g) {
+        xcd_permute(t1g[g]);
+        B->n1g[g] = (int)t1g[g].size();
+        upload_table(B->d_tab1g[g], t1g[g], B->ctx->stream);
+    }
     xcd_permute(t1s);
-    B->n1 = (int)t1.size();
     B->n1s = (int)t1s.size();
     B->n2 = (int)t2.size();
     B->nrun = (int)run.size();
@@ -744,7 +767,6 @@ void build_tables(fx_batch *B) {
         B->d_run.alloc(std::max<size_t>(B->d_run.n, run.size()));
         FX_HIP(hipMemcpyAsync(B->d_run.p, run.data(), run.size() * sizeof(int), hipMemcpyHostToDevice, s));
     }
-    upload_table(B->d_tab1, t1, s);
     upload_table(B->d_tab1s, t1s, s);
     upload_table(B->d_tab2, t2, s);
     FX_HIP(hipStreamSynchronize(s));  // the host tables are temporaries
@@ -812,15 +834,17 @@ void launch_iterations(fx_batch *B, int n) {
     for (int k = 0; k < n; ++k) {
         if (B->timing)
             FX_HIP(hipEventRecord(B->ev[2 * k], s));
-        if (is_mixed(B)) {
-            if (B->n1)
-                fx::launch_bspmv<float>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
-            if (B->n1s)
+        for (int g = 0; g < fx_batch::kGroups; ++g)
+            if (B->n1g[g]) {
+                if (is_mixed(B))
+                    fx::launch_bspmv<float>(2 << g, B->n1g[g], s, B->d_sys.p, B->d_tab1g[g].p);
+                else
+                    fx::launch_bspmv<double>(2 << g, B->n1g[g], s, B->d_sys.p, B->d_tab1g[g].p);
+            }
+        if (B->n1s) {
+            if (is_mixed(B))
                 hipLaunchKernelGGL(fx::kb_spmv_sell<float>, dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p, B->d_tab1s.p);
-        } else {
-            if (B->n1)
-                fx::launch_bspmv<double>(B->G, B->n1, s, B->d_sys.p, B->d_tab1.p);
-            if (B->n1s)
+            else
                 hipLaunchKernelGGL(fx::kb_spmv_sell<double>, dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p,
                                    B->d_tab1s.p);
         }
@@ -902,27 +926,36 @@ bool advance_phases(fx_batch *B) {
     // outer pass for the slots whose inner solves finished
     hipStream_t s = B->ctx->stream;
     std::vector<int2> t3, t4;
-    for (int k : outer) {
-        const BSys &S = B->h_sys[k];
-        for (int b = 0; b < S.nparts2; ++b)
-            t4.push_back(make_int2(k, b));
-        for (int b = 0; b < S.nouter; ++b)
-            t3.push_back(make_int2(k, b));
+    int seg[fx_batch::kGroups + 1] = {0, 0, 0, 0, 0, 0, 0};  // t3 segment of each lane group
+    for (int g = 0; g < fx_batch::kGroups; ++g) {
+        seg[g] = (int)t3.size();
+        for (int k : outer) {
+            const BSys &S = B->h_sys[k];
+            if (group_index(S.G) == g)
+                for (int b = 0; b < S.nouter; ++b)
+                    t3.push_back(make_int2(k, b));
+        }
     }
+    seg[fx_batch::kGroups] = (int)t3.size();
+    for (int k : outer)
+        for (int b = 0; b < B->h_sys[k].nparts2; ++b)
+            t4.push_back(make_int2(k, b));
     upload_sys(B);  // want masks
     B->d_tab3.alloc(std::max<size_t>(B->d_tab3.n, t3.size()));
     B->d_tab4.alloc(std::max<size_t>(B->d_tab4.n, t4.size()));
     B->d_list.alloc(std::max<size_t>(B->d_list.n, outer.size()));
-    B->d_out.alloc(std::max<size_t>(B->d_out.n, outer.size() * 3 * kRmax));
+    B->d_out.alloc(std::max<size_t>(B->d_out.n, outer.size() * 4 * kRmax));
     FX_HIP(hipMemcpyAsync(B->d_tab3.p, t3.data(), t3.size() * sizeof(int2), hipMemcpyHostToDevice, s));
     FX_HIP(hipMemcpyAsync(B->d_tab4.p, t4.data(), t4.size() * sizeof(int2), hipMemcpyHostToDevice, s));
     FX_HIP(hipMemcpyAsync(B->d_list.p, outer.data(), outer.size() * sizeof(int), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(fx::kb_accumulate, dim3((unsigned)t4.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_tab4.p);
-    fx::launch_bouter(B->G, (unsigned)t3.size(), s, B->d_sys.p, B->d_tab3.p);
+    for (int g = 0; g < fx_batch::kGroups; ++g)
+        if (seg[g + 1] > seg[g])
+            fx::launch_bouter(2 << g, (unsigned)(seg[g + 1] - seg[g]), s, B->d_sys.p, B->d_tab3.p + seg[g]);
     hipLaunchKernelGGL(fx::kb_outer_sum, dim3((unsigned)outer.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_list.p,
                        B->d_out.p);
     FX_HIP(hipGetLastError());
-    std::vector<double> norms(outer.size() * 3 * kRmax);
+    std::vector<double> norms(outer.size() * 4 * kRmax);
     FX_HIP(hipMemcpyAsync(norms.data(), B->d_out.p, norms.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     FX_HIP(hipStreamSynchronize(s));
     std::vector<int> restart;
@@ -935,8 +968,8 @@ bool advance_phases(fx_batch *B) {
         for (int r = 0; r < sl.R; ++r) {
             if (!((S.want >> r) & 1))
                 continue;
-            const double rr = norms[j * 3 * kRmax + 3 * r], bb = norms[j * 3 * kRmax + 3 * r + 1];
-            const double rzv = norms[j * 3 * kRmax + 3 * r + 2];
+            const double *nv = &norms[j * 4 * kRmax + 4 * r];
+            const double rr = nv[0], bb = nv[1], rzv = nv[2], aa = nv[3];
             double crit;
             bool conv;
             if (B->opts.stop_mode == FX_STOP_ABS_SQRT_RZ) {
@@ -946,7 +979,7 @@ bool advance_phases(fx_batch *B) {
                 crit = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : 0.0;
                 conv = crit <= B->opts.tol;
             }
-            conv = conv || fx::at_rounding_floor(crit, sl.prev[r], B->opts.tol);
+            conv = conv || fx::at_rounding_floor(crit, sl.prev[r], std::sqrt(rr), std::sqrt(aa));
             sl.prev[r] = crit;
             sl.crit[r] = crit;
             if (conv)
@@ -1063,8 +1096,7 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
             throw fx::Error{FX_ERR_INVALID, "fx_batch_add: n_dof x n_rhs must stay below 2^31"};
         FX_HIP(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
-        if (B->G == 0)
-            B->G = m->spmv_group > 0 ? m->spmv_group : fx::default_group(m);
+        const int G = B->G > 0 ? B->G : (m->spmv_group > 0 ? m->spmv_group : fx::default_group(m));
         int k = 0;
         while (k < (int)B->slots.size() && B->slots[k] && B->slots[k]->used)
             ++k;
@@ -1092,8 +1124,8 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         }
         fx::launch_sell(m, is_mixed(B));
         fx::launch_jacobi(m, false);
-        const int G = B->G;
-        sl.part1.alloc((size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 3);  // >= SpMV and outer groups
+        sl.G = G;
+        sl.part1.alloc((size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 4);  // >= SpMV and outer groups
         sl.part2.alloc((size_t)std::max(fx::nblk(n, kBB), 1) * n_rhs * 3);
         sl.budget = B->opts.max_iter;
         sl.used = true;
@@ -1172,7 +1204,7 @@ fx_status fx_batch_poll(fx_batch *B, int32_t slot, int32_t *state, fx_solve_repo
         rep->iterations = sl.iters;
         rep->outer_spmv = sl.outer;
         rep->refinements = is_mixed(B) ? sl.passes : 0;
-        rep->spmv_group = B->G;
+        rep->spmv_group = sl.G;
         double c = 0.0;
         for (int r = 0; r < sl.R; ++r)
             c = std::max(c, sl.crit[r]);

@@ -212,10 +212,10 @@ void reset_status(fx_mesh *m);
 double device_max_diag(fx_mesh *m);
 // PCG (fx_pcg.hip)
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep);
-void spmv64(fx_mesh *m, const double *x, double *y, int group);
+void spmv64(fx_mesh *m, const double *x, double *y, int group, double *abs_y = nullptr);
 double profile_spmv(fx_mesh *m, int precision, int reps, int group, double *bytes);
 int default_group(fx_mesh *m);  // lanes per node row for the SpMV
-bool at_rounding_floor(double crit, double crit_prev, double tol);  // mixed outer stop
+bool at_rounding_floor(double crit, double crit_prev, double rnorm, double absnorm);  // mixed outer stop
 void batch_free(fx_batch *b);                 // deletes a batch (its context's stream is idle)
 void batch_drop_mesh(fx_batch *b, fx_mesh *m);  // removes the slots that use m
 void axpy_dev(fx_mesh *m, int64_t n, double a, const double *x, double *y);

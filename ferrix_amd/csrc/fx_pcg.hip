@@ -31,6 +31,7 @@
 // range of node rows and its L2 keeps the gathered columns.
 #include "fx_reduce.h"
 
+#include <cfloat>
 #include <chrono>
 #include <cstdlib>
 
@@ -282,11 +283,13 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, double *__restrict
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_nodes, const int32_t *__restrict__ nb_ptr,
                                                  const int32_t *__restrict__ nb_col, const double *__restrict__ val,
-                                                 const double *__restrict__ x, double *__restrict__ y) {
+                                                 const double *__restrict__ x, double *__restrict__ y,
+                                                 double *__restrict__ ya) {
     const unsigned bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int64_t a = (int64_t)bid * (kBlock / G) + threadIdx.x / G;
     const int lane = threadIdx.x % G;
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    double abs0 = 0.0, abs1 = 0.0, abs2 = 0.0;  // |K| |x|: the magnitude the row sums cancel from
     if (a < n_nodes) {
         const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
         const double *v0 = val + 9 * (int64_t)b0;
@@ -296,6 +299,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_nodes, const int32_t 
             acc0 += v0[j] * xj;
             acc1 += v0[ncol + j] * xj;
             acc2 += v0[2 * ncol + j] * xj;
+            abs0 += fabs(v0[j] * xj);
+            abs1 += fabs(v0[ncol + j] * xj);
+            abs2 += fabs(v0[2 * ncol + j] * xj);
         }
     }
 #pragma unroll
@@ -303,19 +309,29 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_nodes, const int32_t 
         acc0 += __shfl_xor(acc0, off, G);
         acc1 += __shfl_xor(acc1, off, G);
         acc2 += __shfl_xor(acc2, off, G);
+        abs0 += __shfl_xor(abs0, off, G);
+        abs1 += __shfl_xor(abs1, off, G);
+        abs2 += __shfl_xor(abs2, off, G);
     }
     if (lane == 0 && a < n_nodes) {
         y[3 * a] = acc0;
         y[3 * a + 1] = acc1;
         y[3 * a + 2] = acc2;
+        if (ya) {
+            ya[3 * a] = abs0;
+            ya[3 * a + 1] = abs1;
+            ya[3 * a + 2] = abs2;
+        }
     }
 }
 
-// r = b - K x ; block partials of (r.r, b.b, r.(inv r)) -> part.
+// r = b - K x ; block partials of (r.r, b.b, r.(inv r), (|K||x|).(|K||x|)) -> part.
 __global__ __launch_bounds__(kBlock) void k_true_residual(int64_t n, const double *__restrict__ b,
-                                                          const double *__restrict__ kx, const double *__restrict__ inv,
+                                                          const double *__restrict__ kx,
+                                                          const double *__restrict__ kxa,
+                                                          const double *__restrict__ inv,
                                                           double *__restrict__ r, double *__restrict__ part) {
-    double v[3] = {0.0, 0.0, 0.0};
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
         const double ri = b[i] - kx[i];
@@ -323,12 +339,13 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(int64_t n, const doubl
         v[0] = ri * ri;
         v[1] = b[i] * b[i];
         v[2] = ri * (ri * inv[i]);
+        v[3] = kxa[i] * kxa[i];
     }
-    block_sum<3>(v);
+    block_sum<4>(v);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            part[3 * blockIdx.x + k] = v[k];
+        for (int k = 0; k < 4; ++k)
+            part[4 * blockIdx.x + k] = v[k];
 }
 
 // One block: fixed-order sum of NV-wide partials into out (device).
@@ -387,9 +404,9 @@ void launch_iteration(fx_mesh *m, const VT *val, int G, hipStream_t s) {
 }
 
 template <int G>
-void launch_spmv_g(fx_mesh *m, const double *x, double *y, hipStream_t s) {
+void launch_spmv_g(fx_mesh *m, const double *x, double *y, double *ya, hipStream_t s) {
     hipLaunchKernelGGL((k_spmv<G>), dim3(nblocks(m->n_nodes, kBlock / G)), dim3(kBlock), 0, s, m->n_nodes,
-                       m->nb_ptr.p, m->nb_col.p, m->kval.p, x, y);
+                       m->nb_ptr.p, m->nb_col.p, m->kval.p, x, y, ya);
 }
 
 }  // namespace
@@ -403,15 +420,15 @@ int default_group(fx_mesh *m) {
     return G;
 }
 
-void spmv64(fx_mesh *m, const double *x, double *y, int G) {
+void spmv64(fx_mesh *m, const double *x, double *y, int G, double *ya) {
     hipStream_t s = m->ctx->stream;
     switch (G) {
-    case 2: launch_spmv_g<2>(m, x, y, s); break;
-    case 4: launch_spmv_g<4>(m, x, y, s); break;
-    case 8: launch_spmv_g<8>(m, x, y, s); break;
-    case 16: launch_spmv_g<16>(m, x, y, s); break;
-    case 32: launch_spmv_g<32>(m, x, y, s); break;
-    default: launch_spmv_g<64>(m, x, y, s); break;
+    case 2: launch_spmv_g<2>(m, x, y, ya, s); break;
+    case 4: launch_spmv_g<4>(m, x, y, ya, s); break;
+    case 8: launch_spmv_g<8>(m, x, y, ya, s); break;
+    case 16: launch_spmv_g<16>(m, x, y, ya, s); break;
+    case 32: launch_spmv_g<32>(m, x, y, ya, s); break;
+    default: launch_spmv_g<64>(m, x, y, ya, s); break;
     }
     FX_HIP(hipGetLastError());
 }
@@ -550,11 +567,16 @@ static int run_pcg(fx_mesh *m, const VT *val, const double *b_dev, double tol, i
 }
 
 // The outer test is on the true FP64 residual b - K x, which has a rounding
-// floor (the FP64 PCG's recursively updated residual keeps falling below
-// it).  A refinement pass that no longer halves the criterion while it is
-// within 10x of tol has reached that floor: no further pass can improve x.
-bool at_rounding_floor(double crit, double crit_prev, double tol) {
-    return crit <= 10.0 * tol && crit > 0.5 * crit_prev;
+// floor: each row sum cancels terms of total magnitude (|K| |x|)_i, so the
+// computed residual cannot fall much below eps * || |K| |x| ||.  (The FP64
+// PCG's recursively updated residual keeps falling below that floor, which
+// is why its stop test can ask for more than the true residual can show --
+// for a small right-hand side, a Newton correction, by orders of
+// magnitude.)  A refinement pass that no longer halves the criterion while
+// ||r|| is within 64 eps of || |K| |x| || has reached the floor: no further
+// pass can improve x.
+bool at_rounding_floor(double crit, double crit_prev, double rnorm, double absnorm) {
+    return crit > 0.5 * crit_prev && rnorm <= 64.0 * DBL_EPSILON * absnorm;
 }
 
 static int64_t iterations_done(const PcgScalars &h) {
@@ -591,24 +613,25 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
         //   solve K32 d = r to a relative reduction inner_tol (PCG with FP64
         //   vectors), x += d.
         launch_convert32(m);
-        DBuf<double> xacc, rtrue, kx, out3;
+        DBuf<double> xacc, rtrue, kx, kxa, out4;
         xacc.alloc(n);
         rtrue.alloc(n);
         kx.alloc(n);
-        out3.alloc(3);
+        kxa.alloc(n);
+        out4.alloc(4);
         launch_fill(m, xacc.p, n, 0.0);
         double bb = -1.0, crit_prev = HUGE_VAL;
         int64_t budget = o.max_iter;
         int passes = 0;
         const unsigned nb = nblocks(n, kBlock);
         for (;;) {
-            spmv64(m, xacc.p, kx.p, G);
-            hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, n, b_dev, kx.p, m->inv_diag.p, rtrue.p,
-                               m->tmp.p);
-            hipLaunchKernelGGL(k_sum_partials<3>, dim3(1), dim3(kBlock), 0, s, m->tmp.p, (int)nb, out3.p);
+            spmv64(m, xacc.p, kx.p, G, kxa.p);
+            hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(kBlock), 0, s, n, b_dev, kx.p, kxa.p, m->inv_diag.p,
+                               rtrue.p, m->tmp.p);
+            hipLaunchKernelGGL(k_sum_partials<4>, dim3(1), dim3(kBlock), 0, s, m->tmp.p, (int)nb, out4.p);
             FX_HIP(hipGetLastError());
-            double h3[3];
-            FX_HIP(hipMemcpyAsync(h3, out3.p, sizeof h3, hipMemcpyDeviceToHost, s));
+            double h3[4];
+            FX_HIP(hipMemcpyAsync(h3, out4.p, sizeof h3, hipMemcpyDeviceToHost, s));
             FX_HIP(hipStreamSynchronize(s));
             R.outer_spmv++;
             if (bb < 0.0)
@@ -622,7 +645,7 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
                 crit = bb > 0.0 ? std::sqrt(h3[0]) / std::sqrt(bb) : 0.0;
                 conv = crit <= o.tol;
             }
-            conv = conv || at_rounding_floor(crit, crit_prev, o.tol);
+            conv = conv || at_rounding_floor(crit, crit_prev, std::sqrt(h3[0]), std::sqrt(h3[3]));
             crit_prev = crit;
             R.final_criterion = crit;
             if (conv)

@@ -238,6 +238,29 @@ def c3d4_box(nx, ny, nz, h, E, nu, loads_dirs: List[Tuple[float, float, float]],
     return jobs
 
 
+def _cfg4_draw(rng: SplitMix64, index: int, dof_lo: float, dof_hi: float) -> dict:
+    """One mesh of the batched-solve population (draw order: target DOF,
+    aspect, E, nu, load-case count, directions)."""
+    target = math.exp(math.log(dof_lo) + rng.uniform() * (math.log(dof_hi) - math.log(dof_lo)))
+    aspect = 1.0 + 9.0 * rng.uniform()
+    E = math.exp(math.log(1e3) + rng.uniform() * (math.log(3e5) - math.log(1e3)))
+    nu = 0.2 + 0.25 * rng.uniform()
+    ncase = 1 + int(rng.next_u64() % 4)
+    dirs = []
+    for _ in range(ncase):
+        # uniform direction: z = U(-1,1), phi = U(0, 2pi)
+        z = 2.0 * rng.uniform() - 1.0
+        phi = 2.0 * math.pi * rng.uniform()
+        r = math.sqrt(max(0.0, 1.0 - z * z))
+        dirs.append((r * math.cos(phi), r * math.sin(phi), z))
+    # cells: nx = aspect * n, ny = nz = n; 3 (nx+1)(n+1)^2 ~ target
+    nodes = target / 3.0
+    n = max(1, int(round((nodes / aspect) ** (1.0 / 3.0) - 1.0)))
+    nx = max(1, int(round(aspect * (n + 1) - 1)))
+    return dict(index=index, nx=nx, ny=n, nz=n, h=(10.0, 10.0, 10.0), E=E, nu=nu, dirs=dirs,
+                n_dof=3 * (nx + 1) * (n + 1) * (n + 1))
+
+
 def cfg4_batch_specs(n_meshes=512, seed=4, dof_lo=2e3, dof_hi=2e5):
     """Parameters of the batched-solve population: per mesh a target DOF
     (log-uniform), an aspect ratio U(1,10) of the x edge over the y/z edges,
@@ -245,31 +268,65 @@ def cfg4_batch_specs(n_meshes=512, seed=4, dof_lo=2e3, dof_hi=2e5):
     random unit direction times 100 N on the x=max face.  Returns a list of
     dicts; :func:`cfg4_mesh` builds one."""
     rng = SplitMix64(seed)
-    specs = []
-    for m in range(n_meshes):
-        target = math.exp(math.log(dof_lo) + rng.uniform() * (math.log(dof_hi) - math.log(dof_lo)))
-        aspect = 1.0 + 9.0 * rng.uniform()
-        E = math.exp(math.log(1e3) + rng.uniform() * (math.log(3e5) - math.log(1e3)))
-        nu = 0.2 + 0.25 * rng.uniform()
-        ncase = 1 + int(rng.next_u64() % 4)
-        dirs = []
-        for _ in range(ncase):
-            # uniform direction: z = U(-1,1), phi = U(0, 2pi)
-            z = 2.0 * rng.uniform() - 1.0
-            phi = 2.0 * math.pi * rng.uniform()
-            r = math.sqrt(max(0.0, 1.0 - z * z))
-            dirs.append((r * math.cos(phi), r * math.sin(phi), z))
-        # cells: nx = aspect * n, ny = nz = n; 3 (nx+1)(n+1)^2 ~ target
-        nodes = target / 3.0
-        n = max(1, int(round((nodes / aspect) ** (1.0 / 3.0) - 1.0)))
-        nx = max(1, int(round(aspect * (n + 1) - 1)))
-        specs.append(dict(index=m, nx=nx, ny=n, nz=n, h=(10.0, 10.0, 10.0), E=E, nu=nu, dirs=dirs,
-                          n_dof=3 * (nx + 1) * (n + 1) * (n + 1)))
-    return specs
+    return [_cfg4_draw(rng, m, dof_lo, dof_hi) for m in range(n_meshes)]
 
 
 def cfg4_mesh(spec) -> List[Job]:
     return c3d4_box(spec["nx"], spec["ny"], spec["nz"], spec["h"], spec["E"], spec["nu"], spec["dirs"])
+
+
+def c3d4_box_nlgeom(spec) -> Job:
+    """NLGEOM variant of a population mesh: its first load case, ramped by
+    *AMPLITUDE (0,0, 1,1) over *STATIC 0.25, 1., 1e-5, 0.25 (Picard NLGEOM as
+    in the reference), PCG to rel 1e-10."""
+    g = BlockGrid(spec["nx"], spec["ny"], spec["nz"], *spec["h"], quadratic=False)
+    mesh = _mesh(g, g.coords(), g.c3d4(), spec["E"], spec["nu"])
+    amp = Amplitude(False, 0.0, 0.0, ([0.0, 1.0], [0.0, 1.0]))
+    loads, bcs = _clamp_and_load(g, spec["dirs"][0], 100.0, amp)
+    step = StepSpec.build(loads, bcs, inc=IncrementData(0.25, 1.0, 1e-5, 0.25, 1000), nlgeom=True,
+                          solver=SOLVER_ITERATIVE, stop_mode=STOP_REL_R2, tol=1e-10, max_iter=100_000)
+    return Job(mesh, np.zeros(g.n_nodes), [step], f"c3d4_box_nlgeom_{spec['nx']}x{spec['ny']}x{spec['nz']}")
+
+
+def c3d20_block_cells(target_dof: float):
+    """(nx, n) of a 4:1:1 C3D20 block with n_dof closest to target_dof."""
+    best = None
+    for n in range(1, 64):
+        g = BlockGrid(4 * n, n, n, 1.0, 1.0, 1.0, quadratic=True)
+        d = abs(3 * g.n_nodes - target_dof)
+        if best is None or d < best[0]:
+            best = (d, n)
+    return 4 * best[1], best[1]
+
+
+def cfg5_request_specs(n_requests=1000, seed=5, mix=(0.7, 0.2, 0.1)):
+    """The serving stream's requests (arrival times are drawn separately):
+    kind 'linear' (a population mesh with its 1-4 load cases), 'nlgeom' (a
+    population mesh, NLGEOM, first load case) or 'c3d20' (a jittered 4:1:1
+    C3D20 block of log-uniform 1e4-5e5 DOF, one load case)."""
+    rng = SplitMix64(seed)
+    out = []
+    for k in range(n_requests):
+        u = rng.uniform()
+        if u < mix[0]:
+            out.append(dict(kind="linear", mesh=_cfg4_draw(rng, k, 2e3, 2e5)))
+        elif u < mix[0] + mix[1]:
+            out.append(dict(kind="nlgeom", mesh=_cfg4_draw(rng, k, 2e3, 2e5)))
+        else:
+            target = math.exp(math.log(1e4) + rng.uniform() * (math.log(5e5) - math.log(1e4)))
+            nx, n = c3d20_block_cells(target)
+            out.append(dict(kind="c3d20", nx=nx, n=n, seed=int(rng.next_u64() % (1 << 31))))
+    return out
+
+
+def cfg5_jobs(req) -> List[Job]:
+    """Jobs of one request: the load cases (sharing mesh and BCs) of a linear
+    request, or the single job of an NLGEOM / C3D20 request."""
+    if req["kind"] == "linear":
+        return cfg4_mesh(req["mesh"])
+    if req["kind"] == "nlgeom":
+        return [c3d4_box_nlgeom(req["mesh"])]
+    return [cfg2_c3d20_block(seed=req["seed"], nx=req["nx"], ny=req["n"], nz=req["n"])]
 
 
 def to_inp(job: Job) -> str:

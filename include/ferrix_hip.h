@@ -51,8 +51,8 @@ typedef struct fx_mesh fx_mesh;
 /* ---- context ---------------------------------------------------------- */
 /* Binds HIP device `device`, creates one stream (default priority). */
 fx_status fx_ctx_create(int device, fx_ctx **out);
-/* Destroys the context and every mesh still alive on it (their handles
- * become invalid). */
+/* Destroys the context and every batch and mesh still alive on it (their
+ * handles become invalid). */
 void fx_ctx_destroy(fx_ctx *ctx);
 /* Message of the last failing call on this context ("" if none). */
 const char *fx_last_error(const fx_ctx *ctx);
@@ -261,9 +261,10 @@ fx_status fx_profile_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_
  * row sums associate differently from fx_solve's lane-group SpMV: results
  * agree to rounding.  Meshes on the lane-group SpMV (C3D20, or uploaded with
  * FX_SELL=0) give FP64 solutions bitwise equal to fx_solve's.  Either way a
- * system's result does not depend on what else shares the batch.  Systems may join while others run
- * (continuous batching); a mesh must not be reassembled while it is in a
- * batch.  All calls run on the context stream.
+ * system's result does not depend on what else shares the batch.  Systems
+ * may join while others run (continuous batching); a mesh must not be
+ * reassembled while it is in a batch, and destroying a mesh drops its slots.
+ * All calls run on the context stream.
  */
 typedef struct fx_batch fx_batch;
 

@@ -194,6 +194,40 @@ def test_step_mixed_precision(ctx):
     assert np.linalg.norm(u - u_o) / np.linalg.norm(u_o) < 1e-8
 
 
+@pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
+def test_step_nlgeom_beam_matches_oracle(ctx, precision):
+    """Picard NLGEOM (static_step.rs:101-258, coordinates X + u) on a small
+    C3D20 beam, 10 ramped increments: same NR history as the oracle and the
+    same displacements to 1e-7 (each linear solve to rel 1e-10)."""
+    job = meshgen.cfg3_c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)
+    u_o, logs = O.run_job(job)
+    u, reps = run_job(ctx, job, precision=precision)
+    say("nr", reps[0].nr_iterations, "solves", reps[0].linear_solves, "oracle solves", logs[0].linear_solves,
+        "pcg", reps[0].pcg_iterations, "tip", u.reshape(-1, 3)[:, 2].min())
+    assert reps[0].linear_solves == logs[0].linear_solves
+    assert np.linalg.norm(u - u_o) / np.linalg.norm(u_o) < 1e-7
+
+
+@pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
+def test_reference_nlgeom_deck(ctx, precision):
+    """calculix_parity.rs:183-272: NLGEOM_test, two steps, node 1 against
+    CalculiX at the reference's 5e-3."""
+    p = Project.from_jobname(deck("NLGEOM_test"))
+    p.default_initial_temperature = 20.0
+    job = Job.from_project(p)
+    idx = p.get_index_for_node_id(1)
+    expected = [(-9.163880e-02, 3.500324e-01, 1.324598e-01), (8.591845e-01, 3.624377e-01, 2.653401e-01)]
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.set_u(np.zeros(job.n_dof))
+    timer = L.fx_timer(0.0, 0.0, 1.0)
+    so = SolveOptions(precision=precision, stop_mode=STOP_REL_R2, tol=1e-12, max_iter=200_000)
+    for k, st in enumerate(job.steps):
+        dm.static_step(st, k + 1, timer, direct_opts=so, solve_opts=so)
+        got = dm.get_u()[3 * idx:3 * idx + 3]
+        say("step", k + 1, got, expected[k])
+        assert np.all(np.abs(got - np.array(expected[k])) < 5e-3)
+
+
 def test_singular_jacobian(ctx):
     job = meshgen.cfg1_c3d4_cantilever()
     m = job.mesh
