@@ -439,22 +439,31 @@ def run_serve(a):
           f"DOF-solves/s; arrival rate {rate:.2f} req/s", flush=True)
     reqs = make_requests(a.requests, rate=rate, seed=a.seed)
     rb = serve_batched(ctx, reqs, prec)
-    print(f"# serve batched: makespan {rb.makespan:.1f} s, failures {len(rb.failures)}", flush=True)
+    print(f"# serve batched ({a.serve_precision}): makespan {rb.makespan:.1f} s, failures {len(rb.failures)}",
+          flush=True)
+    other = L.FX_PREC_FP64 if prec == L.FX_PREC_MIXED else L.FX_PREC_MIXED
+    ro = serve_batched(ctx, reqs, other)
+    print(f"# serve batched (other precision): makespan {ro.makespan:.1f} s, failures {len(ro.failures)}",
+          flush=True)
     rs = serve_serial(ctx, reqs)
     print(f"# serve serial fp64: makespan {rs.makespan:.1f} s, failures {len(rs.failures)}", flush=True)
-    err = 0.0
-    for k, x in rb.x.items():
-        if k in rs.x:
-            e = np.linalg.norm(x - rs.x[k], axis=1) / np.maximum(np.linalg.norm(rs.x[k], axis=1), 1e-300)
-            err = max(err, float(e.max()))
+
+    def max_err(r):
+        err = 0.0
+        for k, x in r.x.items():
+            if k in rs.x:
+                e = np.linalg.norm(x - rs.x[k], axis=1) / np.maximum(np.linalg.norm(rs.x[k], axis=1), 1e-300)
+                err = max(err, float(e.max()))
+        return err
 
     def summary(r):
         lat = np.array(list(r.latency.values()))
         return {"dof_solves_per_s": r.dof_solves(reqs) / r.makespan, "makespan_s": r.makespan,
                 "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
-                "failures": len(r.failures)}
+                "failures": len(r.failures), "u_rel_err_vs_serial_fp64_max": max_err(r) if r is not rs else 0.0}
 
-    sb, ss = summary(rb), summary(rs)
+    sb, so_, ss = summary(rb), summary(ro), summary(rs)
+    err = sb["u_rel_err_vs_serial_fp64_max"]
     kinds = {k: sum(1 for q in reqs if q.kind == k) for k in ("linear", "nlgeom", "c3d20")}
     out = {
         "metric": "DOF-solves/s + p50/p99 latency, mixed vs FP64 PCG at rel err <=1e-8",
@@ -468,7 +477,9 @@ def run_serve(a):
         "config": {"workload": f"serving_{len(reqs)}_requests_poisson_70lin_20nlgeom_10c3d20",
                    "requests": kinds, "rate_per_s": rate, "capacity_per_s": capacity},
         "p50_ms": round(sb["p50_ms"], 3), "p99_ms": round(sb["p99_ms"], 3),
-        "arms": {"batched": sb, "serial_fp64": ss},
+        "arms": {f"batched_{a.serve_precision}": sb,
+                 "batched_" + ("fp64" if a.serve_precision == "mixed" else "mixed"): so_,
+                 "serial_fp64": ss},
         "u_rel_err_vs_serial_fp64_max": err,
     }
     print(json.dumps(out), flush=True)

@@ -17,13 +17,13 @@ using fx::Error;
 
 namespace {
 
-thread_local std::string g_no_ctx_error;
+// The last failure message is kept per calling thread (like errno): one
+// context may be driven from several threads (admission, batch), and a
+// message must not be overwritten under a reader.
+thread_local std::string g_last_error;
 
-fx_status fail_ctx(fx_ctx *ctx, fx_status code, const std::string &msg) {
-    if (ctx)
-        ctx->err = msg;
-    else
-        g_no_ctx_error = msg;
+fx_status fail_ctx(fx_ctx *, fx_status code, const std::string &msg) {
+    g_last_error = msg;
     return code;
 }
 
@@ -227,6 +227,10 @@ double amp_y(const fx_amplitudes &A, int a, const fx_timer &t, int origin, int c
 
 }  // namespace
 
+namespace fx {
+void set_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace fx
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -234,7 +238,7 @@ extern "C" {
 
 const char *fx_version(void) { return "ferrix_hip 0.1 (gfx950)"; }
 
-const char *fx_last_error(const fx_ctx *ctx) { return ctx ? ctx->err.c_str() : g_no_ctx_error.c_str(); }
+const char *fx_last_error(const fx_ctx *) { return g_last_error.c_str(); }
 
 fx_status fx_ctx_create(int device, fx_ctx **out) {
     fx_ctx *ctx = nullptr;
@@ -287,6 +291,7 @@ void fx_ctx_destroy(fx_ctx *ctx) {
         return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    std::lock_guard<std::mutex> lk(ctx->reg);
     for (fx_batch *b : ctx->batches)
         fx::batch_free(b);
     ctx->batches.clear();
@@ -422,7 +427,10 @@ fx_status fx_mesh_upload(fx_ctx *ctx, int64_t n_nodes, const double *xyz, const 
                      m->kval.bytes(), m->keep.bytes(), m->part.bytes()})
         m->device_bytes += (int64_t)b;
     m->device_bytes += 13 * m->n_dof * (int64_t)sizeof(double);
-    ctx->meshes.push_back(m);
+    {
+        std::lock_guard<std::mutex> lk(ctx->reg);
+        ctx->meshes.push_back(m);
+    }
     *out = m;
     m = nullptr;
     }
@@ -445,6 +453,7 @@ void fx_mesh_destroy(fx_mesh *m) {
     if (!m)
         return;
     fx_ctx *ctx = m->ctx;
+    std::lock_guard<std::mutex> lk(ctx->reg);
     auto it = std::find(ctx->meshes.begin(), ctx->meshes.end(), m);
     if (it == ctx->meshes.end())
         return;  // not a live mesh of its context

@@ -310,8 +310,7 @@ __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double
         for (int r = 0; r < RR; ++r)
             acc[p][r] = 0.0;
     if (a < S.n_nodes) {
-        sell_node_rows<VT, RR>(gp(S.nb_ptr), gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold, beta,
-                               acc);
+        sell_node_rows<VT, RR>(gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold, beta, acc);
         bspmv_tail<VT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
     }
 }
@@ -1040,13 +1039,13 @@ fx_status fx_batch_create(fx_ctx *ctx, const fx_solve_opts *opts, fx_batch **out
             fx_solve_opts_default(&B->opts);
         if (B->opts.precision != FX_PREC_FP64 && B->opts.precision != FX_PREC_MIXED) {
             delete B;
-            ctx->err = "fx_batch_create: unknown precision";
+            fx::set_error("fx_batch_create: unknown precision");
             return FX_ERR_INVALID;
         }
         const int g = B->opts.spmv_group;
         if (!(g == 0 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64)) {
             delete B;
-            ctx->err = "fx_batch_create: spmv_group must be 0 or a power of two in 2..64";
+            fx::set_error("fx_batch_create: spmv_group must be 0 or a power of two in 2..64");
             return FX_ERR_INVALID;
         }
         // 0: chosen from the first system admitted (as fx_solve chooses it)
@@ -1054,15 +1053,18 @@ fx_status fx_batch_create(fx_ctx *ctx, const fx_solve_opts *opts, fx_batch **out
         B->d_bytes.alloc(1);
         FX_HIP(hipMemsetAsync(B->d_bytes.p, 0, sizeof(double), ctx->stream));
         FX_HIP(hipStreamSynchronize(ctx->stream));
-        ctx->batches.push_back(B);
+        {
+            std::lock_guard<std::mutex> lk(ctx->reg);
+            ctx->batches.push_back(B);
+        }
         *out = B;
     } catch (const fx::Error &e) {
         delete B;
-        ctx->err = e.msg;
+        fx::set_error(e.msg);
         return e.code;
     } catch (const std::exception &e) {
         delete B;
-        ctx->err = e.what();
+        fx::set_error(e.what());
         return FX_ERR_OOM;
     }
     return FX_OK;
@@ -1072,10 +1074,13 @@ void fx_batch_destroy(fx_batch *B) {
     if (!B)
         return;
     fx_ctx *ctx = B->ctx;
-    auto it = std::find(ctx->batches.begin(), ctx->batches.end(), B);
-    if (it == ctx->batches.end())
-        return;  // not a live batch of its context
-    ctx->batches.erase(it);
+    {
+        std::lock_guard<std::mutex> lk(ctx->reg);
+        auto it = std::find(ctx->batches.begin(), ctx->batches.end(), B);
+        if (it == ctx->batches.end())
+            return;  // not a live batch of its context
+        ctx->batches.erase(it);
+    }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete B;
@@ -1146,10 +1151,10 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         B->tables_dirty = true;
         *slot_out = k;
     } catch (const fx::Error &e) {
-        ctx->err = e.msg;
+        fx::set_error(e.msg);
         return e.code;
     } catch (const std::exception &e) {
-        ctx->err = e.what();
+        fx::set_error(e.what());
         return FX_ERR_OOM;
     }
     return FX_OK;
@@ -1184,10 +1189,10 @@ fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
             *n_running_out = c;
         }
     } catch (const fx::Error &e) {
-        ctx->err = e.msg;
+        fx::set_error(e.msg);
         return e.code;
     } catch (const std::exception &e) {
-        ctx->err = e.what();
+        fx::set_error(e.what());
         return FX_ERR_OOM;
     }
     return FX_OK;
@@ -1229,7 +1234,7 @@ fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
             for (int64_t i = 0; i < n; ++i)
                 x[(int64_t)r * n + i] = inter[i * sl.R + r];
     } catch (const fx::Error &e) {
-        B->ctx->err = e.msg;
+        fx::set_error(e.msg);
         return e.code;
     }
     return FX_OK;
@@ -1251,7 +1256,7 @@ fx_status fx_batch_spmv_stats(fx_batch *B, int64_t *launches, int64_t *timed, do
         FX_HIP(hipMemcpyAsync(&by, B->d_bytes.p, sizeof(double), hipMemcpyDeviceToHost, B->ctx->stream));
         FX_HIP(hipStreamSynchronize(B->ctx->stream));
     } catch (const fx::Error &e) {
-        B->ctx->err = e.msg;
+        fx::set_error(e.msg);
         return e.code;
     }
     if (launches)

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -145,8 +146,8 @@ struct fx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int n_cu = 0;
-    std::string err;
     fx::DevTables tables;
+    std::mutex reg;                   // guards the two registries below
     std::vector<fx_mesh *> meshes;    // live meshes, destroyed with the context
     std::vector<fx_batch *> batches;  // live batches, destroyed with the context
 };
@@ -215,6 +216,7 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
 void spmv64(fx_mesh *m, const double *x, double *y, int group, double *abs_y = nullptr);
 double profile_spmv(fx_mesh *m, int precision, int reps, int group, double *bytes);
 int default_group(fx_mesh *m);  // lanes per node row for the SpMV
+void set_error(const std::string &msg);  // message of the calling thread's last failure
 bool at_rounding_floor(double crit, double crit_prev, double rnorm, double absnorm);  // mixed outer stop
 void batch_free(fx_batch *b);                 // deletes a batch (its context's stream is idle)
 void batch_drop_mesh(fx_batch *b, fx_mesh *m);  // removes the slots that use m

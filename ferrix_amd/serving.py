@@ -8,8 +8,9 @@ NLGEOM static job (Newton-Raphson with the reference's increment control,
 * admits an arrived request when its estimated device memory fits the
   budget (FIFO; the head of the queue waits until memory frees up);
 * uploads its mesh (host symbolic phase + copies), assembles K and applies
-  the penalty BCs, and adds the system to the running :class:`Batch` --
-  it joins the next iteration chunk while the other systems keep going;
+  the penalty BCs on an admission thread -- the host work overlaps with the
+  running batch -- and adds the system to the running :class:`Batch`: it
+  joins the next iteration chunk while the other systems keep going;
 * runs NLGEOM requests on worker threads, each with its own context and
   stream (``fx_static_step``), so their Newton solves overlap with the
   batch on the device;
@@ -123,15 +124,23 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
     res = ServeResult()
     bt = Batch(ctx, _static_opts(precision))
     pool = ThreadPoolExecutor(max_workers=nl_workers)
+    prep = ThreadPoolExecutor(max_workers=1)
     worker = _NlWorker(precision)
     pending = collections.deque(sorted(reqs, key=lambda r: r.arrival))
     waiting: collections.deque = collections.deque()
     running: Dict[int, tuple] = {}                   # slot -> (request, System, bytes)
     nl: Dict[int, tuple] = {}                        # rid -> (future, request, bytes)
+    staged: collections.deque = collections.deque()  # (future System, request, bytes), admission order
     used = 0
+
+    def prepare(r: Request) -> System:
+        s = System(r.rid, r.jobs, DeviceMesh(ctx, r.jobs[0].mesh, r.jobs[0].node_temp))
+        assemble(s)
+        return s
+
     t0 = time.perf_counter()
     try:
-        while pending or waiting or running or nl:
+        while pending or waiting or running or nl or staged:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
@@ -139,7 +148,7 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
             while waiting:
                 r = waiting[0]
                 need = r.est_bytes()
-                if (running or nl) and used + need > mem_budget:
+                if (running or nl or staged) and used + need > mem_budget:
                     break
                 waiting.popleft()
                 used += need
@@ -147,8 +156,16 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
                 if r.kind == "nlgeom":
                     nl[r.rid] = (pool.submit(worker, r), r, need)
                     continue
-                s = System(r.rid, r.jobs, DeviceMesh(ctx, r.jobs[0].mesh, r.jobs[0].node_temp))
-                assemble(s)
+                staged.append((prep.submit(prepare, r), r, need))
+            while staged and staged[0][0].done():
+                fut, r, need = staged.popleft()
+                try:
+                    s = fut.result()
+                except Exception as e:  # noqa: BLE001 -- a failed request is reported, the server goes on
+                    res.failures[r.rid] = f"{type(e).__name__}: {e}"
+                    res.latency[r.rid] = time.perf_counter() - t0 - r.arrival
+                    used -= need
+                    continue
                 running[bt.add(s.dm, s.rhs)] = (r, s, need)
             if running:
                 bt.run(max_iters=chunk_iters)
@@ -177,11 +194,12 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
                 res.latency[rid] = time.perf_counter() - t0 - r.arrival
                 used -= need
             if not running and not waiting:
-                # idle: wait for the next arrival or a worker
+                # idle: wait for the next arrival, a worker or an admission
                 nxt = pending[0].arrival - (time.perf_counter() - t0) if pending else 0.0
-                time.sleep(min(max(nxt, 0.0), 0.002) if nl or pending else 0.0)
+                time.sleep(min(max(nxt, 0.0), 0.002) if nl or pending or staged else 0.0)
         res.makespan = time.perf_counter() - t0
     finally:
+        prep.shutdown(wait=True)
         pool.shutdown(wait=True)
         bt.close()
     return res

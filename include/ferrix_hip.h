@@ -18,7 +18,9 @@
  * ownership.  Node index = rank of the node ID in ascending order and
  * DOF = 3*index + d (src/solver/mesh_lib/mesh.rs:35-49).  Elements are
  * visited in the order given (the caller passes ascending element ID).
- * A context is used by one host thread at a time.  Errors never cross the
+ * One host thread drives a batch; meshes may be uploaded, assembled and
+ * destroyed from other threads on the same context (uploads overlap with a
+ * running batch).  Errors never cross the
  * boundary as exceptions: every call returns an fx_status and
  * fx_last_error() holds the message.
  */
@@ -54,7 +56,8 @@ fx_status fx_ctx_create(int device, fx_ctx **out);
 /* Destroys the context and every batch and mesh still alive on it (their
  * handles become invalid). */
 void fx_ctx_destroy(fx_ctx *ctx);
-/* Message of the last failing call on this context ("" if none). */
+/* Message of the calling thread's last failing call ("" if none); ctx is
+ * accepted for symmetry and may be NULL. */
 const char *fx_last_error(const fx_ctx *ctx);
 /* Library version string. */
 const char *fx_version(void);
