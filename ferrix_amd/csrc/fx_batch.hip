@@ -310,7 +310,8 @@ __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double
         for (int r = 0; r < RR; ++r)
             acc[p][r] = 0.0;
     if (a < S.n_nodes) {
-        sell_node_rows<VT, RR>(gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold, beta, acc);
+        sell_node_rows<VT, RR>(gp(S.nb_ptr), gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold, beta,
+                               acc);
         bspmv_tail<VT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
     }
 }

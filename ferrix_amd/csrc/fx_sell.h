@@ -6,8 +6,8 @@
 // sl_ptr[slice] + k, and the block's column node at scol[kk*64 + lane].  One
 // thread owns one node row: every load of a block column is one coalesced
 // 64-lane access, and the gathered operand entries of a block are 3
-// consecutive DOFs.  Rows shorter than their slice's width are padded with
-// zero values whose column is the row's own node.
+// consecutive DOFs.  Rows shorter than their slice's width are padded; the
+// padding is never read (each thread stops at its own row length).
 //
 // The accumulation order of a DOF row is block column k ascending (the
 // node-block pattern's sorted columns), q = 0, 1, 2 within a block, one
@@ -31,66 +31,50 @@ __device__ __forceinline__ gptr<T> gp(T *p) {
     return (gptr<T>)p;
 }
 
-// acc[p][r] += K(a,p; c,q) * (z + beta_r p_old)(c,q,r) over q for one block
-// column of the slice (v: its 9 values, c: its column node).
-template <typename VT, int RR>
-__device__ __forceinline__ void sell_block(int c, const VT (&v)[9], gptr<const double> z, gptr<const double> pold,
-                                           const double (&beta)[RR], double (&acc)[3][RR]) {
-    double x[3][RR];
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int r = 0; r < RR; ++r) {
-            const int64_t at = (3 * (int64_t)c + q) * RR + r;
-            x[q][r] = z[at] + beta[r] * pold[at];
-        }
-#pragma unroll
-    for (int r = 0; r < RR; ++r)
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            acc[p][r] += (double)v[3 * p] * x[0][r];
-            acc[p][r] += (double)v[3 * p + 1] * x[1][r];
-            acc[p][r] += (double)v[3 * p + 2] * x[2][r];
-        }
-}
-
 // acc[p][r] += sum over the row's blocks of K(a,p; c,q) * (z + beta_r p_old)(c,q,r)
-// for r < RR right-hand sides (vectors interleaved: v[dof*RR + r]).  Every
-// thread of a wavefront (one slice) runs the slice width: the padding of
-// shorter rows holds zero values and the row's own node as column, and
-// adding the exact zeros it contributes leaves every sum unchanged.  With a
-// trip count uniform over the wavefront, both block columns of a step load
-// unconditionally, so their 20 loads are in flight together.
+// for r < RR right-hand sides (vectors interleaved: v[dof*RR + r]).
 template <typename VT, int RR>
-__device__ __forceinline__ void sell_node_rows(gptr<const int32_t> sl_ptr, gptr<const int32_t> scol,
-                                               gptr<const VT> sval, int64_t a, gptr<const double> z,
-                                               gptr<const double> pold, const double (&beta)[RR],
-                                               double (&acc)[3][RR]) {
+__device__ __forceinline__ void sell_node_rows(gptr<const int32_t> nb_ptr, gptr<const int32_t> sl_ptr,
+                                               gptr<const int32_t> scol, gptr<const VT> sval, int64_t a,
+                                               gptr<const double> z, gptr<const double> pold,
+                                               const double (&beta)[RR], double (&acc)[3][RR]) {
     const int64_t s = a / kSlice;
     const int lane = (int)(a % kSlice);
-    const size_t k0 = (size_t)sl_ptr[s];
-    const int w = sl_ptr[s + 1] - (int)k0;
-    int k = 0;
-    for (; k + 1 < w; k += 2) {
-        const size_t ka = k0 + k, kb = ka + 1;
-        const int ca = scol[ka * kSlice + lane], cb = scol[kb * kSlice + lane];
-        VT va[9], vb[9];
+    const int k0 = sl_ptr[s];
+    const int k1 = k0 + (nb_ptr[a + 1] - nb_ptr[a]);
+    for (int k = k0; k < k1; k += 2) {
+        int c[2];
+        double v[2][9];
 #pragma unroll
-        for (int e = 0; e < 9; ++e) {
-            va[e] = sval[(ka * 9 + e) * kSlice + lane];
-            vb[e] = sval[(kb * 9 + e) * kSlice + lane];
+        for (int u = 0; u < 2; ++u) {
+            const int kk = k + u;
+            c[u] = kk < k1 ? scol[(size_t)kk * kSlice + lane] : -1;
+            if (kk < k1)
+#pragma unroll
+                for (int e = 0; e < 9; ++e)
+                    v[u][e] = (double)sval[((size_t)kk * 9 + e) * kSlice + lane];
         }
-        sell_block<VT, RR>(ca, va, z, pold, beta, acc);
-        sell_block<VT, RR>(cb, vb, z, pold, beta, acc);
-    }
-    if (k < w) {
-        const size_t ka = k0 + k;
-        const int ca = scol[ka * kSlice + lane];
-        VT va[9];
 #pragma unroll
-        for (int e = 0; e < 9; ++e)
-            va[e] = sval[(ka * 9 + e) * kSlice + lane];
-        sell_block<VT, RR>(ca, va, z, pold, beta, acc);
+        for (int u = 0; u < 2; ++u) {
+            if (c[u] < 0)
+                continue;
+            double x[3][RR];
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int r = 0; r < RR; ++r) {
+                    const int64_t at = (3 * (int64_t)c[u] + q) * RR + r;
+                    x[q][r] = z[at] + beta[r] * pold[at];
+                }
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) {
+                    acc[p][r] += v[u][3 * p] * x[0][r];
+                    acc[p][r] += v[u][3 * p + 1] * x[1][r];
+                    acc[p][r] += v[u][3 * p + 2] * x[2][r];
+                }
+        }
     }
 }
 
