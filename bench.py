@@ -282,10 +282,16 @@ def counter_traffic(name, bytes_per_launch):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
+    ratio = ref = None
     for line in open(path):
         if line.startswith("traffic_over_algorithmic:"):
-            return round(float(line.split(":", 1)[1]) * bytes_per_launch)
-    return None
+            ratio = float(line.split(":", 1)[1])
+        if line.startswith("algorithmic_bytes_per_launch:"):
+            ref = float(line.split(":", 1)[1])
+    # the counters describe this workload only if its launches move the same bytes
+    if ratio is None or ref is None or abs(ref - bytes_per_launch) > 1e-3 * ref:
+        return None
+    return round(ratio * bytes_per_launch)
 
 
 def run_batch(a):

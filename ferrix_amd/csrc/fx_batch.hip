@@ -46,7 +46,7 @@ struct BSys {
     int32_t sell;
     double *b;                          // inner right-hand sides
     const double *btrue;                // true right-hand sides (mixed)
-    double *x, *r, *z, *p0, *p1, *ap;  // inner PCG vectors
+    void *x, *r, *z, *p0, *p1, *ap;    // inner PCG vectors, XT (double, or float for the mixed inner solve)
     double *xacc;                       // FP64 solution of the refinement (mixed)
     double *part1, *part2;
     int64_t n_nodes, n_dof;
@@ -87,7 +87,9 @@ __device__ __forceinline__ void bsum_partials(const double *part, int nparts, in
 }
 
 // Inner-solve start for the right-hand sides with status 0:
-// x = 0, r = b, z = r inv, p0 = z; partials (r.z, r.r, b.b) -> part2.
+// x = 0, r = b, z = r inv, p0 = z (stored as XT); partials (r.z, r.r, b.b)
+// of the stored values -> part2.
+template <typename XT>
 __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     BSys &S = sys[t.x];
@@ -103,12 +105,12 @@ __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int
             if (r >= R || S.rhs[r].status != 0)  // set by the host before this launch
                 continue;
             const int64_t q = i * R + r;
-            const double bi = S.b[q];
-            const double zi = bi * S.inv[i];
-            S.x[q] = 0.0;
-            S.r[q] = bi;
-            S.z[q] = zi;
-            S.p0[q] = zi;
+            const double bi = (double)(XT)S.b[q];
+            const double zi = (double)(XT)(bi * S.inv[i]);
+            ((XT *)S.x)[q] = (XT)0;
+            ((XT *)S.r)[q] = (XT)bi;
+            ((XT *)S.z)[q] = (XT)zi;
+            ((XT *)S.p0)[q] = (XT)zi;
             v[3 * r] = bi * zi;
             v[3 * r + 1] = bi * bi;
             v[3 * r + 2] = bi * bi;
@@ -179,18 +181,18 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
 // Tail of a batched SpMV row group: p_new = z + beta p_old at the node's
 // DOFs, the FP32 operator's correction block, Ap, and p.Ap block partials
 // (right-hand sides outside `go` are computed but not stored).
-template <typename VT, int RR>
+template <typename VT, typename XT, int RR>
 __device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, const double (&beta)[RR],
-                                           gptr<const double> z, gptr<const double> pold, gptr<double> pnew,
+                                           gptr<const XT> z, gptr<const XT> pold, gptr<XT> pnew,
                                            double (&acc)[3][RR], double (&loc)[kRmax]) {
-    gptr<double> ap = gp(S.ap);
+    gptr<XT> ap = gp((XT *)S.ap);
     double qv[3][RR];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
             const int64_t q = (3 * a + p) * RR + r;
-            qv[p][r] = z[q] + beta[r] * pold[q];
+            qv[p][r] = (double)(XT)((double)z[q] + beta[r] * (double)pold[q]);  // p_new as stored
         }
     if constexpr (sizeof(VT) == 4) {
         gptr<const float> c = gp(S.kc) + 9 * a;
@@ -213,24 +215,25 @@ __device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, con
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
             const int64_t q = (3 * a + p) * RR + r;
-            pnew[q] = qv[p][r];
-            ap[q] = acc[p][r];
-            sum += qv[p][r] * acc[p][r];
+            const double aps = (double)(XT)acc[p][r];  // Ap as stored
+            pnew[q] = (XT)qv[p][r];
+            ap[q] = (XT)aps;
+            sum += qv[p][r] * aps;
         }
         loc[r] = sum;
     }
 }
 
-template <int G, typename VT, int RR>
+template <int G, typename VT, typename XT, int RR>
 __device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, double (&loc)[kRmax]) {
     double beta[RR];
 #pragma unroll
     for (int r = 0; r < RR; ++r)
         beta[r] = ((go >> r) & 1) ? S.rhs[r].beta : 0.0;
     const int64_t itv = S.it;
-    gptr<const double> z = gp((const double *)S.z);
-    gptr<const double> pold = gp((const double *)((itv & 1) ? S.p1 : S.p0));
-    gptr<double> pnew = gp((itv & 1) ? S.p0 : S.p1);
+    gptr<const XT> z = gp((const XT *)S.z);
+    gptr<const XT> pold = gp((const XT *)((itv & 1) ? S.p1 : S.p0));
+    gptr<XT> pnew = gp((XT *)((itv & 1) ? S.p0 : S.p1));
     gptr<const VT> val = gp((const VT *)S.val);
     gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
     const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
@@ -271,7 +274,7 @@ __device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, doubl
                 double xj[RR];
 #pragma unroll
                 for (int r = 0; r < RR; ++r)
-                    xj[r] = z[c[u] + r] + beta[r] * pold[c[u] + r];
+                    xj[r] = (double)(XT)((double)z[c[u] + r] + beta[r] * (double)pold[c[u] + r]);
 #pragma unroll
                 for (int r = 0; r < RR; ++r) {
                     acc[0][r] += w[u][0] * xj[r];
@@ -289,19 +292,19 @@ __device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, doubl
             for (int r = 0; r < RR; ++r)
                 acc[p][r] += __shfl_xor(acc[p][r], off, G);
     if (lane == 0 && a < S.n_nodes)
-        bspmv_tail<VT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
+        bspmv_tail<VT, XT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
 }
 
-template <typename VT, int RR>
+template <typename VT, typename XT, int RR>
 __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double (&loc)[kRmax]) {
     double beta[RR];
 #pragma unroll
     for (int r = 0; r < RR; ++r)
         beta[r] = ((go >> r) & 1) ? S.rhs[r].beta : 0.0;
     const int64_t itv = S.it;
-    gptr<const double> z = gp((const double *)S.z);
-    gptr<const double> pold = gp((const double *)((itv & 1) ? S.p1 : S.p0));
-    gptr<double> pnew = gp((itv & 1) ? S.p0 : S.p1);
+    gptr<const XT> z = gp((const XT *)S.z);
+    gptr<const XT> pold = gp((const XT *)((itv & 1) ? S.p1 : S.p0));
+    gptr<XT> pnew = gp((XT *)((itv & 1) ? S.p0 : S.p1));
     const int64_t a = (int64_t)lb * kBB + threadIdx.x;
     double acc[3][RR];
 #pragma unroll
@@ -310,9 +313,9 @@ __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double
         for (int r = 0; r < RR; ++r)
             acc[p][r] = 0.0;
     if (a < S.n_nodes) {
-        sell_node_rows<VT, RR>(gp(S.nb_ptr), gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold, beta,
-                               acc);
-        bspmv_tail<VT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
+        sell_node_rows<VT, XT, RR>(gp(S.nb_ptr), gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold,
+                                   beta, acc);
+        bspmv_tail<VT, XT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
     }
 }
 
@@ -320,7 +323,7 @@ __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double
 // groups on the node-block layout (kb_spmv_p) or one thread per node row on
 // the sliced-ELL operator (kb_spmv_sell); the right-hand side count of the
 // block's system selects a straight-line instance.
-template <int G, typename VT>
+template <int G, typename VT, typename XT>
 __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
@@ -329,10 +332,10 @@ __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const i
         return;
     double loc[kRmax] = {0.0, 0.0, 0.0, 0.0};
     switch (S.R) {
-    case 1: bspmv_lanes<G, VT, 1>(S, t.y, go, loc); break;
-    case 2: bspmv_lanes<G, VT, 2>(S, t.y, go, loc); break;
-    case 3: bspmv_lanes<G, VT, 3>(S, t.y, go, loc); break;
-    default: bspmv_lanes<G, VT, 4>(S, t.y, go, loc); break;
+    case 1: bspmv_lanes<G, VT, XT, 1>(S, t.y, go, loc); break;
+    case 2: bspmv_lanes<G, VT, XT, 2>(S, t.y, go, loc); break;
+    case 3: bspmv_lanes<G, VT, XT, 3>(S, t.y, go, loc); break;
+    default: bspmv_lanes<G, VT, XT, 4>(S, t.y, go, loc); break;
     }
     block_sum<kRmax>(loc);
     if (threadIdx.x == 0)
@@ -340,7 +343,7 @@ __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const i
             S.part1[(size_t)t.y * S.R + r] = loc[r];
 }
 
-template <typename VT>
+template <typename VT, typename XT>
 __global__ __launch_bounds__(kBB) void kb_spmv_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
@@ -349,10 +352,10 @@ __global__ __launch_bounds__(kBB) void kb_spmv_sell(BSys *__restrict__ sys, cons
         return;
     double loc[kRmax] = {0.0, 0.0, 0.0, 0.0};
     switch (S.R) {
-    case 1: bspmv_sell<VT, 1>(S, t.y, go, loc); break;
-    case 2: bspmv_sell<VT, 2>(S, t.y, go, loc); break;
-    case 3: bspmv_sell<VT, 3>(S, t.y, go, loc); break;
-    default: bspmv_sell<VT, 4>(S, t.y, go, loc); break;
+    case 1: bspmv_sell<VT, XT, 1>(S, t.y, go, loc); break;
+    case 2: bspmv_sell<VT, XT, 2>(S, t.y, go, loc); break;
+    case 3: bspmv_sell<VT, XT, 3>(S, t.y, go, loc); break;
+    default: bspmv_sell<VT, XT, 4>(S, t.y, go, loc); break;
     }
     block_sum<kRmax>(loc);
     if (threadIdx.x == 0)
@@ -391,15 +394,15 @@ __global__ __launch_bounds__(kBB) void kb_red1(BSys *__restrict__ sys, const int
     S.go = go2;
 }
 
-template <int RR>
+template <typename XT, int RR>
 __device__ __forceinline__ void bupdate(const BSys &S, int lb, int go, double (&v)[2 * kRmax]) {
     double alpha[RR];
 #pragma unroll
     for (int r = 0; r < RR; ++r)
         alpha[r] = ((go >> r) & 1) ? S.rhs[r].alpha : 0.0;
-    gptr<const double> p = gp((const double *)((S.it & 1) ? S.p0 : S.p1));  // p_i: the "new" buffer of iteration i
-    gptr<double> x = gp(S.x), rv = gp(S.r), z = gp(S.z);
-    gptr<const double> ap = gp((const double *)S.ap);
+    gptr<const XT> p = gp((const XT *)((S.it & 1) ? S.p0 : S.p1));  // p_i: the "new" buffer of iteration i
+    gptr<XT> x = gp((XT *)S.x), rv = gp((XT *)S.r), z = gp((XT *)S.z);
+    gptr<const XT> ap = gp((const XT *)S.ap);
     const int64_t i = (int64_t)lb * kBB + threadIdx.x;
     if (i >= S.n_dof)
         return;
@@ -419,16 +422,17 @@ __device__ __forceinline__ void bupdate(const BSys &S, int lb, int go, double (&
             continue;
         const int64_t q = i * RR + r;
         const double xi = xo[r] + alpha[r] * po[r];
-        const double ri = ro[r] - alpha[r] * apo[r];
-        const double zi = ri * inv;
-        x[q] = xi;
-        rv[q] = ri;
-        z[q] = zi;
+        const double ri = (double)(XT)(ro[r] - alpha[r] * apo[r]);  // values as stored
+        const double zi = (double)(XT)(ri * inv);
+        x[q] = (XT)xi;
+        rv[q] = (XT)ri;
+        z[q] = (XT)zi;
         v[2 * r] = ri * zi;
         v[2 * r + 1] = ri * ri;
     }
 }
 
+template <typename XT>
 __global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
@@ -440,10 +444,10 @@ __global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const i
     for (int k = 0; k < 2 * kRmax; ++k)
         v[k] = 0.0;
     switch (S.R) {
-    case 1: bupdate<1>(S, t.y, go, v); break;
-    case 2: bupdate<2>(S, t.y, go, v); break;
-    case 3: bupdate<3>(S, t.y, go, v); break;
-    default: bupdate<4>(S, t.y, go, v); break;
+    case 1: bupdate<XT, 1>(S, t.y, go, v); break;
+    case 2: bupdate<XT, 2>(S, t.y, go, v); break;
+    case 3: bupdate<XT, 3>(S, t.y, go, v); break;
+    default: bupdate<XT, 4>(S, t.y, go, v); break;
     }
     block_sum<2 * kRmax>(v);
     if (threadIdx.x == 0)
@@ -455,6 +459,7 @@ __global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const i
 }
 
 // Mixed refinement, step 1: xacc += x for the right-hand sides in S.want.
+template <typename XT>
 __global__ __launch_bounds__(kBB) void kb_accumulate(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     BSys &S = sys[t.x];
@@ -464,7 +469,7 @@ __global__ __launch_bounds__(kBB) void kb_accumulate(BSys *__restrict__ sys, con
     for (int r = 0; r < S.R; ++r)
         if ((S.want >> r) & 1) {
             const int64_t q = i * S.R + r;
-            S.xacc[q] += S.x[q];
+            S.xacc[q] += (double)((const XT *)S.x)[q];
         }
 }
 
@@ -562,12 +567,12 @@ inline int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
 template <typename VT>
 void launch_bspmv(int G, unsigned nb, hipStream_t s, BSys *sys, const int2 *tab) {
     switch (G) {
-    case 2: hipLaunchKernelGGL((kb_spmv_p<2, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    case 4: hipLaunchKernelGGL((kb_spmv_p<4, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    case 8: hipLaunchKernelGGL((kb_spmv_p<8, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    case 16: hipLaunchKernelGGL((kb_spmv_p<16, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    case 32: hipLaunchKernelGGL((kb_spmv_p<32, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    default: hipLaunchKernelGGL((kb_spmv_p<64, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 2: hipLaunchKernelGGL((kb_spmv_p<2, VT, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 4: hipLaunchKernelGGL((kb_spmv_p<4, VT, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 8: hipLaunchKernelGGL((kb_spmv_p<8, VT, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 16: hipLaunchKernelGGL((kb_spmv_p<16, VT, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    case 32: hipLaunchKernelGGL((kb_spmv_p<32, VT, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
+    default: hipLaunchKernelGGL((kb_spmv_p<64, VT, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
     }
 }
 
@@ -702,11 +707,12 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     // SpMV algorithmic bytes: values in the working precision, one column
     // index per 3x3 block, node-row pointers (+ the FP32 operator's 3x3
     // correction per node); per right-hand side z and p_old gathered once
-    // and p_new, Ap written
+    // and p_new, Ap written, in the vector precision (FP32 in the mixed inner
+    // solve)
     S.bytes_acc = B->d_bytes.p;
     S.spmv_bytes0 = (is_mixed(B) ? 4.0 : 8.0) * (double)m->nnz_struct + 4.0 * (double)m->nblk +
                     4.0 * (double)(m->n_nodes + 1) + (is_mixed(B) ? 36.0 * (double)m->n_nodes : 0.0);
-    S.spmv_bytes_rhs = 32.0 * (double)m->n_dof;
+    S.spmv_bytes_rhs = (is_mixed(B) ? 16.0 : 32.0) * (double)m->n_dof;
     S.nparts2 = fx::nblk(m->n_dof, kBB);
     S.want = 0;
 }
@@ -817,7 +823,11 @@ void start_slots(fx_batch *B, const std::vector<int> &slots) {
     B->d_list2.alloc(std::max<size_t>(B->d_list2.n, slots.size()));
     FX_HIP(hipMemcpyAsync(B->d_tab4.p, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice, s));
     FX_HIP(hipMemcpyAsync(B->d_list2.p, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(fx::kb_init, dim3((unsigned)t.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_tab4.p);
+    if (is_mixed(B))
+        hipLaunchKernelGGL(fx::kb_init<float>, dim3((unsigned)t.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_tab4.p);
+    else
+        hipLaunchKernelGGL(fx::kb_init<double>, dim3((unsigned)t.size()), dim3(kBB), 0, s, B->d_sys.p,
+                           B->d_tab4.p);
     hipLaunchKernelGGL(fx::kb_red2, dim3((unsigned)slots.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_list2.p);
     FX_HIP(hipGetLastError());
     FX_HIP(hipStreamSynchronize(s));  // the host tables are reused
@@ -843,15 +853,19 @@ void launch_iterations(fx_batch *B, int n) {
             }
         if (B->n1s) {
             if (is_mixed(B))
-                hipLaunchKernelGGL(fx::kb_spmv_sell<float>, dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p, B->d_tab1s.p);
+                hipLaunchKernelGGL((fx::kb_spmv_sell<float, float>), dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p,
+                                   B->d_tab1s.p);
             else
-                hipLaunchKernelGGL(fx::kb_spmv_sell<double>, dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p,
+                hipLaunchKernelGGL((fx::kb_spmv_sell<double, double>), dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p,
                                    B->d_tab1s.p);
         }
         if (B->timing)
             FX_HIP(hipEventRecord(B->ev[2 * k + 1], s));
         hipLaunchKernelGGL(fx::kb_red1, dim3(B->nrun), dim3(kBB), 0, s, B->d_sys.p, B->d_run.p);
-        hipLaunchKernelGGL(fx::kb_update, dim3(B->n2), dim3(kBB), 0, s, B->d_sys.p, B->d_tab2.p);
+        if (is_mixed(B))
+            hipLaunchKernelGGL(fx::kb_update<float>, dim3(B->n2), dim3(kBB), 0, s, B->d_sys.p, B->d_tab2.p);
+        else
+            hipLaunchKernelGGL(fx::kb_update<double>, dim3(B->n2), dim3(kBB), 0, s, B->d_sys.p, B->d_tab2.p);
         hipLaunchKernelGGL(fx::kb_red2, dim3(B->nrun), dim3(kBB), 0, s, B->d_sys.p, B->d_run.p);
     }
     if (B->timing)
@@ -948,7 +962,8 @@ bool advance_phases(fx_batch *B) {
     FX_HIP(hipMemcpyAsync(B->d_tab3.p, t3.data(), t3.size() * sizeof(int2), hipMemcpyHostToDevice, s));
     FX_HIP(hipMemcpyAsync(B->d_tab4.p, t4.data(), t4.size() * sizeof(int2), hipMemcpyHostToDevice, s));
     FX_HIP(hipMemcpyAsync(B->d_list.p, outer.data(), outer.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(fx::kb_accumulate, dim3((unsigned)t4.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_tab4.p);
+    hipLaunchKernelGGL(fx::kb_accumulate<float>, dim3((unsigned)t4.size()), dim3(kBB), 0, s, B->d_sys.p,
+                       B->d_tab4.p);  // outer passes run for mixed batches only
     for (int g = 0; g < fx_batch::kGroups; ++g)
         if (seg[g + 1] > seg[g])
             fx::launch_bouter(2 << g, (unsigned)(seg[g + 1] - seg[g]), s, B->d_sys.p, B->d_tab3.p + seg[g]);
@@ -1121,7 +1136,7 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
                 inter[i * n_rhs + r] = rhs[(int64_t)r * n + i];
         sl.b.upload(inter, s);
         for (auto *v : {&sl.x, &sl.r, &sl.z, &sl.p0, &sl.p1, &sl.ap})
-            v->alloc(nv);
+            v->alloc(is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed inner solve stores floats
         if (is_mixed(B)) {
             sl.rin.upload(inter, s);  // x = 0: the first residual is b
             sl.xacc.alloc(nv);

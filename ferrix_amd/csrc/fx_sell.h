@@ -32,11 +32,12 @@ __device__ __forceinline__ gptr<T> gp(T *p) {
 }
 
 // acc[p][r] += sum over the row's blocks of K(a,p; c,q) * (z + beta_r p_old)(c,q,r)
-// for r < RR right-hand sides (vectors interleaved: v[dof*RR + r]).
-template <typename VT, int RR>
+// for r < RR right-hand sides (vectors interleaved: v[dof*RR + r], stored as
+// XT: double, or float in the mixed-precision inner solve).
+template <typename VT, typename XT, int RR>
 __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> nb_ptr, gptr<const int32_t> sl_ptr,
                                                gptr<const int32_t> scol, gptr<const VT> sval, int64_t a,
-                                               gptr<const double> z, gptr<const double> pold,
+                                               gptr<const XT> z, gptr<const XT> pold,
                                                const double (&beta)[RR], double (&acc)[3][RR]) {
     const int64_t s = a / kSlice;
     const int lane = (int)(a % kSlice);
@@ -64,7 +65,8 @@ __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> nb_ptr, gptr<
 #pragma unroll
                 for (int r = 0; r < RR; ++r) {
                     const int64_t at = (3 * (int64_t)c[u] + q) * RR + r;
-                    x[q][r] = z[at] + beta[r] * pold[at];
+                    // p_new as its owner stores it (rounded to XT)
+                    x[q][r] = (double)(XT)((double)z[at] + beta[r] * (double)pold[at]);
                 }
 #pragma unroll
             for (int r = 0; r < RR; ++r)

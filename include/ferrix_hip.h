@@ -139,8 +139,9 @@ fx_status fx_apply_penalty(fx_mesh *mesh, int64_t n_bc, const int64_t *dofs, con
 /* ---- solver seam ------------------------------------------------------ */
 typedef enum fx_precision {
     FX_PREC_FP64 = 0,  /* Jacobi-PCG in FP64 (the reference's IterativeSolver)   */
-    FX_PREC_MIXED = 1  /* FP32-stored K and Jacobi inside FP64 iterative
-                          refinement; FP64 residuals and norms               */
+    FX_PREC_MIXED = 1  /* FP32-stored K inside FP64 iterative refinement;
+                          FP64 residuals and norms.  fx_solve keeps FP64
+                          Krylov vectors, a batch FP32 ones (dots in FP64) */
 } fx_precision;
 
 typedef enum fx_stop {
@@ -307,8 +308,9 @@ fx_status fx_batch_time_spmv(fx_batch *batch, int32_t on);
  * (ms), and the summed algorithmic bytes of all launches: per system whose
  * load cases are running, the operator values in the working precision, one
  * 4-byte column index per 3x3 block, node-row pointers (and the FP32
- * operator's 3x3 correction per node), plus 32 bytes per DOF and running
- * load case (z and p_old gathered once, p_new and Ap written). */
+ * operator's 3x3 correction per node), plus per DOF and running load case
+ * z and p_old gathered once and p_new and Ap written (32 bytes in FP64, 16
+ * in the mixed inner solve, whose Krylov vectors are FP32). */
 fx_status fx_batch_spmv_stats(fx_batch *batch, int64_t *launches, int64_t *timed, double *ms_timed,
                               double *bytes_total);
 
