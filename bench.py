@@ -384,7 +384,7 @@ def run_batch(a):
                 "frac": round(st_["achieved_GBps"] / HBM_PEAK_GBPS, 4),
                 "traffic": (counter_traffic("kb_spmv_sell_float_batch512_counters.md", st_["bytes_per_launch"])
                             if head == "mixed_batched" and a.meshes == 512 else None),
-                "kernel": "kb_spmv_sell<float>" if head == "mixed_batched" else "kb_spmv_sell<double>",
+                "kernel": "kb_spmv_sell<float, float>" if head == "mixed_batched" else "kb_spmv_sell<double, double>",
                 "us_per_launch": round(st_["us_per_launch"], 2), "bytes_per_launch": round(st_["bytes_per_launch"]),
                 "launches": st_["launches"]}
     print(f"# roofline {json.dumps(roof)}", flush=True)
