@@ -81,6 +81,7 @@ SIGNATURES = {
     "fx_assemble": (i32, [P, i32, P, P]),
     "fx_internal_force": (i32, [P, i32, P]),
     "fx_matrix_get": (i32, [P, P, P, P]),
+    "fx_stress_strain": (i32, [P, P, P]),
     "fx_apply_penalty": (i32, [P, i64, P, P, P, P]),
     "fx_solve_opts_default": (None, [P]),
     "fx_solve": (i32, [P, P, P, P, P]),

@@ -267,6 +267,7 @@ fx_status fx_ctx_create(int device, fx_ctx **out) {
     T.n20.upload(t20.N, ctx->stream);
     T.dn20.upload(t20.dN, ctx->stream);
     T.w20.upload(t20.w, ctx->stream);
+    T.dn20n.upload(fx::c3d20_node_dn(), ctx->stream);
     FX_HIP(hipStreamSynchronize(ctx->stream));
     *out = ctx;
     ctx = nullptr;
@@ -299,7 +300,7 @@ void fx_ctx_destroy(fx_ctx *ctx) {
         delete m;
     ctx->meshes.clear();
     for (auto *b : {&ctx->tables.n4, &ctx->tables.dn4, &ctx->tables.w4, &ctx->tables.n20, &ctx->tables.dn20,
-                    &ctx->tables.w20})
+                    &ctx->tables.w20, &ctx->tables.dn20n})
         b->release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -634,6 +635,26 @@ fx_status fx_internal_force(fx_mesh *m, int32_t nlgeom, double *f_int_out) {
                               m->ctx->stream));
         FX_HIP(hipStreamSynchronize(m->ctx->stream));
     }
+    FX_API_END(m->ctx)
+}
+
+fx_status fx_stress_strain(fx_mesh *m, double *stress_out, double *strain_out) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(stress_out && strain_out, "fx_stress_strain: NULL argument");
+    set_device(m->ctx);
+    fx::reset_status(m);
+    fx::launch_recover(m);
+    fx::check_status(m, "StaticStep::calculate_stress_strain");
+    DBuf<double> S, E;
+    S.alloc(std::max<int64_t>(1, 6 * m->n_nodes));
+    E.alloc(std::max<int64_t>(1, 6 * m->n_nodes));
+    fx::launch_gather_recover(m, S.p, E.p);
+    hipStream_t s = m->ctx->stream;
+    FX_HIP(hipMemcpyAsync(stress_out, S.p, 6 * m->n_nodes * sizeof(double), hipMemcpyDeviceToHost, s));
+    FX_HIP(hipMemcpyAsync(strain_out, E.p, 6 * m->n_nodes * sizeof(double), hipMemcpyDeviceToHost, s));
+    FX_HIP(hipStreamSynchronize(s));
     FX_API_END(m->ctx)
 }
 

@@ -76,6 +76,35 @@ __global__ void k_gather_f(int64_t n_dof, const int32_t *__restrict__ cptr, cons
     f[d] = s;
 }
 
+// Nodal averages of the recovered strain and stress: the (element, node)
+// contributions of node a in element order (the f_int contribution lists,
+// whose bases count 3 values per contribution against 12 here), divided by
+// their count (static_step.rs:362-377).
+__global__ void k_gather_recover(int64_t n_nodes, const int32_t *__restrict__ cptr,
+                                 const int32_t *__restrict__ cbase, const double *__restrict__ se,
+                                 double *__restrict__ S, double *__restrict__ E) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_nodes)
+        return;
+    double sum[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+        sum[k] = 0.0;
+    const int c0 = cptr[a], c1 = cptr[a + 1];
+    for (int c = c0; c < c1; ++c) {
+        const double *v = se + 4 * (int64_t)cbase[c];
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            sum[k] += v[k];
+    }
+    const double cnt = (double)(c1 - c0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        E[6 * a + k] = c1 > c0 ? sum[k] / cnt : 0.0;
+        S[6 * a + k] = c1 > c0 ? sum[6 + k] / cnt : 0.0;
+    }
+}
+
 __global__ void k_jacobi(int64_t n, const int32_t *__restrict__ diag_slot, const double *__restrict__ kval,
                          double *__restrict__ inv, float *__restrict__ inv32) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,6 +284,12 @@ void launch_assemble(fx_mesh *m) {
 void launch_gather_f(fx_mesh *m) {
     hipLaunchKernelGGL(k_gather_f, dim3(grid1(m->n_dof, 256)), dim3(256), 0, m->ctx->stream, m->n_dof,
                        m->nod_cptr.p, m->nod_cbase.p, m->fe.p, m->f_int.p);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_gather_recover(fx_mesh *m, double *S, double *E) {
+    hipLaunchKernelGGL(k_gather_recover, dim3(grid1(m->n_nodes, 256)), dim3(256), 0, m->ctx->stream, m->n_nodes,
+                       m->nod_cptr.p, m->nod_cbase.p, m->se.p, S, E);
     FX_HIP(hipGetLastError());
 }
 

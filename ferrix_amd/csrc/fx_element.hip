@@ -383,6 +383,106 @@ void launch_types(fx_mesh *m, const double *u, int nlgeom, const double *const *
 
 }  // namespace
 
+// Strain and stress at node i of element e (element.rs:526-564 with the
+// recovery of static_step.rs:271-379): B at the node's local coordinates on
+// the undeformed geometry, eps = B u_el (the thermal-strain increment is
+// identically zero: T never changes), sigma = D(T_node) eps.  One thread per
+// (element, node); se[(e NN + i) 12 + k] = eps[k], se[... + 6 + k] = sigma[k].
+// Sums run in the oracle's dense index order with B's and D's structural
+// zeros skipped (exact, as in the element kernels).
+template <int NN>
+__global__ __launch_bounds__(128) void k_recover(int64_t n_el, const int32_t *__restrict__ conn,
+                                                 const int32_t *__restrict__ mat, const int64_t *__restrict__ gpos,
+                                                 const double *__restrict__ xyz, const double *__restrict__ temp,
+                                                 const double *__restrict__ u, const MatLut *__restrict__ mats,
+                                                 const double *__restrict__ dn_nodes, double *__restrict__ se,
+                                                 DevStatus *__restrict__ st) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n_el * NN)
+        return;
+    const int64_t e = gid / NN;
+    const int i = (int)(gid % NN);
+    double xc[3 * NN], ue[3 * NN];
+    const int32_t *cn = conn + NN * e;
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+        const int64_t nd = cn[a];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xc[3 * a + c] = xyz[3 * nd + c];
+            ue[3 * a + c] = u[3 * nd + c];
+        }
+    }
+    // C3D4: dN is the same at every point; C3D20: the table row of node i
+    const double *dN = NN == 4 ? dn_nodes : dn_nodes + 3 * NN * i;
+    double Ji[9], det;
+    if (!jac<NN>(dN, xc, Ji, det)) {
+        fail(st, FX_ERR_SINGULAR_JACOBIAN, gpos[e]);
+        return;
+    }
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+        double dg[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                sum += Ji[3 * r + k] * dN[NN * k + a];
+            dg[r] = sum;
+        }
+        const double dx = dg[0], dy = dg[1], dz = dg[2];
+        const double ux = ue[3 * a], uy = ue[3 * a + 1], uz = ue[3 * a + 2];
+        e0 += dx * ux;
+        e1 += dy * uy;
+        e2 += dz * uz;
+        e3 += dy * ux;
+        e3 += dx * uy;
+        e4 += dz * uy;
+        e4 += dy * uz;
+        e5 += dz * ux;
+        e5 += dx * uz;
+    }
+    double d1, d2, d3;
+    if (!d_coeffs(mats[mat[e]], temp[cn[i]], d1, d2, d3)) {
+        fail(st, FX_ERR_INVALID, gpos[e]);
+        return;
+    }
+    double *out = se + gid * 12;
+    out[0] = e0;
+    out[1] = e1;
+    out[2] = e2;
+    out[3] = e3;
+    out[4] = e4;
+    out[5] = e5;
+    out[6] = ((0.0 + d1 * e0) + d2 * e1) + d2 * e2;
+    out[7] = ((0.0 + d2 * e0) + d1 * e1) + d2 * e2;
+    out[8] = ((0.0 + d2 * e0) + d2 * e1) + d1 * e2;
+    out[9] = 0.0 + d3 * e3;
+    out[10] = 0.0 + d3 * e4;
+    out[11] = 0.0 + d3 * e5;
+}
+
+void launch_recover(fx_mesh *m) {
+    DevTables *t = &m->ctx->tables;
+    hipStream_t s = m->ctx->stream;
+    m->se.alloc(std::max<int64_t>(1, 4 * m->fe_total));
+    if (m->blk4.n_el) {
+        const int64_t n = 4 * m->blk4.n_el;
+        hipLaunchKernelGGL(k_recover<4>, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, m->blk4.n_el,
+                           m->blk4.conn.p, m->blk4.mat.p, m->blk4.gpos.p, m->xyz.p, m->temp.p, m->u.p, m->mats.p,
+                           t->dn4.p, m->se.p + 4 * m->blk4.fe_offset, m->status.p);
+    }
+    if (m->blk20.n_el) {
+        const int64_t n = 20 * m->blk20.n_el;
+        hipLaunchKernelGGL(k_recover<20>, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, m->blk20.n_el,
+                           m->blk20.conn.p, m->blk20.mat.p, m->blk20.gpos.p, m->xyz.p, m->temp.p, m->u.p,
+                           m->mats.p, t->dn20n.p, m->se.p + 4 * m->blk20.fe_offset, m->status.p);
+    }
+    FX_HIP(hipGetLastError());
+}
+
 void launch_element(fx_mesh *m, bool want_k, bool want_f, bool nlgeom) {
     DevTables *t = &m->ctx->tables;
     const double *tab4[3] = {t->n4.p, t->dn4.p, t->w4.p};

@@ -94,6 +94,7 @@ struct GaussTable {
     std::vector<double> w;   // ngp
 };
 GaussTable gauss_table(int nn);  // fx_shape.cpp
+std::vector<double> c3d20_node_dn();  // fx_shape.cpp: dN at the 20 node positions
 
 // device-side status word written by kernels
 struct DevStatus {
@@ -138,6 +139,7 @@ struct PcgGraph {
 // Gauss tables in device memory, one set per context (uploaded at creation).
 struct DevTables {
     DBuf<double> n4, dn4, w4, n20, dn20, w20;
+    DBuf<double> dn20n;  // dN at the C3D20 node positions (stress recovery)
 };
 
 }  // namespace fx
@@ -188,6 +190,7 @@ struct fx_mesh {
     fx::DBuf<uint32_t> counters;
     // vectors
     fx::DBuf<double> u, f_int, f_ext, rhs, du, x, r, z, p0, p1, ap, tmp;
+    fx::DBuf<double> se;                 // stress recovery: 12 values per (element, node)
     fx::DBuf<double> red2;               // 2 doubles: norm results
     double last_max_diag = 0.0;
     bool k_valid = false;
@@ -203,6 +206,8 @@ struct fx_mesh {
 namespace fx {
 // launchers (defined in the .hip translation units)
 void launch_element(fx_mesh *m, bool want_k, bool want_f, bool nlgeom);
+void launch_recover(fx_mesh *m);         // per (element, node) strain and stress at the resident u
+void launch_gather_recover(fx_mesh *m, double *S, double *E);  // nodal averages (device arrays)
 void launch_assemble(fx_mesh *m);
 void launch_gather_f(fx_mesh *m);
 void launch_jacobi(fx_mesh *m, bool want32);

@@ -139,4 +139,17 @@ GaussTable gauss_table(int nn) {
     return t;
 }
 
+// dN_local (3 x 20, row-major) at each of the 20 C3D20 node positions
+// (c3d20.rs:61-84), for the nodal stress recovery (element.rs:526-564).
+std::vector<double> c3d20_node_dn() {
+    static const double loc[60] = {-1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, -1, -1, -1, 1, 1, -1, 1, 1, 1,
+                                   1,  -1, 1,  1, 0, -1, -1, 1, 0,  -1, 0, 1, -1, -1, 0, -1, 0, -1, 1, 1,
+                                   0,  1,  0,  1, 1, -1, 0,  1, -1, -1, 0, 1, -1, 0,  1, 1, 0,  -1, 1, 0};
+    std::vector<double> out(20 * 60);
+    double N[20];
+    for (int i = 0; i < 20; ++i)
+        c3d20_eval(loc[3 * i], loc[3 * i + 1], loc[3 * i + 2], N, &out[60 * i]);
+    return out;
+}
+
 }  // namespace fx

@@ -4,7 +4,8 @@ assembly, solver and step seams.
 * :class:`Context` -- one HIP device and stream (``fx_ctx``).
 * :class:`DeviceMesh` -- a resident mesh (``fx_mesh``): ``assemble``
   (``Assembler::assemble``, ``src/solver/assembler.rs:37-151``),
-  ``internal_force`` (``:157-241``), ``apply_penalty``
+  ``internal_force`` (``:157-241``), ``stress_strain``
+  (``static_step.rs:271-379``), ``apply_penalty``
   (``src/solver/step/static_step.rs:183-195``), ``solve``
   (``Solver::solve``, ``src/solver/solvers/mod.rs:18-44``) and
   ``static_step`` (``StaticStep::solve``, ``static_step.rs:47-267``).
@@ -183,6 +184,15 @@ class DeviceMesh:
         f = np.zeros(self.n_dof)
         self.ctx.check(self.lib.fx_internal_force(self.h, int(nlgeom), _p(f)))
         return f
+
+    def stress_strain(self):
+        """Nodal (stress, strain), each (n_nodes, 6), at the resident u
+        (StaticStep::calculate_stress_strain, static_step.rs:271-379)."""
+        n = self.n_dof // 3
+        S = np.zeros((n, 6))
+        E = np.zeros((n, 6))
+        self.ctx.check(self.lib.fx_stress_strain(self.h, _p(S), _p(E)))
+        return S, E
 
     def matrix(self):
         """Filtered CSR (row_ptr int64, col int32, values float64)."""

@@ -120,6 +120,17 @@ fx_status fx_assemble(fx_mesh *mesh, int32_t nlgeom, double *max_diag_out, int64
 fx_status fx_internal_force(fx_mesh *mesh, int32_t nlgeom, double *f_int_out);
 
 /*
+ * Nodal stress and strain at the resident u (StaticStep::calculate_stress_strain,
+ * static_step.rs:271-379, with Element::calculate_stress_strain_at_local_coords,
+ * element.rs:526-564): per element and node, B at the node's local coordinates
+ * on the undeformed geometry, strain = B u_el, stress = D(T_node) strain;
+ * summed over the elements of each node in element order and divided by
+ * their number.  stress_out, strain_out: n_nodes x 6 (host), Voigt
+ * [xx, yy, zz, xy, yz, zx], engineering shear; nodes of no element get 0.
+ */
+fx_status fx_stress_strain(fx_mesh *mesh, double *stress_out, double *strain_out);
+
+/*
  * Filtered CSR of the resident K (the sprs CsMat the reference hands to the
  * solver, static_step.rs:198): row_ptr (n_dof+1), col_idx (nnz), values
  * (nnz, may be NULL).  Columns ascending per row.

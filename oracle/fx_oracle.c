@@ -575,6 +575,93 @@ int fxo_internal_force(int64_t n_el, const int64_t *el_ptr, const int32_t *conn,
 }
 
 /*
+ * Nodal strain and stress recovery (static_step.rs:271-379,
+ * element.rs:526-564): for every element, in the order given, and every
+ * node i of it, B at the node's local coordinates on the undeformed
+ * geometry, strain = B u_el (the thermal-strain increment is identically
+ * zero: T never changes, state.rs:47-78), stress = D(T_node) strain; both
+ * summed per node and divided by the number of elements containing the
+ * node.  Nodes of no element stay zero.  S and E are n_nodes x 6, Voigt
+ * order [xx, yy, zz, xy, yz, zx] with engineering shear.
+ */
+static const double C3D4_NODES[12] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1};  /* c3d4.rs:53-61 */
+static const double C3D20_NODES[60] = {                                        /* c3d20.rs:61-84 */
+    -1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, -1, -1, -1, 1, 1, -1, 1, 1, 1, 1, -1, 1, 1,
+    0,  -1, -1, 1, 0,  -1, 0, 1, -1, -1, 0, -1, 0,  -1, 1, 1, 0,  1, 0, 1, 1, -1, 0, 1,
+    -1, -1, 0,  1, -1, 0,  1, 1, 0,  -1, 1, 0};
+
+int fxo_stress_strain(int64_t n_nodes, int64_t n_el, const int64_t *el_ptr, const int32_t *conn,
+                      const int32_t *mat, const double *xyz, const double *Tnode, const double *u,
+                      const int32_t *n_e, const double *t_e, const double *v_e, const int32_t *n_nu,
+                      const double *t_nu, const double *v_nu, double *S, double *E, int64_t *bad_el)
+{
+    mat_table mt = {n_e, t_e, v_e, n_nu, t_nu, v_nu};
+    double X[60], ue[60], N[20], dN[60], dg[60], Jinv[9], det, D[36], B[6 * 60];
+    int32_t *count = (int32_t *)calloc((size_t)n_nodes, sizeof(int32_t));
+    if (!count)
+        return FXO_ERR_OOM;
+    for (int64_t i = 0; i < 6 * n_nodes; ++i)
+        S[i] = E[i] = 0.0;
+    for (int64_t e = 0; e < n_el; ++e) {
+        const int32_t *nodes = conn + el_ptr[e];
+        int nn = (int)(el_ptr[e + 1] - el_ptr[e]);
+        if (nn != 4 && nn != 20) {
+            free(count);
+            return FXO_ERR_INVALID;
+        }
+        const double *loc = nn == 4 ? C3D4_NODES : C3D20_NODES;
+        int nd = 3 * nn;
+        gather3(xyz, nodes, nn, X);
+        gather3(u, nodes, nn, ue);
+        for (int i = 0; i < nn; ++i) {
+            shape(nn, loc[3 * i], loc[3 * i + 1], loc[3 * i + 2], N, dN);
+            if (!jacobian(nn, dN, X, Jinv, &det)) {
+                free(count);
+                if (bad_el)
+                    *bad_el = e;
+                return FXO_ERR_SINGULAR_JACOBIAN;
+            }
+            for (int r = 0; r < 3; ++r)
+                for (int j = 0; j < nn; ++j) {
+                    double sum = 0.0;
+                    for (int k = 0; k < 3; ++k)
+                        sum += Jinv[3 * r + k] * dN[k * nn + j];
+                    dg[r * nn + j] = sum;
+                }
+            build_b(nn, dg, B);
+            double eps[6], sig[6];
+            for (int r = 0; r < 6; ++r) {
+                double sum = 0.0;
+                for (int j = 0; j < nd; ++j)
+                    sum += B[r * nd + j] * ue[j];
+                eps[r] = sum;
+            }
+            material_d(&mt, mat[e], Tnode[nodes[i]], D);
+            for (int r = 0; r < 6; ++r) {
+                double sum = 0.0;
+                for (int k = 0; k < 6; ++k)
+                    sum += D[6 * r + k] * eps[k];
+                sig[r] = sum;
+            }
+            for (int r = 0; r < 6; ++r) {
+                S[6 * (int64_t)nodes[i] + r] += sig[r];
+                E[6 * (int64_t)nodes[i] + r] += eps[r];
+            }
+        }
+        for (int i = 0; i < nn; ++i)
+            count[nodes[i]] += 1;
+    }
+    for (int64_t n = 0; n < n_nodes; ++n)
+        if (count[n] > 0)
+            for (int r = 0; r < 6; ++r) {
+                S[6 * n + r] /= (double)count[n];
+                E[6 * n + r] /= (double)count[n];
+            }
+    free(count);
+    return FXO_OK;
+}
+
+/*
  * Triplets -> CSR, summing duplicates in insertion order and sorting the
  * columns of each row ascending (the sprs CsMat invariant).  Counting sort by
  * row, then a stable insertion/merge sort by column within each row.

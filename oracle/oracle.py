@@ -55,6 +55,7 @@ def lib():
         L.fxo_element_thermal_flat.argtypes = [i32, P, P, P, P, P, P, P, P, P, i32, i32, P, P, dbl, P]
         L.fxo_assemble.argtypes = [i64, P, P, P, P, P, P, P, P, P, P, P, P, dbl, P, P, P, P, P, P]
         L.fxo_internal_force.argtypes = [i64, P, P, P, P, P, P, P, P, P, P, P, P, P, i64, P, P]
+        L.fxo_stress_strain.argtypes = [i64, i64, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]
         L.fxo_triplets_to_csr.argtypes = [i64, i64, P, P, P, P, P, P, P]
         L.fxo_pcg.argtypes = [i64, P, P, P, P, P, dbl, i64, i32, i32, P, P]
         L.fxo_spmv.argtypes = [i64, P, P, P, P, P, i32]
@@ -220,6 +221,22 @@ def internal_force(mesh, Tnode, u, ucfg=None):
     if rc != OK:
         raise OracleError(rc, f"internal force failed rc={rc} element={int(bad[0])}")
     return f
+
+
+def stress_strain(mesh, Tnode, u):
+    """Nodal stress and strain (n_nodes x 6 each, Voigt xx yy zz xy yz zx),
+    averaged over the elements of each node (static_step.rs:271-379)."""
+    n = len(mesh.xyz) if mesh.xyz.ndim == 2 else len(mesh.xyz) // 3
+    S = np.zeros((n, 6))
+    E = np.zeros((n, 6))
+    bad = np.zeros(1, np.int64)
+    rc = lib().fxo_stress_strain(n, mesh.n_el, _p(mesh.el_ptr), _p(mesh.conn), _p(mesh.mat), _p(mesh.xyz),
+                                 _p(np.ascontiguousarray(Tnode, np.float64)),
+                                 _p(np.ascontiguousarray(u, np.float64)),
+                                 *[_p(a) for a in _mat_args(mesh)], _p(S), _p(E), _p(bad))
+    if rc != OK:
+        raise OracleError(rc, f"stress recovery failed rc={rc} element={int(bad[0])}")
+    return S, E
 
 
 def assemble_csr(mesh, Tnode, u=None, bc_dofs=None, penalty_scale=1.0e6):

@@ -350,3 +350,76 @@ def test_golden_fixture_metadata(name):
         u, _ = O.run_job(gen, solver_override=SOLVER_DIRECT)
         g = np.load(os.path.join(ROOT, "tests", "golden", "cfg1_oracle.npz"))["u"]
         assert np.max(np.abs(u - g)) <= 1e-14 * np.max(np.abs(g))
+
+
+def _voigt_field(eps):
+    """Displacement gradient of a linear field with engineering strains eps."""
+    return np.array([[eps[0], 0.5 * eps[3], 0.5 * eps[5]],
+                     [0.5 * eps[3], eps[1], 0.5 * eps[4]],
+                     [0.5 * eps[5], 0.5 * eps[4], eps[2]]])
+
+
+@pytest.mark.parametrize("nn", [4, 20])
+def test_stress_recovery_patch(nn):
+    """A linear displacement field gives every node the constant strain and
+    sigma = D eps (C3D4 and jittered C3D20 reproduce linear fields exactly)."""
+    job = meshgen.cfg1_c3d4_cantilever() if nn == 4 else meshgen.cfg2_c3d20_block(seed=9, nx=3, ny=2, nz=2)
+    eps = np.array([1e-3, -2e-4, 3e-4, 5e-4, -1e-4, 2e-4])
+    X = job.mesh.xyz.reshape(-1, 3)
+    u = (X @ _voigt_field(eps).T).ravel()
+    S, E = O.stress_strain(job.mesh, job.node_temp, u)
+    E_mod, nu = float(job.mesh.lut_v_e[0, 0]), float(job.mesh.lut_v_nu[0, 0])
+    assert np.max(np.abs(E - eps)) < 1e-15
+    assert np.max(np.abs(S - O.d_matrix(E_mod, nu) @ eps)) < 1e-9 * np.max(np.abs(O.d_matrix(E_mod, nu) @ eps))
+
+
+def test_stress_recovery_single_element_node_by_differences():
+    """A block corner belongs to one element: its recovered strain is the
+    symmetric gradient of the element's isoparametric displacement at that
+    corner, computed here by central differences of the shape functions N
+    (not the derivative tables) through the geometric map."""
+    job = meshgen.cfg2_c3d20_block(seed=4, nx=2, ny=2, nz=2)
+    rng = np.random.default_rng(8)
+    u = rng.normal(scale=1e-3, size=job.n_dof)
+    S, E = O.stress_strain(job.mesh, job.node_temp, u)
+    m = job.mesh
+    e = 0  # the element holding node index 0 (block corner i=j=k=0)
+    nodes = m.conn[m.el_ptr[e]:m.el_ptr[e + 1]]
+    assert nodes[0] == 0 and np.sum(m.conn == 0) == 1
+    X = m.xyz.reshape(-1, 3)[nodes]
+    U = u.reshape(-1, 3)[nodes]
+    h = 1e-6
+    dX, dU = np.zeros((3, 3)), np.zeros((3, 3))
+    for k in range(3):
+        xi = np.array([-1.0, -1.0, -1.0])
+        a, b = xi.copy(), xi.copy()
+        a[k] += h
+        b[k] -= h
+        Na, Nb = O.shape_c3d20(*a)[0], O.shape_c3d20(*b)[0]
+        dX[:, k] = (Na - Nb) @ X / (2 * h)
+        dU[:, k] = (Na - Nb) @ U / (2 * h)
+    grad = dU @ np.linalg.inv(dX)
+    eps = np.array([grad[0, 0], grad[1, 1], grad[2, 2], grad[0, 1] + grad[1, 0], grad[1, 2] + grad[2, 1],
+                    grad[2, 0] + grad[0, 2]])
+    assert np.max(np.abs(E[0] - eps)) < 1e-8 * np.max(np.abs(eps))
+    D = O.d_matrix(float(m.lut_v_e[0, 0]), float(m.lut_v_nu[0, 0]))
+    assert np.max(np.abs(S[0] - D @ E[0])) < 1e-9 * np.max(np.abs(S[0]))
+
+
+def test_stress_recovery_uses_node_temperature():
+    """update_state is called with the node's own temperature
+    (static_step.rs:345-346), so with a temperature-dependent E every node's
+    averaged stress is D(T_node) times its averaged strain."""
+    job = meshgen.cfg2_c3d20_block(seed=6, nx=2, ny=2, nz=1)
+    m = job.mesh
+    m.lut_n_e[0] = 2
+    m.lut_t_e[0, :2] = [0.0, 100.0]
+    m.lut_v_e[0, :2] = [70000.0, 35000.0]
+    rng = np.random.default_rng(3)
+    T = rng.uniform(0.0, 100.0, size=len(m.xyz))
+    u = rng.normal(scale=1e-3, size=job.n_dof)
+    S, E = O.stress_strain(m, T, u)
+    nu = float(m.lut_v_nu[0, 0])
+    for n in range(len(T)):
+        D = O.d_matrix(70000.0 + (35000.0 - 70000.0) * (T[n] / 100.0), nu)
+        assert np.allclose(S[n], D @ E[n], rtol=1e-12, atol=1e-12 * np.abs(S).max())
