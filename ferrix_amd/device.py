@@ -194,6 +194,17 @@ class DeviceMesh:
         self.ctx.check(self.lib.fx_stress_strain(self.h, _p(S), _p(E)))
         return S, E
 
+    def reaction_forces(self, dofs, f_ext=None, nlgeom: bool = False) -> np.ndarray:
+        """RF at the constrained DOFs: f_int(u) - f_ext there, at the
+        resident u (the reference writes no RF; with a converged solution
+        they balance the applied loads, sum RF = -sum f_ext per direction)."""
+        f = self.internal_force(nlgeom)
+        dofs = np.asarray(dofs, np.int64)
+        rf = f[dofs]
+        if f_ext is not None:
+            rf = rf - np.asarray(f_ext, np.float64)[dofs]
+        return rf
+
     def matrix(self):
         """Filtered CSR (row_ptr int64, col int32, values float64)."""
         rp = np.zeros(self.n_dof + 1, np.int64)

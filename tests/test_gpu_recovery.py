@@ -55,3 +55,22 @@ def test_recovery_after_step(ctx):
     root = X[:, 0] < 20.0
     top, bottom = root & (X[:, 2] > 15.0), root & (X[:, 2] < 5.0)
     assert S[top, 0].mean() > 0.0 > S[bottom, 0].mean()
+
+
+def test_reaction_forces_balance_the_load(ctx):
+    """RF = f_int(u) at the clamped DOFs, bitwise the oracle's f_int of the
+    same u, and in equilibrium with the applied load: the rigid translations
+    are in K's null space, so sum f_int = 0 per direction and the reactions
+    carry exactly minus the applied total (penalty BCs, converged step)."""
+    job = meshgen.cfg1_c3d4_cantilever()
+    st = job.steps[0]
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    u, _ = run_job(ctx, job, mesh=dm)
+    bc = np.unique(st.bc_dof)
+    rf = dm.reaction_forces(bc)
+    fo = O.internal_force(job.mesh, job.node_temp, u)
+    assert np.array_equal(rf, fo[bc])
+    applied = np.zeros(3)
+    np.add.at(applied, st.load_dof % 3, st.load_val)
+    total = np.array([rf[bc % 3 == d].sum() for d in range(3)])
+    assert np.allclose(total, -applied, rtol=1e-6, atol=1e-6 * np.abs(applied).max())
