@@ -532,7 +532,7 @@ def main():
     else:
         a.steps = 1 if a.steps is None else a.steps
         a.warmup = 1 if a.warmup is None else a.warmup
-        a.arms = a.arms or "fp64_serial,mixed_batched"
+        a.arms = a.arms or "fp64_serial,mixed_batched,fp64_batched"
         run_batch(a)
 
 

@@ -12,12 +12,28 @@
 // recurrences and the stop rule are those of fx_pcg.hip (the reference's
 // Jacobi-PCG, src/solver/solvers/iterative.rs:26-121).
 //
-// Mixed precision: FP64 iterative refinement per slot around the FP32-stored
-// K.  The inner PCG (FP32 values, FP64 vectors) reduces the current residual
-// by inner_tol; the correction is added to the FP64 solution and an FP64
-// residual pass decides, per right-hand side, done or another inner solve.
-// Slots move between phases independently, and new slots join between chunks
-// of iterations (continuous batching).
+// Mixed precision: PCG on the FP32-stored K (with its row-sum correction
+// blocks) and FP32 Krylov vectors, with reliable updates of the residual
+// (Sleijpen & van der Vorst, "Reliable updated residuals in hybrid Bi-CG
+// methods", Computing 56, 1996; the mixed-precision CG form of Clark et al.,
+// "Solving lattice QCD systems of equations using mixed precision solvers on
+// GPUs", Comput. Phys. Commun. 181, 2010).  When the iterated residual of a
+// slot's load case has fallen below inner_tol times its value at the last
+// replacement (or meets the stop rule), the slot spends one iteration on a
+// replacement instead of a Krylov step:
+//   SpMV launch:  r64 = b - K64 (xacc + x), in FP64, with the true stop
+//                 criterion's partial sums;
+//   kb_red1:      stop rule on r64 (and the rounding-floor test);
+//   kb_update:    xacc += x, x = 0, r = r64, z = r inv;
+//   kb_red2:      beta = (r.z)/(r.z)_old and the correction that makes the
+//                 kept direction p orthogonal to the new r again,
+//                 p <- p - ((r.p)/(r.z)) z, folded into the next p update as
+//                 p_new = gamma z + beta p_old.
+// The Krylov direction survives the replacement when the new residual is
+// within 10% of the iterated one, so a well-conditioned system keeps its
+// convergence rate; otherwise the solve restarts from p = z, as a restarted
+// refinement would.  Slots run independently,
+// and new slots join between chunks of iterations (continuous batching).
 #include "fx_reduce.h"
 #include "fx_sell.h"
 
@@ -31,9 +47,15 @@ constexpr int kBB = 256;  // threads per block in the batch kernels
 
 struct BRhs {
     double rz[2], rr, bb, crit, pap, alpha, beta, tol;
+    double gamma;  // p_new = gamma z + beta p_old (1 except right after a replacement)
+    double rmax;   // mixed: true residual norm at the last replacement
+    double delta;  // mixed: replace when the iterated residual norm < delta * rmax (0: FP64 slot)
+    double prev;   // mixed: true criterion at the last replacement its own residual asked for (floor test)
     int64_t it, it_k1, max_iter;
-    int32_t status;  // 0 running, 1 converged, 2 breakdown, 3 max_iter, 4 idle
+    int32_t status;  // 0 running, 1 converged, 2 breakdown, 3 max_iter / replacement limit, 4 idle,
+                     // 5 residual replacement in this iteration
     int32_t stop_mode;
+    int32_t nupd, max_upd;  // mixed: replacements done, allowed
 };
 
 struct BSys {
@@ -44,19 +66,21 @@ struct BSys {
     const int32_t *sl_ptr, *scol;       // sliced-ELL operator (fx_sell.h) when sell != 0
     const void *sval;
     int32_t sell;
-    double *b;                          // inner right-hand sides
-    const double *btrue;                // true right-hand sides (mixed)
-    void *x, *r, *z, *p0, *p1, *ap;    // inner PCG vectors, XT (double, or float for the mixed inner solve)
-    double *xacc;                       // FP64 solution of the refinement (mixed)
+    double *b;                          // FP64: right-hand sides; mixed: the last replaced residual b - K64 xacc
+    const double *btrue;                // right-hand sides
+    void *x, *r, *z, *p0, *p1, *ap;    // PCG vectors, XT (double, or float in the mixed solve)
+    double *xacc;                       // mixed: FP64 solution; x holds the increment since the last replacement
     double *part1, *part2;
     int64_t n_nodes, n_dof;
     int64_t it;                         // iteration index of the running right-hand sides
-    int32_t R, nparts1, nparts2, want;  // want: right-hand sides of an outer pass (bit mask)
-    int32_t nouter;                     // row groups (partials) of the FP64 outer residual pass
+    int32_t R, nparts1, nparts2;
     int32_t G;                          // lanes per node row of the lane-group SpMV of this system
     double *bytes_acc;                  // instrumentation: algorithmic bytes of the SpMV launches issued
     double spmv_bytes0, spmv_bytes_rhs; // per launch: operator part, per active right-hand side
+    double upd_bytes0, upd_bytes_rhs;   // the same for a residual replacement pass
     int32_t go;                         // right-hand sides the next SpMV / update work on (bit mask)
+    int32_t upd;                        // mixed: right-hand sides whose residual is replaced in this iteration
+    int32_t trig;                       // mixed: those of them whose own residual asked for it
     BRhs rhs[kRmax];
 };
 
@@ -86,9 +110,9 @@ __device__ __forceinline__ void bsum_partials(const double *part, int nparts, in
     block_sum<NV>(out);
 }
 
-// Inner-solve start for the right-hand sides with status 0:
-// x = 0, r = b, z = r inv, p0 = z (stored as XT); partials (r.z, r.r, b.b)
-// of the stored values -> part2.
+// Solve start for the right-hand sides with status 0:
+// x = 0, r = b, z = r inv, p0 = z (stored as XT); partials (r.z, r.r of the
+// stored values, b.b of the FP64 right-hand side) -> part2.
 template <typename XT>
 __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
@@ -105,7 +129,8 @@ __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int
             if (r >= R || S.rhs[r].status != 0)  // set by the host before this launch
                 continue;
             const int64_t q = i * R + r;
-            const double bi = (double)(XT)S.b[q];
+            const double bt = S.btrue[q];
+            const double bi = (double)(XT)bt;
             const double zi = (double)(XT)(bi * S.inv[i]);
             ((XT *)S.x)[q] = (XT)0;
             ((XT *)S.r)[q] = (XT)bi;
@@ -113,7 +138,7 @@ __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int
             ((XT *)S.p0)[q] = (XT)zi;
             v[3 * r] = bi * zi;
             v[3 * r + 1] = bi * bi;
-            v[3 * r + 2] = bi * bi;
+            v[3 * r + 2] = bt * bt;
         }
     }
     block_sum<3 * kRmax>(v);
@@ -126,19 +151,56 @@ __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int
 // One block per listed system, after kb_init or kb_update: r.z, r.r (and
 // b.b at iteration 0) per right-hand side, the stop rule of fx_pcg.hip's
 // k_spmv_p, beta, and the mask of right-hand sides the next SpMV works on.
+// Mixed slots: the stop rule on the iterated residual, or its fall below
+// delta * rmax, starts a residual replacement of all the slot's running
+// right-hand sides (status 5, S.upd) instead of a Krylov step.  After a
+// replacement (S.upd set, partials r.z, r.r, r.p of the new residual):
+// beta and gamma of the next direction.
 __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int32_t *__restrict__ list) {
     BSys &S = sys[list[blockIdx.x]];
     const int R = S.R;
+    const int upd = S.upd;
     bool any = false;
     for (int r = 0; r < R; ++r)
         any |= S.rhs[r].status == 0;
-    if (!any)  // uniform: only this block's thread 0 writes these states, in earlier launches
+    if (!any) {  // uniform: only this block's thread 0 writes these states, in earlier launches
+        if (upd && threadIdx.x == 0)
+            S.upd = 0;  // every replaced right-hand side stopped
         return;
+    }
     double red[3 * kRmax];
     bsum_partials<3 * kRmax>(S.part2, S.nparts2, 3 * R, red);
     if (threadIdx.x != 0)
         return;
     int go = 0;
+    if (upd) {
+        for (int r = 0; r < R; ++r) {
+            BRhs &h = S.rhs[r];
+            if (h.status != 0)
+                continue;
+            const double rz = red[3 * r], rr = red[3 * r + 1], rp = red[3 * r + 2];
+            const int64_t it = h.it;
+            // keep the direction only while the replaced residual is within
+            // 10% of the iterated one (h.rr): a larger gap -- the FP32
+            // operator's error on this x, of order eps32 * cond(K) -- means
+            // the iterated Krylov space belongs to the FP32 system, and
+            // continuing it can stall the solve for thousands of iterations;
+            // restart from p = z instead
+            const double beta = it == 0 || rr > 1.21 * h.rr ? 0.0 : rz / h.rz[(it - 1) & 1];
+            h.beta = beta;
+            h.gamma = 1.0 - beta * (rz > 0.0 ? rp / rz : 0.0);
+            h.rz[it & 1] = rz;
+            h.rr = rr;
+            h.it_k1 = it;
+            go |= 1 << r;
+        }
+        S.upd = 0;
+        S.go = go;
+        if (go)
+            atomicAdd(S.bytes_acc, S.spmv_bytes0 + (double)__popc(go) * S.spmv_bytes_rhs);
+        return;
+    }
+    int trig = 0;
     int64_t itv = S.it;
     for (int r = 0; r < R; ++r) {
         BRhs &h = S.rhs[r];
@@ -158,14 +220,19 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
         }
         const bool stop_max = it >= h.max_iter;
         h.beta = it == 0 ? 0.0 : rz / h.rz[(it - 1) & 1];
+        h.gamma = 1.0;  // a plain CG step (gamma != 1 only right after a replacement)
         h.rz[it & 1] = rz;
         h.rr = rr;
         h.crit = crit;
         h.it_k1 = it;
-        if (it == 0)
+        if (it == 0) {
             h.bb = bb;
+            h.rmax = sqrt(bb);
+        }
         if (stop_max)
             h.status = 3;
+        else if (h.delta > 0.0 && (conv || sqrt(rr) < h.delta * h.rmax))
+            trig |= 1 << r;
         else if (conv)
             h.status = 1;
         else
@@ -173,6 +240,17 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
         itv = it;  // the running right-hand sides of a system share it
     }
     S.it = itv;
+    if (trig) {  // replace the residual of every running right-hand side of the slot
+        const int all = trig | go;
+        for (int r = 0; r < R; ++r)
+            if ((all >> r) & 1)
+                S.rhs[r].status = 5;
+        S.upd = all;
+        S.trig = trig;
+        S.go = 0;
+        atomicAdd(S.bytes_acc, S.upd_bytes0 + (double)__popc(all) * S.upd_bytes_rhs);
+        return;
+    }
     S.go = go;
     if (go)  // the next SpMV launch works on this system (integer-valued: exact in any order)
         atomicAdd(S.bytes_acc, S.spmv_bytes0 + (double)__popc(go) * S.spmv_bytes_rhs);
@@ -182,9 +260,9 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
 // DOFs, the FP32 operator's correction block, Ap, and p.Ap block partials
 // (right-hand sides outside `go` are computed but not stored).
 template <typename VT, typename XT, int RR>
-__device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, const double (&beta)[RR],
-                                           gptr<const XT> z, gptr<const XT> pold, gptr<XT> pnew,
-                                           double (&acc)[3][RR], double (&loc)[kRmax]) {
+__device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, const double (&gam)[RR],
+                                           const double (&beta)[RR], gptr<const XT> z, gptr<const XT> pold,
+                                           gptr<XT> pnew, double (&acc)[3][RR], double (&loc)[kRmax]) {
     gptr<XT> ap = gp((XT *)S.ap);
     double qv[3][RR];
 #pragma unroll
@@ -192,7 +270,7 @@ __device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, con
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
             const int64_t q = (3 * a + p) * RR + r;
-            qv[p][r] = (double)(XT)((double)z[q] + beta[r] * (double)pold[q]);  // p_new as stored
+            qv[p][r] = pnew_at<XT>(z, pold, q, gam[r], beta[r]);  // p_new as stored
         }
     if constexpr (sizeof(VT) == 4) {
         gptr<const float> c = gp(S.kc) + 9 * a;
@@ -224,12 +302,20 @@ __device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, con
     }
 }
 
+template <int RR>
+__device__ __forceinline__ void load_coefs(const BSys &S, int go, double (&gam)[RR], double (&beta)[RR]) {
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        const bool on = (go >> r) & 1;
+        beta[r] = on ? S.rhs[r].beta : 0.0;
+        gam[r] = on ? S.rhs[r].gamma : 1.0;
+    }
+}
+
 template <int G, typename VT, typename XT, int RR>
 __device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, double (&loc)[kRmax]) {
-    double beta[RR];
-#pragma unroll
-    for (int r = 0; r < RR; ++r)
-        beta[r] = ((go >> r) & 1) ? S.rhs[r].beta : 0.0;
+    double gam[RR], beta[RR];
+    load_coefs<RR>(S, go, gam, beta);
     const int64_t itv = S.it;
     gptr<const XT> z = gp((const XT *)S.z);
     gptr<const XT> pold = gp((const XT *)((itv & 1) ? S.p1 : S.p0));
@@ -274,7 +360,7 @@ __device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, doubl
                 double xj[RR];
 #pragma unroll
                 for (int r = 0; r < RR; ++r)
-                    xj[r] = (double)(XT)((double)z[c[u] + r] + beta[r] * (double)pold[c[u] + r]);
+                    xj[r] = pnew_at<XT>(z, pold, c[u] + r, gam[r], beta[r]);
 #pragma unroll
                 for (int r = 0; r < RR; ++r) {
                     acc[0][r] += w[u][0] * xj[r];
@@ -292,15 +378,13 @@ __device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, doubl
             for (int r = 0; r < RR; ++r)
                 acc[p][r] += __shfl_xor(acc[p][r], off, G);
     if (lane == 0 && a < S.n_nodes)
-        bspmv_tail<VT, XT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
+        bspmv_tail<VT, XT, RR>(S, a, go, gam, beta, z, pold, pnew, acc, loc);
 }
 
 template <typename VT, typename XT, int RR>
 __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double (&loc)[kRmax]) {
-    double beta[RR];
-#pragma unroll
-    for (int r = 0; r < RR; ++r)
-        beta[r] = ((go >> r) & 1) ? S.rhs[r].beta : 0.0;
+    double gam[RR], beta[RR];
+    load_coefs<RR>(S, go, gam, beta);
     const int64_t itv = S.it;
     gptr<const XT> z = gp((const XT *)S.z);
     gptr<const XT> pold = gp((const XT *)((itv & 1) ? S.p1 : S.p0));
@@ -314,9 +398,93 @@ __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double
             acc[p][r] = 0.0;
     if (a < S.n_nodes) {
         sell_node_rows<VT, XT, RR>(gp(S.nb_ptr), gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold,
-                                   beta, acc);
-        bspmv_tail<VT, XT, RR>(S, a, go, beta, z, pold, pnew, acc, loc);
+                                   gam, beta, acc);
+        bspmv_tail<VT, XT, RR>(S, a, go, gam, beta, z, pold, pnew, acc, loc);
     }
+}
+
+// Residual replacement, SpMV part, for node row a (G lanes): r64 = b -
+// K64 (xacc + x) at the right-hand sides in `upd` -> S.b, and the partial
+// sums r64.r64, r64.(inv r64) and (|K||y|).(|K||y|), y = xacc + x, -> v
+// (lane 0).  xacc + x is the FP64 solution the update kernel stores next.
+template <int G>
+__device__ __forceinline__ void bresid_row(const BSys &S, int64_t a, int lane, int upd, double (&v)[3 * kRmax]) {
+    const int R = S.R;
+    gptr<const double> xacc = gp((const double *)S.xacc), val = gp(S.val64);
+    gptr<const float> x = gp((const float *)S.x);
+    gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
+    double acc[3][kRmax], mag[3][kRmax];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r)
+            acc[p][r] = mag[p][r] = 0.0;
+    if (a < S.n_nodes) {
+        const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
+        gptr<const double> v0 = val + 9 * (int64_t)b0;
+        for (int j = lane; j < ncol; j += G) {
+            const int kb = j / 3;
+            const int64_t c = (3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb)) * R;
+            const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
+#pragma unroll
+            for (int r = 0; r < kRmax; ++r) {
+                if (r >= R || !((upd >> r) & 1))
+                    continue;
+                const double yj = xacc[c + r] + (double)x[c + r];
+                acc[0][r] += w0 * yj;
+                acc[1][r] += w1 * yj;
+                acc[2][r] += w2 * yj;
+                mag[0][r] += fabs(w0 * yj);
+                mag[1][r] += fabs(w1 * yj);
+                mag[2][r] += fabs(w2 * yj);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int r = 0; r < kRmax; ++r) {
+                acc[p][r] += __shfl_xor(acc[p][r], off, G);
+                mag[p][r] += __shfl_xor(mag[p][r], off, G);
+            }
+    if (lane != 0 || a >= S.n_nodes)
+        return;
+    gptr<const double> bt = gp(S.btrue), inv = gp(S.inv);
+    gptr<double> rb = gp(S.b);
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r) {
+        if (r >= R || !((upd >> r) & 1))
+            continue;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const int64_t d = 3 * a + p, q = d * R + r;
+            const double ri = bt[q] - acc[p][r];
+            rb[q] = ri;
+            v[3 * r] += ri * ri;
+            v[3 * r + 1] += ri * (ri * inv[d]);
+            v[3 * r + 2] += mag[p][r] * mag[p][r];
+        }
+    }
+}
+
+// one block of a replacement pass: `sub` groups of kBB / G node rows from a0;
+// block partials -> part1 (3 per right-hand side)
+template <int G>
+__device__ __forceinline__ void bresid_block(const BSys &S, int lb, int64_t a0, int sub) {
+    double v[3 * kRmax];
+#pragma unroll
+    for (int k = 0; k < 3 * kRmax; ++k)
+        v[k] = 0.0;
+    const int upd = S.upd;
+    for (int k = 0; k < sub; ++k)
+        bresid_row<G>(S, a0 + (int64_t)k * (kBB / G) + threadIdx.x / G, threadIdx.x % G, upd, v);
+    block_sum<3 * kRmax>(v);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < S.R; ++r)
+            for (int k = 0; k < 3; ++k)
+                S.part1[((size_t)lb * S.R + r) * 3 + k] = v[3 * r + k];
 }
 
 // Batched SpMV + p update for every (system, row group) of the table: lane
@@ -327,6 +495,12 @@ template <int G, typename VT, typename XT>
 __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
+    if constexpr (sizeof(VT) == 4) {
+        if (S.upd) {  // this system replaces its residuals in this iteration
+            bresid_block<G>(S, t.y, (int64_t)t.y * (kBB / G), 1);
+            return;
+        }
+    }
     const int go = S.go;
     if (!go)
         return;
@@ -347,6 +521,12 @@ template <typename VT, typename XT>
 __global__ __launch_bounds__(kBB) void kb_spmv_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
+    if constexpr (sizeof(VT) == 4) {
+        if (S.upd) {  // the block's kBB node rows, 8 lanes per row on the FP64 node-block values
+            bresid_block<8>(S, t.y, (int64_t)t.y * kBB, 8);
+            return;
+        }
+    }
     const int go = S.go;
     if (!go)
         return;
@@ -364,9 +544,56 @@ __global__ __launch_bounds__(kBB) void kb_spmv_sell(BSys *__restrict__ sys, cons
 }
 
 // One block per listed system, after kb_spmv_p: p.Ap, alpha, breakdown
-// (k_update's rule in fx_pcg.hip) and the mask the update works on.
+// (k_update's rule in fx_pcg.hip) and the mask the update works on.  After
+// a replacement pass: the stop rule on the FP64 residual (with the
+// rounding-floor test) per replaced right-hand side; S.go = those that go on.
 __global__ __launch_bounds__(kBB) void kb_red1(BSys *__restrict__ sys, const int32_t *__restrict__ list) {
     BSys &S = sys[list[blockIdx.x]];
+    const int upd = S.upd;
+    if (upd) {
+        const int R = S.R;
+        double red[3 * kRmax];
+        bsum_partials<3 * kRmax>(S.part1, S.nparts1, 3 * R, red);
+        if (threadIdx.x != 0)
+            return;
+        int go2 = 0;
+        for (int r = 0; r < R; ++r) {
+            if (!((upd >> r) & 1))
+                continue;
+            BRhs &h = S.rhs[r];
+            const double rr = red[3 * r], rzt = red[3 * r + 1], aa = red[3 * r + 2];
+            double crit;
+            bool conv;
+            if (h.stop_mode == FX_STOP_ABS_SQRT_RZ) {
+                crit = sqrt(rzt);
+                conv = crit < h.tol;
+            } else {
+                crit = h.bb > 0.0 ? sqrt(rr) / sqrt(h.bb) : 0.0;
+                conv = crit <= h.tol;
+            }
+            // the floor test compares the ends of two full cycles: it runs,
+            // and prev moves, only for a right-hand side whose own residual
+            // fell by delta (or met the stop rule), not for one replaced
+            // early along with the others of its slot
+            if ((S.trig >> r) & 1) {
+                conv = conv || at_rounding_floor(crit, h.prev, sqrt(rr), sqrt(aa));
+                h.prev = crit;
+            }
+            h.crit = crit;
+            h.nupd++;
+            if (conv) {
+                h.status = 1;
+            } else if (!isfinite(crit) || h.nupd >= h.max_upd) {
+                h.status = 3;
+            } else {
+                h.status = 0;
+                h.rmax = sqrt(rr);
+                go2 |= 1 << r;
+            }
+        }
+        S.go = go2;
+        return;
+    }
     const int go = S.go;
     if (!go)
         return;
@@ -432,11 +659,59 @@ __device__ __forceinline__ void bupdate(const BSys &S, int lb, int go, double (&
     }
 }
 
+// Residual replacement, update part (own DOFs only): xacc += x, x = 0 for
+// the replaced right-hand sides; r = r64, z = r inv for those that go on,
+// with the partials r.z, r.r, r.p (p: the kept direction).
+__device__ __forceinline__ void breplace(const BSys &S, int lb, int upd, int go, double (&v)[3 * kRmax]) {
+    const int64_t i = (int64_t)lb * kBB + threadIdx.x;
+    if (i >= S.n_dof)
+        return;
+    const int R = S.R;
+    gptr<float> x = gp((float *)S.x), rv = gp((float *)S.r), z = gp((float *)S.z);
+    gptr<const float> p = gp((const float *)((S.it & 1) ? S.p1 : S.p0));  // what the next SpMV reads as p_old
+    gptr<double> xacc = gp(S.xacc);
+    gptr<const double> rb = gp((const double *)S.b);
+    const double inv = gp(S.inv)[i];
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r) {
+        if (r >= R || !((upd >> r) & 1))
+            continue;
+        const int64_t q = i * R + r;
+        xacc[q] += (double)x[q];
+        x[q] = 0.f;
+        if ((go >> r) & 1) {
+            const double ri = (double)(float)rb[q];
+            const double zi = (double)(float)(ri * inv);
+            rv[q] = (float)ri;
+            z[q] = (float)zi;
+            v[3 * r] = ri * zi;
+            v[3 * r + 1] = ri * ri;
+            v[3 * r + 2] = ri * (double)p[q];
+        }
+    }
+}
+
 template <typename XT>
 __global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
     const int go = S.go;
+    if constexpr (sizeof(XT) == 4) {
+        const int upd = S.upd;
+        if (upd) {
+            double v[3 * kRmax];
+#pragma unroll
+            for (int k = 0; k < 3 * kRmax; ++k)
+                v[k] = 0.0;
+            breplace(S, t.y, upd, go, v);
+            block_sum<3 * kRmax>(v);
+            if (threadIdx.x == 0)
+                for (int r = 0; r < S.R; ++r)
+                    for (int k = 0; k < 3; ++k)
+                        S.part2[((size_t)t.y * S.R + r) * 3 + k] = v[3 * r + k];
+            return;
+        }
+    }
     if (!go)
         return;
     double v[2 * kRmax];
@@ -458,110 +733,6 @@ __global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const i
         }
 }
 
-// Mixed refinement, step 1: xacc += x for the right-hand sides in S.want.
-template <typename XT>
-__global__ __launch_bounds__(kBB) void kb_accumulate(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    BSys &S = sys[t.x];
-    const int64_t i = (int64_t)t.y * kBB + threadIdx.x;
-    if (i >= S.n_dof)
-        return;
-    for (int r = 0; r < S.R; ++r)
-        if ((S.want >> r) & 1) {
-            const int64_t q = i * S.R + r;
-            S.xacc[q] += (double)((const XT *)S.x)[q];
-        }
-}
-
-// Mixed refinement, step 2: b = btrue - K64 xacc (the next inner
-// right-hand side) for the right-hand sides in S.want, with block partials
-// (r.r, b.b, r.(inv r), (|K||x|).(|K||x|)) -> part1 (4 per right-hand side).
-template <int G>
-__global__ __launch_bounds__(kBB) void kb_outer(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    const BSys &S = sys[t.x];
-    const int lb = t.y, R = S.R, w = S.want;
-    gptr<const double> xacc = gp((const double *)S.xacc), val = gp(S.val64);
-    gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
-    const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
-    const int lane = threadIdx.x % G;
-    double acc[3][kRmax], mag[3][kRmax];
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int r = 0; r < kRmax; ++r)
-            acc[p][r] = mag[p][r] = 0.0;
-    if (a < S.n_nodes) {
-        const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
-        gptr<const double> v0 = val + 9 * (int64_t)b0;
-        for (int j = lane; j < ncol; j += G) {
-            const int kb = j / 3;
-            const int64_t c = (3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb)) * R;
-            const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
-#pragma unroll
-            for (int r = 0; r < kRmax; ++r) {
-                if (r >= R || !((w >> r) & 1))
-                    continue;
-                const double xj = xacc[c + r];
-                acc[0][r] += w0 * xj;
-                acc[1][r] += w1 * xj;
-                acc[2][r] += w2 * xj;
-                mag[0][r] += fabs(w0 * xj);
-                mag[1][r] += fabs(w1 * xj);
-                mag[2][r] += fabs(w2 * xj);
-            }
-        }
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1)
-#pragma unroll
-        for (int p = 0; p < 3; ++p)
-#pragma unroll
-            for (int r = 0; r < kRmax; ++r) {
-                acc[p][r] += __shfl_xor(acc[p][r], off, G);
-                mag[p][r] += __shfl_xor(mag[p][r], off, G);
-            }
-    double v[4 * kRmax];
-#pragma unroll
-    for (int k = 0; k < 4 * kRmax; ++k)
-        v[k] = 0.0;
-    if (lane == 0 && a < S.n_nodes) {
-#pragma unroll
-        for (int r = 0; r < kRmax; ++r) {
-            if (r >= R || !((w >> r) & 1))
-                continue;
-#pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                const int64_t d = 3 * a + p;
-                const int64_t q = d * R + r;
-                const double bt = S.btrue[q];
-                const double ri = bt - acc[p][r];
-                S.b[q] = ri;
-                v[4 * r] += ri * ri;
-                v[4 * r + 1] += bt * bt;
-                v[4 * r + 2] += ri * (ri * S.inv[d]);
-                v[4 * r + 3] += mag[p][r] * mag[p][r];
-            }
-        }
-    }
-    block_sum<4 * kRmax>(v);
-    if (threadIdx.x == 0)
-        for (int r = 0; r < R; ++r)
-            for (int k = 0; k < 4; ++k)
-                S.part1[((size_t)lb * R + r) * 4 + k] = v[4 * r + k];
-}
-
-// one block per listed slot: fixed-order sums of the outer partials
-__global__ __launch_bounds__(kBB) void kb_outer_sum(BSys *__restrict__ sys, const int32_t *__restrict__ list,
-                                                    double *__restrict__ out) {
-    BSys &S = sys[list[blockIdx.x]];
-    double red[4 * kRmax];
-    bsum_partials<4 * kRmax>(S.part1, S.nouter, 4 * S.R, red);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 4 * kRmax; ++k)
-            out[(size_t)blockIdx.x * 4 * kRmax + k] = k < 4 * S.R ? red[k] : 0.0;
-}
-
 inline int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
 template <typename VT>
@@ -573,17 +744,6 @@ void launch_bspmv(int G, unsigned nb, hipStream_t s, BSys *sys, const int2 *tab)
     case 16: hipLaunchKernelGGL((kb_spmv_p<16, VT, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
     case 32: hipLaunchKernelGGL((kb_spmv_p<32, VT, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
     default: hipLaunchKernelGGL((kb_spmv_p<64, VT, VT>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    }
-}
-
-void launch_bouter(int G, unsigned nb, hipStream_t s, BSys *sys, const int2 *tab) {
-    switch (G) {
-    case 2: hipLaunchKernelGGL((kb_outer<2>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    case 4: hipLaunchKernelGGL((kb_outer<4>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    case 8: hipLaunchKernelGGL((kb_outer<8>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    case 16: hipLaunchKernelGGL((kb_outer<16>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    case 32: hipLaunchKernelGGL((kb_outer<32>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
-    default: hipLaunchKernelGGL((kb_outer<64>), dim3(nb), dim3(kBB), 0, s, sys, tab); break;
     }
 }
 
@@ -602,13 +762,10 @@ struct fx_batch {
         fx_mesh *mesh = nullptr;
         int R = 0;
         int G = 8;      // lanes per node row (lane-group SpMV, outer pass)
-        int phase = 0;  // 0 inner running, 2 finished
-        int passes = 0;
+        int phase = 0;  // 0 running, 2 finished
         fx_status err = FX_OK;
-        int64_t iters = 0, outer = 0, budget = 0;
+        int64_t iters = 0, outer = 0;  // Krylov iterations (slowest load case), residual replacements
         double crit[fx::kRmax] = {0, 0, 0, 0};
-        double prev[fx::kRmax] = {HUGE_VAL, HUGE_VAL, HUGE_VAL, HUGE_VAL};  // last outer criterion
-        bool done[fx::kRmax] = {false, false, false, false};
         fx::DBuf<double> b, rin, x, r, z, p0, p1, ap, xacc, part1, part2;
         std::chrono::steady_clock::time_point t0;
         double seconds = 0.0;
@@ -618,10 +775,9 @@ struct fx_batch {
     std::vector<fx::BSys> h_sys;
     fx::DBuf<fx::BSys> d_sys;
     static constexpr int kGroups = 6;  // lane groups 2, 4, ..., 64
-    fx::DBuf<int2> d_tab1g[kGroups], d_tab1s, d_tab2, d_tab3, d_tab4;
+    fx::DBuf<int2> d_tab1g[kGroups], d_tab1s, d_tab2, d_tab4;
     int n1g[kGroups] = {0, 0, 0, 0, 0, 0};
-    fx::DBuf<int32_t> d_list, d_list2, d_run;
-    fx::DBuf<double> d_out;
+    fx::DBuf<int32_t> d_list2, d_run;
     std::vector<int2> h_tab_start;
     int n1s = 0, n2 = 0, nrun = 0;
     bool tables_dirty = true;
@@ -651,28 +807,25 @@ using fx::kRmax;
 
 bool is_mixed(const fx_batch *B) { return B->opts.precision == FX_PREC_MIXED; }
 
-void start_inner(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
-    // right-hand sides not done start an inner solve from x = 0
-    for (int r = 0; r < sl.R; ++r) {
+void start_solve(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
+    for (int r = 0; r < kRmax; ++r) {
         BRhs &h = S.rhs[r];
         h = BRhs{};
-        if (sl.done[r]) {
+        if (r >= sl.R) {
             h.status = 4;
             continue;
         }
         h.status = 0;
+        h.tol = B->opts.tol;
+        h.stop_mode = B->opts.stop_mode;
+        h.max_iter = B->opts.max_iter;
+        h.gamma = 1.0;
         if (is_mixed(B)) {
-            h.tol = B->opts.inner_tol;
-            h.stop_mode = FX_STOP_REL_R2;
-            h.max_iter = std::max<int64_t>(sl.budget, 0);
-        } else {
-            h.tol = B->opts.tol;
-            h.stop_mode = B->opts.stop_mode;
-            h.max_iter = B->opts.max_iter;
+            h.delta = B->opts.inner_tol;
+            h.max_upd = B->opts.max_refine;
+            h.prev = HUGE_VAL;
         }
     }
-    for (int r = sl.R; r < kRmax; ++r)
-        S.rhs[r].status = 4;
 }
 
 void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
@@ -703,7 +856,6 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.R = sl.R;
     S.G = sl.G;
     S.nparts1 = m->sell ? fx::nblk(m->n_nodes, kBB) : fx::nblk(m->n_nodes, kBB / sl.G);
-    S.nouter = fx::nblk(m->n_nodes, kBB / sl.G);  // kb_outer: lane groups on the FP64 node-block values
     // SpMV algorithmic bytes: values in the working precision, one column
     // index per 3x3 block, node-row pointers (+ the FP32 operator's 3x3
     // correction per node); per right-hand side z and p_old gathered once
@@ -713,8 +865,13 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.spmv_bytes0 = (is_mixed(B) ? 4.0 : 8.0) * (double)m->nnz_struct + 4.0 * (double)m->nblk +
                     4.0 * (double)(m->n_nodes + 1) + (is_mixed(B) ? 36.0 * (double)m->n_nodes : 0.0);
     S.spmv_bytes_rhs = (is_mixed(B) ? 16.0 : 32.0) * (double)m->n_dof;
+    // replacement pass: FP64 values, block columns, row pointers, the
+    // Jacobi scaling; per right-hand side xacc and x gathered once, b read
+    // and the new residual written
+    S.upd_bytes0 = 8.0 * (double)m->nnz_struct + 4.0 * (double)m->nblk + 4.0 * (double)(m->n_nodes + 1) +
+                   8.0 * (double)m->n_dof;
+    S.upd_bytes_rhs = 28.0 * (double)m->n_dof;
     S.nparts2 = fx::nblk(m->n_dof, kBB);
-    S.want = 0;
 }
 
 // workgroups are dispatched round-robin over the 8 XCDs: permute an SpMV
@@ -885,141 +1042,42 @@ void collect_sample(fx_batch *B) {
     B->ev_used = 0;
 }
 
-// After a chunk: per slot, see whether its running inner solves finished;
-// FP64 slots finish, mixed slots take an outer refinement pass.  Returns
-// true when the set of running slots changed.
+// After a chunk: the slots none of whose load cases still runs finish.
+// Returns true when the set of running slots changed.
 bool advance_phases(fx_batch *B) {
     download_sys(B);
     collect_sample(B);
     bool changed = false;
-    std::vector<int> outer;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         auto &sl = *B->slots[k];
         if (!sl.used || sl.phase != 0)
             continue;
-        BSys &S = B->h_sys[k];
+        const BSys &S = B->h_sys[k];
         bool running = false;
-        int64_t it = 0;
+        int64_t it = 0, nupd = 0;
         for (int r = 0; r < sl.R; ++r) {
-            if (S.rhs[r].status == 0)
+            const BRhs &h = S.rhs[r];
+            if (h.status == 0 || h.status == 5)
                 running = true;
-            if (S.rhs[r].status != 4)
-                it = std::max(it, S.rhs[r].status == 2 ? S.rhs[r].it_k1 + 1 : S.rhs[r].it_k1);
+            it = std::max(it, h.status == 2 ? h.it_k1 + 1 : h.it_k1);
+            nupd = std::max<int64_t>(nupd, h.nupd);
         }
         if (running)
             continue;
-        sl.iters += it;
-        sl.budget -= it;
-        sl.passes++;
-        int want = 0;
+        sl.iters = it;
+        sl.outer = nupd;
         for (int r = 0; r < sl.R; ++r) {
             const BRhs &h = S.rhs[r];
-            if (h.status == 4)
-                continue;
+            sl.crit[r] = h.crit;
             if (h.status == 2)
                 sl.err = FX_ERR_BREAKDOWN;
-            else if (h.status == 3)
+            else if (h.status == 3 && sl.err == FX_OK)
                 sl.err = FX_ERR_NOT_CONVERGED;
-            else if (!is_mixed(B)) {
-                sl.done[r] = true;
-                sl.crit[r] = h.crit;
-            } else
-                want |= 1 << r;
         }
-        if (!is_mixed(B) || sl.err != FX_OK) {
-            sl.phase = 2;
-            sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
-            changed = true;
-            continue;
-        }
-        S.want = want;
-        outer.push_back((int)k);
+        sl.phase = 2;
+        sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
+        changed = true;
     }
-    if (outer.empty())
-        return changed;
-    // outer pass for the slots whose inner solves finished
-    hipStream_t s = B->ctx->stream;
-    std::vector<int2> t3, t4;
-    int seg[fx_batch::kGroups + 1] = {0, 0, 0, 0, 0, 0, 0};  // t3 segment of each lane group
-    for (int g = 0; g < fx_batch::kGroups; ++g) {
-        seg[g] = (int)t3.size();
-        for (int k : outer) {
-            const BSys &S = B->h_sys[k];
-            if (group_index(S.G) == g)
-                for (int b = 0; b < S.nouter; ++b)
-                    t3.push_back(make_int2(k, b));
-        }
-    }
-    seg[fx_batch::kGroups] = (int)t3.size();
-    for (int k : outer)
-        for (int b = 0; b < B->h_sys[k].nparts2; ++b)
-            t4.push_back(make_int2(k, b));
-    upload_sys(B);  // want masks
-    B->d_tab3.alloc(std::max<size_t>(B->d_tab3.n, t3.size()));
-    B->d_tab4.alloc(std::max<size_t>(B->d_tab4.n, t4.size()));
-    B->d_list.alloc(std::max<size_t>(B->d_list.n, outer.size()));
-    B->d_out.alloc(std::max<size_t>(B->d_out.n, outer.size() * 4 * kRmax));
-    FX_HIP(hipMemcpyAsync(B->d_tab3.p, t3.data(), t3.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-    FX_HIP(hipMemcpyAsync(B->d_tab4.p, t4.data(), t4.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-    FX_HIP(hipMemcpyAsync(B->d_list.p, outer.data(), outer.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(fx::kb_accumulate<float>, dim3((unsigned)t4.size()), dim3(kBB), 0, s, B->d_sys.p,
-                       B->d_tab4.p);  // outer passes run for mixed batches only
-    for (int g = 0; g < fx_batch::kGroups; ++g)
-        if (seg[g + 1] > seg[g])
-            fx::launch_bouter(2 << g, (unsigned)(seg[g + 1] - seg[g]), s, B->d_sys.p, B->d_tab3.p + seg[g]);
-    hipLaunchKernelGGL(fx::kb_outer_sum, dim3((unsigned)outer.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_list.p,
-                       B->d_out.p);
-    FX_HIP(hipGetLastError());
-    std::vector<double> norms(outer.size() * 4 * kRmax);
-    FX_HIP(hipMemcpyAsync(norms.data(), B->d_out.p, norms.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    FX_HIP(hipStreamSynchronize(s));
-    std::vector<int> restart;
-    for (size_t j = 0; j < outer.size(); ++j) {
-        const int k = outer[j];
-        auto &sl = *B->slots[k];
-        BSys &S = B->h_sys[k];
-        sl.outer++;
-        bool all = true;
-        for (int r = 0; r < sl.R; ++r) {
-            if (!((S.want >> r) & 1))
-                continue;
-            const double *nv = &norms[j * 4 * kRmax + 4 * r];
-            const double rr = nv[0], bb = nv[1], rzv = nv[2], aa = nv[3];
-            double crit;
-            bool conv;
-            if (B->opts.stop_mode == FX_STOP_ABS_SQRT_RZ) {
-                crit = std::sqrt(rzv);
-                conv = crit < B->opts.tol;
-            } else {
-                crit = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : 0.0;
-                conv = crit <= B->opts.tol;
-            }
-            conv = conv || fx::at_rounding_floor(crit, sl.prev[r], std::sqrt(rr), std::sqrt(aa));
-            sl.prev[r] = crit;
-            sl.crit[r] = crit;
-            if (conv)
-                sl.done[r] = true;
-            else
-                all = false;
-        }
-        S.want = 0;
-        if (all) {
-            sl.phase = 2;
-            sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
-            changed = true;
-        } else if (sl.passes >= B->opts.max_refine || sl.budget <= 0) {
-            sl.err = FX_ERR_NOT_CONVERGED;
-            sl.phase = 2;
-            sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
-            changed = true;
-        } else {
-            start_inner(B, sl, S);  // next refinement pass
-            restart.push_back(k);
-            changed = true;
-        }
-    }
-    upload_sys(B);
-    start_slots(B, restart);
     return changed;
 }
 
@@ -1056,6 +1114,12 @@ fx_status fx_batch_create(fx_ctx *ctx, const fx_solve_opts *opts, fx_batch **out
         if (B->opts.precision != FX_PREC_FP64 && B->opts.precision != FX_PREC_MIXED) {
             delete B;
             fx::set_error("fx_batch_create: unknown precision");
+            return FX_ERR_INVALID;
+        }
+        if (B->opts.precision == FX_PREC_MIXED &&
+            !(B->opts.inner_tol > 0.0 && B->opts.inner_tol < 1.0 && B->opts.max_refine >= 1)) {
+            delete B;
+            fx::set_error("fx_batch_create: MIXED needs 0 < inner_tol < 1 and max_refine >= 1");
             return FX_ERR_INVALID;
         }
         const int g = B->opts.spmv_group;
@@ -1148,7 +1212,6 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         sl.G = G;
         sl.part1.alloc((size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 4);  // >= SpMV and outer groups
         sl.part2.alloc((size_t)std::max(fx::nblk(n, kBB), 1) * n_rhs * 3);
-        sl.budget = B->opts.max_iter;
         sl.used = true;
         sl.phase = 0;
         sl.t0 = std::chrono::steady_clock::now();
@@ -1158,12 +1221,12 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         BSys &S = B->h_sys[k];
         S = BSys{};
         fill_sys(B, sl, S);
-        start_inner(B, sl, S);
+        start_solve(B, sl, S);
         // running slots' device state is newer than the host copy: upload
         // this entry only
         reserve_sys(B, B->h_sys.size());
         FX_HIP(hipMemcpyAsync(B->d_sys.p + k, &S, sizeof(BSys), hipMemcpyHostToDevice, s));
-        start_slots(B, std::vector<int>{k});  // start this slot's inner solve now
+        start_slots(B, std::vector<int>{k});  // start this slot's solve now
         B->tables_dirty = true;
         *slot_out = k;
     } catch (const fx::Error &e) {
@@ -1224,7 +1287,7 @@ fx_status fx_batch_poll(fx_batch *B, int32_t slot, int32_t *state, fx_solve_repo
         *rep = fx_solve_report{};
         rep->iterations = sl.iters;
         rep->outer_spmv = sl.outer;
-        rep->refinements = is_mixed(B) ? sl.passes : 0;
+        rep->refinements = (int32_t)sl.outer;
         rep->spmv_group = sl.G;
         double c = 0.0;
         for (int r = 0; r < sl.R; ++r)

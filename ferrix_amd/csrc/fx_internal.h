@@ -1,6 +1,8 @@
 // fx_internal.h -- shared declarations of the gfx950 backend (not part of the ABI).
 #pragma once
 
+#include <cfloat>
+
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -222,7 +224,18 @@ void spmv64(fx_mesh *m, const double *x, double *y, int group, double *abs_y = n
 double profile_spmv(fx_mesh *m, int precision, int reps, int group, double *bytes);
 int default_group(fx_mesh *m);  // lanes per node row for the SpMV
 void set_error(const std::string &msg);  // message of the calling thread's last failure
-bool at_rounding_floor(double crit, double crit_prev, double rnorm, double absnorm);  // mixed outer stop
+// Mixed-precision stop on the true FP64 residual b - K x, which has a
+// rounding floor: each row sum cancels terms of total magnitude (|K| |x|)_i,
+// so the computed residual cannot fall much below eps * || |K| |x| ||.  (The
+// FP64 PCG's recursively updated residual keeps falling below that floor,
+// which is why its stop test can ask for more than the true residual can
+// show -- for a small right-hand side, a Newton correction, by orders of
+// magnitude.)  A residual evaluation that no longer halves the criterion
+// while ||r|| is within 64 eps of || |K| |x| || has reached the floor: no
+// further correction can improve x.
+__host__ __device__ inline bool at_rounding_floor(double crit, double crit_prev, double rnorm, double absnorm) {
+    return crit > 0.5 * crit_prev && rnorm <= 64.0 * DBL_EPSILON * absnorm;
+}
 void batch_free(fx_batch *b);                 // deletes a batch (its context's stream is idle)
 void batch_drop_mesh(fx_batch *b, fx_mesh *m);  // removes the slots that use m
 void axpy_dev(fx_mesh *m, int64_t n, double a, const double *x, double *y);

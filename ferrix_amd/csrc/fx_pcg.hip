@@ -566,19 +566,6 @@ static int run_pcg(fx_mesh *m, const VT *val, const double *b_dev, double tol, i
     return h.status;
 }
 
-// The outer test is on the true FP64 residual b - K x, which has a rounding
-// floor: each row sum cancels terms of total magnitude (|K| |x|)_i, so the
-// computed residual cannot fall much below eps * || |K| |x| ||.  (The FP64
-// PCG's recursively updated residual keeps falling below that floor, which
-// is why its stop test can ask for more than the true residual can show --
-// for a small right-hand side, a Newton correction, by orders of
-// magnitude.)  A refinement pass that no longer halves the criterion while
-// ||r|| is within 64 eps of || |K| |x| || has reached the floor: no further
-// pass can improve x.
-bool at_rounding_floor(double crit, double crit_prev, double rnorm, double absnorm) {
-    return crit > 0.5 * crit_prev && rnorm <= 64.0 * DBL_EPSILON * absnorm;
-}
-
 static int64_t iterations_done(const PcgScalars &h) {
     // the stop rule fires in k_spmv_p of iteration it_k1 (it_k1 SpMVs done);
     // a breakdown fires in k_update after that iteration's SpMV

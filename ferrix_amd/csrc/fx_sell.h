@@ -31,13 +31,26 @@ __device__ __forceinline__ gptr<T> gp(T *p) {
     return (gptr<T>)p;
 }
 
-// acc[p][r] += sum over the row's blocks of K(a,p; c,q) * (z + beta_r p_old)(c,q,r)
+// p_new = z + beta p_old at one vector entry, rounded to the stored type;
+// the mixed-precision solve scales z by gamma (1 except in the iteration
+// after a residual replacement, where it folds in the re-orthogonalisation
+// of the kept direction).  FP64 vectors: the plain update.
+template <typename XT>
+__device__ __forceinline__ double pnew_at(gptr<const XT> z, gptr<const XT> pold, int64_t at, double gam,
+                                          double beta) {
+    if constexpr (sizeof(XT) == 4)
+        return (double)(XT)(gam * (double)z[at] + beta * (double)pold[at]);
+    else
+        return (double)(XT)((double)z[at] + beta * (double)pold[at]);
+}
+
+// acc[p][r] += sum over the row's blocks of K(a,p; c,q) * p_new(c,q,r)
 // for r < RR right-hand sides (vectors interleaved: v[dof*RR + r], stored as
 // XT: double, or float in the mixed-precision inner solve).
 template <typename VT, typename XT, int RR>
 __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> nb_ptr, gptr<const int32_t> sl_ptr,
                                                gptr<const int32_t> scol, gptr<const VT> sval, int64_t a,
-                                               gptr<const XT> z, gptr<const XT> pold,
+                                               gptr<const XT> z, gptr<const XT> pold, const double (&gam)[RR],
                                                const double (&beta)[RR], double (&acc)[3][RR]) {
     const int64_t s = a / kSlice;
     const int lane = (int)(a % kSlice);
@@ -65,8 +78,7 @@ __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> nb_ptr, gptr<
 #pragma unroll
                 for (int r = 0; r < RR; ++r) {
                     const int64_t at = (3 * (int64_t)c[u] + q) * RR + r;
-                    // p_new as its owner stores it (rounded to XT)
-                    x[q][r] = (double)(XT)((double)z[at] + beta[r] * (double)pold[at]);
+                    x[q][r] = pnew_at<XT>(z, pold, at, gam[r], beta[r]);  // as its owner stores it
                 }
 #pragma unroll
             for (int r = 0; r < RR; ++r)

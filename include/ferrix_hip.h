@@ -150,9 +150,13 @@ fx_status fx_apply_penalty(fx_mesh *mesh, int64_t n_bc, const int64_t *dofs, con
 /* ---- solver seam ------------------------------------------------------ */
 typedef enum fx_precision {
     FX_PREC_FP64 = 0,  /* Jacobi-PCG in FP64 (the reference's IterativeSolver)   */
-    FX_PREC_MIXED = 1  /* FP32-stored K inside FP64 iterative refinement;
-                          FP64 residuals and norms.  fx_solve keeps FP64
-                          Krylov vectors, a batch FP32 ones (dots in FP64) */
+    FX_PREC_MIXED = 1  /* FP32-stored K, FP64 residuals and norms.  fx_solve:
+                          FP64 iterative refinement around an inner PCG with
+                          FP64 Krylov vectors.  A batch: PCG with FP32 Krylov
+                          vectors (dots in FP64) and reliable updates -- the
+                          residual is replaced by the FP64 b - K x whenever
+                          the iterated one has fallen by inner_tol, keeping
+                          the search direction if the two agree within 10% */
 } fx_precision;
 
 typedef enum fx_stop {
@@ -165,15 +169,17 @@ typedef struct fx_solve_opts {
     int32_t stop_mode;  /* fx_stop */
     double tol;         /* reference: 1e-8 (static_step.rs:204) */
     int64_t max_iter;   /* reference: 1000; iterations with the working operator */
-    double inner_tol;   /* MIXED: relative residual reduction per refinement pass */
-    int32_t max_refine; /* MIXED: refinement passes before NOT_CONVERGED */
+    double inner_tol;   /* MIXED: residual reduction per refinement pass (fx_solve)
+                           or between residual replacements (batch); 1e-3 */
+    int32_t max_refine; /* MIXED: refinement passes / replacements before
+                           NOT_CONVERGED */
     int32_t spmv_group; /* lanes per node row in the SpMV (0 = autotuned) */
 } fx_solve_opts;
 
 typedef struct fx_solve_report {
     int64_t iterations;     /* SpMVs with the working operator (PCG iterations) */
     int64_t outer_spmv;     /* MIXED: FP64 residual evaluations */
-    int32_t refinements;    /* MIXED: refinement passes */
+    int32_t refinements;    /* MIXED: refinement passes (batch: replacements) */
     int32_t spmv_group;     /* lanes per node row used */
     double final_criterion; /* last stopping-rule value */
     double seconds;         /* wall time of the solve on the context stream */
@@ -321,7 +327,10 @@ fx_status fx_batch_time_spmv(fx_batch *batch, int32_t on);
  * 4-byte column index per 3x3 block, node-row pointers (and the FP32
  * operator's 3x3 correction per node), plus per DOF and running load case
  * z and p_old gathered once and p_new and Ap written (32 bytes in FP64, 16
- * in the mixed inner solve, whose Krylov vectors are FP32). */
+ * in the mixed solve, whose Krylov vectors are FP32).  A mixed system's
+ * residual replacement runs in the same launch: FP64 values, column indices,
+ * row pointers and the Jacobi scaling, plus per DOF and load case xacc and
+ * x gathered once, b read and the new residual written (28 bytes). */
 fx_status fx_batch_spmv_stats(fx_batch *batch, int64_t *launches, int64_t *timed, double *ms_timed,
                               double *bytes_total);
 

@@ -207,6 +207,31 @@ def test_mixed_batch_matches_fp64_solve(ctx):
             assert np.linalg.norm(x[r] - x64) / np.linalg.norm(x64) < 1e-8
 
 
+def test_mixed_batch_ill_conditioned_does_not_stall(ctx):
+    """Member 216 of the 512-mesh population (26x4x4 cells, 2,025 DOF, two
+    load cases; FP64 PCG needs about 680 iterations): the FP32 operator's
+    error is large enough here that a residual replacement which kept the
+    Krylov direction once left load case 0 on a stalling FP32 Krylov space
+    (23,000 iterations).  Each replacement now restarts the direction unless
+    the new residual is within 10% of the iterated one."""
+    spec = meshgen.cfg4_batch_specs()[216]
+    assert (spec["nx"], spec["ny"], spec["nz"]) == (26, 4, 4)
+    (dm, b), = population(ctx, [spec])
+    bt = Batch(ctx, SolveOptions(precision=L.FX_PREC_MIXED, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=6000,
+                                 inner_tol=1e-3, max_refine=100))
+    s = bt.add(dm, b)
+    bt.run()
+    st, rep = bt.poll(s)
+    say("iterations", rep.iterations, "replacements", rep.refinements, "criterion", rep.final_criterion)
+    assert st == 1 and rep.iterations < 6000
+    x = bt.result(s)
+    jobs = meshgen.cfg4_mesh(spec)
+    csr, _, _ = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
+    for r in range(b.shape[0]):
+        xd = O.direct_solve(csr, b[r])
+        assert np.linalg.norm(x[r] - xd) / np.linalg.norm(xd) < 1e-8
+
+
 def test_batch_not_converged(ctx):
     sys = population(ctx, small_specs(2, seed=3))
     bt = Batch(ctx, SolveOptions(max_iter=3, spmv_group=8))
