@@ -44,6 +44,12 @@ namespace fx {
 
 constexpr int kRmax = 4;
 constexpr int kBB = 256;  // threads per block in the batch kernels
+// DOF tiles of kBB per block of the vector kernels (kb_init, kb_update):
+// four with FP32 vectors, so a block's fixed costs (descriptor reads, the
+// block reduction) are spread over enough bytes; one with FP64 vectors,
+// whose partial sums then associate exactly as fx_pcg.hip's.
+template <typename XT>
+constexpr int kTiles = sizeof(XT) == 4 ? 4 : 1;
 
 struct BRhs {
     double rz[2], rr, bb, crit, pap, alpha, beta, tol;
@@ -122,8 +128,10 @@ __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int
 #pragma unroll
     for (int k = 0; k < 3 * kRmax; ++k)
         v[k] = 0.0;
-    const int64_t i = (int64_t)lb * kBB + threadIdx.x;
-    if (i < S.n_dof) {
+    for (int tile = 0; tile < kTiles<XT>; ++tile) {
+        const int64_t i = ((int64_t)lb * kTiles<XT> + tile) * kBB + threadIdx.x;
+        if (i >= S.n_dof)
+            break;
 #pragma unroll
         for (int r = 0; r < kRmax; ++r) {
             if (r >= R || S.rhs[r].status != 0)  // set by the host before this launch
@@ -136,9 +144,9 @@ __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int
             ((XT *)S.r)[q] = (XT)bi;
             ((XT *)S.z)[q] = (XT)zi;
             ((XT *)S.p0)[q] = (XT)zi;
-            v[3 * r] = bi * zi;
-            v[3 * r + 1] = bi * bi;
-            v[3 * r + 2] = bt * bt;
+            v[3 * r] += bi * zi;
+            v[3 * r + 1] += bi * bi;
+            v[3 * r + 2] += bt * bt;
         }
     }
     block_sum<3 * kRmax>(v);
@@ -173,10 +181,12 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
     if (threadIdx.x != 0)
         return;
     int go = 0;
+    // loops over kRmax with a guard (not to R) keep red[] in registers
     if (upd) {
-        for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r) {
             BRhs &h = S.rhs[r];
-            if (h.status != 0)
+            if (r >= R || h.status != 0)
                 continue;
             const double rz = red[3 * r], rr = red[3 * r + 1], rp = red[3 * r + 2];
             const int64_t it = h.it;
@@ -202,9 +212,10 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
     }
     int trig = 0;
     int64_t itv = S.it;
-    for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r) {
         BRhs &h = S.rhs[r];
-        if (h.status != 0)
+        if (r >= R || h.status != 0)
             continue;
         const double rz = red[3 * r], rr = red[3 * r + 1];
         const int64_t it = h.it;
@@ -397,8 +408,7 @@ __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double
         for (int r = 0; r < RR; ++r)
             acc[p][r] = 0.0;
     if (a < S.n_nodes) {
-        sell_node_rows<VT, XT, RR>(gp(S.nb_ptr), gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold,
-                                   gam, beta, acc);
+        sell_node_rows<VT, XT, RR>(gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold, gam, beta, acc);
         bspmv_tail<VT, XT, RR>(S, a, go, gam, beta, z, pold, pnew, acc, loc);
     }
 }
@@ -557,8 +567,9 @@ __global__ __launch_bounds__(kBB) void kb_red1(BSys *__restrict__ sys, const int
         if (threadIdx.x != 0)
             return;
         int go2 = 0;
-        for (int r = 0; r < R; ++r) {
-            if (!((upd >> r) & 1))
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r) {
+            if (r >= R || !((upd >> r) & 1))
                 continue;
             BRhs &h = S.rhs[r];
             const double rr = red[3 * r], rzt = red[3 * r + 1], aa = red[3 * r + 2];
@@ -603,8 +614,9 @@ __global__ __launch_bounds__(kBB) void kb_red1(BSys *__restrict__ sys, const int
     if (threadIdx.x != 0)
         return;
     int go2 = 0;
-    for (int r = 0; r < R; ++r) {
-        if (!((go >> r) & 1))
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r) {
+        if (r >= R || !((go >> r) & 1))
             continue;
         BRhs &h = S.rhs[r];
         const double pap = red[r];
@@ -630,32 +642,34 @@ __device__ __forceinline__ void bupdate(const BSys &S, int lb, int go, double (&
     gptr<const XT> p = gp((const XT *)((S.it & 1) ? S.p0 : S.p1));  // p_i: the "new" buffer of iteration i
     gptr<XT> x = gp((XT *)S.x), rv = gp((XT *)S.r), z = gp((XT *)S.z);
     gptr<const XT> ap = gp((const XT *)S.ap);
-    const int64_t i = (int64_t)lb * kBB + threadIdx.x;
-    if (i >= S.n_dof)
-        return;
-    const double inv = gp(S.inv)[i];
-    double xo[RR], ro[RR], po[RR], apo[RR];
+    for (int tile = 0; tile < kTiles<XT>; ++tile) {
+        const int64_t i = ((int64_t)lb * kTiles<XT> + tile) * kBB + threadIdx.x;
+        if (i >= S.n_dof)
+            break;
+        const double inv = gp(S.inv)[i];
+        double xo[RR], ro[RR], po[RR], apo[RR];
 #pragma unroll
-    for (int r = 0; r < RR; ++r) {
-        const int64_t q = i * RR + r;
-        xo[r] = x[q];
-        ro[r] = rv[q];
-        po[r] = p[q];
-        apo[r] = ap[q];
-    }
+        for (int r = 0; r < RR; ++r) {
+            const int64_t q = i * RR + r;
+            xo[r] = x[q];
+            ro[r] = rv[q];
+            po[r] = p[q];
+            apo[r] = ap[q];
+        }
 #pragma unroll
-    for (int r = 0; r < RR; ++r) {
-        if (!((go >> r) & 1))
-            continue;
-        const int64_t q = i * RR + r;
-        const double xi = xo[r] + alpha[r] * po[r];
-        const double ri = (double)(XT)(ro[r] - alpha[r] * apo[r]);  // values as stored
-        const double zi = (double)(XT)(ri * inv);
-        x[q] = (XT)xi;
-        rv[q] = (XT)ri;
-        z[q] = (XT)zi;
-        v[2 * r] = ri * zi;
-        v[2 * r + 1] = ri * ri;
+        for (int r = 0; r < RR; ++r) {
+            if (!((go >> r) & 1))
+                continue;
+            const int64_t q = i * RR + r;
+            const double xi = xo[r] + alpha[r] * po[r];
+            const double ri = (double)(XT)(ro[r] - alpha[r] * apo[r]);  // values as stored
+            const double zi = (double)(XT)(ri * inv);
+            x[q] = (XT)xi;
+            rv[q] = (XT)ri;
+            z[q] = (XT)zi;
+            v[2 * r] += ri * zi;
+            v[2 * r + 1] += ri * ri;
+        }
     }
 }
 
@@ -663,30 +677,32 @@ __device__ __forceinline__ void bupdate(const BSys &S, int lb, int go, double (&
 // the replaced right-hand sides; r = r64, z = r inv for those that go on,
 // with the partials r.z, r.r, r.p (p: the kept direction).
 __device__ __forceinline__ void breplace(const BSys &S, int lb, int upd, int go, double (&v)[3 * kRmax]) {
-    const int64_t i = (int64_t)lb * kBB + threadIdx.x;
-    if (i >= S.n_dof)
-        return;
     const int R = S.R;
     gptr<float> x = gp((float *)S.x), rv = gp((float *)S.r), z = gp((float *)S.z);
     gptr<const float> p = gp((const float *)((S.it & 1) ? S.p1 : S.p0));  // what the next SpMV reads as p_old
     gptr<double> xacc = gp(S.xacc);
     gptr<const double> rb = gp((const double *)S.b);
-    const double inv = gp(S.inv)[i];
+    for (int tile = 0; tile < kTiles<float>; ++tile) {
+        const int64_t i = ((int64_t)lb * kTiles<float> + tile) * kBB + threadIdx.x;
+        if (i >= S.n_dof)
+            break;
+        const double inv = gp(S.inv)[i];
 #pragma unroll
-    for (int r = 0; r < kRmax; ++r) {
-        if (r >= R || !((upd >> r) & 1))
-            continue;
-        const int64_t q = i * R + r;
-        xacc[q] += (double)x[q];
-        x[q] = 0.f;
-        if ((go >> r) & 1) {
-            const double ri = (double)(float)rb[q];
-            const double zi = (double)(float)(ri * inv);
-            rv[q] = (float)ri;
-            z[q] = (float)zi;
-            v[3 * r] = ri * zi;
-            v[3 * r + 1] = ri * ri;
-            v[3 * r + 2] = ri * (double)p[q];
+        for (int r = 0; r < kRmax; ++r) {
+            if (r >= R || !((upd >> r) & 1))
+                continue;
+            const int64_t q = i * R + r;
+            xacc[q] += (double)x[q];
+            x[q] = 0.f;
+            if ((go >> r) & 1) {
+                const double ri = (double)(float)rb[q];
+                const double zi = (double)(float)(ri * inv);
+                rv[q] = (float)ri;
+                z[q] = (float)zi;
+                v[3 * r] += ri * zi;
+                v[3 * r + 1] += ri * ri;
+                v[3 * r + 2] += ri * (double)p[q];
+            }
         }
     }
 }
@@ -871,7 +887,7 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.upd_bytes0 = 8.0 * (double)m->nnz_struct + 4.0 * (double)m->nblk + 4.0 * (double)(m->n_nodes + 1) +
                    8.0 * (double)m->n_dof;
     S.upd_bytes_rhs = 28.0 * (double)m->n_dof;
-    S.nparts2 = fx::nblk(m->n_dof, kBB);
+    S.nparts2 = fx::nblk(m->n_dof, kBB * (is_mixed(B) ? fx::kTiles<float> : fx::kTiles<double>));
 }
 
 // workgroups are dispatched round-robin over the 8 XCDs: permute an SpMV

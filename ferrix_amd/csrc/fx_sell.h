@@ -6,13 +6,13 @@
 // sl_ptr[slice] + k, and the block's column node at scol[kk*64 + lane].  One
 // thread owns one node row: every load of a block column is one coalesced
 // 64-lane access, and the gathered operand entries of a block are 3
-// consecutive DOFs.  Rows shorter than their slice's width are padded; the
-// padding is never read (each thread stops at its own row length).
+// consecutive DOFs.  Rows shorter than their slice's width are padded with
+// zero blocks whose column is the row's own node (node 0 for the lanes past
+// the last row of the mesh).
 //
 // The accumulation order of a DOF row is block column k ascending (the
 // node-block pattern's sorted columns), q = 0, 1, 2 within a block, one
-// multiply-add at a time: the single-system and the batched solves share
-// this function and so produce bitwise equal results.
+// multiply-add at a time.
 #pragma once
 
 #include "fx_internal.h"
@@ -46,49 +46,71 @@ __device__ __forceinline__ double pnew_at(gptr<const XT> z, gptr<const XT> pold,
 
 // acc[p][r] += sum over the row's blocks of K(a,p; c,q) * p_new(c,q,r)
 // for r < RR right-hand sides (vectors interleaved: v[dof*RR + r], stored as
-// XT: double, or float in the mixed-precision inner solve).
+// XT: double, or float in the mixed-precision solve).
+//
+// The loop runs over the slice's width, which is the same for the 64 rows
+// of a slice, so one wavefront has one trip count (scalar loop control, no
+// per-lane branches).  A row's padding blocks past its own length have zero
+// values and point at the row's own node, so they add exact zeros.  Column
+// indices are loaded one step ahead: a step issues the operand gathers and
+// the value loads of two blocks together, behind one memory latency instead
+// of two.
 template <typename VT, typename XT, int RR>
-__device__ __forceinline__ void sell_node_rows(gptr<const int32_t> nb_ptr, gptr<const int32_t> sl_ptr,
-                                               gptr<const int32_t> scol, gptr<const VT> sval, int64_t a,
-                                               gptr<const XT> z, gptr<const XT> pold, const double (&gam)[RR],
+__device__ __forceinline__ void gather_block(gptr<const XT> z, gptr<const XT> pold, int c, const double (&gam)[RR],
+                                             const double (&beta)[RR], double (&x)[3][RR]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+            x[q][r] = pnew_at<XT>(z, pold, (3 * (int64_t)c + q) * RR + r, gam[r], beta[r]);  // as its owner stores it
+}
+
+template <typename VT, int RR>
+__device__ __forceinline__ void madd_block(const VT (&v)[9], const double (&x)[3][RR], double (&acc)[3][RR]) {
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            acc[p][r] += (double)v[3 * p] * x[0][r];
+            acc[p][r] += (double)v[3 * p + 1] * x[1][r];
+            acc[p][r] += (double)v[3 * p + 2] * x[2][r];
+        }
+}
+
+template <typename VT, typename XT, int RR>
+__device__ __forceinline__ void sell_node_rows(gptr<const int32_t> sl_ptr, gptr<const int32_t> scol,
+                                               gptr<const VT> sval, int64_t a, gptr<const XT> z,
+                                               gptr<const XT> pold, const double (&gam)[RR],
                                                const double (&beta)[RR], double (&acc)[3][RR]) {
     const int64_t s = a / kSlice;
     const int lane = (int)(a % kSlice);
-    const int k0 = sl_ptr[s];
-    const int k1 = k0 + (nb_ptr[a + 1] - nb_ptr[a]);
-    for (int k = k0; k < k1; k += 2) {
-        int c[2];
-        double v[2][9];
+    // every active lane of the wavefront belongs to slice s
+    const int k0 = __builtin_amdgcn_readfirstlane(sl_ptr[s]);
+    const int kw = __builtin_amdgcn_readfirstlane(sl_ptr[s + 1]);
+    int cn0 = scol[(size_t)k0 * kSlice + lane];
+    int cn1 = k0 + 1 < kw ? scol[(size_t)(k0 + 1) * kSlice + lane] : 0;
+    for (int k = k0; k < kw; k += 2) {
+        const bool two = k + 1 < kw;
+        const int c0 = cn0, c1 = cn1;
+        double x0[3][RR], x1[3][RR];
+        VT v0[9], v1[9];
+        gather_block<VT, XT, RR>(z, pold, c0, gam, beta, x0);
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int kk = k + u;
-            c[u] = kk < k1 ? scol[(size_t)kk * kSlice + lane] : -1;
-            if (kk < k1)
+        for (int e = 0; e < 9; ++e)
+            v0[e] = sval[((size_t)k * 9 + e) * kSlice + lane];
+        if (two) {
+            gather_block<VT, XT, RR>(z, pold, c1, gam, beta, x1);
 #pragma unroll
-                for (int e = 0; e < 9; ++e)
-                    v[u][e] = (double)sval[((size_t)kk * 9 + e) * kSlice + lane];
+            for (int e = 0; e < 9; ++e)
+                v1[e] = sval[((size_t)(k + 1) * 9 + e) * kSlice + lane];
         }
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            if (c[u] < 0)
-                continue;
-            double x[3][RR];
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-#pragma unroll
-                for (int r = 0; r < RR; ++r) {
-                    const int64_t at = (3 * (int64_t)c[u] + q) * RR + r;
-                    x[q][r] = pnew_at<XT>(z, pold, at, gam[r], beta[r]);  // as its owner stores it
-                }
-#pragma unroll
-            for (int r = 0; r < RR; ++r)
-#pragma unroll
-                for (int p = 0; p < 3; ++p) {
-                    acc[p][r] += v[u][3 * p] * x[0][r];
-                    acc[p][r] += v[u][3 * p + 1] * x[1][r];
-                    acc[p][r] += v[u][3 * p + 2] * x[2][r];
-                }
-        }
+        if (k + 2 < kw)
+            cn0 = scol[(size_t)(k + 2) * kSlice + lane];
+        if (k + 3 < kw)
+            cn1 = scol[(size_t)(k + 3) * kSlice + lane];
+        madd_block<VT, RR>(v0, x0, acc);
+        if (two)
+            madd_block<VT, RR>(v1, x1, acc);
     }
 }
 
