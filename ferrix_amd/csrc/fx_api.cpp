@@ -53,7 +53,7 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 // symbolic phase
 // ---------------------------------------------------------------------------
 struct Symbolic {
-    std::vector<int32_t> nb_ptr, nb_col, blk_cptr, blk_cbase, nod_cptr, nod_cbase, diag_slot;
+    std::vector<int32_t> nb_ptr, nb_col, blk_row, blk_cptr, blk_cbase, nod_cptr, nod_cbase, diag_slot;
 };
 
 // Elements are visited in the given order.  Contribution bases point into
@@ -158,6 +158,10 @@ Symbolic build_symbolic(int64_t n_nodes, int64_t n_el, const int64_t *el_ptr, co
     for (int64_t b = 0; b < nblk; ++b)
         bc[b + 1] += bc[b];
     S.blk_cptr.assign(bc.begin(), bc.end());
+    S.blk_row.resize(nblk);
+    for (int64_t a = 0; a < n_nodes; ++a)
+        for (int32_t b = S.nb_ptr[a]; b < S.nb_ptr[a + 1]; ++b)
+            S.blk_row[b] = (int32_t)a;
     S.blk_cbase.resize(npairs);
     std::vector<int64_t> fill(bc.begin(), bc.end() - 1);
     q = 0;
@@ -397,6 +401,7 @@ fx_status fx_mesh_upload(fx_ctx *ctx, int64_t n_nodes, const double *xyz, const 
     m->nb_ptr.upload(S.nb_ptr, s);
     m->nb_col.upload(S.nb_col, s);
     build_sell(m, s);
+    m->blk_row.upload(S.blk_row, s);
     m->blk_cptr.upload(S.blk_cptr, s);
     m->blk_cbase.upload(S.blk_cbase, s);
     m->nod_cptr.upload(S.nod_cptr, s);
@@ -423,7 +428,7 @@ fx_status fx_mesh_upload(fx_ctx *ctx, int64_t n_nodes, const double *xyz, const 
     FX_HIP(hipStreamSynchronize(s));
     m->device_bytes = 0;
     for (size_t b : {m->xyz.bytes(), m->temp.bytes(), m->mats.bytes(), m->blk4.conn.bytes(), m->blk20.conn.bytes(),
-                     m->nb_ptr.bytes(), m->nb_col.bytes(), m->blk_cptr.bytes(), m->blk_cbase.bytes(),
+                     m->nb_ptr.bytes(), m->nb_col.bytes(), m->blk_row.bytes(), m->blk_cptr.bytes(), m->blk_cbase.bytes(),
                      m->nod_cptr.bytes(), m->nod_cbase.bytes(), m->diag_slot.bytes(), m->ke.bytes(), m->fe.bytes(),
                      m->kval.bytes(), m->keep.bytes(), m->part.bytes()})
         m->device_bytes += (int64_t)b;

@@ -11,10 +11,12 @@
 //
 // Assembly is a gather, not a scatter: the host symbolic phase lists, for
 // every 3x3 node block of the structural pattern, the element contributions
-// in element order.  One wavefront owns one DOF row; each lane owns slots of
-// that row and sums the kept contributions in list order, which is the order
-// in which the reference's triplets would be summed when elements are
-// visited by ascending ID.  No floating-point atomics.  The structural
+// in element order.  One lane owns one block: it reads each contribution's
+// nine values (72 contiguous bytes) once and sums the kept ones per slot in
+// list order, which is the order in which the reference's triplets would be
+// summed when elements are visited by ascending ID.  No floating-point
+// atomics (the max diagonal is an integer max over non-negative doubles'
+// bits).  The structural
 // pattern stays the compute pattern; slots whose contributions were all
 // dropped hold 0.0 and keep = 0, so the exported CSR matches the filtered
 // one while SpMV sums are unchanged (0 * x adds nothing).
@@ -25,42 +27,66 @@ namespace {
 
 constexpr int kWave = 64;
 
-__global__ __launch_bounds__(256) void k_assemble(int64_t n_dof, const int32_t *__restrict__ nb_ptr,
+__global__ __launch_bounds__(256) void k_assemble(int64_t nblk, const int32_t *__restrict__ nb_ptr,
                                                   const int32_t *__restrict__ nb_col,
+                                                  const int32_t *__restrict__ blk_row,
                                                   const int32_t *__restrict__ blk_cptr,
                                                   const int32_t *__restrict__ blk_cbase,
                                                   const double *__restrict__ ke, double *__restrict__ kval,
                                                   uint8_t *__restrict__ keep,
                                                   unsigned long long *__restrict__ max_diag_bits) {
-    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int lane = threadIdx.x % kWave;
-    if (row >= n_dof)
-        return;
-    const int a = (int)(row / 3), p = (int)(row % 3);
-    const int b0 = nb_ptr[a], deg = nb_ptr[a + 1] - b0;
-    const int64_t base = 9 * (int64_t)b0 + (int64_t)p * 3 * deg;
+    __shared__ double s_max[256 / kWave];
+    const int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double dmax = 0.0;
-    for (int k = lane; k < 3 * deg; k += kWave) {
-        const int kb = k / 3, q = k % 3;
-        const int blk = b0 + kb;
-        const bool diag = (q == p) && (nb_col[blk] == a);
-        double s = 0.0;
-        uint8_t kept = 0;
+    if (blk < nblk) {
+        const int a = blk_row[blk];
+        const int b0 = nb_ptr[a], deg = nb_ptr[a + 1] - b0;
+        const bool diag = nb_col[blk] == a;
+        double sum[9];
+        bool kept[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            sum[e] = 0.0;
+            kept[e] = false;
+        }
         const int c1 = blk_cptr[blk + 1];
         for (int c = blk_cptr[blk]; c < c1; ++c) {
-            double v = ke[(int64_t)blk_cbase[c] + 3 * p + q];
-            if (fabs(v) > kDropTol) {
-                s += v;
-                kept = 1;
-                if (diag)
-                    dmax = fmax(dmax, fabs(v));
-            }
+            const double *v = ke + blk_cbase[c];  // the element's 3x3 block, row-major
+            double w[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                w[e] = v[e];
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                if (fabs(w[e]) > kDropTol) {
+                    sum[e] += w[e];
+                    kept[e] = true;
+                    if (diag && e % 4 == 0)  // (p, p) of the diagonal block
+                        dmax = fmax(dmax, fabs(w[e]));
+                }
         }
-        kval[base + k] = s;
-        keep[base + k] = kept;
+        const int64_t base = 9 * (int64_t)b0 + 3 * (blk - b0);  // slot (a, p=0, k, q=0)
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                kval[base + (int64_t)p * 3 * deg + q] = sum[3 * p + q];
+                keep[base + (int64_t)p * 3 * deg + q] = kept[3 * p + q];
+            }
     }
-    if (dmax > 0.0)
-        atomicMax(max_diag_bits, (unsigned long long)__double_as_longlong(dmax));
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1)
+        dmax = fmax(dmax, __shfl_xor(dmax, off, kWave));
+    if (threadIdx.x % kWave == 0)
+        s_max[threadIdx.x / kWave] = dmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = s_max[0];
+        for (int w = 1; w < 256 / kWave; ++w)
+            m = fmax(m, s_max[w]);
+        if (m > 0.0)
+            atomicMax(max_diag_bits, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // f_int[3a+p] = sum over the elements of node a, in element order.
@@ -273,11 +299,10 @@ inline unsigned grid1(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 void launch_assemble(fx_mesh *m) {
     hipStream_t s = m->ctx->stream;
     FX_HIP(hipMemsetAsync(m->max_diag_bits.p, 0, sizeof(unsigned long long), s));
-    const int threads = 256;
-    int64_t waves = m->n_dof;
-    unsigned blocks = (unsigned)((waves * kWave + threads - 1) / threads);
-    hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(threads), 0, s, m->n_dof, m->nb_ptr.p, m->nb_col.p,
-                       m->blk_cptr.p, m->blk_cbase.p, m->ke.p, m->kval.p, m->keep.p, m->max_diag_bits.p);
+    if (m->nblk > 0)
+        hipLaunchKernelGGL(k_assemble, dim3(grid1(m->nblk, 256)), dim3(256), 0, s, m->nblk, m->nb_ptr.p,
+                           m->nb_col.p, m->blk_row.p, m->blk_cptr.p, m->blk_cbase.p, m->ke.p, m->kval.p, m->keep.p,
+                           m->max_diag_bits.p);
     FX_HIP(hipGetLastError());
 }
 

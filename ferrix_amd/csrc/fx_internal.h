@@ -170,6 +170,7 @@ struct fx_mesh {
     int64_t ke_total = 0, fe_total = 0;
     fx::DBuf<double> ke, fe;             // element contributions
     fx::DBuf<int32_t> nb_ptr, nb_col;    // n_nodes+1, nblk
+    fx::DBuf<int32_t> blk_row;              // per 3x3 block: its row node
     fx::DBuf<int32_t> blk_cptr, blk_cbase;  // per block: contribution bases into ke
     fx::DBuf<int32_t> nod_cptr, nod_cbase;  // per node: bases into fe
     fx::DBuf<int32_t> diag_slot;         // per dof: slot of K[d,d]
