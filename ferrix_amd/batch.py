@@ -65,8 +65,8 @@ def assemble(s: System):
         if k == 0:
             b, s.penalty = s.dm.apply_penalty(st0.bc_dof, st0.bc_val, b)
         elif s.penalty > 0.0:
-            for d, t in zip(st0.bc_dof, st0.bc_val):
-                b[d] += s.penalty * (t - 0.0)
+            # one addition per record, in record order (np.add.at is unbuffered)
+            np.add.at(b, np.asarray(st0.bc_dof), s.penalty * (np.asarray(st0.bc_val, np.float64) - 0.0))
         rows.append(b)
     s.rhs = np.array(rows)
 
@@ -111,11 +111,14 @@ def operator_bytes(s: System, opts: SolveOptions) -> int:
     return s.dm.nnz_struct * vs + (s.dm.nnz_struct // 9) * 4
 
 
-def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int = 0, chunk_iters: int = 32,
+def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int = 0, chunk_iters: int = 256,
                   time_spmv: bool = False) -> PopulationResult:
     """Continuous batching over `systems` in order.  ws_bytes > 0 caps the
     summed operator bytes of the running systems (at least one always runs);
-    0 admits everything at once."""
+    0 admits everything at once.  Each Batch.run call advances the running
+    systems by up to chunk_iters iterations (the library overlaps the host's
+    bookkeeping of one 16-iteration chunk with the next); finished systems
+    are collected and new ones admitted between calls."""
     res = PopulationResult()
     bt = Batch(ctx, opts)
     if time_spmv:
@@ -137,7 +140,8 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
                 running[slot] = s
                 ws += need
                 qi += 1
-            bt.run(max_iters=chunk_iters)
+            if bt.run(max_iters=chunk_iters) == len(running):
+                continue  # none finished: nothing to collect, no memory freed
             for slot in list(running):
                 st, rep = bt.poll(slot)
                 if st == 0:

@@ -38,6 +38,7 @@
 #include "fx_sell.h"
 
 #include <chrono>
+#include <cstring>
 #include <memory>
 
 namespace fx {
@@ -64,6 +65,16 @@ struct BRhs {
     int32_t nupd, max_upd;  // mixed: replacements done, allowed
 };
 
+// What the host needs of a slot once none of its load cases runs, written
+// by the GPU into pinned host memory (no copy command at chunk ends).
+struct BFin {
+    int64_t it, nupd;
+    double crit[kRmax];
+    int32_t status[kRmax];
+    int32_t done;  // written last
+    int32_t pad;
+};
+
 struct BSys {
     const int32_t *nb_ptr, *nb_col;
     const void *val;                    // double (FP64 slots) or float (mixed inner)
@@ -87,6 +98,7 @@ struct BSys {
     int32_t go;                         // right-hand sides the next SpMV / update work on (bit mask)
     int32_t upd;                        // mixed: right-hand sides whose residual is replaced in this iteration
     int32_t trig;                       // mixed: those of them whose own residual asked for it
+    int32_t reported;                   // the slot's BFin has been written
     BRhs rhs[kRmax];
 };
 
@@ -164,16 +176,38 @@ __global__ __launch_bounds__(kBB) void kb_init(BSys *__restrict__ sys, const int
 // right-hand sides (status 5, S.upd) instead of a Krylov step.  After a
 // replacement (S.upd set, partials r.z, r.r, r.p of the new residual):
 // beta and gamma of the next direction.
-__global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int32_t *__restrict__ list) {
-    BSys &S = sys[list[blockIdx.x]];
+__global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int32_t *__restrict__ list,
+                                               BFin *__restrict__ fin) {
+    const int slot = list[blockIdx.x];
+    BSys &S = sys[slot];
     const int R = S.R;
     const int upd = S.upd;
     bool any = false;
     for (int r = 0; r < R; ++r)
         any |= S.rhs[r].status == 0;
     if (!any) {  // uniform: only this block's thread 0 writes these states, in earlier launches
-        if (upd && threadIdx.x == 0)
-            S.upd = 0;  // every replaced right-hand side stopped
+        if (threadIdx.x == 0) {
+            if (upd)
+                S.upd = 0;  // every replaced right-hand side stopped
+            if (!S.reported) {  // the slot has just finished: tell the host
+                BFin &F = fin[slot];
+                int64_t it = 0, nupd = 0;
+                for (int r = 0; r < kRmax; ++r) {
+                    const BRhs &h = S.rhs[r];
+                    F.crit[r] = h.crit;
+                    F.status[r] = h.status;
+                    if (r < R) {
+                        it = max(it, h.status == 2 ? h.it_k1 + 1 : h.it_k1);
+                        nupd = max(nupd, (int64_t)h.nupd);
+                    }
+                }
+                F.it = it;
+                F.nupd = nupd;
+                __threadfence_system();
+                F.done = 1;
+                S.reported = 1;
+            }
+        }
         return;
     }
     double red[3 * kRmax];
@@ -791,12 +825,30 @@ struct fx_batch {
     std::vector<fx::BSys> h_sys;
     fx::DBuf<fx::BSys> d_sys;
     static constexpr int kGroups = 6;  // lane groups 2, 4, ..., 64
-    fx::DBuf<int2> d_tab1g[kGroups], d_tab1s, d_tab2, d_tab4;
-    int n1g[kGroups] = {0, 0, 0, 0, 0, 0};
-    fx::DBuf<int32_t> d_list2, d_run;
+    // Iteration chunks are pipelined: while chunk k runs, the host processes
+    // the state of chunk k-1.  Each of the two chunks in flight has its own
+    // block tables (device, and their pinned host staging), its own pinned
+    // copy of the system table taken at its end, and its own events.
+    struct ChunkSet {
+        fx::DBuf<int2> tab1g[kGroups], tab1s, tab2;
+        fx::DBuf<int32_t> run;
+        int n1g[kGroups] = {0, 0, 0, 0, 0, 0};
+        int n1s = 0, n2 = 0, nrun = 0;
+        int64_t version = -1;          // of the running-slot set these tables were built for
+        int2 *h_tab = nullptr;         // pinned staging of the tables
+        size_t h_tab_cap = 0;
+        hipEvent_t done = nullptr;
+        std::vector<hipEvent_t> ev;    // SpMV timing: 2 per iteration
+        int ev_used = 0;
+    };
+    ChunkSet cs[2];
+    int64_t run_version = 0;           // bumped when the tables must be rebuilt
+    int64_t tab_blocks = 0, dead_blocks = 0;  // in the current tables, and of slots finished since
+    fx::BFin *fin = nullptr;           // pinned host memory, one per slot capacity
+    size_t fin_cap = 0;
+    fx::DBuf<int2> d_tab4;
+    fx::DBuf<int32_t> d_list2;
     std::vector<int2> h_tab_start;
-    int n1s = 0, n2 = 0, nrun = 0;
-    bool tables_dirty = true;
     int64_t resident_bytes = 0;
     // SpMV timing (instrumentation): when on, every SpMV launch is bracketed
     // by two HIP events on the stream; the device counter sums the launches'
@@ -804,13 +856,19 @@ struct fx_batch {
     bool timing = false;
     int64_t spmv_launches = 0, timed = 0;
     double timed_ms = 0.0;
-    std::vector<hipEvent_t> ev;  // 2 per iteration of a chunk
-    int ev_used = 0;
     fx::DBuf<double> d_bytes;
     ~fx_batch() {
-        for (auto &e : ev)
-            if (e)
-                (void)hipEventDestroy(e);
+        if (fin)
+            (void)hipHostFree(fin);
+        for (auto &c : cs) {
+            for (auto &e : c.ev)
+                if (e)
+                    (void)hipEventDestroy(e);
+            if (c.done)
+                (void)hipEventDestroy(c.done);
+            if (c.h_tab)
+                (void)hipHostFree(c.h_tab);
+        }
     }
 };
 
@@ -902,13 +960,6 @@ void xcd_permute(std::vector<int2> &t) {
     t.swap(perm);
 }
 
-void upload_table(fx::DBuf<int2> &d, const std::vector<int2> &t, hipStream_t s) {
-    if (t.empty())
-        return;
-    d.alloc(std::max<size_t>(d.n, t.size()));
-    FX_HIP(hipMemcpyAsync(d.p, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-}
-
 inline int group_index(int G) {  // 2 -> 0, 4 -> 1, ..., 64 -> 5
     int i = 0;
     while ((2 << i) < G)
@@ -916,9 +967,25 @@ inline int group_index(int G) {  // 2 -> 0, 4 -> 1, ..., 64 -> 5
     return i;
 }
 
-// block tables of the running slots: tab1g / tab1s SpMV blocks (lane-group
-// and sliced-ELL operators), tab2 vector blocks, run the running slots
-void build_tables(fx_batch *B) {
+void ensure_pinned(int2 *&p, size_t &cap, size_t n) {
+    if (cap >= n)
+        return;
+    if (p)
+        FX_HIP(hipHostFree(p));
+    p = nullptr;
+    cap = 0;
+    const size_t c = std::max<size_t>(n, 2 * cap + 1024);
+    FX_HIP(hipHostMalloc((void **)&p, c * sizeof(int2), hipHostMallocDefault));
+    cap = c;
+}
+
+// Block tables of the running slots for chunk set `set`: tab1g / tab1s SpMV
+// blocks (lane-group and sliced-ELL operators), tab2 vector blocks, run the
+// running slots.  The tables are staged in the set's pinned buffer and
+// uploaded on the stream without waiting: the set's previous chunk has been
+// processed (its event waited), so neither buffer is still being read.
+void build_tables(fx_batch *B, int set) {
+    fx_batch::ChunkSet &C = B->cs[set];
     std::vector<int2> t1g[fx_batch::kGroups], t1s, t2;
     std::vector<int> run;
     for (size_t k = 0; k < B->slots.size(); ++k) {
@@ -932,32 +999,64 @@ void build_tables(fx_batch *B) {
         for (int b = 0; b < S.nparts2; ++b)
             t2.push_back(make_int2((int)k, b));
     }
-    for (int g = 0; g < fx_batch::kGroups; ++g) {
-        xcd_permute(t1g[g]);
-        B->n1g[g] = (int)t1g[g].size();
-        upload_table(B->d_tab1g[g], t1g[g], B->ctx->stream);
-    }
-    xcd_permute(t1s);
-    B->n1s = (int)t1s.size();
-    B->n2 = (int)t2.size();
-    B->nrun = (int)run.size();
+    size_t total = t1s.size() + t2.size() + (run.size() + 1) / 2 + 1;
+    for (auto &t : t1g)
+        total += t.size();
+    ensure_pinned(C.h_tab, C.h_tab_cap, total);
     hipStream_t s = B->ctx->stream;
-    if (B->nrun) {
-        B->d_run.alloc(std::max<size_t>(B->d_run.n, run.size()));
-        FX_HIP(hipMemcpyAsync(B->d_run.p, run.data(), run.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    size_t off = 0;
+    auto stage = [&](fx::DBuf<int2> &d, std::vector<int2> &t) {
+        if (t.empty())
+            return;
+        xcd_permute(t);
+        std::copy(t.begin(), t.end(), C.h_tab + off);
+        d.alloc(std::max<size_t>(d.n, t.size()));
+        FX_HIP(hipMemcpyAsync(d.p, C.h_tab + off, t.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+        off += t.size();
+    };
+    for (int g = 0; g < fx_batch::kGroups; ++g) {
+        C.n1g[g] = (int)t1g[g].size();
+        stage(C.tab1g[g], t1g[g]);
     }
-    upload_table(B->d_tab1s, t1s, s);
-    upload_table(B->d_tab2, t2, s);
-    FX_HIP(hipStreamSynchronize(s));  // the host tables are temporaries
-    B->tables_dirty = false;
+    C.n1s = (int)t1s.size();
+    stage(C.tab1s, t1s);
+    C.n2 = (int)t2.size();
+    std::copy(t2.begin(), t2.end(), C.h_tab + off);  // vector blocks: no XCD permutation
+    if (!t2.empty()) {
+        C.tab2.alloc(std::max<size_t>(C.tab2.n, t2.size()));
+        FX_HIP(hipMemcpyAsync(C.tab2.p, C.h_tab + off, t2.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+        off += t2.size();
+    }
+    C.nrun = (int)run.size();
+    if (C.nrun) {
+        int32_t *h = (int32_t *)(C.h_tab + off);
+        std::copy(run.begin(), run.end(), h);
+        C.run.alloc(std::max<size_t>(C.run.n, run.size()));
+        FX_HIP(hipMemcpyAsync(C.run.p, h, run.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    }
+    C.version = B->run_version;
+    int64_t blocks = (int64_t)t1s.size() + (int64_t)t2.size();
+    for (auto &t : t1g)
+        blocks += (int64_t)t.size();
+    B->tab_blocks = blocks;
+    B->dead_blocks = 0;
 }
 
-// grows the device system table keeping its contents (running slots' state
-// lives there)
 void reserve_sys(fx_batch *B, size_t n) {
     if (B->d_sys.p && B->d_sys.n >= n)
         return;
     const size_t cap = std::max<size_t>({n, 2 * B->d_sys.n, 16});
+    // the finish summaries: pinned host memory the kernels write (no chunk is
+    // in flight between fx_batch_run calls, when this runs)
+    fx::BFin *f = nullptr;
+    FX_HIP(hipHostMalloc((void **)&f, cap * sizeof(fx::BFin), hipHostMallocDefault));
+    std::memset(f, 0, cap * sizeof(fx::BFin));
+    if (B->fin) {
+        std::memcpy(f, B->fin, B->fin_cap * sizeof(fx::BFin));
+        FX_HIP(hipHostFree(B->fin));
+    }
+    B->fin = f;
+    B->fin_cap = cap;
     BSys *p = nullptr;
     FX_HIP(hipMalloc(&p, cap * sizeof(BSys)));
     if (B->d_sys.p) {
@@ -967,18 +1066,6 @@ void reserve_sys(fx_batch *B, size_t n) {
     }
     B->d_sys.p = p;
     B->d_sys.n = cap;
-}
-
-void upload_sys(fx_batch *B) {
-    reserve_sys(B, B->h_sys.size());
-    FX_HIP(hipMemcpyAsync(B->d_sys.p, B->h_sys.data(), B->h_sys.size() * sizeof(BSys), hipMemcpyHostToDevice,
-                          B->ctx->stream));
-}
-
-void download_sys(fx_batch *B) {
-    FX_HIP(hipMemcpyAsync(B->h_sys.data(), B->d_sys.p, B->h_sys.size() * sizeof(BSys), hipMemcpyDeviceToHost,
-                          B->ctx->stream));
-    FX_HIP(hipStreamSynchronize(B->ctx->stream));
 }
 
 // kb_init over the vector blocks of `slots`, then kb_red2 on them: their
@@ -1001,100 +1088,98 @@ void start_slots(fx_batch *B, const std::vector<int> &slots) {
     else
         hipLaunchKernelGGL(fx::kb_init<double>, dim3((unsigned)t.size()), dim3(kBB), 0, s, B->d_sys.p,
                            B->d_tab4.p);
-    hipLaunchKernelGGL(fx::kb_red2, dim3((unsigned)slots.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_list2.p);
+    hipLaunchKernelGGL(fx::kb_red2, dim3((unsigned)slots.size()), dim3(kBB), 0, s, B->d_sys.p, B->d_list2.p,
+                       B->fin);
     FX_HIP(hipGetLastError());
     FX_HIP(hipStreamSynchronize(s));  // the host tables are reused
 }
 
-void launch_iterations(fx_batch *B, int n) {
+void launch_iterations(fx_batch *B, int n, int set) {
+    fx_batch::ChunkSet &C = B->cs[set];
     hipStream_t s = B->ctx->stream;
-    if (B->timing && (int)B->ev.size() < 2 * n) {
-        const size_t old = B->ev.size();
-        B->ev.resize(2 * n, nullptr);
-        for (size_t k = old; k < B->ev.size(); ++k)
-            FX_HIP(hipEventCreate(&B->ev[k]));
+    if (B->timing && (int)C.ev.size() < 2 * n) {
+        const size_t old = C.ev.size();
+        C.ev.resize(2 * n, nullptr);
+        for (size_t k = old; k < C.ev.size(); ++k)
+            FX_HIP(hipEventCreate(&C.ev[k]));
     }
     for (int k = 0; k < n; ++k) {
         if (B->timing)
-            FX_HIP(hipEventRecord(B->ev[2 * k], s));
+            FX_HIP(hipEventRecord(C.ev[2 * k], s));
         for (int g = 0; g < fx_batch::kGroups; ++g)
-            if (B->n1g[g]) {
+            if (C.n1g[g]) {
                 if (is_mixed(B))
-                    fx::launch_bspmv<float>(2 << g, B->n1g[g], s, B->d_sys.p, B->d_tab1g[g].p);
+                    fx::launch_bspmv<float>(2 << g, C.n1g[g], s, B->d_sys.p, C.tab1g[g].p);
                 else
-                    fx::launch_bspmv<double>(2 << g, B->n1g[g], s, B->d_sys.p, B->d_tab1g[g].p);
+                    fx::launch_bspmv<double>(2 << g, C.n1g[g], s, B->d_sys.p, C.tab1g[g].p);
             }
-        if (B->n1s) {
+        if (C.n1s) {
             if (is_mixed(B))
-                hipLaunchKernelGGL((fx::kb_spmv_sell<float, float>), dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p,
-                                   B->d_tab1s.p);
+                hipLaunchKernelGGL((fx::kb_spmv_sell<float, float>), dim3(C.n1s), dim3(kBB), 0, s, B->d_sys.p,
+                                   C.tab1s.p);
             else
-                hipLaunchKernelGGL((fx::kb_spmv_sell<double, double>), dim3(B->n1s), dim3(kBB), 0, s, B->d_sys.p,
-                                   B->d_tab1s.p);
+                hipLaunchKernelGGL((fx::kb_spmv_sell<double, double>), dim3(C.n1s), dim3(kBB), 0, s, B->d_sys.p,
+                                   C.tab1s.p);
         }
         if (B->timing)
-            FX_HIP(hipEventRecord(B->ev[2 * k + 1], s));
-        hipLaunchKernelGGL(fx::kb_red1, dim3(B->nrun), dim3(kBB), 0, s, B->d_sys.p, B->d_run.p);
+            FX_HIP(hipEventRecord(C.ev[2 * k + 1], s));
+        hipLaunchKernelGGL(fx::kb_red1, dim3(C.nrun), dim3(kBB), 0, s, B->d_sys.p, C.run.p);
         if (is_mixed(B))
-            hipLaunchKernelGGL(fx::kb_update<float>, dim3(B->n2), dim3(kBB), 0, s, B->d_sys.p, B->d_tab2.p);
+            hipLaunchKernelGGL(fx::kb_update<float>, dim3(C.n2), dim3(kBB), 0, s, B->d_sys.p, C.tab2.p);
         else
-            hipLaunchKernelGGL(fx::kb_update<double>, dim3(B->n2), dim3(kBB), 0, s, B->d_sys.p, B->d_tab2.p);
-        hipLaunchKernelGGL(fx::kb_red2, dim3(B->nrun), dim3(kBB), 0, s, B->d_sys.p, B->d_run.p);
+            hipLaunchKernelGGL(fx::kb_update<double>, dim3(C.n2), dim3(kBB), 0, s, B->d_sys.p, C.tab2.p);
+        hipLaunchKernelGGL(fx::kb_red2, dim3(C.nrun), dim3(kBB), 0, s, B->d_sys.p, C.run.p, B->fin);
     }
-    if (B->timing)
-        B->ev_used = 2 * n;
+    C.ev_used = B->timing ? 2 * n : 0;
     B->spmv_launches += n;
     FX_HIP(hipGetLastError());
+    if (!C.done)
+        FX_HIP(hipEventCreateWithFlags(&C.done, hipEventDisableTiming));
+    FX_HIP(hipEventRecord(C.done, s));
 }
 
-// after the stream has been synchronized
-void collect_sample(fx_batch *B) {
-    for (int k = 0; k + 1 < B->ev_used; k += 2) {
+// after the set's chunk has completed
+void collect_sample(fx_batch *B, int set) {
+    fx_batch::ChunkSet &C = B->cs[set];
+    for (int k = 0; k + 1 < C.ev_used; k += 2) {
         float ms = 0.f;
-        FX_HIP(hipEventElapsedTime(&ms, B->ev[k], B->ev[k + 1]));
+        FX_HIP(hipEventElapsedTime(&ms, C.ev[k], C.ev[k + 1]));
         B->timed_ms += ms;
         B->timed++;
     }
-    B->ev_used = 0;
+    C.ev_used = 0;
 }
 
-// After a chunk: the slots none of whose load cases still runs finish.
-// Returns true when the set of running slots changed.
-bool advance_phases(fx_batch *B) {
-    download_sys(B);
-    collect_sample(B);
-    bool changed = false;
+// After a chunk has completed: the slots whose summary kb_red2 has written
+// (none of their load cases runs) finish.  A finished slot may stay in the
+// block tables for a while; its blocks see no running load case and return
+// at once.  The tables are rebuilt when a quarter of their blocks belong to
+// finished slots.
+void advance_phases(fx_batch *B, int set) {
+    collect_sample(B, set);
     for (size_t k = 0; k < B->slots.size(); ++k) {
         auto &sl = *B->slots[k];
         if (!sl.used || sl.phase != 0)
             continue;
-        const BSys &S = B->h_sys[k];
-        bool running = false;
-        int64_t it = 0, nupd = 0;
-        for (int r = 0; r < sl.R; ++r) {
-            const BRhs &h = S.rhs[r];
-            if (h.status == 0 || h.status == 5)
-                running = true;
-            it = std::max(it, h.status == 2 ? h.it_k1 + 1 : h.it_k1);
-            nupd = std::max<int64_t>(nupd, h.nupd);
-        }
-        if (running)
+        const fx::BFin &F = B->fin[k];
+        if (!((volatile const int32_t &)F.done))
             continue;
-        sl.iters = it;
-        sl.outer = nupd;
+        sl.iters = F.it;
+        sl.outer = F.nupd;
         for (int r = 0; r < sl.R; ++r) {
-            const BRhs &h = S.rhs[r];
-            sl.crit[r] = h.crit;
-            if (h.status == 2)
+            sl.crit[r] = F.crit[r];
+            if (F.status[r] == 2)
                 sl.err = FX_ERR_BREAKDOWN;
-            else if (h.status == 3 && sl.err == FX_OK)
+            else if (F.status[r] == 3 && sl.err == FX_OK)
                 sl.err = FX_ERR_NOT_CONVERGED;
         }
         sl.phase = 2;
         sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
-        changed = true;
+        const BSys &S = B->h_sys[k];
+        B->dead_blocks += S.nparts1 + S.nparts2;
     }
-    return changed;
+    if (4 * B->dead_blocks > B->tab_blocks)
+        B->run_version++;
 }
 
 }  // namespace
@@ -1106,7 +1191,7 @@ void batch_drop_mesh(fx_batch *B, fx_mesh *m) {
     for (size_t k = 0; k < B->slots.size(); ++k)
         if (B->slots[k] && B->slots[k]->used && B->slots[k]->mesh == m) {
             if (B->slots[k]->phase == 0)
-                B->tables_dirty = true;
+                B->run_version++;
             B->h_sys[k] = BSys{};
             B->slots[k].reset(new fx_batch::Slot());
         }
@@ -1241,9 +1326,10 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         // running slots' device state is newer than the host copy: upload
         // this entry only
         reserve_sys(B, B->h_sys.size());
+        std::memset(&B->fin[k], 0, sizeof(fx::BFin));
         FX_HIP(hipMemcpyAsync(B->d_sys.p + k, &S, sizeof(BSys), hipMemcpyHostToDevice, s));
         start_slots(B, std::vector<int>{k});  // start this slot's solve now
-        B->tables_dirty = true;
+        B->run_version++;
         *slot_out = k;
     } catch (const fx::Error &e) {
         fx::set_error(e.msg);
@@ -1262,20 +1348,30 @@ fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
     try {
         FX_HIP(hipSetDevice(ctx->device));
         int64_t done_iters = 0;
+        int inflight = 0, head = 0;  // chunks launched and not yet processed; the oldest one's set
         for (;;) {
-            if (B->tables_dirty)
-                build_tables(B);
-            if (B->nrun == 0)
+            // keep two chunks in flight while slots run and the budget lasts
+            int running = 0;
+            for (auto &sl : B->slots)
+                running += sl && sl->used && sl->phase == 0;
+            if (inflight < 2 && running > 0 && (max_iters <= 0 || done_iters < max_iters)) {
+                const int set = (head + inflight) % 2;
+                if (B->cs[set].version != B->run_version)
+                    build_tables(B, set);
+                int chunk = 16;
+                if (max_iters > 0)
+                    chunk = (int)std::min<int64_t>(chunk, max_iters - done_iters);
+                launch_iterations(B, chunk, set);
+                done_iters += chunk;
+                ++inflight;
+                continue;
+            }
+            if (inflight == 0)
                 break;
-            int chunk = 16;
-            if (max_iters > 0)
-                chunk = (int)std::min<int64_t>(chunk, max_iters - done_iters);
-            launch_iterations(B, chunk);
-            done_iters += chunk;
-            if (advance_phases(B))
-                B->tables_dirty = true;
-            if (max_iters > 0 && done_iters >= max_iters)
-                break;
+            FX_HIP(hipEventSynchronize(B->cs[head].done));
+            advance_phases(B, head);
+            head ^= 1;
+            --inflight;
         }
         if (n_running_out) {
             int c = 0;
@@ -1373,7 +1469,7 @@ fx_status fx_batch_remove(fx_batch *B, int32_t slot) {
     if (slot < (int)B->h_sys.size())
         B->h_sys[slot] = BSys{};
     if (was_running)
-        B->tables_dirty = true;
+        B->run_version++;
     B->slots[slot].reset(new fx_batch::Slot());
     return FX_OK;
 }
