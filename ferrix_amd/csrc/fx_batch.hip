@@ -811,7 +811,7 @@ struct fx_batch {
         bool used = false;
         fx_mesh *mesh = nullptr;
         int R = 0;
-        int G = 8;      // lanes per node row (lane-group SpMV, outer pass)
+        int G = 8;      // lanes per node row (lane-group SpMV, replacement pass)
         int phase = 0;  // 0 running, 2 finished
         fx_status err = FX_OK;
         int64_t iters = 0, outer = 0;  // Krylov iterations (slowest load case), residual replacements
@@ -819,7 +819,6 @@ struct fx_batch {
         fx::DBuf<double> b, rin, x, r, z, p0, p1, ap, xacc, part1, part2;
         std::chrono::steady_clock::time_point t0;
         double seconds = 0.0;
-        int64_t bytes = 0;
     };
     std::vector<std::unique_ptr<Slot>> slots;
     std::vector<fx::BSys> h_sys;
@@ -849,7 +848,6 @@ struct fx_batch {
     fx::DBuf<int2> d_tab4;
     fx::DBuf<int32_t> d_list2;
     std::vector<int2> h_tab_start;
-    int64_t resident_bytes = 0;
     // SpMV timing (instrumentation): when on, every SpMV launch is bracketed
     // by two HIP events on the stream; the device counter sums the launches'
     // algorithmic bytes (kb_red2)
@@ -1068,8 +1066,18 @@ void reserve_sys(fx_batch *B, size_t n) {
     B->d_sys.n = cap;
 }
 
+// an admission that threw: the slot goes back to the free list, and the
+// running slots' tables are unchanged
+void release_failed_slot(fx_batch *B, int k) {
+    if (k < 0 || k >= (int)B->slots.size())
+        return;
+    B->slots[k].reset(new fx_batch::Slot());
+    if (k < (int)B->h_sys.size())
+        B->h_sys[k] = BSys{};
+}
+
 // kb_init over the vector blocks of `slots`, then kb_red2 on them: their
-// inner solves start with the next iteration chunk.
+// solves start with the next iteration chunk.
 void start_slots(fx_batch *B, const std::vector<int> &slots) {
     if (slots.empty())
         return;
@@ -1271,6 +1279,7 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
     if (!B || !m || !rhs || !slot_out)
         return FX_ERR_INVALID;
     fx_ctx *ctx = B->ctx;
+    int k = -1;  // the slot taken, released again if admission fails
     try {
         if (m->ctx != ctx)
             throw fx::Error{FX_ERR_INVALID, "fx_batch_add: mesh belongs to another context"};
@@ -1283,7 +1292,7 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         FX_HIP(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         const int G = B->G > 0 ? B->G : (m->spmv_group > 0 ? m->spmv_group : fx::default_group(m));
-        int k = 0;
+        k = 0;
         while (k < (int)B->slots.size() && B->slots[k] && B->slots[k]->used)
             ++k;
         if (k == (int)B->slots.size())
@@ -1301,9 +1310,9 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
                 inter[i * n_rhs + r] = rhs[(int64_t)r * n + i];
         sl.b.upload(inter, s);
         for (auto *v : {&sl.x, &sl.r, &sl.z, &sl.p0, &sl.p1, &sl.ap})
-            v->alloc(is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed inner solve stores floats
+            v->alloc(is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed solve stores FP32 vectors
         if (is_mixed(B)) {
-            sl.rin.upload(inter, s);  // x = 0: the first residual is b
+            sl.rin.alloc(nv);  // replaced residuals: written by each replacement pass before use
             sl.xacc.alloc(nv);
             fx::launch_fill(m, sl.xacc.p, nv, 0.0);
             fx::launch_convert32(m);  // K may have been reassembled since the last conversion
@@ -1311,12 +1320,13 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         fx::launch_sell(m, is_mixed(B));
         fx::launch_jacobi(m, false);
         sl.G = G;
-        sl.part1.alloc((size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 4);  // >= SpMV and outer groups
+        // >= the SpMV's row groups (1 partial per load case) and the
+        // replacement pass's (3 per load case)
+        sl.part1.alloc((size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 4);
         sl.part2.alloc((size_t)std::max(fx::nblk(n, kBB), 1) * n_rhs * 3);
         sl.used = true;
         sl.phase = 0;
         sl.t0 = std::chrono::steady_clock::now();
-        sl.bytes = (int64_t)(nv * 8 * (is_mixed(B) ? 9 : 7));
         if ((int)B->h_sys.size() < (int)B->slots.size())
             B->h_sys.resize(B->slots.size());
         BSys &S = B->h_sys[k];
@@ -1332,9 +1342,11 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         B->run_version++;
         *slot_out = k;
     } catch (const fx::Error &e) {
+        release_failed_slot(B, k);
         fx::set_error(e.msg);
         return e.code;
     } catch (const std::exception &e) {
+        release_failed_slot(B, k);
         fx::set_error(e.what());
         return FX_ERR_OOM;
     }

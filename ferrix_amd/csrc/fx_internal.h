@@ -129,6 +129,13 @@ struct PcgScalars {
     double rz[2];    // r.z at iteration parity (k_spmv_p)
     double rr, bb, crit;  // last r.r, b.b, stop-rule value (k_spmv_p)
     double pap, alpha;    // last p.Ap and alpha (k_update)
+    // mixed precision: the first iteration after a residual replacement
+    // (set by k_resume_scalars, cleared by k_update) decides whether the
+    // direction is kept, comparing the replaced r.r with rr_prev, the
+    // iterated r.r the previous segment ended on
+    int32_t resume;
+    int32_t pad;
+    double rr_prev;
 };
 
 // One captured batch of PCG iterations.

@@ -97,9 +97,14 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t n, const double *__rest
 
 // Head of an SpMV launch: the block-uniform PCG status, r.z and r.r (and
 // b.b at iteration 0) from the previous update's partials, the stop rule
-// (block 0 records it), beta.  Returns false when the solve has stopped.
+// (block 0 records it), beta and gamma (p_new = gamma z + beta p_old).
+// Right after a residual replacement (sc->resume; the third partial is then
+// r.p of the kept direction) the direction is kept only when the replaced
+// residual is within 10% of the iterated one the segment ended on, with
+// gamma = 1 - beta (r.p)/(r.z) restoring r.p = 0; otherwise beta = 0.
+// Returns false when the solve has stopped.
 __device__ __forceinline__ bool pcg_head(PcgScalars *__restrict__ sc, const double *__restrict__ part2, int nparts2,
-                                         int64_t &it_out, double &beta_out) {
+                                         int64_t &it_out, double &beta_out, double &gamma_out) {
     __shared__ int32_t s_status;
     if (threadIdx.x == 0)
         s_status = sc->status;
@@ -136,9 +141,16 @@ __device__ __forceinline__ bool pcg_head(PcgScalars *__restrict__ sc, const doub
     }
     if (stop_max || conv)
         return false;
-    const double beta = it == 0 ? 0.0 : rz / sc->rz[(it - 1) & 1];
+    double beta = it == 0 ? 0.0 : rz / sc->rz[(it - 1) & 1];
+    double gamma = 1.0;
+    if (sc->resume) {
+        const bool keep = it > 0 && rr <= 1.21 * sc->rr_prev;
+        beta = keep ? beta : 0.0;
+        gamma = keep ? 1.0 - beta * (rz > 0.0 ? red[2] / rz : 0.0) : 1.0;
+    }
     it_out = it;
     beta_out = beta;
+    gamma_out = gamma;
     return true;
 }
 
@@ -151,9 +163,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                                                    int nparts2, double *__restrict__ part1,
                                                    const float *__restrict__ kc) {
     int64_t it;
-    double beta;
-    if (!pcg_head(sc, part2, nparts2, it, beta))
+    double beta, gamma;
+    if (!pcg_head(sc, part2, nparts2, it, beta, gamma))
         return;
+    // the FP64 solve never replaces its residual: keep its arithmetic as is
+    const double gz = sizeof(VT) == 4 ? gamma : 1.0;
     const double *pold = (it & 1) ? pb : pa;
     double *pnew = (it & 1) ? pa : pb;
 
@@ -189,7 +203,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 if (c[u] >= 0) {
-                    const double xj = zz[u] + beta * pp[u];
+                    const double xj = (sizeof(VT) == 4 ? gz * zz[u] : zz[u]) + beta * pp[u];
                     acc0 += w0[u] * xj;
                     acc1 += w1[u] * xj;
                     acc2 += w2[u] * xj;
@@ -206,9 +220,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     double loc[1] = {0.0};
     if (lane == 0 && a < n_nodes) {
         const int64_t d = 3 * a;
-        const double q0 = z[d] + beta * pold[d];
-        const double q1 = z[d + 1] + beta * pold[d + 1];
-        const double q2 = z[d + 2] + beta * pold[d + 2];
+        const double q0 = (sizeof(VT) == 4 ? gz * z[d] : z[d]) + beta * pold[d];
+        const double q1 = (sizeof(VT) == 4 ? gz * z[d + 1] : z[d + 1]) + beta * pold[d + 1];
+        const double q2 = (sizeof(VT) == 4 ? gz * z[d + 2] : z[d + 2]) + beta * pold[d + 2];
         if constexpr (sizeof(VT) == 4) {
             // FP32 operator: add the node's 3x3 row-sum correction block
             const float *c = kc + 9 * a;
@@ -275,6 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, double *__restrict
             sc->it = it + 1;
             sc->alpha = alpha;
             sc->pap = pap;
+            sc->resume = 0;  // read by the SpMV head that ran before this kernel
         }
     }
 }
@@ -358,6 +373,47 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restric
 #pragma unroll
         for (int k = 0; k < NV; ++k)
             out[k] = red[k];
+}
+
+// Residual replacement in the mixed solve (own DOFs): x = 0, r = r64,
+// z = r inv; partials (r.z, r.r, r.p) with p the kept direction (what the
+// next SpMV reads as p_old) -> part2.
+__global__ __launch_bounds__(kBlock) void k_resume(int64_t n, const double *__restrict__ r64,
+                                                   double *__restrict__ x, double *__restrict__ r,
+                                                   double *__restrict__ z, const double *__restrict__ p,
+                                                   const double *__restrict__ inv, double *__restrict__ part2) {
+    double v[3] = {0.0, 0.0, 0.0};
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const double ri = r64[i];
+        const double zi = ri * inv[i];
+        x[i] = 0.0;
+        r[i] = ri;
+        z[i] = zi;
+        v[0] = ri * zi;
+        v[1] = ri * ri;
+        v[2] = ri * p[i];
+    }
+    block_sum<3>(v);
+    if (threadIdx.x == 0) {
+        part2[3 * blockIdx.x] = v[0];
+        part2[3 * blockIdx.x + 1] = v[1];
+        part2[3 * blockIdx.x + 2] = v[2];
+    }
+}
+
+// The next segment of the mixed solve: running again, relative stop at
+// `tol` of the replaced residual (bb = its r.r), the resume flag for the
+// first SpMV head, and the iterated r.r the last segment ended on.
+__global__ void k_resume_scalars(PcgScalars *__restrict__ sc, double bb, double tol) {
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    sc->rr_prev = sc->rr;
+    sc->bb = bb;
+    sc->tol = tol;
+    sc->stop_mode = FX_STOP_REL_R2;
+    sc->status = 0;
+    sc->resume = 1;
 }
 
 __global__ void k_add(int64_t n, const double *__restrict__ d, double *__restrict__ x) {
@@ -524,13 +580,12 @@ static hipGraphExec_t batch_graph(fx_mesh *m, const VT *val, int G, int chunk) {
     return exec;
 }
 
-// One PCG solve on the resident operator `val` with rhs b_dev; x in m->x.
-// Returns the final status (1 converged, 2 breakdown, 3 max_iter).
+// Iterates the PCG state in m (started by pcg_start, or resumed) until its
+// stop rule fires; returns the final status (1 converged, 2 breakdown,
+// 3 max_iter).
 template <typename VT>
-static int run_pcg(fx_mesh *m, const VT *val, const double *b_dev, double tol, int64_t max_iter, int stop_mode,
-                   int G, PcgScalars *out) {
+static int pcg_iterate(fx_mesh *m, const VT *val, double tol, int64_t max_iter, int G, PcgScalars *out) {
     hipStream_t s = m->ctx->stream;
-    pcg_start(m, b_dev, tol, max_iter, stop_mode);
     const bool graphs = use_graphs();
     PcgScalars h{};
     int chunk = 8;
@@ -566,6 +621,26 @@ static int run_pcg(fx_mesh *m, const VT *val, const double *b_dev, double tol, i
     return h.status;
 }
 
+// One PCG solve on the resident operator `val` with rhs b_dev; x in m->x.
+template <typename VT>
+static int run_pcg(fx_mesh *m, const VT *val, const double *b_dev, double tol, int64_t max_iter, int stop_mode,
+                   int G, PcgScalars *out) {
+    pcg_start(m, b_dev, tol, max_iter, stop_mode);
+    return pcg_iterate<VT>(m, val, tol, max_iter, G, out);
+}
+
+// Continues a mixed solve after its residual was replaced by r64 (r.r =
+// rr64): x = 0, r = r64, z = r inv, the kept direction's r.p, and a new
+// segment that stops at a relative reduction `tol` of r64.
+static void pcg_resume(fx_mesh *m, const double *r64, double rr64, double tol, const PcgScalars &h) {
+    hipStream_t s = m->ctx->stream;
+    const double *p = (h.it & 1) ? m->p1.p : m->p0.p;  // what the next SpMV reads as p_old
+    hipLaunchKernelGGL(k_resume, dim3(nblocks(m->n_dof, kBlock)), dim3(kBlock), 0, s, m->n_dof, r64, m->x.p,
+                       m->r.p, m->z.p, p, m->inv_diag.p, part2_of(m));
+    hipLaunchKernelGGL(k_resume_scalars, dim3(1), dim3(64), 0, s, m->pcg.p, rr64, tol);
+    FX_HIP(hipGetLastError());
+}
+
 static int64_t iterations_done(const PcgScalars &h) {
     // the stop rule fires in k_spmv_p of iteration it_k1 (it_k1 SpMVs done);
     // a breakdown fires in k_update after that iteration's SpMV
@@ -595,10 +670,12 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
         if (x_dev != m->x.p)
             FX_HIP(hipMemcpyAsync(x_dev, m->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
-        // FP64 iterative refinement around the FP32-stored operator:
-        //   x = 0;  repeat: r = b - K64 x (FP64), stop on the outer rule,
-        //   solve K32 d = r to a relative reduction inner_tol (PCG with FP64
-        //   vectors), x += d.
+        // The FP32-stored operator with FP64 vectors, and reliable updates
+        // of the residual: x = 0 (in xacc); repeat r64 = b - K64 xacc
+        // (FP64), stop on the outer rule; the first time start PCG on r64,
+        // afterwards replace PCG's residual by r64 keeping its direction
+        // when the two residuals agree within 10% (k_resume, pcg_head); run
+        // until the iterated residual has fallen by inner_tol; xacc += x.
         launch_convert32(m);
         DBuf<double> xacc, rtrue, kx, kxa, out4;
         xacc.alloc(n);
@@ -608,8 +685,8 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
         out4.alloc(4);
         launch_fill(m, xacc.p, n, 0.0);
         double bb = -1.0, crit_prev = HUGE_VAL;
-        int64_t budget = o.max_iter;
         int passes = 0;
+        PcgScalars h{};
         const unsigned nb = nblocks(n, kBlock);
         for (;;) {
             spmv64(m, xacc.p, kx.p, G, kxa.p);
@@ -637,15 +714,18 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
             R.final_criterion = crit;
             if (conv)
                 break;
-            if (passes >= o.max_refine || budget <= 0 || !std::isfinite(crit)) {
+            if (passes >= o.max_refine || (passes > 0 && h.it >= o.max_iter) || !std::isfinite(crit)) {
                 st = FX_ERR_NOT_CONVERGED;
                 break;
             }
-            PcgScalars h;
-            const int code = run_pcg<float>(m, m->kval32.p, rtrue.p, o.inner_tol, budget, FX_STOP_REL_R2, G, &h);
-            const int64_t done = iterations_done(h);
-            R.iterations += done;
-            budget -= done;
+            int code;
+            if (passes == 0) {
+                code = run_pcg<float>(m, m->kval32.p, rtrue.p, o.inner_tol, o.max_iter, FX_STOP_REL_R2, G, &h);
+            } else {
+                pcg_resume(m, rtrue.p, h3[0], o.inner_tol, h);
+                code = pcg_iterate<float>(m, m->kval32.p, o.inner_tol, o.max_iter, G, &h);
+            }
+            R.iterations = iterations_done(h);  // the iteration index runs on across segments
             ++passes;
             if (code == 2) {
                 st = FX_ERR_BREAKDOWN;

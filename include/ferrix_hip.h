@@ -150,13 +150,12 @@ fx_status fx_apply_penalty(fx_mesh *mesh, int64_t n_bc, const int64_t *dofs, con
 /* ---- solver seam ------------------------------------------------------ */
 typedef enum fx_precision {
     FX_PREC_FP64 = 0,  /* Jacobi-PCG in FP64 (the reference's IterativeSolver)   */
-    FX_PREC_MIXED = 1  /* FP32-stored K, FP64 residuals and norms.  fx_solve:
-                          FP64 iterative refinement around an inner PCG with
-                          FP64 Krylov vectors.  A batch: PCG with FP32 Krylov
-                          vectors (dots in FP64) and reliable updates -- the
-                          residual is replaced by the FP64 b - K x whenever
-                          the iterated one has fallen by inner_tol, keeping
-                          the search direction if the two agree within 10% */
+    FX_PREC_MIXED = 1  /* FP32-stored K, FP64 residuals and norms; PCG with
+                          reliable updates: the residual is replaced by the
+                          FP64 b - K x whenever the iterated one has fallen
+                          by inner_tol, keeping the search direction if the
+                          two agree within 10%.  fx_solve keeps FP64 Krylov
+                          vectors, a batch FP32 ones (dots in FP64) */
 } fx_precision;
 
 typedef enum fx_stop {
@@ -169,17 +168,16 @@ typedef struct fx_solve_opts {
     int32_t stop_mode;  /* fx_stop */
     double tol;         /* reference: 1e-8 (static_step.rs:204) */
     int64_t max_iter;   /* reference: 1000; iterations with the working operator */
-    double inner_tol;   /* MIXED: residual reduction per refinement pass (fx_solve)
-                           or between residual replacements (batch); 1e-3 */
-    int32_t max_refine; /* MIXED: refinement passes / replacements before
-                           NOT_CONVERGED */
+    double inner_tol;   /* MIXED: reduction of the iterated residual between
+                           residual replacements; 1e-3 */
+    int32_t max_refine; /* MIXED: residual replacements before NOT_CONVERGED */
     int32_t spmv_group; /* lanes per node row in the SpMV (0 = autotuned) */
 } fx_solve_opts;
 
 typedef struct fx_solve_report {
     int64_t iterations;     /* SpMVs with the working operator (PCG iterations) */
     int64_t outer_spmv;     /* MIXED: FP64 residual evaluations */
-    int32_t refinements;    /* MIXED: refinement passes (batch: replacements) */
+    int32_t refinements;    /* MIXED: residual replacements (segments) */
     int32_t spmv_group;     /* lanes per node row used */
     double final_criterion; /* last stopping-rule value */
     double seconds;         /* wall time of the solve on the context stream */
