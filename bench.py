@@ -274,11 +274,14 @@ def cpu_baseline_batch(specs, iters=None, threads=4, sample_iters=50):
                        f"{secs:.1f} s of modelled CPU time ({time.perf_counter() - t_all:.1f} s measured)")}
 
 
-def counter_traffic(name, bytes_per_launch):
+def counter_traffic(name, bytes_total, bytes_per_launch):
     """HBM bytes per launch from the committed rocprofv3 counter summary of
     this workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel):
     its measured traffic/algorithmic ratio times this run's algorithmic
-    bytes per launch; None when no summary is committed."""
+    bytes per launch; None when no summary is committed or it was taken on
+    other kernels.  The match is on the step's total algorithmic bytes,
+    which the workload fixes exactly (the launch count can differ by one
+    trailing chunk of launches that find every system finished)."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -286,10 +289,9 @@ def counter_traffic(name, bytes_per_launch):
     for line in open(path):
         if line.startswith("traffic_over_algorithmic:"):
             ratio = float(line.split(":", 1)[1])
-        if line.startswith("algorithmic_bytes_per_launch:"):
+        if line.startswith("algorithmic_bytes_per_step:"):
             ref = float(line.split(":", 1)[1])
-    # the counters describe this workload only if its launches move the same bytes
-    if ratio is None or ref is None or abs(ref - bytes_per_launch) > 1e-3 * ref:
+    if ratio is None or ref is None or abs(ref - bytes_total) > 1e-6 * ref:
         return None
     return round(ratio * bytes_per_launch)
 
@@ -345,7 +347,7 @@ def run_batch(a):
         if last.spmv_stats:
             launches, nt, ms, by = last.spmv_stats
             res["spmv"] = {"launches": launches, "timed": nt, "us_per_launch": 1e3 * ms / max(nt, 1),
-                           "bytes_per_launch": by / max(launches, 1),
+                           "bytes_per_launch": by / max(launches, 1), "bytes_total": by,
                            "achieved_GBps": by / (ms * 1e-3) / 1e9 if ms > 0 and nt == launches else None}
         arms[arm] = res
         results[arm] = last
@@ -382,7 +384,8 @@ def run_batch(a):
     if st_ and st_["achieved_GBps"]:
         roof = {"bound": "hbm", "achieved": round(st_["achieved_GBps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(st_["achieved_GBps"] / HBM_PEAK_GBPS, 4),
-                "traffic": (counter_traffic("kb_spmv_sell_float_batch512_counters.md", st_["bytes_per_launch"])
+                "traffic": (counter_traffic("kb_spmv_sell_float_batch512_counters.md", st_["bytes_total"],
+                                            st_["bytes_per_launch"])
                             if head == "mixed_batched" and a.meshes == 512 else None),
                 "kernel": "kb_spmv_sell<float, float>" if head == "mixed_batched" else "kb_spmv_sell<double, double>",
                 "us_per_launch": round(st_["us_per_launch"], 2), "bytes_per_launch": round(st_["bytes_per_launch"]),
