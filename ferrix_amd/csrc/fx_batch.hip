@@ -104,8 +104,10 @@ struct BSys {
 
 namespace {
 
-// Per-iteration kernel sequence of a batch: kb_spmv_p, kb_red1, kb_update,
-// kb_red2.  The vector kernels write block partial sums; one block per
+// Per-iteration kernel sequence of a batch: kb_pupdate (p = gamma z +
+// beta p_old, streamed, so the SpMV gathers one vector instead of two),
+// kb_spmv_*, kb_red1, kb_update, kb_red2.  The vector kernels write block
+// partial sums; one block per
 // system (kb_red1 / kb_red2) sums them in a fixed order and takes the
 // scalar decisions (alpha, beta, stop rule), so every block of the next
 // vector kernel reads a handful of scalars instead of re-summing all
@@ -301,22 +303,19 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
         atomicAdd(S.bytes_acc, S.spmv_bytes0 + (double)__popc(go) * S.spmv_bytes_rhs);
 }
 
-// Tail of a batched SpMV row group: p_new = z + beta p_old at the node's
-// DOFs, the FP32 operator's correction block, Ap, and p.Ap block partials
-// (right-hand sides outside `go` are computed but not stored).
+// Tail of a batched SpMV row group: the FP32 operator's correction block
+// on the node's own p, Ap, and p.Ap block partials (right-hand sides
+// outside `go` are computed but not stored).
 template <typename VT, typename XT, int RR>
-__device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, const double (&gam)[RR],
-                                           const double (&beta)[RR], gptr<const XT> z, gptr<const XT> pold,
-                                           gptr<XT> pnew, double (&acc)[3][RR], double (&loc)[kRmax]) {
+__device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, gptr<const XT> pv,
+                                           double (&acc)[3][RR], double (&loc)[kRmax]) {
     gptr<XT> ap = gp((XT *)S.ap);
     double qv[3][RR];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
-        for (int r = 0; r < RR; ++r) {
-            const int64_t q = (3 * a + p) * RR + r;
-            qv[p][r] = pnew_at<XT>(z, pold, q, gam[r], beta[r]);  // p_new as stored
-        }
+        for (int r = 0; r < RR; ++r)
+            qv[p][r] = (double)pv[(3 * a + p) * RR + r];
     if constexpr (sizeof(VT) == 4) {
         gptr<const float> c = gp(S.kc) + 9 * a;
         float cc[9];
@@ -339,7 +338,6 @@ __device__ __forceinline__ void bspmv_tail(const BSys &S, int64_t a, int go, con
         for (int p = 0; p < 3; ++p) {
             const int64_t q = (3 * a + p) * RR + r;
             const double aps = (double)(XT)acc[p][r];  // Ap as stored
-            pnew[q] = (XT)qv[p][r];
             ap[q] = (XT)aps;
             sum += qv[p][r] * aps;
         }
@@ -359,12 +357,8 @@ __device__ __forceinline__ void load_coefs(const BSys &S, int go, double (&gam)[
 
 template <int G, typename VT, typename XT, int RR>
 __device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, double (&loc)[kRmax]) {
-    double gam[RR], beta[RR];
-    load_coefs<RR>(S, go, gam, beta);
-    const int64_t itv = S.it;
-    gptr<const XT> z = gp((const XT *)S.z);
-    gptr<const XT> pold = gp((const XT *)((itv & 1) ? S.p1 : S.p0));
-    gptr<XT> pnew = gp((XT *)((itv & 1) ? S.p0 : S.p1));
+    // p of this iteration, written by kb_pupdate into the "new" buffer
+    gptr<const XT> pv = gp((const XT *)((S.it & 1) ? S.p0 : S.p1));
     gptr<const VT> val = gp((const VT *)S.val);
     gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
     const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
@@ -405,7 +399,7 @@ __device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, doubl
                 double xj[RR];
 #pragma unroll
                 for (int r = 0; r < RR; ++r)
-                    xj[r] = pnew_at<XT>(z, pold, c[u] + r, gam[r], beta[r]);
+                    xj[r] = (double)pv[c[u] + r];
 #pragma unroll
                 for (int r = 0; r < RR; ++r) {
                     acc[0][r] += w[u][0] * xj[r];
@@ -423,17 +417,12 @@ __device__ __forceinline__ void bspmv_lanes(const BSys &S, int lb, int go, doubl
             for (int r = 0; r < RR; ++r)
                 acc[p][r] += __shfl_xor(acc[p][r], off, G);
     if (lane == 0 && a < S.n_nodes)
-        bspmv_tail<VT, XT, RR>(S, a, go, gam, beta, z, pold, pnew, acc, loc);
+        bspmv_tail<VT, XT, RR>(S, a, go, pv, acc, loc);
 }
 
 template <typename VT, typename XT, int RR>
 __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double (&loc)[kRmax]) {
-    double gam[RR], beta[RR];
-    load_coefs<RR>(S, go, gam, beta);
-    const int64_t itv = S.it;
-    gptr<const XT> z = gp((const XT *)S.z);
-    gptr<const XT> pold = gp((const XT *)((itv & 1) ? S.p1 : S.p0));
-    gptr<XT> pnew = gp((XT *)((itv & 1) ? S.p0 : S.p1));
+    gptr<const XT> pv = gp((const XT *)((S.it & 1) ? S.p0 : S.p1));  // written by kb_pupdate
     const int64_t a = (int64_t)lb * kBB + threadIdx.x;
     double acc[3][RR];
 #pragma unroll
@@ -442,8 +431,8 @@ __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double
         for (int r = 0; r < RR; ++r)
             acc[p][r] = 0.0;
     if (a < S.n_nodes) {
-        sell_node_rows<VT, XT, RR>(gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, z, pold, gam, beta, acc);
-        bspmv_tail<VT, XT, RR>(S, a, go, gam, beta, z, pold, pnew, acc, loc);
+        sell_node_rows<VT, XT, RR>(gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, pv, acc);
+        bspmv_tail<VT, XT, RR>(S, a, go, pv, acc, loc);
     }
 }
 
@@ -531,10 +520,40 @@ __device__ __forceinline__ void bresid_block(const BSys &S, int lb, int64_t a0, 
                 S.part1[((size_t)lb * S.R + r) * 3 + k] = v[3 * r + k];
 }
 
-// Batched SpMV + p update for every (system, row group) of the table: lane
+// p_new = gamma z + beta p_old (as stored) for the running right-hand
+// sides, into the buffer the next SpMV gathers: the "new" buffer of
+// iteration S.it.  Own DOFs only; kTiles<XT> DOF tiles per block.
+template <typename XT>
+__global__ __launch_bounds__(kBB) void kb_pupdate(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    const int R = S.R;
+    double gam[kRmax], beta[kRmax];
+    load_coefs<kRmax>(S, go, gam, beta);
+    gptr<const XT> z = gp((const XT *)S.z);
+    gptr<const XT> pold = gp((const XT *)((S.it & 1) ? S.p1 : S.p0));
+    gptr<XT> pnew = gp((XT *)((S.it & 1) ? S.p0 : S.p1));
+    for (int tile = 0; tile < kTiles<XT>; ++tile) {
+        const int64_t i = ((int64_t)t.y * kTiles<XT> + tile) * kBB + threadIdx.x;
+        if (i >= S.n_dof)
+            break;
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r)
+            if (r < R && ((go >> r) & 1)) {
+                const int64_t q = i * R + r;
+                pnew[q] = (XT)pnew_at<XT>(z, pold, q, gam[r], beta[r]);
+            }
+    }
+}
+
+// Batched SpMV for every (system, row group) of the table: lane
 // groups on the node-block layout (kb_spmv_p) or one thread per node row on
 // the sliced-ELL operator (kb_spmv_sell); the right-hand side count of the
-// block's system selects a straight-line instance.
+// block's system selects a straight-line instance.  The operand is the p
+// that kb_pupdate stored: one gathered vector per load case.
 template <int G, typename VT, typename XT>
 __global__ __launch_bounds__(kBB) void kb_spmv_p(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
@@ -930,13 +949,12 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.nparts1 = m->sell ? fx::nblk(m->n_nodes, kBB) : fx::nblk(m->n_nodes, kBB / sl.G);
     // SpMV algorithmic bytes: values in the working precision, one column
     // index per 3x3 block, node-row pointers (+ the FP32 operator's 3x3
-    // correction per node); per right-hand side z and p_old gathered once
-    // and p_new, Ap written, in the vector precision (FP32 in the mixed inner
-    // solve)
+    // correction per node); per right-hand side p gathered once and Ap
+    // written, in the vector precision (FP32 in the mixed solve)
     S.bytes_acc = B->d_bytes.p;
     S.spmv_bytes0 = (is_mixed(B) ? 4.0 : 8.0) * (double)m->nnz_struct + 4.0 * (double)m->nblk +
                     4.0 * (double)(m->n_nodes + 1) + (is_mixed(B) ? 36.0 * (double)m->n_nodes : 0.0);
-    S.spmv_bytes_rhs = (is_mixed(B) ? 16.0 : 32.0) * (double)m->n_dof;
+    S.spmv_bytes_rhs = (is_mixed(B) ? 8.0 : 16.0) * (double)m->n_dof;
     // replacement pass: FP64 values, block columns, row pointers, the
     // Jacobi scaling; per right-hand side xacc and x gathered once, b read
     // and the new residual written
@@ -1112,6 +1130,10 @@ void launch_iterations(fx_batch *B, int n, int set) {
             FX_HIP(hipEventCreate(&C.ev[k]));
     }
     for (int k = 0; k < n; ++k) {
+        if (is_mixed(B))
+            hipLaunchKernelGGL(fx::kb_pupdate<float>, dim3(C.n2), dim3(kBB), 0, s, B->d_sys.p, C.tab2.p);
+        else
+            hipLaunchKernelGGL(fx::kb_pupdate<double>, dim3(C.n2), dim3(kBB), 0, s, B->d_sys.p, C.tab2.p);
         if (B->timing)
             FX_HIP(hipEventRecord(C.ev[2 * k], s));
         for (int g = 0; g < fx_batch::kGroups; ++g)

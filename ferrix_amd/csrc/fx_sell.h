@@ -44,7 +44,7 @@ __device__ __forceinline__ double pnew_at(gptr<const XT> z, gptr<const XT> pold,
         return (double)(XT)((double)z[at] + beta * (double)pold[at]);
 }
 
-// acc[p][r] += sum over the row's blocks of K(a,p; c,q) * p_new(c,q,r)
+// acc[p][r] += sum over the row's blocks of K(a,p; c,q) * p(c,q,r)
 // for r < RR right-hand sides (vectors interleaved: v[dof*RR + r], stored as
 // XT: double, or float in the mixed-precision solve).
 //
@@ -56,13 +56,12 @@ __device__ __forceinline__ double pnew_at(gptr<const XT> z, gptr<const XT> pold,
 // the value loads of two blocks together, behind one memory latency instead
 // of two.
 template <typename VT, typename XT, int RR>
-__device__ __forceinline__ void gather_block(gptr<const XT> z, gptr<const XT> pold, int c, const double (&gam)[RR],
-                                             const double (&beta)[RR], double (&x)[3][RR]) {
+__device__ __forceinline__ void gather_block(gptr<const XT> pv, int c, double (&x)[3][RR]) {
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int r = 0; r < RR; ++r)
-            x[q][r] = pnew_at<XT>(z, pold, (3 * (int64_t)c + q) * RR + r, gam[r], beta[r]);  // as its owner stores it
+            x[q][r] = (double)pv[(3 * (int64_t)c + q) * RR + r];
 }
 
 template <typename VT, int RR>
@@ -79,9 +78,8 @@ __device__ __forceinline__ void madd_block(const VT (&v)[9], const double (&x)[3
 
 template <typename VT, typename XT, int RR>
 __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> sl_ptr, gptr<const int32_t> scol,
-                                               gptr<const VT> sval, int64_t a, gptr<const XT> z,
-                                               gptr<const XT> pold, const double (&gam)[RR],
-                                               const double (&beta)[RR], double (&acc)[3][RR]) {
+                                               gptr<const VT> sval, int64_t a, gptr<const XT> pv,
+                                               double (&acc)[3][RR]) {
     const int64_t s = a / kSlice;
     const int lane = (int)(a % kSlice);
     // every active lane of the wavefront belongs to slice s
@@ -94,12 +92,12 @@ __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> sl_ptr, gptr<
         const int c0 = cn0, c1 = cn1;
         double x0[3][RR], x1[3][RR];
         VT v0[9], v1[9];
-        gather_block<VT, XT, RR>(z, pold, c0, gam, beta, x0);
+        gather_block<VT, XT, RR>(pv, c0, x0);
 #pragma unroll
         for (int e = 0; e < 9; ++e)
             v0[e] = sval[((size_t)k * 9 + e) * kSlice + lane];
         if (two) {
-            gather_block<VT, XT, RR>(z, pold, c1, gam, beta, x1);
+            gather_block<VT, XT, RR>(pv, c1, x1);
 #pragma unroll
             for (int e = 0; e < 9; ++e)
                 v1[e] = sval[((size_t)(k + 1) * 9 + e) * kSlice + lane];

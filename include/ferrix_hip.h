@@ -324,8 +324,9 @@ fx_status fx_batch_time_spmv(fx_batch *batch, int32_t on);
  * load cases are running, the operator values in the working precision, one
  * 4-byte column index per 3x3 block, node-row pointers (and the FP32
  * operator's 3x3 correction per node), plus per DOF and running load case
- * z and p_old gathered once and p_new and Ap written (32 bytes in FP64, 16
- * in the mixed solve, whose Krylov vectors are FP32).  A mixed system's
+ * p gathered once and Ap written (16 bytes in FP64, 8 in the mixed solve,
+ * whose Krylov vectors are FP32; p itself is formed by a separate streaming
+ * kernel).  A mixed system's
  * residual replacement runs in the same launch: FP64 values, column indices,
  * row pointers and the Jacobi scaling, plus per DOF and load case xacc and
  * x gathered once, b read and the new residual written (28 bytes). */
