@@ -232,6 +232,32 @@ def test_mixed_batch_ill_conditioned_does_not_stall(ctx):
         assert np.linalg.norm(x[r] - xd) / np.linalg.norm(xd) < 1e-8
 
 
+@pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
+def test_run_chunking_does_not_change_results(ctx, precision):
+    """fx_batch_run keeps two 16-iteration chunks in flight, learns of
+    finished slots from summaries the GPU writes to host memory, and
+    rebuilds its block tables lazily.  None of that may change a slot's
+    arithmetic: one run to completion, many 5-iteration runs and 37-iteration
+    runs give bitwise equal solutions and iteration counts."""
+    opts = SolveOptions(precision=precision, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000,
+                        inner_tol=1e-3)
+    sys = population(ctx, small_specs(5, seed=17))
+    out = []
+    for step in (0, 5, 37):
+        bt = Batch(ctx, opts)
+        slots = [bt.add(dm, b) for dm, b in sys]
+        calls = 0
+        while bt.run(max_iters=step) > 0:
+            calls += 1
+            assert calls < 100_000
+        out.append([(bt.result(s), bt.poll(s)[1].iterations) for s in slots])
+        bt.close()
+    for k in range(len(sys)):
+        for other in out[1:]:
+            assert np.array_equal(out[0][k][0], other[k][0]), f"system {k}"
+            assert out[0][k][1] == other[k][1]
+
+
 def test_batch_not_converged(ctx):
     sys = population(ctx, small_specs(2, seed=3))
     bt = Batch(ctx, SolveOptions(max_iter=3, spmv_group=8))
