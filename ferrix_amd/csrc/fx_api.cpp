@@ -519,6 +519,21 @@ fx_status fx_profile_spmv(fx_mesh *m, int32_t precision, int32_t reps, int32_t s
     FX_API_END(m->ctx)
 }
 
+fx_status fx_mesh_autotune_spmv(fx_mesh *m, int32_t precision, int32_t reps, int32_t *spmv_group_out,
+                                double *ms_per_launch) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(m->k_valid, "fx_mesh_autotune_spmv: assemble first");
+    require(reps > 0, "fx_mesh_autotune_spmv: reps must be positive");
+    require(precision == FX_PREC_FP64 || precision == FX_PREC_MIXED, "fx_mesh_autotune_spmv: unknown precision");
+    set_device(m->ctx);
+    const int G = fx::autotune_spmv(m, precision, reps, ms_per_launch);
+    if (spmv_group_out)
+        *spmv_group_out = G;
+    FX_API_END(m->ctx)
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -204,7 +204,7 @@ struct fx_mesh {
     fx::DBuf<double> red2;               // 2 doubles: norm results
     double last_max_diag = 0.0;
     bool k_valid = false;
-    int spmv_group = 0;                  // autotuned lanes per node row
+    int spmv_group = 0;                  // lanes per node row from fx_mesh_autotune_spmv (0: heuristic)
     int64_t device_bytes = 0;
     std::vector<fx::PcgGraph> graphs;    // captured iteration batches
     ~fx_mesh() {
@@ -230,7 +230,8 @@ double device_max_diag(fx_mesh *m);
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep);
 void spmv64(fx_mesh *m, const double *x, double *y, int group, double *abs_y = nullptr);
 double profile_spmv(fx_mesh *m, int precision, int reps, int group, double *bytes);
-int default_group(fx_mesh *m);  // lanes per node row for the SpMV
+int default_group(fx_mesh *m);  // lanes per node row for the SpMV (row-length heuristic)
+int autotune_spmv(fx_mesh *m, int precision, int reps, double *best_ms);  // sets m->spmv_group
 void set_error(const std::string &msg);  // message of the calling thread's last failure
 // Mixed-precision stop on the true FP64 residual b - K x, which has a
 // rounding floor: each row sum cancels terms of total magnitude (|K| |x|)_i,

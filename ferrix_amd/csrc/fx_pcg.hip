@@ -551,6 +551,27 @@ double profile_spmv(fx_mesh *m, int precision, int reps, int G, double *bytes) {
     return (double)ms / reps;
 }
 
+// The SpMV autotuner: times the PCG SpMV kernel (profile_spmv, `reps`
+// launches after 3 warm-up launches) for every lane-group size and keeps
+// the fastest as the mesh's group, which solves and batches then use when
+// their options leave spmv_group at 0.
+int autotune_spmv(fx_mesh *m, int precision, int reps, double *best_ms) {
+    int best = 0;
+    double best_t = HUGE_VAL;
+    for (int G = 2; G <= 64; G *= 2) {
+        double bytes = 0.0;
+        const double t = profile_spmv(m, precision, reps, G, &bytes);
+        if (t < best_t) {
+            best_t = t;
+            best = G;
+        }
+    }
+    m->spmv_group = best;
+    if (best_ms)
+        *best_ms = best_t;
+    return best;
+}
+
 // Iterations are issued in batches (a captured hipGraph per batch size) and
 // the status block is read after each batch.  The next batch is the largest
 // power of two not above the iterations still predicted from the observed

@@ -167,6 +167,14 @@ class DeviceMesh:
         self.ctx.check(self.lib.fx_profile_spmv(self.h, precision, reps, group, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def autotune_spmv(self, precision: int, reps: int = 20):
+        """Times the PCG SpMV for every lane-group size and keeps the
+        fastest for this mesh's later solves; returns (group, ms per
+        launch)."""
+        g, ms = C.c_int32(), C.c_double()
+        self.ctx.check(self.lib.fx_mesh_autotune_spmv(self.h, precision, reps, C.byref(g), C.byref(ms)))
+        return g.value, ms.value
+
     def get_u(self) -> np.ndarray:
         u = np.zeros(self.n_dof)
         self.ctx.check(self.lib.fx_state_get_u(self.h, _p(u)))

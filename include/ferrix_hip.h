@@ -171,7 +171,9 @@ typedef struct fx_solve_opts {
     double inner_tol;   /* MIXED: reduction of the iterated residual between
                            residual replacements; 1e-3 */
     int32_t max_refine; /* MIXED: residual replacements before NOT_CONVERGED */
-    int32_t spmv_group; /* lanes per node row in the SpMV (0 = autotuned) */
+    int32_t spmv_group; /* lanes per node row in the SpMV (0: the mesh's tuned
+                           group, fx_mesh_autotune_spmv, else a row-length
+                           heuristic) */
 } fx_solve_opts;
 
 typedef struct fx_solve_report {
@@ -269,6 +271,16 @@ fx_status fx_static_step(fx_mesh *mesh, const fx_step_def *step, fx_timer *timer
  * the benchmark's roofline figure. */
 fx_status fx_profile_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_t spmv_group,
                           double *ms_per_launch, double *bytes_per_launch);
+
+/* SpMV autotuner (no counterpart in the reference): times the PCG SpMV
+ * kernel as fx_profile_spmv does for every lane-group size 2, 4, ..., 64 on
+ * the mesh's assembled K in the given precision and keeps the fastest as the
+ * mesh's group; fx_solve and fx_batch_add use it whenever their options
+ * leave spmv_group at 0 (until then a row-length heuristic picks it).
+ * *spmv_group_out, *ms_per_launch (either may be NULL): the choice and its
+ * time.  Overwrites the mesh's PCG work vectors; call between solves. */
+fx_status fx_mesh_autotune_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_t *spmv_group_out,
+                                double *ms_per_launch);
 
 /* ---- batched solves ------------------------------------------------------
  * The reference runs one job at a time: one Solver::solve per step on one K

@@ -289,3 +289,25 @@ def test_pcg_graph_replays_match_eager(ctx, monkeypatch):
         assert np.array_equal(x, x_eager * scale)
     x, r = dm.solve(b, o)
     assert np.array_equal(x, x_eager)
+
+
+@pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
+def test_spmv_autotune(ctx, precision):
+    """fx_mesh_autotune_spmv times every lane-group size and keeps the
+    fastest; a solve whose options leave spmv_group at 0 uses it, and its
+    solution still matches the oracle's direct solve."""
+    job = meshgen.cfg1_c3d4_cantilever()
+    st = job.steps[0]
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.assemble(False)
+    b = np.zeros(job.n_dof)
+    np.add.at(b, st.load_dof, st.load_val)
+    b, _ = dm.apply_penalty(st.bc_dof, st.bc_val, b)
+    g, ms = dm.autotune_spmv(precision, reps=10)
+    say("tuned group", g, "ms", ms)
+    assert g in (2, 4, 8, 16, 32, 64) and ms > 0.0
+    x, rep = dm.solve(b, SolveOptions(precision=precision, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000))
+    assert rep.spmv_group == g
+    csr, _, _ = O.assemble_csr(job.mesh, job.node_temp, None, st.bc_dof)
+    xd = O.direct_solve(csr, b)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
