@@ -14,20 +14,25 @@ timing; every step reassembles from u = 0.
 Arms (identical work and stopping rule):
 * "fp64_serial": the reference's pattern on the GPU -- one FP64 fx_solve per
   load case, system after system;
-* "mixed_batched" (headline): continuous batching (fx_batch) with FP32-stored
-  K inside FP64 iterative refinement, load cases of a system sharing each
-  pass over K, systems admitted under an operator working-set budget;
-* "fp64_batched" (optional): the same batching in FP64.
+* "mixed_batched" (headline): continuous batching (fx_batch) with the
+  FP32-stored, row-sum-corrected K and FP32 Krylov vectors, the residual
+  replaced by the FP64 b - K x whenever the iterated one has fallen by
+  inner_tol; load cases of a system share each pass over K, systems are
+  admitted under an operator working-set budget;
+* "fp64_batched": the same batching in FP64.
 value = DOF-solves/s of the headline arm (sum of n_dof x load cases / wall
 seconds over the K timed steps).  p50/p99: per-system time from the start of
 the step to its solution on the host.  Accuracy: every solution's relative
 L2 error against the fp64_serial solution, and a sample of systems against
 the CPU oracle's direct solve.
 
-roofline: the dominant kernel is the batched SpMV kb_spmv_p<G,float>; the
-first SpMV of every 4th iteration chunk of the timed run is bracketed by HIP
-events on the library's stream; achieved = sampled algorithmic bytes /
-sampled duration.
+roofline: the dominant kernel is the batched SpMV kb_spmv_sell<float,
+float>; every SpMV launch of the timed run is bracketed by HIP events on
+the library's stream and the library sums each launch's algorithmic bytes
+on the device; achieved = algorithmic bytes / measured duration.  traffic:
+the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary's ratio to the
+algorithmic bytes, times this run's bytes per launch (when the summary was
+taken on the same kernels and workload).
 
 cpu_baseline: the oracle (C restatement of the reference CPU path: serial
 element loop + triplets + CSR, Jacobi-PCG with a row-parallel SpMV on 4
