@@ -238,6 +238,49 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 // ===========================================================================
 // C ABI
 // ===========================================================================
+namespace fx {
+constexpr size_t kPoolCap = (size_t)16 << 30;
+
+void *pool_take(fx_ctx *ctx, size_t bytes, size_t *cap) {
+    {
+        std::lock_guard<std::mutex> lk(ctx->pool_mx);
+        auto it = ctx->pool.lower_bound(bytes);
+        if (it != ctx->pool.end() && it->first <= 2 * bytes) {
+            void *p = it->second;
+            *cap = it->first;
+            ctx->pool_bytes -= it->first;
+            ctx->pool.erase(it);
+            return p;
+        }
+    }
+    const size_t b = (bytes + 255) / 256 * 256;
+    void *p = nullptr;
+    FX_HIP(hipMalloc(&p, b));
+    *cap = b;
+    return p;
+}
+
+void pool_give(fx_ctx *ctx, void *p, size_t cap) {
+    {
+        std::lock_guard<std::mutex> lk(ctx->pool_mx);
+        if (ctx->pool_bytes + cap <= kPoolCap) {
+            ctx->pool.emplace(cap, p);
+            ctx->pool_bytes += cap;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
+void pool_clear(fx_ctx *ctx) {
+    std::lock_guard<std::mutex> lk(ctx->pool_mx);
+    for (auto &kv : ctx->pool)
+        (void)hipFree(kv.second);
+    ctx->pool.clear();
+    ctx->pool_bytes = 0;
+}
+}  // namespace fx
+
 extern "C" {
 
 const char *fx_version(void) { return "ferrix_hip 0.1 (gfx950)"; }
@@ -303,6 +346,7 @@ void fx_ctx_destroy(fx_ctx *ctx) {
     for (fx_mesh *m : ctx->meshes)
         delete m;
     ctx->meshes.clear();
+    fx::pool_clear(ctx);  // after the batches, whose slots return their blocks
     for (auto *b : {&ctx->tables.n4, &ctx->tables.dn4, &ctx->tables.w4, &ctx->tables.n20, &ctx->tables.dn20,
                     &ctx->tables.w20, &ctx->tables.dn20n})
         b->release();

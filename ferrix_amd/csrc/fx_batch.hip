@@ -835,7 +835,8 @@ struct fx_batch {
         fx_status err = FX_OK;
         int64_t iters = 0, outer = 0;  // Krylov iterations (slowest load case), residual replacements
         double crit[fx::kRmax] = {0, 0, 0, 0};
-        fx::DBuf<double> b, rin, x, r, z, p0, p1, ap, xacc, part1, part2;
+        // device vectors from the context's pool: a later slot reuses them
+        fx::PoolBuf<double> b, rin, x, r, z, p0, p1, ap, xacc, part1, part2;
         std::chrono::steady_clock::time_point t0;
         double seconds = 0.0;
     };
@@ -1330,12 +1331,12 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         for (int r = 0; r < n_rhs; ++r)
             for (int64_t i = 0; i < n; ++i)
                 inter[i * n_rhs + r] = rhs[(int64_t)r * n + i];
-        sl.b.upload(inter, s);
+        sl.b.upload(ctx, inter, s);
         for (auto *v : {&sl.x, &sl.r, &sl.z, &sl.p0, &sl.p1, &sl.ap})
-            v->alloc(is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed solve stores FP32 vectors
+            v->alloc(ctx, is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed solve stores FP32 vectors
         if (is_mixed(B)) {
-            sl.rin.alloc(nv);  // replaced residuals: written by each replacement pass before use
-            sl.xacc.alloc(nv);
+            sl.rin.alloc(ctx, nv);  // replaced residuals: written by each replacement pass before use
+            sl.xacc.alloc(ctx, nv);
             fx::launch_fill(m, sl.xacc.p, nv, 0.0);
             fx::launch_convert32(m);  // K may have been reassembled since the last conversion
         }
@@ -1344,8 +1345,8 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         sl.G = G;
         // >= the SpMV's row groups (1 partial per load case) and the
         // replacement pass's (3 per load case)
-        sl.part1.alloc((size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 4);
-        sl.part2.alloc((size_t)std::max(fx::nblk(n, kBB), 1) * n_rhs * 3);
+        sl.part1.alloc(ctx, (size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 4);
+        sl.part2.alloc(ctx, (size_t)std::max(fx::nblk(n, kBB), 1) * n_rhs * 3);
         sl.used = true;
         sl.phase = 0;
         sl.t0 = std::chrono::steady_clock::now();

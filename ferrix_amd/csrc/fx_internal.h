@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -161,7 +162,54 @@ struct fx_ctx {
     std::mutex reg;                   // guards the two registries below
     std::vector<fx_mesh *> meshes;    // live meshes, destroyed with the context
     std::vector<fx_batch *> batches;  // live batches, destroyed with the context
+    // device blocks of released batch slots, reused by later slots (bytes ->
+    // block); freed with the context
+    std::mutex pool_mx;
+    std::multimap<size_t, void *> pool;
+    size_t pool_bytes = 0;
 };
+
+namespace fx {
+// A device block of at least `bytes` from the context's pool (the smallest
+// that fits, if it is at most twice the request), else a new hipMalloc;
+// *cap: the block's size.
+void *pool_take(fx_ctx *ctx, size_t bytes, size_t *cap);
+// Returns a block to the pool (freed instead once the pool holds 16 GiB);
+// work that used it must have completed on the context stream.
+void pool_give(fx_ctx *ctx, void *p, size_t cap);
+void pool_clear(fx_ctx *ctx);
+
+// A device array owned through the context's pool.
+template <typename T>
+struct PoolBuf {
+    fx_ctx *ctx = nullptr;
+    T *p = nullptr;
+    size_t n = 0, cap = 0;
+    PoolBuf() = default;
+    PoolBuf(const PoolBuf &) = delete;
+    PoolBuf &operator=(const PoolBuf &) = delete;
+    ~PoolBuf() { release(); }
+    void release() {
+        if (p)
+            pool_give(ctx, p, cap);
+        p = nullptr;
+        n = cap = 0;
+    }
+    void alloc(fx_ctx *c, size_t count) {
+        release();
+        ctx = c;
+        if (count == 0)
+            return;
+        p = (T *)pool_take(c, count * sizeof(T), &cap);
+        n = count;
+    }
+    void upload(fx_ctx *c, const std::vector<T> &v, hipStream_t s) {
+        alloc(c, v.size());
+        if (n)
+            FX_HIP(hipMemcpyAsync(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+}  // namespace fx
 
 struct fx_mesh {
     fx_ctx *ctx = nullptr;
