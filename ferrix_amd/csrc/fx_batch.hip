@@ -522,7 +522,10 @@ __device__ __forceinline__ void bresid_block(const BSys &S, int lb, int64_t a0, 
 
 // p_new = gamma z + beta p_old (as stored) for the running right-hand
 // sides, into the buffer the next SpMV gathers: the "new" buffer of
-// iteration S.it.  Own DOFs only; kTiles<XT> DOF tiles per block.
+// iteration S.it.  Own DOFs only; kTiles<XT> DOF tiles per block.  gamma is
+// 1 except right after a residual replacement (the re-orthogonalisation of
+// the kept direction); the FP64 solve never replaces its residual and
+// forms z + beta p_old exactly as fx_pcg.hip does.
 template <typename XT>
 __global__ __launch_bounds__(kBB) void kb_pupdate(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
@@ -536,15 +539,28 @@ __global__ __launch_bounds__(kBB) void kb_pupdate(BSys *__restrict__ sys, const 
     gptr<const XT> z = gp((const XT *)S.z);
     gptr<const XT> pold = gp((const XT *)((S.it & 1) ? S.p1 : S.p0));
     gptr<XT> pnew = gp((XT *)((S.it & 1) ? S.p0 : S.p1));
-    for (int tile = 0; tile < kTiles<XT>; ++tile) {
-        const int64_t i = ((int64_t)t.y * kTiles<XT> + tile) * kBB + threadIdx.x;
-        if (i >= S.n_dof)
-            break;
+    // all tiles' loads first, then the stores: every load of the block is in
+    // flight together
+    constexpr int T = kTiles<XT>;
+    XT zz[T][kRmax], pp[T][kRmax];
+#pragma unroll
+    for (int tile = 0; tile < T; ++tile) {
+        const int64_t i = ((int64_t)t.y * T + tile) * kBB + threadIdx.x;
 #pragma unroll
         for (int r = 0; r < kRmax; ++r)
-            if (r < R && ((go >> r) & 1)) {
-                const int64_t q = i * R + r;
-                pnew[q] = (XT)pnew_at<XT>(z, pold, q, gam[r], beta[r]);
+            if (i < S.n_dof && r < R && ((go >> r) & 1)) {
+                zz[tile][r] = z[i * R + r];
+                pp[tile][r] = pold[i * R + r];
+            }
+    }
+#pragma unroll
+    for (int tile = 0; tile < T; ++tile) {
+        const int64_t i = ((int64_t)t.y * T + tile) * kBB + threadIdx.x;
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r)
+            if (i < S.n_dof && r < R && ((go >> r) & 1)) {
+                const double zv = (double)zz[tile][r], pv = (double)pp[tile][r];
+                pnew[i * R + r] = sizeof(XT) == 4 ? (XT)(gam[r] * zv + beta[r] * pv) : (XT)(zv + beta[r] * pv);
             }
     }
 }

@@ -31,19 +31,6 @@ __device__ __forceinline__ gptr<T> gp(T *p) {
     return (gptr<T>)p;
 }
 
-// p_new = z + beta p_old at one vector entry, rounded to the stored type;
-// the mixed-precision solve scales z by gamma (1 except in the iteration
-// after a residual replacement, where it folds in the re-orthogonalisation
-// of the kept direction).  FP64 vectors: the plain update.
-template <typename XT>
-__device__ __forceinline__ double pnew_at(gptr<const XT> z, gptr<const XT> pold, int64_t at, double gam,
-                                          double beta) {
-    if constexpr (sizeof(XT) == 4)
-        return (double)(XT)(gam * (double)z[at] + beta * (double)pold[at]);
-    else
-        return (double)(XT)((double)z[at] + beta * (double)pold[at]);
-}
-
 // acc[p][r] += sum over the row's blocks of K(a,p; c,q) * p(c,q,r)
 // for r < RR right-hand sides (vectors interleaved: v[dof*RR + r], stored as
 // XT: double, or float in the mixed-precision solve).
