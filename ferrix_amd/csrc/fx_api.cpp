@@ -94,6 +94,73 @@ void build_sell(fx_mesh *m, hipStream_t s) {
             }
     m->sl_ptr.upload(slp, s);
     m->scol.upload(sc, s);
+
+    // symmetric part: diagonal position of every row, upper/lower widths
+    std::vector<int32_t> d(n);
+    for (int64_t a = 0; a < n; ++a) {
+        const int32_t *c0 = bc.data() + bp[a], *c1 = bc.data() + bp[a + 1];
+        const int32_t *it = std::lower_bound(c0, c1, (int32_t)a);
+        if (it == c1 || *it != (int32_t)a)
+            throw Error{FX_ERR_INVALID, "node-block pattern without a diagonal block"};
+        d[a] = (int32_t)(it - c0);
+    }
+    std::vector<int32_t> su(ns + 1, 0), sl(ns + 1, 0);
+    for (int64_t q = 0; q < ns; ++q) {
+        int32_t wu = 0, wl = 0;
+        for (int64_t a = q * fx::kSliceRows; a < std::min(n, (q + 1) * fx::kSliceRows); ++a) {
+            wu = std::max(wu, bp[a + 1] - bp[a] - d[a]);
+            wl = std::max(wl, d[a]);
+        }
+        su[q + 1] = su[q] + wu;
+        sl[q + 1] = sl[q] + wl;
+        if ((int64_t)su[q + 1] * 9 * fx::kSliceRows + 9 * fx::kSliceRows > INT32_MAX)
+            throw Error{FX_ERR_INVALID, "mesh too large for the symmetric sliced-ELL operator"};
+    }
+    const int32_t zero_rec = su[ns] * 9 * fx::kSliceRows;  // the zero record after the last block column
+    std::vector<int32_t> scu((size_t)su[ns] * fx::kSliceRows), lr((size_t)sl[ns] * fx::kSliceRows),
+        lc((size_t)sl[ns] * fx::kSliceRows), lb((size_t)sl[ns] * fx::kSliceRows);
+    int64_t nu = 0, nl = 0;
+    for (int64_t q = 0; q < ns; ++q)
+        for (int lane = 0; lane < fx::kSliceRows; ++lane) {
+            const int64_t a = q * fx::kSliceRows + lane;
+            const int32_t deg = a < n ? bp[a + 1] - bp[a] : 0, da = a < n ? d[a] : 0;
+            for (int32_t k = 0; k < su[q + 1] - su[q]; ++k) {
+                int32_t c = a < n ? (int32_t)a : 0;
+                if (a < n && da + k < deg) {
+                    c = bc[bp[a] + da + k];
+                    ++nu;
+                }
+                scu[((size_t)su[q] + k) * fx::kSliceRows + lane] = c;
+            }
+            for (int32_t j = 0; j < sl[q + 1] - sl[q]; ++j) {
+                const size_t at = ((size_t)sl[q] + j) * fx::kSliceRows + lane;
+                lr[at] = zero_rec;
+                lc[at] = a < n ? (int32_t)a : 0;
+                lb[at] = -1;
+                if (a < n && j < da) {
+                    const int32_t c = bc[bp[a] + j];  // c < a; the block (c, a) is in c's upper part
+                    const int32_t *c0 = bc.data() + bp[c], *c1 = bc.data() + bp[c + 1];
+                    const int32_t t = (int32_t)(std::lower_bound(c0, c1, (int32_t)a) - c0);
+                    if (c0 + t == c1 || c0[t] != (int32_t)a || t < d[c])
+                        throw Error{FX_ERR_INVALID, "node-block pattern is not structurally symmetric"};
+                    const int64_t qc = c / fx::kSliceRows;
+                    lr[at] = (su[qc] + (t - d[c])) * 9 * fx::kSliceRows + (int32_t)(c % fx::kSliceRows);
+                    lc[at] = c;
+                    lb[at] = bp[c] + t;
+                    ++nl;
+                }
+            }
+        }
+    m->nblk_u = nu;
+    m->nblk_l = nl;
+    m->su_cols = su[ns];
+    m->ndiag.upload(d, s);
+    m->su_ptr.upload(su, s);
+    m->sll_ptr.upload(sl, s);
+    m->scol_u.upload(scu, s);
+    m->lrec.upload(lr, s);
+    m->lcol.upload(lc, s);
+    m->lblk.upload(lb, s);
 }
 
 Symbolic build_symbolic(int64_t n_nodes, int64_t n_el, const int64_t *el_ptr, const int32_t *conn,

@@ -203,6 +203,63 @@ __global__ void k_sell_convert(int64_t n_nodes, const int32_t *__restrict__ nb_p
     }
 }
 
+// Symmetric sliced-ELL FP32 operator (one slice of 64 node rows per block):
+// the row's upper blocks (from its diagonal on) rounded to FP32 into uval in
+// the sval layout, zeros in the padding; and the row-sum correction of this
+// operator, C_a = sum_b (K_ab - Kt_ab), where Kt_ab = fl32(K_ab) for the upper
+// blocks and fl32(K_ba)^T -- the owner row's stored record, transposed -- for
+// the lower ones.
+__global__ void k_ssym_convert(int64_t n_nodes, const int32_t *__restrict__ nb_ptr,
+                               const int32_t *__restrict__ ndiag, const int32_t *__restrict__ su_ptr,
+                               const int32_t *__restrict__ sll_ptr, const int32_t *__restrict__ lcol,
+                               const int32_t *__restrict__ lblk, const double *__restrict__ kval,
+                               float *__restrict__ uval, float *__restrict__ kcs) {
+    const int64_t q = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t a = q * kSliceRows + lane;
+    int64_t b0 = 0;
+    int deg = 0, da = 0;
+    if (a < n_nodes) {
+        b0 = nb_ptr[a];
+        deg = nb_ptr[a + 1] - (int)b0;
+        da = ndiag[a];
+    }
+    double C[9];
+    for (int e = 0; e < 9; ++e)
+        C[e] = 0.0;
+    const int k0 = su_ptr[q], wu = su_ptr[q + 1] - k0;
+    for (int k = 0; k < wu; ++k) {
+        const size_t kk = (size_t)k0 + k;
+        for (int p = 0; p < 3; ++p)
+            for (int c = 0; c < 3; ++c) {
+                float v = 0.f;
+                if (a < n_nodes && da + k < deg) {
+                    const double v64 = kval[9 * b0 + (int64_t)p * 3 * deg + 3 * (da + k) + c];
+                    v = (float)v64;
+                    C[3 * p + c] += v64 - (double)v;
+                }
+                uval[(kk * 9 + 3 * p + c) * kSliceRows + lane] = v;
+            }
+    }
+    const int l0 = sll_ptr[q], wl = sll_ptr[q + 1] - l0;
+    for (int j = 0; j < wl && a < n_nodes && j < da; ++j) {
+        const size_t at = ((size_t)l0 + j) * kSliceRows + lane;
+        const int cn = lcol[at];
+        const int64_t bc0 = nb_ptr[cn];
+        const int degc = nb_ptr[cn + 1] - (int)bc0;
+        const int t = lblk[at] - (int)bc0;
+        for (int p = 0; p < 3; ++p)
+            for (int c = 0; c < 3; ++c) {
+                const double kac = kval[9 * b0 + (int64_t)p * 3 * deg + 3 * j + c];
+                const double kca = kval[9 * bc0 + (int64_t)c * 3 * degc + 3 * t + p];  // K(c, a) entry (c, p)
+                C[3 * p + c] += kac - (double)(float)kca;
+            }
+    }
+    if (a < n_nodes)
+        for (int e = 0; e < 9; ++e)
+            kcs[9 * a + e] = (float)C[e];
+}
+
 __global__ void k_residual(int64_t n, const double *__restrict__ fe, const double *__restrict__ fi,
                            double *__restrict__ r) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -347,6 +404,21 @@ void launch_sell(fx_mesh *m, bool fp32) {
         hipLaunchKernelGGL((k_sell_convert<double, double>), dim3((unsigned)ns), dim3(kSliceRows), 0, s,
                            m->n_nodes, m->nb_ptr.p, m->sl_ptr.p, m->kval.p, m->sval64.p);
     }
+    FX_HIP(hipGetLastError());
+}
+
+void launch_ssym(fx_mesh *m) {
+    if (!m->sell)
+        return;
+    const int64_t ns = (m->n_nodes + kSliceRows - 1) / kSliceRows;
+    const size_t col = (size_t)9 * kSliceRows;  // floats of one block column
+    hipStream_t s = m->ctx->stream;
+    m->uval32.alloc(((size_t)m->su_cols + 1) * col);
+    m->kcorr32s.alloc((size_t)9 * m->n_nodes);
+    FX_HIP(hipMemsetAsync(m->uval32.p + (size_t)m->su_cols * col, 0, col * sizeof(float), s));  // zero record
+    hipLaunchKernelGGL(k_ssym_convert, dim3((unsigned)ns), dim3(kSliceRows), 0, s, m->n_nodes, m->nb_ptr.p,
+                       m->ndiag.p, m->su_ptr.p, m->sll_ptr.p, m->lcol.p, m->lblk.p, m->kval.p, m->uval32.p,
+                       m->kcorr32s.p);
     FX_HIP(hipGetLastError());
 }
 

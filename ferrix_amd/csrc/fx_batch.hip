@@ -38,6 +38,7 @@
 #include "fx_sell.h"
 
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -83,6 +84,9 @@ struct BSys {
     const int32_t *sl_ptr, *scol;       // sliced-ELL operator (fx_sell.h) when sell != 0
     const void *sval;
     int32_t sell;
+    int32_t ssym;                       // mixed sliced-ELL slot on the symmetric FP32 operator
+    const int32_t *su_ptr, *scol_u, *sll_ptr, *lrec, *lcol;
+    const float *uval;
     double *b;                          // FP64: right-hand sides; mixed: the last replaced residual b - K64 xacc
     const double *btrue;                // right-hand sides
     void *x, *r, *z, *p0, *p1, *ap;    // PCG vectors, XT (double, or float in the mixed solve)
@@ -431,7 +435,13 @@ __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double
         for (int r = 0; r < RR; ++r)
             acc[p][r] = 0.0;
     if (a < S.n_nodes) {
-        sell_node_rows<VT, XT, RR>(gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, pv, acc);
+        if (sizeof(VT) == 4 && S.ssym) {
+            // lower blocks from their owners' records, then the row's own upper blocks
+            ssym_lower_rows<XT, RR>(gp(S.sll_ptr), gp(S.lrec), gp(S.lcol), gp(S.uval), a, pv, acc);
+            sell_node_rows<float, XT, RR>(gp(S.su_ptr), gp(S.scol_u), gp(S.uval), a, pv, acc);
+        } else {
+            sell_node_rows<VT, XT, RR>(gp(S.sl_ptr), gp(S.scol), gp((const VT *)S.sval), a, pv, acc);
+        }
         bspmv_tail<VT, XT, RR>(S, a, go, pv, acc, loc);
     }
 }
@@ -915,6 +925,13 @@ using fx::kRmax;
 
 bool is_mixed(const fx_batch *B) { return B->opts.precision == FX_PREC_MIXED; }
 
+// mixed batches on sliced-ELL meshes use the symmetric FP32 operator
+// (FX_SSYM=0: the full one)
+bool use_ssym() {
+    const char *e = std::getenv("FX_SSYM");
+    return !(e && e[0] == '0');
+}
+
 void start_solve(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     for (int r = 0; r < kRmax; ++r) {
         BRhs &h = S.rhs[r];
@@ -947,6 +964,16 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.sl_ptr = m->sl_ptr.p;
     S.scol = m->scol.p;
     S.sval = is_mixed(B) ? (const void *)m->sval32.p : (const void *)m->sval64.p;
+    S.ssym = is_mixed(B) && m->sell && use_ssym() ? 1 : 0;
+    if (S.ssym) {
+        S.su_ptr = m->su_ptr.p;
+        S.scol_u = m->scol_u.p;
+        S.sll_ptr = m->sll_ptr.p;
+        S.lrec = m->lrec.p;
+        S.lcol = m->lcol.p;
+        S.uval = m->uval32.p;
+        S.kc = m->kcorr32s.p;  // the correction of the symmetric operator
+    }
     S.inv = m->inv_diag.p;
     S.b = is_mixed(B) ? sl.rin.p : sl.b.p;
     S.btrue = sl.b.p;
@@ -971,6 +998,9 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.bytes_acc = B->d_bytes.p;
     S.spmv_bytes0 = (is_mixed(B) ? 4.0 : 8.0) * (double)m->nnz_struct + 4.0 * (double)m->nblk +
                     4.0 * (double)(m->n_nodes + 1) + (is_mixed(B) ? 36.0 * (double)m->n_nodes : 0.0);
+    if (S.ssym)  // upper blocks once (values + column), lower blocks as (record, column), the correction
+        S.spmv_bytes0 = 40.0 * (double)m->nblk_u + 8.0 * (double)m->nblk_l + 36.0 * (double)m->n_nodes +
+                        16.0 * (double)((m->n_nodes + fx::kSliceRows - 1) / fx::kSliceRows + 1);
     S.spmv_bytes_rhs = (is_mixed(B) ? 8.0 : 16.0) * (double)m->n_dof;
     // replacement pass: FP64 values, block columns, row pointers, the
     // Jacobi scaling; per right-hand side xacc and x gathered once, b read
@@ -1356,7 +1386,10 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
             fx::launch_fill(m, sl.xacc.p, nv, 0.0);
             fx::launch_convert32(m);  // K may have been reassembled since the last conversion
         }
-        fx::launch_sell(m, is_mixed(B));
+        if (is_mixed(B) && m->sell && use_ssym())
+            fx::launch_ssym(m);
+        else
+            fx::launch_sell(m, is_mixed(B));
         fx::launch_jacobi(m, false);
         sl.G = G;
         // >= the SpMV's row groups (1 partial per load case) and the

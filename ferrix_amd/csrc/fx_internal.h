@@ -238,6 +238,18 @@ struct fx_mesh {
     fx::DBuf<int32_t> sl_ptr, scol;
     fx::DBuf<double> sval64;
     fx::DBuf<float> sval32;
+    // symmetric sliced ELL (FP32, the mixed batch's operator on sliced-ELL
+    // meshes): per row its upper blocks (column >= row) stored like sval
+    // (uval, su_ptr, scol_u; padding: zero values, own column), and per
+    // lower block the index of its transpose's record in the owner row
+    // (lrec: kk * 576 + lane) with its column (lcol); padding points at the
+    // zero record after the last block column.  lblk: the transpose's
+    // node-block index (conversion only).  kcorr32s: the row-sum correction
+    // of this operator.
+    fx::DBuf<int32_t> su_ptr, scol_u, sll_ptr, lrec, lcol, lblk, ndiag;
+    fx::DBuf<float> uval32, kcorr32s;
+    int64_t nblk_u = 0, nblk_l = 0;       // upper (diagonal included) and lower blocks
+    int64_t su_cols = 0;                  // block columns of the upper part (the zero record follows)
     fx::DBuf<uint8_t> keep;              // 1 if the slot survives the drop rule
     fx::DBuf<double> inv_diag;
     fx::DBuf<float> inv_diag32;
@@ -271,6 +283,7 @@ void launch_gather_f(fx_mesh *m);
 void launch_jacobi(fx_mesh *m, bool want32);
 void launch_convert32(fx_mesh *m);
 void launch_sell(fx_mesh *m, bool fp32);  // sliced-ELL copy of kval (FP64) or kval32 (FP32)
+void launch_ssym(fx_mesh *m);             // symmetric FP32 sliced-ELL operator + its row-sum correction
 void check_status(fx_mesh *m, const char *what);
 void reset_status(fx_mesh *m);
 double device_max_diag(fx_mesh *m);

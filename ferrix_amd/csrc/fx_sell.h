@@ -99,4 +99,54 @@ __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> sl_ptr, gptr<
     }
 }
 
+// acc[p][r] += K(a,p; c,q) p(c,q,r) over the row's lower blocks (c < a) of
+// the symmetric operator: K(a, c) = K(c, a)^T, read from the record the
+// owner row c stores (lrec: its offset in the sval layout, entry e at
+// +e*64), in ascending c.  Padding points at the zero record.
+template <typename XT, int RR>
+__device__ __forceinline__ void ssym_lower_rows(gptr<const int32_t> sll_ptr, gptr<const int32_t> lrec,
+                                                gptr<const int32_t> lcol, gptr<const float> uval, int64_t a,
+                                                gptr<const XT> pv, double (&acc)[3][RR]) {
+    const int64_t s = a / kSlice;
+    const int lane = (int)(a % kSlice);
+    const int k0 = __builtin_amdgcn_readfirstlane(sll_ptr[s]);
+    const int kw = __builtin_amdgcn_readfirstlane(sll_ptr[s + 1]);
+    for (int k = k0; k < kw; k += 2) {
+        const bool two = k + 1 < kw;
+        const int r0 = lrec[(size_t)k * kSlice + lane], c0 = lcol[(size_t)k * kSlice + lane];
+        const int r1 = two ? lrec[(size_t)(k + 1) * kSlice + lane] : 0;
+        const int c1 = two ? lcol[(size_t)(k + 1) * kSlice + lane] : 0;
+        double x0[3][RR], x1[3][RR];
+        float v0[9], v1[9];
+        gather_block<float, XT, RR>(pv, c0, x0);
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            v0[e] = uval[r0 + e * kSlice];
+        if (two) {
+            gather_block<float, XT, RR>(pv, c1, x1);
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                v1[e] = uval[r1 + e * kSlice];
+        }
+        // K(a,c)(p,q) = K(c,a)(q,p) = v[3q + p]
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                acc[p][r] += (double)v0[p] * x0[0][r];
+                acc[p][r] += (double)v0[3 + p] * x0[1][r];
+                acc[p][r] += (double)v0[6 + p] * x0[2][r];
+            }
+        if (two)
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) {
+                    acc[p][r] += (double)v1[p] * x1[0][r];
+                    acc[p][r] += (double)v1[3 + p] * x1[1][r];
+                    acc[p][r] += (double)v1[6 + p] * x1[2][r];
+                }
+    }
+}
+
 }  // namespace fx
