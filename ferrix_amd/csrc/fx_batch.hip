@@ -450,64 +450,60 @@ __device__ __forceinline__ void bspmv_sell(const BSys &S, int lb, int go, double
 // K64 (xacc + x) at the right-hand sides in `upd` -> S.b, and the partial
 // sums r64.r64, r64.(inv r64) and (|K||y|).(|K||y|), y = xacc + x, -> v
 // (lane 0).  xacc + x is the FP64 solution the update kernel stores next.
+// One right-hand side at a time: the row's FP64 values are read once per
+// replaced load case (from cache after the first), and only 6 running sums
+// are live, so the pass does not raise the register count of the SpMV
+// kernel it shares.  Every sum keeps its order: columns in lane-strided
+// order, the lane reduction, then rows into v in p order.
 template <int G>
 __device__ __forceinline__ void bresid_row(const BSys &S, int64_t a, int lane, int upd, double (&v)[3 * kRmax]) {
     const int R = S.R;
     gptr<const double> xacc = gp((const double *)S.xacc), val = gp(S.val64);
     gptr<const float> x = gp((const float *)S.x);
     gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
-    double acc[3][kRmax], mag[3][kRmax];
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int r = 0; r < kRmax; ++r)
-            acc[p][r] = mag[p][r] = 0.0;
-    if (a < S.n_nodes) {
-        const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
-        gptr<const double> v0 = val + 9 * (int64_t)b0;
-        for (int j = lane; j < ncol; j += G) {
-            const int kb = j / 3;
-            const int64_t c = (3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb)) * R;
-            const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
-#pragma unroll
-            for (int r = 0; r < kRmax; ++r) {
-                if (r >= R || !((upd >> r) & 1))
-                    continue;
-                const double yj = xacc[c + r] + (double)x[c + r];
-                acc[0][r] += w0 * yj;
-                acc[1][r] += w1 * yj;
-                acc[2][r] += w2 * yj;
-                mag[0][r] += fabs(w0 * yj);
-                mag[1][r] += fabs(w1 * yj);
-                mag[2][r] += fabs(w2 * yj);
-            }
-        }
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1)
-#pragma unroll
-        for (int p = 0; p < 3; ++p)
-#pragma unroll
-            for (int r = 0; r < kRmax; ++r) {
-                acc[p][r] += __shfl_xor(acc[p][r], off, G);
-                mag[p][r] += __shfl_xor(mag[p][r], off, G);
-            }
-    if (lane != 0 || a >= S.n_nodes)
-        return;
     gptr<const double> bt = gp(S.btrue), inv = gp(S.inv);
     gptr<double> rb = gp(S.b);
+    const bool own = a < S.n_nodes;
+    const int b0 = own ? nb_ptr[a] : 0, ncol = own ? 3 * (nb_ptr[a + 1] - b0) : 0;
+    gptr<const double> v0 = val + 9 * (int64_t)b0;
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+        if (!((upd >> r) & 1))
+            continue;
+        double acc[3] = {0.0, 0.0, 0.0}, mag[3] = {0.0, 0.0, 0.0};
+        for (int j = lane; j < ncol; j += G) {
+            const int kb = j / 3;
+            const int64_t c = (3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb)) * R + r;
+            const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
+            const double yj = xacc[c] + (double)x[c];
+            acc[0] += w0 * yj;
+            acc[1] += w1 * yj;
+            acc[2] += w2 * yj;
+            mag[0] += fabs(w0 * yj);
+            mag[1] += fabs(w1 * yj);
+            mag[2] += fabs(w2 * yj);
+        }
 #pragma unroll
-    for (int r = 0; r < kRmax; ++r) {
-        if (r >= R || !((upd >> r) & 1))
+        for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                acc[p] += __shfl_xor(acc[p], off, G);
+                mag[p] += __shfl_xor(mag[p], off, G);
+            }
+        if (lane != 0 || !own)
             continue;
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
             const int64_t d = 3 * a + p, q = d * R + r;
-            const double ri = bt[q] - acc[p][r];
+            const double ri = bt[q] - acc[p];
             rb[q] = ri;
-            v[3 * r] += ri * ri;
-            v[3 * r + 1] += ri * (ri * inv[d]);
-            v[3 * r + 2] += mag[p][r] * mag[p][r];
+#pragma unroll
+            for (int k = 0; k < kRmax; ++k)  // v[3 r + .] with r in a register
+                if (k == r) {
+                    v[3 * k] += ri * ri;
+                    v[3 * k + 1] += ri * (ri * inv[d]);
+                    v[3 * k + 2] += mag[p] * mag[p];
+                }
         }
     }
 }
