@@ -338,7 +338,12 @@ fx_status fx_batch_time_spmv(fx_batch *batch, int32_t on);
  * operator's 3x3 correction per node), plus per DOF and running load case
  * p gathered once and Ap written (16 bytes in FP64, 8 in the mixed solve,
  * whose Krylov vectors are FP32; p itself is formed by a separate streaming
- * kernel).  A mixed system's
+ * kernel).  A mixed system on the sliced-ELL operator (C3D4 meshes) uses the
+ * symmetric FP32 operator: its upper 3x3 blocks (diagonal included) count
+ * once with their column index (40 bytes), each lower block, read transposed
+ * from the upper record of its column's row, as its record offset and
+ * column (8 bytes); set FX_SSYM=0 in the environment before fx_batch_add to
+ * keep the full FP32 operator.  A mixed system's
  * residual replacement runs in the same launch: FP64 values, column indices,
  * row pointers and the Jacobi scaling, plus per DOF and load case xacc and
  * x gathered once, b read and the new residual written (28 bytes). */

@@ -207,6 +207,44 @@ def test_mixed_batch_matches_fp64_solve(ctx):
             assert np.linalg.norm(x[r] - x64) / np.linalg.norm(x64) < 1e-8
 
 
+def test_mixed_batch_symmetric_and_full_fp32_operators(ctx):
+    """Mixed batches on sliced-ELL meshes use the symmetric FP32 operator by
+    default (upper blocks stored once, lower blocks read transposed);
+    FX_SSYM=0 at fx_batch_add keeps the full FP32 operator.  Both solve to
+    1e-8 of the oracle's direct solve, and the symmetric one streams fewer
+    algorithmic bytes per iteration."""
+    specs = small_specs(3, seed=43)
+    sys = population(ctx, specs)
+    mixed = SolveOptions(precision=L.FX_PREC_MIXED, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000)
+    out, per_launch = {}, {}
+    old = os.environ.get("FX_SSYM")
+    try:
+        for mode in ("0", "1"):
+            os.environ["FX_SSYM"] = mode
+            bt = Batch(ctx, mixed)
+            slots = [bt.add(dm, b) for dm, b in sys]
+            bt.run()
+            out[mode] = [bt.result(s) for s in slots]
+            launches, _, _, nbytes = bt.spmv_stats()
+            per_launch[mode] = nbytes / launches
+            bt.close()
+    finally:
+        if old is None:
+            os.environ.pop("FX_SSYM", None)
+        else:
+            os.environ["FX_SSYM"] = old
+    say("algorithmic bytes per SpMV launch: full", per_launch["0"], "symmetric", per_launch["1"])
+    assert per_launch["1"] < 0.8 * per_launch["0"]
+    for k, spec in enumerate(specs):
+        jobs = meshgen.cfg4_mesh(spec)
+        csr, _, _ = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
+        b = sys[k][1]
+        for r in range(b.shape[0]):
+            xd = O.direct_solve(csr, b[r])
+            for mode in ("0", "1"):
+                assert np.linalg.norm(out[mode][k][r] - xd) / np.linalg.norm(xd) < 1e-8, (k, r, mode)
+
+
 def test_mixed_batch_ill_conditioned_does_not_stall(ctx):
     """Member 216 of the 512-mesh population (26x4x4 cells, 2,025 DOF, two
     load cases; FP64 PCG needs about 680 iterations): the FP32 operator's
