@@ -65,10 +65,14 @@ struct Symbolic {
 // systems fill the GPU.  A single solve (fx_solve) keeps the lane-group SpMV
 // on the node-block layout, which has the shorter per-thread dependency
 // chains and wins when one system has to hide its own latency; longer rows
-// (C3D20) use lane groups everywhere.  FX_SELL=0 disables it.
+// (C3D20) use lane groups everywhere.  FX_SELL=0 disables it.  Both sliced
+// operators are optional: a mesh whose block columns or record offsets do
+// not fit their int32 indices keeps the node-block operator (no sliced ELL)
+// or the full FP32 one (no symmetric operator) instead of failing.
 void build_sell(fx_mesh *m, hipStream_t s) {
     const char *e = std::getenv("FX_SELL");
     m->sell = m->n_nodes > 0 && (double)m->nblk <= 32.0 * (double)m->n_nodes && !(e && e[0] == '0');
+    m->ssym = false;
     if (!m->sell)
         return;
     const int64_t n = m->n_nodes, ns = (n + fx::kSliceRows - 1) / fx::kSliceRows;
@@ -78,8 +82,10 @@ void build_sell(fx_mesh *m, hipStream_t s) {
         int32_t w = 0;
         for (int64_t a = sl * fx::kSliceRows; a < std::min(n, (sl + 1) * fx::kSliceRows); ++a)
             w = std::max(w, bp[a + 1] - bp[a]);
-        if ((int64_t)slp[sl] + w > INT32_MAX)
-            throw Error{FX_ERR_INVALID, "mesh too large for the sliced-ELL operator"};
+        if ((int64_t)slp[sl] + w > INT32_MAX) {
+            m->sell = false;  // too many block columns for int32 slice pointers
+            return;
+        }
         slp[sl + 1] = slp[sl] + w;
     }
     std::vector<int32_t> sc((size_t)slp[ns] * fx::kSliceRows);
@@ -101,7 +107,7 @@ void build_sell(fx_mesh *m, hipStream_t s) {
         const int32_t *c0 = bc.data() + bp[a], *c1 = bc.data() + bp[a + 1];
         const int32_t *it = std::lower_bound(c0, c1, (int32_t)a);
         if (it == c1 || *it != (int32_t)a)
-            throw Error{FX_ERR_INVALID, "node-block pattern without a diagonal block"};
+            return;  // a row without its diagonal block (build_symbolic keeps one in every row)
         d[a] = (int32_t)(it - c0);
     }
     std::vector<int32_t> su(ns + 1, 0), sl(ns + 1, 0);
@@ -114,7 +120,7 @@ void build_sell(fx_mesh *m, hipStream_t s) {
         su[q + 1] = su[q] + wu;
         sl[q + 1] = sl[q] + wl;
         if ((int64_t)su[q + 1] * 9 * fx::kSliceRows + 9 * fx::kSliceRows > INT32_MAX)
-            throw Error{FX_ERR_INVALID, "mesh too large for the symmetric sliced-ELL operator"};
+            return;  // record offsets past int32: the full FP32 operator serves this mesh
     }
     const int32_t zero_rec = su[ns] * 9 * fx::kSliceRows;  // the zero record after the last block column
     std::vector<int32_t> scu((size_t)su[ns] * fx::kSliceRows), lr((size_t)sl[ns] * fx::kSliceRows),
@@ -142,7 +148,7 @@ void build_sell(fx_mesh *m, hipStream_t s) {
                     const int32_t *c0 = bc.data() + bp[c], *c1 = bc.data() + bp[c + 1];
                     const int32_t t = (int32_t)(std::lower_bound(c0, c1, (int32_t)a) - c0);
                     if (c0 + t == c1 || c0[t] != (int32_t)a || t < d[c])
-                        throw Error{FX_ERR_INVALID, "node-block pattern is not structurally symmetric"};
+                        return;  // not structurally symmetric (element adjacency always is)
                     const int64_t qc = c / fx::kSliceRows;
                     lr[at] = (su[qc] + (t - d[c])) * 9 * fx::kSliceRows + (int32_t)(c % fx::kSliceRows);
                     lc[at] = c;
@@ -161,6 +167,7 @@ void build_sell(fx_mesh *m, hipStream_t s) {
     m->lrec.upload(lr, s);
     m->lcol.upload(lc, s);
     m->lblk.upload(lb, s);
+    m->ssym = true;
 }
 
 Symbolic build_symbolic(int64_t n_nodes, int64_t n_el, const int64_t *el_ptr, const int32_t *conn,

@@ -408,7 +408,7 @@ void launch_sell(fx_mesh *m, bool fp32) {
 }
 
 void launch_ssym(fx_mesh *m) {
-    if (!m->sell)
+    if (!m->sell || !m->ssym)
         return;
     const int64_t ns = (m->n_nodes + kSliceRows - 1) / kSliceRows;
     const size_t col = (size_t)9 * kSliceRows;  // floats of one block column

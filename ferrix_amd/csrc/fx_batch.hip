@@ -960,7 +960,7 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     S.sl_ptr = m->sl_ptr.p;
     S.scol = m->scol.p;
     S.sval = is_mixed(B) ? (const void *)m->sval32.p : (const void *)m->sval64.p;
-    S.ssym = is_mixed(B) && m->sell && use_ssym() ? 1 : 0;
+    S.ssym = is_mixed(B) && m->sell && m->ssym && use_ssym() ? 1 : 0;
     if (S.ssym) {
         S.su_ptr = m->su_ptr.p;
         S.scol_u = m->scol_u.p;
@@ -1382,7 +1382,7 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
             fx::launch_fill(m, sl.xacc.p, nv, 0.0);
             fx::launch_convert32(m);  // K may have been reassembled since the last conversion
         }
-        if (is_mixed(B) && m->sell && use_ssym())
+        if (is_mixed(B) && m->sell && m->ssym && use_ssym())
             fx::launch_ssym(m);
         else
             fx::launch_sell(m, is_mixed(B));

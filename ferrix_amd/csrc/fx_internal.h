@@ -246,6 +246,7 @@ struct fx_mesh {
     // zero record after the last block column.  lblk: the transpose's
     // node-block index (conversion only).  kcorr32s: the row-sum correction
     // of this operator.
+    bool ssym = false;  // built (int32 record offsets fit, pattern symmetric)
     fx::DBuf<int32_t> su_ptr, scol_u, sll_ptr, lrec, lcol, lblk, ndiag;
     fx::DBuf<float> uval32, kcorr32s;
     int64_t nblk_u = 0, nblk_l = 0;       // upper (diagonal included) and lower blocks

@@ -13,6 +13,7 @@ Bars:
   solve of the oracle's matrix;
 * the reference's stop rule and error behaviour (not converged) per slot.
 """
+import dataclasses
 import os
 
 import numpy as np
@@ -243,6 +244,41 @@ def test_mixed_batch_symmetric_and_full_fp32_operators(ctx):
             xd = O.direct_solve(csr, b[r])
             for mode in ("0", "1"):
                 assert np.linalg.norm(out[mode][k][r] - xd) / np.linalg.norm(xd) < 1e-8, (k, r, mode)
+
+
+def test_batch_with_unused_node(ctx):
+    """A node that no element uses still gets its three DOFs (mesh.rs:35-55):
+    its block row is the diagonal block alone, zero until the penalty of its
+    clamp.  Placed inside a slice, that row goes through the sliced-ELL
+    operators (FP64, and the symmetric FP32 one of the mixed batch); both
+    batches match the oracle's direct solve."""
+    job = meshgen.cfg4_mesh(small_specs(1, seed=47)[0])[0]
+    m, st = job.mesh, job.steps[0]
+    k = m.n_nodes // 2 + 5
+    mesh = dataclasses.replace(m, node_ids=np.arange(1, m.n_nodes + 2, dtype=m.node_ids.dtype),
+                               xyz=np.insert(m.xyz, k, m.xyz[k] + 0.25, axis=0),
+                               conn=np.where(m.conn >= k, m.conn + 1, m.conn).astype(m.conn.dtype))
+    temp = np.insert(job.node_temp, k, job.node_temp[k])
+    shift = lambda d: np.where(d >= 3 * k, d + 3, d)
+    bc_dof = np.concatenate([shift(st.bc_dof), 3 * k + np.arange(3)])
+    bc_val = np.concatenate([st.bc_val, np.zeros(3)])
+    b = np.zeros(mesh.n_dof)
+    np.add.at(b, shift(st.load_dof), st.load_val)
+    csr, _, _ = O.assemble_csr(mesh, temp, None, bc_dof)
+    for prec in (L.FX_PREC_FP64, L.FX_PREC_MIXED):
+        dm = DeviceMesh(ctx, mesh, temp)
+        dm.assemble(False)
+        bp, _ = dm.apply_penalty(bc_dof, bc_val, b)
+        bt = Batch(ctx, SolveOptions(precision=prec, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000))
+        s = bt.add(dm, bp[None, :])
+        bt.run()
+        x = bt.result(s)[0]
+        xd = O.direct_solve(csr, bp)
+        say("precision", prec, "rel err", np.linalg.norm(x - xd) / np.linalg.norm(xd))
+        assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+        assert np.all(x[3 * k:3 * k + 3] == 0.0)
+        bt.close()
+        dm.close()
 
 
 def test_mixed_batch_ill_conditioned_does_not_stall(ctx):
