@@ -56,6 +56,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64, vector and matrix (datasheet)
 
 
 def device_sync():
@@ -524,11 +525,20 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--sweep-groups", action="store_true", help="print SpMV time per lane-group size")
+    ap.add_argument("--fp64-peak", action="store_true",
+                    help="only measure the device's FP64 VALU and MFMA throughput and exit")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (counter calibration) and exit")
     a = ap.parse_args()
     if a.calib_copy_mb > 0:
         calib_copy(a.calib_copy_mb)
+        return
+    if a.fp64_peak:
+        from ferrix_amd.device import Context
+
+        valu, mfma = Context(0).measure_fp64_peak(20)
+        print(json.dumps({"fp64_peak_tflops": {"datasheet": FP64_PEAK_TFLOPS, "measured_valu_fma": round(valu, 2),
+                                               "measured_mfma_16x16x4": round(mfma, 2)}}), flush=True)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "

@@ -637,6 +637,16 @@ fx_status fx_profile_spmv(fx_mesh *m, int32_t precision, int32_t reps, int32_t s
     FX_API_END(m->ctx)
 }
 
+fx_status fx_measure_fp64_peak(fx_ctx *ctx, int32_t reps, double *valu_tflops, double *mfma_tflops) {
+    if (!ctx)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "ctx is NULL");
+    FX_API_BEGIN
+    require(reps > 0 && valu_tflops && mfma_tflops, "fx_measure_fp64_peak: bad argument");
+    set_device(ctx);
+    fx::measure_fp64_peak(ctx, reps, valu_tflops, mfma_tflops);
+    FX_API_END(ctx)
+}
+
 fx_status fx_mesh_autotune_spmv(fx_mesh *m, int32_t precision, int32_t reps, int32_t *spmv_group_out,
                                 double *ms_per_launch) {
     if (!m)

@@ -294,6 +294,7 @@ void spmv64(fx_mesh *m, const double *x, double *y, int group, double *abs_y = n
 double profile_spmv(fx_mesh *m, int precision, int reps, int group, double *bytes);
 int default_group(fx_mesh *m);  // lanes per node row for the SpMV (row-length heuristic)
 int autotune_spmv(fx_mesh *m, int precision, int reps, double *best_ms);  // sets m->spmv_group
+void measure_fp64_peak(fx_ctx *ctx, int reps, double *valu_tflops, double *mfma_tflops);  // fx_peak.hip
 void set_error(const std::string &msg);  // message of the calling thread's last failure
 // Mixed-precision stop on the true FP64 residual b - K x, which has a
 // rounding floor: each row sum cancels terms of total magnitude (|K| |x|)_i,

@@ -98,6 +98,13 @@ class Context:
             msg = self.lib.fx_last_error(self.h).decode()
             raise _ERR.get(rc, FerrixError)(msg)
 
+    def measure_fp64_peak(self, reps: int = 10):
+        """(FP64 VALU, FP64 MFMA) TFLOP/s measured on this device by
+        register-only kernels (fx_measure_fp64_peak)."""
+        v, m = C.c_double(), C.c_double()
+        self.check(self.lib.fx_measure_fp64_peak(self.h, reps, C.byref(v), C.byref(m)))
+        return v.value, m.value
+
     def close(self):
         if getattr(self, "h", None):
             for b in list(self._batches):

@@ -272,6 +272,13 @@ fx_status fx_static_step(fx_mesh *mesh, const fx_step_def *step, fx_timer *timer
 fx_status fx_profile_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_t spmv_group,
                           double *ms_per_launch, double *bytes_per_launch);
 
+/* FP64 throughput ceilings of the device (instrumentation, no counterpart
+ * in the reference): register-only kernels that keep every SIMD busy with
+ * v_fma_f64 chains and with v_mfma_f64_16x16x4_f64 accumulators, `reps`
+ * timed launches each.  The measured TFLOP/s that the element kernels' rates
+ * are stated against, beside the datasheet figure. */
+fx_status fx_measure_fp64_peak(fx_ctx *ctx, int32_t reps, double *valu_tflops, double *mfma_tflops);
+
 /* SpMV autotuner (no counterpart in the reference): times the PCG SpMV
  * kernel as fx_profile_spmv does for every lane-group size 2, 4, ..., 64 on
  * the mesh's assembled K in the given precision and keeps the fastest as the
