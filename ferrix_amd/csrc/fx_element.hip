@@ -65,7 +65,8 @@ __device__ __forceinline__ void fail(DevStatus *st, int code, int64_t gpos) {
 }
 
 // J from dN (row-major 3 x nn) and coordinates xc (nn x 3); returns det, Jinv.
-template <int NN>
+// U: unroll factor of the node sums (the order is the same for any U).
+template <int NN, int U = NN>
 __device__ __forceinline__ bool jac(const double *dN, const double *xc, double *Ji, double &det) {
     double J[9];
 #pragma unroll
@@ -73,7 +74,7 @@ __device__ __forceinline__ bool jac(const double *dN, const double *xc, double *
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             double s = 0.0;
-#pragma unroll
+#pragma unroll U
             for (int i = 0; i < NN; ++i)
                 s += dN[r * NN + i] * xc[3 * i + c];
             J[3 * r + c] = s;
@@ -229,9 +230,12 @@ __global__ __launch_bounds__(128) void k_c3d4(int64_t n_el, const int32_t *__res
 }
 
 // ---------------------------------------------------------------------------
-// C3D20: one workgroup per element
+// C3D20: one workgroup per element.  27 threads take the Gauss points'
+// Jacobians, every thread then a share of the 27 x 60 global derivatives,
+// and each of the 400 node pairs its own thread for K (one round).  Every
+// value keeps the operation order of the oracle's restatement.
 // ---------------------------------------------------------------------------
-constexpr int kC20Threads = 256;
+constexpr int kC20Threads = 512;
 
 template <bool DO_K, bool DO_F>
 __global__ __launch_bounds__(kC20Threads) void k_c3d20(
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(kC20Threads) void k_c3d20(
     double *__restrict__ fe, DevStatus *__restrict__ st) {
     __shared__ double s_xc[60], s_u[60], s_T[20];
     __shared__ double s_dg[27][60];  // [g][r*20 + i]
-    __shared__ double s_w[27], s_d[27][3], s_sig[27][6];
+    __shared__ double s_w[27], s_d[27][3], s_sig[27][6], s_Ji[27][9];
     __shared__ int s_bad;
     const int64_t e = blockIdx.x;
     const int tid = threadIdx.x;
@@ -264,18 +268,15 @@ __global__ __launch_bounds__(kC20Threads) void k_c3d20(
         const int g = tid;
         const double *dN = gdN + 60 * g;
         double Ji[9], det;
-        if (!jac<20>(dN, s_xc, Ji, det)) {
+        // node sums unrolled by 4: fully unrolled they held the kernel at
+        // 222-256 VGPRs (one or two waves per SIMD); the order is the same
+        if (!jac<20, 4>(dN, s_xc, Ji, det)) {
             s_bad = 1;
             fail(st, FX_ERR_SINGULAR_JACOBIAN, gpos[e]);
         } else {
-            for (int r = 0; r < 3; ++r)
-                for (int i = 0; i < 20; ++i) {
-                    double s = 0.0;
 #pragma unroll
-                    for (int k = 0; k < 3; ++k)
-                        s += Ji[3 * r + k] * dN[20 * k + i];
-                    s_dg[g][20 * r + i] = s;
-                }
+            for (int k = 0; k < 9; ++k)
+                s_Ji[g][k] = Ji[k];
             s_w[g] = fabs(det) * gw[g];
             double t = 0.0;
             for (int i = 0; i < 20; ++i)
@@ -293,6 +294,17 @@ __global__ __launch_bounds__(kC20Threads) void k_c3d20(
     __syncthreads();
     if (s_bad)
         return;
+    // global derivatives dN/dx = J^-1 dN/dxi at every Gauss point
+    for (int idx = tid; idx < 27 * 60; idx += kC20Threads) {
+        const int g = idx / 60, q = idx % 60, r = q / 20, i = q % 20;
+        const double *dN = gdN + 60 * g;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            s += s_Ji[g][3 * r + k] * dN[20 * k + i];
+        s_dg[g][q] = s;
+    }
+    __syncthreads();
     if (DO_K) {
         double *out = ke + e * 3600;
         for (int pair = tid; pair < 400; pair += kC20Threads) {
