@@ -54,7 +54,12 @@ class fx_step_def(C.Structure):
                 ("max_time_increment", dbl), ("max_increments", i64),
                 ("n_loads", i64), ("load_dof", P), ("load_val", P), ("load_amp", P), ("load_origin", P),
                 ("n_bcs", i64), ("bc_dof", P), ("bc_val", P), ("bc_amp", P), ("bc_origin", P),
-                ("amps", fx_amplitudes), ("solve_opts", fx_solve_opts), ("direct_opts", fx_solve_opts)]
+                ("amps", fx_amplitudes), ("solve_opts", fx_solve_opts), ("direct_opts", fx_solve_opts),
+                ("on_increment", P), ("on_increment_user", P)]
+
+
+# fx_increment_fn: int32_t (*)(void *user, int32_t step_id, int32_t inc_id, double global_time)
+fx_increment_fn = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_double)
 
 
 class fx_timer(C.Structure):

@@ -1038,6 +1038,12 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
         if (converged) {
             current_time += dt;
             FX_HIP(hipMemcpyAsync(u_start.p, m->u.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            if (sd->on_increment) {  // writer.write_increment(&inc_res, timer)
+                FX_HIP(hipStreamSynchronize(s));
+                if (sd->on_increment(sd->on_increment_user, step_id, (int32_t)n_inc, timer->global_time) != 0)
+                    throw Error{FX_ERR_INVALID, "result writer failed at step " + std::to_string(step_id) +
+                                                    ", increment " + std::to_string(n_inc)};
+            }
         } else {
             rep.cutbacks++;
             dt /= 2.0;

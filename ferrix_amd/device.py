@@ -248,10 +248,28 @@ class DeviceMesh:
 
     # -- step driver ---------------------------------------------------------
     def static_step(self, step: StepSpec, step_id: int, timer: L.fx_timer, direct_opts: SolveOptions = None,
-                    solve_opts: Optional[SolveOptions] = None) -> L.fx_step_report:
+                    solve_opts: Optional[SolveOptions] = None, on_increment=None) -> L.fx_step_report:
+        """StaticStep::solve.  on_increment(step_id, inc_id, global_time), if
+        given, runs after every converged increment with its displacement
+        resident (the reference's writer.write_increment); an exception
+        raised there stops the step and is raised again here."""
         sd, keep = step_def(step, step_id, direct_opts or DIRECT_STANDIN, solve_opts)
+        raised = []
+        if on_increment is not None:
+            def hook(_user, sid, inc, t):
+                try:
+                    on_increment(sid, inc, t)
+                    return 0
+                except BaseException as e:  # carried across the C frames, raised below
+                    raised.append(e)
+                    return 1
+            keep["hook"] = L.fx_increment_fn(hook)
+            sd.on_increment = C.cast(keep["hook"], C.c_void_p)
         rep = L.fx_step_report()
-        self.ctx.check(self.lib.fx_static_step(self.h, C.byref(sd), C.byref(timer), C.byref(rep)))
+        rc = self.lib.fx_static_step(self.h, C.byref(sd), C.byref(timer), C.byref(rep))
+        if raised:
+            raise raised[0]
+        self.ctx.check(rc)
         del keep
         return rep
 
@@ -363,14 +381,29 @@ def step_def(step: StepSpec, step_id: int, direct_opts: SolveOptions, solve_opts
 
 
 def run_job(ctx: Context, job: Job, precision=L.FX_PREC_FP64, direct_opts: Optional[SolveOptions] = None,
-            solve_opts: Optional[SolveOptions] = None, mesh: Optional[DeviceMesh] = None):
-    """All steps of ``job`` from u = 0; returns (u, [fx_step_report])."""
+            solve_opts: Optional[SolveOptions] = None, mesh: Optional[DeviceMesh] = None, writer=None):
+    """All steps of ``job`` from u = 0; returns (u, [fx_step_report]).
+    writer: a result writer (``ferrix_amd.vtk.VtkWriter``) driven as
+    main.rs:45-66 drives the reference's: init, one write per converged
+    increment (U, NT, and nodal S and E from fx_stress_strain), finish --
+    also after a failed step."""
     dm = mesh or DeviceMesh(ctx, job.mesh, job.node_temp)
     dm.set_u(np.zeros(job.n_dof))
     timer = L.fx_timer(0.0, 0.0, 1.0)
     reps = []
-    for k, st in enumerate(job.steps):
-        so = solve_opts or step_options(st, precision)
-        do = direct_opts or SolveOptions(precision=precision, stop_mode=STOP_REL_R2, tol=1e-12, max_iter=200_000)
-        reps.append(dm.static_step(st, k + 1, timer, direct_opts=do, solve_opts=so))
+    hook = None
+    if writer is not None:
+        writer.init()
+
+        def hook(step_id, inc_id, global_time):
+            stress, strain = dm.stress_strain()
+            writer.write_increment(step_id, inc_id, global_time, dm.get_u(), job.node_temp, stress, strain)
+    try:
+        for k, st in enumerate(job.steps):
+            so = solve_opts or step_options(st, precision)
+            do = direct_opts or SolveOptions(precision=precision, stop_mode=STOP_REL_R2, tol=1e-12, max_iter=200_000)
+            reps.append(dm.static_step(st, k + 1, timer, direct_opts=do, solve_opts=so, on_increment=hook))
+    finally:
+        if writer is not None:
+            writer.finish()
     return dm.get_u(), reps

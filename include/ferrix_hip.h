@@ -216,6 +216,17 @@ typedef enum fx_solver_kind {
     FX_SOLVER_ITERATIVE = 2 /* PCG with solve_opts                            */
 } fx_solver_kind;
 
+/* Result-writer hook (ResultWriter::write_increment, io/writer.rs:21-25,
+ * called at static_step.rs:211-250): after every converged increment, with
+ * its displacement committed as the resident u, fx_static_step calls
+ * on_increment(user, step_id, inc_id, global_time).  inc_id counts the
+ * step's increments as the reference's n_inc does (attempts that were cut
+ * back included); global_time is SolverTime::global_time.  The callee may
+ * read the state (fx_state_get_u, fx_stress_strain) but must not change it
+ * or run a step on the mesh.  A nonzero return stops the step with
+ * FX_ERR_INVALID (the reference's GenericIo error).  NULL: no call. */
+typedef int32_t (*fx_increment_fn)(void *user, int32_t step_id, int32_t inc_id, double global_time);
+
 typedef struct fx_step_def {
     int32_t step_id; /* 1-based, as passed to Step::solve */
     int32_t nlgeom;
@@ -235,6 +246,8 @@ typedef struct fx_step_def {
     fx_amplitudes amps;
     fx_solve_opts solve_opts;  /* ITERATIVE */
     fx_solve_opts direct_opts; /* DEFAULT / DIRECT */
+    fx_increment_fn on_increment; /* NULL, or the result writer's hook */
+    void *on_increment_user;
 } fx_step_def;
 
 /* SolverTime (time.rs:8-63). */
