@@ -22,6 +22,7 @@ Errors use :class:`FerrixError` subclasses named after the reference's
 from __future__ import annotations
 
 import gzip
+import os
 import re
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
@@ -754,3 +755,13 @@ def initial_node_temperatures(p: Project) -> np.ndarray:
         if j is not None:
             t[j] = temp
     return t
+
+
+def deck_stem(path: str) -> str:
+    """The job name of a deck path: its file stem (``Project::jobname``,
+    ``project.rs:133-147``), with a ``.gz`` of a compressed deck dropped
+    first."""
+    base = os.path.basename(str(path))
+    if base.lower().endswith(".gz"):
+        base = base[:-3]
+    return os.path.splitext(base)[0]

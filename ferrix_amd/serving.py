@@ -36,6 +36,7 @@ from . import _lib as L
 from . import meshgen
 from .batch import System, assemble
 from .device import Batch, Context, DeviceMesh, SolveOptions, run_job
+from .inp import Project, deck_stem
 from .model import Job
 
 
@@ -60,6 +61,27 @@ class Request:
         element contributions, plus the vectors."""
         per_row = 160 if self.kind == "c3d20" else 45
         return int(self.n_dof * (per_row * 40 + 8 * 24 * self.n_rhs))
+
+
+def request_from_deck(rid: int, arrival: float, deck_path: str,
+                      default_initial_temperature: Optional[float] = None) -> Request:
+    """A request given as an ``.inp`` deck, read with the reference's keyword
+    surface and quirks (``ferrix_amd.inp``, ``parser.rs``).  A deck of one
+    linear ``*STATIC`` step joins the batch like a generated request (its
+    loads are the load case; a C3D20 mesh keeps the lane-group operator).
+    A deck with NLGEOM or several steps runs through the step driver like an
+    NLGEOM request, each step from the previous step's displacement, as
+    ``main.rs:49-63`` runs them.  default_initial_temperature: the
+    project-wide temperature before ``*INITIAL CONDITIONS``
+    (``Project::default_initial_temperature``; 0.0 unless given)."""
+    p = Project.from_jobname(deck_path)
+    if default_initial_temperature is not None:
+        p.default_initial_temperature = default_initial_temperature
+    job = Job.from_project(p, name=deck_stem(deck_path))
+    single_linear = len(job.steps) == 1 and not job.steps[0].nlgeom
+    c3d20 = bool(np.any(np.diff(np.asarray(job.mesh.el_ptr)) == 20))
+    kind = ("c3d20" if c3d20 else "linear") if single_linear else "nlgeom"
+    return Request(rid, arrival, kind, [job])
 
 
 def make_requests(n: int, rate: float, seed: int = 5, arrival_seed: int = 55) -> List[Request]:

@@ -32,7 +32,7 @@ from typing import Optional
 
 import numpy as np
 
-from .inp import MeshArrays
+from .inp import MeshArrays, deck_stem
 
 SERIES_FILENAME = "results.vtk.series"
 VTK_TETRA = 10
@@ -68,7 +68,7 @@ class VtkWriter:
     (``Project::jobname``/``job_dir``, ``project.rs:133-154``)."""
 
     def __init__(self, deck_path: str, mesh: MeshArrays):
-        stem = os.path.splitext(os.path.basename(deck_path))[0] or "Unknown"
+        stem = deck_stem(deck_path) or "Unknown"
         self.dirpath = os.path.join(os.path.dirname(deck_path) or ".", stem)
         self.mesh = mesh
         n = mesh.n_nodes

@@ -50,3 +50,48 @@ def test_serving_alone_vs_mix(ctx, precision):
             "latency ms", 1e3 * mix.latency[r.rid])
         assert e.max() < 1e-8
     assert mix.dof_solves(reqs) > 0
+
+
+# calculix_parity.rs known answers: node 17's displacement after the last step
+DECK_ANSWERS = {
+    "C3D4_staticstep": (9.356107e-06, -1.313901e-04, 1.719218e-03),
+    "C3D20": (5.752112e-02, -2.497816e-04, -8.074951e-03),
+    "C3D4_2staticstep": (2.603480e-04, -1.377070e-04, 1.001299e-03),
+}
+
+
+def test_serving_deck_requests(ctx):
+    """Requests that arrive as .inp decks (the reference's own test decks),
+    interleaved with generated ones: a one-step linear deck joins the batch,
+    a two-step deck runs through the step driver.  Each deck's node 17 meets
+    CalculiX's answer at the reference's 1e-6, and every request matches the
+    stream served one at a time in plain FP64 to 1e-8."""
+    from conftest import deck
+    from ferrix_amd.serving import request_from_deck
+
+    gen = stream()[:3]
+    reqs = []
+    t = 0.0
+    for k, name in enumerate(DECK_ANSWERS):
+        reqs.append(request_from_deck(len(reqs), t, deck(name), default_initial_temperature=20.0))
+        t += 0.003
+        r = gen[k]
+        reqs.append(Request(len(reqs), t, r.kind, r.jobs))
+        t += 0.003
+    say("requests", [(r.kind, r.n_dof, r.jobs[0].name) for r in reqs])
+    assert [r.kind for r in reqs[0::2]] == ["linear", "c3d20", "nlgeom"]
+    mix = serve_batched(ctx, reqs, L.FX_PREC_FP64, chunk_iters=8)
+    assert not mix.failures, mix.failures
+    ser = serve_serial(ctx, reqs)
+    assert not ser.failures, ser.failures
+    for r in reqs:
+        e = np.linalg.norm(mix.x[r.rid] - ser.x[r.rid], axis=1) / np.linalg.norm(ser.x[r.rid], axis=1)
+        say("request", r.rid, r.kind, r.jobs[0].name, "rel err vs serial fp64", e.max())
+        assert e.max() < 1e-8
+    from ferrix_amd.inp import Project
+    for r in reqs[0::2]:
+        name = r.jobs[0].name
+        idx = Project.from_jobname(deck(name)).get_index_for_node_id(17)
+        got = mix.x[r.rid][0][3 * idx:3 * idx + 3]
+        say(name, "node 17", got, DECK_ANSWERS[name])
+        assert np.all(np.abs(got - np.array(DECK_ANSWERS[name])) < 1e-6)
