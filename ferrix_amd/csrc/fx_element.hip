@@ -64,9 +64,10 @@ __device__ __forceinline__ void fail(DevStatus *st, int code, int64_t gpos) {
     atomicMin(reinterpret_cast<unsigned long long *>(&st->bad_element), (unsigned long long)gpos);
 }
 
+__device__ __forceinline__ bool jinv(const double (&J)[9], double *Ji, double &det);
+
 // J from dN (row-major 3 x nn) and coordinates xc (nn x 3); returns det, Jinv.
-// U: unroll factor of the node sums (the order is the same for any U).
-template <int NN, int U = NN>
+template <int NN>
 __device__ __forceinline__ bool jac(const double *dN, const double *xc, double *Ji, double &det) {
     double J[9];
 #pragma unroll
@@ -74,11 +75,16 @@ __device__ __forceinline__ bool jac(const double *dN, const double *xc, double *
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             double s = 0.0;
-#pragma unroll U
+#pragma unroll
             for (int i = 0; i < NN; ++i)
                 s += dN[r * NN + i] * xc[3 * i + c];
             J[3 * r + c] = s;
         }
+    return jinv(J, Ji, det);
+}
+
+// det J and J^-1 = adj(J) / det from J (row-major); false when det == 0.
+__device__ __forceinline__ bool jinv(const double (&J)[9], double *Ji, double &det) {
     double m0 = J[4] * J[8] - J[5] * J[7];
     double m1 = J[3] * J[8] - J[5] * J[6];
     double m2 = J[3] * J[7] - J[4] * J[6];
@@ -246,7 +252,7 @@ __global__ __launch_bounds__(kC20Threads) void k_c3d20(
     double *__restrict__ fe, DevStatus *__restrict__ st) {
     __shared__ double s_xc[60], s_u[60], s_T[20];
     __shared__ double s_dg[27][60];  // [g][r*20 + i]
-    __shared__ double s_w[27], s_d[27][3], s_sig[27][6], s_Ji[27][9];
+    __shared__ double s_w[27], s_d[27][3], s_sig[27][6], s_Ji[27][9], s_J[27][9];
     __shared__ int s_bad;
     const int64_t e = blockIdx.x;
     const int tid = threadIdx.x;
@@ -264,13 +270,25 @@ __global__ __launch_bounds__(kC20Threads) void k_c3d20(
         }
     }
     __syncthreads();
+    // J at every Gauss point, one entry per thread: J(r,c) = sum over the
+    // 20 nodes of dN(r,i) x(i,c), in node order (jac's sums)
+    if (tid < 27 * 9) {
+        const int g = tid / 9, r = (tid % 9) / 3, c = tid % 3;
+        const double *dN = gdN + 60 * g;
+        double sj = 0.0;
+#pragma unroll 4
+        for (int i = 0; i < 20; ++i)
+            sj += dN[20 * r + i] * s_xc[3 * i + c];
+        s_J[g][3 * r + c] = sj;
+    }
+    __syncthreads();
     if (tid < 27) {
         const int g = tid;
-        const double *dN = gdN + 60 * g;
-        double Ji[9], det;
-        // node sums unrolled by 4: fully unrolled they held the kernel at
-        // 222-256 VGPRs (one or two waves per SIMD); the order is the same
-        if (!jac<20, 4>(dN, s_xc, Ji, det)) {
+        double J[9], Ji[9], det;
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            J[k] = s_J[g][k];
+        if (!jinv(J, Ji, det)) {
             s_bad = 1;
             fail(st, FX_ERR_SINGULAR_JACOBIAN, gpos[e]);
         } else {
