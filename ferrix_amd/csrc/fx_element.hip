@@ -236,10 +236,11 @@ __global__ __launch_bounds__(128) void k_c3d4(int64_t n_el, const int32_t *__res
 }
 
 // ---------------------------------------------------------------------------
-// C3D20: one workgroup per element.  27 threads take the Gauss points'
-// Jacobians, every thread then a share of the 27 x 60 global derivatives,
-// and each of the 400 node pairs its own thread for K (one round).  Every
-// value keeps the operation order of the oracle's restatement.
+// C3D20: one workgroup per element.  243 threads take one Jacobian entry
+// each (27 Gauss points x 9), 27 threads the inverses, every thread a share
+// of the 27 x 60 global derivatives, and each of the 400 node pairs its own
+// thread for K (one round).  Every value keeps the operation order of the
+// oracle's restatement.
 // ---------------------------------------------------------------------------
 constexpr int kC20Threads = 512;
 
