@@ -84,3 +84,21 @@ def test_vtk_writer_nlgeom_increments(ctx, tmp_path):
     lines = open(tmp_path / "beam" / files[-1]["name"]).read().split("\n")
     assert np.array_equal(section(lines, "VECTORS U float", n), u.reshape(n, 3))
     assert lines.count("CELL_TYPES %d" % job.mesh.n_el) == 1 and "25" in lines
+
+
+def test_writer_error_stops_the_step(ctx, tmp_path):
+    """A writer that fails (the reference's GenericIo error out of
+    write_increment) stops the step with that error; the series file is
+    still closed, as main.rs:66 calls finish after a failed step."""
+    p = Project.from_jobname(deck("C3D4_staticstep"))
+    p.default_initial_temperature = 20.0
+    job = Job.from_project(p)
+    w = VtkWriter(str(tmp_path / "C3D4_staticstep.inp"), job.mesh)
+
+    def broken(*a, **k):
+        raise OSError("disk full")
+
+    w.write_increment = broken
+    with pytest.raises(OSError, match="disk full"):
+        run_job(ctx, job, writer=w)
+    assert open(tmp_path / "C3D4_staticstep" / "results.vtk.series").read().endswith("  ]\n}\n")

@@ -107,3 +107,17 @@ def test_writer_file_layout(tmp_path):
     doc = json.loads(series.replace("},\n  ]", "}\n  ]"))
     assert [f["name"] for f in doc["files"]] == ["step_1_1.vtk", "step_1_3.vtk"]
     assert os.path.exists(tmp_path / "job" / "step_1_3.vtk")
+
+
+def test_deck_requests_pick_their_path():
+    """One linear *STATIC step joins the batch (C3D20 meshes as "c3d20");
+    a deck with several steps or NLGEOM runs through the step driver."""
+    from conftest import deck
+    from ferrix_amd.inp import deck_stem
+    from ferrix_amd.serving import request_from_deck
+
+    kinds = {name: request_from_deck(0, 0.0, deck(name), 20.0).kind
+             for name in ("C3D4_staticstep", "C3D20", "C3D4_2staticstep", "NLGEOM_test")}
+    assert kinds == {"C3D4_staticstep": "linear", "C3D20": "c3d20", "C3D4_2staticstep": "nlgeom",
+                     "NLGEOM_test": "nlgeom"}
+    assert deck_stem("/a/b/C3D4.inp") == "C3D4" and deck_stem("x/C3D20.inp.gz") == "C3D20"
