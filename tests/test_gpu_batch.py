@@ -354,3 +354,24 @@ def test_mesh_destroyed_while_batched(ctx):
     sys[1][0].close()  # drops its slot from the batch
     assert bt.run() == 0
     assert bt.poll(s0)[0] == 1
+
+
+@pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
+def test_population_driver_budget(ctx, precision):
+    """solve_batched admitting everything at once, and under a working-set
+    budget that makes systems wait for others to finish (admitted between
+    chunks, into reused slots): every system's result is bit for bit the
+    same, since a slot's arithmetic does not depend on when it joins."""
+    from ferrix_amd.batch import System, operator_bytes, solve_batched, upload
+
+    systems = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(small_specs(7, seed=53))]
+    upload(ctx, systems)
+    opts = SolveOptions(precision=precision, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000)
+    a = solve_batched(ctx, systems, opts)
+    budget = 2 * max(operator_bytes(s, opts) for s in systems)
+    c = solve_batched(ctx, systems, opts, ws_bytes=budget, chunk_iters=32)
+    for r in (a, c):
+        assert not r.failures and sorted(r.x) == list(range(len(systems)))
+    for k in range(len(systems)):
+        assert np.array_equal(a.x[k], c.x[k]), k
+    say("seconds", a.seconds, c.seconds)
