@@ -23,8 +23,7 @@ Arms (identical work and stopping rule):
 value = DOF-solves/s of the headline arm (sum of n_dof x load cases / wall
 seconds over the K timed steps).  p50/p99: per-system time from the start of
 the step to its solution on the host.  Accuracy: every solution's relative
-L2 error against the fp64_serial solution, and a sample of systems against
-the CPU oracle's direct solve.
+L2 error against the fp64_serial solution.
 
 roofline: the dominant kernel is the batched SpMV kb_spmv_sell<float,
 float>; every SpMV launch of the timed run is bracketed by HIP events on
@@ -37,7 +36,9 @@ taken on the same kernels and workload).
 cpu_baseline: the oracle (C restatement of the reference CPU path: serial
 element loop + triplets + CSR, Jacobi-PCG with a row-parallel SpMV on 4
 OpenMP threads, the reference's default pool size) on a bounded sample of
-the same population, DOF-solves/s over the sample.
+the same population, DOF-solves/s over the sample.  The same host leg checks
+the GPU: the oracle's direct solves of the smallest systems against every
+arm's solutions (u_rel_err_vs_oracle_sample_max).
 
 "--workload single": the C3D20 block (40x10x10, 56,463 DOF), one linear
 static step per timed step, FP64 vs mixed, with the same roofline and
@@ -285,6 +286,25 @@ def cpu_baseline_batch(specs, iters=None, threads=4, sample_iters=50):
                        f"{secs:.1f} s of modelled CPU time ({time.perf_counter() - t_all:.1f} s measured)")}
 
 
+def oracle_sample_check(systems, results, arms, n_sample):
+    """Part of the host leg: the oracle's direct solves of the smallest
+    `n_sample` systems (its matrix, factorised on the host) as the checker of
+    every arm's solutions; sets u_rel_err_vs_oracle_sample_max per arm."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    small = sorted(systems, key=lambda s: s.n_dof)[:n_sample]
+    refs = {}
+    for s in small:
+        st = s.jobs[0].steps[0]
+        csr, _, _ = O.assemble_csr(s.jobs[0].mesh, s.jobs[0].node_temp, None, st.bc_dof)
+        refs[s.key] = [O.direct_solve(csr, s.rhs[k]) for k in range(s.n_rhs)]
+    for arm, r in results.items():
+        e = [np.linalg.norm(r.x[key][k] - xd) / np.linalg.norm(xd)
+             for key, xds in refs.items() if key in r.x for k, xd in enumerate(xds)]
+        arms[arm]["u_rel_err_vs_oracle_sample_max"] = float(max(e)) if e else None
+
+
 def counter_traffic(name, bytes_total, bytes_per_launch):
     """HBM bytes per launch from the committed rocprofv3 counter summary of
     this workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel):
@@ -373,22 +393,6 @@ def run_batch(a):
             errs = [float(np.max(np.linalg.norm(r.x[k] - ref.x[k], axis=1) / np.linalg.norm(ref.x[k], axis=1)))
                     for k in ref.x if k in r.x]
             arms[arm]["u_rel_err_vs_fp64_max"] = float(max(errs)) if errs else None
-    if not a.no_oracle_check:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-
-        small = sorted(systems, key=lambda s: s.n_dof)[:a.oracle_sample]
-        for arm, r in results.items():
-            e = []
-            for s in small:
-                if s.key not in r.x:
-                    continue
-                st = s.jobs[0].steps[0]
-                csr, _, _ = O.assemble_csr(s.jobs[0].mesh, s.jobs[0].node_temp, None, st.bc_dof)
-                for k in range(s.n_rhs):
-                    xd = O.direct_solve(csr, s.rhs[k])
-                    e.append(np.linalg.norm(r.x[s.key][k] - xd) / np.linalg.norm(xd))
-            arms[arm]["u_rel_err_vs_oracle_sample_max"] = float(max(e)) if e else None
     head = "mixed_batched" if "mixed_batched" in arms else next(iter(arms))
     roof = None
     st_ = arms[head].get("spmv")
@@ -403,9 +407,11 @@ def run_batch(a):
                 "launches": st_["launches"]}
     print(f"# roofline {json.dumps(roof)}", flush=True)
     cpu = None
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline:  # the host leg: the oracle times the CPU path and checks a sample
         ref = results.get("fp64_serial")
         cpu = cpu_baseline_batch(specs, ref.iters_rhs if ref is not None else None)
+        if not a.no_oracle_check:
+            oracle_sample_check(systems, results, arms, a.oracle_sample)
     if cpu:
         print(f"# cpu {json.dumps(cpu)}", flush=True)
     h = arms[head]
