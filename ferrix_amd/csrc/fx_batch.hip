@@ -532,6 +532,44 @@ __device__ __forceinline__ void bresid_block(const BSys &S, int lb, int64_t a0, 
 // 1 except right after a residual replacement (the re-orthogonalisation of
 // the kept direction); the FP64 solve never replaces its residual and
 // forms z + beta p_old exactly as fx_pcg.hip does.
+// Every tile's loads are issued before the first store: a lane past the
+// system's last DOF loads that DOF again (in bounds, value unused), so no
+// load sits behind a per-lane branch, where the compiler would wait for it
+// before the branch joins.
+template <typename XT, int RR>
+__device__ __forceinline__ void bpupdate(const BSys &S, int lb, int go) {
+    double gam[RR], beta[RR];
+    load_coefs<RR>(S, go, gam, beta);
+    gptr<const XT> z = gp((const XT *)S.z);
+    gptr<const XT> pold = gp((const XT *)((S.it & 1) ? S.p1 : S.p0));
+    gptr<XT> pnew = gp((XT *)((S.it & 1) ? S.p0 : S.p1));
+    constexpr int T = kTiles<XT>;
+    const int64_t n = S.n_dof;  // >= 1: a system with no DOF has no blocks
+    XT zz[T][RR], pp[T][RR];
+#pragma unroll
+    for (int tile = 0; tile < T; ++tile) {
+        const int64_t i = ((int64_t)lb * T + tile) * kBB + threadIdx.x;
+        const int64_t ic = i < n ? i : n - 1;
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            zz[tile][r] = z[ic * RR + r];
+            pp[tile][r] = pold[ic * RR + r];
+        }
+    }
+#pragma unroll
+    for (int tile = 0; tile < T; ++tile) {
+        const int64_t i = ((int64_t)lb * T + tile) * kBB + threadIdx.x;
+        if (i >= n)
+            break;
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+            if ((go >> r) & 1) {
+                const double zv = (double)zz[tile][r], pv = (double)pp[tile][r];
+                pnew[i * RR + r] = sizeof(XT) == 4 ? (XT)(gam[r] * zv + beta[r] * pv) : (XT)(zv + beta[r] * pv);
+            }
+    }
+}
+
 template <typename XT>
 __global__ __launch_bounds__(kBB) void kb_pupdate(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
@@ -539,35 +577,11 @@ __global__ __launch_bounds__(kBB) void kb_pupdate(BSys *__restrict__ sys, const 
     const int go = S.go;
     if (!go)
         return;
-    const int R = S.R;
-    double gam[kRmax], beta[kRmax];
-    load_coefs<kRmax>(S, go, gam, beta);
-    gptr<const XT> z = gp((const XT *)S.z);
-    gptr<const XT> pold = gp((const XT *)((S.it & 1) ? S.p1 : S.p0));
-    gptr<XT> pnew = gp((XT *)((S.it & 1) ? S.p0 : S.p1));
-    // all tiles' loads first, then the stores: every load of the block is in
-    // flight together
-    constexpr int T = kTiles<XT>;
-    XT zz[T][kRmax], pp[T][kRmax];
-#pragma unroll
-    for (int tile = 0; tile < T; ++tile) {
-        const int64_t i = ((int64_t)t.y * T + tile) * kBB + threadIdx.x;
-#pragma unroll
-        for (int r = 0; r < kRmax; ++r)
-            if (i < S.n_dof && r < R && ((go >> r) & 1)) {
-                zz[tile][r] = z[i * R + r];
-                pp[tile][r] = pold[i * R + r];
-            }
-    }
-#pragma unroll
-    for (int tile = 0; tile < T; ++tile) {
-        const int64_t i = ((int64_t)t.y * T + tile) * kBB + threadIdx.x;
-#pragma unroll
-        for (int r = 0; r < kRmax; ++r)
-            if (i < S.n_dof && r < R && ((go >> r) & 1)) {
-                const double zv = (double)zz[tile][r], pv = (double)pp[tile][r];
-                pnew[i * R + r] = sizeof(XT) == 4 ? (XT)(gam[r] * zv + beta[r] * pv) : (XT)(zv + beta[r] * pv);
-            }
+    switch (S.R) {
+    case 1: bpupdate<XT, 1>(S, t.y, go); break;
+    case 2: bpupdate<XT, 2>(S, t.y, go); break;
+    case 3: bpupdate<XT, 3>(S, t.y, go); break;
+    default: bpupdate<XT, 4>(S, t.y, go); break;
     }
 }
 
@@ -717,28 +731,39 @@ __device__ __forceinline__ void bupdate(const BSys &S, int lb, int go, double (&
     gptr<const XT> p = gp((const XT *)((S.it & 1) ? S.p0 : S.p1));  // p_i: the "new" buffer of iteration i
     gptr<XT> x = gp((XT *)S.x), rv = gp((XT *)S.r), z = gp((XT *)S.z);
     gptr<const XT> ap = gp((const XT *)S.ap);
-    for (int tile = 0; tile < kTiles<XT>; ++tile) {
-        const int64_t i = ((int64_t)lb * kTiles<XT> + tile) * kBB + threadIdx.x;
-        if (i >= S.n_dof)
-            break;
-        const double inv = gp(S.inv)[i];
-        double xo[RR], ro[RR], po[RR], apo[RR];
+    // every tile's loads before the first store, as in bpupdate (lanes past
+    // the last DOF reload it and store nothing)
+    constexpr int T = kTiles<XT>;
+    const int64_t n = S.n_dof;
+    XT xo[T][RR], ro[T][RR], po[T][RR], apo[T][RR];
+    double inv[T];
+#pragma unroll
+    for (int tile = 0; tile < T; ++tile) {
+        const int64_t i = ((int64_t)lb * T + tile) * kBB + threadIdx.x;
+        const int64_t ic = i < n ? i : n - 1;
+        inv[tile] = gp(S.inv)[ic];
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
-            const int64_t q = i * RR + r;
-            xo[r] = x[q];
-            ro[r] = rv[q];
-            po[r] = p[q];
-            apo[r] = ap[q];
+            const int64_t q = ic * RR + r;
+            xo[tile][r] = x[q];
+            ro[tile][r] = rv[q];
+            po[tile][r] = p[q];
+            apo[tile][r] = ap[q];
         }
+    }
+#pragma unroll
+    for (int tile = 0; tile < T; ++tile) {
+        const int64_t i = ((int64_t)lb * T + tile) * kBB + threadIdx.x;
+        if (i >= n)
+            break;
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
             if (!((go >> r) & 1))
                 continue;
             const int64_t q = i * RR + r;
-            const double xi = xo[r] + alpha[r] * po[r];
-            const double ri = (double)(XT)(ro[r] - alpha[r] * apo[r]);  // values as stored
-            const double zi = (double)(XT)(ri * inv);
+            const double xi = (double)xo[tile][r] + alpha[r] * (double)po[tile][r];
+            const double ri = (double)(XT)((double)ro[tile][r] - alpha[r] * (double)apo[tile][r]);  // values as stored
+            const double zi = (double)(XT)(ri * inv[tile]);
             x[q] = (XT)xi;
             rv[q] = (XT)ri;
             z[q] = (XT)zi;
