@@ -111,11 +111,16 @@ __device__ __forceinline__ void ssym_lower_rows(gptr<const int32_t> sll_ptr, gpt
     const int lane = (int)(a % kSlice);
     const int k0 = __builtin_amdgcn_readfirstlane(sll_ptr[s]);
     const int kw = __builtin_amdgcn_readfirstlane(sll_ptr[s + 1]);
+    if (k0 >= kw)
+        return;  // no lower blocks in this slice
+    // record offsets and columns one step ahead, as in sell_node_rows; an
+    // index past the slice's width reloads its last entry (unused)
+    auto at = [&](int k) { return (size_t)(k < kw ? k : kw - 1) * kSlice + lane; };
+    int rn0 = lrec[at(k0)], cn0 = lcol[at(k0)];
+    int rn1 = lrec[at(k0 + 1)], cn1 = lcol[at(k0 + 1)];
     for (int k = k0; k < kw; k += 2) {
         const bool two = k + 1 < kw;
-        const int r0 = lrec[(size_t)k * kSlice + lane], c0 = lcol[(size_t)k * kSlice + lane];
-        const int r1 = two ? lrec[(size_t)(k + 1) * kSlice + lane] : 0;
-        const int c1 = two ? lcol[(size_t)(k + 1) * kSlice + lane] : 0;
+        const int r0 = rn0, c0 = cn0, r1 = rn1, c1 = cn1;
         double x0[3][RR], x1[3][RR];
         float v0[9], v1[9];
         gather_block<float, XT, RR>(pv, c0, x0);
@@ -128,6 +133,10 @@ __device__ __forceinline__ void ssym_lower_rows(gptr<const int32_t> sll_ptr, gpt
             for (int e = 0; e < 9; ++e)
                 v1[e] = uval[r1 + e * kSlice];
         }
+        rn0 = lrec[at(k + 2)];
+        cn0 = lcol[at(k + 2)];
+        rn1 = lrec[at(k + 3)];
+        cn1 = lcol[at(k + 3)];
         // K(a,c)(p,q) = K(c,a)(q,p) = v[3q + p]
 #pragma unroll
         for (int r = 0; r < RR; ++r)
