@@ -120,17 +120,24 @@ namespace {
 // bitwise those of fx_solve.
 
 // sums nv <= NV values over nparts blocks (layout part[b*nv + k]); valid in
-// thread 0; fixed order
+// thread 0; fixed order.  A row's NV loads are issued together: out[k] for
+// k >= nv (never read) sums the row's first value again, an in-bounds load
+// with no branch around it.
 template <int NV>
 __device__ __forceinline__ void bsum_partials(const double *part, int nparts, int nv, double (&out)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; ++k)
         out[k] = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kBB)
+    gptr<const double> pp = gp(part);
+    for (int b = threadIdx.x; b < nparts; b += kBB) {
+        double v[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k)
-            if (k < nv)
-                out[k] += part[(size_t)b * nv + k];
+            v[k] = pp[(size_t)b * nv + (k < nv ? k : 0)];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            out[k] += v[k];
+    }
     block_sum<NV>(out);
 }
 
