@@ -141,6 +141,19 @@ __device__ __forceinline__ void bsum_partials(const double *part, int nparts, in
     block_sum<NV>(out);
 }
 
+// The launch's view of a system's right-hand side states, copied to LDS by
+// one load per thread: the one-block reduction kernels take their decisions'
+// inputs from it instead of waiting on a load per right-hand side, and write
+// the new states to the global record (a field written there is not read
+// back in the same launch).
+__device__ __forceinline__ void stage_rhs(const BSys &S, BRhs (&H)[kRmax]) {
+    constexpr int kWords = kRmax * (int)sizeof(BRhs) / 4;
+    static_assert(sizeof(BRhs) % 4 == 0 && kWords <= kBB, "one word per thread");
+    if (threadIdx.x < kWords)
+        ((int32_t *)H)[threadIdx.x] = gp((const int32_t *)S.rhs)[threadIdx.x];
+    __syncthreads();
+}
+
 // Solve start for the right-hand sides with status 0:
 // x = 0, r = b, z = r inv, p0 = z (stored as XT); partials (r.z, r.r of the
 // stored values, b.b of the FP64 right-hand side) -> part2.
@@ -193,11 +206,13 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
                                                BFin *__restrict__ fin) {
     const int slot = list[blockIdx.x];
     BSys &S = sys[slot];
+    __shared__ BRhs H[kRmax];
+    stage_rhs(S, H);
     const int R = S.R;
     const int upd = S.upd;
     bool any = false;
     for (int r = 0; r < R; ++r)
-        any |= S.rhs[r].status == 0;
+        any |= H[r].status == 0;
     if (!any) {  // uniform: only this block's thread 0 writes these states, in earlier launches
         if (threadIdx.x == 0) {
             if (upd)
@@ -206,7 +221,7 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
                 BFin &F = fin[slot];
                 int64_t it = 0, nupd = 0;
                 for (int r = 0; r < kRmax; ++r) {
-                    const BRhs &h = S.rhs[r];
+                    const BRhs &h = H[r];
                     F.crit[r] = h.crit;
                     F.status[r] = h.status;
                     if (r < R) {
@@ -232,18 +247,19 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
     if (upd) {
 #pragma unroll
         for (int r = 0; r < kRmax; ++r) {
+            const BRhs &c = H[r];
             BRhs &h = S.rhs[r];
-            if (r >= R || h.status != 0)
+            if (r >= R || c.status != 0)
                 continue;
             const double rz = red[3 * r], rr = red[3 * r + 1], rp = red[3 * r + 2];
-            const int64_t it = h.it;
+            const int64_t it = c.it;
             // keep the direction only while the replaced residual is within
-            // 10% of the iterated one (h.rr): a larger gap -- the FP32
+            // 10% of the iterated one (c.rr): a larger gap -- the FP32
             // operator's error on this x, of order eps32 * cond(K) -- means
             // the iterated Krylov space belongs to the FP32 system, and
             // continuing it can stall the solve for thousands of iterations;
             // restart from p = z instead
-            const double beta = it == 0 || rr > 1.21 * h.rr ? 0.0 : rz / h.rz[(it - 1) & 1];
+            const double beta = it == 0 || rr > 1.21 * c.rr ? 0.0 : rz / c.rz[(it - 1) & 1];
             h.beta = beta;
             h.gamma = 1.0 - beta * (rz > 0.0 ? rp / rz : 0.0);
             h.rz[it & 1] = rz;
@@ -261,23 +277,25 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
     int64_t itv = S.it;
 #pragma unroll
     for (int r = 0; r < kRmax; ++r) {
+        const BRhs &c = H[r];
         BRhs &h = S.rhs[r];
-        if (r >= R || h.status != 0)
+        if (r >= R || c.status != 0)
             continue;
         const double rz = red[3 * r], rr = red[3 * r + 1];
-        const int64_t it = h.it;
-        const double bb = it == 0 ? red[3 * r + 2] : h.bb;
+        const int64_t it = c.it;
+        const double bb = it == 0 ? red[3 * r + 2] : c.bb;
         double crit;
         bool conv;
-        if (h.stop_mode == FX_STOP_ABS_SQRT_RZ) {
+        if (c.stop_mode == FX_STOP_ABS_SQRT_RZ) {
             crit = sqrt(rz);
-            conv = crit < h.tol;
+            conv = crit < c.tol;
         } else {
             crit = bb > 0.0 ? sqrt(rr) / sqrt(bb) : 0.0;
-            conv = crit <= h.tol;
+            conv = crit <= c.tol;
         }
-        const bool stop_max = it >= h.max_iter;
-        h.beta = it == 0 ? 0.0 : rz / h.rz[(it - 1) & 1];
+        const bool stop_max = it >= c.max_iter;
+        const double rmax = it == 0 ? sqrt(bb) : c.rmax;
+        h.beta = it == 0 ? 0.0 : rz / c.rz[(it - 1) & 1];
         h.gamma = 1.0;  // a plain CG step (gamma != 1 only right after a replacement)
         h.rz[it & 1] = rz;
         h.rr = rr;
@@ -285,11 +303,11 @@ __global__ __launch_bounds__(kBB) void kb_red2(BSys *__restrict__ sys, const int
         h.it_k1 = it;
         if (it == 0) {
             h.bb = bb;
-            h.rmax = sqrt(bb);
+            h.rmax = rmax;
         }
         if (stop_max)
             h.status = 3;
-        else if (h.delta > 0.0 && (conv || sqrt(rr) < h.delta * h.rmax))
+        else if (c.delta > 0.0 && (conv || sqrt(rr) < c.delta * rmax))
             trig |= 1 << r;
         else if (conv)
             h.status = 1;
@@ -655,6 +673,8 @@ __global__ __launch_bounds__(kBB) void kb_spmv_sell(BSys *__restrict__ sys, cons
 // rounding-floor test) per replaced right-hand side; S.go = those that go on.
 __global__ __launch_bounds__(kBB) void kb_red1(BSys *__restrict__ sys, const int32_t *__restrict__ list) {
     BSys &S = sys[list[blockIdx.x]];
+    __shared__ BRhs H[kRmax];
+    stage_rhs(S, H);
     const int upd = S.upd;
     if (upd) {
         const int R = S.R;
@@ -667,30 +687,32 @@ __global__ __launch_bounds__(kBB) void kb_red1(BSys *__restrict__ sys, const int
         for (int r = 0; r < kRmax; ++r) {
             if (r >= R || !((upd >> r) & 1))
                 continue;
+            const BRhs &c = H[r];
             BRhs &h = S.rhs[r];
             const double rr = red[3 * r], rzt = red[3 * r + 1], aa = red[3 * r + 2];
             double crit;
             bool conv;
-            if (h.stop_mode == FX_STOP_ABS_SQRT_RZ) {
+            if (c.stop_mode == FX_STOP_ABS_SQRT_RZ) {
                 crit = sqrt(rzt);
-                conv = crit < h.tol;
+                conv = crit < c.tol;
             } else {
-                crit = h.bb > 0.0 ? sqrt(rr) / sqrt(h.bb) : 0.0;
-                conv = crit <= h.tol;
+                crit = c.bb > 0.0 ? sqrt(rr) / sqrt(c.bb) : 0.0;
+                conv = crit <= c.tol;
             }
             // the floor test compares the ends of two full cycles: it runs,
             // and prev moves, only for a right-hand side whose own residual
             // fell by delta (or met the stop rule), not for one replaced
             // early along with the others of its slot
             if ((S.trig >> r) & 1) {
-                conv = conv || at_rounding_floor(crit, h.prev, sqrt(rr), sqrt(aa));
+                conv = conv || at_rounding_floor(crit, c.prev, sqrt(rr), sqrt(aa));
                 h.prev = crit;
             }
             h.crit = crit;
-            h.nupd++;
+            const int32_t nupd = c.nupd + 1;
+            h.nupd = nupd;
             if (conv) {
                 h.status = 1;
-            } else if (!isfinite(crit) || h.nupd >= h.max_upd) {
+            } else if (!isfinite(crit) || nupd >= c.max_upd) {
                 h.status = 3;
             } else {
                 h.status = 0;
@@ -714,15 +736,16 @@ __global__ __launch_bounds__(kBB) void kb_red1(BSys *__restrict__ sys, const int
     for (int r = 0; r < kRmax; ++r) {
         if (r >= R || !((go >> r) & 1))
             continue;
+        const BRhs &c = H[r];
         BRhs &h = S.rhs[r];
         const double pap = red[r];
-        const double al = h.rz[h.it_k1 & 1] / pap;
+        const double al = c.rz[c.it_k1 & 1] / pap;
         h.pap = pap;
         h.alpha = al;
         if (!(pap > 0.0) || !isfinite(al)) {
             h.status = 2;
         } else {
-            h.it = h.it_k1 + 1;
+            h.it = c.it_k1 + 1;
             go2 |= 1 << r;
         }
     }
