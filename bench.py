@@ -337,8 +337,11 @@ def run_batch(a):
     n_solves = sum(s.n_dof * s.n_rhs for s in systems)
     print(f"# workload batch: {len(systems)} C3D4 meshes, {n_dof} DOF, {n_solves} DOF-solves per step", flush=True)
     ctx = Context(0)
+    # the meshes live on a context of their own (its own stream), so their
+    # assembly overlaps the batch's iterations
+    mctx = Context(0) if a.assemble_ahead else ctx
     t = time.perf_counter()
-    upload(ctx, systems)
+    upload(mctx, systems)
     print(f"# upload + symbolic {time.perf_counter() - t:.1f} s", flush=True)
     fp64 = SolveOptions(precision=L.FX_PREC_FP64, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000)
     fp64_b = SolveOptions(precision=L.FX_PREC_FP64, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000,
@@ -346,14 +349,17 @@ def run_batch(a):
     mixed = SolveOptions(precision=L.FX_PREC_MIXED, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000,
                          inner_tol=a.inner_tol, max_refine=100, spmv_group=a.spmv_group)
     ws = int(a.ws_mb * (1 << 20))
+    admit = (sorted(systems, key=lambda s: -s.n_dof * s.n_rhs) if a.admit_order == "largest" else systems)
     arms, results = {}, {}
     for arm in a.arms.split(","):
         if arm == "fp64_serial":
             step = lambda: solve_serial(systems, fp64)
         elif arm == "mixed_batched":
-            step = lambda: solve_batched(ctx, systems, mixed, ws_bytes=ws, time_spmv=True)
+            step = lambda: solve_batched(ctx, admit, mixed, ws_bytes=ws, time_spmv=True,
+                                         assemble_ahead=a.assemble_ahead)
         elif arm == "fp64_batched":
-            step = lambda: solve_batched(ctx, systems, fp64_b, ws_bytes=ws, time_spmv=True)
+            step = lambda: solve_batched(ctx, admit, fp64_b, ws_bytes=ws, time_spmv=True,
+                                         assemble_ahead=a.assemble_ahead)
         else:
             raise SystemExit(f"unknown arm {arm}")
         for _ in range(a.warmup):
@@ -430,7 +436,8 @@ def run_batch(a):
         "data": "synthetic",
         "config": {"workload": f"batched_linear_static_c3d4_{len(systems)}_meshes_2k-200k_dof_1-4_load_cases_rel1e-10",
                    "n_meshes": len(systems), "n_dof": n_dof, "dof_solves_per_step": n_solves,
-                   "working_set_mb": a.ws_mb},
+                   "working_set_mb": a.ws_mb, "assembly": "overlapped" if a.assemble_ahead else "serial",
+                   "admit_order": a.admit_order},
         "p50_ms": round(h["p50_ms"], 3),
         "p99_ms": round(h["p99_ms"], 3),
         "arms": arms,
@@ -458,17 +465,17 @@ def run_serve(a):
     t = time.perf_counter()
     sat = make_requests(a.requests, rate=0.0, seed=a.seed)
     print(f"# serve: {len(sat)} requests generated in {time.perf_counter() - t:.1f} s", flush=True)
-    r_sat = serve_batched(ctx, sat, prec)
+    r_sat = serve_batched(ctx, sat, prec, prep_context=not a.serve_shared_prep)
     capacity = len(sat) / r_sat.makespan
     rate = a.rate if a.rate > 0 else 0.7 * capacity
     print(f"# serve capacity (all at t=0): {capacity:.2f} req/s, {r_sat.dof_solves(sat) / r_sat.makespan:.0f} "
           f"DOF-solves/s; arrival rate {rate:.2f} req/s", flush=True)
     reqs = make_requests(a.requests, rate=rate, seed=a.seed)
-    rb = serve_batched(ctx, reqs, prec)
+    rb = serve_batched(ctx, reqs, prec, prep_context=not a.serve_shared_prep)
     print(f"# serve batched ({a.serve_precision}): makespan {rb.makespan:.1f} s, failures {len(rb.failures)}",
           flush=True)
     other = L.FX_PREC_FP64 if prec == L.FX_PREC_MIXED else L.FX_PREC_MIXED
-    ro = serve_batched(ctx, reqs, other)
+    ro = serve_batched(ctx, reqs, other, prep_context=not a.serve_shared_prep)
     print(f"# serve batched (other precision): makespan {ro.makespan:.1f} s, failures {len(ro.failures)}",
           flush=True)
     rs = serve_serial(ctx, reqs)
@@ -501,7 +508,8 @@ def run_serve(a):
         "dtype": "fp32-stored K / fp64" if prec == L.FX_PREC_MIXED else "f64",
         "data": "synthetic",
         "config": {"workload": f"serving_{len(reqs)}_requests_poisson_70lin_20nlgeom_10c3d20",
-                   "requests": kinds, "rate_per_s": rate, "capacity_per_s": capacity},
+                   "requests": kinds, "rate_per_s": rate, "capacity_per_s": capacity,
+                   "admission_context": "shared" if a.serve_shared_prep else "own"},
         "p50_ms": round(sb["p50_ms"], 3), "p99_ms": round(sb["p99_ms"], 3),
         "arms": {f"batched_{a.serve_precision}": sb,
                  "batched_" + ("fp64" if a.serve_precision == "mixed" else "mixed"): so_,
@@ -519,11 +527,18 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 0.7 x capacity)")
     ap.add_argument("--seed", type=int, default=5, help="serve: request stream seed")
     ap.add_argument("--serve-precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--serve-shared-prep", action="store_true",
+                    help="serve: admission thread on the batch's context (default: a context of its own)")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--arms", default=None)
     ap.add_argument("--meshes", type=int, default=512)
     ap.add_argument("--ws-mb", type=float, default=0.0, help="operator working-set budget of the batch (0: all)")
+    ap.add_argument("--assemble-ahead", action="store_true",
+                    help="batched arms: a host thread assembles the systems on a second context's stream while "
+                         "the batch runs (default: each system is assembled just before its admission)")
+    ap.add_argument("--admit-order", choices=["given", "largest"], default="given",
+                    help="batched arms: admission order (largest: descending DOF x load cases)")
     ap.add_argument("--spmv-group", type=int, default=0, help="lanes per node row of the batched SpMV (0: auto)")
     ap.add_argument("--oracle-sample", type=int, default=4)
     ap.add_argument("--no-oracle-check", action="store_true")

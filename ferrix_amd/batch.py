@@ -14,6 +14,8 @@ solve a population of systems:
 """
 from __future__ import annotations
 
+import queue
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
@@ -112,34 +114,91 @@ def operator_bytes(s: System, opts: SolveOptions) -> int:
 
 
 def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int = 0, chunk_iters: int = 256,
-                  time_spmv: bool = False) -> PopulationResult:
+                  time_spmv: bool = False, assemble_ahead: bool = False) -> PopulationResult:
     """Continuous batching over `systems` in order.  ws_bytes > 0 caps the
     summed operator bytes of the running systems (at least one always runs);
     0 admits everything at once.  Each Batch.run call advances the running
     systems by up to chunk_iters iterations (the library overlaps the host's
     bookkeeping of one 16-iteration chunk with the next); finished systems
-    are collected and new ones admitted between calls."""
+    are collected and new ones admitted between calls.
+
+    assemble_ahead: the systems' meshes belong to another context than the
+    batch's (their own stream).  A host thread assembles them in order while
+    the batch runs, and each pass admits the ones that are ready, so the GPU
+    iterates the first systems while later ones are assembled.  Otherwise
+    each system is assembled just before its admission."""
+    if assemble_ahead and any(s.dm.ctx is ctx for s in systems):
+        raise ValueError("assemble_ahead: the meshes must belong to another context than the batch")
     res = PopulationResult()
     bt = Batch(ctx, opts)
     if time_spmv:
         bt.time_spmv(True)
     running: Dict[int, System] = {}   # slot -> system
     ws = 0
-    queue = list(systems)
-    t0 = time.perf_counter()
-    qi = 0
-    try:
-        while qi < len(queue) or running:
-            while qi < len(queue):
-                s = queue[qi]
-                need = operator_bytes(s, opts)
-                if running and ws_bytes > 0 and ws + need > ws_bytes:
+    ready: "queue.Queue[Optional[System]]" = queue.Queue()
+    stop = threading.Event()
+    failed: List[BaseException] = []
+
+    def produce():
+        try:
+            for s in systems:
+                if stop.is_set():
                     break
                 assemble(s)
-                slot = bt.add(s.dm, s.rhs)
-                running[slot] = s
-                ws += need
+                ready.put(s)
+        except BaseException as e:  # re-raised by the admitting thread
+            failed.append(e)
+        finally:
+            ready.put(None)
+
+    worker = threading.Thread(target=produce, daemon=True) if assemble_ahead else None
+    qi = 0
+    exhausted = False
+
+    def fetch(block: bool) -> Optional[System]:
+        """The next system to admit (assembled when assemble_ahead), or None
+        if none is ready now or none is left."""
+        nonlocal qi, exhausted
+        if worker is None:
+            if qi < len(systems):
                 qi += 1
+                return systems[qi - 1]
+            exhausted = True
+            return None
+        try:
+            s = ready.get(block=block)
+        except queue.Empty:
+            return None
+        if s is None:
+            exhausted = True
+            if failed:
+                raise failed[0]
+        return s
+
+    nxt: Optional[System] = None
+    t0 = time.perf_counter()
+    if worker is not None:
+        worker.start()
+    try:
+        while True:
+            while True:
+                if nxt is None and not exhausted:
+                    nxt = fetch(block=not running)  # wait only when nothing runs
+                if nxt is None:
+                    break
+                need = operator_bytes(nxt, opts)
+                if running and ws_bytes > 0 and ws + need > ws_bytes:
+                    break
+                if worker is None:
+                    assemble(nxt)
+                slot = bt.add(nxt.dm, nxt.rhs)
+                running[slot] = nxt
+                ws += need
+                nxt = None
+            if not running:
+                if exhausted and nxt is None:
+                    break
+                continue
             if bt.run(max_iters=chunk_iters) == len(running):
                 continue  # none finished: nothing to collect, no memory freed
             for slot in list(running):
@@ -158,5 +217,13 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
         res.seconds = time.perf_counter() - t0
         res.spmv_stats = bt.spmv_stats()
     finally:
+        stop.set()
+        if worker is not None:
+            while worker.is_alive():  # the worker's last put must not block
+                try:
+                    ready.get(timeout=0.1)
+                except queue.Empty:
+                    pass
+            worker.join()
         bt.close()
     return res

@@ -417,8 +417,10 @@ void fx_ctx_destroy(fx_ctx *ctx) {
     for (fx_batch *b : ctx->batches)
         fx::batch_free(b);
     ctx->batches.clear();
-    for (fx_mesh *m : ctx->meshes)
+    for (fx_mesh *m : ctx->meshes) {
+        fx::batch_drop_mesh_foreign(m);  // slots in batches of other contexts
         delete m;
+    }
     ctx->meshes.clear();
     fx::pool_clear(ctx);  // after the batches, whose slots return their blocks
     for (auto *b : {&ctx->tables.n4, &ctx->tables.dn4, &ctx->tables.w4, &ctx->tables.n20, &ctx->tables.dn20,
@@ -586,6 +588,7 @@ void fx_mesh_destroy(fx_mesh *m) {
     (void)hipStreamSynchronize(ctx->stream);
     for (fx_batch *b : ctx->batches)
         fx::batch_drop_mesh(b, m);
+    fx::batch_drop_mesh_foreign(m);
     delete m;
 }
 

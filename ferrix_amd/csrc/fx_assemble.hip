@@ -375,26 +375,27 @@ void launch_gather_recover(fx_mesh *m, double *S, double *E) {
     FX_HIP(hipGetLastError());
 }
 
-void launch_jacobi(fx_mesh *m, bool want32) {
-    hipLaunchKernelGGL(k_jacobi, dim3(grid1(m->n_dof, 256)), dim3(256), 0, m->ctx->stream, m->n_dof,
+void launch_jacobi(fx_mesh *m, bool want32, hipStream_t s) {
+    hipLaunchKernelGGL(k_jacobi, dim3(grid1(m->n_dof, 256)), dim3(256), 0, s ? s : m->ctx->stream, m->n_dof,
                        m->diag_slot.p, m->kval.p, m->inv_diag.p, want32 ? m->inv_diag32.p : nullptr);
     FX_HIP(hipGetLastError());
 }
 
-void launch_convert32(fx_mesh *m) {
+void launch_convert32(fx_mesh *m, hipStream_t s) {
     m->kval32.alloc(m->nnz_struct);
     m->kcorr32.alloc(9 * m->n_nodes);
-    hipLaunchKernelGGL(k_convert32, dim3(grid1(m->n_dof, 256)), dim3(256), 0, m->ctx->stream, m->n_dof,
+    hipLaunchKernelGGL(k_convert32, dim3(grid1(m->n_dof, 256)), dim3(256), 0, s ? s : m->ctx->stream, m->n_dof,
                        m->nb_ptr.p, m->kval.p, m->kval32.p, m->kcorr32.p);
     FX_HIP(hipGetLastError());
 }
 
-void launch_sell(fx_mesh *m, bool fp32) {
+void launch_sell(fx_mesh *m, bool fp32, hipStream_t s) {
     if (!m->sell)
         return;
     const int64_t ns = (m->n_nodes + kSliceRows - 1) / kSliceRows;
     const size_t slots = (size_t)m->scol.n * 9;  // block columns x 64 lanes x 9 entries
-    hipStream_t s = m->ctx->stream;
+    if (!s)
+        s = m->ctx->stream;
     if (fp32) {
         m->sval32.alloc(slots);
         hipLaunchKernelGGL((k_sell_convert<float, float>), dim3((unsigned)ns), dim3(kSliceRows), 0, s, m->n_nodes,
@@ -407,12 +408,13 @@ void launch_sell(fx_mesh *m, bool fp32) {
     FX_HIP(hipGetLastError());
 }
 
-void launch_ssym(fx_mesh *m) {
+void launch_ssym(fx_mesh *m, hipStream_t s) {
     if (!m->sell || !m->ssym)
         return;
     const int64_t ns = (m->n_nodes + kSliceRows - 1) / kSliceRows;
     const size_t col = (size_t)9 * kSliceRows;  // floats of one block column
-    hipStream_t s = m->ctx->stream;
+    if (!s)
+        s = m->ctx->stream;
     m->uval32.alloc(((size_t)m->su_cols + 1) * col);
     m->kcorr32s.alloc((size_t)9 * m->n_nodes);
     FX_HIP(hipMemsetAsync(m->uval32.p + (size_t)m->su_cols * col, 0, col * sizeof(float), s));  // zero record
@@ -427,10 +429,10 @@ void launch_residual(fx_mesh *m, const double *f_ext, const double *f_int, doubl
     FX_HIP(hipGetLastError());
 }
 
-void launch_fill(fx_mesh *m, double *dst, int64_t n, double v) {
+void launch_fill(fx_mesh *m, double *dst, int64_t n, double v, hipStream_t s) {
     if (n <= 0)
         return;
-    hipLaunchKernelGGL(k_fill, dim3(grid1(n, 256)), dim3(256), 0, m->ctx->stream, n, dst, v);
+    hipLaunchKernelGGL(k_fill, dim3(grid1(n, 256)), dim3(256), 0, s ? s : m->ctx->stream, n, dst, v);
     FX_HIP(hipGetLastError());
 }
 

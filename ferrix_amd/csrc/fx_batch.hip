@@ -906,6 +906,11 @@ struct fx_batch {
     struct Slot {
         bool used = false;
         fx_mesh *mesh = nullptr;
+        bool foreign = false;  // the mesh belongs to another context (counted in its foreign_slots)
+        ~Slot() {
+            if (foreign)
+                mesh->foreign_slots--;
+        }
         int R = 0;
         int G = 8;      // lanes per node row (lane-group SpMV, replacement pass)
         int phase = 0;  // 0 running, 2 finished
@@ -952,7 +957,33 @@ struct fx_batch {
     int64_t spmv_launches = 0, timed = 0;
     double timed_ms = 0.0;
     fx::DBuf<double> d_bytes;
-    ~fx_batch() {
+    // orders the batch's stream after work queued on the stream of an
+    // admitted mesh's own context (re-recorded at each such admission)
+    hipEvent_t adm_ev = nullptr;
+    fx_batch();
+    ~fx_batch();
+};
+
+namespace {
+// every live batch of the process, whatever its context: a mesh admitted
+// to a batch of another context is dropped from it when the mesh goes
+std::mutex g_batches_mx;
+std::vector<fx_batch *> g_batches;
+}  // namespace
+
+fx_batch::fx_batch() {
+    std::lock_guard<std::mutex> lk(g_batches_mx);
+    g_batches.push_back(this);
+}
+
+fx_batch::~fx_batch() {
+    {
+        std::lock_guard<std::mutex> lk(g_batches_mx);
+        g_batches.erase(std::find(g_batches.begin(), g_batches.end(), this));
+    }
+    {
+        if (adm_ev)
+            (void)hipEventDestroy(adm_ev);
         if (fin)
             (void)hipHostFree(fin);
         for (auto &c : cs) {
@@ -965,7 +996,7 @@ struct fx_batch {
                 (void)hipHostFree(c.h_tab);
         }
     }
-};
+}
 
 namespace {
 
@@ -1315,6 +1346,24 @@ void advance_phases(fx_batch *B, int set) {
 namespace fx {
 void batch_free(fx_batch *b) { delete b; }
 
+void batch_drop_mesh_foreign(fx_mesh *m) {
+    if (m->foreign_slots.load() == 0)
+        return;
+    std::lock_guard<std::mutex> lk(g_batches_mx);
+    for (fx_batch *B : g_batches) {
+        if (B->ctx == m->ctx)
+            continue;
+        bool uses = false;
+        for (auto &sl : B->slots)
+            uses |= sl && sl->used && sl->mesh == m;
+        if (!uses)
+            continue;
+        (void)hipSetDevice(B->ctx->device);
+        (void)hipStreamSynchronize(B->ctx->stream);  // the batch's stream may still read the mesh
+        batch_drop_mesh(B, m);
+    }
+}
+
 void batch_drop_mesh(fx_batch *B, fx_mesh *m) {
     for (size_t k = 0; k < B->slots.size(); ++k)
         if (B->slots[k] && B->slots[k]->used && B->slots[k]->mesh == m) {
@@ -1401,8 +1450,8 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
     fx_ctx *ctx = B->ctx;
     int k = -1;  // the slot taken, released again if admission fails
     try {
-        if (m->ctx != ctx)
-            throw fx::Error{FX_ERR_INVALID, "fx_batch_add: mesh belongs to another context"};
+        if (m->ctx->device != ctx->device)
+            throw fx::Error{FX_ERR_INVALID, "fx_batch_add: mesh on another device"};
         if (n_rhs < 1 || n_rhs > kRmax)
             throw fx::Error{FX_ERR_INVALID, "fx_batch_add: 1 to 4 right-hand sides per system"};
         if (!m->k_valid)
@@ -1411,6 +1460,16 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
             throw fx::Error{FX_ERR_INVALID, "fx_batch_add: n_dof x n_rhs must stay below 2^31"};
         FX_HIP(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
+        if (m->ctx != ctx) {
+            // a mesh of another context on this device, e.g. assembled by
+            // another host thread on its own stream while this batch runs:
+            // the batch's stream waits for what that stream holds so far,
+            // and the mesh-level conversions below run on the batch's stream
+            if (!B->adm_ev)
+                FX_HIP(hipEventCreateWithFlags(&B->adm_ev, hipEventDisableTiming));
+            FX_HIP(hipEventRecord(B->adm_ev, m->ctx->stream));
+            FX_HIP(hipStreamWaitEvent(s, B->adm_ev, 0));
+        }
         const int G = B->G > 0 ? B->G : (m->spmv_group > 0 ? m->spmv_group : fx::default_group(m));
         k = 0;
         while (k < (int)B->slots.size() && B->slots[k] && B->slots[k]->used)
@@ -1434,14 +1493,14 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         if (is_mixed(B)) {
             sl.rin.alloc(ctx, nv);  // replaced residuals: written by each replacement pass before use
             sl.xacc.alloc(ctx, nv);
-            fx::launch_fill(m, sl.xacc.p, nv, 0.0);
-            fx::launch_convert32(m);  // K may have been reassembled since the last conversion
+            fx::launch_fill(m, sl.xacc.p, nv, 0.0, s);
+            fx::launch_convert32(m, s);  // K may have been reassembled since the last conversion
         }
         if (is_mixed(B) && m->sell && m->ssym && use_ssym())
-            fx::launch_ssym(m);
+            fx::launch_ssym(m, s);
         else
-            fx::launch_sell(m, is_mixed(B));
-        fx::launch_jacobi(m, false);
+            fx::launch_sell(m, is_mixed(B), s);
+        fx::launch_jacobi(m, false, s);
         sl.G = G;
         // >= the SpMV's row groups (1 partial per load case) and the
         // replacement pass's (3 per load case)
@@ -1463,6 +1522,10 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         FX_HIP(hipMemcpyAsync(B->d_sys.p + k, &S, sizeof(BSys), hipMemcpyHostToDevice, s));
         start_slots(B, std::vector<int>{k});  // start this slot's solve now
         B->run_version++;
+        if (m->ctx != ctx) {
+            sl.foreign = true;
+            m->foreign_slots++;
+        }
         *slot_out = k;
     } catch (const fx::Error &e) {
         release_failed_slot(B, k);

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -213,6 +214,9 @@ struct PoolBuf {
 
 struct fx_mesh {
     fx_ctx *ctx = nullptr;
+    // slots of this mesh in batches of other contexts (a slot decrements it
+    // when it goes): while 0, destroying the mesh touches no such batch
+    std::atomic<int> foreign_slots{0};
     int64_t n_nodes = 0, n_dof = 0, n_el = 0;
     // node-block structural pattern (3x3 blocks)
     int64_t nblk = 0;           // structural node-pair blocks
@@ -281,10 +285,11 @@ void launch_recover(fx_mesh *m);         // per (element, node) strain and stres
 void launch_gather_recover(fx_mesh *m, double *S, double *E);  // nodal averages (device arrays)
 void launch_assemble(fx_mesh *m);
 void launch_gather_f(fx_mesh *m);
-void launch_jacobi(fx_mesh *m, bool want32);
-void launch_convert32(fx_mesh *m);
-void launch_sell(fx_mesh *m, bool fp32);  // sliced-ELL copy of kval (FP64) or kval32 (FP32)
-void launch_ssym(fx_mesh *m);             // symmetric FP32 sliced-ELL operator + its row-sum correction
+// s: the stream to launch on (nullptr: the mesh's context stream)
+void launch_jacobi(fx_mesh *m, bool want32, hipStream_t s = nullptr);
+void launch_convert32(fx_mesh *m, hipStream_t s = nullptr);
+void launch_sell(fx_mesh *m, bool fp32, hipStream_t s = nullptr);  // sliced-ELL copy of kval (FP64) or kval32 (FP32)
+void launch_ssym(fx_mesh *m, hipStream_t s = nullptr);  // symmetric FP32 sliced-ELL operator + its row-sum correction
 void check_status(fx_mesh *m, const char *what);
 void reset_status(fx_mesh *m);
 double device_max_diag(fx_mesh *m);
@@ -310,6 +315,7 @@ __host__ __device__ inline bool at_rounding_floor(double crit, double crit_prev,
 }
 void batch_free(fx_batch *b);                 // deletes a batch (its context's stream is idle)
 void batch_drop_mesh(fx_batch *b, fx_mesh *m);  // removes the slots that use m
+void batch_drop_mesh_foreign(fx_mesh *m);        // the same in live batches of other contexts
 void axpy_dev(fx_mesh *m, int64_t n, double a, const double *x, double *y);
 // step helpers (fx_assemble.hip)
 void launch_residual(fx_mesh *m, const double *f_ext, const double *f_int, double *r, int64_t n);
@@ -317,6 +323,6 @@ void launch_masked_norms(fx_mesh *m, const double *r, const double *f, const uin
 void launch_penalty(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const int32_t *ud_ptr,
                     const double *rec_target, const int32_t *rec_order, double *rhs, const double *u);
 void launch_scatter(fx_mesh *m, int64_t n, const int32_t *idx, const double *val, double *dst);
-void launch_fill(fx_mesh *m, double *dst, int64_t n, double v);
+void launch_fill(fx_mesh *m, double *dst, int64_t n, double v, hipStream_t s = nullptr);
 int64_t count_kept(fx_mesh *m);
 }  // namespace fx

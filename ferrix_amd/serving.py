@@ -141,10 +141,15 @@ class _NlWorker:
 
 
 def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_MIXED,
-                  mem_budget: int = 200 << 30, chunk_iters: int = 16, nl_workers: int = 2) -> ServeResult:
-    """Continuous batching over the request stream (see the module doc)."""
+                  mem_budget: int = 200 << 30, chunk_iters: int = 16, nl_workers: int = 2,
+                  prep_context: bool = True) -> ServeResult:
+    """Continuous batching over the request stream (see the module doc).
+    prep_context: the admission thread uploads and assembles on a context of
+    its own (its own stream), so its synchronisations do not wait behind the
+    batch's queued chunks; False: on the batch's context."""
     res = ServeResult()
     bt = Batch(ctx, _static_opts(precision))
+    pctx = Context(0) if prep_context else ctx
     pool = ThreadPoolExecutor(max_workers=nl_workers)
     prep = ThreadPoolExecutor(max_workers=1)
     worker = _NlWorker(precision)
@@ -156,7 +161,7 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
     used = 0
 
     def prepare(r: Request) -> System:
-        s = System(r.rid, r.jobs, DeviceMesh(ctx, r.jobs[0].mesh, r.jobs[0].node_temp))
+        s = System(r.rid, r.jobs, DeviceMesh(pctx, r.jobs[0].mesh, r.jobs[0].node_temp))
         assemble(s)
         return s
 
@@ -203,7 +208,10 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
                     res.solves[r.rid] = 1
                     res.latency[r.rid] = time.perf_counter() - t0 - r.arrival
                     bt.remove(slot)
-                    s.dm.close()
+                    if pctx is ctx:
+                        s.dm.close()
+                    else:  # on the admission thread: its stream sync does not stall this loop
+                        prep.submit(s.dm.close)
                     used -= need
             for rid in [k for k, v in nl.items() if v[0].done()]:
                 fut, r, need = nl.pop(rid)
@@ -224,6 +232,8 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
         prep.shutdown(wait=True)
         pool.shutdown(wait=True)
         bt.close()
+        if pctx is not ctx:
+            pctx.close()  # after the batch; destroys any mesh a failure left behind
     return res
 
 

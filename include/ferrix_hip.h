@@ -315,7 +315,10 @@ fx_status fx_mesh_autotune_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, 
  * system's result does not depend on what else shares the batch.  Systems
  * may join while others run (continuous batching); a mesh must not be
  * reassembled while it is in a batch, and destroying a mesh drops its slots.
- * All calls run on the context stream.
+ * All calls run on the context stream.  A mesh may belong to another context
+ * on the same device: its assembly can then run on that context's stream
+ * (from another host thread) while the batch runs, and fx_batch_add orders
+ * the batch after the work queued on the mesh's stream so far.
  */
 typedef struct fx_batch fx_batch;
 

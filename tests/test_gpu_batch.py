@@ -375,3 +375,65 @@ def test_population_driver_budget(ctx, precision):
     for k in range(len(systems)):
         assert np.array_equal(a.x[k], c.x[k]), k
     say("seconds", a.seconds, c.seconds)
+
+
+@pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
+def test_population_driver_assembles_ahead(ctx, precision):
+    """Meshes on a second context, assembled by a host thread on that
+    context's stream while the batch runs (assemble_ahead): every solution and
+    iteration count is bit for bit the serial driver's, with everything
+    admitted at once and under a working-set budget."""
+    from ferrix_amd.batch import System, operator_bytes, solve_batched, upload
+    from ferrix_amd.device import Context
+
+    specs = small_specs(7, seed=53)
+    ref = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
+    upload(ctx, ref)
+    opts = SolveOptions(precision=precision, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000)
+    a = solve_batched(ctx, ref, opts)
+    with pytest.raises(ValueError):
+        solve_batched(ctx, ref, opts, assemble_ahead=True)  # meshes on the batch's own context
+    mctx = Context(0)
+    try:
+        systems = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
+        upload(mctx, systems)
+        b = solve_batched(ctx, systems, opts, chunk_iters=16, assemble_ahead=True)
+        budget = 2 * max(operator_bytes(s, opts) for s in systems)
+        c = solve_batched(ctx, systems, opts, ws_bytes=budget, chunk_iters=32, assemble_ahead=True)
+        for r in (a, b, c):
+            assert not r.failures and sorted(r.x) == list(range(len(systems)))
+        for r in (b, c):
+            for k in range(len(systems)):
+                assert np.array_equal(a.x[k], r.x[k]), k
+                assert a.iterations[k] == r.iterations[k], k
+        say("seconds", a.seconds, b.seconds, c.seconds)
+    finally:
+        mctx.close()
+
+
+def test_mesh_of_another_context_destroyed_while_batched(ctx):
+    """A mesh of a second context in a batch: destroying it drops its slot
+    (after the batch's stream has finished with it), the other system still
+    converges, and closing the mesh's context with the batch alive drops
+    the rest."""
+    from ferrix_amd.device import Context
+
+    mctx = Context(0)
+    bt = Batch(ctx, REL)
+    try:
+        sys = population(mctx, small_specs(2, seed=9))
+        s0 = bt.add(*sys[0])
+        bt.add(*sys[1])
+        bt.run(max_iters=5)
+        sys[1][0].close()
+        assert bt.run() == 0
+        assert bt.poll(s0)[0] == 1
+        x0 = bt.get(s0)
+        for r, b in enumerate(sys[0][1]):  # the same system through fx_solve
+            x1, _ = sys[0][0].solve(b, REL)
+            assert np.max(np.abs(x0[r] - x1)) <= 1e-8 * np.max(np.abs(x1))
+        mctx.close()
+        assert bt.run() == 0
+    finally:
+        bt.close()
+        mctx.close()
