@@ -43,23 +43,23 @@ __device__ __forceinline__ gptr<T> gp(T *p) {
 // the value loads of two blocks together, behind one memory latency instead
 // of two.
 template <typename VT, typename XT, int RR>
-__device__ __forceinline__ void gather_block(gptr<const XT> pv, int c, XT (&x)[3][RR]) {
+__device__ __forceinline__ void gather_block(gptr<const XT> pv, int c, double (&x)[3][RR]) {
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int r = 0; r < RR; ++r)
-            x[q][r] = pv[(3 * (int64_t)c + q) * RR + r];
+            x[q][r] = (double)pv[(3 * (int64_t)c + q) * RR + r];
 }
 
-template <typename VT, typename XT, int RR>
-__device__ __forceinline__ void madd_block(const VT (&v)[9], const XT (&x)[3][RR], double (&acc)[3][RR]) {
+template <typename VT, int RR>
+__device__ __forceinline__ void madd_block(const VT (&v)[9], const double (&x)[3][RR], double (&acc)[3][RR]) {
 #pragma unroll
     for (int r = 0; r < RR; ++r)
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-            acc[p][r] += (double)v[3 * p] * (double)x[0][r];
-            acc[p][r] += (double)v[3 * p + 1] * (double)x[1][r];
-            acc[p][r] += (double)v[3 * p + 2] * (double)x[2][r];
+            acc[p][r] += (double)v[3 * p] * x[0][r];
+            acc[p][r] += (double)v[3 * p + 1] * x[1][r];
+            acc[p][r] += (double)v[3 * p + 2] * x[2][r];
         }
 }
 
@@ -77,7 +77,7 @@ __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> sl_ptr, gptr<
     for (int k = k0; k < kw; k += 2) {
         const bool two = k + 1 < kw;
         const int c0 = cn0, c1 = cn1;
-        XT x0[3][RR], x1[3][RR];
+        double x0[3][RR], x1[3][RR];
         VT v0[9], v1[9];
         gather_block<VT, XT, RR>(pv, c0, x0);
 #pragma unroll
@@ -93,9 +93,9 @@ __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> sl_ptr, gptr<
             cn0 = scol[(size_t)(k + 2) * kSlice + lane];
         if (k + 3 < kw)
             cn1 = scol[(size_t)(k + 3) * kSlice + lane];
-        madd_block<VT, XT, RR>(v0, x0, acc);
+        madd_block<VT, RR>(v0, x0, acc);
         if (two)
-            madd_block<VT, XT, RR>(v1, x1, acc);
+            madd_block<VT, RR>(v1, x1, acc);
     }
 }
 
@@ -121,7 +121,7 @@ __device__ __forceinline__ void ssym_lower_rows(gptr<const int32_t> sll_ptr, gpt
     for (int k = k0; k < kw; k += 2) {
         const bool two = k + 1 < kw;
         const int r0 = rn0, c0 = cn0, r1 = rn1, c1 = cn1;
-        XT x0[3][RR], x1[3][RR];
+        double x0[3][RR], x1[3][RR];
         float v0[9], v1[9];
         gather_block<float, XT, RR>(pv, c0, x0);
 #pragma unroll
@@ -142,18 +142,18 @@ __device__ __forceinline__ void ssym_lower_rows(gptr<const int32_t> sll_ptr, gpt
         for (int r = 0; r < RR; ++r)
 #pragma unroll
             for (int p = 0; p < 3; ++p) {
-                acc[p][r] += (double)v0[p] * (double)x0[0][r];
-                acc[p][r] += (double)v0[3 + p] * (double)x0[1][r];
-                acc[p][r] += (double)v0[6 + p] * (double)x0[2][r];
+                acc[p][r] += (double)v0[p] * x0[0][r];
+                acc[p][r] += (double)v0[3 + p] * x0[1][r];
+                acc[p][r] += (double)v0[6 + p] * x0[2][r];
             }
         if (two)
 #pragma unroll
             for (int r = 0; r < RR; ++r)
 #pragma unroll
                 for (int p = 0; p < 3; ++p) {
-                    acc[p][r] += (double)v1[p] * (double)x1[0][r];
-                    acc[p][r] += (double)v1[3 + p] * (double)x1[1][r];
-                    acc[p][r] += (double)v1[6 + p] * (double)x1[2][r];
+                    acc[p][r] += (double)v1[p] * x1[0][r];
+                    acc[p][r] += (double)v1[3 + p] * x1[1][r];
+                    acc[p][r] += (double)v1[6 + p] * x1[2][r];
                 }
     }
 }
