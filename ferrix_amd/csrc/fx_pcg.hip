@@ -46,18 +46,33 @@ __device__ __forceinline__ unsigned xcd_swizzle(unsigned id, unsigned nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
-// Sum of part[NV*b + k] over b < nparts in a fixed order; the result is
-// returned to every thread of the block.
+// Sum of part[NV*b + k] over b < nparts in a fixed order (each thread its
+// rows b = tid, tid + kBlock, ... in turn, then the block sum); the result
+// is returned to every thread of the block.  A thread loads four of its
+// rows before adding them: a row past the end reloads the last one, and
+// its add is not taken.
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double *__restrict__ part, int nparts, double (&out)[NV]) {
     __shared__ double bcast[NV];
+    constexpr int U = 4;
 #pragma unroll
     for (int k = 0; k < NV; ++k)
         out[k] = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kBlock)
+    for (int b0 = threadIdx.x; b0 < nparts; b0 += U * kBlock) {
+        double t[U][NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            out[k] += part[NV * b + k];
+        for (int u = 0; u < U; ++u) {
+            const int b = b0 + u * kBlock < nparts ? b0 + u * kBlock : nparts - 1;
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                t[u][k] = part[NV * b + k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                out[k] = b0 + u * kBlock < nparts ? out[k] + t[u][k] : out[k];
+    }
     block_sum<NV>(out);
     if (threadIdx.x == 0)
 #pragma unroll
