@@ -351,7 +351,9 @@ def run_batch(a):
     ws = int(a.ws_mb * (1 << 20))
     admit = (sorted(systems, key=lambda s: -s.n_dof * s.n_rhs) if a.admit_order == "largest" else systems)
     arms, results = {}, {}
-    for arm in a.arms.split(","):
+    arm_list = a.arms.split(",")
+    head = "mixed_batched" if "mixed_batched" in arm_list else arm_list[0]
+    for arm in arm_list:
         if arm == "fp64_serial":
             step = lambda: solve_serial(systems, fp64)
         elif arm == "mixed_batched":
@@ -362,20 +364,25 @@ def run_batch(a):
                                          assemble_ahead=a.assemble_ahead)
         else:
             raise SystemExit(f"unknown arm {arm}")
-        for _ in range(a.warmup):
+        # the headline arm times --steps after --warmup; the comparison arms
+        # a fixed, stated count (one of their steps is several times longer)
+        n_steps, n_warm = (a.steps, a.warmup) if arm == head else (a.compare_steps, a.compare_warmup)
+        for _ in range(n_warm):
             step()
         device_sync()
         t0 = time.perf_counter()
         runs = []
-        for _ in range(a.steps):
+        for _ in range(n_steps):
             runs.append(step())
         device_sync()
         wall = time.perf_counter() - t0
         last = runs[-1]
         lat = np.concatenate([np.array(list(r.latency.values())) for r in runs])
         res = {
-            "dof_solves_per_s": n_solves * a.steps / wall,
-            "ms_per_step": 1e3 * wall / a.steps,
+            "steps": n_steps,
+            "warmup": n_warm,
+            "dof_solves_per_s": n_solves * n_steps / wall,
+            "ms_per_step": 1e3 * wall / n_steps,
             "p50_ms": 1e3 * float(np.percentile(lat, 50)),
             "p99_ms": 1e3 * float(np.percentile(lat, 99)),
             "failures": len(last.failures),
@@ -399,7 +406,6 @@ def run_batch(a):
             errs = [float(np.max(np.linalg.norm(r.x[k] - ref.x[k], axis=1) / np.linalg.norm(ref.x[k], axis=1)))
                     for k in ref.x if k in r.x]
             arms[arm]["u_rel_err_vs_fp64_max"] = float(max(errs)) if errs else None
-    head = "mixed_batched" if "mixed_batched" in arms else next(iter(arms))
     roof = None
     st_ = arms[head].get("spmv")
     if st_ and st_["achieved_GBps"]:
@@ -443,6 +449,9 @@ def run_batch(a):
         "arms": arms,
         "mixed_over_fp64": (round(arms["mixed_batched"]["dof_solves_per_s"] / arms["fp64_serial"]["dof_solves_per_s"],
                                   3) if "mixed_batched" in arms and "fp64_serial" in arms else None),
+        "mixed_over_fp64_batched": (round(arms["mixed_batched"]["dof_solves_per_s"] /
+                                          arms["fp64_batched"]["dof_solves_per_s"], 3)
+                                    if "mixed_batched" in arms and "fp64_batched" in arms else None),
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -532,6 +541,10 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--arms", default=None)
+    ap.add_argument("--compare-steps", type=int, default=1,
+                    help="batch: timed steps of the comparison arms (fp64_serial, fp64_batched)")
+    ap.add_argument("--compare-warmup", type=int, default=1,
+                    help="batch: warm-up steps of the comparison arms")
     ap.add_argument("--meshes", type=int, default=512)
     ap.add_argument("--ws-mb", type=float, default=0.0, help="operator working-set budget of the batch (0: all)")
     ap.add_argument("--assemble-ahead", action="store_true",
