@@ -367,13 +367,17 @@ def run_batch(a):
         # the headline arm times --steps after --warmup; the comparison arms
         # a fixed, stated count (one of their steps is several times longer)
         n_steps, n_warm = (a.steps, a.warmup) if arm == head else (a.compare_steps, a.compare_warmup)
-        for _ in range(n_warm):
+        for k in range(n_warm):
+            ts = time.perf_counter()
             step()
+            print(f"# {arm} warm-up step {k + 1}/{n_warm}: {time.perf_counter() - ts:.2f} s", flush=True)
         device_sync()
         t0 = time.perf_counter()
         runs = []
-        for _ in range(n_steps):
+        for k in range(n_steps):
+            ts = time.perf_counter()
             runs.append(step())
+            print(f"# {arm} step {k + 1}/{n_steps}: {time.perf_counter() - ts:.2f} s", flush=True)
         device_sync()
         wall = time.perf_counter() - t0
         last = runs[-1]

@@ -885,3 +885,28 @@ double fxo_pcg_time_iters(int64_t n, const int64_t *rp, const int32_t *ci, const
     free(x);
     return t1 - t0;
 }
+
+/* FNV-1a (64-bit, offset basis 0xcbf29ce484222325, prime 0x100000001b3) over
+ * row_ptr as little-endian int64 followed by col as little-endian int32: the
+ * pattern fingerprint of the golden fixtures (test helper, not part of the
+ * restated path). */
+uint64_t fxo_fnv1a_pattern(int64_t n_rows, const int64_t *row_ptr, int64_t nnz, const int32_t *col)
+{
+    uint64_t h = 0xcbf29ce484222325ULL;
+    const uint64_t prime = 0x100000001b3ULL;
+    for (int64_t i = 0; i <= n_rows; ++i) {
+        uint64_t v = (uint64_t)row_ptr[i];
+        for (int b = 0; b < 8; ++b) {
+            h ^= (v >> (8 * b)) & 0xffu;
+            h *= prime;
+        }
+    }
+    for (int64_t i = 0; i < nnz; ++i) {
+        uint32_t v = (uint32_t)col[i];
+        for (int b = 0; b < 4; ++b) {
+            h ^= (v >> (8 * b)) & 0xffu;
+            h *= prime;
+        }
+    }
+    return h;
+}

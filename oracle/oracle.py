@@ -68,12 +68,22 @@ def lib():
         L.fxo_shape_c3d4.argtypes = [dbl, dbl, dbl, P, P]
         L.fxo_gauss_points.argtypes = [i32, P, P]
         L.fxo_d_matrix.argtypes = [dbl, dbl, P]
+        L.fxo_fnv1a_pattern.argtypes = [i64, P, i64, P]
+        L.fxo_fnv1a_pattern.restype = C.c_uint64
         _LIB = L
     return _LIB
 
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def pattern_fnv1a(row_ptr, col) -> str:
+    """FNV-1a fingerprint of a CSR pattern: row_ptr as int64 LE, then col as
+    int32 LE (the golden fixtures' pattern hash; a test helper)."""
+    rp = np.ascontiguousarray(row_ptr, np.int64)
+    ci = np.ascontiguousarray(col, np.int32)
+    return f"{int(lib().fxo_fnv1a_pattern(len(rp) - 1, _p(rp), len(ci), _p(ci))):016x}"
 
 
 class OracleError(Exception):

@@ -7,7 +7,9 @@ For every config: the oracle's filtered CSR pattern size and FNV-1a hash
 displacement of the oracle's step driver with the DIRECT stand-in (SuperLU)
 -- the FP64 CPU oracle the GPU results are compared against.  cfg3 is the
 NLGEOM beam (10 increments + the 1e-16 micro-increment); its per-increment
-NR iteration counts are stored too.
+NR iteration counts, relative residuals and the step's outcome (the error
+that stops it, if any) are stored too.  cfg3_conv is the same beam at a tip
+load the step completes.
 """
 import json
 import os
@@ -35,23 +37,15 @@ def fnv1a(*arrays) -> str:
     return f"{h:016x}"
 
 
-def fnv1a_np(row_ptr, col) -> str:
-    # byte-wise FNV-1a, vectorised per chunk (same result as fnv1a)
-    h = np.uint64(0xCBF29CE484222325)
-    prime = np.uint64(0x100000001B3)
-    data = np.concatenate([np.ascontiguousarray(row_ptr, "<i8").view(np.uint8),
-                           np.ascontiguousarray(col, "<i4").view(np.uint8)])
-    with np.errstate(over="ignore"):
-        for b in data:
-            h = (h ^ np.uint64(b)) * prime
-    return f"{int(h):016x}"
-
-
 CONFIGS = {
     "cfg1": meshgen.cfg1_c3d4_cantilever,
     "cfg2": meshgen.cfg2_c3d20_block,
     "cfg3": meshgen.cfg3_c3d20_nlgeom_beam,
+    # the same 80x4x4 beam at a tip load the Picard NLGEOM iteration follows
+    # to the end of the step (cfg3's 110 kN stops at the minimum increment)
+    "cfg3_conv": lambda: meshgen.cfg3_c3d20_nlgeom_beam(tip_load=CFG3_CONV_LOAD),
 }
+CFG3_CONV_LOAD = 20000.0
 
 
 def main(names):
@@ -61,18 +55,39 @@ def main(names):
         csr, md, _ = O.assemble_csr(job.mesh, job.node_temp)
         meta = dict(config=name, workload=job.name, n_nodes=job.mesh.n_nodes, n_el=job.mesh.n_el,
                     n_dof=job.n_dof, nnz=csr.nnz, max_diag=md,
-                    pattern_fnv1a=fnv1a(np.ascontiguousarray(csr.row_ptr, "<i8"),
-                                        np.ascontiguousarray(csr.col, "<i4")) if csr.nnz < 200_000 else None)
-        u, logs = O.run_job(job, solver_override=SOLVER_DIRECT)
+                    pattern_fnv1a=O.pattern_fnv1a(csr.row_ptr, csr.col))
+        if name == "cfg3_conv":
+            meta["tip_load"] = CFG3_CONV_LOAD
+        # the step driver with the DIRECT stand-in; an error that stops the
+        # step (as the reference's would) is part of the fixture
+        u = np.zeros(job.n_dof)
+        timer = O.SolverTime()
+        logs = []
+        outcome = "ok"
+        try:
+            for k in range(len(job.steps)):
+                lg = O.StepLog()
+                logs.append(lg)
+                u = O.static_step(job, k, u, timer, SOLVER_DIRECT, lg)
+        except O.OracleError as e:
+            outcome = str(e)
+            u = None
+        meta["outcome"] = outcome
         meta["linear_solves"] = sum(lg.linear_solves for lg in logs)
+        # per increment: NR iterations that solved, converged, dt, and the
+        # relative residual of every NR iteration
         meta["increments"] = [[len(i["iters"]), i["converged"], i["dt"]] for lg in logs for i in lg.increments]
-        meta["u_max_abs"] = float(np.max(np.abs(u)))
+        meta["increment_rel_residuals"] = [i["rel"] for lg in logs for i in lg.increments]
+        if u is not None:
+            meta["u_max_abs"] = float(np.max(np.abs(u)))
         meta["seconds"] = time.time() - t0
-        np.savez_compressed(os.path.join(HERE, f"{name}_oracle.npz"), u=u, row_ptr_sum=np.int64(csr.row_ptr.sum()),
-                            col_sum=np.int64(csr.col.astype(np.int64).sum()))
+        arrays = dict(row_ptr_sum=np.int64(csr.row_ptr.sum()), col_sum=np.int64(csr.col.astype(np.int64).sum()))
+        if u is not None:
+            arrays["u"] = u
+        np.savez_compressed(os.path.join(HERE, f"{name}_oracle.npz"), **arrays)
         with open(os.path.join(HERE, f"{name}_oracle.json"), "w") as fh:
             json.dump(meta, fh, indent=1)
-        print(json.dumps(meta), flush=True)
+        print(json.dumps({k: v for k, v in meta.items() if k != "increment_rel_residuals"}), flush=True)
 
 
 if __name__ == "__main__":
