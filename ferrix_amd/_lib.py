@@ -90,6 +90,7 @@ SIGNATURES = {
     "fx_apply_penalty": (i32, [P, i64, P, P, P, P]),
     "fx_solve_opts_default": (None, [P]),
     "fx_solve": (i32, [P, P, P, P, P]),
+    "fx_solve_csr": (i32, [P, i64, P, P, P, P, P, P, P]),
     "fx_static_step": (i32, [P, P, P, P]),
     "fx_profile_spmv": (i32, [P, i32, i32, i32, P, P]),
     "fx_measure_fp64_peak": (i32, [P, i32, P, P]),

@@ -898,6 +898,37 @@ fx_status fx_solve(fx_mesh *m, const double *rhs, double *x_out, const fx_solve_
     FX_API_END(m->ctx)
 }
 
+fx_status fx_solve_csr(fx_ctx *ctx, int64_t n, const int64_t *row_ptr, const int64_t *col_idx,
+                       const double *values, const double *rhs, double *x_out, const fx_solve_opts *opts,
+                       fx_solve_report *report) {
+    if (!ctx)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "context is NULL");
+    FX_API_BEGIN
+    require(n >= 0, "fx_solve_csr: n < 0");
+    require(row_ptr && (n == 0 || (rhs && x_out)), "fx_solve_csr: NULL argument");
+    require(row_ptr[n] == 0 || (col_idx && values), "fx_solve_csr: NULL argument");
+    fx_solve_opts o;
+    if (opts)
+        o = *opts;
+    else
+        fx_solve_opts_default(&o);
+    check_solve_opts(o);
+    require(o.precision == FX_PREC_FP64, "fx_solve_csr: only FX_PREC_FP64 (the mixed solve needs a mesh's "
+                                         "node-block operator: fx_solve)");
+    if (report)
+        *report = fx_solve_report{};
+    if (n == 0)
+        return FX_OK;
+    set_device(ctx);
+    fx_solve_report r{};
+    const fx_status st = fx::solve_csr_host(ctx, n, row_ptr, col_idx, values, rhs, x_out, o, &r);
+    if (report)
+        *report = r;
+    if (st != FX_OK)
+        throw Error{st, solve_error(st, o, r)};
+    FX_API_END(ctx)
+}
+
 // ---------------------------------------------------------------------------
 // StaticStep::solve (static_step.rs:47-267)
 // ---------------------------------------------------------------------------

@@ -403,11 +403,8 @@ int autotune_spmv(fx_mesh *m, int precision, int reps, double *best_ms) {
     return best;
 }
 
-// Iterations are issued in batches (a captured hipGraph per batch size) and
-// the status block is read after each batch.  The next batch is the largest
-// power of two not above the iterations still predicted from the observed
-// convergence rate, so few launches run past convergence (they return at
-// once).  FX_PCG_GRAPH=0 issues plain launches instead of graphs.
+// Each chunk of iterations (iterate_chunks) is one captured hipGraph per
+// chunk size; FX_PCG_GRAPH=0 issues plain launches instead of graphs.
 static bool use_graphs() {
     const char *e = std::getenv("FX_PCG_GRAPH");
     return !(e && e[0] == '0');
@@ -439,38 +436,18 @@ template <typename VT>
 static int pcg_iterate(fx_mesh *m, const VT *val, double tol, int64_t max_iter, int G, PcgScalars *out) {
     hipStream_t s = m->ctx->stream;
     const bool graphs = use_graphs();
-    PcgScalars h{};
-    int chunk = 8;
-    double crit_prev = -1.0;
-    int64_t it_prev = 0;
-    for (;;) {
-        if (graphs) {
-            FX_HIP(hipGraphLaunch(batch_graph<VT>(m, val, G, chunk), s));
-        } else {
-            for (int k = 0; k < chunk; ++k)
-                launch_iteration<VT>(m, val, G, s);
-            FX_HIP(hipGetLastError());
-        }
-        FX_HIP(hipMemcpyAsync(&h, m->pcg.p, sizeof h, hipMemcpyDeviceToHost, s));
-        FX_HIP(hipStreamSynchronize(s));
-        if (h.status != 0)
-            break;
-        // predicted iterations left from the mean log-reduction per iteration
-        double left = 64.0;
-        const int64_t did = h.it_k1 - it_prev;
-        if (crit_prev > 0.0 && did > 0 && h.crit > 0.0 && h.crit < crit_prev) {
-            const double rate = (std::log(crit_prev) - std::log(h.crit)) / (double)did;
-            left = (std::log(h.crit) - std::log(tol > 0.0 ? tol : 1e-300)) / rate;
-        }
-        left = std::min<double>(left, (double)(max_iter - h.it + 1));
-        chunk = 1;
-        while (chunk * 2 <= left && chunk < 64)
-            chunk *= 2;
-        crit_prev = h.crit;
-        it_prev = h.it_k1;
-    }
-    *out = h;
-    return h.status;
+    return iterate_chunks(
+        s, m->pcg.p, tol, max_iter,
+        [&](int chunk) {
+            if (graphs) {
+                FX_HIP(hipGraphLaunch(batch_graph<VT>(m, val, G, chunk), s));
+            } else {
+                for (int k = 0; k < chunk; ++k)
+                    launch_iteration<VT>(m, val, G, s);
+                FX_HIP(hipGetLastError());
+            }
+        },
+        out);
 }
 
 // One PCG solve on the resident operator `val` with rhs b_dev; x in m->x.
@@ -491,12 +468,6 @@ static void pcg_resume(fx_mesh *m, const double *r64, double rr64, double tol, c
                        m->r.p, m->z.p, p, m->inv_diag.p, part2_of(m));
     hipLaunchKernelGGL(k_resume_scalars, dim3(1), dim3(64), 0, s, m->pcg.p, rr64, tol);
     FX_HIP(hipGetLastError());
-}
-
-static int64_t iterations_done(const PcgScalars &h) {
-    // the stop rule fires in k_spmv_p of iteration it_k1 (it_k1 SpMVs done);
-    // a breakdown fires in k_update after that iteration's SpMV
-    return h.status == 2 ? h.it_k1 + 1 : h.it_k1;
 }
 
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep) {

@@ -196,5 +196,48 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, double *__restrict
 
 inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// Iterations are issued in chunks and the status block `sc_dev` is read
+// after each chunk (one host round trip per chunk).  The next chunk is the
+// largest power of two (at most 64) not above the iterations still
+// predicted from the observed convergence rate, so few launches run past
+// convergence (they return at once).  launch(chunk) issues that many
+// iterations on stream s.  Returns the final status (1 converged,
+// 2 breakdown, 3 max_iter) with the scalars in *out.
+template <class LaunchChunk>
+int iterate_chunks(hipStream_t s, PcgScalars *sc_dev, double tol, int64_t max_iter, LaunchChunk &&launch,
+                   PcgScalars *out) {
+    PcgScalars h{};
+    int chunk = 8;
+    double crit_prev = -1.0;
+    int64_t it_prev = 0;
+    for (;;) {
+        launch(chunk);
+        FX_HIP(hipMemcpyAsync(&h, sc_dev, sizeof h, hipMemcpyDeviceToHost, s));
+        FX_HIP(hipStreamSynchronize(s));
+        if (h.status != 0)
+            break;
+        // predicted iterations left from the mean log-reduction per iteration
+        double left = 64.0;
+        const int64_t did = h.it_k1 - it_prev;
+        if (crit_prev > 0.0 && did > 0 && h.crit > 0.0 && h.crit < crit_prev) {
+            const double rate = (std::log(crit_prev) - std::log(h.crit)) / (double)did;
+            left = (std::log(h.crit) - std::log(tol > 0.0 ? tol : 1e-300)) / rate;
+        }
+        left = std::min<double>(left, (double)(max_iter - h.it + 1));
+        chunk = 1;
+        while (chunk * 2 <= left && chunk < 64)
+            chunk *= 2;
+        crit_prev = h.crit;
+        it_prev = h.it_k1;
+    }
+    *out = h;
+    return h.status;
+}
+
+// SpMVs done when the solve stopped: the stop rule fires in the SpMV head
+// of iteration it_k1 (it_k1 SpMVs done); a breakdown fires in the update
+// after that iteration's SpMV.
+inline int64_t iterations_done(const PcgScalars &h) { return h.status == 2 ? h.it_k1 + 1 : h.it_k1; }
+
 }  // namespace
 }  // namespace fx

@@ -1,7 +1,8 @@
 """Python host layer over the C ABI: the GPU counterparts of FerriX's
 assembly, solver and step seams.
 
-* :class:`Context` -- one HIP device and stream (``fx_ctx``).
+* :class:`Context` -- one HIP device and stream (``fx_ctx``); ``solve_csr``
+  solves a caller's CSR matrix (``Solver::solve``, ``solvers/mod.rs:18-44``).
 * :class:`DeviceMesh` -- a resident mesh (``fx_mesh``): ``assemble``
   (``Assembler::assemble``, ``src/solver/assembler.rs:37-151``),
   ``internal_force`` (``:157-241``), ``stress_strain``
@@ -104,6 +105,26 @@ class Context:
         v, m = C.c_double(), C.c_double()
         self.check(self.lib.fx_measure_fp64_peak(self.h, reps, C.byref(v), C.byref(m)))
         return v.value, m.value
+
+    def solve_csr(self, row_ptr, col, val, rhs, opts: Optional[SolveOptions] = None):
+        """``Solver::solve(k_global, f_global, ...)`` on the caller's CSR
+        (``solvers/mod.rs:18-44``; fx_solve_csr): Jacobi-PCG from the
+        matrix's own diagonal, x0 = 0.  Returns (x, fx_solve_report)."""
+        rp = _c(row_ptr, np.int64)
+        ci = _c(col, np.int64)
+        v = _c(val, np.float64)
+        b = _c(rhs, np.float64)
+        n = len(rp) - 1
+        if len(b) != n:
+            raise InvalidModelState(f"solve_csr: rhs has {len(b)} entries for {n} rows")
+        if len(ci) < rp[-1] or len(v) < rp[-1]:
+            raise InvalidModelState("solve_csr: fewer column indices or values than row_ptr[n]")
+        x = np.zeros(n)
+        rep = L.fx_solve_report()
+        o = (opts or SolveOptions()).c()
+        self.check(self.lib.fx_solve_csr(self.h, n, _p(rp), _p(ci), _p(v), _p(b), _p(x), C.byref(o),
+                                         C.byref(rep)))
+        return x, rep
 
     def close(self):
         if getattr(self, "h", None):

@@ -197,6 +197,30 @@ void fx_solve_opts_default(fx_solve_opts *opts);
 fx_status fx_solve(fx_mesh *mesh, const double *rhs, double *x_out, const fx_solve_opts *opts,
                    fx_solve_report *report);
 
+/*
+ * Solver::solve on the caller's matrix: the seam's exact contract
+ * (src/solver/solvers/mod.rs:18-44), called by the step driver with the CSR
+ * it has just built (static_step.rs:198-204), here with the Jacobi
+ * preconditioner built from that matrix's diagonal (preconditioner.rs:39-50:
+ * 1/K_ii if |K_ii| > 1e-9, else 1; no stored diagonal: 1) and
+ * IterativeSolver's recurrences (iterative.rs:26-121), x0 = 0.
+ *   n        rows (= columns)
+ *   row_ptr  n+1 offsets, row_ptr[0] = 0, non-decreasing (sprs indptr, usize)
+ *   col_idx  row_ptr[n] column indices, strictly ascending per row (sprs
+ *            indices, usize); both triangles stored
+ *   values   row_ptr[n] values
+ *   rhs, x_out  n each (host)
+ * All arrays are host memory and are copied; the call returns after the
+ * solution is in x_out.  opts: FX_PREC_FP64 only (the mixed solve needs the
+ * node-block operator of a mesh, fx_solve); NULL = the reference's values.
+ * A malformed matrix gives FX_ERR_INVALID before anything runs;
+ * FX_ERR_NOT_CONVERGED after max_iter iterations and FX_ERR_BREAKDOWN as for
+ * fx_solve, with *report filled either way.
+ */
+fx_status fx_solve_csr(fx_ctx *ctx, int64_t n, const int64_t *row_ptr, const int64_t *col_idx,
+                       const double *values, const double *rhs, double *x_out, const fx_solve_opts *opts,
+                       fx_solve_report *report);
+
 /* ---- step driver (StaticStep::solve, static_step.rs:47-267) ----------- */
 /* Amplitudes (amplitude.rs:10-98); entry 0 must be the default ramp. */
 typedef struct fx_amplitudes {
