@@ -290,7 +290,11 @@ class DeviceMesh:
         rc = self.lib.fx_static_step(self.h, C.byref(sd), C.byref(timer), C.byref(rep))
         if raised:
             raise raised[0]
-        self.ctx.check(rc)
+        try:
+            self.ctx.check(rc)
+        except FerrixError as e:
+            e.report = rep  # the step's counters up to the error
+            raise
         del keep
         return rep
 
