@@ -104,6 +104,7 @@ SIGNATURES = {
     "fx_batch_remove": (i32, [P, i32]),
     "fx_batch_time_spmv": (i32, [P, i32]),
     "fx_batch_spmv_stats": (i32, [P, P, P, P, P]),
+    "fx_solve_csr_noalloc": (i32, [P, i64, i64, P, P, P, P, P, P, P]),
 }
 
 _LIB = None

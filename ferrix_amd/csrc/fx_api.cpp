@@ -379,6 +379,7 @@ fx_status fx_ctx_create(int device, fx_ctx **out) {
     ctx->device = device;
     ctx->n_cu = prop.multiProcessorCount;
     FX_HIP(hipSetDevice(device));
+    fx::csr_scratch_setup(device);
     FX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     fx::GaussTable t4 = fx::gauss_table(4), t20 = fx::gauss_table(20);
     auto &T = ctx->tables;
@@ -922,6 +923,36 @@ fx_status fx_solve_csr(fx_ctx *ctx, int64_t n, const int64_t *row_ptr, const int
     set_device(ctx);
     fx_solve_report r{};
     const fx_status st = fx::solve_csr_host(ctx, n, row_ptr, col_idx, values, rhs, x_out, o, &r);
+    if (report)
+        *report = r;
+    if (st != FX_OK)
+        throw Error{st, solve_error(st, o, r)};
+    FX_API_END(ctx)
+}
+
+fx_status fx_solve_csr_noalloc(fx_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr_dev,
+                               const int64_t *col_dev, const double *val_dev, const double *rhs_dev, double *x_dev,
+                               const fx_solve_opts *opts, fx_solve_report *report) {
+    if (!ctx)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "context is NULL");
+    FX_API_BEGIN
+    require(n >= 0 && nnz >= 0, "fx_solve_csr_noalloc: n < 0 or nnz < 0");
+    require(row_ptr_dev && (n == 0 || (rhs_dev && x_dev)) && (nnz == 0 || (col_dev && val_dev)),
+            "fx_solve_csr_noalloc: NULL argument");
+    fx_solve_opts o;
+    if (opts)
+        o = *opts;
+    else
+        fx_solve_opts_default(&o);
+    check_solve_opts(o);
+    require(o.precision == FX_PREC_FP64, "fx_solve_csr_noalloc: only FX_PREC_FP64");
+    if (report)
+        *report = fx_solve_report{};
+    if (n == 0)
+        return FX_OK;
+    set_device(ctx);
+    fx_solve_report r{};
+    const fx_status st = fx::solve_csr_noalloc(ctx, n, nnz, row_ptr_dev, col_dev, val_dev, rhs_dev, x_dev, o, &r);
     if (report)
         *report = r;
     if (st != FX_OK)

@@ -22,9 +22,14 @@
 // sequential row sum, which the PCG tolerance absorbs (as the reference's
 // rayon reductions do).
 //
-// fx_solve_csr takes host arrays in and out; the matrix is checked on the
-// host, copied with 32-bit column indices (n < 2^31) and solved with work
-// vectors from the context's device pool.
+// Two entry points share the kernels:
+//  * fx_solve_csr: host arrays in and out; the matrix is checked on the
+//    host, copied with 32-bit column indices (n < 2^31) and solved with
+//    work vectors from the context's device pool.
+//  * fx_solve_csr_noalloc: device arrays in and out, no device allocation;
+//    the work vectors live in `scratch`, a file-scope device array of
+//    kScratchBytes per GPU, and the matrix is checked by a kernel before
+//    any gather.
 #include "fx_pcg_core.h"
 
 #include <chrono>
@@ -32,13 +37,27 @@
 namespace fx {
 namespace {
 
+// ---------------------------------------------------------------------------
+// the no-allocation entry point's scratch memory: one array per GPU, held
+// until the process ends.  Host code never uses the symbol itself (its host
+// name is a stand-in, not a device address): csr_scratch_setup resolves the
+// device address once per device with hipGetSymbolAddress.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mx;
+void *g_scratch_ptr[kMaxDevices] = {};
+
 // Work vectors of one solve carved out of one device block: r, z, p (two
 // buffers), Ap, inv_diag (n each), the SpMV block partials (one per row
-// group), the update partials (three per 256 rows) and the PCG scalars.
-// Every section starts on a 256-byte boundary.
+// group), the update partials (three per 256 rows), the PCG scalars and the
+// check flag.  Every section starts on a 256-byte boundary.
 struct CsrWork {
     double *r, *z, *p0, *p1, *ap, *inv, *part1, *part2;
     PcgScalars *sc;
+    int32_t *flag;
 };
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -62,10 +81,12 @@ size_t csr_work_layout(int64_t n, int G, char *base, CsrWork *w) {
     char *p1 = take(sizeof(double) * (size_t)nblocks(n, kBlock / G));
     char *p2 = take(sizeof(double) * 3 * (size_t)nblocks(n, kBlock));
     char *sc = take(sizeof(PcgScalars));
+    char *fl = take(sizeof(int32_t));
     if (base) {
         w->part1 = (double *)p1;
         w->part2 = (double *)p2;
         w->sc = (PcgScalars *)sc;
+        w->flag = (int32_t *)fl;
     }
     return off;
 }
@@ -73,6 +94,40 @@ size_t csr_work_layout(int64_t n, int G, char *base, CsrWork *w) {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+
+// Structure check of a device CSR before any gather: rp[0] = 0, rp[n] = nnz,
+// rows non-decreasing and inside [0, nnz], columns in [0, n) and strictly
+// ascending per row.  A thread scans its row only when the row's bounds are
+// valid, so every read stays inside rp[0..n] and col[0..nnz).  *flag (zero on
+// entry) gets bit 1 for a row-pointer fault and bit 2 for a column fault.
+template <typename CI>
+__global__ __launch_bounds__(kBlock) void k_csr_check(int64_t n, int64_t nnz, const int64_t *__restrict__ rp,
+                                                      const CI *__restrict__ col, int32_t *__restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0 && (rp[0] != 0 || rp[n] != nnz))
+        atomicOr(flag, 1);
+    if (i >= n)
+        return;
+    const int64_t k0 = rp[i], k1 = rp[i + 1];
+    if (k0 < 0 || k1 < k0 || k1 > nnz) {
+        atomicOr(flag, 1);
+        return;
+    }
+    int64_t prev = -1;
+    for (int64_t k = k0; k < k1; ++k) {
+        const int64_t c = (int64_t)col[k];
+        if (c <= prev || c >= n) {
+            atomicOr(flag, 2);
+            return;
+        }
+        prev = c;
+    }
+}
+
+__global__ void k_csr_clear_flag(int32_t *flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *flag = 0;
+}
 
 // inv[i] = 1/K_ii if |K_ii| > 1e-9, else 1 (no stored diagonal: 1); the
 // diagonal is found by binary search in the ascending row.
@@ -241,6 +296,23 @@ fx_status csr_pcg(hipStream_t s, int64_t n, const int64_t *rp, const CI *col, co
 
 }  // namespace
 
+// Resolves this device's scratch address once (the device must be
+// current); called by fx_ctx_create.
+void csr_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        throw Error{FX_ERR_INVALID, "device index beyond the scratch table"};
+    std::lock_guard<std::mutex> lk(g_scratch_mx);
+    if (g_scratch_ptr[device])
+        return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw Error{FX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+    g_scratch_ptr[device] = ptr;
+}
+
+size_t csr_scratch_bytes() { return kScratchBytes; }
+
 fx_status solve_csr_host(fx_ctx *ctx, int64_t n, const int64_t *row_ptr, const int64_t *col_idx,
                          const double *values, const double *rhs, double *x_out, const fx_solve_opts &o,
                          fx_solve_report *rep) {
@@ -294,6 +366,39 @@ fx_status solve_csr_host(fx_ctx *ctx, int64_t n, const int64_t *row_ptr, const i
     }
     FX_HIP(hipMemcpyAsync(x_out, x.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
     FX_HIP(hipStreamSynchronize(s));  // before the pool blocks go back
+    return st;
+}
+
+fx_status solve_csr_noalloc(fx_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr_dev,
+                            const int64_t *col_dev, const double *val_dev, const double *rhs_dev, double *x_dev,
+                            const fx_solve_opts &o, fx_solve_report *rep) {
+    hipStream_t s = ctx->stream;
+    void *base = ctx->device >= 0 && ctx->device < kMaxDevices ? g_scratch_ptr[ctx->device] : nullptr;
+    if (!base)
+        throw Error{FX_ERR_INVALID, "fx_solve_csr_noalloc: no scratch memory resolved for this device"};
+    const int G = csr_group(n, nnz, o.spmv_group);
+    CsrWork w{};
+    const size_t need = csr_work_layout(n, G, nullptr, &w);
+    if (need > kScratchBytes)
+        throw Error{FX_ERR_INVALID, "fx_solve_csr_noalloc: " + std::to_string(n) + " rows need " +
+                                        std::to_string(need) + " bytes of work memory, more than the " +
+                                        std::to_string(kScratchBytes) + "-byte scratch array holds"};
+    csr_work_layout(n, G, (char *)base, &w);
+    // structure check before any gather
+    hipLaunchKernelGGL(k_csr_clear_flag, dim3(1), dim3(64), 0, s, w.flag);
+    hipLaunchKernelGGL((k_csr_check<int64_t>), dim3(nblocks(std::max<int64_t>(n, 1), kBlock)), dim3(kBlock), 0, s,
+                       n, nnz, row_ptr_dev, col_dev, w.flag);
+    FX_HIP(hipGetLastError());
+    int32_t flag = 0;
+    FX_HIP(hipMemcpyAsync(&flag, w.flag, sizeof flag, hipMemcpyDeviceToHost, s));
+    FX_HIP(hipStreamSynchronize(s));
+    if (flag & 1)
+        throw Error{FX_ERR_INVALID, "fx_solve_csr_noalloc: row_ptr must start at 0, not decrease and end at nnz"};
+    if (flag & 2)
+        throw Error{FX_ERR_INVALID,
+                    "fx_solve_csr_noalloc: a column index is out of range or not strictly ascending in its row"};
+    const fx_status st = csr_pcg<int64_t>(s, n, row_ptr_dev, col_dev, val_dev, rhs_dev, x_dev, w, o, G, rep);
+    FX_HIP(hipStreamSynchronize(s));
     return st;
 }
 

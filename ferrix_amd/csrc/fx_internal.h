@@ -301,11 +301,17 @@ int default_group(fx_mesh *m);  // lanes per node row for the SpMV (row-length h
 int autotune_spmv(fx_mesh *m, int precision, int reps, double *best_ms);  // sets m->spmv_group
 void measure_fp64_peak(fx_ctx *ctx, int reps, double *valu_tflops, double *mfma_tflops);  // fx_peak.hip
 void set_error(const std::string &msg);  // message of the calling thread's last failure
-// Solver::solve on a caller's CSR (fx_csr.hip), host arrays, work vectors
-// from the context pool
+// Solver::solve on a caller's CSR (fx_csr.hip): host arrays (work vectors
+// from the context pool), or device arrays with the work vectors in the
+// per-device scratch array (no device allocation)
 fx_status solve_csr_host(fx_ctx *ctx, int64_t n, const int64_t *row_ptr, const int64_t *col_idx,
                          const double *values, const double *rhs, double *x_out, const fx_solve_opts &o,
                          fx_solve_report *rep);
+fx_status solve_csr_noalloc(fx_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr_dev,
+                            const int64_t *col_dev, const double *val_dev, const double *rhs_dev, double *x_dev,
+                            const fx_solve_opts &o, fx_solve_report *rep);
+void csr_scratch_setup(int device);  // resolves the device's scratch address (device current)
+size_t csr_scratch_bytes();
 // Mixed-precision stop on the true FP64 residual b - K x, which has a
 // rounding floor: each row sum cancels terms of total magnitude (|K| |x|)_i,
 // so the computed residual cannot fall much below eps * || |K| |x| ||.  (The

@@ -397,6 +397,31 @@ fx_status fx_batch_time_spmv(fx_batch *batch, int32_t on);
 fx_status fx_batch_spmv_stats(fx_batch *batch, int64_t *launches, int64_t *timed, double *ms_timed,
                               double *bytes_total);
 
+/* ---- solve without device allocation --------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops that must not call hipMalloc, harnesses that forbid
+ * it): fx_solve_csr on arrays already in device memory, with every work
+ * vector (r, z, p, Ap, the Jacobi scaling, block partials, PCG scalars) in
+ * one static device array of 64 MiB per GPU, resolved once per device by
+ * fx_ctx_create and held until the process ends.  Same kernels, same
+ * results as fx_solve_csr, bit for bit.
+ *   n, nnz                 rows and stored entries; the work vectors must
+ *                          fit the 64 MiB array (about 1.3 M rows), else
+ *                          FX_ERR_INVALID and nothing runs
+ *   row_ptr_dev, col_dev   int64, as for fx_solve_csr, in device memory;
+ *                          checked on the device (row_ptr[0] = 0,
+ *                          row_ptr[n] = nnz, non-decreasing, columns in
+ *                          range and strictly ascending) before any gather
+ *   val_dev, rhs_dev, x_dev  device memory; x_dev receives the solution
+ * Runs on the context's stream and returns after the solve has finished.
+ * Every call on one GPU uses that GPU's one array and overwrites what the
+ * last call left there: calls that overlap on one GPU (from several host
+ * threads or contexts) race and must be serialised by the caller.
+ */
+fx_status fx_solve_csr_noalloc(fx_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr_dev,
+                               const int64_t *col_dev, const double *val_dev, const double *rhs_dev, double *x_dev,
+                               const fx_solve_opts *opts, fx_solve_report *report);
+
 #ifdef __cplusplus
 }
 #endif
