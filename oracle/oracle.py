@@ -352,6 +352,7 @@ def amp_y(amps, a, timer: SolverTime, origin, current_step):
 class StepLog:
     increments: List[dict] = field(default_factory=list)
     linear_solves: int = 0
+    u_committed: Optional[np.ndarray] = None  # displacement of the last converged increment
     pcg_iterations: List[int] = field(default_factory=list)
     nnz: List[int] = field(default_factory=list)
 
@@ -424,6 +425,7 @@ def static_step(job, step_index, u, timer: SolverTime, solver_override=None, log
         if converged:
             current_time += dt
             u_start = u_curr
+            log.u_committed = u_start
         else:
             dt /= 2.0
             if dt < st.min_time_increment:

@@ -84,6 +84,11 @@ def main(names):
         arrays = dict(row_ptr_sum=np.int64(csr.row_ptr.sum()), col_sum=np.int64(csr.col.astype(np.int64).sum()))
         if u is not None:
             arrays["u"] = u
+        elif logs and logs[-1].u_committed is not None:
+            # the displacement the step leaves behind when it stops: that of
+            # its last converged increment
+            arrays["u_last_converged"] = logs[-1].u_committed
+            meta["converged_increments"] = sum(1 for i in logs[-1].increments if i["converged"])
         np.savez_compressed(os.path.join(HERE, f"{name}_oracle.npz"), **arrays)
         with open(os.path.join(HERE, f"{name}_oracle.json"), "w") as fh:
             json.dump(meta, fh, indent=1)

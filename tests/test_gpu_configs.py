@@ -71,3 +71,72 @@ def test_cfg2_full_size_step(ctx, precision):
     say("precision", precision, "pcg", reps[0].pcg_iterations, "rel err", e)
     assert reps[0].linear_solves == meta["linear_solves"]
     assert e < 1e-8
+
+
+def nlgeom_run(ctx, job, precision):
+    """The NLGEOM step through fx_static_step with a hook recording every
+    converged increment (inc_id, global time); returns (u, report, error,
+    converged increments)."""
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.set_u(np.zeros(job.n_dof))
+    timer = L.fx_timer(0.0, 0.0, 1.0)
+    st = job.steps[0]
+    so = SolveOptions(precision=precision, stop_mode=st.stop_mode, tol=st.tol, max_iter=st.max_iter)
+    seen = []
+    err = None
+    try:
+        rep = dm.static_step(st, 1, timer, direct_opts=so, solve_opts=so,
+                             on_increment=lambda sid, inc, t: seen.append((inc, t)))
+    except ConvergenceError as e:
+        err, rep = e, e.report
+    return dm.get_u(), rep, err, seen
+
+
+@pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
+def test_cfg3_full_size_stops_like_the_oracle(ctx, precision):
+    """80x4x4 C3D20 beam, 110 kN: the oracle's Picard iteration converges six
+    increments (t = 0.1 ... 0.6), then fails 15 NR iterations at every
+    halving of dt until dt < 1e-5 and stops with "Minimum time increment
+    reached" (static_step.rs:251-258).  The GPU step stops with the same
+    error after the same increments, cutbacks, NR iterations and linear
+    solves, and leaves the displacement of the sixth increment."""
+    meta, arr = golden("cfg3")
+    assert meta["outcome"].startswith("Minimum time increment reached")
+    job = meshgen.cfg3_c3d20_nlgeom_beam()
+    assert job.n_dof == meta["n_dof"] == 21_795
+    u, rep, err, seen = nlgeom_run(ctx, job, precision)
+    incs = meta["increments"]
+    nr = sum(len(r) for r in meta["increment_rel_residuals"])
+    conv = [k + 1 for k, i in enumerate(incs) if i[1]]
+    say("precision", precision, "increments", rep.increments, "cutbacks", rep.cutbacks, "nr", rep.nr_iterations,
+        "solves", rep.linear_solves, "pcg", rep.pcg_iterations, "converged", seen)
+    assert err is not None and "Minimum time increment reached" in str(err)
+    assert rep.increments == len(incs)
+    assert rep.cutbacks == sum(1 for i in incs if not i[1])
+    assert rep.nr_iterations == nr
+    assert rep.linear_solves == meta["linear_solves"]
+    assert [inc for inc, _ in seen] == conv
+    assert np.allclose([t for _, t in seen], np.cumsum([incs[k - 1][2] for k in conv]), rtol=0, atol=1e-12)
+    e = rel(u, arr["u_last_converged"])
+    say("u after the error vs the oracle's last converged increment:", e)
+    assert e < 1e-8
+
+
+@pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
+def test_cfg3_mesh_converging_load(ctx, precision):
+    """The same 80x4x4 beam at a tip load the step completes: the same NR
+    history as the oracle and the displacement within 1e-8 of its."""
+    meta, arr = golden("cfg3_conv")
+    assert meta["outcome"] == "ok"
+    job = meshgen.cfg3_c3d20_nlgeom_beam(tip_load=meta["tip_load"])
+    u, rep, err, seen = nlgeom_run(ctx, job, precision)
+    assert err is None
+    incs = meta["increments"]
+    say("precision", precision, "increments", rep.increments, "nr", rep.nr_iterations, "solves", rep.linear_solves,
+        "pcg", rep.pcg_iterations, "tip w", u.reshape(-1, 3)[:, 2].min())
+    assert rep.increments == len(incs)
+    assert rep.linear_solves == meta["linear_solves"]
+    assert rep.nr_iterations == sum(len(r) for r in meta["increment_rel_residuals"])
+    e = rel(u, arr["u"])
+    say("u vs the oracle:", e)
+    assert e < 1e-8
