@@ -4,8 +4,8 @@
 // tables.  Expressions follow the reference term by term so the tables are
 // bit-identical to what the reference's per-call evaluation produces:
 //   C3D4  rule and N/dN   src/solver/mesh_lib/elements/c3d4.rs:31-51
-//   C3D20 rule            src/solver/mesh_lib/elements/c3d20.rs:115-135
-//   C3D20 N/dN            src/solver/mesh_lib/elements/c3d20.rs:174-297
+//   C3D20 rule            src/solver/mesh_lib/elements/c3d20.rs:31-51
+//   C3D20 N/dN            src/solver/mesh_lib/elements/c3d20.rs:90-213
 // Built with -ffp-contract=off: no multiply-add is fused.
 #include "fx_internal.h"
 

@@ -8,7 +8,7 @@
  * What it restates (reference paths relative to the FerriX tree):
  *   - C3D4 / C3D20 Gauss rules and shape functions
  *       src/solver/mesh_lib/elements/c3d4.rs:31-51
- *       src/solver/mesh_lib/elements/c3d20.rs:115-135, 174-297
+ *       src/solver/mesh_lib/elements/c3d20.rs:31-51 (rule), 90-213 (N, dN)
  *   - temperature LUT and isotropic D matrix
  *       src/solver/material.rs:36-68, 180-230
  *   - B matrix, element stiffness, internal force, thermal force
@@ -48,7 +48,7 @@
 #define FXO_LUT_MAX 16
 
 /* ------------------------------------------------------------------ */
-/* Gauss rules (c3d4.rs:31-36, c3d20.rs:115-135)                       */
+/* Gauss rules (c3d4.rs:31-36, c3d20.rs:31-51)                         */
 /* ------------------------------------------------------------------ */
 int fxo_gauss_points(int nn, double *coords, double *w)
 {
@@ -88,7 +88,7 @@ void fxo_shape_c3d4(double xi, double eta, double zeta, double *N, double *dN)
     memcpy(dN, d, sizeof d);
 }
 
-/* c3d20.rs:174-297: 20-node serendipity brick, CalculiX node order.
+/* c3d20.rs:90-213: 20-node serendipity brick, CalculiX node order.
  * dN is row-major 3x20 (row 0: d/dxi, row 1: d/deta, row 2: d/dzeta). */
 void fxo_shape_c3d20(double xi, double et, double ze, double *N, double *dN)
 {

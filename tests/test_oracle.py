@@ -89,7 +89,7 @@ def test_gauss_rules():
     # 3-point Gauss-Legendre is exact to degree 5 per axis
     val = np.sum(w * c[:, 0] ** 4 * c[:, 1] ** 4 * c[:, 2] ** 4)
     assert abs(val - (2.0 / 5.0) ** 3) < 1e-12
-    # xi outermost, zeta innermost (c3d20.rs:124-133)
+    # xi outermost, zeta innermost (c3d20.rs:40-50)
     assert c[0, 2] < c[1, 2] and c[0, 0] == c[8, 0] and c[9, 0] > c[8, 0]
 
 
