@@ -423,3 +423,24 @@ def test_stress_recovery_uses_node_temperature():
     for n in range(len(T)):
         D = O.d_matrix(70000.0 + (35000.0 - 70000.0) * (T[n] / 100.0), nu)
         assert np.allclose(S[n], D @ E[n], rtol=1e-12, atol=1e-12 * np.abs(S).max())
+
+
+def test_pattern_fingerprint_is_fnv1a():
+    """The golden fixtures' pattern hash (oracle C helper) is byte-wise
+    FNV-1a over row_ptr (int64 LE) then col (int32 LE): equal to a plain
+    Python FNV-1a on a small pattern, and to the committed cfg1 hash."""
+    def fnv(data: bytes) -> str:
+        h = 0xCBF29CE484222325
+        for b in data:
+            h ^= b
+            h = (h * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+        return f"{h:016x}"
+
+    rp = np.array([0, 2, 3, 5], np.int64)
+    col = np.array([0, 2, 1, 0, 2], np.int32)
+    assert O.pattern_fnv1a(rp, col) == fnv(rp.astype("<i8").tobytes() + col.astype("<i4").tobytes())
+    assert O.pattern_fnv1a(rp, col) != O.pattern_fnv1a(rp, col[::-1].copy())
+    job = meshgen.cfg1_c3d4_cantilever()
+    csr, _, _ = O.assemble_csr(job.mesh, job.node_temp)
+    meta = json.load(open(os.path.join(ROOT, "tests", "golden", "cfg1_oracle.json")))
+    assert O.pattern_fnv1a(csr.row_ptr, csr.col) == meta["pattern_fnv1a"]
