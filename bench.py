@@ -327,6 +327,14 @@ def counter_traffic(name, bytes_total, bytes_per_launch):
     return round(ratio * bytes_per_launch)
 
 
+# dominant kernel of each batched arm, and the committed rocprofv3 counter
+# summary of it on the 512-mesh population
+DOMINANT_KERNEL = {"mixed_poly": "kp_cheb_sell", "mixed_batched": "kb_spmv_sell<float, float>",
+                   "fp64_batched": "kb_spmv_sell<double, double>"}
+COUNTER_SUMMARY = {"mixed_poly": "kp_cheb_sell_batch512_counters.md",
+                   "mixed_batched": "kb_spmv_sell_float_batch512_counters.md"}
+
+
 def run_batch(a):
     from ferrix_amd import _lib as L
     from ferrix_amd.batch import solve_batched, solve_serial, upload
@@ -348,16 +356,21 @@ def run_batch(a):
                           spmv_group=a.spmv_group)
     mixed = SolveOptions(precision=L.FX_PREC_MIXED, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000,
                          inner_tol=a.inner_tol, max_refine=100, spmv_group=a.spmv_group)
+    poly = SolveOptions(precision=L.FX_PREC_MIXED_POLY, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=400_000,
+                        spmv_group=a.spmv_group, poly_degree=a.poly_degree)
     ws = int(a.ws_mb * (1 << 20))
     admit = (sorted(systems, key=lambda s: -s.n_dof * s.n_rhs) if a.admit_order == "largest" else systems)
     arms, results = {}, {}
     arm_list = a.arms.split(",")
-    head = "mixed_batched" if "mixed_batched" in arm_list else arm_list[0]
+    head = next((h for h in ("mixed_poly", "mixed_batched") if h in arm_list), arm_list[0])
     for arm in arm_list:
         if arm == "fp64_serial":
             step = lambda: solve_serial(systems, fp64)
         elif arm == "mixed_batched":
             step = lambda: solve_batched(ctx, admit, mixed, ws_bytes=ws, time_spmv=True,
+                                         assemble_ahead=a.assemble_ahead)
+        elif arm == "mixed_poly":
+            step = lambda: solve_batched(ctx, admit, poly, ws_bytes=ws, time_spmv=True,
                                          assemble_ahead=a.assemble_ahead)
         elif arm == "fp64_batched":
             step = lambda: solve_batched(ctx, admit, fp64_b, ws_bytes=ws, time_spmv=True,
@@ -415,10 +428,9 @@ def run_batch(a):
     if st_ and st_["achieved_GBps"]:
         roof = {"bound": "hbm", "achieved": round(st_["achieved_GBps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(st_["achieved_GBps"] / HBM_PEAK_GBPS, 4),
-                "traffic": (counter_traffic("kb_spmv_sell_float_batch512_counters.md", st_["bytes_total"],
-                                            st_["bytes_per_launch"])
-                            if head == "mixed_batched" and a.meshes == 512 else None),
-                "kernel": "kb_spmv_sell<float, float>" if head == "mixed_batched" else "kb_spmv_sell<double, double>",
+                "traffic": (counter_traffic(COUNTER_SUMMARY[head], st_["bytes_total"], st_["bytes_per_launch"])
+                            if head in COUNTER_SUMMARY and a.meshes == 512 else None),
+                "kernel": DOMINANT_KERNEL.get(head, "kb_spmv_sell<double, double>"),
                 "us_per_launch": round(st_["us_per_launch"], 2), "bytes_per_launch": round(st_["bytes_per_launch"]),
                 "launches": st_["launches"]}
     print(f"# roofline {json.dumps(roof)}", flush=True)
@@ -562,6 +574,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--poly-degree", type=int, default=0,
+                    help="mixed_poly: FP32 Chebyshev steps per outer iteration (0: the library's 15)")
     ap.add_argument("--sweep-groups", action="store_true", help="print SpMV time per lane-group size")
     ap.add_argument("--fp64-peak", action="store_true",
                     help="only measure the device's FP64 VALU and MFMA throughput and exit")

@@ -21,7 +21,7 @@ FX_ERR_NOT_CONVERGED = 5
 FX_ERR_BREAKDOWN = 6
 FX_ERR_NO_DEVICE = 7
 
-FX_PREC_FP64, FX_PREC_MIXED = 0, 1
+FX_PREC_FP64, FX_PREC_MIXED, FX_PREC_MIXED_POLY = 0, 1, 2
 FX_STOP_ABS_SQRT_RZ, FX_STOP_REL_R2 = 0, 1
 FX_LUT_MAX = 16
 
@@ -35,7 +35,7 @@ class fx_material_table(C.Structure):
 
 class fx_solve_opts(C.Structure):
     _fields_ = [("precision", i32), ("stop_mode", i32), ("tol", dbl), ("max_iter", i64), ("inner_tol", dbl),
-                ("max_refine", i32), ("spmv_group", i32)]
+                ("max_refine", i32), ("spmv_group", i32), ("poly_degree", i32)]
 
 
 class fx_solve_report(C.Structure):

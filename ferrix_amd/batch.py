@@ -109,7 +109,7 @@ def solve_serial(systems: List[System], opts: SolveOptions) -> PopulationResult:
 def operator_bytes(s: System, opts: SolveOptions) -> int:
     """Resident bytes one SpMV pass of this system streams: K values in the
     working precision plus one 4-byte column index per 3x3 block."""
-    vs = 4 if opts.precision == L.FX_PREC_MIXED else 8
+    vs = 4 if opts.precision in (L.FX_PREC_MIXED, L.FX_PREC_MIXED_POLY) else 8
     return s.dm.nnz_struct * vs + (s.dm.nnz_struct // 9) * 4
 
 

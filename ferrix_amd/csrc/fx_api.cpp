@@ -730,7 +730,9 @@ struct DevBc {
 };
 
 void check_solve_opts(const fx_solve_opts &o) {
-    require(o.precision == FX_PREC_FP64 || o.precision == FX_PREC_MIXED, "solve: unknown precision");
+    require(o.precision == FX_PREC_FP64 || o.precision == FX_PREC_MIXED,
+            o.precision == FX_PREC_MIXED_POLY ? "solve: FX_PREC_MIXED_POLY is a batch precision (fx_batch_create)"
+                                              : "solve: unknown precision");
     require(o.stop_mode == FX_STOP_ABS_SQRT_RZ || o.stop_mode == FX_STOP_REL_R2, "solve: unknown stop mode");
     require(o.max_iter >= 0, "solve: max_iter < 0");
     require(o.spmv_group == 0 || o.spmv_group == 2 || o.spmv_group == 4 || o.spmv_group == 8 ||
@@ -760,6 +762,7 @@ void fx_solve_opts_default(fx_solve_opts *o) {
     o->inner_tol = 1e-3;
     o->max_refine = 50;
     o->spmv_group = 0;
+    o->poly_degree = 0;
 }
 
 fx_status fx_assemble(fx_mesh *m, int32_t nlgeom, double *max_diag_out, int64_t *nnz_out) {

@@ -34,8 +34,7 @@
 // convergence rate; otherwise the solve restarts from p = z, as a restarted
 // refinement would.  Slots run independently,
 // and new slots join between chunks of iterations (continuous batching).
-#include "fx_reduce.h"
-#include "fx_sell.h"
+#include "fx_batch.h"
 
 #include <chrono>
 #include <cstdlib>
@@ -43,68 +42,6 @@
 #include <memory>
 
 namespace fx {
-
-constexpr int kRmax = 4;
-constexpr int kBB = 256;  // threads per block in the batch kernels
-// DOF tiles of kBB per block of the vector kernels (kb_init, kb_update):
-// four with FP32 vectors, so a block's fixed costs (descriptor reads, the
-// block reduction) are spread over enough bytes; one with FP64 vectors,
-// whose partial sums then associate exactly as fx_pcg.hip's.
-template <typename XT>
-constexpr int kTiles = sizeof(XT) == 4 ? 4 : 1;
-
-struct BRhs {
-    double rz[2], rr, bb, crit, pap, alpha, beta, tol;
-    double gamma;  // p_new = gamma z + beta p_old (1 except right after a replacement)
-    double rmax;   // mixed: true residual norm at the last replacement
-    double delta;  // mixed: replace when the iterated residual norm < delta * rmax (0: FP64 slot)
-    double prev;   // mixed: true criterion at the last replacement its own residual asked for (floor test)
-    int64_t it, it_k1, max_iter;
-    int32_t status;  // 0 running, 1 converged, 2 breakdown, 3 max_iter / replacement limit, 4 idle,
-                     // 5 residual replacement in this iteration
-    int32_t stop_mode;
-    int32_t nupd, max_upd;  // mixed: replacements done, allowed
-};
-
-// What the host needs of a slot once none of its load cases runs, written
-// by the GPU into pinned host memory (no copy command at chunk ends).
-struct BFin {
-    int64_t it, nupd;
-    double crit[kRmax];
-    int32_t status[kRmax];
-    int32_t done;  // written last
-    int32_t pad;
-};
-
-struct BSys {
-    const int32_t *nb_ptr, *nb_col;
-    const void *val;                    // double (FP64 slots) or float (mixed inner)
-    const double *val64, *inv;
-    const float *kc;                    // FP32 operator: per-node 3x3 row-sum corrections
-    const int32_t *sl_ptr, *scol;       // sliced-ELL operator (fx_sell.h) when sell != 0
-    const void *sval;
-    int32_t sell;
-    int32_t ssym;                       // mixed sliced-ELL slot on the symmetric FP32 operator
-    const int32_t *su_ptr, *scol_u, *sll_ptr, *lrec, *lcol;
-    const float *uval;
-    double *b;                          // FP64: right-hand sides; mixed: the last replaced residual b - K64 xacc
-    const double *btrue;                // right-hand sides
-    void *x, *r, *z, *p0, *p1, *ap;    // PCG vectors, XT (double, or float in the mixed solve)
-    double *xacc;                       // mixed: FP64 solution; x holds the increment since the last replacement
-    double *part1, *part2;
-    int64_t n_nodes, n_dof;
-    int64_t it;                         // iteration index of the running right-hand sides
-    int32_t R, nparts1, nparts2;
-    int32_t G;                          // lanes per node row of the lane-group SpMV of this system
-    double *bytes_acc;                  // instrumentation: algorithmic bytes of the SpMV launches issued
-    double spmv_bytes0, spmv_bytes_rhs; // per launch: operator part, per active right-hand side
-    double upd_bytes0, upd_bytes_rhs;   // the same for a residual replacement pass
-    int32_t go;                         // right-hand sides the next SpMV / update work on (bit mask)
-    int32_t upd;                        // mixed: right-hand sides whose residual is replaced in this iteration
-    int32_t trig;                       // mixed: those of them whose own residual asked for it
-    int32_t reported;                   // the slot's BFin has been written
-    BRhs rhs[kRmax];
-};
 
 namespace {
 
@@ -118,41 +55,6 @@ namespace {
 // partials.  The sums have exactly the association of fx_pcg.hip's
 // reduce_partials (thread-strided, then block_sum), so FP64 results are
 // bitwise those of fx_solve.
-
-// sums nv <= NV values over nparts blocks (layout part[b*nv + k]); valid in
-// thread 0; fixed order.  A row's NV loads are issued together: out[k] for
-// k >= nv (never read) sums the row's first value again, an in-bounds load
-// with no branch around it.
-template <int NV>
-__device__ __forceinline__ void bsum_partials(const double *part, int nparts, int nv, double (&out)[NV]) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-        out[k] = 0.0;
-    gptr<const double> pp = gp(part);
-    for (int b = threadIdx.x; b < nparts; b += kBB) {
-        double v[NV];
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            v[k] = pp[(size_t)b * nv + (k < nv ? k : 0)];
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            out[k] += v[k];
-    }
-    block_sum<NV>(out);
-}
-
-// The launch's view of a system's right-hand side states, copied to LDS by
-// one load per thread: the one-block reduction kernels take their decisions'
-// inputs from it instead of waiting on a load per right-hand side, and write
-// the new states to the global record (a field written there is not read
-// back in the same launch).
-__device__ __forceinline__ void stage_rhs(const BSys &S, BRhs (&H)[kRmax]) {
-    constexpr int kWords = kRmax * (int)sizeof(BRhs) / 4;
-    static_assert(sizeof(BRhs) % 4 == 0 && kWords <= kBB, "one word per thread");
-    if (threadIdx.x < kWords)
-        ((int32_t *)H)[threadIdx.x] = gp((const int32_t *)S.rhs)[threadIdx.x];
-    __syncthreads();
-}
 
 // Solve start for the right-hand sides with status 0:
 // x = 0, r = b, z = r inv, p0 = z (stored as XT); partials (r.z, r.r of the
@@ -879,8 +781,6 @@ __global__ __launch_bounds__(kBB) void kb_update(BSys *__restrict__ sys, const i
         }
 }
 
-inline int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
-
 template <typename VT>
 void launch_bspmv(int G, unsigned nb, hipStream_t s, BSys *sys, const int2 *tab) {
     switch (G) {
@@ -903,6 +803,7 @@ struct fx_batch {
     fx_ctx *ctx = nullptr;
     fx_solve_opts opts{};
     int G = 0;  // lane group forced by the options (0: each system's own)
+    int poly_deg = 0;  // polynomial batch: FP32 Chebyshev steps per outer iteration
     struct Slot {
         bool used = false;
         fx_mesh *mesh = nullptr;
@@ -919,6 +820,8 @@ struct fx_batch {
         double crit[fx::kRmax] = {0, 0, 0, 0};
         // device vectors from the context's pool: a later slot reuses them
         fx::PoolBuf<double> b, rin, x, r, z, p0, p1, ap, xacc, part1, part2;
+        fx::PoolBuf<float> zq0, zq1, zq2, rf;  // polynomial batch: Chebyshev iterates, FP32 residual
+        bool pending = false;                  // polynomial batch: admitted, its start sequence not yet run
         std::chrono::steady_clock::time_point t0;
         double seconds = 0.0;
     };
@@ -931,10 +834,10 @@ struct fx_batch {
     // block tables (device, and their pinned host staging), its own pinned
     // copy of the system table taken at its end, and its own events.
     struct ChunkSet {
-        fx::DBuf<int2> tab1g[kGroups], tab1s, tab2;
+        fx::DBuf<int2> tab1g[kGroups], tab1s, tab2, tab3;  // tab3: the polynomial batch's FP64 outer SpMV
         fx::DBuf<int32_t> run;
         int n1g[kGroups] = {0, 0, 0, 0, 0, 0};
-        int n1s = 0, n2 = 0, nrun = 0;
+        int n1s = 0, n2 = 0, n3 = 0, nrun = 0;
         int64_t version = -1;          // of the running-slot set these tables were built for
         int2 *h_tab = nullptr;         // pinned staging of the tables
         size_t h_tab_cap = 0;
@@ -1006,6 +909,9 @@ using fx::kBB;
 using fx::kRmax;
 
 bool is_mixed(const fx_batch *B) { return B->opts.precision == FX_PREC_MIXED; }
+bool is_poly(const fx_batch *B) { return B->opts.precision == FX_PREC_MIXED_POLY; }
+// FP32 operator (K32 and its row-sum correction) in the batch's SpMVs
+bool uses_fp32(const fx_batch *B) { return is_mixed(B) || is_poly(B); }
 
 // mixed batches on sliced-ELL meshes use the symmetric FP32 operator
 // (FX_SSYM=0: the full one)
@@ -1039,14 +945,14 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     fx_mesh *m = sl.mesh;
     S.nb_ptr = m->nb_ptr.p;
     S.nb_col = m->nb_col.p;
-    S.val = is_mixed(B) ? (const void *)m->kval32.p : (const void *)m->kval.p;
+    S.val = uses_fp32(B) ? (const void *)m->kval32.p : (const void *)m->kval.p;
     S.val64 = m->kval.p;
-    S.kc = is_mixed(B) ? m->kcorr32.p : nullptr;
+    S.kc = uses_fp32(B) ? m->kcorr32.p : nullptr;
     S.sell = m->sell ? 1 : 0;
     S.sl_ptr = m->sl_ptr.p;
     S.scol = m->scol.p;
-    S.sval = is_mixed(B) ? (const void *)m->sval32.p : (const void *)m->sval64.p;
-    S.ssym = is_mixed(B) && m->sell && m->ssym && use_ssym() ? 1 : 0;
+    S.sval = uses_fp32(B) ? (const void *)m->sval32.p : (const void *)m->sval64.p;
+    S.ssym = uses_fp32(B) && m->sell && m->ssym && use_ssym() ? 1 : 0;
     if (S.ssym) {
         S.su_ptr = m->su_ptr.p;
         S.scol_u = m->scol_u.p;
@@ -1078,8 +984,8 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     // correction per node); per right-hand side p gathered once and Ap
     // written, in the vector precision (FP32 in the mixed solve)
     S.bytes_acc = B->d_bytes.p;
-    S.spmv_bytes0 = (is_mixed(B) ? 4.0 : 8.0) * (double)m->nnz_struct + 4.0 * (double)m->nblk +
-                    4.0 * (double)(m->n_nodes + 1) + (is_mixed(B) ? 36.0 * (double)m->n_nodes : 0.0);
+    S.spmv_bytes0 = (uses_fp32(B) ? 4.0 : 8.0) * (double)m->nnz_struct + 4.0 * (double)m->nblk +
+                    4.0 * (double)(m->n_nodes + 1) + (uses_fp32(B) ? 36.0 * (double)m->n_nodes : 0.0);
     if (S.ssym)  // upper blocks once (values + column), lower blocks as (record, column), the correction
         S.spmv_bytes0 = 40.0 * (double)m->nblk_u + 8.0 * (double)m->nblk_l + 36.0 * (double)m->n_nodes +
                         16.0 * (double)((m->n_nodes + fx::kSliceRows - 1) / fx::kSliceRows + 1);
@@ -1091,6 +997,27 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
                    8.0 * (double)m->n_dof;
     S.upd_bytes_rhs = 28.0 * (double)m->n_dof;
     S.nparts2 = fx::nblk(m->n_dof, kBB * (is_mixed(B) ? fx::kTiles<float> : fx::kTiles<double>));
+    if (is_poly(B)) {
+        // FP64 x, r, direction pp, its image ps, w = K z; FP32 Chebyshev
+        // iterates and residual copy
+        S.poly = 1;
+        S.deg = B->poly_deg;
+        S.b = sl.b.p;
+        S.pp = sl.p0.p;
+        S.ps = sl.p1.p;
+        S.pw = sl.ap.p;
+        S.zq[0] = sl.zq0.p;
+        S.zq[1] = sl.zq1.p;
+        S.zq[2] = sl.zq2.p;
+        S.rf = sl.rf.p;
+        S.inv32 = m->inv_diag32.p;
+        S.nparts3 = fx::nblk(m->n_nodes, kBB / fx::kOuterG);
+        // one inner step: the FP32 operator as the reliable-update batch's
+        // SpMV counts it, the FP32 Jacobi scaling, and per load case z_j
+        // gathered once, z_{j-1} and rf read, z_{j+1} written
+        S.cheb_bytes0 = S.spmv_bytes0 + 4.0 * (double)m->n_dof;
+        S.cheb_bytes_rhs = 16.0 * (double)m->n_dof;
+    }
 }
 
 // workgroups are dispatched round-robin over the 8 XCDs: permute an SpMV
@@ -1131,11 +1058,11 @@ void ensure_pinned(int2 *&p, size_t &cap, size_t n) {
 // processed (its event waited), so neither buffer is still being read.
 void build_tables(fx_batch *B, int set) {
     fx_batch::ChunkSet &C = B->cs[set];
-    std::vector<int2> t1g[fx_batch::kGroups], t1s, t2;
+    std::vector<int2> t1g[fx_batch::kGroups], t1s, t2, t3;
     std::vector<int> run;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         auto &sl = *B->slots[k];
-        if (!sl.used || sl.phase != 0)
+        if (!sl.used || sl.phase != 0 || sl.pending)
             continue;
         run.push_back((int)k);
         const BSys &S = B->h_sys[k];
@@ -1143,8 +1070,11 @@ void build_tables(fx_batch *B, int set) {
             (S.sell ? t1s : t1g[group_index(S.G)]).push_back(make_int2((int)k, b));
         for (int b = 0; b < S.nparts2; ++b)
             t2.push_back(make_int2((int)k, b));
+        if (S.poly)
+            for (int b = 0; b < S.nparts3; ++b)
+                t3.push_back(make_int2((int)k, b));
     }
-    size_t total = t1s.size() + t2.size() + (run.size() + 1) / 2 + 1;
+    size_t total = t1s.size() + t2.size() + t3.size() + (run.size() + 1) / 2 + 1;
     for (auto &t : t1g)
         total += t.size();
     ensure_pinned(C.h_tab, C.h_tab_cap, total);
@@ -1165,6 +1095,8 @@ void build_tables(fx_batch *B, int set) {
     }
     C.n1s = (int)t1s.size();
     stage(C.tab1s, t1s);
+    C.n3 = (int)t3.size();
+    stage(C.tab3, t3);
     C.n2 = (int)t2.size();
     std::copy(t2.begin(), t2.end(), C.h_tab + off);  // vector blocks: no XCD permutation
     if (!t2.empty()) {
@@ -1180,7 +1112,7 @@ void build_tables(fx_batch *B, int set) {
         FX_HIP(hipMemcpyAsync(C.run.p, h, run.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     }
     C.version = B->run_version;
-    int64_t blocks = (int64_t)t1s.size() + (int64_t)t2.size();
+    int64_t blocks = (int64_t)t1s.size() + (int64_t)t2.size() + (int64_t)t3.size();
     for (auto &t : t1g)
         blocks += (int64_t)t.size();
     B->tab_blocks = blocks;
@@ -1223,6 +1155,59 @@ void release_failed_slot(fx_batch *B, int k) {
         B->h_sys[k] = BSys{};
 }
 
+// The start sequence of the polynomial batch's pending slots, all together:
+// the power iteration (lambda_max of D^-1 K32), the Chebyshev coefficients,
+// x = 0, r = b and the first stop test; their solves run from the next
+// chunk on.
+void start_pending_poly(fx_batch *B) {
+    std::vector<int> list;
+    for (size_t k = 0; k < B->slots.size(); ++k)
+        if (B->slots[k] && B->slots[k]->used && B->slots[k]->pending)
+            list.push_back((int)k);
+    if (list.empty())
+        return;
+    hipStream_t s = B->ctx->stream;
+    std::vector<int2> tv, ts, tg[fx_batch::kGroups];
+    for (int k : list) {
+        const BSys &S = B->h_sys[k];
+        for (int b = 0; b < S.nparts2; ++b)
+            tv.push_back(make_int2(k, b));
+        for (int b = 0; b < S.nparts1; ++b)
+            (S.sell ? ts : tg[group_index(S.G)]).push_back(make_int2(k, b));
+    }
+    xcd_permute(ts);
+    for (auto &t : tg)
+        xcd_permute(t);
+    // one staging array: vector table, sliced-ELL table, lane-group tables, slot list
+    std::vector<int2> all(tv);
+    all.insert(all.end(), ts.begin(), ts.end());
+    for (auto &t : tg)
+        all.insert(all.end(), t.begin(), t.end());
+    const size_t n_list2 = (list.size() + 1) / 2;
+    all.resize(all.size() + n_list2);
+    std::memcpy(all.data() + (all.size() - n_list2), list.data(), list.size() * sizeof(int));
+    B->d_tab4.alloc(std::max<size_t>(B->d_tab4.n, all.size()));
+    FX_HIP(hipMemcpyAsync(B->d_tab4.p, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    const int2 *base = B->d_tab4.p;
+    const int2 *p_tv = base, *p_ts = base + tv.size();
+    const int2 *p_tg[fx_batch::kGroups];
+    int n_tg[fx_batch::kGroups];
+    size_t off = tv.size() + ts.size();
+    for (int g = 0; g < fx_batch::kGroups; ++g) {
+        p_tg[g] = base + off;
+        n_tg[g] = (int)tg[g].size();
+        off += tg[g].size();
+    }
+    const int32_t *p_list = (const int32_t *)(base + off);
+    fx::launch_poly_power_start(s, B->d_sys.p, p_tv, (int)tv.size(), p_ts, (int)ts.size(), p_tg, n_tg, p_list,
+                                (int)list.size(), B->fin);
+    FX_HIP(hipGetLastError());
+    FX_HIP(hipStreamSynchronize(s));  // the staging array is reused
+    for (int k : list)
+        B->slots[k]->pending = false;
+    B->run_version++;
+}
+
 // kb_init over the vector blocks of `slots`, then kb_red2 on them: their
 // solves start with the next iteration chunk.
 void start_slots(fx_batch *B, const std::vector<int> &slots) {
@@ -1257,6 +1242,22 @@ void launch_iterations(fx_batch *B, int n, int set) {
         C.ev.resize(2 * n, nullptr);
         for (size_t k = old; k < C.ev.size(); ++k)
             FX_HIP(hipEventCreate(&C.ev[k]));
+    }
+    if (is_poly(B)) {  // n outer iterations, each poly_deg FP32 inner steps
+        const int2 *tg[fx_batch::kGroups];
+        for (int g = 0; g < fx_batch::kGroups; ++g)
+            tg[g] = C.tab1g[g].p;
+        for (int k = 0; k < n; ++k)
+            fx::launch_poly_outer(s, B->d_sys.p, B->poly_deg, C.tab2.p, C.n2, C.tab1s.p, C.n1s, tg, C.n1g, C.tab3.p,
+                                  C.n3, C.run.p, C.nrun, B->fin, B->timing ? C.ev[2 * k] : nullptr,
+                                  B->timing ? C.ev[2 * k + 1] : nullptr);
+        C.ev_used = B->timing ? 2 * n : 0;
+        B->spmv_launches += (int64_t)n * B->poly_deg;
+        FX_HIP(hipGetLastError());
+        if (!C.done)
+            FX_HIP(hipEventCreateWithFlags(&C.done, hipEventDisableTiming));
+        FX_HIP(hipEventRecord(C.done, s));
+        return;
     }
     for (int k = 0; k < n; ++k) {
         if (is_mixed(B))
@@ -1304,7 +1305,7 @@ void collect_sample(fx_batch *B, int set) {
         float ms = 0.f;
         FX_HIP(hipEventElapsedTime(&ms, C.ev[k], C.ev[k + 1]));
         B->timed_ms += ms;
-        B->timed++;
+        B->timed += is_poly(B) ? B->poly_deg : 1;  // a polynomial pair brackets the inner steps
     }
     C.ev_used = 0;
 }
@@ -1335,7 +1336,7 @@ void advance_phases(fx_batch *B, int set) {
         sl.phase = 2;
         sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
         const BSys &S = B->h_sys[k];
-        B->dead_blocks += S.nparts1 + S.nparts2;
+        B->dead_blocks += S.nparts1 + S.nparts2 + (S.poly ? S.nparts3 : 0);
     }
     if (4 * B->dead_blocks > B->tab_blocks)
         B->run_version++;
@@ -1389,10 +1390,20 @@ fx_status fx_batch_create(fx_ctx *ctx, const fx_solve_opts *opts, fx_batch **out
             B->opts = *opts;
         else
             fx_solve_opts_default(&B->opts);
-        if (B->opts.precision != FX_PREC_FP64 && B->opts.precision != FX_PREC_MIXED) {
+        if (B->opts.precision != FX_PREC_FP64 && B->opts.precision != FX_PREC_MIXED &&
+            B->opts.precision != FX_PREC_MIXED_POLY) {
             delete B;
             fx::set_error("fx_batch_create: unknown precision");
             return FX_ERR_INVALID;
+        }
+        if (B->opts.precision == FX_PREC_MIXED_POLY) {
+            B->poly_deg = B->opts.poly_degree > 0 ? B->opts.poly_degree : 15;
+            if (B->poly_deg > fx::kPolyMax) {
+                delete B;
+                fx::set_error("fx_batch_create: poly_degree must be 0 (default 15) or 1.." +
+                              std::to_string(fx::kPolyMax));
+                return FX_ERR_INVALID;
+            }
         }
         if (B->opts.precision == FX_PREC_MIXED &&
             !(B->opts.inner_tol > 0.0 && B->opts.inner_tol < 1.0 && B->opts.max_refine >= 1)) {
@@ -1489,22 +1500,34 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
                 inter[i * n_rhs + r] = rhs[(int64_t)r * n + i];
         sl.b.upload(ctx, inter, s);
         for (auto *v : {&sl.x, &sl.r, &sl.z, &sl.p0, &sl.p1, &sl.ap})
-            v->alloc(ctx, is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed solve stores FP32 vectors
+            if (v != &sl.z || !is_poly(B))
+                v->alloc(ctx, is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed solve stores FP32 vectors
+        if (is_poly(B)) {
+            for (auto *v : {&sl.zq0, &sl.zq1, &sl.zq2, &sl.rf})
+                v->alloc(ctx, nv);
+            fx::launch_convert32(m, s);
+            m->inv_diag32.alloc(n);
+        }
         if (is_mixed(B)) {
             sl.rin.alloc(ctx, nv);  // replaced residuals: written by each replacement pass before use
             sl.xacc.alloc(ctx, nv);
             fx::launch_fill(m, sl.xacc.p, nv, 0.0, s);
             fx::launch_convert32(m, s);  // K may have been reassembled since the last conversion
         }
-        if (is_mixed(B) && m->sell && m->ssym && use_ssym())
+        if (uses_fp32(B) && m->sell && m->ssym && use_ssym())
             fx::launch_ssym(m, s);
         else
-            fx::launch_sell(m, is_mixed(B), s);
-        fx::launch_jacobi(m, false, s);
+            fx::launch_sell(m, uses_fp32(B), s);
+        fx::launch_jacobi(m, is_poly(B), s);
         sl.G = G;
         // >= the SpMV's row groups (1 partial per load case) and the
-        // replacement pass's (3 per load case)
-        sl.part1.alloc(ctx, (size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 4);
+        // replacement pass's (3 per load case); polynomial batch: the power
+        // iteration's (2 per row group) and the outer SpMV's (5 per load case)
+        size_t np1 = (size_t)std::max(fx::nblk(m->n_nodes, kBB / G), 1) * n_rhs * 4;
+        if (is_poly(B))
+            np1 = std::max<size_t>(2 * (size_t)std::max(fx::nblk(m->n_nodes, m->sell ? kBB : kBB / G), 1),
+                                   5 * (size_t)n_rhs * (size_t)std::max(fx::nblk(m->n_nodes, kBB / fx::kOuterG), 1));
+        sl.part1.alloc(ctx, np1);
         sl.part2.alloc(ctx, (size_t)std::max(fx::nblk(n, kBB), 1) * n_rhs * 3);
         sl.used = true;
         sl.phase = 0;
@@ -1520,7 +1543,10 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         reserve_sys(B, B->h_sys.size());
         std::memset(&B->fin[k], 0, sizeof(fx::BFin));
         FX_HIP(hipMemcpyAsync(B->d_sys.p + k, &S, sizeof(BSys), hipMemcpyHostToDevice, s));
-        start_slots(B, std::vector<int>{k});  // start this slot's solve now
+        if (is_poly(B))
+            sl.pending = true;  // started with the other pending slots by the next fx_batch_run
+        else
+            start_slots(B, std::vector<int>{k});  // start this slot's solve now
         B->run_version++;
         if (m->ctx != ctx) {
             sl.foreign = true;
@@ -1545,6 +1571,8 @@ fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
     fx_ctx *ctx = B->ctx;
     try {
         FX_HIP(hipSetDevice(ctx->device));
+        if (is_poly(B))
+            start_pending_poly(B);
         int64_t done_iters = 0;
         int inflight = 0, head = 0;  // chunks launched and not yet processed; the oldest one's set
         for (;;) {
@@ -1556,11 +1584,14 @@ fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
                 const int set = (head + inflight) % 2;
                 if (B->cs[set].version != B->run_version)
                     build_tables(B, set);
-                int chunk = 16;
+                // a chunk: 16 iterations, or (polynomial batch) the outer
+                // iterations holding about 16 FP32 inner steps, at least one
+                const int per = is_poly(B) ? B->poly_deg : 1;
+                int chunk = std::max(1, 16 / per);
                 if (max_iters > 0)
-                    chunk = (int)std::min<int64_t>(chunk, max_iters - done_iters);
+                    chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(1, (max_iters - done_iters) / per));
                 launch_iterations(B, chunk, set);
-                done_iters += chunk;
+                done_iters += (int64_t)chunk * per;
                 ++inflight;
                 continue;
             }

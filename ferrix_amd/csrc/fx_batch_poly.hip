@@ -1,0 +1,677 @@
+// fx_batch_poly.hip -- the polynomial-preconditioned mixed-precision batch
+// (FX_PREC_MIXED_POLY).
+//
+// Per load case: flexible CG in FP64 (FCG(1): each new direction is made
+// K-orthogonal to the previous one; Notay, "Flexible conjugate gradients",
+// SIAM J. Sci. Comput. 22, 2000) whose preconditioner is `deg` steps of the
+// Chebyshev iteration for K z = r with the Jacobi scaling (Saad, "Iterative
+// Methods for Sparse Linear Systems", 2nd ed., Alg. 12.1, in its three-term
+// form), run in FP32 on the FP32 operator (fl32(K) plus its per-node
+// row-sum correction, the operator of the reliable-update batch).  The outer
+// iteration -- w = K64 z, every dot product, x, r and the stop rule -- is
+// FP64; r is the FP64 recursively updated residual, as in the FP64 PCG.
+//
+// Why: on these penalised elasticity systems eps32 * cond(K) is not small,
+// so FP32 Krylov recurrences (the reliable-update batch) lose 2-4x in
+// iterations.  A Chebyshev polynomial needs no inner products: its FP32
+// rounding only perturbs the preconditioner, which the flexible outer
+// iteration absorbs, and each inner step is one launch (SpMV and update
+// fused) that streams the FP32 operator.  The outer FP64 iterations are
+// about deg+1 times fewer than FP64 PCG iterations.
+//
+// Chebyshev interval [lo, hi] of D^-1 K32 (D the Jacobi diagonal):
+// hi = 1.1 x the Rayleigh quotient after kPowerSteps power steps from a fixed
+// start vector (within 2-4% below lambda_max on the batched population's
+// meshes, so hi stays above it), lo = 1e-3 hi.  Eigenvalues below lo are
+// damped less, the outer iteration resolves them.
+//
+// Sequences (every slot of a launch at the same stage, each slot's
+// arithmetic independent of the others):
+//   start:  kp_pinit, kPowerSteps x kp_power, kp_lambda, kp_init, kp_stop
+//   outer:  kp_cheb0, deg x kp_cheb, kp_outer, kp_coef, kp_upd, kp_stop
+#include "fx_batch.h"
+
+namespace fx {
+namespace {
+
+constexpr double kPolyMargin = 1.1;     // hi = margin x Rayleigh quotient
+constexpr double kPolyLoRatio = 1e-3;   // lo = ratio x hi
+constexpr float kPowerScale = 0.25f;    // keeps power iterates in FP32 range
+
+// start vector of the power iteration: a fixed hash of the DOF index in [-1, 1)
+__device__ __forceinline__ float power_v0(int64_t i) {
+    uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull;
+    h ^= (uint64_t)i >> 7;
+    h = (h * 0xBF58476D1CE4E5B9ull) & 0xFFFFFFFFull;
+    return (float)((double)h / 2147483648.0 - 1.0);
+}
+
+// acc[p][r] += (K32 q)(3a+p, r) over the node row's blocks, sliced-ELL
+// operator (the symmetric one when the slot has it), one thread per row.
+template <int RR>
+__device__ __forceinline__ void sell_apply32(const BSys &S, int64_t a, gptr<const float> pv, double (&acc)[3][RR]) {
+    if (S.ssym) {
+        ssym_lower_rows<float, RR>(gp(S.sll_ptr), gp(S.lrec), gp(S.lcol), gp(S.uval), a, pv, acc);
+        sell_node_rows<float, float, RR>(gp(S.su_ptr), gp(S.scol_u), gp(S.uval), a, pv, acc);
+    } else {
+        sell_node_rows<float, float, RR>(gp(S.sl_ptr), gp(S.scol), gp((const float *)S.sval), a, pv, acc);
+    }
+}
+
+// The same on the node-block FP32 values, G lanes per row (column slots
+// j = lane, lane + G, ...), reduced over the lanes.
+template <int G, int RR>
+__device__ __forceinline__ void lanes_apply32(const BSys &S, int64_t a, int lane, gptr<const float> pv,
+                                              double (&acc)[3][RR]) {
+    if (a < S.n_nodes) {
+        gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
+        const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
+        gptr<const float> v0 = gp((const float *)S.val) + 9 * (int64_t)b0;
+        for (int j = lane; j < ncol; j += G) {
+            const int kb = j / 3;
+            const int64_t c = (3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb)) * RR;
+            const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const double xj = (double)pv[c + r];
+                acc[0][r] += w0 * xj;
+                acc[1][r] += w1 * xj;
+                acc[2][r] += w2 * xj;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+                acc[p][r] += __shfl_xor(acc[p][r], off, G);
+}
+
+// own-row operand values and the FP32 operator's row-sum correction block
+template <int RR>
+__device__ __forceinline__ void own_and_correction(const BSys &S, int64_t a, gptr<const float> pv,
+                                                   double (&q)[3][RR], double (&acc)[3][RR]) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+            q[p][r] = (double)pv[(3 * a + p) * RR + r];
+    gptr<const float> c = gp(S.kc) + 9 * a;
+    float cc[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+        cc[e] = c[e];
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+            acc[p][r] += ((double)cc[3 * p] * q[0][r] + (double)cc[3 * p + 1] * q[1][r]) + (double)cc[3 * p + 2] * q[2][r];
+}
+
+// ---------------------------------------------------------------------------
+// power iteration (RR = 1 layout in zq[0], zq[1]): v_{j+1} = 0.25 inv (K32 v_j);
+// after the last step the partials v.(K32 v) and v.(D v) -> part1 (2 per row group)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void power_row_tail(const BSys &S, int64_t a, int step, bool last, gptr<const float> pv,
+                                               double (&acc)[3][1], double (&loc)[2]) {
+    double q[3][1];
+    own_and_correction<1>(S, a, pv, q, acc);
+    if (last) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const int64_t d = 3 * a + p;
+            loc[0] += q[p][0] * acc[p][0];
+            loc[1] += q[p][0] * (q[p][0] / gp(S.inv)[d]);
+        }
+    } else {
+        gptr<float> vn = gp(S.zq[(step + 1) & 1]);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const int64_t d = 3 * a + p;
+            vn[d] = kPowerScale * gp(S.inv32)[d] * (float)acc[p][0];
+        }
+    }
+}
+
+__device__ __forceinline__ void write_power_partials(const BSys &S, int lb, bool last, double (&loc)[2]) {
+    if (!last)
+        return;
+    block_sum<2>(loc);
+    if (threadIdx.x == 0) {
+        S.part1[2 * (size_t)lb] = loc[0];
+        S.part1[2 * (size_t)lb + 1] = loc[1];
+    }
+}
+
+__global__ __launch_bounds__(kBB) void kp_power_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab, int step,
+                                                     int last) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    gptr<const float> pv = gp((const float *)S.zq[step & 1]);
+    const int64_t a = (int64_t)t.y * kBB + threadIdx.x;
+    double loc[2] = {0.0, 0.0};
+    if (a < S.n_nodes) {
+        double acc[3][1] = {{0.0}, {0.0}, {0.0}};
+        sell_apply32<1>(S, a, pv, acc);
+        power_row_tail(S, a, step, last, pv, acc, loc);
+    }
+    write_power_partials(S, t.y, last, loc);
+}
+
+template <int G>
+__global__ __launch_bounds__(kBB) void kp_power_lanes(BSys *__restrict__ sys, const int2 *__restrict__ tab, int step,
+                                                      int last) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    gptr<const float> pv = gp((const float *)S.zq[step & 1]);
+    const int64_t a = (int64_t)t.y * (kBB / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    double loc[2] = {0.0, 0.0};
+    double acc[3][1] = {{0.0}, {0.0}, {0.0}};
+    lanes_apply32<G, 1>(S, a, lane, pv, acc);
+    if (lane == 0 && a < S.n_nodes)
+        power_row_tail(S, a, step, last, pv, acc, loc);
+    write_power_partials(S, t.y, last, loc);
+}
+
+// start vector of the power iteration (own DOFs, vector tiles)
+__global__ __launch_bounds__(kBB) void kp_pinit(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int64_t i = (int64_t)t.y * kBB + threadIdx.x;
+    if (i < S.n_dof)
+        gp(S.zq[0])[i] = power_v0(i);
+}
+
+// One block per slot: lambda_max estimate, the Chebyshev coefficients of the
+// three-term recurrence z_{j+1} = z_j + c1_j (z_j - z_{j-1}) + c2_j D^-1 (r - K z_j),
+// z_0 = 0, z_1 = D^-1 r / theta (Saad Alg. 12.1: rho_0 = 1/sigma,
+// rho_j = 1/(2 sigma - rho_{j-1}), c1_j = rho_j rho_{j-1}, c2_j = 2 rho_j / delta).
+__global__ __launch_bounds__(kBB) void kp_lambda(BSys *__restrict__ sys, const int32_t *__restrict__ list) {
+    BSys &S = sys[list[blockIdx.x]];
+    double red[2];
+    bsum_partials<2>(S.part1, S.nparts1, 2, red);
+    if (threadIdx.x != 0)
+        return;
+    const double lam = red[1] > 0.0 ? red[0] / red[1] : 1.0;
+    const double hi = kPolyMargin * (lam > 0.0 && isfinite(lam) ? lam : 1.0), lo = kPolyLoRatio * hi;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    S.lam = lam;
+    S.cheb[0] = 1.0 / theta;
+    double rho = 1.0 / sigma;
+    for (int j = 1; j <= S.deg; ++j) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        S.cheb[2 * j] = rn * rho;
+        S.cheb[2 * j + 1] = 2.0 * rn / delta;
+        rho = rn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// outer iteration
+// ---------------------------------------------------------------------------
+
+// x = 0, r = b, pp = ps = 0 for the load cases with status 0; partials
+// (r.r, r.(D^-1 r), b.b) -> part2 (3 per load case)
+__global__ __launch_bounds__(kBB) void kp_init(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int R = S.R;
+    double v[3 * kRmax];
+#pragma unroll
+    for (int k = 0; k < 3 * kRmax; ++k)
+        v[k] = 0.0;
+    const int64_t i = (int64_t)t.y * kBB + threadIdx.x;
+    if (i < S.n_dof) {
+        const double inv = gp(S.inv)[i];
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r) {
+            if (r >= R || S.rhs[r].status != 0)
+                continue;
+            const int64_t q = i * R + r;
+            const double bi = gp(S.btrue)[q];
+            gp((double *)S.x)[q] = 0.0;
+            gp((double *)S.r)[q] = bi;
+            gp(S.pp)[q] = 0.0;
+            gp(S.ps)[q] = 0.0;
+            v[3 * r] += bi * bi;
+            v[3 * r + 1] += bi * (bi * inv);
+            v[3 * r + 2] += bi * bi;
+        }
+    }
+    block_sum<3 * kRmax>(v);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < R; ++r)
+            for (int k = 0; k < 3; ++k)
+                S.part2[((size_t)t.y * R + r) * 3 + k] = v[3 * r + k];
+}
+
+// rf = fl32(r), z_0 = 0, z_1 = fl32(D^-1 r / theta) for the running load cases
+__global__ __launch_bounds__(kBB) void kp_cheb0(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    const int64_t i = (int64_t)t.y * kBB + threadIdx.x;
+    if (i >= S.n_dof)
+        return;
+    const int R = S.R;
+    const double inv = gp(S.inv)[i], ith = S.cheb[0];
+    for (int r = 0; r < R; ++r) {
+        if (!((go >> r) & 1))
+            continue;
+        const int64_t q = i * R + r;
+        const double ri = gp((const double *)S.r)[q];
+        gp(S.rf)[q] = (float)ri;
+        gp(S.zq[0])[q] = 0.f;
+        gp(S.zq[1])[q] = (float)(inv * ri * ith);
+    }
+}
+
+// own-row update of inner step j: z_{j+1} = z_j + c1 (z_j - z_{j-1}) + c2 D^-1 (rf - K32 z_j)
+template <int RR>
+__device__ __forceinline__ void cheb_row_tail(const BSys &S, int64_t a, int j, int go, gptr<const float> pv,
+                                              double (&acc)[3][RR]) {
+    double q[3][RR];
+    own_and_correction<RR>(S, a, pv, q, acc);
+    const double c1 = S.cheb[2 * j], c2 = S.cheb[2 * j + 1];
+    gptr<const float> zm = gp((const float *)S.zq[(j + 2) % 3]);  // z_{j-1}
+    gptr<float> zn = gp(S.zq[(j + 1) % 3]);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const int64_t d = 3 * a + p;
+        const double inv = (double)gp(S.inv32)[d];
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            if (!((go >> r) & 1))
+                continue;
+            const int64_t qd = d * RR + r;
+            const double zj = q[p][r];
+            const double res = (double)gp((const float *)S.rf)[qd] - acc[p][r];
+            zn[qd] = (float)(zj + c1 * (zj - (double)zm[qd]) + c2 * (inv * res));
+        }
+    }
+}
+
+template <int RR>
+__device__ __forceinline__ void cheb_sell(const BSys &S, int lb, int j, int go) {
+    const int64_t a = (int64_t)lb * kBB + threadIdx.x;
+    if (a >= S.n_nodes)
+        return;
+    gptr<const float> pv = gp((const float *)S.zq[j % 3]);
+    double acc[3][RR];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+            acc[p][r] = 0.0;
+    sell_apply32<RR>(S, a, pv, acc);
+    cheb_row_tail<RR>(S, a, j, go, pv, acc);
+}
+
+__global__ __launch_bounds__(kBB) void kp_cheb_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    switch (S.R) {
+    case 1: cheb_sell<1>(S, t.y, j, go); break;
+    case 2: cheb_sell<2>(S, t.y, j, go); break;
+    case 3: cheb_sell<3>(S, t.y, j, go); break;
+    default: cheb_sell<4>(S, t.y, j, go); break;
+    }
+}
+
+template <int G, int RR>
+__device__ __forceinline__ void cheb_lanes(const BSys &S, int lb, int j, int go) {
+    const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    gptr<const float> pv = gp((const float *)S.zq[j % 3]);
+    double acc[3][RR];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+            acc[p][r] = 0.0;
+    lanes_apply32<G, RR>(S, a, lane, pv, acc);
+    if (lane == 0 && a < S.n_nodes)
+        cheb_row_tail<RR>(S, a, j, go, pv, acc);
+}
+
+template <int G>
+__global__ __launch_bounds__(kBB) void kp_cheb_lanes(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    switch (S.R) {
+    case 1: cheb_lanes<G, 1>(S, t.y, j, go); break;
+    case 2: cheb_lanes<G, 2>(S, t.y, j, go); break;
+    case 3: cheb_lanes<G, 3>(S, t.y, j, go); break;
+    default: cheb_lanes<G, 4>(S, t.y, j, go); break;
+    }
+}
+
+// FP64 outer SpMV w = K64 z (z = zq[(deg+1) % 3], FP32) on the node-block
+// values, kOuterG lanes per row; partials per load case (z.s, z.r, p.r,
+// z.w, p.w) with p, s the previous direction and its image -> part1.
+template <int RR>
+__device__ __forceinline__ void outer_row(const BSys &S, int64_t a, int lane, int go, double (&v)[5 * kRmax]) {
+    constexpr int G = kOuterG;
+    gptr<const float> zv = gp((const float *)S.zq[(S.deg + 1) % 3]);
+    double acc[3][RR];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+            acc[p][r] = 0.0;
+    if (a < S.n_nodes) {
+        gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
+        const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
+        gptr<const double> v0 = gp(S.val64) + 9 * (int64_t)b0;
+        for (int j = lane; j < ncol; j += G) {
+            const int kb = j / 3;
+            const int64_t c = (3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb)) * RR;
+            const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const double xj = (double)zv[c + r];
+                acc[0][r] += w0 * xj;
+                acc[1][r] += w1 * xj;
+                acc[2][r] += w2 * xj;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+                acc[p][r] += __shfl_xor(acc[p][r], off, G);
+    if (lane != 0 || a >= S.n_nodes)
+        return;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const int64_t d = 3 * a + p;
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            if (!((go >> r) & 1))
+                continue;
+            const int64_t q = d * RR + r;
+            const double z = (double)zv[q], w = acc[p][r];
+            const double so = gp((const double *)S.ps)[q], po = gp((const double *)S.pp)[q];
+            const double rr = gp((const double *)S.r)[q];
+            gp(S.pw)[q] = w;
+            v[5 * r] += z * so;
+            v[5 * r + 1] += z * rr;
+            v[5 * r + 2] += po * rr;
+            v[5 * r + 3] += z * w;
+            v[5 * r + 4] += po * w;
+        }
+    }
+}
+
+template <int RR>
+__device__ __forceinline__ void outer_block(const BSys &S, int lb, int go) {
+    double v[5 * kRmax];
+#pragma unroll
+    for (int k = 0; k < 5 * kRmax; ++k)
+        v[k] = 0.0;
+    outer_row<RR>(S, (int64_t)lb * (kBB / kOuterG) + threadIdx.x / kOuterG, threadIdx.x % kOuterG, go, v);
+    block_sum<5 * kRmax>(v);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < RR; ++r)
+            for (int k = 0; k < 5; ++k)
+                S.part1[((size_t)lb * RR + r) * 5 + k] = v[5 * r + k];
+}
+
+__global__ __launch_bounds__(kBB) void kp_outer(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    switch (S.R) {
+    case 1: outer_block<1>(S, t.y, go); break;
+    case 2: outer_block<2>(S, t.y, go); break;
+    case 3: outer_block<3>(S, t.y, go); break;
+    default: outer_block<4>(S, t.y, go); break;
+    }
+}
+
+// One block per slot: beta (FCG(1): the new direction p = z - beta p_old is
+// K-orthogonal to p_old), p.s by expansion, alpha = (p.r)/(p.s); a p.s that
+// is not positive (or not finite) is a breakdown.
+__global__ __launch_bounds__(kBB) void kp_coef(BSys *__restrict__ sys, const int32_t *__restrict__ list) {
+    BSys &S = sys[list[blockIdx.x]];
+    const int go = S.go;
+    if (!go)
+        return;
+    __shared__ BRhs H[kRmax];
+    stage_rhs(S, H);
+    const int R = S.R;
+    double red[5 * kRmax];
+    bsum_partials<5 * kRmax>(S.part1, S.nparts3, 5 * R, red);
+    if (threadIdx.x != 0)
+        return;
+    int go2 = 0;
+#pragma unroll
+    for (int r = 0; r < kRmax; ++r) {
+        if (r >= R || !((go >> r) & 1))
+            continue;
+        const BRhs &c = H[r];
+        BRhs &h = S.rhs[r];
+        const double zs = red[5 * r], zr = red[5 * r + 1], pr = red[5 * r + 2], zw = red[5 * r + 3],
+                     pw = red[5 * r + 4];
+        const double beta = c.it == 0 ? 0.0 : zs / c.pap;
+        const double prn = zr - beta * pr;
+        const double psn = ((zw - beta * zs) - beta * pw) + beta * beta * c.pap;
+        const double al = prn / psn;
+        h.beta = beta;
+        h.alpha = al;
+        h.pap = psn;
+        if (!(psn > 0.0) || !isfinite(al))
+            h.status = 2;
+        else
+            go2 |= 1 << r;
+    }
+    S.go = go2;
+}
+
+// p = z - beta p_old, s = w - beta s_old, x += alpha p, r -= alpha s;
+// partials (r.r, r.(D^-1 r), 0) -> part2
+template <int RR>
+__device__ __forceinline__ void upd_tile(const BSys &S, int lb, int go, double (&v)[3 * kRmax]) {
+    const int64_t i = (int64_t)lb * kBB + threadIdx.x;
+    if (i >= S.n_dof)
+        return;
+    const double inv = gp(S.inv)[i];
+    gptr<const float> zv = gp((const float *)S.zq[(S.deg + 1) % 3]);
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        if (!((go >> r) & 1))
+            continue;
+        const double al = S.rhs[r].alpha, be = S.rhs[r].beta;
+        const int64_t q = i * RR + r;
+        const double pn = (double)zv[q] - be * gp((const double *)S.pp)[q];
+        const double sn = gp((const double *)S.pw)[q] - be * gp((const double *)S.ps)[q];
+        const double xn = gp((const double *)S.x)[q] + al * pn;
+        const double rn = gp((const double *)S.r)[q] - al * sn;
+        gp(S.pp)[q] = pn;
+        gp(S.ps)[q] = sn;
+        gp((double *)S.x)[q] = xn;
+        gp((double *)S.r)[q] = rn;
+        v[3 * r] += rn * rn;
+        v[3 * r + 1] += rn * (rn * inv);
+    }
+}
+
+__global__ __launch_bounds__(kBB) void kp_upd(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    double v[3 * kRmax];
+#pragma unroll
+    for (int k = 0; k < 3 * kRmax; ++k)
+        v[k] = 0.0;
+    switch (S.R) {
+    case 1: upd_tile<1>(S, t.y, go, v); break;
+    case 2: upd_tile<2>(S, t.y, go, v); break;
+    case 3: upd_tile<3>(S, t.y, go, v); break;
+    default: upd_tile<4>(S, t.y, go, v); break;
+    }
+    block_sum<3 * kRmax>(v);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < S.R; ++r)
+            for (int k = 0; k < 3; ++k)
+                S.part2[((size_t)t.y * S.R + r) * 3 + k] = v[3 * r + k];
+}
+
+// One block per slot, after kp_init (after_update 0) or kp_upd: the stop rule
+// per load case (as fx_pcg.hip's: sqrt(r.z) < tol, or ||r||/||b|| <= tol),
+// the outer iteration limit, and the mask of load cases the next outer
+// iteration works on.  A slot with no load case left running writes its
+// finish summary for the host (iterations: FP32 inner SpMVs; nupd: outer
+// iterations).
+__global__ __launch_bounds__(kBB) void kp_stop(BSys *__restrict__ sys, const int32_t *__restrict__ list,
+                                               BFin *__restrict__ fin, int after_update) {
+    const int slot = list[blockIdx.x];
+    BSys &S = sys[slot];
+    __shared__ BRhs H[kRmax];
+    stage_rhs(S, H);
+    const int R = S.R;
+    bool any = false;
+    for (int r = 0; r < R; ++r)
+        any |= H[r].status == 0;
+    if (any) {  // block-uniform: every thread read the same staged states
+        double red[3 * kRmax];
+        bsum_partials<3 * kRmax>(S.part2, S.nparts2, 3 * R, red);
+        if (threadIdx.x != 0)
+            return;
+        int go = 0;
+#pragma unroll
+        for (int r = 0; r < kRmax; ++r) {
+            const BRhs &c = H[r];
+            BRhs &h = S.rhs[r];
+            if (r >= R || c.status != 0)
+                continue;
+            const int64_t it = after_update ? c.it + 1 : c.it;
+            const double rr = red[3 * r], rzt = red[3 * r + 1];
+            const double bb = after_update ? c.bb : red[3 * r + 2];
+            double crit;
+            bool conv;
+            if (c.stop_mode == FX_STOP_ABS_SQRT_RZ) {
+                crit = sqrt(rzt);
+                conv = crit < c.tol;
+            } else {
+                crit = bb > 0.0 ? sqrt(rr) / sqrt(bb) : 0.0;
+                conv = crit <= c.tol;
+            }
+            h.it = it;
+            h.rr = rr;
+            h.crit = crit;
+            if (!after_update)
+                h.bb = bb;
+            if (conv)
+                h.status = 1;
+            else if (it * (int64_t)S.deg >= c.max_iter || !isfinite(crit))
+                h.status = 3;
+            else
+                go |= 1 << r;
+        }
+        S.go = go;
+        if (go) {  // the inner steps of the next outer iteration (integer-valued sums: exact in any order)
+            atomicAdd(S.bytes_acc, (double)S.deg * (S.cheb_bytes0 + (double)__popc(go) * S.cheb_bytes_rhs));
+            return;
+        }
+    } else if (threadIdx.x != 0) {
+        return;
+    }
+    if (!S.reported) {  // thread 0, no load case runs any more: tell the host
+        BFin &F = fin[slot];
+        int64_t it = 0;
+        for (int r = 0; r < kRmax; ++r) {
+            const BRhs &h = S.rhs[r];  // written by this thread above or in earlier launches
+            F.crit[r] = h.crit;
+            F.status[r] = h.status;
+            if (r < R)
+                it = max(it, h.it);
+        }
+        F.it = it * (int64_t)S.deg;
+        F.nupd = it;
+        __threadfence_system();
+        F.done = 1;
+        S.reported = 1;
+    }
+}
+
+template <typename K>
+void launch_inner_tables(hipStream_t s, BSys *sys, const int2 *tab_sell, int n_sell, const int2 *const *tab_g,
+                         const int *n_g, K &&launch) {
+    if (n_sell)
+        launch(0, tab_sell, n_sell);
+    for (int g = 0; g < 6; ++g)
+        if (n_g[g])
+            launch(2 << g, tab_g[g], n_g[g]);
+}
+
+}  // namespace
+
+void launch_poly_power_start(hipStream_t s, BSys *sys, const int2 *tab_vec, int n_vec, const int2 *tab_in_sell,
+                             int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int32_t *list,
+                             int n_list, BFin *fin) {
+    hipLaunchKernelGGL(kp_pinit, dim3(n_vec), dim3(kBB), 0, s, sys, tab_vec);
+    for (int step = 0; step <= kPowerSteps; ++step) {
+        const int last = step == kPowerSteps;
+        launch_inner_tables(s, sys, tab_in_sell, n_in_sell, tab_in_g, n_in_g, [&](int G, const int2 *tab, int n) {
+            switch (G) {
+            case 0: hipLaunchKernelGGL(kp_power_sell, dim3(n), dim3(kBB), 0, s, sys, tab, step, last); break;
+            case 2: hipLaunchKernelGGL(kp_power_lanes<2>, dim3(n), dim3(kBB), 0, s, sys, tab, step, last); break;
+            case 4: hipLaunchKernelGGL(kp_power_lanes<4>, dim3(n), dim3(kBB), 0, s, sys, tab, step, last); break;
+            case 8: hipLaunchKernelGGL(kp_power_lanes<8>, dim3(n), dim3(kBB), 0, s, sys, tab, step, last); break;
+            case 16: hipLaunchKernelGGL(kp_power_lanes<16>, dim3(n), dim3(kBB), 0, s, sys, tab, step, last); break;
+            case 32: hipLaunchKernelGGL(kp_power_lanes<32>, dim3(n), dim3(kBB), 0, s, sys, tab, step, last); break;
+            default: hipLaunchKernelGGL(kp_power_lanes<64>, dim3(n), dim3(kBB), 0, s, sys, tab, step, last); break;
+            }
+        });
+    }
+    hipLaunchKernelGGL(kp_lambda, dim3(n_list), dim3(kBB), 0, s, sys, list);
+    hipLaunchKernelGGL(kp_init, dim3(n_vec), dim3(kBB), 0, s, sys, tab_vec);
+    hipLaunchKernelGGL(kp_stop, dim3(n_list), dim3(kBB), 0, s, sys, list, fin, 0);
+}
+
+void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, int n_vec, const int2 *tab_in_sell,
+                       int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int2 *tab_out, int n_out,
+                       const int32_t *list, int n_list, BFin *fin, hipEvent_t ev0, hipEvent_t ev1) {
+    hipLaunchKernelGGL(kp_cheb0, dim3(n_vec), dim3(kBB), 0, s, sys, tab_vec);
+    if (ev0)
+        FX_HIP(hipEventRecord(ev0, s));
+    for (int j = 1; j <= deg; ++j)
+        launch_inner_tables(s, sys, tab_in_sell, n_in_sell, tab_in_g, n_in_g, [&](int G, const int2 *tab, int n) {
+            switch (G) {
+            case 0: hipLaunchKernelGGL(kp_cheb_sell, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
+            case 2: hipLaunchKernelGGL(kp_cheb_lanes<2>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
+            case 4: hipLaunchKernelGGL(kp_cheb_lanes<4>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
+            case 8: hipLaunchKernelGGL(kp_cheb_lanes<8>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
+            case 16: hipLaunchKernelGGL(kp_cheb_lanes<16>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
+            case 32: hipLaunchKernelGGL(kp_cheb_lanes<32>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
+            default: hipLaunchKernelGGL(kp_cheb_lanes<64>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
+            }
+        });
+    if (ev1)
+        FX_HIP(hipEventRecord(ev1, s));
+    hipLaunchKernelGGL(kp_outer, dim3(n_out), dim3(kBB), 0, s, sys, tab_out);
+    hipLaunchKernelGGL(kp_coef, dim3(n_list), dim3(kBB), 0, s, sys, list);
+    hipLaunchKernelGGL(kp_upd, dim3(n_vec), dim3(kBB), 0, s, sys, tab_vec);
+    hipLaunchKernelGGL(kp_stop, dim3(n_list), dim3(kBB), 0, s, sys, list, fin, 1);
+}
+
+}  // namespace fx

@@ -150,12 +150,20 @@ fx_status fx_apply_penalty(fx_mesh *mesh, int64_t n_bc, const int64_t *dofs, con
 /* ---- solver seam ------------------------------------------------------ */
 typedef enum fx_precision {
     FX_PREC_FP64 = 0,  /* Jacobi-PCG in FP64 (the reference's IterativeSolver)   */
-    FX_PREC_MIXED = 1  /* FP32-stored K, FP64 residuals and norms; PCG with
+    FX_PREC_MIXED = 1, /* FP32-stored K, FP64 residuals and norms; PCG with
                           reliable updates: the residual is replaced by the
                           FP64 b - K x whenever the iterated one has fallen
                           by inner_tol, keeping the search direction if the
                           two agree within 10%.  fx_solve keeps FP64 Krylov
                           vectors, a batch FP32 ones (dots in FP64) */
+    FX_PREC_MIXED_POLY = 2 /* batches only: flexible CG in FP64 (x, r, the
+                          directions, K64 z, every dot product, the stop
+                          rule on the FP64 residual) preconditioned by
+                          poly_degree Chebyshev-Jacobi steps in FP32 on the
+                          FP32-stored K; the Chebyshev interval comes from a
+                          power iteration per system when it joins.
+                          iterations in the report count FP32 inner SpMVs,
+                          outer_spmv the FP64 outer iterations */
 } fx_precision;
 
 typedef enum fx_stop {
@@ -174,6 +182,9 @@ typedef struct fx_solve_opts {
     int32_t spmv_group; /* lanes per node row in the SpMV (0: the mesh's tuned
                            group, fx_mesh_autotune_spmv, else a row-length
                            heuristic) */
+    int32_t poly_degree; /* MIXED_POLY: FP32 Chebyshev steps per outer
+                            iteration, 1..32 (0: 15); max_iter bounds the
+                            FP32 steps of a load case */
 } fx_solve_opts;
 
 typedef struct fx_solve_report {

@@ -1,0 +1,189 @@
+"""Polynomial-preconditioned mixed-precision batches (FX_PREC_MIXED_POLY):
+flexible CG in FP64 with FP32 Chebyshev-Jacobi inner steps on the FP32
+operator.
+
+Bars:
+* every load case within 1e-8 relative of the direct solve of the oracle's
+  penalised matrix (solved to ||b - K x||/||b|| <= 1e-10);
+* bit for bit alone vs in a mix, whatever was admitted before or later, and
+  whatever the run chunking (a slot's arithmetic never depends on its
+  neighbours or on the host's pacing);
+* both FP32 operators: the sliced-ELL one (C3D4 meshes; its symmetric
+  storage) and the node-block lane-group one (C3D20 meshes, or C3D4
+  uploaded with FX_SELL=0);
+* the reference's stop rule (sqrt(r.z) < 1e-8 on the FP64 residual) and the
+  not-converged error of the reference's solver.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from ferrix_amd import _lib as L
+from ferrix_amd import meshgen
+from ferrix_amd.device import Batch, DeviceMesh, SolveOptions
+from ferrix_amd.inp import ConvergenceError
+
+from test_gpu_batch import population, small_specs, system
+
+pytestmark = pytest.mark.gpu
+
+POLY = SolveOptions(precision=L.FX_PREC_MIXED_POLY, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def oracle_errors(jobs, x, b):
+    csr, _, _ = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
+    out = []
+    for r in range(b.shape[0]):
+        xd = O.direct_solve(csr, b[r])
+        out.append(float(np.linalg.norm(x[r] - xd) / np.linalg.norm(xd)))
+    return out
+
+
+def solve_alone(ctx, sys, opts):
+    out = []
+    for dm, b in sys:
+        bt = Batch(ctx, opts)
+        s = bt.add(dm, b)
+        bt.run()
+        out.append((bt.result(s), bt.poll(s)[1]))
+        bt.close()
+    return out
+
+
+def test_poly_batch_alone_vs_mix_and_oracle(ctx):
+    specs = small_specs(5, seed=31)
+    sys = population(ctx, specs)
+    solo = solve_alone(ctx, sys, POLY)
+    bt = Batch(ctx, POLY)
+    slots = [bt.add(dm, b) for dm, b in sys[:2]]
+    bt.run(max_iters=40)  # the first two run a while before the others join
+    slots += [bt.add(dm, b) for dm, b in sys[2:]]
+    bt.run()
+    for k, (s, spec) in enumerate(zip(slots, specs)):
+        st, rep = bt.poll(s)
+        assert st == 1
+        x = bt.get(s)
+        assert np.array_equal(x, solo[k][0]), f"system {k}: mix differs from alone"
+        assert rep.iterations == solo[k][1].iterations
+        errs = oracle_errors(meshgen.cfg4_mesh(spec), x, sys[k][1])
+        say("system", k, "n_dof", spec["n_dof"], "FP32 steps", rep.iterations, "outer", rep.outer_spmv,
+            "rel err", max(errs))
+        assert max(errs) < 1e-8
+        assert rep.outer_spmv * 15 == rep.iterations
+
+
+@pytest.mark.parametrize("sell", [True, False])
+def test_poly_fp32_operators(ctx, sell):
+    """The sliced-ELL FP32 operator (symmetric storage) and the node-block
+    lane-group one (FX_SELL=0) each meet the bar; their summation orders
+    differ, so their results agree to the tolerance, not bitwise."""
+    specs = small_specs(3, seed=5)
+    sys = population(ctx, specs, sell=sell)
+    bt = Batch(ctx, POLY)
+    slots = [bt.add(dm, b) for dm, b in sys]
+    bt.run()
+    for k, (s, spec) in enumerate(zip(slots, specs)):
+        assert bt.poll(s)[0] == 1
+        errs = oracle_errors(meshgen.cfg4_mesh(spec), bt.result(s), sys[k][1])
+        say("sell", sell, "system", k, "rel err", max(errs))
+        assert max(errs) < 1e-8
+
+
+def test_poly_c3d20_block(ctx):
+    """A jittered 10x4x3 C3D20 block (node-block operator, 27 blocks per node
+    row) beside two C3D4 systems in one batch."""
+    job = meshgen.cfg2_c3d20_block(seed=7, nx=10, ny=4, nz=3)
+    dm20, b20 = system(ctx, [job])
+    specs = small_specs(2, seed=8)
+    sys = population(ctx, specs)
+    bt = Batch(ctx, POLY)
+    s20 = bt.add(dm20, b20)
+    slots = [bt.add(dm, b) for dm, b in sys]
+    bt.run()
+    assert bt.poll(s20)[0] == 1
+    errs = oracle_errors([job], bt.result(s20), b20)
+    say("c3d20 rel err", errs, "report", bt.poll(s20)[1].iterations)
+    assert max(errs) < 1e-8
+    for k, s in enumerate(slots):
+        assert max(oracle_errors(meshgen.cfg4_mesh(specs[k]), bt.result(s), sys[k][1])) < 1e-8
+
+
+@pytest.mark.parametrize("degree", [1, 8, 32])
+def test_poly_degrees(ctx, degree):
+    specs = small_specs(2, seed=13)
+    sys = population(ctx, specs)
+    opts = SolveOptions(precision=L.FX_PREC_MIXED_POLY, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=200_000,
+                        poly_degree=degree)
+    bt = Batch(ctx, opts)
+    slots = [bt.add(dm, b) for dm, b in sys]
+    bt.run()
+    for k, s in enumerate(slots):
+        st, rep = bt.poll(s)
+        assert st == 1 and rep.iterations == degree * rep.outer_spmv
+        assert max(oracle_errors(meshgen.cfg4_mesh(specs[k]), bt.result(s), sys[k][1])) < 1e-8
+
+
+def test_poly_chunking_does_not_change_results(ctx):
+    sys = population(ctx, small_specs(4, seed=17))
+    out = []
+    for step in (0, 15, 100):
+        bt = Batch(ctx, POLY)
+        slots = [bt.add(dm, b) for dm, b in sys]
+        calls = 0
+        while bt.run(max_iters=step) > 0:
+            calls += 1
+            assert calls < 100_000
+        out.append([(bt.result(s), bt.poll(s)[1].iterations) for s in slots])
+        bt.close()
+    for k in range(len(sys)):
+        for other in out[1:]:
+            assert np.array_equal(out[0][k][0], other[k][0]), f"system {k}"
+            assert out[0][k][1] == other[k][1]
+
+
+def test_poly_reference_stop_rule(ctx):
+    """sqrt(r.z) < 1e-8 with z = D^-1 r on the FP64 residual (iterative.rs:63)."""
+    specs = small_specs(2, seed=23)
+    sys = population(ctx, specs)
+    opts = SolveOptions(precision=L.FX_PREC_MIXED_POLY, stop_mode=L.FX_STOP_ABS_SQRT_RZ, tol=1e-8, max_iter=100_000)
+    bt = Batch(ctx, opts)
+    slots = [bt.add(dm, b) for dm, b in sys]
+    bt.run()
+    for k, s in enumerate(slots):
+        st, rep = bt.poll(s)
+        assert st == 1 and rep.final_criterion < 1e-8
+        dm, b = sys[k]
+        x = bt.result(s)
+        jobs = meshgen.cfg4_mesh(specs[k])
+        csr, _, _ = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
+        inv = O.jacobi(csr)
+        for r in range(b.shape[0]):
+            res = b[r] - O.spmv(csr, x[r])
+            assert np.sqrt(res @ (inv * res)) < 2e-8  # the true residual, to rounding
+
+
+def test_poly_not_converged(ctx):
+    sys = population(ctx, small_specs(2, seed=3))
+    bt = Batch(ctx, SolveOptions(precision=L.FX_PREC_MIXED_POLY, stop_mode=L.FX_STOP_REL_R2, tol=1e-10,
+                                 max_iter=30))
+    slots = [bt.add(dm, b) for dm, b in sys]
+    bt.run()
+    for s in slots:
+        st, rep = bt.poll(s)
+        assert st == -L.FX_ERR_NOT_CONVERGED
+        assert rep.iterations == 30
+        with pytest.raises(ConvergenceError):
+            bt.result(s)
+
+
+def test_poly_single_solve_is_rejected(ctx):
+    job = meshgen.cfg1_c3d4_cantilever()
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.assemble(False)
+    from ferrix_amd.inp import InvalidModelState
+    with pytest.raises(InvalidModelState, match="batch precision"):
+        dm.solve(np.ones(job.n_dof), POLY)
