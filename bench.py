@@ -575,7 +575,7 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--poly-degree", type=int, default=0,
-                    help="mixed_poly: FP32 Chebyshev steps per outer iteration (0: the library's 15)")
+                    help="mixed_poly: Chebyshev steps per outer iteration (0: the library default, 15)")
     ap.add_argument("--sweep-groups", action="store_true", help="print SpMV time per lane-group size")
     ap.add_argument("--fp64-peak", action="store_true",
                     help="only measure the device's FP64 VALU and MFMA throughput and exit")

@@ -174,6 +174,68 @@ __global__ void k_convert32(int64_t n_dof, const int32_t *__restrict__ nb_ptr, c
     corr[9 * node + 3 * p + 2] = (float)c2;
 }
 
+// The Jacobi-scaled operator in FP16: fl16(s_i K_ij s_j) with s = sqrt(inv_diag)
+// (1/sqrt(K_ii), or 1 where the Jacobi guard applies), so its diagonal is 1
+// and its entries lie in [-1, 1].  Node-block layout (one thread per DOF row)
+// for the lane-group operator ...
+__global__ void k_scaled16(int64_t n_dof, const int32_t *__restrict__ nb_ptr, const int32_t *__restrict__ nb_col,
+                           const double *__restrict__ a, const double *__restrict__ inv, _Float16 *__restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_dof)
+        return;
+    const int64_t node = i / 3;
+    const int p = (int)(i % 3);
+    const int64_t b0 = nb_ptr[node];
+    const int deg = nb_ptr[node + 1] - (int)b0;
+    const int64_t base = 9 * b0 + (int64_t)p * 3 * deg;
+    const double si = sqrt(inv[i]);
+    for (int k = 0; k < deg; ++k) {
+        const int64_t c = 3 * (int64_t)nb_col[b0 + k];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            b[base + 3 * k + q] = (_Float16)(si * a[base + 3 * k + q] * sqrt(inv[c + q]));
+    }
+}
+
+// ... and in the sliced-ELL arrangement of fx_sell.h with FP16 records: per
+// block column kk and lane, entries e < 8 of the 3x3 block as one 16-byte
+// record at rec8[(kk * 64 + lane) * 8 + e] and entry 8 at rec1[kk * 64 + lane]
+// (two loads per block instead of nine); one 64-thread block per slice,
+// padding blocks are zeros.
+__global__ void k_sell_scaled16(int64_t n_nodes, const int32_t *__restrict__ nb_ptr,
+                                const int32_t *__restrict__ nb_col, const int32_t *__restrict__ sl_ptr,
+                                const double *__restrict__ a, const double *__restrict__ inv,
+                                _Float16 *__restrict__ rec8, _Float16 *__restrict__ rec1) {
+    const int64_t sl = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t node = sl * kSliceRows + lane;
+    const int k0 = sl_ptr[sl], w = sl_ptr[sl + 1] - k0;
+    int64_t b0 = 0;
+    int deg = 0;
+    if (node < n_nodes) {
+        b0 = nb_ptr[node];
+        deg = nb_ptr[node + 1] - (int)b0;
+    }
+    for (int k = 0; k < w; ++k) {
+        const size_t kk = (size_t)k0 + k;
+        const int64_t c = k < deg ? 3 * (int64_t)nb_col[b0 + k] : 0;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                _Float16 v = (_Float16)0.0f;
+                if (k < deg)
+                    v = (_Float16)(sqrt(inv[3 * node + p]) * a[9 * b0 + (int64_t)p * 3 * deg + 3 * k + q] *
+                                   sqrt(inv[c + q]));
+                const int e = 3 * p + q;
+                if (e < 8)
+                    rec8[(kk * kSliceRows + lane) * 8 + e] = v;
+                else
+                    rec1[kk * kSliceRows + lane] = v;
+            }
+    }
+}
+
 // Sliced-ELL copy (fx_sell.h) of the node-block values: one 64-thread block
 // per slice, one thread per node row; padding is written as zeros.
 template <typename ST, typename VT>
@@ -404,6 +466,23 @@ void launch_sell(fx_mesh *m, bool fp32, hipStream_t s) {
         m->sval64.alloc(slots);
         hipLaunchKernelGGL((k_sell_convert<double, double>), dim3((unsigned)ns), dim3(kSliceRows), 0, s,
                            m->n_nodes, m->nb_ptr.p, m->sl_ptr.p, m->kval.p, m->sval64.p);
+    }
+    FX_HIP(hipGetLastError());
+}
+
+void launch_scaled16(fx_mesh *m, hipStream_t s) {
+    if (!s)
+        s = m->ctx->stream;
+    if (m->sell) {
+        const int64_t ns = (m->n_nodes + kSliceRows - 1) / kSliceRows;
+        m->sval16.alloc((size_t)m->scol.n * 9);  // 8-entry records, then the 9th entries
+        hipLaunchKernelGGL(k_sell_scaled16, dim3((unsigned)ns), dim3(kSliceRows), 0, s, m->n_nodes, m->nb_ptr.p,
+                           m->nb_col.p, m->sl_ptr.p, m->kval.p, m->inv_diag.p, m->sval16.p,
+                           m->sval16.p + (size_t)m->scol.n * 8);
+    } else {
+        m->kval16.alloc(m->nnz_struct);
+        hipLaunchKernelGGL(k_scaled16, dim3(grid1(m->n_dof, 256)), dim3(256), 0, s, m->n_dof, m->nb_ptr.p,
+                           m->nb_col.p, m->kval.p, m->inv_diag.p, m->kval16.p);
     }
     FX_HIP(hipGetLastError());
 }

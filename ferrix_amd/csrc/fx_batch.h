@@ -75,13 +75,16 @@ struct BSys {
     int32_t reported;                   // the slot's BFin has been written
     // polynomial-preconditioned mixed batch (FX_PREC_MIXED_POLY): flexible
     // CG in FP64 on x, r (FP64 in x and r), the direction pp, ps = K pp and
-    // pw = K z; z from `deg` Chebyshev-Jacobi steps in FP32 on the FP32
-    // operator, with three rotating iterates zq and the FP32 copy rf of r
+    // pw = K z; z from `deg` Chebyshev steps on the Jacobi-scaled FP16
+    // operator (val16: node blocks, sval16: sliced ELL) with FP64 iterates:
+    // three rotating iterates zq and the scaled residual rs
     int32_t poly, deg;
     double *pp, *ps, *pw;
-    float *zq[3], *rf;
-    const float *inv32;
-    int32_t nparts3;                    // row groups of the FP64 outer SpMV (kBB / kOuterG node rows each)
+    double *zq[3], *rs;
+    const _Float16 *val16, *sval16, *sval16b;  // sliced ELL: 8-entry records, then the 9th entries
+    const double *sval64;               // sliced-ELL FP64 operator (sell slots): the outer SpMV's
+    int32_t nparts3;                    // row groups of the FP64 outer SpMV (sliced ELL: kBB node rows;
+                                        // node blocks: kBB / kOuterG)
     int32_t pad3;
     double lam;                         // Rayleigh-quotient estimate of lambda_max(D^-1 K32)
     double cheb[2 * kPolyMax + 2];      // [0] = 1/theta; per inner step j >= 1: c1_j at 2j, c2_j at 2j+1
@@ -96,8 +99,9 @@ void launch_poly_power_start(hipStream_t s, BSys *sys, const int2 *tab_vec, int 
                              int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int32_t *list,
                              int n_list, BFin *fin);
 void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, int n_vec, const int2 *tab_in_sell,
-                       int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int2 *tab_out, int n_out,
-                       const int32_t *list, int n_list, BFin *fin, hipEvent_t ev0, hipEvent_t ev1);
+                       int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int2 *tab_out_sell,
+                       int n_out_sell, const int2 *tab_out, int n_out, const int32_t *list, int n_list, BFin *fin,
+                       hipEvent_t ev0, hipEvent_t ev1);
 
 inline int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
 

@@ -820,7 +820,7 @@ struct fx_batch {
         double crit[fx::kRmax] = {0, 0, 0, 0};
         // device vectors from the context's pool: a later slot reuses them
         fx::PoolBuf<double> b, rin, x, r, z, p0, p1, ap, xacc, part1, part2;
-        fx::PoolBuf<float> zq0, zq1, zq2, rf;  // polynomial batch: Chebyshev iterates, FP32 residual
+        fx::PoolBuf<double> zq0, zq1, zq2, rs;  // polynomial batch: Chebyshev iterates, scaled residual
         bool pending = false;                  // polynomial batch: admitted, its start sequence not yet run
         std::chrono::steady_clock::time_point t0;
         double seconds = 0.0;
@@ -834,10 +834,11 @@ struct fx_batch {
     // block tables (device, and their pinned host staging), its own pinned
     // copy of the system table taken at its end, and its own events.
     struct ChunkSet {
-        fx::DBuf<int2> tab1g[kGroups], tab1s, tab2, tab3;  // tab3: the polynomial batch's FP64 outer SpMV
+        // tab3s / tab3: the polynomial batch's FP64 outer SpMV (sliced-ELL and node-block operators)
+        fx::DBuf<int2> tab1g[kGroups], tab1s, tab2, tab3s, tab3;
         fx::DBuf<int32_t> run;
         int n1g[kGroups] = {0, 0, 0, 0, 0, 0};
-        int n1s = 0, n2 = 0, n3 = 0, nrun = 0;
+        int n1s = 0, n2 = 0, n3s = 0, n3 = 0, nrun = 0;
         int64_t version = -1;          // of the running-slot set these tables were built for
         int2 *h_tab = nullptr;         // pinned staging of the tables
         size_t h_tab_cap = 0;
@@ -910,8 +911,6 @@ using fx::kRmax;
 
 bool is_mixed(const fx_batch *B) { return B->opts.precision == FX_PREC_MIXED; }
 bool is_poly(const fx_batch *B) { return B->opts.precision == FX_PREC_MIXED_POLY; }
-// FP32 operator (K32 and its row-sum correction) in the batch's SpMVs
-bool uses_fp32(const fx_batch *B) { return is_mixed(B) || is_poly(B); }
 
 // mixed batches on sliced-ELL meshes use the symmetric FP32 operator
 // (FX_SSYM=0: the full one)
@@ -945,14 +944,14 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     fx_mesh *m = sl.mesh;
     S.nb_ptr = m->nb_ptr.p;
     S.nb_col = m->nb_col.p;
-    S.val = uses_fp32(B) ? (const void *)m->kval32.p : (const void *)m->kval.p;
+    S.val = is_mixed(B) ? (const void *)m->kval32.p : (const void *)m->kval.p;
     S.val64 = m->kval.p;
-    S.kc = uses_fp32(B) ? m->kcorr32.p : nullptr;
+    S.kc = is_mixed(B) ? m->kcorr32.p : nullptr;
     S.sell = m->sell ? 1 : 0;
     S.sl_ptr = m->sl_ptr.p;
     S.scol = m->scol.p;
-    S.sval = uses_fp32(B) ? (const void *)m->sval32.p : (const void *)m->sval64.p;
-    S.ssym = uses_fp32(B) && m->sell && m->ssym && use_ssym() ? 1 : 0;
+    S.sval = is_mixed(B) ? (const void *)m->sval32.p : (const void *)m->sval64.p;
+    S.ssym = is_mixed(B) && m->sell && m->ssym && use_ssym() ? 1 : 0;
     if (S.ssym) {
         S.su_ptr = m->su_ptr.p;
         S.scol_u = m->scol_u.p;
@@ -984,8 +983,8 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
     // correction per node); per right-hand side p gathered once and Ap
     // written, in the vector precision (FP32 in the mixed solve)
     S.bytes_acc = B->d_bytes.p;
-    S.spmv_bytes0 = (uses_fp32(B) ? 4.0 : 8.0) * (double)m->nnz_struct + 4.0 * (double)m->nblk +
-                    4.0 * (double)(m->n_nodes + 1) + (uses_fp32(B) ? 36.0 * (double)m->n_nodes : 0.0);
+    S.spmv_bytes0 = (is_mixed(B) ? 4.0 : 8.0) * (double)m->nnz_struct + 4.0 * (double)m->nblk +
+                    4.0 * (double)(m->n_nodes + 1) + (is_mixed(B) ? 36.0 * (double)m->n_nodes : 0.0);
     if (S.ssym)  // upper blocks once (values + column), lower blocks as (record, column), the correction
         S.spmv_bytes0 = 40.0 * (double)m->nblk_u + 8.0 * (double)m->nblk_l + 36.0 * (double)m->n_nodes +
                         16.0 * (double)((m->n_nodes + fx::kSliceRows - 1) / fx::kSliceRows + 1);
@@ -1009,14 +1008,17 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
         S.zq[0] = sl.zq0.p;
         S.zq[1] = sl.zq1.p;
         S.zq[2] = sl.zq2.p;
-        S.rf = sl.rf.p;
-        S.inv32 = m->inv_diag32.p;
-        S.nparts3 = fx::nblk(m->n_nodes, kBB / fx::kOuterG);
-        // one inner step: the FP32 operator as the reliable-update batch's
-        // SpMV counts it, the FP32 Jacobi scaling, and per load case z_j
-        // gathered once, z_{j-1} and rf read, z_{j+1} written
-        S.cheb_bytes0 = S.spmv_bytes0 + 4.0 * (double)m->n_dof;
-        S.cheb_bytes_rhs = 16.0 * (double)m->n_dof;
+        S.rs = sl.rs.p;
+        S.val16 = m->kval16.p;
+        S.sval16 = m->sval16.p;
+        S.sval16b = m->sval16.p ? m->sval16.p + (size_t)m->scol.n * 8 : nullptr;
+        S.sval64 = m->sval64.p;
+        S.nparts3 = m->sell ? S.nparts1 : fx::nblk(m->n_nodes, kBB / fx::kOuterG);
+        // one inner step: the FP16 scaled operator (values, one column index
+        // per 3x3 block, node-row pointers), and per load case the FP64 z_j
+        // gathered once, z_{j-1} and rs read, z_{j+1} written
+        S.cheb_bytes0 = 2.0 * (double)m->nnz_struct + 4.0 * (double)m->nblk + 4.0 * (double)(m->n_nodes + 1);
+        S.cheb_bytes_rhs = 32.0 * (double)m->n_dof;
     }
 }
 
@@ -1058,7 +1060,7 @@ void ensure_pinned(int2 *&p, size_t &cap, size_t n) {
 // processed (its event waited), so neither buffer is still being read.
 void build_tables(fx_batch *B, int set) {
     fx_batch::ChunkSet &C = B->cs[set];
-    std::vector<int2> t1g[fx_batch::kGroups], t1s, t2, t3;
+    std::vector<int2> t1g[fx_batch::kGroups], t1s, t2, t3s, t3;
     std::vector<int> run;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         auto &sl = *B->slots[k];
@@ -1072,9 +1074,9 @@ void build_tables(fx_batch *B, int set) {
             t2.push_back(make_int2((int)k, b));
         if (S.poly)
             for (int b = 0; b < S.nparts3; ++b)
-                t3.push_back(make_int2((int)k, b));
+                (S.sell ? t3s : t3).push_back(make_int2((int)k, b));
     }
-    size_t total = t1s.size() + t2.size() + t3.size() + (run.size() + 1) / 2 + 1;
+    size_t total = t1s.size() + t2.size() + t3s.size() + t3.size() + (run.size() + 1) / 2 + 1;
     for (auto &t : t1g)
         total += t.size();
     ensure_pinned(C.h_tab, C.h_tab_cap, total);
@@ -1095,6 +1097,8 @@ void build_tables(fx_batch *B, int set) {
     }
     C.n1s = (int)t1s.size();
     stage(C.tab1s, t1s);
+    C.n3s = (int)t3s.size();
+    stage(C.tab3s, t3s);
     C.n3 = (int)t3.size();
     stage(C.tab3, t3);
     C.n2 = (int)t2.size();
@@ -1112,7 +1116,7 @@ void build_tables(fx_batch *B, int set) {
         FX_HIP(hipMemcpyAsync(C.run.p, h, run.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     }
     C.version = B->run_version;
-    int64_t blocks = (int64_t)t1s.size() + (int64_t)t2.size() + (int64_t)t3.size();
+    int64_t blocks = (int64_t)t1s.size() + (int64_t)t2.size() + (int64_t)t3s.size() + (int64_t)t3.size();
     for (auto &t : t1g)
         blocks += (int64_t)t.size();
     B->tab_blocks = blocks;
@@ -1248,9 +1252,9 @@ void launch_iterations(fx_batch *B, int n, int set) {
         for (int g = 0; g < fx_batch::kGroups; ++g)
             tg[g] = C.tab1g[g].p;
         for (int k = 0; k < n; ++k)
-            fx::launch_poly_outer(s, B->d_sys.p, B->poly_deg, C.tab2.p, C.n2, C.tab1s.p, C.n1s, tg, C.n1g, C.tab3.p,
-                                  C.n3, C.run.p, C.nrun, B->fin, B->timing ? C.ev[2 * k] : nullptr,
-                                  B->timing ? C.ev[2 * k + 1] : nullptr);
+            fx::launch_poly_outer(s, B->d_sys.p, B->poly_deg, C.tab2.p, C.n2, C.tab1s.p, C.n1s, tg, C.n1g,
+                                  C.tab3s.p, C.n3s, C.tab3.p, C.n3, C.run.p, C.nrun, B->fin,
+                                  B->timing ? C.ev[2 * k] : nullptr, B->timing ? C.ev[2 * k + 1] : nullptr);
         C.ev_used = B->timing ? 2 * n : 0;
         B->spmv_launches += (int64_t)n * B->poly_deg;
         FX_HIP(hipGetLastError());
@@ -1502,23 +1506,28 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         for (auto *v : {&sl.x, &sl.r, &sl.z, &sl.p0, &sl.p1, &sl.ap})
             if (v != &sl.z || !is_poly(B))
                 v->alloc(ctx, is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed solve stores FP32 vectors
-        if (is_poly(B)) {
-            for (auto *v : {&sl.zq0, &sl.zq1, &sl.zq2, &sl.rf})
+        if (is_poly(B))
+            for (auto *v : {&sl.zq0, &sl.zq1, &sl.zq2, &sl.rs})
                 v->alloc(ctx, nv);
-            fx::launch_convert32(m, s);
-            m->inv_diag32.alloc(n);
-        }
         if (is_mixed(B)) {
             sl.rin.alloc(ctx, nv);  // replaced residuals: written by each replacement pass before use
             sl.xacc.alloc(ctx, nv);
             fx::launch_fill(m, sl.xacc.p, nv, 0.0, s);
             fx::launch_convert32(m, s);  // K may have been reassembled since the last conversion
         }
-        if (uses_fp32(B) && m->sell && m->ssym && use_ssym())
-            fx::launch_ssym(m, s);
-        else
-            fx::launch_sell(m, uses_fp32(B), s);
-        fx::launch_jacobi(m, is_poly(B), s);
+        if (is_poly(B)) {
+            // the FP64 outer SpMV's sliced-ELL operator, the Jacobi scaling
+            // and the inner steps' scaled FP16 operator built from it
+            fx::launch_sell(m, false, s);
+            fx::launch_jacobi(m, false, s);
+            fx::launch_scaled16(m, s);
+        } else {
+            if (is_mixed(B) && m->sell && m->ssym && use_ssym())
+                fx::launch_ssym(m, s);
+            else
+                fx::launch_sell(m, is_mixed(B), s);
+            fx::launch_jacobi(m, false, s);
+        }
         sl.G = G;
         // >= the SpMV's row groups (1 partial per load case) and the
         // replacement pass's (3 per load case); polynomial batch: the power

@@ -4,31 +4,36 @@
 // Per load case: flexible CG in FP64 (FCG(1): each new direction is made
 // K-orthogonal to the previous one; Notay, "Flexible conjugate gradients",
 // SIAM J. Sci. Comput. 22, 2000) whose preconditioner is `deg` steps of the
-// Chebyshev iteration for K z = r with the Jacobi scaling (Saad, "Iterative
-// Methods for Sparse Linear Systems", 2nd ed., Alg. 12.1, in its three-term
-// form), run in FP32 on the FP32 operator (fl32(K) plus its per-node
-// row-sum correction, the operator of the reliable-update batch).  The outer
-// iteration -- w = K64 z, every dot product, x, r and the stop rule -- is
-// FP64; r is the FP64 recursively updated residual, as in the FP64 PCG.
+// Chebyshev iteration (Saad, "Iterative Methods for Sparse Linear Systems",
+// 2nd ed., Alg. 12.1, in its three-term form) on the Jacobi-scaled operator
+// A = S K S, S = diag(sqrt(inv_diag)), stored in FP16: z = S p(A) S r.  The
+// outer iteration -- w = K64 z, every dot product, x, r and the stop rule --
+// is FP64; r is the FP64 recursively updated residual, as in the FP64 PCG.
 //
-// Why: on these penalised elasticity systems eps32 * cond(K) is not small,
-// so FP32 Krylov recurrences (the reliable-update batch) lose 2-4x in
-// iterations.  A Chebyshev polynomial needs no inner products: its FP32
+// Precision split, measured on the batched population's hardest members: the
+// rounding of the Chebyshev *iterates* decides the preconditioner's quality
+// (FP32 iterates cost 1.6x the inner steps of exact ones), the operator's
+// does not (FP16 and FP64 operators take the same steps).  So the operator
+// the inner steps stream is FP16 (2 bytes a value, a quarter of FP64) and
+// the iterates FP64.  A Chebyshev polynomial needs no inner products: each
+// inner step is one launch with the SpMV and the update fused, and its
 // rounding only perturbs the preconditioner, which the flexible outer
-// iteration absorbs, and each inner step is one launch (SpMV and update
-// fused) that streams the FP32 operator.  The outer FP64 iterations are
-// about deg+1 times fewer than FP64 PCG iterations.
+// iteration absorbs.  The outer FP64 iterations are about deg times fewer
+// than FP64 PCG iterations.
 //
-// Chebyshev interval [lo, hi] of D^-1 K32 (D the Jacobi diagonal):
-// hi = 1.1 x the Rayleigh quotient after kPowerSteps power steps from a fixed
-// start vector (within 2-4% below lambda_max on the batched population's
-// meshes, so hi stays above it), lo = 1e-3 hi.  Eigenvalues below lo are
-// damped less, the outer iteration resolves them.
+// Chebyshev interval [lo, hi] of A (the spectrum of D^-1 K): hi = 1.1 x the
+// Rayleigh quotient after kPowerSteps power steps from a fixed start vector
+// (2-4% below lambda_max on the population's meshes, so hi stays above it),
+// lo = 1e-3 hi.  Eigenvalues below lo are damped less; the outer iteration
+// resolves them.
 //
 // Sequences (every slot of a launch at the same stage, each slot's
 // arithmetic independent of the others):
-//   start:  kp_pinit, kPowerSteps x kp_power, kp_lambda, kp_init, kp_stop
-//   outer:  kp_cheb0, deg x kp_cheb, kp_outer, kp_coef, kp_upd, kp_stop
+//   start:  kp_pinit, kPowerSteps+1 x kp_power, kp_lambda, kp_init, kp_stop
+//   outer:  deg x kp_cheb, kp_outer, kp_coef, kp_upd, kp_stop
+// kp_init and kp_upd also write the next outer iteration's inner start (the
+// scaled residual rs = S r and z_1 = rs / theta; z_0 = 0 is not stored), and
+// the last inner step stores its iterate unscaled (z = S z_deg+1).
 #include "fx_batch.h"
 
 namespace fx {
@@ -36,44 +41,78 @@ namespace {
 
 constexpr double kPolyMargin = 1.1;     // hi = margin x Rayleigh quotient
 constexpr double kPolyLoRatio = 1e-3;   // lo = ratio x hi
-constexpr float kPowerScale = 0.25f;    // keeps power iterates in FP32 range
+constexpr double kPowerScale = 0.25;    // keeps the unnormalised power iterates in range
 
 // start vector of the power iteration: a fixed hash of the DOF index in [-1, 1)
-__device__ __forceinline__ float power_v0(int64_t i) {
+__device__ __forceinline__ double power_v0(int64_t i) {
     uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull;
     h ^= (uint64_t)i >> 7;
     h = (h * 0xBF58476D1CE4E5B9ull) & 0xFFFFFFFFull;
-    return (float)((double)h / 2147483648.0 - 1.0);
+    return (double)h / 2147483648.0 - 1.0;
 }
 
-// acc[p][r] += (K32 q)(3a+p, r) over the node row's blocks, sliced-ELL
-// operator (the symmetric one when the slot has it), one thread per row.
+// the Jacobi scaling of DOF i
+__device__ __forceinline__ double jscale(const BSys &S, int64_t i) { return sqrt(gp(S.inv)[i]); }
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// One block column of the FP16 sliced ELL: its 8-entry record and 9th entry.
+__device__ __forceinline__ void load_block16(const BSys &S, size_t kk, int lane, _Float16 (&v)[9]) {
+    const u32x4 w = gp((const u32x4 *)S.sval16)[kk * kSlice + lane];
+    __builtin_memcpy(v, &w, sizeof w);
+    v[8] = gp(S.sval16b)[kk * kSlice + lane];
+}
+
+// acc[p][r] += (A q)(3a+p, r) over the node row's blocks, sliced-ELL FP16
+// operator, one thread per row: the loop of fx_sell.h's sell_node_rows (the
+// slice's width as the trip count, two blocks a step, column indices one step
+// ahead) with two loads per block's values.
 template <int RR>
-__device__ __forceinline__ void sell_apply32(const BSys &S, int64_t a, gptr<const float> pv, double (&acc)[3][RR]) {
-    if (S.ssym) {
-        ssym_lower_rows<float, RR>(gp(S.sll_ptr), gp(S.lrec), gp(S.lcol), gp(S.uval), a, pv, acc);
-        sell_node_rows<float, float, RR>(gp(S.su_ptr), gp(S.scol_u), gp(S.uval), a, pv, acc);
-    } else {
-        sell_node_rows<float, float, RR>(gp(S.sl_ptr), gp(S.scol), gp((const float *)S.sval), a, pv, acc);
+__device__ __forceinline__ void sell_apply16(const BSys &S, int64_t a, gptr<const double> pv, double (&acc)[3][RR]) {
+    gptr<const int32_t> sl_ptr = gp(S.sl_ptr), scol = gp(S.scol);
+    const int64_t s = a / kSlice;
+    const int lane = (int)(a % kSlice);
+    const int k0 = __builtin_amdgcn_readfirstlane(sl_ptr[s]);
+    const int kw = __builtin_amdgcn_readfirstlane(sl_ptr[s + 1]);
+    int cn0 = scol[(size_t)k0 * kSlice + lane];
+    int cn1 = k0 + 1 < kw ? scol[(size_t)(k0 + 1) * kSlice + lane] : 0;
+    for (int k = k0; k < kw; k += 2) {
+        const bool two = k + 1 < kw;
+        const int c0 = cn0, c1 = cn1;
+        double x0[3][RR], x1[3][RR];
+        _Float16 v0[9], v1[9];
+        gather_block<_Float16, double, RR>(pv, c0, x0);
+        load_block16(S, (size_t)k, lane, v0);
+        if (two) {
+            gather_block<_Float16, double, RR>(pv, c1, x1);
+            load_block16(S, (size_t)k + 1, lane, v1);
+        }
+        if (k + 2 < kw)
+            cn0 = scol[(size_t)(k + 2) * kSlice + lane];
+        if (k + 3 < kw)
+            cn1 = scol[(size_t)(k + 3) * kSlice + lane];
+        madd_block<_Float16, RR>(v0, x0, acc);
+        if (two)
+            madd_block<_Float16, RR>(v1, x1, acc);
     }
 }
 
-// The same on the node-block FP32 values, G lanes per row (column slots
+// The same on the node-block FP16 values, G lanes per row (column slots
 // j = lane, lane + G, ...), reduced over the lanes.
 template <int G, int RR>
-__device__ __forceinline__ void lanes_apply32(const BSys &S, int64_t a, int lane, gptr<const float> pv,
+__device__ __forceinline__ void lanes_apply16(const BSys &S, int64_t a, int lane, gptr<const double> pv,
                                               double (&acc)[3][RR]) {
     if (a < S.n_nodes) {
         gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
         const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
-        gptr<const float> v0 = gp((const float *)S.val) + 9 * (int64_t)b0;
+        gptr<const _Float16> v0 = gp(S.val16) + 9 * (int64_t)b0;
         for (int j = lane; j < ncol; j += G) {
             const int kb = j / 3;
             const int64_t c = (3 * (int64_t)nb_col[b0 + kb] + (j - 3 * kb)) * RR;
-            const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
+            const double w0 = (double)v0[j], w1 = (double)v0[ncol + j], w2 = (double)v0[2 * ncol + j];
 #pragma unroll
             for (int r = 0; r < RR; ++r) {
-                const double xj = (double)pv[c + r];
+                const double xj = pv[c + r];
                 acc[0][r] += w0 * xj;
                 acc[1][r] += w1 * xj;
                 acc[2][r] += w2 * xj;
@@ -89,49 +128,33 @@ __device__ __forceinline__ void lanes_apply32(const BSys &S, int64_t a, int lane
                 acc[p][r] += __shfl_xor(acc[p][r], off, G);
 }
 
-// own-row operand values and the FP32 operator's row-sum correction block
 template <int RR>
-__device__ __forceinline__ void own_and_correction(const BSys &S, int64_t a, gptr<const float> pv,
-                                                   double (&q)[3][RR], double (&acc)[3][RR]) {
+__device__ __forceinline__ void zero_acc(double (&acc)[3][RR]) {
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int r = 0; r < RR; ++r)
-            q[p][r] = (double)pv[(3 * a + p) * RR + r];
-    gptr<const float> c = gp(S.kc) + 9 * a;
-    float cc[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e)
-        cc[e] = c[e];
-#pragma unroll
-    for (int r = 0; r < RR; ++r)
-#pragma unroll
-        for (int p = 0; p < 3; ++p)
-            acc[p][r] += ((double)cc[3 * p] * q[0][r] + (double)cc[3 * p + 1] * q[1][r]) + (double)cc[3 * p + 2] * q[2][r];
+            acc[p][r] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
-// power iteration (RR = 1 layout in zq[0], zq[1]): v_{j+1} = 0.25 inv (K32 v_j);
-// after the last step the partials v.(K32 v) and v.(D v) -> part1 (2 per row group)
+// power iteration on A (RR = 1 layout in zq[0], zq[1]): v_{j+1} = 0.25 A v_j;
+// after the last step the partials v.(A v) and v.v -> part1 (2 per row group)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void power_row_tail(const BSys &S, int64_t a, int step, bool last, gptr<const float> pv,
-                                               double (&acc)[3][1], double (&loc)[2]) {
-    double q[3][1];
-    own_and_correction<1>(S, a, pv, q, acc);
+__device__ __forceinline__ void power_row_tail(const BSys &S, int64_t a, int step, bool last, gptr<const double> pv,
+                                               const double (&acc)[3][1], double (&loc)[2]) {
     if (last) {
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-            const int64_t d = 3 * a + p;
-            loc[0] += q[p][0] * acc[p][0];
-            loc[1] += q[p][0] * (q[p][0] / gp(S.inv)[d]);
+            const double q = pv[3 * a + p];
+            loc[0] += q * acc[p][0];
+            loc[1] += q * q;
         }
     } else {
-        gptr<float> vn = gp(S.zq[(step + 1) & 1]);
+        gptr<double> vn = gp(S.zq[(step + 1) & 1]);
 #pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            const int64_t d = 3 * a + p;
-            vn[d] = kPowerScale * gp(S.inv32)[d] * (float)acc[p][0];
-        }
+        for (int p = 0; p < 3; ++p)
+            vn[3 * a + p] = kPowerScale * acc[p][0];
     }
 }
 
@@ -149,12 +172,13 @@ __global__ __launch_bounds__(kBB) void kp_power_sell(BSys *__restrict__ sys, con
                                                      int last) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
-    gptr<const float> pv = gp((const float *)S.zq[step & 1]);
+    gptr<const double> pv = gp((const double *)S.zq[step & 1]);
     const int64_t a = (int64_t)t.y * kBB + threadIdx.x;
     double loc[2] = {0.0, 0.0};
     if (a < S.n_nodes) {
-        double acc[3][1] = {{0.0}, {0.0}, {0.0}};
-        sell_apply32<1>(S, a, pv, acc);
+        double acc[3][1];
+        zero_acc<1>(acc);
+        sell_apply16<1>(S, a, pv, acc);
         power_row_tail(S, a, step, last, pv, acc, loc);
     }
     write_power_partials(S, t.y, last, loc);
@@ -165,12 +189,13 @@ __global__ __launch_bounds__(kBB) void kp_power_lanes(BSys *__restrict__ sys, co
                                                       int last) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
-    gptr<const float> pv = gp((const float *)S.zq[step & 1]);
+    gptr<const double> pv = gp((const double *)S.zq[step & 1]);
     const int64_t a = (int64_t)t.y * (kBB / G) + threadIdx.x / G;
     const int lane = threadIdx.x % G;
     double loc[2] = {0.0, 0.0};
-    double acc[3][1] = {{0.0}, {0.0}, {0.0}};
-    lanes_apply32<G, 1>(S, a, lane, pv, acc);
+    double acc[3][1];
+    zero_acc<1>(acc);
+    lanes_apply16<G, 1>(S, a, lane, pv, acc);
     if (lane == 0 && a < S.n_nodes)
         power_row_tail(S, a, step, last, pv, acc, loc);
     write_power_partials(S, t.y, last, loc);
@@ -186,8 +211,8 @@ __global__ __launch_bounds__(kBB) void kp_pinit(BSys *__restrict__ sys, const in
 }
 
 // One block per slot: lambda_max estimate, the Chebyshev coefficients of the
-// three-term recurrence z_{j+1} = z_j + c1_j (z_j - z_{j-1}) + c2_j D^-1 (r - K z_j),
-// z_0 = 0, z_1 = D^-1 r / theta (Saad Alg. 12.1: rho_0 = 1/sigma,
+// three-term recurrence z_{j+1} = z_j + c1_j (z_j - z_{j-1}) + c2_j (rs - A z_j),
+// z_0 = 0, z_1 = rs / theta (Saad Alg. 12.1: rho_0 = 1/sigma,
 // rho_j = 1/(2 sigma - rho_{j-1}), c1_j = rho_j rho_{j-1}, c2_j = 2 rho_j / delta).
 __global__ __launch_bounds__(kBB) void kp_lambda(BSys *__restrict__ sys, const int32_t *__restrict__ list) {
     BSys &S = sys[list[blockIdx.x]];
@@ -213,8 +238,9 @@ __global__ __launch_bounds__(kBB) void kp_lambda(BSys *__restrict__ sys, const i
 // outer iteration
 // ---------------------------------------------------------------------------
 
-// x = 0, r = b, pp = ps = 0 for the load cases with status 0; partials
-// (r.r, r.(D^-1 r), b.b) -> part2 (3 per load case)
+// x = 0, r = b, pp = ps = 0 for the load cases with status 0, and the first
+// inner start (rs = S b, z_1 = rs / theta); partials (r.r, r.(D^-1 r), b.b)
+// -> part2 (3 per load case)
 __global__ __launch_bounds__(kBB) void kp_init(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
@@ -225,17 +251,20 @@ __global__ __launch_bounds__(kBB) void kp_init(BSys *__restrict__ sys, const int
         v[k] = 0.0;
     const int64_t i = (int64_t)t.y * kBB + threadIdx.x;
     if (i < S.n_dof) {
-        const double inv = gp(S.inv)[i];
+        const double inv = gp(S.inv)[i], si = jscale(S, i), ith = S.cheb[0];
 #pragma unroll
         for (int r = 0; r < kRmax; ++r) {
             if (r >= R || S.rhs[r].status != 0)
                 continue;
             const int64_t q = i * R + r;
             const double bi = gp(S.btrue)[q];
+            const double rsi = si * bi;
             gp((double *)S.x)[q] = 0.0;
             gp((double *)S.r)[q] = bi;
             gp(S.pp)[q] = 0.0;
             gp(S.ps)[q] = 0.0;
+            gp(S.rs)[q] = rsi;
+            gp(S.zq[1])[q] = rsi * ith;
             v[3 * r] += bi * bi;
             v[3 * r + 1] += bi * (bi * inv);
             v[3 * r + 2] += bi * bi;
@@ -248,68 +277,96 @@ __global__ __launch_bounds__(kBB) void kp_init(BSys *__restrict__ sys, const int
                 S.part2[((size_t)t.y * R + r) * 3 + k] = v[3 * r + k];
 }
 
-// rf = fl32(r), z_0 = 0, z_1 = fl32(D^-1 r / theta) for the running load cases
-__global__ __launch_bounds__(kBB) void kp_cheb0(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
-    const int2 t = tab[blockIdx.x];
-    const BSys &S = sys[t.x];
-    const int go = S.go;
-    if (!go)
-        return;
-    const int64_t i = (int64_t)t.y * kBB + threadIdx.x;
-    if (i >= S.n_dof)
-        return;
-    const int R = S.R;
-    const double inv = gp(S.inv)[i], ith = S.cheb[0];
-    for (int r = 0; r < R; ++r) {
-        if (!((go >> r) & 1))
-            continue;
-        const int64_t q = i * R + r;
-        const double ri = gp((const double *)S.r)[q];
-        gp(S.rf)[q] = (float)ri;
-        gp(S.zq[0])[q] = 0.f;
-        gp(S.zq[1])[q] = (float)(inv * ri * ith);
-    }
-}
-
-// own-row update of inner step j: z_{j+1} = z_j + c1 (z_j - z_{j-1}) + c2 D^-1 (rf - K32 z_j)
+// own-row update of inner step j: z_{j+1} = z_j + c1 (z_j - z_{j-1}) + c2 (rs - A z_j),
+// stored unscaled (S z_{j+1}) at the last step
 template <int RR>
-__device__ __forceinline__ void cheb_row_tail(const BSys &S, int64_t a, int j, int go, gptr<const float> pv,
-                                              double (&acc)[3][RR]) {
-    double q[3][RR];
-    own_and_correction<RR>(S, a, pv, q, acc);
+__device__ __forceinline__ void cheb_row_tail(const BSys &S, int64_t a, int j, int go, gptr<const double> pv,
+                                              const double (&acc)[3][RR]) {
     const double c1 = S.cheb[2 * j], c2 = S.cheb[2 * j + 1];
-    gptr<const float> zm = gp((const float *)S.zq[(j + 2) % 3]);  // z_{j-1}
-    gptr<float> zn = gp(S.zq[(j + 1) % 3]);
+    const bool last = j == S.deg;
+    gptr<const double> zm = gp((const double *)S.zq[(j + 2) % 3]);  // z_{j-1}
+    gptr<const double> rs = gp((const double *)S.rs);
+    gptr<double> zn = gp(S.zq[(j + 1) % 3]);
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const int64_t d = 3 * a + p;
-        const double inv = (double)gp(S.inv32)[d];
+        const double sd = last ? jscale(S, d) : 1.0;
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
             if (!((go >> r) & 1))
                 continue;
             const int64_t qd = d * RR + r;
-            const double zj = q[p][r];
-            const double res = (double)gp((const float *)S.rf)[qd] - acc[p][r];
-            zn[qd] = (float)(zj + c1 * (zj - (double)zm[qd]) + c2 * (inv * res));
+            const double zj = pv[qd];
+            const double zmj = j == 1 ? 0.0 : zm[qd];  // z_0 = 0 (not stored)
+            const double zv = zj + c1 * (zj - zmj) + c2 * (rs[qd] - acc[p][r]);
+            zn[qd] = last ? sd * zv : zv;
         }
     }
 }
 
+// A wavefront's rows in the sliced-ELL kernels: 64 consecutive node rows,
+// i.e. one contiguous range of 3 * 64 * RR entries of every vector.  Such a
+// range is written through LDS with lane-consecutive stores instead of 3 * RR
+// strided ones per thread.
+struct WaveRows {
+    int w0;        // first thread of the wavefront in the block
+    int64_t row0;  // its first node row
+    int nrow;      // its rows inside the system (0..64)
+};
+
+__device__ __forceinline__ WaveRows wave_rows(const BSys &S, int lb) {
+    WaveRows w;
+    w.w0 = (threadIdx.x / 64) * 64;
+    w.row0 = (int64_t)lb * kBB + w.w0;
+    const int64_t left = S.n_nodes - w.row0;
+    w.nrow = left <= 0 ? 0 : (left < 64 ? (int)left : 64);
+    return w;
+}
+
+// dst[row0 * W + e] = tile[w0 * W + e] for the wavefront's entries (W = 3 RR per row)
+template <int W>
+__device__ __forceinline__ void stage_out(const WaveRows &w, const double *tile, gptr<double> dst) {
+    const int lane = threadIdx.x % 64;
+    for (int e = lane; e < w.nrow * W; e += 64)
+        dst[w.row0 * W + e] = tile[w.w0 * W + e];
+}
+
+// The sliced-ELL inner step.  The new iterate goes out through an LDS tile
+// as lane-consecutive stores (staging the z_{j-1} and rs reads the same way
+// was slower: those reads hit lines the wavefront's other loads bring in).
+// Entries of load cases that do not run are written as well (their values
+// are never read).
 template <int RR>
 __device__ __forceinline__ void cheb_sell(const BSys &S, int lb, int j, int go) {
+    constexpr int W = 3 * RR;  // entries per node row
+    __shared__ double tile[kBB * 3 * kRmax];
     const int64_t a = (int64_t)lb * kBB + threadIdx.x;
-    if (a >= S.n_nodes)
-        return;
-    gptr<const float> pv = gp((const float *)S.zq[j % 3]);
-    double acc[3][RR];
+    gptr<const double> pv = gp((const double *)S.zq[j % 3]);
+    if (a < S.n_nodes) {
+        double acc[3][RR];
+        zero_acc<RR>(acc);
+        sell_apply16<RR>(S, a, pv, acc);
+        const double c1 = S.cheb[2 * j], c2 = S.cheb[2 * j + 1];
+        const bool last = j == S.deg;
+        gptr<const double> zm = gp((const double *)S.zq[(j + 2) % 3]);  // z_{j-1}
+        gptr<const double> rs = gp((const double *)S.rs);
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < 3; ++p) {
+            const int64_t d = 3 * a + p;
+            const double sd = last ? jscale(S, d) : 1.0;
 #pragma unroll
-        for (int r = 0; r < RR; ++r)
-            acc[p][r] = 0.0;
-    sell_apply32<RR>(S, a, pv, acc);
-    cheb_row_tail<RR>(S, a, j, go, pv, acc);
+            for (int r = 0; r < RR; ++r) {
+                const int64_t qd = d * RR + r;
+                const double zj = pv[qd];
+                const double zmj = j == 1 ? 0.0 : zm[qd];  // z_0 = 0 (not stored)
+                const double zv = zj + c1 * (zj - zmj) + c2 * (rs[qd] - acc[p][r]);
+                tile[threadIdx.x * W + p * RR + r] = last ? sd * zv : zv;
+            }
+        }
+    }
+    __syncthreads();
+    stage_out<W>(wave_rows(S, lb), tile, gp(S.zq[(j + 1) % 3]));
+    (void)go;
 }
 
 __global__ __launch_bounds__(kBB) void kp_cheb_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
@@ -330,14 +387,10 @@ template <int G, int RR>
 __device__ __forceinline__ void cheb_lanes(const BSys &S, int lb, int j, int go) {
     const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
     const int lane = threadIdx.x % G;
-    gptr<const float> pv = gp((const float *)S.zq[j % 3]);
+    gptr<const double> pv = gp((const double *)S.zq[j % 3]);
     double acc[3][RR];
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int r = 0; r < RR; ++r)
-            acc[p][r] = 0.0;
-    lanes_apply32<G, RR>(S, a, lane, pv, acc);
+    zero_acc<RR>(acc);
+    lanes_apply16<G, RR>(S, a, lane, pv, acc);
     if (lane == 0 && a < S.n_nodes)
         cheb_row_tail<RR>(S, a, j, go, pv, acc);
 }
@@ -357,19 +410,44 @@ __global__ __launch_bounds__(kBB) void kp_cheb_lanes(BSys *__restrict__ sys, con
     }
 }
 
-// FP64 outer SpMV w = K64 z (z = zq[(deg+1) % 3], FP32) on the node-block
-// values, kOuterG lanes per row; partials per load case (z.s, z.r, p.r,
-// z.w, p.w) with p, s the previous direction and its image -> part1.
+// FP64 outer SpMV w = K64 z (z = zq[(deg+1) % 3], unscaled): on the
+// node-block values with kOuterG lanes per row, or on the sliced-ELL FP64
+// operator one thread per row; partials per load case (z.s, z.r, p.r, z.w,
+// p.w) with p, s the previous direction and its image -> part1.
+template <int RR>
+__device__ __forceinline__ void outer_row_tail(const BSys &S, int64_t a, int go, gptr<const double> zv,
+                                               const double (&acc)[3][RR], double (&v)[5 * kRmax],
+                                               double *tile = nullptr) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const int64_t d = 3 * a + p;
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            if (!((go >> r) & 1))
+                continue;
+            const int64_t q = d * RR + r;
+            const double z = zv[q], w = acc[p][r];
+            const double so = gp((const double *)S.ps)[q], po = gp((const double *)S.pp)[q];
+            const double rr = gp((const double *)S.r)[q];
+            if (tile)
+                tile[threadIdx.x * 3 * RR + p * RR + r] = w;
+            else
+                gp(S.pw)[q] = w;
+            v[5 * r] += z * so;
+            v[5 * r + 1] += z * rr;
+            v[5 * r + 2] += po * rr;
+            v[5 * r + 3] += z * w;
+            v[5 * r + 4] += po * w;
+        }
+    }
+}
+
 template <int RR>
 __device__ __forceinline__ void outer_row(const BSys &S, int64_t a, int lane, int go, double (&v)[5 * kRmax]) {
     constexpr int G = kOuterG;
-    gptr<const float> zv = gp((const float *)S.zq[(S.deg + 1) % 3]);
+    gptr<const double> zv = gp((const double *)S.zq[(S.deg + 1) % 3]);
     double acc[3][RR];
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int r = 0; r < RR; ++r)
-            acc[p][r] = 0.0;
+    zero_acc<RR>(acc);
     if (a < S.n_nodes) {
         gptr<const int32_t> nb_ptr = gp(S.nb_ptr), nb_col = gp(S.nb_col);
         const int b0 = nb_ptr[a], ncol = 3 * (nb_ptr[a + 1] - b0);
@@ -380,7 +458,7 @@ __device__ __forceinline__ void outer_row(const BSys &S, int64_t a, int lane, in
             const double w0 = v0[j], w1 = v0[ncol + j], w2 = v0[2 * ncol + j];
 #pragma unroll
             for (int r = 0; r < RR; ++r) {
-                const double xj = (double)zv[c + r];
+                const double xj = zv[c + r];
                 acc[0][r] += w0 * xj;
                 acc[1][r] += w1 * xj;
                 acc[2][r] += w2 * xj;
@@ -396,34 +474,30 @@ __device__ __forceinline__ void outer_row(const BSys &S, int64_t a, int lane, in
                 acc[p][r] += __shfl_xor(acc[p][r], off, G);
     if (lane != 0 || a >= S.n_nodes)
         return;
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-        const int64_t d = 3 * a + p;
-#pragma unroll
-        for (int r = 0; r < RR; ++r) {
-            if (!((go >> r) & 1))
-                continue;
-            const int64_t q = d * RR + r;
-            const double z = (double)zv[q], w = acc[p][r];
-            const double so = gp((const double *)S.ps)[q], po = gp((const double *)S.pp)[q];
-            const double rr = gp((const double *)S.r)[q];
-            gp(S.pw)[q] = w;
-            v[5 * r] += z * so;
-            v[5 * r + 1] += z * rr;
-            v[5 * r + 2] += po * rr;
-            v[5 * r + 3] += z * w;
-            v[5 * r + 4] += po * w;
-        }
-    }
+    outer_row_tail<RR>(S, a, go, zv, acc, v);
 }
 
-template <int RR>
+template <int RR, bool SELL>
 __device__ __forceinline__ void outer_block(const BSys &S, int lb, int go) {
     double v[5 * kRmax];
 #pragma unroll
     for (int k = 0; k < 5 * kRmax; ++k)
         v[k] = 0.0;
-    outer_row<RR>(S, (int64_t)lb * (kBB / kOuterG) + threadIdx.x / kOuterG, threadIdx.x % kOuterG, go, v);
+    if constexpr (SELL) {  // w leaves through LDS as lane-consecutive stores (see cheb_sell)
+        __shared__ double tile[kBB * 3 * kRmax];
+        const int64_t a = (int64_t)lb * kBB + threadIdx.x;
+        if (a < S.n_nodes) {
+            gptr<const double> zv = gp((const double *)S.zq[(S.deg + 1) % 3]);
+            double acc[3][RR];
+            zero_acc<RR>(acc);
+            sell_node_rows<double, double, RR>(gp(S.sl_ptr), gp(S.scol), gp(S.sval64), a, zv, acc);
+            outer_row_tail<RR>(S, a, go, zv, acc, v, tile);
+        }
+        __syncthreads();
+        stage_out<3 * RR>(wave_rows(S, lb), tile, gp(S.pw));
+    } else {
+        outer_row<RR>(S, (int64_t)lb * (kBB / kOuterG) + threadIdx.x / kOuterG, threadIdx.x % kOuterG, go, v);
+    }
     block_sum<5 * kRmax>(v);
     if (threadIdx.x == 0)
         for (int r = 0; r < RR; ++r)
@@ -431,6 +505,7 @@ __device__ __forceinline__ void outer_block(const BSys &S, int lb, int go) {
                 S.part1[((size_t)lb * RR + r) * 5 + k] = v[5 * r + k];
 }
 
+template <bool SELL>
 __global__ __launch_bounds__(kBB) void kp_outer(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
@@ -438,10 +513,10 @@ __global__ __launch_bounds__(kBB) void kp_outer(BSys *__restrict__ sys, const in
     if (!go)
         return;
     switch (S.R) {
-    case 1: outer_block<1>(S, t.y, go); break;
-    case 2: outer_block<2>(S, t.y, go); break;
-    case 3: outer_block<3>(S, t.y, go); break;
-    default: outer_block<4>(S, t.y, go); break;
+    case 1: outer_block<1, SELL>(S, t.y, go); break;
+    case 2: outer_block<2, SELL>(S, t.y, go); break;
+    case 3: outer_block<3, SELL>(S, t.y, go); break;
+    default: outer_block<4, SELL>(S, t.y, go); break;
     }
 }
 
@@ -484,29 +559,35 @@ __global__ __launch_bounds__(kBB) void kp_coef(BSys *__restrict__ sys, const int
     S.go = go2;
 }
 
-// p = z - beta p_old, s = w - beta s_old, x += alpha p, r -= alpha s;
-// partials (r.r, r.(D^-1 r), 0) -> part2
+// p = z - beta p_old, s = w - beta s_old, x += alpha p, r -= alpha s, and
+// the next inner start (rs = S r, z_1 = rs / theta); partials (r.r,
+// r.(D^-1 r), 0) -> part2
 template <int RR>
 __device__ __forceinline__ void upd_tile(const BSys &S, int lb, int go, double (&v)[3 * kRmax]) {
     const int64_t i = (int64_t)lb * kBB + threadIdx.x;
     if (i >= S.n_dof)
         return;
-    const double inv = gp(S.inv)[i];
-    gptr<const float> zv = gp((const float *)S.zq[(S.deg + 1) % 3]);
+    const double inv = gp(S.inv)[i], si = jscale(S, i), ith = S.cheb[0];
+    gptr<const double> zv = gp((const double *)S.zq[(S.deg + 1) % 3]);
 #pragma unroll
     for (int r = 0; r < RR; ++r) {
         if (!((go >> r) & 1))
             continue;
         const double al = S.rhs[r].alpha, be = S.rhs[r].beta;
         const int64_t q = i * RR + r;
-        const double pn = (double)zv[q] - be * gp((const double *)S.pp)[q];
+        const double pn = zv[q] - be * gp((const double *)S.pp)[q];
         const double sn = gp((const double *)S.pw)[q] - be * gp((const double *)S.ps)[q];
         const double xn = gp((const double *)S.x)[q] + al * pn;
         const double rn = gp((const double *)S.r)[q] - al * sn;
+        const double rsn = si * rn;
         gp(S.pp)[q] = pn;
         gp(S.ps)[q] = sn;
         gp((double *)S.x)[q] = xn;
         gp((double *)S.r)[q] = rn;
+        // the next outer iteration's inner start (unused if this one stops);
+        // z_1 may share z's buffer: this thread has read its entry above
+        gp(S.rs)[q] = rsn;
+        gp(S.zq[1])[q] = rsn * ith;
         v[3 * r] += rn * rn;
         v[3 * r + 1] += rn * (rn * inv);
     }
@@ -539,7 +620,7 @@ __global__ __launch_bounds__(kBB) void kp_upd(BSys *__restrict__ sys, const int2
 // per load case (as fx_pcg.hip's: sqrt(r.z) < tol, or ||r||/||b|| <= tol),
 // the outer iteration limit, and the mask of load cases the next outer
 // iteration works on.  A slot with no load case left running writes its
-// finish summary for the host (iterations: FP32 inner SpMVs; nupd: outer
+// finish summary for the host (iterations: inner steps; nupd: outer
 // iterations).
 __global__ __launch_bounds__(kBB) void kp_stop(BSys *__restrict__ sys, const int32_t *__restrict__ list,
                                                BFin *__restrict__ fin, int after_update) {
@@ -614,8 +695,7 @@ __global__ __launch_bounds__(kBB) void kp_stop(BSys *__restrict__ sys, const int
 }
 
 template <typename K>
-void launch_inner_tables(hipStream_t s, BSys *sys, const int2 *tab_sell, int n_sell, const int2 *const *tab_g,
-                         const int *n_g, K &&launch) {
+void launch_inner_tables(const int2 *tab_sell, int n_sell, const int2 *const *tab_g, const int *n_g, K &&launch) {
     if (n_sell)
         launch(0, tab_sell, n_sell);
     for (int g = 0; g < 6; ++g)
@@ -631,7 +711,7 @@ void launch_poly_power_start(hipStream_t s, BSys *sys, const int2 *tab_vec, int 
     hipLaunchKernelGGL(kp_pinit, dim3(n_vec), dim3(kBB), 0, s, sys, tab_vec);
     for (int step = 0; step <= kPowerSteps; ++step) {
         const int last = step == kPowerSteps;
-        launch_inner_tables(s, sys, tab_in_sell, n_in_sell, tab_in_g, n_in_g, [&](int G, const int2 *tab, int n) {
+        launch_inner_tables(tab_in_sell, n_in_sell, tab_in_g, n_in_g, [&](int G, const int2 *tab, int n) {
             switch (G) {
             case 0: hipLaunchKernelGGL(kp_power_sell, dim3(n), dim3(kBB), 0, s, sys, tab, step, last); break;
             case 2: hipLaunchKernelGGL(kp_power_lanes<2>, dim3(n), dim3(kBB), 0, s, sys, tab, step, last); break;
@@ -649,13 +729,13 @@ void launch_poly_power_start(hipStream_t s, BSys *sys, const int2 *tab_vec, int 
 }
 
 void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, int n_vec, const int2 *tab_in_sell,
-                       int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int2 *tab_out, int n_out,
-                       const int32_t *list, int n_list, BFin *fin, hipEvent_t ev0, hipEvent_t ev1) {
-    hipLaunchKernelGGL(kp_cheb0, dim3(n_vec), dim3(kBB), 0, s, sys, tab_vec);
+                       int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int2 *tab_out_sell,
+                       int n_out_sell, const int2 *tab_out, int n_out, const int32_t *list, int n_list, BFin *fin,
+                       hipEvent_t ev0, hipEvent_t ev1) {
     if (ev0)
         FX_HIP(hipEventRecord(ev0, s));
     for (int j = 1; j <= deg; ++j)
-        launch_inner_tables(s, sys, tab_in_sell, n_in_sell, tab_in_g, n_in_g, [&](int G, const int2 *tab, int n) {
+        launch_inner_tables(tab_in_sell, n_in_sell, tab_in_g, n_in_g, [&](int G, const int2 *tab, int n) {
             switch (G) {
             case 0: hipLaunchKernelGGL(kp_cheb_sell, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
             case 2: hipLaunchKernelGGL(kp_cheb_lanes<2>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
@@ -668,7 +748,10 @@ void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, i
         });
     if (ev1)
         FX_HIP(hipEventRecord(ev1, s));
-    hipLaunchKernelGGL(kp_outer, dim3(n_out), dim3(kBB), 0, s, sys, tab_out);
+    if (n_out_sell)
+        hipLaunchKernelGGL(kp_outer<true>, dim3(n_out_sell), dim3(kBB), 0, s, sys, tab_out_sell);
+    if (n_out)
+        hipLaunchKernelGGL(kp_outer<false>, dim3(n_out), dim3(kBB), 0, s, sys, tab_out);
     hipLaunchKernelGGL(kp_coef, dim3(n_list), dim3(kBB), 0, s, sys, list);
     hipLaunchKernelGGL(kp_upd, dim3(n_vec), dim3(kBB), 0, s, sys, tab_vec);
     hipLaunchKernelGGL(kp_stop, dim3(n_list), dim3(kBB), 0, s, sys, list, fin, 1);

@@ -236,6 +236,7 @@ struct fx_mesh {
     fx::DBuf<double> kval;               // nnz_struct values (super-pattern)
     fx::DBuf<float> kval32;              // FP32 operator of the mixed solve ...
     fx::DBuf<float> kcorr32;             // ... and its per-node 3x3 row-sum corrections
+    fx::DBuf<_Float16> kval16, sval16;   // Jacobi-scaled FP16 operator (node blocks / sliced ELL)
     // sliced-ELL copy of the operator for the solve (meshes with short node
     // rows, see fx_sell.h): slice pointers, column nodes, values
     bool sell = false;
@@ -290,6 +291,7 @@ void launch_jacobi(fx_mesh *m, bool want32, hipStream_t s = nullptr);
 void launch_convert32(fx_mesh *m, hipStream_t s = nullptr);
 void launch_sell(fx_mesh *m, bool fp32, hipStream_t s = nullptr);  // sliced-ELL copy of kval (FP64) or kval32 (FP32)
 void launch_ssym(fx_mesh *m, hipStream_t s = nullptr);  // symmetric FP32 sliced-ELL operator + its row-sum correction
+void launch_scaled16(fx_mesh *m, hipStream_t s = nullptr);  // Jacobi-scaled FP16 operator (after launch_jacobi)
 void check_status(fx_mesh *m, const char *what);
 void reset_status(fx_mesh *m);
 double device_max_diag(fx_mesh *m);

@@ -72,7 +72,7 @@ class SolveOptions:
     inner_tol: float = 1e-3
     max_refine: int = 50
     spmv_group: int = 0
-    poly_degree: int = 0  # MIXED_POLY batches: FP32 Chebyshev steps per outer iteration (0: 15)
+    poly_degree: int = 0  # MIXED_POLY batches: Chebyshev steps per outer iteration (0: 15)
 
     def c(self) -> L.fx_solve_opts:
         return L.fx_solve_opts(self.precision, self.stop_mode, self.tol, self.max_iter, self.inner_tol,

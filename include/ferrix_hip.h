@@ -159,11 +159,11 @@ typedef enum fx_precision {
     FX_PREC_MIXED_POLY = 2 /* batches only: flexible CG in FP64 (x, r, the
                           directions, K64 z, every dot product, the stop
                           rule on the FP64 residual) preconditioned by
-                          poly_degree Chebyshev-Jacobi steps in FP32 on the
-                          FP32-stored K; the Chebyshev interval comes from a
-                          power iteration per system when it joins.
-                          iterations in the report count FP32 inner SpMVs,
-                          outer_spmv the FP64 outer iterations */
+                          poly_degree Chebyshev steps on the Jacobi-scaled K
+                          stored in FP16, with FP64 iterates; the Chebyshev
+                          interval comes from a power iteration per system
+                          when it joins.  iterations in the report count the
+                          inner steps, outer_spmv the FP64 outer iterations */
 } fx_precision;
 
 typedef enum fx_stop {
@@ -182,9 +182,9 @@ typedef struct fx_solve_opts {
     int32_t spmv_group; /* lanes per node row in the SpMV (0: the mesh's tuned
                            group, fx_mesh_autotune_spmv, else a row-length
                            heuristic) */
-    int32_t poly_degree; /* MIXED_POLY: FP32 Chebyshev steps per outer
-                            iteration, 1..32 (0: 15); max_iter bounds the
-                            FP32 steps of a load case */
+    int32_t poly_degree; /* MIXED_POLY: Chebyshev steps per outer iteration,
+                            1..32 (0: 15); max_iter bounds the inner steps
+                            of a load case */
 } fx_solve_opts;
 
 typedef struct fx_solve_report {

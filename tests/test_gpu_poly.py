@@ -1,6 +1,6 @@
 """Polynomial-preconditioned mixed-precision batches (FX_PREC_MIXED_POLY):
-flexible CG in FP64 with FP32 Chebyshev-Jacobi inner steps on the FP32
-operator.
+flexible CG in FP64 preconditioned by Chebyshev steps on the Jacobi-scaled
+operator stored in FP16, with FP64 iterates.
 
 Bars:
 * every load case within 1e-8 relative of the direct solve of the oracle's
@@ -8,9 +8,8 @@ Bars:
 * bit for bit alone vs in a mix, whatever was admitted before or later, and
   whatever the run chunking (a slot's arithmetic never depends on its
   neighbours or on the host's pacing);
-* both FP32 operators: the sliced-ELL one (C3D4 meshes; its symmetric
-  storage) and the node-block lane-group one (C3D20 meshes, or C3D4
-  uploaded with FX_SELL=0);
+* both layouts of the FP16 operator: sliced ELL (C3D4 meshes) and node
+  blocks with lane groups (C3D20 meshes, or C3D4 uploaded with FX_SELL=0);
 * the reference's stop rule (sqrt(r.z) < 1e-8 on the FP64 residual) and the
   not-converged error of the reference's solver.
 """
@@ -70,17 +69,17 @@ def test_poly_batch_alone_vs_mix_and_oracle(ctx):
         assert np.array_equal(x, solo[k][0]), f"system {k}: mix differs from alone"
         assert rep.iterations == solo[k][1].iterations
         errs = oracle_errors(meshgen.cfg4_mesh(spec), x, sys[k][1])
-        say("system", k, "n_dof", spec["n_dof"], "FP32 steps", rep.iterations, "outer", rep.outer_spmv,
+        say("system", k, "n_dof", spec["n_dof"], "inner steps", rep.iterations, "outer", rep.outer_spmv,
             "rel err", max(errs))
         assert max(errs) < 1e-8
         assert rep.outer_spmv * 15 == rep.iterations
 
 
 @pytest.mark.parametrize("sell", [True, False])
-def test_poly_fp32_operators(ctx, sell):
-    """The sliced-ELL FP32 operator (symmetric storage) and the node-block
-    lane-group one (FX_SELL=0) each meet the bar; their summation orders
-    differ, so their results agree to the tolerance, not bitwise."""
+def test_poly_operator_layouts(ctx, sell):
+    """The sliced-ELL FP16 operator and the node-block lane-group one
+    (FX_SELL=0) each meet the bar; their summation orders differ, so their
+    results agree to the tolerance, not bitwise."""
     specs = small_specs(3, seed=5)
     sys = population(ctx, specs, sell=sell)
     bt = Batch(ctx, POLY)
