@@ -14,31 +14,40 @@ timing; every step reassembles from u = 0.
 Arms (identical work and stopping rule):
 * "fp64_serial": the reference's pattern on the GPU -- one FP64 fx_solve per
   load case, system after system;
-* "mixed_batched" (headline): continuous batching (fx_batch) with the
-  FP32-stored, row-sum-corrected K and FP32 Krylov vectors, the residual
-  replaced by the FP64 b - K x whenever the iterated one has fallen by
-  inner_tol; load cases of a system share each pass over K, systems are
-  admitted under an operator working-set budget;
-* "fp64_batched": the same batching in FP64.
+* "fp64_batched": continuous batching (fx_batch) of every load case in FP64
+  Jacobi-PCG;
+* "mixed_batched": the same batching with the FP32-stored, row-sum-corrected
+  K and FP32 Krylov vectors, the residual replaced by the FP64 b - K x
+  whenever the iterated one has fallen by inner_tol;
+* "mixed_poly" (headline): the same batching with FP64 flexible CG whose
+  preconditioner is a fixed number of Chebyshev steps on the Jacobi-scaled
+  K stored in FP16 (FP64 iterates); outer SpMV, dots, residual and stop rule
+  in FP64.
+Load cases of a system share each pass over K; systems are admitted under an
+operator working-set budget.  The headline arm times --steps after --warmup;
+the comparison arms a fixed count (--compare-steps/--compare-warmup, one
+each by default), recorded per arm.
 value = DOF-solves/s of the headline arm (sum of n_dof x load cases / wall
 seconds over the K timed steps).  p50/p99: per-system time from the start of
 the step to its solution on the host.  Accuracy: every solution's relative
-L2 error against the fp64_serial solution.
+L2 error against the fp64_serial solution, and the hardest members'
+solutions against the oracle's direct solves (tests/golden/population_hard).
 
-roofline: the dominant kernel is the batched SpMV kb_spmv_sell<float,
-float>; every SpMV launch of the timed run is bracketed by HIP events on
-the library's stream and the library sums each launch's algorithmic bytes
-on the device; achieved = algorithmic bytes / measured duration.  traffic:
-the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary's ratio to the
-algorithmic bytes, times this run's bytes per launch (when the summary was
-taken on the same kernels and workload).
+roofline: the dominant kernel of the headline arm (kp_cheb_sell, the fused
+FP16 SpMV + Chebyshev update; kb_spmv_sell<float, float> for mixed_batched);
+every launch of it in the timed run is bracketed by HIP events on the
+library's stream and the library sums each launch's algorithmic bytes on the
+device; achieved = algorithmic bytes / measured duration.  traffic: the
+committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary's ratio to the algorithmic
+bytes, times this run's bytes per launch (when the summary was taken on the
+same kernels and workload).
 
 cpu_baseline: the oracle (C restatement of the reference CPU path: serial
-element loop + triplets + CSR, Jacobi-PCG with a row-parallel SpMV on 4
-OpenMP threads, the reference's default pool size) on a bounded sample of
-the same population, DOF-solves/s over the sample.  The same host leg checks
-the GPU: the oracle's direct solves of the smallest systems against every
-arm's solutions (u_rel_err_vs_oracle_sample_max).
+element loop + triplets + CSR, Jacobi-PCG with the reference's rayon
+structure on OpenMP threads) on 8 systems of the same population, every
+phase run in full, on 4 threads (the reference's default pool) and on every
+CPU the process may use; value is the faster leg, and the host's CPU model,
+socket and core counts are recorded.
 
 "--workload single": the C3D20 block (40x10x10, 56,463 DOF), one linear
 static step per timed step, FP64 vs mixed, with the same roofline and
@@ -242,67 +251,115 @@ def population(n_meshes):
     return specs, [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
 
 
-def cpu_baseline_batch(specs, iters=None, threads=4, sample_iters=50):
-    """Oracle on a bounded sample of the population: every 32nd system (16
-    systems spanning the size range).  Per system: serial assembly + CSR
-    timed in full; Jacobi-PCG timed over `sample_iters` iterations on
-    `threads` OpenMP threads and extrapolated to the iterations each load
-    case needs (`iters`: key -> per-load-case counts of the FP64 solve, same
-    algorithm and stopping rule); without counts the solves run in full."""
+def host_info():
+    """The host the CPU leg ran on: logical CPUs, the CPUs this process may
+    use, the CPU model and the socket count (from /proc/cpuinfo)."""
+    model, sockets = None, set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k == "model name" and model is None:
+                model = v.strip()
+            elif k == "physical id":
+                sockets.add(v.strip())
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model,
+            "sockets": len(sockets) or None}
+
+
+def host_threads():
+    """The all-cores leg's thread count: the CPUs this process may use,
+    capped by OMP_NUM_THREADS when that is set (the host's share)."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(cap)) if cap.isdigit() and int(cap) > 0 else n
+
+
+def cpu_baseline_batch(specs, stride=64, thread_legs=(4, None)):
+    """The reference CPU path restated by the oracle on a bounded sample of
+    the population: every `stride`-th system (8 systems spanning the size
+    range), each phase timed in full.  Per system: the serial element loop
+    and triplet assembly (assembler.rs:37-151), triplets -> CSR with the
+    penalty (static_step.rs:183-198), then every load case's Jacobi-PCG to
+    ||r||/||b|| <= 1e-10 run to convergence with the reference's rayon
+    structure (fxo_pcg_par: iterative.rs:26-136) on each thread count of
+    `thread_legs` (None: every CPU this process may use, main.rs:20-22 with
+    --num-threads).  Assembly and CSR are serial, as in the reference, and
+    timed once."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from ferrix_amd import meshgen
 
-    secs, dofs, names, t_all = 0.0, 0, [], time.perf_counter()
-    for k in range(0, len(specs), 32):
+    legs = [t if t is not None else host_threads() for t in thread_legs]
+    legs = sorted(set(legs))
+    t_all = time.perf_counter()
+    t_asm = t_csr = 0.0
+    t_solve = {t: 0.0 for t in legs}
+    dofs, names, iters = 0, [], 0
+    for k in range(0, len(specs), stride):
         sp = specs[k]
         jobs = meshgen.cfg4_mesh(sp)
         st = jobs[0].steps[0]
         t0 = time.perf_counter()
         r, c, v, md = O.assemble_triplets(jobs[0].mesh, jobs[0].node_temp)
+        t1 = time.perf_counter()
         bd = np.asarray(st.bc_dof, np.int32)
         K = O.triplets_to_csr(jobs[0].n_dof, np.concatenate([r, bd]), np.concatenate([c, bd]),
                               np.concatenate([v, np.full(len(bd), md * 1e6)]))
-        secs += time.perf_counter() - t0
-        for q, j in enumerate(jobs):
+        t2 = time.perf_counter()
+        t_asm += t1 - t0
+        t_csr += t2 - t1
+        for j in jobs:
             b = np.zeros(j.n_dof)
             np.add.at(b, j.steps[0].load_dof, j.steps[0].load_val)
-            if iters is not None and k in iters:
-                t_it = O.lib().fxo_pcg_time_iters(K.n, O._p(K.row_ptr), O._p(K.col), O._p(K.val), O._p(b),
-                                                  sample_iters, threads) / sample_iters
-                secs += t_it * iters[k][q]
-            else:
-                t0 = time.perf_counter()
-                O.pcg(K, b, 1e-10, 100000, 1, threads)
-                secs += time.perf_counter() - t0
+            for t in legs:
+                sec, it = O.pcg_par_time(K, b, 1e-10, 1, 100_000, t)
+                if sec < 0:
+                    raise RuntimeError(f"oracle PCG did not converge on system {k}")
+                t_solve[t] += sec
+            iters += it
             dofs += j.n_dof
         names.append(f"{sp['n_dof']}x{len(jobs)}")
-    how = (f"PCG timed over {sample_iters} iterations per load case and extrapolated to the FP64 solve's iteration "
-           f"count" if iters is not None else "PCG run in full")
-    return {"value": dofs / secs, "unit": "DOF-solves/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle C restatement on the host over 16 systems of the population (every 32nd: "
-                       f"{', '.join(names)} DOF x load cases): serial assembly + triplets->CSR timed in full, "
-                       f"Jacobi-PCG to rel 1e-10 on {threads} OpenMP threads, {how}; {dofs} DOF-solves in "
-                       f"{secs:.1f} s of modelled CPU time ({time.perf_counter() - t_all:.1f} s measured)")}
+    best = min(legs, key=lambda t: t_solve[t])
+    per_leg = {str(t): {"threads": t, "dof_solves_per_s": dofs / (t_asm + t_csr + t_solve[t]),
+                        "seconds_solve": round(t_solve[t], 2)} for t in legs}
+    return {"value": dofs / (t_asm + t_csr + t_solve[best]), "unit": "DOF-solves/s", "cores": best,
+            "kind": "port",
+            "sample": (f"oracle C restatement of the reference CPU path on the host, {len(names)} systems of the "
+                       f"population (every {stride}th: {', '.join(names)} DOF x load cases, {dofs} DOF-solves): "
+                       f"serial element loop + triplets ({t_asm:.2f} s), triplets->CSR ({t_csr:.2f} s), then "
+                       f"every load case's Jacobi-PCG to rel 1e-10 run in full ({iters} iterations) on "
+                       + ", ".join(f"{t} threads ({t_solve[t]:.1f} s)" for t in legs)
+                       + f"; value is the {best}-thread leg ({time.perf_counter() - t_all:.1f} s measured)"),
+            "phases_s": {"assembly": round(t_asm, 3), "csr": round(t_csr, 3),
+                         "solve": {str(t): round(t_solve[t], 3) for t in legs}},
+            "legs": per_leg, "host": host_info()}
 
 
-def oracle_sample_check(systems, results, arms, n_sample):
-    """Part of the host leg: the oracle's direct solves of the smallest
-    `n_sample` systems (its matrix, factorised on the host) as the checker of
-    every arm's solutions; sets u_rel_err_vs_oracle_sample_max per arm."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-
-    small = sorted(systems, key=lambda s: s.n_dof)[:n_sample]
-    refs = {}
-    for s in small:
-        st = s.jobs[0].steps[0]
-        csr, _, _ = O.assemble_csr(s.jobs[0].mesh, s.jobs[0].node_temp, None, st.bc_dof)
-        refs[s.key] = [O.direct_solve(csr, s.rhs[k]) for k in range(s.n_rhs)]
+def oracle_hard_check(results, arms):
+    """Part of the host leg: every arm's solutions of the population's
+    hardest members (the largest, the most slender, nu near 0.45, and
+    member 216) against the oracle's direct solves committed in
+    tests/golden/population_hard.npz (every 32nd entry of each solution,
+    plus its L2 norm); sets u_rel_err_vs_oracle_hard_max per arm."""
+    base = os.path.join(ROOT, "tests", "golden", "population_hard")
+    if not os.path.exists(base + ".npz"):
+        return
+    meta = json.load(open(base + ".json"))
+    gold = np.load(base + ".npz")
     for arm, r in results.items():
-        e = [np.linalg.norm(r.x[key][k] - xd) / np.linalg.norm(xd)
-             for key, xds in refs.items() if key in r.x for k, xd in enumerate(xds)]
-        arms[arm]["u_rel_err_vs_oracle_sample_max"] = float(max(e)) if e else None
+        errs = []
+        for m in meta:
+            key = m["member"]
+            if key not in r.x:
+                continue
+            sub = gold[f"m{key}_sub"]
+            for q in range(sub.shape[0]):
+                errs.append(float(np.linalg.norm(r.x[key][q][::32] - sub[q]) / np.linalg.norm(sub[q])))
+        arms[arm]["u_rel_err_vs_oracle_hard_max"] = max(errs) if errs else None
+        arms[arm]["oracle_hard_members"] = len(errs)
 
 
 def counter_traffic(name, bytes_total, bytes_per_launch):
@@ -333,6 +390,17 @@ DOMINANT_KERNEL = {"mixed_poly": "kp_cheb_sell", "mixed_batched": "kb_spmv_sell<
                    "fp64_batched": "kb_spmv_sell<double, double>"}
 COUNTER_SUMMARY = {"mixed_poly": "kp_cheb_sell_batch512_counters.md",
                    "mixed_batched": "kb_spmv_sell_float_batch512_counters.md"}
+
+
+# the arithmetic of each batched arm
+DTYPE = {"mixed_poly": "f64 (inner Chebyshev steps on an f16-stored K, f64 iterates)",
+         "mixed_batched": "fp32-stored K / fp64", "fp64_batched": "f64", "fp64_serial": "f64"}
+
+
+def ratio(arms, x, y):
+    if x in arms and y in arms:
+        return round(arms[x]["dof_solves_per_s"] / arms[y]["dof_solves_per_s"], 3)
+    return None
 
 
 def run_batch(a):
@@ -435,11 +503,10 @@ def run_batch(a):
                 "launches": st_["launches"]}
     print(f"# roofline {json.dumps(roof)}", flush=True)
     cpu = None
-    if not a.no_cpu_baseline:  # the host leg: the oracle times the CPU path and checks a sample
-        ref = results.get("fp64_serial")
-        cpu = cpu_baseline_batch(specs, ref.iters_rhs if ref is not None else None)
-        if not a.no_oracle_check:
-            oracle_sample_check(systems, results, arms, a.oracle_sample)
+    if not a.no_oracle_check:  # the checker: the oracle's direct solves of the hardest members
+        oracle_hard_check(results, arms)
+    if not a.no_cpu_baseline:  # the host leg: the oracle times the CPU path
+        cpu = cpu_baseline_batch(specs)
     if cpu:
         print(f"# cpu {json.dumps(cpu)}", flush=True)
     h = arms[head]
@@ -454,7 +521,7 @@ def run_batch(a):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-stored K / fp64" if head == "mixed_batched" else "f64",
+        "dtype": DTYPE.get(head, "f64"),
         "data": "synthetic",
         "config": {"workload": f"batched_linear_static_c3d4_{len(systems)}_meshes_2k-200k_dof_1-4_load_cases_rel1e-10",
                    "n_meshes": len(systems), "n_dof": n_dof, "dof_solves_per_step": n_solves,
@@ -463,11 +530,11 @@ def run_batch(a):
         "p50_ms": round(h["p50_ms"], 3),
         "p99_ms": round(h["p99_ms"], 3),
         "arms": arms,
-        "mixed_over_fp64": (round(arms["mixed_batched"]["dof_solves_per_s"] / arms["fp64_serial"]["dof_solves_per_s"],
-                                  3) if "mixed_batched" in arms and "fp64_serial" in arms else None),
-        "mixed_over_fp64_batched": (round(arms["mixed_batched"]["dof_solves_per_s"] /
-                                          arms["fp64_batched"]["dof_solves_per_s"], 3)
-                                    if "mixed_batched" in arms and "fp64_batched" in arms else None),
+        "mixed_arm": head,
+        "mixed_over_fp64": ratio(arms, head, "fp64_serial"),
+        "mixed_over_fp64_batched": ratio(arms, head, "fp64_batched"),
+        "ratios": {f"{x}_over_{y}": ratio(arms, x, y) for x in arms for y in ("fp64_serial", "fp64_batched")
+                   if x != y and y in arms},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -569,7 +636,6 @@ def main():
     ap.add_argument("--admit-order", choices=["given", "largest"], default="given",
                     help="batched arms: admission order (largest: descending DOF x load cases)")
     ap.add_argument("--spmv-group", type=int, default=0, help="lanes per node row of the batched SpMV (0: auto)")
-    ap.add_argument("--oracle-sample", type=int, default=4)
     ap.add_argument("--no-oracle-check", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=100)
@@ -607,7 +673,7 @@ def main():
     else:
         a.steps = 1 if a.steps is None else a.steps
         a.warmup = 1 if a.warmup is None else a.warmup
-        a.arms = a.arms or "fp64_serial,mixed_batched,fp64_batched"
+        a.arms = a.arms or "fp64_serial,fp64_batched,mixed_batched,mixed_poly"
         run_batch(a)
 
 

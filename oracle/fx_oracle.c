@@ -861,6 +861,114 @@ int fxo_pcg(int64_t n, const int64_t *rp, const int32_t *ci, const double *v, co
     return rc;
 }
 
+/*
+ * The same PCG with the reference's parallel structure, for the CPU
+ * baseline's timing (iterative.rs:26-121 on a rayon pool): the SpMV
+ * (iterative.rs:125-136), the three dot products, the x/r and p updates and
+ * the Jacobi apply (preconditioner.rs:53-58) are parallel loops, and each
+ * iteration allocates K*p zeroed and the preconditioned residual afresh, as
+ * the reference's `vec![0.0; n]` and `collect()` do.  Dot products are OpenMP
+ * reductions, so their summation order depends on the thread count (rayon's
+ * does too); the parity tests use fxo_pcg.
+ */
+static double dot_par(int64_t n, const double *a, const double *b)
+{
+    double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+    for (int64_t i = 0; i < n; ++i)
+        s += a[i] * b[i];
+    return s;
+}
+
+int fxo_pcg_par(int64_t n, const int64_t *rp, const int32_t *ci, const double *v, const double *b, double *x,
+                double tol, int64_t max_iter, int stop_mode, int nthreads, int64_t *iters, double *final_res)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#endif
+    double *r = malloc(sizeof(double) * (size_t)n), *z = malloc(sizeof(double) * (size_t)n);
+    double *p = malloc(sizeof(double) * (size_t)n), *md = malloc(sizeof(double) * (size_t)n);
+    if (!r || !z || !p || !md) {
+        free(r); free(z); free(p); free(md);
+        return FXO_ERR_OOM;
+    }
+    fxo_jacobi(n, rp, ci, v, md);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = 0.0;
+        r[i] = b[i];
+        z[i] = r[i] * md[i];
+        p[i] = z[i];
+    }
+    double bnorm = sqrt(dot_par(n, b, b));
+    double rs_old = dot_par(n, r, z);
+    int rc = FXO_ERR_NOT_CONVERGED;
+    int64_t it;
+    for (it = 0; it < max_iter; ++it) {
+        double crit = stop_mode == 0 ? sqrt(rs_old) : (bnorm > 0.0 ? sqrt(dot_par(n, r, r)) / bnorm : 0.0);
+        if (stop_mode == 0 ? crit < tol : crit <= tol) {
+            rc = FXO_OK;
+            if (final_res)
+                *final_res = crit;
+            break;
+        }
+        double *ap = calloc((size_t)n, sizeof(double));
+        if (!ap) {
+            rc = FXO_ERR_OOM;
+            break;
+        }
+        spmv(n, rp, ci, v, p, ap);
+        double alpha = rs_old / dot_par(n, p, ap);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i)
+            x[i] += alpha * p[i];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i)
+            r[i] -= alpha * ap[i];
+        free(ap);
+        free(z);
+        z = malloc(sizeof(double) * (size_t)n);
+        if (!z) {
+            rc = FXO_ERR_OOM;
+            break;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i)
+            z[i] = r[i] * md[i];
+        double rs_new = dot_par(n, r, z);
+        double beta = rs_new / rs_old;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i)
+            p[i] = z[i] + beta * p[i];
+        rs_old = rs_new;
+    }
+    if (rc == FXO_ERR_NOT_CONVERGED && final_res)
+        *final_res = stop_mode == 0 ? sqrt(rs_old) : sqrt(dot_par(n, r, r)) / (bnorm > 0.0 ? bnorm : 1.0);
+    *iters = it;
+    free(r); free(z); free(p); free(md);
+    return rc;
+}
+
+/* Wall seconds of fxo_pcg_par to `tol` (or for `count` iterations when
+ * tol is 0): the CPU baseline's solve phase. */
+double fxo_pcg_par_time(int64_t n, const int64_t *rp, const int32_t *ci, const double *v, const double *b,
+                        double tol, int stop_mode, int64_t count, int nthreads, int64_t *iters)
+{
+    double *x = malloc(sizeof(double) * (size_t)n);
+    double fr, t0 = 0.0, t1 = 0.0;
+    *iters = 0;
+#ifdef _OPENMP
+    t0 = omp_get_wtime();
+#endif
+    int rc = fxo_pcg_par(n, rp, ci, v, b, x, tol, count, stop_mode, nthreads, iters, &fr);
+#ifdef _OPENMP
+    t1 = omp_get_wtime();
+#endif
+    free(x);
+    return rc == FXO_OK || (rc == FXO_ERR_NOT_CONVERGED && tol == 0.0) ? t1 - t0 : -1.0;
+}
+
 /* Timed pieces for the CPU baseline: seconds for `count` SpMV+vector
  * passes of PCG on a given system (no convergence check). */
 double fxo_pcg_time_iters(int64_t n, const int64_t *rp, const int32_t *ci, const double *v, const double *b,

@@ -62,6 +62,9 @@ def lib():
         L.fxo_jacobi.argtypes = [i64, P, P, P, P]
         L.fxo_pcg_time_iters.argtypes = [i64, P, P, P, P, i64, i32]
         L.fxo_pcg_time_iters.restype = dbl
+        L.fxo_pcg_par.argtypes = [i64, P, P, P, P, P, dbl, i64, i32, i32, P, P]
+        L.fxo_pcg_par_time.argtypes = [i64, P, P, P, P, dbl, i32, i64, i32, P]
+        L.fxo_pcg_par_time.restype = dbl
         L.fxo_lut_interp.argtypes = [i32, P, P, dbl]
         L.fxo_lut_interp.restype = dbl
         L.fxo_shape_c3d20.argtypes = [dbl, dbl, dbl, P, P]
@@ -276,6 +279,29 @@ def pcg(K: CSR, b, tol=1e-8, max_iter=1000, stop_mode=0, nthreads=0):
     rc = lib().fxo_pcg(K.n, _p(K.row_ptr), _p(K.col), _p(K.val), _p(b), _p(x), tol, max_iter, stop_mode,
                        nthreads, _p(it), _p(fr))
     return x, int(it[0]), float(fr[0]), rc == OK
+
+
+def pcg_par(K: CSR, b, tol=1e-8, max_iter=1000, stop_mode=0, nthreads=0):
+    """The same PCG with the reference's rayon structure (parallel dots,
+    updates and Jacobi apply, per-iteration allocations): the CPU baseline's
+    solver.  Returns (x, iterations, final criterion, converged)."""
+    x = np.zeros(K.n)
+    it = np.zeros(1, np.int64)
+    fr = np.zeros(1, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    rc = lib().fxo_pcg_par(K.n, _p(K.row_ptr), _p(K.col), _p(K.val), _p(b), _p(x), tol, max_iter, stop_mode,
+                           nthreads, _p(it), _p(fr))
+    return x, int(it[0]), float(fr[0]), rc == OK
+
+
+def pcg_par_time(K: CSR, b, tol, stop_mode, max_iter, nthreads):
+    """Wall seconds and iterations of pcg_par to `tol` (tol = 0: exactly
+    `max_iter` iterations); seconds < 0 when it did not converge."""
+    it = np.zeros(1, np.int64)
+    b = np.ascontiguousarray(b, np.float64)
+    sec = lib().fxo_pcg_par_time(K.n, _p(K.row_ptr), _p(K.col), _p(K.val), _p(b), tol, stop_mode, max_iter,
+                                 nthreads, _p(it))
+    return float(sec), int(it[0])
 
 
 def spmv(K: CSR, x, nthreads=0):

@@ -1,0 +1,78 @@
+"""Accuracy of the batched solves on the hardest members of the 512-mesh
+population (seed 4): the two largest (about 199k DOF), the most slender
+(aspect near 10), the ones with nu near 0.45, and member 216 whose mixed
+solve once stalled.
+
+The fixture tests/golden/population_hard.npz holds the oracle's direct
+solves of these members (SuperLU on the oracle's penalised CSR, the stand-in
+for the reference's DIRECT solver): every 32nd displacement entry, the full
+vector's L2 norm and its max |u|.
+
+Bar, per load case and for every batched precision (FP64, mixed with
+reliable updates, mixed polynomial): relative L2 error of the strided
+subset <= 1e-8, and the full vector's L2 norm and max |u| within 1e-8
+relative of the oracle's.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ferrix_amd import _lib as L
+from ferrix_amd import meshgen
+from ferrix_amd.device import Batch, SolveOptions
+
+from test_gpu_batch import system
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "population_hard")
+STRIDE = 32
+
+OPTS = {
+    "fp64": SolveOptions(precision=L.FX_PREC_FP64, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000),
+    "mixed": SolveOptions(precision=L.FX_PREC_MIXED, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000,
+                          inner_tol=1e-3, max_refine=100),
+    "mixed_poly": SolveOptions(precision=L.FX_PREC_MIXED_POLY, stop_mode=L.FX_STOP_REL_R2, tol=1e-10,
+                               max_iter=400_000),
+}
+
+
+@pytest.fixture(scope="module")
+def hard(ctx):
+    meta = json.load(open(GOLD + ".json"))
+    gold = np.load(GOLD + ".npz")
+    specs = meshgen.cfg4_batch_specs()
+    out = []
+    for m in meta:
+        k = m["member"]
+        jobs = meshgen.cfg4_mesh(specs[k])
+        assert jobs[0].n_dof == m["n_dof"] and len(jobs) == m["n_rhs"]
+        dm, b = system(ctx, jobs)
+        out.append((k, dm, b, gold[f"m{k}_sub"], gold[f"m{k}_norm"], gold[f"m{k}_maxabs"]))
+    return out
+
+
+@pytest.mark.parametrize("prec", list(OPTS))
+def test_hard_members_match_oracle_direct(ctx, hard, prec):
+    bt = Batch(ctx, OPTS[prec])
+    slots = [bt.add(dm, b) for _, dm, b, *_ in hard]
+    bt.run()
+    worst = 0.0
+    for (k, dm, b, sub, nrm, mx), s in zip(hard, slots):
+        st, rep = bt.poll(s)
+        assert st == 1, f"member {k}: status {st}"
+        x = bt.result(s)
+        assert x.shape == (b.shape[0], b.shape[1])
+        for r in range(x.shape[0]):
+            e_sub = float(np.linalg.norm(x[r, ::STRIDE] - sub[r]) / np.linalg.norm(sub[r]))
+            e_nrm = abs(float(np.linalg.norm(x[r])) - float(nrm[r])) / float(nrm[r])
+            e_max = abs(float(np.max(np.abs(x[r]))) - float(mx[r])) / float(mx[r])
+            print(f"{prec} member {k} load case {r}: n_dof {x.shape[1]} iterations {rep.iterations} "
+                  f"rel err (strided) {e_sub:.2e} norm {e_nrm:.2e} max {e_max:.2e}", flush=True)
+            assert e_sub <= 1e-8 and e_nrm <= 1e-8 and e_max <= 1e-8, (k, r)
+            worst = max(worst, e_sub)
+    bt.close()
+    print(f"{prec}: worst strided rel err {worst:.2e}", flush=True)

@@ -283,6 +283,34 @@ def test_pcg_reports_not_converged():
     assert not ok and it == 5
 
 
+def test_pcg_par_is_the_same_algorithm():
+    """The CPU baseline's PCG (parallel loops, per-iteration allocations as
+    in iterative.rs) on one thread is bit for bit fxo_pcg (same sums in the
+    same order); on several threads only the dot products' association
+    changes; both stop rules and the not-converged outcome agree."""
+    job = meshgen.cfg1_c3d4_cantilever()
+    st = job.steps[0]
+    csr, _, _ = O.assemble_csr(job.mesh, job.node_temp, None, st.bc_dof)
+    b = np.zeros(job.n_dof)
+    np.add.at(b, st.load_dof, st.load_val)
+    for mode, tol in ((0, 1e-8), (1, 1e-10)):
+        x, it, crit, ok = O.pcg(csr, b, tol, 100000, mode, 1)
+        x1, it1, crit1, ok1 = O.pcg_par(csr, b, tol, 100000, mode, 1)
+        assert ok and ok1 and it == it1 and crit == crit1
+        assert np.array_equal(x, x1)
+        x4, it4, _, ok4 = O.pcg_par(csr, b, tol, 100000, mode, 4)
+        assert ok4 and abs(it4 - it) <= 2
+        assert np.linalg.norm(x4 - x) / np.linalg.norm(x) < 1e-9
+    _, it5, _, ok5 = O.pcg_par(csr, b, 1e-8, 5, 0, 2)
+    assert not ok5 and it5 == 5
+    sec, its = O.pcg_par_time(csr, b, 1e-10, 1, 100000, 2)
+    assert sec > 0 and abs(its - it) <= 2
+    sec, its = O.pcg_par_time(csr, b, 1e-10, 1, 3, 2)
+    assert sec < 0 and its == 3  # not converged within the limit
+    sec, its = O.pcg_par_time(csr, b, 0.0, 0, 7, 2)
+    assert sec > 0 and its == 7  # tol 0: a fixed iteration count
+
+
 def test_jacobi_guard():
     csr = O.CSR(3, np.array([0, 1, 2, 2], np.int64), np.array([0, 1], np.int32), np.array([4.0, 1e-12]))
     assert np.array_equal(O.jacobi(csr), [0.25, 1.0, 1.0])
