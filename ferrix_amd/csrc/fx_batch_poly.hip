@@ -337,9 +337,8 @@ __device__ __forceinline__ void stage_out(const WaveRows &w, const double *tile,
 // Entries of load cases that do not run are written as well (their values
 // are never read).
 template <int RR>
-__device__ __forceinline__ void cheb_sell(const BSys &S, int lb, int j, int go) {
+__device__ __forceinline__ void cheb_sell(const BSys &S, int lb, int j, int go, double *tile) {
     constexpr int W = 3 * RR;  // entries per node row
-    __shared__ double tile[kBB * 3 * kRmax];
     const int64_t a = (int64_t)lb * kBB + threadIdx.x;
     gptr<const double> pv = gp((const double *)S.zq[j % 3]);
     if (a < S.n_nodes) {
@@ -375,11 +374,14 @@ __global__ __launch_bounds__(kBB) void kp_cheb_sell(BSys *__restrict__ sys, cons
     const int go = S.go;
     if (!go)
         return;
+    // one tile for every load-case count (a __shared__ array per template
+    // instance would reserve four, and LDS would then hold one block per CU)
+    __shared__ double tile[kBB * 3 * kRmax];
     switch (S.R) {
-    case 1: cheb_sell<1>(S, t.y, j, go); break;
-    case 2: cheb_sell<2>(S, t.y, j, go); break;
-    case 3: cheb_sell<3>(S, t.y, j, go); break;
-    default: cheb_sell<4>(S, t.y, j, go); break;
+    case 1: cheb_sell<1>(S, t.y, j, go, tile); break;
+    case 2: cheb_sell<2>(S, t.y, j, go, tile); break;
+    case 3: cheb_sell<3>(S, t.y, j, go, tile); break;
+    default: cheb_sell<4>(S, t.y, j, go, tile); break;
     }
 }
 
@@ -478,13 +480,12 @@ __device__ __forceinline__ void outer_row(const BSys &S, int64_t a, int lane, in
 }
 
 template <int RR, bool SELL>
-__device__ __forceinline__ void outer_block(const BSys &S, int lb, int go) {
+__device__ __forceinline__ void outer_block(const BSys &S, int lb, int go, double *tile) {
     double v[5 * kRmax];
 #pragma unroll
     for (int k = 0; k < 5 * kRmax; ++k)
         v[k] = 0.0;
     if constexpr (SELL) {  // w leaves through LDS as lane-consecutive stores (see cheb_sell)
-        __shared__ double tile[kBB * 3 * kRmax];
         const int64_t a = (int64_t)lb * kBB + threadIdx.x;
         if (a < S.n_nodes) {
             gptr<const double> zv = gp((const double *)S.zq[(S.deg + 1) % 3]);
@@ -512,11 +513,12 @@ __global__ __launch_bounds__(kBB) void kp_outer(BSys *__restrict__ sys, const in
     const int go = S.go;
     if (!go)
         return;
+    __shared__ double tile[SELL ? kBB * 3 * kRmax : 1];  // one tile for every load-case count
     switch (S.R) {
-    case 1: outer_block<1, SELL>(S, t.y, go); break;
-    case 2: outer_block<2, SELL>(S, t.y, go); break;
-    case 3: outer_block<3, SELL>(S, t.y, go); break;
-    default: outer_block<4, SELL>(S, t.y, go); break;
+    case 1: outer_block<1, SELL>(S, t.y, go, tile); break;
+    case 2: outer_block<2, SELL>(S, t.y, go, tile); break;
+    case 3: outer_block<3, SELL>(S, t.y, go, tile); break;
+    default: outer_block<4, SELL>(S, t.y, go, tile); break;
     }
 }
 
