@@ -69,7 +69,7 @@ class fx_timer(C.Structure):
 class fx_step_report(C.Structure):
     _fields_ = [("increments", i32), ("cutbacks", i32), ("nr_iterations", i32), ("linear_solves", i32),
                 ("pcg_iterations", i64), ("outer_spmv", i64), ("last_rel_residual", dbl), ("seconds", dbl),
-                ("seconds_assembly", dbl), ("seconds_solve", dbl)]
+                ("seconds_assembly", dbl), ("seconds_solve", dbl), ("host_syncs", i64)]
 
 
 # every symbol the header declares, with its signature

@@ -4,6 +4,7 @@
 // element-to-slot contribution lists), the C entry points and the step
 // driver restated from src/solver/step/static_step.rs:47-267.
 #include "fx_internal.h"
+#include "fx_newton.h"
 
 #include <chrono>
 #include <climits>
@@ -11,6 +12,10 @@
 #include <exception>
 #include <new>
 #include <numeric>
+
+namespace fx {
+thread_local int64_t t_host_syncs = 0;
+}
 
 using fx::DBuf;
 using fx::Error;
@@ -689,6 +694,53 @@ void internal_force_dev(fx_mesh *m, bool nlgeom) {
     fx::launch_gather_f(m);
 }
 
+bool env_on(const char *name) {
+    const char *e = std::getenv(name);
+    return !(e && e[0] == '0');
+}
+
+// three timing events of a Newton iteration (front start, front end, queued solve end)
+struct Events3 {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    Events3() {
+        for (auto &x : e)
+            FX_HIP(hipEventCreate(&x));
+    }
+    ~Events3() {
+        for (auto &x : e)
+            if (x)
+                (void)hipEventDestroy(x);
+    }
+};
+
+// the element failure message of check_status
+std::string element_error(const char *what, int code, int64_t bad_element) {
+    std::string msg = std::string(what) + ": ";
+    if (code == FX_ERR_SINGULAR_JACOBIAN)
+        msg += "Singular Jacobian";
+    else
+        msg += "element has no Young's modulus or Poisson's ratio at its temperature";
+    return msg + " (element position " + std::to_string(bad_element) + ")";
+}
+
+// f_ext at the timer's time: amplitude-scaled loads summed per DOF in deck
+// order (static_step.rs:111-126), scattered into a zeroed m->f_ext
+void set_fext(fx_mesh *m, const fx_step_def *sd, const fx_timer &timer, int step_id,
+              const std::vector<int32_t> &touched, std::vector<double> &fext_h, std::vector<double> &tval,
+              const fx::DBuf<int32_t> &d_touched, fx::DBuf<double> &d_tval) {
+    hipStream_t s = m->ctx->stream;
+    for (int32_t d : touched)
+        fext_h[d] = 0.0;
+    for (int64_t k = 0; k < sd->n_loads; ++k)
+        fext_h[sd->load_dof[k]] += amp_y(sd->amps, sd->load_amp[k], timer, sd->load_origin[k], step_id) * sd->load_val[k];
+    for (size_t k = 0; k < touched.size(); ++k)
+        tval[k] = fext_h[touched[k]];
+    if (!touched.empty())
+        FX_HIP(hipMemcpyAsync(d_tval.p, tval.data(), tval.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    fx::launch_fill(m, m->f_ext.p, m->n_dof, 0.0);
+    fx::launch_scatter(m, (int64_t)touched.size(), d_touched.p, d_tval.p, m->f_ext.p);
+}
+
 // BC records grouped per DOF in first-appearance order, records in deck order
 struct BcGroups {
     std::vector<int32_t> ud_dof, ud_ptr, rec_order;
@@ -1024,6 +1076,19 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
     norms.alloc(2);
     std::vector<double> tval(touched.size());
     FX_HIP(hipMemcpyAsync(u_start.p, m->u.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // The Newton iteration on the device (fx_newton.hip) unless FX_NR_DEVICE=0;
+    // its front as a replayed graph unless FX_NR_GRAPH=0 (plain launches)
+    const bool dev_nr = env_on("FX_NR_DEVICE"), use_graph = env_on("FX_NR_GRAPH");
+    const bool mixed = opts.precision == FX_PREC_MIXED;
+    fx::NewtonGraph ng(m);
+    Events3 ev;
+    bool nr_graph_ready = false;
+    // SpMVs of the FP64 solves by NR index: the previous increment's and this one's
+    std::vector<int64_t> iters_prev, iters_cur;
+    const int G = fx::pcg_group(m, opts);
+    if (dev_nr && !mixed)
+        fx::pcg_prepare(m);
+    const int64_t syncs0 = fx::t_host_syncs;
 
     double current_time = 0.0;
     int64_t n_inc = 0;
@@ -1040,22 +1105,107 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
         rep.increments++;
         FX_HIP(hipMemcpyAsync(m->u.p, u_start.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         bool converged = false;
+        if (dev_nr) {
+            // The device-resident Newton iteration (fx_newton.hip).  The
+            // loads and BC targets depend only on the increment's time:
+            // formed once here (the host loop forms the same values every
+            // iteration).
+            set_fext(m, sd, *timer, step_id, touched, fext_h, tval, d_touched, d_tval);
+            if (sd->n_bcs > 0) {
+                for (int64_t k = 0; k < sd->n_bcs; ++k)
+                    targets[k] = amp_y(sd->amps, sd->bc_amp[k], *timer, sd->bc_origin[k], step_id) * sd->bc_val[k];
+                FX_HIP(hipMemcpyAsync(dbc.target.p, targets.data(), targets.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, s));
+            }
+            double rel_prev = -1.0;
+            iters_prev.swap(iters_cur);
+            iters_cur.clear();
+            for (int it = 0; it < 15; ++it) {
+                rep.nr_iterations++;
+                fx::NewtonGraph::FrontArgs fa{nr_graph_ready, nlgeom, !mixed, &opts, dmask.p, dbc.n_unique,
+                                              dbc.ud_dof.p, dbc.ud_ptr.p, dbc.rec_order.p, dbc.target.p};
+                FX_HIP(hipEventRecord(ev.e[0], s));
+                ng.launch_front(fa);
+                nr_graph_ready = use_graph;  // the first iteration runs plain launches, the rest the graph
+                FX_HIP(hipEventRecord(ev.e[1], s));
+                // the solve's iterations, queued without a status read: 15/16
+                // of what the solve at this NR index of the previous
+                // increment needed (else the last solve; 64 for the step's
+                // first), none when this iteration is expected to converge
+                // (a linear step after its first solve; NLGEOM when the
+                // residual's last reduction carried on would pass 1e-3).
+                // Launches past the stop rule return at once but are not
+                // free (about a tenth of a wait each), so the queue stops a
+                // little short and the chunked finish reads the rest.
+                const bool expect_conv = it > 0 && (!nlgeom || (rel_prev > 0.0 && rep.last_rel_residual *
+                                                                   (rep.last_rel_residual / rel_prev) < 1e-3));
+                if (!mixed && !expect_conv) {
+                    const int64_t pred = (size_t)it < iters_prev.size() ? iters_prev[it]
+                                         : !iters_cur.empty()            ? iters_cur.back()
+                                                                         : 48;
+                    fx::pcg_queue_fp64(m, G, pred - pred / 16);
+                }
+                ng.launch_post(!mixed);
+                FX_HIP(hipEventRecord(ev.e[2], s));
+                const fx::NrReport &R = ng.wait();
+                float ms_front = 0.f, ms_queued = 0.f;
+                if (R.dev_code != 0)
+                    throw Error{(fx_status)R.dev_code, element_error("Assembler::assemble_internal_force",
+                                                                     R.dev_code, R.bad_element)};
+                rel_prev = it > 0 ? rep.last_rel_residual : -1.0;
+                rep.last_rel_residual = R.rel;
+                FX_HIP(hipEventElapsedTime(&ms_front, ev.e[0], ev.e[1]));
+                FX_HIP(hipEventElapsedTime(&ms_queued, ev.e[1], ev.e[2]));
+                rep.seconds_assembly += 1e-3 * ms_front;
+                if (R.skip) {
+                    converged = true;
+                    break;
+                }
+                memcpy(&m->last_max_diag, &R.max_diag_bits, sizeof(double));
+                m->k_valid = true;
+                fx_solve_report sr{};
+                fx_status st = FX_OK;
+                auto t_sol = clk::now();
+                if (mixed) {
+                    st = fx::pcg_solve(m, m->rhs.p, m->du.p, opts, &sr);
+                    if (st == FX_OK)
+                        fx::axpy_dev(m, n, 1.0, m->du.p, m->u.p);
+                } else {
+                    fx::PcgScalars h = R.pcg;
+                    int code = h.status;
+                    if (code == 0) {  // still running: the adaptive chunked finish, then du = x, u += du
+                        code = fx::pcg_continue_fp64(m, opts.tol, opts.max_iter, G, &h);
+                        if (code == 1) {
+                            FX_HIP(hipMemcpyAsync(m->du.p, m->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+                            fx::axpy_dev(m, n, 1.0, m->du.p, m->u.p);
+                        }
+                    }
+                    sr.iterations = fx::pcg_iterations_done(h);
+                    sr.final_criterion = h.crit;
+                    sr.spmv_group = G;
+                    st = code == 2 ? FX_ERR_BREAKDOWN : (code == 3 ? FX_ERR_NOT_CONVERGED : FX_OK);
+                    iters_cur.push_back(sr.iterations);
+                }
+                rep.seconds_solve += 1e-3 * ms_queued + since(t_sol);
+                rep.linear_solves++;
+                rep.pcg_iterations += sr.iterations;
+                rep.outer_spmv += sr.outer_spmv;
+                if (st != FX_OK) {
+                    if (report) {
+                        rep.host_syncs = fx::t_host_syncs - syncs0;
+                        rep.host_syncs = fx::t_host_syncs - syncs0;
+                    rep.seconds = since(t_start);
+                        *report = rep;
+                    }
+                    throw Error{st, solve_error(st, opts, sr)};
+                }
+            }
+        } else
         for (int it = 0; it < 15; ++it) {
             rep.nr_iterations++;
             auto t_asm = clk::now();
             // 1. external force from amplitude-scaled loads, deck order
-            for (int32_t d : touched)
-                fext_h[d] = 0.0;
-            for (int64_t k = 0; k < sd->n_loads; ++k)
-                fext_h[sd->load_dof[k]] +=
-                    amp_y(sd->amps, sd->load_amp[k], *timer, sd->load_origin[k], step_id) * sd->load_val[k];
-            for (size_t k = 0; k < touched.size(); ++k)
-                tval[k] = fext_h[touched[k]];
-            if (!touched.empty())
-                FX_HIP(hipMemcpyAsync(d_tval.p, tval.data(), tval.size() * sizeof(double), hipMemcpyHostToDevice,
-                                      s));
-            fx::launch_fill(m, m->f_ext.p, n, 0.0);
-            fx::launch_scatter(m, (int64_t)touched.size(), d_touched.p, d_tval.p, m->f_ext.p);
+            set_fext(m, sd, *timer, step_id, touched, fext_h, tval, d_touched, d_tval);
             // 2. internal force at u_curr (coordinates move under NLGEOM);
             //    the thermal force is exactly zero (T never changes)
             internal_force_dev(m, nlgeom);
@@ -1084,6 +1234,7 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
                                    m->rhs.p, m->u.p);
             }
             FX_HIP(hipStreamSynchronize(s));
+            ++fx::t_host_syncs;
             rep.seconds_assembly += since(t_asm);
             // 7. solve
             auto t_sol = clk::now();
@@ -1095,6 +1246,7 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
             rep.outer_spmv += sr.outer_spmv;
             if (st != FX_OK) {
                 if (report) {
+                    rep.host_syncs = fx::t_host_syncs - syncs0;
                     rep.seconds = since(t_start);
                     *report = rep;
                 }
@@ -1119,6 +1271,7 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
                 FX_HIP(hipMemcpyAsync(m->u.p, u_start.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
                 FX_HIP(hipStreamSynchronize(s));
                 if (report) {
+                    rep.host_syncs = fx::t_host_syncs - syncs0;
                     rep.seconds = since(t_start);
                     *report = rep;
                 }
@@ -1129,6 +1282,7 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
     // solution_state.displacements = u_step_start
     FX_HIP(hipMemcpyAsync(m->u.p, u_start.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     FX_HIP(hipStreamSynchronize(s));
+    rep.host_syncs = fx::t_host_syncs - syncs0;
     rep.seconds = since(t_start);
     if (report)
         *report = rep;

@@ -34,7 +34,10 @@ __global__ __launch_bounds__(256) void k_assemble(int64_t nblk, const int32_t *_
                                                   const int32_t *__restrict__ blk_cbase,
                                                   const double *__restrict__ ke, double *__restrict__ kval,
                                                   uint8_t *__restrict__ keep,
-                                                  unsigned long long *__restrict__ max_diag_bits) {
+                                                  unsigned long long *__restrict__ max_diag_bits,
+                                                  const int32_t *__restrict__ skip) {
+    if (skip && *skip)
+        return;
     __shared__ double s_max[256 / kWave];
     const int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double dmax = 0.0;
@@ -344,13 +347,23 @@ __global__ void k_scatter(int64_t n, const int32_t *__restrict__ idx, const doub
 
 // Penalty: one lane per distinct BC DOF; its records are applied in deck
 // order: K[d,d] += penalty once per record, rhs[d] += penalty*(t - u[d]).
+// md_bits != nullptr: the factor is kPenaltyScale x the max_diag the last
+// k_assemble left there (the product the host forms otherwise), and no
+// penalty is added when that is not positive.
 __global__ void k_penalty(int64_t n_unique, const int32_t *__restrict__ ud_dof, const int32_t *__restrict__ ud_ptr,
                           const int32_t *__restrict__ rec_order, const double *__restrict__ rec_target,
                           const int32_t *__restrict__ diag_slot, double penalty, double *__restrict__ kval,
-                          uint8_t *__restrict__ keep, double *__restrict__ rhs, const double *__restrict__ u) {
+                          uint8_t *__restrict__ keep, double *__restrict__ rhs, const double *__restrict__ u,
+                          const unsigned long long *__restrict__ md_bits, const int32_t *__restrict__ skip) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_unique)
+    if (j >= n_unique || (skip && *skip))
         return;
+    if (md_bits) {
+        const double md = __longlong_as_double((long long)*md_bits);
+        if (!(md > 0.0))
+            return;
+        penalty = md * kPenaltyScale;
+    }
     const int d = ud_dof[j];
     const int slot = diag_slot[d];
     double k = kval[slot];
@@ -405,6 +418,16 @@ __global__ void k_count_keep(int64_t n, const uint8_t *__restrict__ keep, unsign
         atomicAdd(out, s[0] + s[1] + s[2] + s[3]);
 }
 
+__global__ void k_zero_u64(unsigned long long *__restrict__ p, const int32_t *__restrict__ skip) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !(skip && *skip))
+        *p = 0ull;
+}
+
+__global__ void k_reset_status(DevStatus *__restrict__ st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *st = DevStatus{0, 0, INT64_MAX};
+}
+
 __global__ void k_axpy(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
@@ -417,11 +440,12 @@ inline unsigned grid1(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 void launch_assemble(fx_mesh *m) {
     hipStream_t s = m->ctx->stream;
-    FX_HIP(hipMemsetAsync(m->max_diag_bits.p, 0, sizeof(unsigned long long), s));
+    // a kernel rather than a memset, so that a captured graph can hold it
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, m->max_diag_bits.p, m->k_skip);
     if (m->nblk > 0)
         hipLaunchKernelGGL(k_assemble, dim3(grid1(m->nblk, 256)), dim3(256), 0, s, m->nblk, m->nb_ptr.p,
                            m->nb_col.p, m->blk_row.p, m->blk_cptr.p, m->blk_cbase.p, m->ke.p, m->kval.p, m->keep.p,
-                           m->max_diag_bits.p);
+                           m->max_diag_bits.p, m->k_skip);
     FX_HIP(hipGetLastError());
 }
 
@@ -528,7 +552,23 @@ void launch_penalty(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const i
         return;
     double pen = m->last_max_diag * kPenaltyScale;
     hipLaunchKernelGGL(k_penalty, dim3(grid1(n_unique, 256)), dim3(256), 0, m->ctx->stream, n_unique, ud_dof,
-                       ud_ptr, rec_order, rec_target, m->diag_slot.p, pen, m->kval.p, m->keep.p, rhs, u);
+                       ud_ptr, rec_order, rec_target, m->diag_slot.p, pen, m->kval.p, m->keep.p, rhs, u,
+                       (const unsigned long long *)nullptr, (const int32_t *)nullptr);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_penalty_dev(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const int32_t *ud_ptr,
+                        const double *rec_target, const int32_t *rec_order, double *rhs, const double *u) {
+    if (n_unique <= 0)
+        return;
+    hipLaunchKernelGGL(k_penalty, dim3(grid1(n_unique, 256)), dim3(256), 0, m->ctx->stream, n_unique, ud_dof,
+                       ud_ptr, rec_order, rec_target, m->diag_slot.p, 0.0, m->kval.p, m->keep.p, rhs, u,
+                       (const unsigned long long *)m->max_diag_bits.p, m->k_skip);
+    FX_HIP(hipGetLastError());
+}
+
+void launch_reset_status(fx_mesh *m) {
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, m->ctx->stream, m->status.p);
     FX_HIP(hipGetLastError());
 }
 
@@ -565,6 +605,7 @@ void check_status(fx_mesh *m, const char *what) {
     DevStatus st;
     FX_HIP(hipMemcpyAsync(&st, m->status.p, sizeof st, hipMemcpyDeviceToHost, m->ctx->stream));
     FX_HIP(hipStreamSynchronize(m->ctx->stream));
+    ++t_host_syncs;
     if (st.code != 0) {
         std::string msg = std::string(what) + ": ";
         if (st.code == FX_ERR_SINGULAR_JACOBIAN)
@@ -581,6 +622,7 @@ double device_max_diag(fx_mesh *m) {
     unsigned long long bits = 0;
     FX_HIP(hipMemcpyAsync(&bits, m->max_diag_bits.p, sizeof bits, hipMemcpyDeviceToHost, m->ctx->stream));
     FX_HIP(hipStreamSynchronize(m->ctx->stream));
+    ++t_host_syncs;
     double d;
     memcpy(&d, &bits, sizeof d);
     return d;

@@ -144,9 +144,9 @@ __global__ __launch_bounds__(128) void k_c3d4(int64_t n_el, const int32_t *__res
                                               const MatLut *__restrict__ mats, const double *__restrict__ gN,
                                               const double *__restrict__ gdN, const double *__restrict__ gw,
                                               double *__restrict__ ke, double *__restrict__ fe,
-                                              DevStatus *__restrict__ st) {
+                                              DevStatus *__restrict__ st, const int32_t *__restrict__ skip) {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_el)
+    if (e >= n_el || (skip && *skip))
         return;
     double xc[12], ue[12], T[4], dN[12];
     int nd[4];
@@ -250,7 +250,9 @@ __global__ __launch_bounds__(kC20Threads) void k_c3d20(
     const int64_t *__restrict__ gpos, const double *__restrict__ xyz, const double *__restrict__ temp,
     const double *__restrict__ u, int nlgeom, const MatLut *__restrict__ mats, const double *__restrict__ gN,
     const double *__restrict__ gdN, const double *__restrict__ gw, double *__restrict__ ke,
-    double *__restrict__ fe, DevStatus *__restrict__ st) {
+    double *__restrict__ fe, DevStatus *__restrict__ st, const int32_t *__restrict__ skip) {
+    if (skip && *skip)  // the same flag for the whole grid: no barrier is skipped by part of a block
+        return;
     __shared__ double s_xc[60], s_u[60], s_T[20];
     __shared__ double s_dg[27][60];  // [g][r*20 + i]
     __shared__ double s_w[27], s_d[27][3], s_sig[27][6], s_Ji[27][9], s_J[27][9];
@@ -400,14 +402,15 @@ void launch_types(fx_mesh *m, const double *u, int nlgeom, const double *const *
         int64_t blocks = (b.n_el + threads - 1) / threads;
         hipLaunchKernelGGL((k_c3d4<K, F>), dim3((unsigned)blocks), dim3(threads), 0, s, b.n_el, b.conn.p, b.mat.p,
                            b.gpos.p, m->xyz.p, m->temp.p, u, nlgeom, m->mats.p, tab4[0], tab4[1], tab4[2],
-                           m->ke.p + b.ke_offset, m->fe.p + b.fe_offset, m->status.p);
+                           m->ke.p + b.ke_offset, m->fe.p + b.fe_offset, m->status.p, K ? m->k_skip : nullptr);
         FX_HIP(hipGetLastError());
     }
     if (m->blk20.n_el) {
         const ElemBlock &b = m->blk20;
         hipLaunchKernelGGL((k_c3d20<K, F>), dim3((unsigned)b.n_el), dim3(kC20Threads), 0, s, b.n_el, b.conn.p,
                            b.mat.p, b.gpos.p, m->xyz.p, m->temp.p, u, nlgeom, m->mats.p, tab20[0], tab20[1],
-                           tab20[2], m->ke.p + b.ke_offset, m->fe.p + b.fe_offset, m->status.p);
+                           tab20[2], m->ke.p + b.ke_offset, m->fe.p + b.fe_offset, m->status.p,
+                           K ? m->k_skip : nullptr);
         FX_HIP(hipGetLastError());
     }
 }

@@ -21,6 +21,9 @@ struct fx_mesh;
 
 namespace fx {
 constexpr int kSliceRows = 64;  // node rows per slice of the sliced-ELL operator (one wavefront)
+// Host waits on the device (stream synchronisations) made by this thread;
+// fx_static_step reports the count of a step
+extern thread_local int64_t t_host_syncs;
 }
 
 namespace fx {
@@ -269,6 +272,10 @@ struct fx_mesh {
     fx::DBuf<double> se;                 // stress recovery: 12 values per (element, node)
     fx::DBuf<double> red2;               // 2 doubles: norm results
     double last_max_diag = 0.0;
+    // device flag that, when set, makes the K assembly (element K, max_diag
+    // reset, gather, device-side penalty) return at once: the converged
+    // Newton iteration's decision (fx_newton.hip); nullptr outside it
+    const int32_t *k_skip = nullptr;
     bool k_valid = false;
     int spmv_group = 0;                  // lanes per node row from fx_mesh_autotune_spmv (0: heuristic)
     int64_t device_bytes = 0;
@@ -297,6 +304,16 @@ void reset_status(fx_mesh *m);
 double device_max_diag(fx_mesh *m);
 // PCG (fx_pcg.hip)
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep);
+// the FP64 solve in pieces (device-resident Newton iteration, fx_newton.hip):
+// Jacobi + start scalars written on the device (*skip != 0: stopped before
+// the first iteration) + x = 0, r = b; `iters` iterations queued with no
+// status read; the adaptive chunked finish; SpMVs done of a final state
+int pcg_group(fx_mesh *m, const fx_solve_opts &o);
+void pcg_prepare(fx_mesh *m);  // the solve's vectors (allocates; not under graph capture)
+void pcg_start_dev(fx_mesh *m, const double *b_dev, const fx_solve_opts &o, const int32_t *skip);
+void pcg_queue_fp64(fx_mesh *m, int group, int64_t iters);
+int pcg_continue_fp64(fx_mesh *m, double tol, int64_t max_iter, int group, PcgScalars *out);
+int64_t pcg_iterations_done(const PcgScalars &h);
 void spmv64(fx_mesh *m, const double *x, double *y, int group, double *abs_y = nullptr);
 double profile_spmv(fx_mesh *m, int precision, int reps, int group, double *bytes);
 int default_group(fx_mesh *m);  // lanes per node row for the SpMV (row-length heuristic)
@@ -335,6 +352,11 @@ void launch_residual(fx_mesh *m, const double *f_ext, const double *f_int, doubl
 void launch_masked_norms(fx_mesh *m, const double *r, const double *f, const uint8_t *mask, double *out2);
 void launch_penalty(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const int32_t *ud_ptr,
                     const double *rec_target, const int32_t *rec_order, double *rhs, const double *u);
+// the same with the factor 1e6 x max_diag read on the device from the last
+// assembly (no penalty when max_diag is not positive)
+void launch_penalty_dev(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const int32_t *ud_ptr,
+                        const double *rec_target, const int32_t *rec_order, double *rhs, const double *u);
+void launch_reset_status(fx_mesh *m);  // reset_status by a kernel (graph capture)
 void launch_scatter(fx_mesh *m, int64_t n, const int32_t *idx, const double *val, double *dst);
 void launch_fill(fx_mesh *m, double *dst, int64_t n, double v, hipStream_t s = nullptr);
 int64_t count_kept(fx_mesh *m);

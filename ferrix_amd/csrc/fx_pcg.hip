@@ -433,7 +433,8 @@ static hipGraphExec_t batch_graph(fx_mesh *m, const VT *val, int G, int chunk) {
 // stop rule fires; returns the final status (1 converged, 2 breakdown,
 // 3 max_iter).
 template <typename VT>
-static int pcg_iterate(fx_mesh *m, const VT *val, double tol, int64_t max_iter, int G, PcgScalars *out) {
+static int pcg_iterate(fx_mesh *m, const VT *val, double tol, int64_t max_iter, int G, PcgScalars *out,
+                       int first_chunk = 8) {
     hipStream_t s = m->ctx->stream;
     const bool graphs = use_graphs();
     return iterate_chunks(
@@ -447,7 +448,7 @@ static int pcg_iterate(fx_mesh *m, const VT *val, double tol, int64_t max_iter, 
                 FX_HIP(hipGetLastError());
             }
         },
-        out);
+        out, first_chunk);
 }
 
 // One PCG solve on the resident operator `val` with rhs b_dev; x in m->x.
@@ -469,6 +470,69 @@ static void pcg_resume(fx_mesh *m, const double *r64, double rr64, double tol, c
     hipLaunchKernelGGL(k_resume_scalars, dim3(1), dim3(64), 0, s, m->pcg.p, rr64, tol);
     FX_HIP(hipGetLastError());
 }
+
+// ---------------------------------------------------------------------------
+// The FP64 solve in pieces for the device-resident Newton iteration
+// (fx_newton.hip): a start that a graph can hold (the scalars written by a
+// kernel, not copied from the host), iterations queued without a status
+// read, and the adaptive chunked finish of pcg_solve.
+// ---------------------------------------------------------------------------
+namespace {
+// The start scalars of a solve; *skip != 0 (the Newton iteration has already
+// converged) marks the solve stopped before its first iteration, so every
+// queued SpMV and update returns at once.
+__global__ void k_pcg_set(PcgScalars *__restrict__ sc, double tol, int64_t max_iter, int stop_mode,
+                          const int32_t *__restrict__ skip) {
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    PcgScalars init{};
+    init.tol = tol;
+    init.max_iter = max_iter;
+    init.stop_mode = stop_mode;
+    init.status = skip && *skip ? 1 : 0;
+    *sc = init;
+}
+}  // namespace
+
+int pcg_group(fx_mesh *m, const fx_solve_opts &o) {
+    return o.spmv_group > 0 ? o.spmv_group : (m->spmv_group > 0 ? m->spmv_group : default_group(m));
+}
+
+void pcg_prepare(fx_mesh *m) { ensure_vectors(m); }
+
+void pcg_start_dev(fx_mesh *m, const double *b_dev, const fx_solve_opts &o, const int32_t *skip) {
+    hipStream_t s = m->ctx->stream;
+    launch_jacobi(m, false);
+    hipLaunchKernelGGL(k_pcg_set, dim3(1), dim3(64), 0, s, m->pcg.p, o.tol, o.max_iter, o.stop_mode, skip);
+    hipLaunchKernelGGL(k_init, dim3(nblocks(m->n_dof, kBlock)), dim3(kBlock), 0, s, m->n_dof, b_dev, m->x.p, m->r.p,
+                       m->z.p, m->p0.p, m->inv_diag.p, part2_of(m));
+    FX_HIP(hipGetLastError());
+}
+
+void pcg_queue_fp64(fx_mesh *m, int G, int64_t iters) {
+    hipStream_t s = m->ctx->stream;
+    const bool graphs = use_graphs();
+    while (iters > 0) {
+        int chunk = 64;
+        while (chunk > iters)
+            chunk /= 2;
+        if (graphs) {
+            FX_HIP(hipGraphLaunch(batch_graph<double>(m, m->kval.p, G, chunk), s));
+        } else {
+            for (int k = 0; k < chunk; ++k)
+                launch_iteration<double>(m, m->kval.p, G, s);
+            FX_HIP(hipGetLastError());
+        }
+        iters -= chunk;
+    }
+}
+
+int pcg_continue_fp64(fx_mesh *m, double tol, int64_t max_iter, int G, PcgScalars *out) {
+    // a solve that ran past its queued iterations: chunks of 64 from the start
+    return pcg_iterate<double>(m, m->kval.p, tol, max_iter, G, out, 64);
+}
+
+int64_t pcg_iterations_done(const PcgScalars &h) { return iterations_done(h); }
 
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep) {
     hipStream_t s = m->ctx->stream;
@@ -520,6 +584,7 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
             double h3[4];
             FX_HIP(hipMemcpyAsync(h3, out4.p, sizeof h3, hipMemcpyDeviceToHost, s));
             FX_HIP(hipStreamSynchronize(s));
+            ++t_host_syncs;
             R.outer_spmv++;
             if (bb < 0.0)
                 bb = h3[1];
@@ -561,6 +626,7 @@ fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_sol
         FX_HIP(hipMemcpyAsync(x_dev, xacc.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     }
     FX_HIP(hipStreamSynchronize(s));
+    ++t_host_syncs;
     R.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rep)
         *rep = R;

@@ -205,15 +205,16 @@ inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); 
 // 2 breakdown, 3 max_iter) with the scalars in *out.
 template <class LaunchChunk>
 int iterate_chunks(hipStream_t s, PcgScalars *sc_dev, double tol, int64_t max_iter, LaunchChunk &&launch,
-                   PcgScalars *out) {
+                   PcgScalars *out, int first_chunk = 8) {
     PcgScalars h{};
-    int chunk = 8;
+    int chunk = first_chunk;
     double crit_prev = -1.0;
     int64_t it_prev = 0;
     for (;;) {
         launch(chunk);
         FX_HIP(hipMemcpyAsync(&h, sc_dev, sizeof h, hipMemcpyDeviceToHost, s));
         FX_HIP(hipStreamSynchronize(s));
+        ++t_host_syncs;
         if (h.status != 0)
             break;
         // predicted iterations left from the mean log-reduction per iteration

@@ -301,6 +301,8 @@ typedef struct fx_step_report {
     double seconds;
     double seconds_assembly; /* element + assembly + penalty */
     double seconds_solve;
+    int64_t host_syncs;      /* times the host waited on the device (stream
+                                synchronisations) during the step */
 } fx_step_report;
 
 /*
