@@ -51,7 +51,8 @@ socket and core counts are recorded.
 
 "--workload single": the C3D20 block (40x10x10, 56,463 DOF), one linear
 static step per timed step, FP64 vs mixed, with the same roofline and
-baseline figures for its single-system SpMV.
+baseline figures for its single-system SpMV.  "--workload elements": the
+block's C3D20 element stiffness, VALU kernel against FP64 MFMA.
 """
 import argparse
 import ctypes
@@ -241,6 +242,40 @@ def run_single(a):
         "cpu_baseline": {k: v for k, v in cpu.items() if k != "seconds_per_step"} if cpu else None,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_elements(a):
+    """C3D20 element stiffness on the cfg2 block (4,000 elements): the VALU
+    kernel (the oracle's order, bit for bit) against the FP64-MFMA one
+    (FX_C3D20_MFMA=1), each timed over --steps launches between HIP events,
+    beside the device's measured FP64 ceilings."""
+    from ferrix_amd import meshgen
+    from ferrix_amd.device import Context, DeviceMesh
+
+    job = meshgen.cfg2_c3d20_block()
+    ctx = Context(0)
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    valu_peak, mfma_peak = ctx.measure_fp64_peak(20)
+    reps = a.steps or 50
+    out = {}
+    for name, mf in (("valu", False), ("mfma", True)):
+        ms, fl = dm.profile_element(reps, mf)
+        tf = fl / (ms * 1e-3) / 1e12
+        ceiling = mfma_peak if mf else valu_peak
+        out[name] = {"us_per_launch": round(ms * 1e3, 2), "flop_per_launch": fl, "tflops": round(tf, 2),
+                     "frac_of_datasheet": round(tf / FP64_PEAK_TFLOPS, 4),
+                     "frac_of_measured_ceiling": round(tf / ceiling, 4)}
+        print(f"# {name}: {json.dumps(out[name])}", flush=True)
+    m = out["mfma"]
+    print(json.dumps({
+        "metric": "C3D20 element stiffness throughput (elements/s)", "value": round(job.mesh.n_el / (m["us_per_launch"] * 1e-6), 1),
+        "unit": "elements/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": m["us_per_launch"] * 1e-3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "c3d20_element_k_40x10x10_block", "n_elements": job.mesh.n_el},
+        "arms": out, "fp64_ceilings_tflops": {"datasheet": FP64_PEAK_TFLOPS, "measured_valu": round(valu_peak, 2),
+                                              "measured_mfma": round(mfma_peak, 2)},
+        "roofline": {"bound": "mfma", "achieved": m["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": m["frac_of_datasheet"], "traffic": None, "kernel": "k_c3d20_mfma"}}), flush=True)
 
 
 def population(n_meshes):
@@ -614,7 +649,7 @@ def run_serve(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["batch", "single", "serve"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "single", "serve", "elements"], default="batch")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 0.7 x capacity)")
     ap.add_argument("--seed", type=int, default=5, help="serve: request stream seed")
@@ -665,6 +700,8 @@ def main():
         sys.exit(2)
     if a.workload == "serve":
         run_serve(a)
+    elif a.workload == "elements":
+        run_elements(a)
     elif a.workload == "single":
         a.steps = 10 if a.steps is None else a.steps
         a.warmup = 2 if a.warmup is None else a.warmup

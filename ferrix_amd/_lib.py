@@ -94,6 +94,7 @@ SIGNATURES = {
     "fx_static_step": (i32, [P, P, P, P]),
     "fx_profile_spmv": (i32, [P, i32, i32, i32, P, P]),
     "fx_measure_fp64_peak": (i32, [P, i32, P, P]),
+    "fx_profile_element": (i32, [P, i32, i32, P, P]),
     "fx_mesh_autotune_spmv": (i32, [P, i32, i32, P, P]),
     "fx_batch_create": (i32, [P, P, P]),
     "fx_batch_destroy": (None, [P]),

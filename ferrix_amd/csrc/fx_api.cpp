@@ -656,6 +656,17 @@ fx_status fx_measure_fp64_peak(fx_ctx *ctx, int32_t reps, double *valu_tflops, d
     FX_API_END(ctx)
 }
 
+fx_status fx_profile_element(fx_mesh *m, int32_t reps, int32_t mfma, double *ms_per_launch,
+                             double *flop_per_launch) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(reps > 0 && ms_per_launch && flop_per_launch, "fx_profile_element: bad argument");
+    set_device(m->ctx);
+    *ms_per_launch = fx::profile_element(m, reps, mfma != 0, flop_per_launch);
+    FX_API_END(m->ctx)
+}
+
 fx_status fx_mesh_autotune_spmv(fx_mesh *m, int32_t precision, int32_t reps, int32_t *spmv_group_out,
                                 double *ms_per_launch) {
     if (!m)

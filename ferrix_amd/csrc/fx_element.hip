@@ -24,6 +24,8 @@
 // owns node pairs (a, b) and accumulates the 3x3 block over the 27 points.
 #include "fx_internal.h"
 
+#include <cstdlib>
+
 namespace fx {
 namespace {
 
@@ -235,6 +237,97 @@ __global__ __launch_bounds__(128) void k_c3d4(int64_t n_el, const int32_t *__res
     }
 }
 
+// Gauss-point data of one C3D20 element in LDS, shared by the element
+// kernels: J (one entry per thread), det, J^-1, w = |det| w_g and the D
+// coefficients at the point's temperature (27 threads), and the global
+// derivatives dN/dx = J^-1 dN/dxi.  Returns false (for every thread) when
+// the element failed (singular Jacobian or no material values).
+struct C20Shared {
+    double xc[60], u[60], T[20];
+    double dg[27][60];  // [g][r*20 + i]
+    double w[27], d[27][3], sig[27][6], Ji[27][9], J[27][9];
+    int bad;
+};
+
+template <int NT>
+__device__ __forceinline__ bool c20_gauss_data(C20Shared &sh, int64_t e, const int32_t *__restrict__ conn,
+                                               const int32_t *__restrict__ mat, const int64_t *__restrict__ gpos,
+                                               const double *__restrict__ xyz, const double *__restrict__ temp,
+                                               const double *__restrict__ u, int nlgeom,
+                                               const MatLut *__restrict__ mats, const double *__restrict__ gN,
+                                               const double *__restrict__ gdN, const double *__restrict__ gw,
+                                               DevStatus *__restrict__ st) {
+    const int tid = threadIdx.x;
+    if (tid == 0)
+        sh.bad = 0;
+    if (tid < 20) {
+        int nd = conn[20 * e + tid];
+        sh.T[tid] = temp[nd];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double x = xyz[3 * (int64_t)nd + c];
+            double uu = u ? u[3 * (int64_t)nd + c] : 0.0;
+            sh.u[3 * tid + c] = uu;
+            sh.xc[3 * tid + c] = nlgeom ? x + uu : x;
+        }
+    }
+    __syncthreads();
+    // J at every Gauss point, one entry per thread: J(r,c) = sum over the
+    // 20 nodes of dN(r,i) x(i,c), in node order (jac's sums)
+    if (tid < 27 * 9) {
+        const int g = tid / 9, r = (tid % 9) / 3, c = tid % 3;
+        const double *dN = gdN + 60 * g;
+        double sj = 0.0;
+#pragma unroll 4
+        for (int i = 0; i < 20; ++i)
+            sj += dN[20 * r + i] * sh.xc[3 * i + c];
+        sh.J[g][3 * r + c] = sj;
+    }
+    __syncthreads();
+    if (tid < 27) {
+        const int g = tid;
+        double J[9], Ji[9], det;
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            J[k] = sh.J[g][k];
+        if (!jinv(J, Ji, det)) {
+            sh.bad = 1;
+            fail(st, FX_ERR_SINGULAR_JACOBIAN, gpos[e]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                sh.Ji[g][k] = Ji[k];
+            sh.w[g] = fabs(det) * gw[g];
+            double t = 0.0;
+            for (int i = 0; i < 20; ++i)
+                t += gN[20 * g + i] * sh.T[i];
+            double d1, d2, d3;
+            if (!d_coeffs(mats[mat[e]], t, d1, d2, d3)) {
+                sh.bad = 1;
+                fail(st, FX_ERR_INVALID, gpos[e]);
+            }
+            sh.d[g][0] = d1;
+            sh.d[g][1] = d2;
+            sh.d[g][2] = d3;
+        }
+    }
+    __syncthreads();
+    if (sh.bad)
+        return false;
+    // global derivatives dN/dx = J^-1 dN/dxi at every Gauss point
+    for (int idx = tid; idx < 27 * 60; idx += NT) {
+        const int g = idx / 60, q = idx % 60, r = q / 20, i = q % 20;
+        const double *dN = gdN + 60 * g;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            s += sh.Ji[g][3 * r + k] * dN[20 * k + i];
+        sh.dg[g][q] = s;
+    }
+    __syncthreads();
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // C3D20: one workgroup per element.  243 threads take one Jacobian entry
 // each (27 Gauss points x 9), 27 threads the inverses, every thread a share
@@ -253,79 +346,16 @@ __global__ __launch_bounds__(kC20Threads) void k_c3d20(
     double *__restrict__ fe, DevStatus *__restrict__ st, const int32_t *__restrict__ skip) {
     if (skip && *skip)  // the same flag for the whole grid: no barrier is skipped by part of a block
         return;
-    __shared__ double s_xc[60], s_u[60], s_T[20];
-    __shared__ double s_dg[27][60];  // [g][r*20 + i]
-    __shared__ double s_w[27], s_d[27][3], s_sig[27][6], s_Ji[27][9], s_J[27][9];
-    __shared__ int s_bad;
+    __shared__ C20Shared sh;
     const int64_t e = blockIdx.x;
     const int tid = threadIdx.x;
-    if (tid == 0)
-        s_bad = 0;
-    if (tid < 20) {
-        int nd = conn[20 * e + tid];
-        s_T[tid] = temp[nd];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double x = xyz[3 * (int64_t)nd + c];
-            double uu = u ? u[3 * (int64_t)nd + c] : 0.0;
-            s_u[3 * tid + c] = uu;
-            s_xc[3 * tid + c] = nlgeom ? x + uu : x;
-        }
-    }
-    __syncthreads();
-    // J at every Gauss point, one entry per thread: J(r,c) = sum over the
-    // 20 nodes of dN(r,i) x(i,c), in node order (jac's sums)
-    if (tid < 27 * 9) {
-        const int g = tid / 9, r = (tid % 9) / 3, c = tid % 3;
-        const double *dN = gdN + 60 * g;
-        double sj = 0.0;
-#pragma unroll 4
-        for (int i = 0; i < 20; ++i)
-            sj += dN[20 * r + i] * s_xc[3 * i + c];
-        s_J[g][3 * r + c] = sj;
-    }
-    __syncthreads();
-    if (tid < 27) {
-        const int g = tid;
-        double J[9], Ji[9], det;
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-            J[k] = s_J[g][k];
-        if (!jinv(J, Ji, det)) {
-            s_bad = 1;
-            fail(st, FX_ERR_SINGULAR_JACOBIAN, gpos[e]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-                s_Ji[g][k] = Ji[k];
-            s_w[g] = fabs(det) * gw[g];
-            double t = 0.0;
-            for (int i = 0; i < 20; ++i)
-                t += gN[20 * g + i] * s_T[i];
-            double d1, d2, d3;
-            if (!d_coeffs(mats[mat[e]], t, d1, d2, d3)) {
-                s_bad = 1;
-                fail(st, FX_ERR_INVALID, gpos[e]);
-            }
-            s_d[g][0] = d1;
-            s_d[g][1] = d2;
-            s_d[g][2] = d3;
-        }
-    }
-    __syncthreads();
-    if (s_bad)
+    if (!c20_gauss_data<kC20Threads>(sh, e, conn, mat, gpos, xyz, temp, u, nlgeom, mats, gN, gdN, gw, st))
         return;
-    // global derivatives dN/dx = J^-1 dN/dxi at every Gauss point
-    for (int idx = tid; idx < 27 * 60; idx += kC20Threads) {
-        const int g = idx / 60, q = idx % 60, r = q / 20, i = q % 20;
-        const double *dN = gdN + 60 * g;
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            s += s_Ji[g][3 * r + k] * dN[20 * k + i];
-        s_dg[g][q] = s;
-    }
-    __syncthreads();
+    auto &s_dg = sh.dg;
+    auto &s_w = sh.w;
+    auto &s_d = sh.d;
+    auto &s_sig = sh.sig;
+    auto &s_u = sh.u;
     if (DO_K) {
         double *out = ke + e * 3600;
         for (int pair = tid; pair < 400; pair += kC20Threads) {
@@ -393,8 +423,197 @@ __global__ __launch_bounds__(kC20Threads) void k_c3d20(
     }
 }
 
+// ---------------------------------------------------------------------------
+// C3D20 element K on FP64 MFMA (opt-in: FX_C3D20_MFMA=1).  With G the 162 x 60
+// matrix whose rows are the strain rows (Gauss point g, Voigt component c) of
+// B_g, and H = blockdiag(w_g D_g) G, K_el = G^T H: a 60 x 60 x 162 product
+// on v_mfma_f64_16x16x4_f64 (C[16x16] += A[16x4] B[4x16]).  Only the 10
+// upper 16 x 16 tiles of the padded 64 x 64 product are formed, and each
+// value with c >= r is stored at both (r, c) and (c, r): K_el comes out
+// exactly symmetric.  Operands are generated from the LDS
+// Gauss data (B entries are 0 or a global derivative; D is isotropic, so
+// (D B) entries are 0 or d_k times a derivative).  The sum over the 162
+// strain rows runs in the MFMA's own order, so K_el agrees with the
+// oracle's to rounding (tests/test_gpu_mfma.py: 1e-13 relative per
+// element), not bit for bit; the assembly's drop rule then sees these
+// values.  f_int stays on the VALU order of k_c3d20.
+//
+// Operand layout of v_mfma_f64_16x16x4_f64 (one wavefront, measured): A[i][k]
+// in lane i + 16 k, B[k][j] in lane j + 16 k, and C[i][j] in lane
+// j + 16 (i % 4), register i / 4.  Strain rows are taken 4 per instruction, 12 (two Gauss
+// points) per round of 3 instructions; a 28th point of zeros pads 27 to 28.
+// 5 wavefronts per element, 2 tiles each, every tile split into two
+// accumulators (even and odd rounds) so that 4 independent chains
+// interleave.
+// ---------------------------------------------------------------------------
+constexpr int kC20MThreads = 320;
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// B_g[c][3a+p] as (derivative component 0..2, or -1 for a structural zero)
+__device__ __forceinline__ int b_comp(int c, int p) {
+    // x-DOF: [dx, 0, 0, dy, 0, dz]; y-DOF: [0, dy, 0, dx, dz, 0]; z-DOF: [0, 0, dz, 0, dy, dx]
+    constexpr int tab[3][6] = {{0, -1, -1, 1, -1, 2}, {-1, 1, -1, 0, 2, -1}, {-1, -1, 2, -1, 1, 0}};
+    return tab[p][c];
+}
+
+// (D B)[c][3b+q] = d_k x derivative: packed as comp | (k << 2), or -1
+__device__ __forceinline__ int db_code(int c, int q) {
+    // d index 0: d1, 1: d2, 2: d3
+    constexpr int tab[3][6] = {{0 | 0 << 2, 0 | 1 << 2, 0 | 1 << 2, 1 | 2 << 2, -1, 2 | 2 << 2},
+                               {1 | 1 << 2, 1 | 0 << 2, 1 | 1 << 2, 0 | 2 << 2, 2 | 2 << 2, -1},
+                               {2 | 1 << 2, 2 | 1 << 2, 2 | 0 << 2, -1, 1 | 2 << 2, 0 | 2 << 2}};
+    return tab[q][c];
+}
+
+template <bool DO_F>
+__global__ __launch_bounds__(kC20MThreads) void k_c3d20_mfma(
+    int64_t n_el, const int32_t *__restrict__ conn, const int32_t *__restrict__ mat,
+    const int64_t *__restrict__ gpos, const double *__restrict__ xyz, const double *__restrict__ temp,
+    const double *__restrict__ u, int nlgeom, const MatLut *__restrict__ mats, const double *__restrict__ gN,
+    const double *__restrict__ gdN, const double *__restrict__ gw, double *__restrict__ ke,
+    double *__restrict__ fe, DevStatus *__restrict__ st, const int32_t *__restrict__ skip) {
+    if (skip && *skip)
+        return;
+    __shared__ C20Shared sh;
+    __shared__ double s_wd[28][3];  // w_g d_k; point 27 is the zero pad
+    const int64_t e = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (!c20_gauss_data<kC20MThreads>(sh, e, conn, mat, gpos, xyz, temp, u, nlgeom, mats, gN, gdN, gw, st))
+        return;
+    if (tid < 28 * 3) {
+        const int g = tid / 3, k = tid % 3;
+        s_wd[g][k] = g < 27 ? sh.w[g] * sh.d[g][k] : 0.0;
+    }
+    __syncthreads();
+    // tiles (ti, tj), tj >= ti, of the 4 x 4 grid; wave v takes tiles 2v, 2v + 1
+    constexpr int kTi[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+    constexpr int kTj[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+    const int wave = tid / 64, lane = tid % 64;
+    const int li = lane % 16, lk = lane / 16;
+    v4d acc[2][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+        acc[t][0] = acc[t][1] = v4d{0.0, 0.0, 0.0, 0.0};
+    // per tile: this lane's A row (DOF of the tile row) and B column
+    int arow_a[2], arow_p[2], bcol_b[2], bcol_q[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int r = 16 * kTi[2 * wave + t] + li, c = 16 * kTj[2 * wave + t] + li;
+        arow_a[t] = r < 60 ? r / 3 : -1;
+        arow_p[t] = r % 3;
+        bcol_b[t] = c < 60 ? c / 3 : -1;
+        bcol_q[t] = c % 3;
+    }
+    // strain rows of the 3 instructions of a round: kk = 12 j + 4 s + lk, at
+    // Gauss point 2 j + gs[s] and component cs = kk % 6.  Per (s, tile) this
+    // lane's operand offsets within a Gauss row of dg (-1: a structural
+    // zero) and its D coefficient, fixed for the whole loop.
+    int gs[3], ao[3][2], bo[3][2], bk[3][2];
+#pragma unroll
+    for (int s3 = 0; s3 < 3; ++s3) {
+        const int kk = 4 * s3 + lk, c = kk % 6;
+        gs[s3] = kk / 6;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int ca = arow_a[t] >= 0 ? b_comp(c, arow_p[t]) : -1;
+            ao[s3][t] = ca >= 0 ? 20 * ca + arow_a[t] : -1;
+            const int cb = bcol_b[t] >= 0 ? db_code(c, bcol_q[t]) : -1;
+            bo[s3][t] = cb >= 0 ? 20 * (cb & 3) + bcol_b[t] : -1;
+            bk[s3][t] = cb >= 0 ? cb >> 2 : 0;
+        }
+    }
+    // rounds j and j + 1 (points 2 j ... 2 j + 3) into accumulator sets 0
+    // and 1; every load reads a valid address (point 27, the pad, reads
+    // point 26 and selects 0)
+    for (int j0 = 0; j0 < 14; j0 += 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int s3 = 0; s3 < 3; ++s3) {
+                const int g = 2 * (j0 + h) + gs[s3];
+                const bool live = g < 27;
+                const int gg = live ? g : 26;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const double xa = sh.dg[gg][ao[s3][t] >= 0 ? ao[s3][t] : 0];
+                    const double xb = sh.dg[gg][bo[s3][t] >= 0 ? bo[s3][t] : 0];
+                    const double av = (live && ao[s3][t] >= 0) ? xa : 0.0;
+                    const double bv = bo[s3][t] >= 0 ? s_wd[g][bk[s3][t]] * xb : 0.0;
+                    acc[t][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t][h], 0, 0, 0);
+                }
+            }
+        }
+    }
+    double *out = ke + e * 3600;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int ti = kTi[2 * wave + t], tj = kTj[2 * wave + t];
+        const int c = 16 * tj + li;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int r = 16 * ti + lk + 4 * k;
+            if (r >= 60 || c >= 60 || c < r)  // the upper triangle, mirrored
+                continue;
+            const double v = acc[t][0][k] + acc[t][1][k];
+            out[((r / 3) * 20 + c / 3) * 9 + 3 * (r % 3) + c % 3] = v;
+            if (c != r)
+                out[((c / 3) * 20 + r / 3) * 9 + 3 * (c % 3) + r % 3] = v;
+        }
+    }
+    if (DO_F) {  // f_int in k_c3d20's order
+        if (tid < 27) {
+            const int g = tid;
+            double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
+            for (int a = 0; a < 20; ++a) {
+                double dx = sh.dg[g][a], dy = sh.dg[g][20 + a], dz = sh.dg[g][40 + a];
+                double ux = sh.u[3 * a], uy = sh.u[3 * a + 1], uz = sh.u[3 * a + 2];
+                e0 += dx * ux;
+                e1 += dy * uy;
+                e2 += dz * uz;
+                e3 += dy * ux;
+                e3 += dx * uy;
+                e4 += dz * uy;
+                e4 += dy * uz;
+                e5 += dz * ux;
+                e5 += dx * uz;
+            }
+            const double d1 = sh.d[g][0], d2 = sh.d[g][1], d3 = sh.d[g][2];
+            sh.sig[g][0] = ((0.0 + d1 * e0) + d2 * e1) + d2 * e2;
+            sh.sig[g][1] = ((0.0 + d2 * e0) + d1 * e1) + d2 * e2;
+            sh.sig[g][2] = ((0.0 + d2 * e0) + d2 * e1) + d1 * e2;
+            sh.sig[g][3] = 0.0 + d3 * e3;
+            sh.sig[g][4] = 0.0 + d3 * e4;
+            sh.sig[g][5] = 0.0 + d3 * e5;
+        }
+        __syncthreads();
+        if (tid < 60) {
+            const int a = tid / 3, p = tid % 3;
+            double f = 0.0;
+            for (int g = 0; g < 27; ++g) {
+                double dx = sh.dg[g][a], dy = sh.dg[g][20 + a], dz = sh.dg[g][40 + a];
+                double s;
+                if (p == 0)
+                    s = ((0.0 + dx * sh.sig[g][0]) + dy * sh.sig[g][3]) + dz * sh.sig[g][5];
+                else if (p == 1)
+                    s = ((0.0 + dy * sh.sig[g][1]) + dx * sh.sig[g][3]) + dz * sh.sig[g][4];
+                else
+                    s = ((0.0 + dz * sh.sig[g][2]) + dy * sh.sig[g][4]) + dx * sh.sig[g][5];
+                f += s * sh.w[g];
+            }
+            fe[e * 60 + tid] = f;
+        }
+    }
+}
+
+bool c20_mfma_on() {
+    const char *e = std::getenv("FX_C3D20_MFMA");
+    return e && e[0] == '1';
+}
+
 template <bool K, bool F>
-void launch_types(fx_mesh *m, const double *u, int nlgeom, const double *const *tab4, const double *const *tab20) {
+void launch_types(fx_mesh *m, const double *u, int nlgeom, const double *const *tab4, const double *const *tab20,
+                  bool c20_mfma) {
     hipStream_t s = m->ctx->stream;
     if (m->blk4.n_el) {
         const ElemBlock &b = m->blk4;
@@ -405,7 +624,13 @@ void launch_types(fx_mesh *m, const double *u, int nlgeom, const double *const *
                            m->ke.p + b.ke_offset, m->fe.p + b.fe_offset, m->status.p, K ? m->k_skip : nullptr);
         FX_HIP(hipGetLastError());
     }
-    if (m->blk20.n_el) {
+    if (m->blk20.n_el && K && c20_mfma) {
+        const ElemBlock &b = m->blk20;
+        hipLaunchKernelGGL((k_c3d20_mfma<F>), dim3((unsigned)b.n_el), dim3(kC20MThreads), 0, s, b.n_el, b.conn.p,
+                           b.mat.p, b.gpos.p, m->xyz.p, m->temp.p, u, nlgeom, m->mats.p, tab20[0], tab20[1],
+                           tab20[2], m->ke.p + b.ke_offset, m->fe.p + b.fe_offset, m->status.p, m->k_skip);
+        FX_HIP(hipGetLastError());
+    } else if (m->blk20.n_el) {
         const ElemBlock &b = m->blk20;
         hipLaunchKernelGGL((k_c3d20<K, F>), dim3((unsigned)b.n_el), dim3(kC20Threads), 0, s, b.n_el, b.conn.p,
                            b.mat.p, b.gpos.p, m->xyz.p, m->temp.p, u, nlgeom, m->mats.p, tab20[0], tab20[1],
@@ -525,12 +750,39 @@ void launch_element(fx_mesh *m, bool want_k, bool want_f, bool nlgeom) {
     // only under NLGEOM; for a linear K it is not read at all
     const double *u = (want_f || nlgeom) ? m->u.p : nullptr;
     int nl = nlgeom ? 1 : 0;
+    const bool mf = c20_mfma_on();
     if (want_k && want_f)
-        launch_types<true, true>(m, u, nl, tab4, tab20);
+        launch_types<true, true>(m, u, nl, tab4, tab20, mf);
     else if (want_k)
-        launch_types<true, false>(m, u, nl, tab4, tab20);
+        launch_types<true, false>(m, u, nl, tab4, tab20, mf);
     else if (want_f)
-        launch_types<false, true>(m, u, nl, tab4, tab20);
+        launch_types<false, true>(m, u, nl, tab4, tab20, mf);
+}
+
+// fx_profile_element: `reps` launches of the element-K kernels (reference
+// coordinates, K only) between two HIP events after one warm-up launch.
+double profile_element(fx_mesh *m, int reps, bool mfma, double *flop) {
+    DevTables *t = &m->ctx->tables;
+    const double *tab4[3] = {t->n4.p, t->dn4.p, t->w4.p};
+    const double *tab20[3] = {t->n20.p, t->dn20.p, t->w20.p};
+    hipStream_t s = m->ctx->stream;
+    reset_status(m);
+    launch_types<true, false>(m, nullptr, 0, tab4, tab20, mfma);
+    hipEvent_t e0, e1;
+    FX_HIP(hipEventCreate(&e0));
+    FX_HIP(hipEventCreate(&e1));
+    FX_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; ++k)
+        launch_types<true, false>(m, nullptr, 0, tab4, tab20, mfma);
+    FX_HIP(hipEventRecord(e1, s));
+    FX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    check_status(m, "fx_profile_element");
+    *flop = 2592.0 * (double)m->blk4.n_el + (mfma ? 860160.0 : 1283040.0) * (double)m->blk20.n_el;
+    return (double)ms / reps;
 }
 
 }  // namespace fx

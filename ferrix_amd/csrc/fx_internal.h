@@ -289,6 +289,7 @@ struct fx_mesh {
 namespace fx {
 // launchers (defined in the .hip translation units)
 void launch_element(fx_mesh *m, bool want_k, bool want_f, bool nlgeom);
+double profile_element(fx_mesh *m, int reps, bool mfma, double *flop);  // ms per launch
 void launch_recover(fx_mesh *m);         // per (element, node) strain and stress at the resident u
 void launch_gather_recover(fx_mesh *m, double *S, double *E);  // nodal averages (device arrays)
 void launch_assemble(fx_mesh *m);

@@ -196,6 +196,13 @@ class DeviceMesh:
         self.ctx.check(self.lib.fx_profile_spmv(self.h, precision, reps, group, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def profile_element(self, reps: int = 20, mfma: bool = False):
+        """(ms per launch, FLOP per launch) of the element-K kernels
+        (fx_profile_element): the VALU ones, or C3D20 on FP64 MFMA."""
+        ms, fl = C.c_double(0.0), C.c_double(0.0)
+        self.ctx.check(self.lib.fx_profile_element(self.h, reps, 1 if mfma else 0, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
+
     def autotune_spmv(self, precision: int, reps: int = 20):
         """Times the PCG SpMV for every lane-group size and keeps the
         fastest for this mesh's later solves; returns (group, ms per

@@ -329,6 +329,19 @@ fx_status fx_profile_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_
  * are stated against, beside the datasheet figure. */
 fx_status fx_measure_fp64_peak(fx_ctx *ctx, int32_t reps, double *valu_tflops, double *mfma_tflops);
 
+/* Element-K timing (instrumentation, no counterpart in the reference): times
+ * `reps` back-to-back launches of the mesh's element stiffness kernels (K
+ * only, reference coordinates) between two HIP events on the context
+ * stream, after one warm-up launch.  mfma: 0 the VALU kernels (bit for bit
+ * the oracle's order), 1 the C3D20 kernel on FP64 MFMA (the path
+ * FX_C3D20_MFMA=1 selects).  *ms_per_launch: mean launch duration;
+ * *flop_per_launch: the launch's FLOP count (VALU: 1,283,040 per C3D20
+ * element, the dense B^T D B contraction; MFMA: 860,160 per C3D20 element,
+ * 2 x 16 x 16 x 4 per v_mfma_f64_16x16x4_f64 issued; 2,592 per C3D4 element
+ * in either). */
+fx_status fx_profile_element(fx_mesh *mesh, int32_t reps, int32_t mfma, double *ms_per_launch,
+                             double *flop_per_launch);
+
 /* SpMV autotuner (no counterpart in the reference): times the PCG SpMV
  * kernel as fx_profile_spmv does for every lane-group size 2, 4, ..., 64 on
  * the mesh's assembled K in the given precision and keeps the fastest as the
