@@ -584,7 +584,7 @@ def run_serve(a):
     from ferrix_amd.device import Context
     from ferrix_amd.serving import make_requests, serve_batched, serve_serial
 
-    prec = L.FX_PREC_MIXED if a.serve_precision == "mixed" else L.FX_PREC_FP64
+    prec = {"mixed": L.FX_PREC_MIXED, "poly": L.FX_PREC_MIXED_POLY, "fp64": L.FX_PREC_FP64}[a.serve_precision]
     ctx = Context(0)
     warm = make_requests(4, rate=0.0, seed=a.seed + 1000)
     serve_batched(ctx, warm, prec)
@@ -601,7 +601,7 @@ def run_serve(a):
     rb = serve_batched(ctx, reqs, prec, prep_context=not a.serve_shared_prep)
     print(f"# serve batched ({a.serve_precision}): makespan {rb.makespan:.1f} s, failures {len(rb.failures)}",
           flush=True)
-    other = L.FX_PREC_FP64 if prec == L.FX_PREC_MIXED else L.FX_PREC_MIXED
+    other = L.FX_PREC_FP64 if prec != L.FX_PREC_FP64 else L.FX_PREC_MIXED_POLY
     ro = serve_batched(ctx, reqs, other, prep_context=not a.serve_shared_prep)
     print(f"# serve batched (other precision): makespan {ro.makespan:.1f} s, failures {len(ro.failures)}",
           flush=True)
@@ -632,15 +632,18 @@ def run_serve(a):
         "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": round(1e3 * rb.makespan, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32-stored K / fp64" if prec == L.FX_PREC_MIXED else "f64",
+        "dtype": {L.FX_PREC_MIXED: "fp32-stored K / fp64", L.FX_PREC_MIXED_POLY: DTYPE["mixed_poly"]}.get(prec, "f64"),
         "data": "synthetic",
         "config": {"workload": f"serving_{len(reqs)}_requests_poisson_70lin_20nlgeom_10c3d20",
                    "requests": kinds, "rate_per_s": rate, "capacity_per_s": capacity,
                    "admission_context": "shared" if a.serve_shared_prep else "own"},
         "p50_ms": round(sb["p50_ms"], 3), "p99_ms": round(sb["p99_ms"], 3),
         "arms": {f"batched_{a.serve_precision}": sb,
-                 "batched_" + ("fp64" if a.serve_precision == "mixed" else "mixed"): so_,
+                 "batched_" + ("fp64" if a.serve_precision != "fp64" else "poly"): so_,
                  "serial_fp64": ss},
+        "admission": {"rule": "free device memory (hipMemGetInfo + reusable slot blocks) less 4 GiB",
+                      "max_admitted": rb.max_admitted, "least_free_bytes": rb.min_free,
+                      "nlgeom_solves_in_batch": rb.batched_nl_solves},
         "u_rel_err_vs_serial_fp64_max": err,
     }
     print(json.dumps(out), flush=True)
@@ -653,7 +656,7 @@ def main():
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 0.7 x capacity)")
     ap.add_argument("--seed", type=int, default=5, help="serve: request stream seed")
-    ap.add_argument("--serve-precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--serve-precision", choices=["mixed", "poly", "fp64"], default="poly")
     ap.add_argument("--serve-shared-prep", action="store_true",
                     help="serve: admission thread on the batch's context (default: a context of its own)")
     ap.add_argument("--steps", type=int, default=None)

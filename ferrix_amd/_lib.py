@@ -55,11 +55,14 @@ class fx_step_def(C.Structure):
                 ("n_loads", i64), ("load_dof", P), ("load_val", P), ("load_amp", P), ("load_origin", P),
                 ("n_bcs", i64), ("bc_dof", P), ("bc_val", P), ("bc_amp", P), ("bc_origin", P),
                 ("amps", fx_amplitudes), ("solve_opts", fx_solve_opts), ("direct_opts", fx_solve_opts),
-                ("on_increment", P), ("on_increment_user", P)]
+                ("on_increment", P), ("on_increment_user", P), ("solve", P), ("solve_user", P)]
 
 
 # fx_increment_fn: int32_t (*)(void *user, int32_t step_id, int32_t inc_id, double global_time)
 fx_increment_fn = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_double)
+# fx_solve_fn: int32_t (*)(void *user, fx_mesh *mesh, int64_t n, const double *rhs, double *x, int64_t *iterations)
+fx_solve_fn = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double),
+                          C.POINTER(C.c_double), C.POINTER(C.c_int64))
 
 
 class fx_timer(C.Structure):
@@ -76,6 +79,7 @@ class fx_step_report(C.Structure):
 SIGNATURES = {
     "fx_ctx_create": (i32, [i32, P]),
     "fx_ctx_destroy": (None, [P]),
+    "fx_ctx_memory": (i32, [P, P, P, P]),
     "fx_last_error": (C.c_char_p, [P]),
     "fx_version": (C.c_char_p, []),
     "fx_mesh_upload": (i32, [P, i64, P, P, i64, P, P, P, P, P]),

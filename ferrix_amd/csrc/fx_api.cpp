@@ -366,6 +366,24 @@ const char *fx_version(void) { return "ferrix_hip 0.1 (gfx950)"; }
 
 const char *fx_last_error(const fx_ctx *) { return g_last_error.c_str(); }
 
+fx_status fx_ctx_memory(fx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes, int64_t *pool_bytes) {
+    if (!ctx)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "context is NULL");
+    FX_API_BEGIN
+    set_device(ctx);
+    size_t fr = 0, tot = 0;
+    FX_HIP(hipMemGetInfo(&fr, &tot));
+    if (free_bytes)
+        *free_bytes = (int64_t)fr;
+    if (total_bytes)
+        *total_bytes = (int64_t)tot;
+    if (pool_bytes) {
+        std::lock_guard<std::mutex> lk(ctx->pool_mx);
+        *pool_bytes = (int64_t)ctx->pool_bytes;
+    }
+    FX_API_END(ctx)
+}
+
 fx_status fx_ctx_create(int device, fx_ctx **out) {
     fx_ctx *ctx = nullptr;
     FX_API_BEGIN
@@ -752,6 +770,27 @@ void set_fext(fx_mesh *m, const fx_step_def *sd, const fx_timer &timer, int step
     fx::launch_scatter(m, (int64_t)touched.size(), d_touched.p, d_tval.p, m->f_ext.p);
 }
 
+// The step's linear solve through the caller's solver (fx_step_def.solve):
+// rhs (m->rhs) to the host, the call, x back into m->du.
+fx_status hook_solve(fx_mesh *m, const fx_step_def *sd, fx_solve_report *sr) {
+    hipStream_t s = m->ctx->stream;
+    const int64_t n = m->n_dof;
+    std::vector<double> b(n), x(n, 0.0);
+    FX_HIP(hipMemcpyAsync(b.data(), m->rhs.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    FX_HIP(hipStreamSynchronize(s));
+    ++fx::t_host_syncs;
+    int64_t its = 0;
+    const int32_t rc = sd->solve(sd->solve_user, m, n, b.data(), x.data(), &its);
+    *sr = fx_solve_report{};
+    sr->iterations = its;
+    if (rc != FX_OK)
+        return (fx_status)rc;
+    FX_HIP(hipMemcpyAsync(m->du.p, x.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    FX_HIP(hipStreamSynchronize(s));  // x is a local buffer
+    ++fx::t_host_syncs;
+    return FX_OK;
+}
+
 // BC records grouped per DOF in first-appearance order, records in deck order
 struct BcGroups {
     std::vector<int32_t> ud_dof, ud_ptr, rec_order;
@@ -1091,13 +1130,14 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
     // its front as a replayed graph unless FX_NR_GRAPH=0 (plain launches)
     const bool dev_nr = env_on("FX_NR_DEVICE"), use_graph = env_on("FX_NR_GRAPH");
     const bool mixed = opts.precision == FX_PREC_MIXED;
+    const bool host_solve = mixed || sd->solve != nullptr;  // the solve outside the device front
     fx::NewtonGraph ng(m);
     Events3 ev;
     bool nr_graph_ready = false;
     // SpMVs of the FP64 solves by NR index: the previous increment's and this one's
     std::vector<int64_t> iters_prev, iters_cur;
     const int G = fx::pcg_group(m, opts);
-    if (dev_nr && !mixed)
+    if (dev_nr && !host_solve)
         fx::pcg_prepare(m);
     const int64_t syncs0 = fx::t_host_syncs;
 
@@ -1133,7 +1173,7 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
             iters_cur.clear();
             for (int it = 0; it < 15; ++it) {
                 rep.nr_iterations++;
-                fx::NewtonGraph::FrontArgs fa{nr_graph_ready, nlgeom, !mixed, &opts, dmask.p, dbc.n_unique,
+                fx::NewtonGraph::FrontArgs fa{nr_graph_ready, nlgeom, !host_solve, &opts, dmask.p, dbc.n_unique,
                                               dbc.ud_dof.p, dbc.ud_ptr.p, dbc.rec_order.p, dbc.target.p};
                 FX_HIP(hipEventRecord(ev.e[0], s));
                 ng.launch_front(fa);
@@ -1150,13 +1190,13 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
                 // little short and the chunked finish reads the rest.
                 const bool expect_conv = it > 0 && (!nlgeom || (rel_prev > 0.0 && rep.last_rel_residual *
                                                                    (rep.last_rel_residual / rel_prev) < 1e-3));
-                if (!mixed && !expect_conv) {
+                if (!host_solve && !expect_conv) {
                     const int64_t pred = (size_t)it < iters_prev.size() ? iters_prev[it]
                                          : !iters_cur.empty()            ? iters_cur.back()
                                                                          : 48;
                     fx::pcg_queue_fp64(m, G, pred - pred / 16);
                 }
-                ng.launch_post(!mixed);
+                ng.launch_post(!host_solve);
                 FX_HIP(hipEventRecord(ev.e[2], s));
                 const fx::NrReport &R = ng.wait();
                 float ms_front = 0.f, ms_queued = 0.f;
@@ -1177,8 +1217,8 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
                 fx_solve_report sr{};
                 fx_status st = FX_OK;
                 auto t_sol = clk::now();
-                if (mixed) {
-                    st = fx::pcg_solve(m, m->rhs.p, m->du.p, opts, &sr);
+                if (host_solve) {
+                    st = sd->solve ? hook_solve(m, sd, &sr) : fx::pcg_solve(m, m->rhs.p, m->du.p, opts, &sr);
                     if (st == FX_OK)
                         fx::axpy_dev(m, n, 1.0, m->du.p, m->u.p);
                 } else {
@@ -1208,7 +1248,9 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
                     rep.seconds = since(t_start);
                         *report = rep;
                     }
-                    throw Error{st, solve_error(st, opts, sr)};
+                    throw Error{st, sd->solve ? "the step's linear solver (fx_step_def.solve) returned status " +
+                                                    std::to_string((int)st)
+                                              : solve_error(st, opts, sr)};
                 }
             }
         } else
@@ -1250,7 +1292,7 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
             // 7. solve
             auto t_sol = clk::now();
             fx_solve_report sr{};
-            fx_status st = fx::pcg_solve(m, m->rhs.p, m->du.p, opts, &sr);
+            fx_status st = sd->solve ? hook_solve(m, sd, &sr) : fx::pcg_solve(m, m->rhs.p, m->du.p, opts, &sr);
             rep.seconds_solve += since(t_sol);
             rep.linear_solves++;
             rep.pcg_iterations += sr.iterations;
@@ -1261,7 +1303,9 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
                     rep.seconds = since(t_start);
                     *report = rep;
                 }
-                throw Error{st, solve_error(st, opts, sr)};
+                throw Error{st, sd->solve ? "the step's linear solver (fx_step_def.solve) returned status " +
+                                                    std::to_string((int)st)
+                                              : solve_error(st, opts, sr)};
             }
             // 8. u_curr += du
             fx::axpy_dev(m, n, 1.0, m->du.p, m->u.p);

@@ -100,6 +100,13 @@ class Context:
             msg = self.lib.fx_last_error(self.h).decode()
             raise _ERR.get(rc, FerrixError)(msg)
 
+    def memory(self):
+        """(free, total, pool) device bytes: hipMemGetInfo and the
+        released batch-slot blocks this context keeps for reuse."""
+        fr, tot, pool = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        self.check(self.lib.fx_ctx_memory(self.h, C.byref(fr), C.byref(tot), C.byref(pool)))
+        return fr.value, tot.value, pool.value
+
     def measure_fp64_peak(self, reps: int = 10):
         """(FP64 VALU, FP64 MFMA) TFLOP/s measured on this device by
         register-only kernels (fx_measure_fp64_peak)."""
@@ -277,13 +284,31 @@ class DeviceMesh:
 
     # -- step driver ---------------------------------------------------------
     def static_step(self, step: StepSpec, step_id: int, timer: L.fx_timer, direct_opts: SolveOptions = None,
-                    solve_opts: Optional[SolveOptions] = None, on_increment=None) -> L.fx_step_report:
+                    solve_opts: Optional[SolveOptions] = None, on_increment=None, solver=None) -> L.fx_step_report:
         """StaticStep::solve.  on_increment(step_id, inc_id, global_time), if
         given, runs after every converged increment with its displacement
         resident (the reference's writer.write_increment); an exception
-        raised there stops the step and is raised again here."""
+        raised there stops the step and is raised again here.
+        solver(mesh, rhs) -> (x, iterations), if given, solves every Newton
+        iteration's system (K resident on `mesh`, assembled and penalised;
+        rhs the penalised residual) in place of the step's own PCG
+        (fx_step_def.solve, the Solver seam); an exception raised there stops
+        the step and is raised again here."""
         sd, keep = step_def(step, step_id, direct_opts or DIRECT_STANDIN, solve_opts)
         raised = []
+        if solver is not None:
+            def solve_cb(_user, _mesh, n, rhs, x, its):
+                try:
+                    b = np.ctypeslib.as_array(rhs, shape=(n,)).copy()
+                    xs, k = solver(self, b)
+                    np.ctypeslib.as_array(x, shape=(n,))[:] = xs
+                    its[0] = int(k)
+                    return 0
+                except BaseException as e:  # carried across the C frames, raised below
+                    raised.append(e)
+                    return int(getattr(e, "status", L.FX_ERR_NOT_CONVERGED))
+            keep["solve"] = L.fx_solve_fn(solve_cb)
+            sd.solve = C.cast(keep["solve"], C.c_void_p)
         if on_increment is not None:
             def hook(_user, sid, inc, t):
                 try:
@@ -414,7 +439,8 @@ def step_def(step: StepSpec, step_id: int, direct_opts: SolveOptions, solve_opts
 
 
 def run_job(ctx: Context, job: Job, precision=L.FX_PREC_FP64, direct_opts: Optional[SolveOptions] = None,
-            solve_opts: Optional[SolveOptions] = None, mesh: Optional[DeviceMesh] = None, writer=None):
+            solve_opts: Optional[SolveOptions] = None, mesh: Optional[DeviceMesh] = None, writer=None,
+            solver=None):
     """All steps of ``job`` from u = 0; returns (u, [fx_step_report]).
     writer: a result writer (``ferrix_amd.vtk.VtkWriter``) driven as
     main.rs:45-66 drives the reference's: init, one write per converged
@@ -435,7 +461,8 @@ def run_job(ctx: Context, job: Job, precision=L.FX_PREC_FP64, direct_opts: Optio
         for k, st in enumerate(job.steps):
             so = solve_opts or step_options(st, precision)
             do = direct_opts or SolveOptions(precision=precision, stop_mode=STOP_REL_R2, tol=1e-12, max_iter=200_000)
-            reps.append(dm.static_step(st, k + 1, timer, direct_opts=do, solve_opts=so, on_increment=hook))
+            reps.append(dm.static_step(st, k + 1, timer, direct_opts=do, solve_opts=so, on_increment=hook,
+                                       solver=solver))
     finally:
         if writer is not None:
             writer.finish()

@@ -5,15 +5,21 @@ A request is a linear static job (one mesh, 1-4 load cases sharing K), an
 NLGEOM static job (Newton-Raphson with the reference's increment control,
 ``static_step.rs:47-267``), or a C3D20 block.  The server
 
-* admits an arrived request when its estimated device memory fits the
-  budget (FIFO; the head of the queue waits until memory frees up);
+* admits an arrived request when its estimated device memory fits what the
+  device has free: hipMemGetInfo's free bytes plus the released slot blocks
+  the context keeps for reuse, less a reserve and the estimates of admitted
+  requests whose memory is not allocated yet (FIFO; the head of the queue
+  waits until memory frees up).  A fixed budget over the estimates can be
+  given instead;
 * uploads its mesh (host symbolic phase + copies), assembles K and applies
   the penalty BCs on an admission thread -- the host work overlaps with the
   running batch -- and adds the system to the running :class:`Batch`: it
   joins the next iteration chunk while the other systems keep going;
 * runs NLGEOM requests on worker threads, each with its own context and
-  stream (``fx_static_step``), so their Newton solves overlap with the
-  batch on the device;
+  stream (``fx_static_step``: increments, element and assembly kernels on
+  the worker's stream); every Newton iteration's linear solve joins the
+  running batch through the step's solver seam (``fx_step_def.solve``) and
+  advances with the other systems' iterations;
 * retires a request when its result is on the host, freeing its memory.
 
 Latency is completion minus arrival.  The serial comparison
@@ -24,6 +30,7 @@ reference's pattern (one FP64 ``fx_solve`` per load case, one
 from __future__ import annotations
 
 import collections
+import queue
 import threading
 import time
 from concurrent.futures import Future, ThreadPoolExecutor
@@ -36,7 +43,7 @@ from . import _lib as L
 from . import meshgen
 from .batch import System, assemble
 from .device import Batch, Context, DeviceMesh, SolveOptions, run_job
-from .inp import Project, deck_stem
+from .inp import ConvergenceError, Project, deck_stem
 from .model import Job
 
 
@@ -105,7 +112,10 @@ class ServeResult:
     solves: Dict[int, int] = field(default_factory=dict)         # linear solves per request
     failures: Dict[int, str] = field(default_factory=dict)
     makespan: float = 0.0
-    peak_bytes: int = 0
+    peak_bytes: int = 0          # largest sum of admitted requests' estimates
+    min_free: float = float("inf")  # least free + reusable device bytes an admission saw
+    max_admitted: int = 0        # most requests admitted and not yet finished at once
+    batched_nl_solves: int = 0   # Newton iterations' solves of NLGEOM requests done in the batch
 
     def dof_solves(self, reqs: List[Request]) -> int:
         by = {r.rid: r for r in reqs}
@@ -114,17 +124,59 @@ class ServeResult:
 
 
 def _static_opts(precision: int) -> SolveOptions:
+    if precision == L.FX_PREC_MIXED_POLY:
+        return SolveOptions(precision=precision, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=400_000)
     return SolveOptions(precision=precision, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000,
                         inner_tol=1e-3, max_refine=100)
 
 
+class _BatchSolve:
+    """One Newton iteration's linear solve handed from an NLGEOM worker to
+    the serving loop, which adds it to the running batch and sets `done`
+    when its result is in `x`."""
+
+    def __init__(self, dm: DeviceMesh, rhs: np.ndarray):
+        self.dm, self.rhs = dm, rhs
+        self.x: Optional[np.ndarray] = None
+        self.iterations = 0
+        self.status = 0
+        self.done = threading.Event()
+
+
 class _NlWorker:
     """One thread with its own context: NLGEOM requests through the step
-    driver."""
+    driver.  With a solve queue, every Newton iteration's system goes to the
+    serving loop's batch (the step's solver seam) and the worker waits for
+    its result; without one, the step solves with its own PCG."""
 
-    def __init__(self, precision: int):
+    def __init__(self, precision: int, solves: Optional["queue.Queue"] = None):
         self.precision = precision
+        self.solves = solves
         self.local = threading.local()
+        self.lock = threading.Lock()
+        self.closed = False  # set by the serving loop when it stops serving solves
+
+    def close(self):
+        """No more batched solves: queued ones are released with a failure."""
+        with self.lock:
+            self.closed = True
+        while self.solves is not None and not self.solves.empty():
+            job = self.solves.get_nowait()
+            job.status = -L.FX_ERR_INVALID
+            job.done.set()
+
+    def _solver(self, dm: DeviceMesh, rhs: np.ndarray):
+        job = _BatchSolve(dm, rhs)
+        with self.lock:
+            if self.closed:
+                raise ConvergenceError("the server stopped before the solve")
+            self.solves.put(job)
+        job.done.wait()
+        if job.status != 1:
+            e = ConvergenceError(f"batched linear solve ended with status {job.status}")
+            e.status = L.FX_ERR_NOT_CONVERGED if job.status in (-L.FX_ERR_NOT_CONVERGED, 3) else L.FX_ERR_BREAKDOWN
+            raise e
+        return job.x, job.iterations
 
     def __call__(self, req: Request):
         ctx = getattr(self.local, "ctx", None)
@@ -133,32 +185,47 @@ class _NlWorker:
         job = req.jobs[0]
         dm = DeviceMesh(ctx, job.mesh, job.node_temp)
         try:
-            so = _static_opts(self.precision)
-            u, reps = run_job(ctx, job, solve_opts=so, direct_opts=so, mesh=dm)
+            so = _static_opts(L.FX_PREC_MIXED if self.precision == L.FX_PREC_MIXED_POLY else self.precision)
+            u, reps = run_job(ctx, job, solve_opts=so, direct_opts=so, mesh=dm,
+                              solver=self._solver if self.solves is not None else None)
             return u, sum(r.linear_solves for r in reps)
         finally:
             dm.close()
 
 
 def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_MIXED,
-                  mem_budget: int = 200 << 30, chunk_iters: int = 16, nl_workers: int = 2,
-                  prep_context: bool = True) -> ServeResult:
+                  mem_budget: Optional[int] = None, mem_reserve: int = 4 << 30, chunk_iters: int = 16,
+                  nl_workers: int = 4, prep_context: bool = True, nl_in_batch: bool = True) -> ServeResult:
     """Continuous batching over the request stream (see the module doc).
+    mem_budget: None admits by the device's free memory (less mem_reserve);
+    a number admits while the admitted requests' estimates stay under it.
     prep_context: the admission thread uploads and assembles on a context of
     its own (its own stream), so its synchronisations do not wait behind the
-    batch's queued chunks; False: on the batch's context."""
+    batch's queued chunks; False: on the batch's context.  nl_in_batch:
+    NLGEOM requests' linear solves join the batch (False: each step solves
+    with its own PCG on the worker's stream)."""
     res = ServeResult()
     bt = Batch(ctx, _static_opts(precision))
     pctx = Context(0) if prep_context else ctx
     pool = ThreadPoolExecutor(max_workers=nl_workers)
     prep = ThreadPoolExecutor(max_workers=1)
-    worker = _NlWorker(precision)
+    nl_queue: "queue.Queue[_BatchSolve]" = queue.Queue()
+    worker = _NlWorker(precision, nl_queue if nl_in_batch else None)
     pending = collections.deque(sorted(reqs, key=lambda r: r.arrival))
     waiting: collections.deque = collections.deque()
     running: Dict[int, tuple] = {}                   # slot -> (request, System, bytes)
+    nl_slots: Dict[int, _BatchSolve] = {}            # slot -> a Newton iteration's solve
     nl: Dict[int, tuple] = {}                        # rid -> (future, request, bytes)
     staged: collections.deque = collections.deque()  # (future System, request, bytes), admission order
     used = 0
+    inflight = 0  # estimates of admitted requests whose memory is not allocated yet
+
+    def fits(need: int) -> bool:
+        if mem_budget is not None:
+            return used + need <= mem_budget
+        free, _total, cached = ctx.memory()
+        res.min_free = min(res.min_free, free + cached)
+        return need <= free + cached - mem_reserve - inflight
 
     def prepare(r: Request) -> System:
         s = System(r.rid, r.jobs, DeviceMesh(pctx, r.jobs[0].mesh, r.jobs[0].node_temp))
@@ -171,21 +238,26 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            # admission: FIFO under the memory budget
+            # admission: FIFO under the free memory (or the budget)
             while waiting:
                 r = waiting[0]
                 need = r.est_bytes()
-                if (running or nl or staged) and used + need > mem_budget:
+                if (running or nl or staged) and not fits(need):
                     break
                 waiting.popleft()
                 used += need
                 res.peak_bytes = max(res.peak_bytes, used)
                 if r.kind == "nlgeom":
                     nl[r.rid] = (pool.submit(worker, r), r, need)
+                    inflight += need  # allocated by the worker as its step runs; counted until it ends
+                    res.max_admitted = max(res.max_admitted, len(running) + len(nl) + len(staged))
                     continue
                 staged.append((prep.submit(prepare, r), r, need))
+                inflight += need
+                res.max_admitted = max(res.max_admitted, len(running) + len(nl) + len(staged))
             while staged and staged[0][0].done():
                 fut, r, need = staged.popleft()
+                inflight -= need  # uploaded: now in the device's used memory
                 try:
                     s = fut.result()
                 except Exception as e:  # noqa: BLE001 -- a failed request is reported, the server goes on
@@ -194,8 +266,23 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
                     used -= need
                     continue
                 running[bt.add(s.dm, s.rhs)] = (r, s, need)
-            if running:
+            # Newton iterations' solves of the NLGEOM workers join the batch
+            while not nl_queue.empty():
+                job = nl_queue.get_nowait()
+                nl_slots[bt.add(job.dm, job.rhs[None, :])] = job
+            if running or nl_slots:
                 bt.run(max_iters=chunk_iters)
+                for slot in list(nl_slots):
+                    st, rep = bt.poll(slot)
+                    if st == 0:
+                        continue
+                    job = nl_slots.pop(slot)
+                    if st == 1:
+                        job.x = bt.get(slot)[0]
+                    job.status, job.iterations = st, rep.iterations
+                    res.batched_nl_solves += 1
+                    bt.remove(slot)  # before the worker goes on with its mesh
+                    job.done.set()
                 for slot in list(running):
                     st, rep = bt.poll(slot)
                     if st == 0:
@@ -215,6 +302,7 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
                     used -= need
             for rid in [k for k, v in nl.items() if v[0].done()]:
                 fut, r, need = nl.pop(rid)
+                inflight -= need
                 try:
                     u, solves = fut.result()
                     res.x[rid] = u[None, :]
@@ -223,12 +311,19 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
                     res.failures[rid] = f"{type(e).__name__}: {e}"
                 res.latency[rid] = time.perf_counter() - t0 - r.arrival
                 used -= need
-            if not running and not waiting:
-                # idle: wait for the next arrival, a worker or an admission
+            if not running and not nl_slots and not waiting:
+                # idle: wait for the next arrival, a worker's solve or an admission
                 nxt = pending[0].arrival - (time.perf_counter() - t0) if pending else 0.0
-                time.sleep(min(max(nxt, 0.0), 0.002) if nl or pending or staged else 0.0)
+                if nl and nl_queue.empty():
+                    time.sleep(0.0002)
+                else:
+                    time.sleep(min(max(nxt, 0.0), 0.002) if pending or staged else 0.0)
         res.makespan = time.perf_counter() - t0
     finally:
+        worker.close()
+        for job in nl_slots.values():  # solves the loop left in the batch (an error stopped it)
+            job.status = -L.FX_ERR_INVALID
+            job.done.set()
         prep.shutdown(wait=True)
         pool.shutdown(wait=True)
         bt.close()

@@ -62,6 +62,13 @@ const char *fx_last_error(const fx_ctx *ctx);
 /* Library version string. */
 const char *fx_version(void);
 
+/* Device memory as an admission decision needs it (no counterpart in the
+ * reference): *free_bytes and *total_bytes from hipMemGetInfo on the
+ * context's device, *pool_bytes the released batch-slot blocks the context
+ * keeps for reuse (allocated, but free for later slots).  Any pointer may be
+ * NULL. */
+fx_status fx_ctx_memory(fx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes, int64_t *pool_bytes);
+
 /* ---- mesh ------------------------------------------------------------- */
 #define FX_LUT_MAX 16
 
@@ -262,6 +269,17 @@ typedef enum fx_solver_kind {
  * FX_ERR_INVALID (the reference's GenericIo error).  NULL: no call. */
 typedef int32_t (*fx_increment_fn)(void *user, int32_t step_id, int32_t inc_id, double global_time);
 
+/* A caller-supplied linear solver for the step's Newton iterations: the
+ * Solver seam the step driver calls with the matrix it has just built
+ * (static_step.rs:198-204, solvers/mod.rs:18-44).  Called with the mesh's K
+ * assembled and penalised on the device (fx_matrix_get reads it), rhs the
+ * penalised residual (host, n values) and x the solution buffer (host, n
+ * values).  Returns FX_OK with x filled and *iterations set, or an
+ * fx_status that stops the step with that status.  A continuous-batching
+ * server passes one that solves K in its running batch. */
+typedef int32_t (*fx_solve_fn)(void *user, fx_mesh *mesh, int64_t n, const double *rhs, double *x,
+                               int64_t *iterations);
+
 typedef struct fx_step_def {
     int32_t step_id; /* 1-based, as passed to Step::solve */
     int32_t nlgeom;
@@ -283,6 +301,8 @@ typedef struct fx_step_def {
     fx_solve_opts direct_opts; /* DEFAULT / DIRECT */
     fx_increment_fn on_increment; /* NULL, or the result writer's hook */
     void *on_increment_user;
+    fx_solve_fn solve;             /* NULL (the step's own PCG), or a caller solver */
+    void *solve_user;
 } fx_step_def;
 
 /* SolverTime (time.rs:8-63). */

@@ -34,7 +34,7 @@ def stream():
     return reqs
 
 
-@pytest.mark.parametrize("precision", [L.FX_PREC_MIXED, L.FX_PREC_FP64])
+@pytest.mark.parametrize("precision", [L.FX_PREC_MIXED, L.FX_PREC_FP64, L.FX_PREC_MIXED_POLY])
 def test_serving_alone_vs_mix(ctx, precision):
     reqs = stream()
     say("requests", [(r.kind, r.n_dof, r.n_rhs) for r in reqs])
@@ -50,6 +50,44 @@ def test_serving_alone_vs_mix(ctx, precision):
             "latency ms", 1e3 * mix.latency[r.rid])
         assert e.max() < 1e-8
     assert mix.dof_solves(reqs) > 0
+    n_nl = sum(1 for r in reqs if r.kind == "nlgeom")
+    assert mix.batched_nl_solves >= n_nl  # every NLGEOM request solved its Newton systems in the batch
+
+
+def test_serving_nlgeom_own_solver_vs_batch(ctx):
+    """NLGEOM requests whose Newton solves join the batch against the same
+    requests solving with the step's own FP64 PCG on the worker's stream:
+    the same number of linear solves, U within 1e-8."""
+    specs = meshgen.cfg4_batch_specs(n_meshes=3, seed=29, dof_lo=500, dof_hi=6000)
+    reqs = [Request(k, 0.0, "nlgeom", [meshgen.c3d4_box_nlgeom(sp)]) for k, sp in enumerate(specs)]
+    inb = serve_batched(ctx, reqs, L.FX_PREC_FP64, chunk_iters=8)
+    own = serve_batched(ctx, reqs, L.FX_PREC_FP64, chunk_iters=8, nl_in_batch=False)
+    assert not inb.failures and not own.failures
+    assert inb.batched_nl_solves == sum(inb.solves.values()) and own.batched_nl_solves == 0
+    for r in reqs:
+        e = np.linalg.norm(inb.x[r.rid] - own.x[r.rid]) / np.linalg.norm(own.x[r.rid])
+        say("request", r.rid, "n_dof", r.n_dof, "solves", inb.solves[r.rid], own.solves[r.rid], "rel diff", e)
+        assert inb.solves[r.rid] == own.solves[r.rid]
+        assert e < 1e-8
+
+
+def test_serving_admits_by_free_memory(ctx):
+    """Admission against the device's free memory: a reserve that leaves
+    room for about two requests' estimates holds the number admitted at
+    once down, and every result still matches the serial FP64 server."""
+    reqs = stream()
+    free, total, pool = ctx.memory()
+    assert 0 < free <= total
+    room = 2 * max(r.est_bytes() for r in reqs)
+    res = serve_batched(ctx, reqs, L.FX_PREC_FP64, chunk_iters=8, mem_reserve=free + pool - room)
+    unlimited = serve_batched(ctx, reqs, L.FX_PREC_FP64, chunk_iters=8)
+    say("free", free, "total", total, "admitted at once", res.max_admitted, "unlimited", unlimited.max_admitted,
+        "least free seen", res.min_free)
+    assert not res.failures
+    assert res.max_admitted < unlimited.max_admitted
+    assert res.max_admitted <= 3
+    for r in reqs:
+        assert np.array_equal(res.x[r.rid], unlimited.x[r.rid]), r.rid
 
 
 # calculix_parity.rs known answers: node 17's displacement after the last step
