@@ -592,20 +592,20 @@ def run_serve(a):
     t = time.perf_counter()
     sat = make_requests(a.requests, rate=0.0, seed=a.seed)
     print(f"# serve: {len(sat)} requests generated in {time.perf_counter() - t:.1f} s", flush=True)
-    r_sat = serve_batched(ctx, sat, prec, prep_context=not a.serve_shared_prep)
+    r_sat = serve_batched(ctx, sat, prec, prep_context=not a.serve_shared_prep, log_every=100)
     capacity = len(sat) / r_sat.makespan
     rate = a.rate if a.rate > 0 else 0.7 * capacity
     print(f"# serve capacity (all at t=0): {capacity:.2f} req/s, {r_sat.dof_solves(sat) / r_sat.makespan:.0f} "
           f"DOF-solves/s; arrival rate {rate:.2f} req/s", flush=True)
     reqs = make_requests(a.requests, rate=rate, seed=a.seed)
-    rb = serve_batched(ctx, reqs, prec, prep_context=not a.serve_shared_prep)
+    rb = serve_batched(ctx, reqs, prec, prep_context=not a.serve_shared_prep, log_every=100)
     print(f"# serve batched ({a.serve_precision}): makespan {rb.makespan:.1f} s, failures {len(rb.failures)}",
           flush=True)
     other = L.FX_PREC_FP64 if prec != L.FX_PREC_FP64 else L.FX_PREC_MIXED_POLY
-    ro = serve_batched(ctx, reqs, other, prep_context=not a.serve_shared_prep)
+    ro = serve_batched(ctx, reqs, other, prep_context=not a.serve_shared_prep, log_every=100)
     print(f"# serve batched (other precision): makespan {ro.makespan:.1f} s, failures {len(ro.failures)}",
           flush=True)
-    rs = serve_serial(ctx, reqs)
+    rs = serve_serial(ctx, reqs, log_every=100)
     print(f"# serve serial fp64: makespan {rs.makespan:.1f} s, failures {len(rs.failures)}", flush=True)
 
     def max_err(r):
@@ -653,7 +653,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=["batch", "single", "serve", "elements"], default="batch")
-    ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
+    ap.add_argument("--requests", type=int, default=1000, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 0.7 x capacity)")
     ap.add_argument("--seed", type=int, default=5, help="serve: request stream seed")
     ap.add_argument("--serve-precision", choices=["mixed", "poly", "fp64"], default="poly")

@@ -195,7 +195,8 @@ class _NlWorker:
 
 def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_MIXED,
                   mem_budget: Optional[int] = None, mem_reserve: int = 4 << 30, chunk_iters: int = 16,
-                  nl_workers: int = 4, prep_context: bool = True, nl_in_batch: bool = True) -> ServeResult:
+                  nl_workers: int = 4, prep_context: bool = True, nl_in_batch: bool = True,
+                  log_every: int = 0) -> ServeResult:
     """Continuous batching over the request stream (see the module doc).
     mem_budget: None admits by the device's free memory (less mem_reserve);
     a number admits while the admitted requests' estimates stay under it.
@@ -233,8 +234,12 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
         return s
 
     t0 = time.perf_counter()
+    logged = 0
     try:
         while pending or waiting or running or nl or staged:
+            if log_every and len(res.latency) >= logged + log_every:
+                logged = len(res.latency) // log_every * log_every
+                print(f"# served {len(res.latency)}/{len(reqs)} at {time.perf_counter() - t0:.1f} s", flush=True)
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
@@ -332,7 +337,7 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
     return res
 
 
-def serve_serial(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_FP64) -> ServeResult:
+def serve_serial(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_FP64, log_every: int = 0) -> ServeResult:
     """The same stream one request at a time, in arrival order: one
     fx_solve per load case, one step driver call per NLGEOM step."""
     res = ServeResult()
@@ -359,5 +364,7 @@ def serve_serial(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_F
         finally:
             dm.close()
         res.latency[r.rid] = time.perf_counter() - t0 - r.arrival
+        if log_every and len(res.latency) % log_every == 0:
+            print(f"# served {len(res.latency)}/{len(reqs)} at {time.perf_counter() - t0:.1f} s", flush=True)
     res.makespan = time.perf_counter() - t0
     return res
