@@ -52,6 +52,24 @@ def upload(ctx, systems: List[System]):
             s.dm = DeviceMesh(ctx, s.jobs[0].mesh, s.jobs[0].node_temp)
 
 
+def loads(s: System) -> np.ndarray:
+    """(n_rhs, n_dof): every load case's external force before the penalty."""
+    rows = []
+    for j in s.jobs:
+        st = j.steps[0]
+        b = np.zeros(j.n_dof)
+        np.add.at(b, st.load_dof, st.load_val)
+        rows.append(b)
+    return np.array(rows)
+
+
+def admit(bt: Batch, s: System) -> int:
+    """Assembly, penalty and admission of one system with no host wait
+    (Batch.add_system): the same arithmetic as assemble() + Batch.add."""
+    st0 = s.jobs[0].steps[0]
+    return bt.add_system(s.dm, loads(s), st0.bc_dof, st0.bc_val)
+
+
 def assemble(s: System):
     """Linear K at u = 0, penalty BCs of the first load case's step on K and
     every load case's right-hand side (all load cases share the BC set).
@@ -114,7 +132,8 @@ def operator_bytes(s: System, opts: SolveOptions) -> int:
 
 
 def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int = 0, chunk_iters: int = 256,
-                  time_spmv: bool = False, assemble_ahead: bool = False) -> PopulationResult:
+                  time_spmv: bool = False, assemble_ahead: bool = False,
+                  device_admission: bool = True) -> PopulationResult:
     """Continuous batching over `systems` in order.  ws_bytes > 0 caps the
     summed operator bytes of the running systems (at least one always runs);
     0 admits everything at once.  Each Batch.run call advances the running
@@ -126,7 +145,13 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
     batch's (their own stream).  A host thread assembles them in order while
     the batch runs, and each pass admits the ones that are ready, so the GPU
     iterates the first systems while later ones are assembled.  Otherwise
-    each system is assembled just before its admission."""
+    each system is assembled just before its admission.
+
+    device_admission (without assemble_ahead): each system's assembly,
+    penalty and admission are queued on the device with no host wait
+    (fx_batch_add_system); False: the host reads max_diag and the
+    penalised right-hand sides back per system first (assemble()).  The
+    results are the same bit for bit."""
     if assemble_ahead and any(s.dm.ctx is ctx for s in systems):
         raise ValueError("assemble_ahead: the meshes must belong to another context than the batch")
     res = PopulationResult()
@@ -189,9 +214,12 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
                 need = operator_bytes(nxt, opts)
                 if running and ws_bytes > 0 and ws + need > ws_bytes:
                     break
-                if worker is None:
-                    assemble(nxt)
-                slot = bt.add(nxt.dm, nxt.rhs)
+                if worker is None and device_admission:
+                    slot = admit(bt, nxt)
+                else:
+                    if worker is None:
+                        assemble(nxt)
+                    slot = bt.add(nxt.dm, nxt.rhs)
                 running[slot] = nxt
                 ws += need
                 nxt = None

@@ -1068,6 +1068,51 @@ fx_status fx_solve_csr_noalloc(fx_ctx *ctx, int64_t n, int64_t nnz, const int64_
 // ---------------------------------------------------------------------------
 // StaticStep::solve (static_step.rs:47-267)
 // ---------------------------------------------------------------------------
+fx_status fx_batch_add_system(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *loads, int64_t n_bc,
+                              const int64_t *bc_dofs, const double *bc_targets, int32_t *slot_out) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(B && loads && slot_out, "fx_batch_add_system: NULL argument");
+    require(n_rhs >= 1 && n_rhs <= 4, "fx_batch_add_system: 1 to 4 right-hand sides per system");
+    require(n_bc >= 0 && (n_bc == 0 || (bc_dofs && bc_targets)), "fx_batch_add_system: BC arrays missing");
+    set_device(m->ctx);
+    hipStream_t s = m->ctx->stream;
+    const int64_t n = m->n_dof;
+    // the penalty records, uploaded when they differ from the last admission's
+    if (m->adm_key_dof.size() != (size_t)n_bc || !std::equal(bc_dofs, bc_dofs + n_bc, m->adm_key_dof.begin()) ||
+        !std::equal(bc_targets, bc_targets + n_bc, m->adm_key_val.begin())) {
+        BcGroups g = group_bcs(n_bc, bc_dofs, n);
+        m->adm_ud_dof.upload(g.ud_dof, s);
+        m->adm_ud_ptr.upload(g.ud_ptr, s);
+        m->adm_rec_order.upload(g.rec_order, s);
+        m->adm_target.upload(bc_targets, (size_t)n_bc, s);
+        m->adm_n_unique = (int64_t)g.ud_dof.size();
+        m->adm_key_dof.assign(bc_dofs, bc_dofs + n_bc);
+        m->adm_key_val.assign(bc_targets, bc_targets + n_bc);
+    }
+    if (m->adm_rhs_ev)  // another context's batch read the previous admission's right-hand sides
+        FX_HIP(hipStreamWaitEvent(s, m->adm_rhs_ev, 0));
+    m->adm_rhs.alloc((size_t)n * n_rhs);
+    FX_HIP(hipMemcpyAsync(m->adm_rhs.p, loads, (size_t)n * n_rhs * sizeof(double), hipMemcpyHostToDevice, s));
+    // Assembler::assemble with max_diag left on the device, then the penalty
+    // on K and the first right-hand side, and on the others alone
+    fx::launch_reset_status(m);
+    fx::launch_element(m, true, false, false);
+    fx::launch_assemble(m);
+    fx::launch_penalty_dev(m, m->adm_n_unique, m->adm_ud_dof.p, m->adm_ud_ptr.p, m->adm_target.p,
+                           m->adm_rec_order.p, m->adm_rhs.p, m->u.p);
+    for (int r = 1; r < n_rhs; ++r)
+        fx::launch_penalty_dev(m, m->adm_n_unique, m->adm_ud_dof.p, m->adm_ud_ptr.p, m->adm_target.p,
+                               m->adm_rec_order.p, m->adm_rhs.p + (size_t)r * n, m->u.p, true);
+    m->k_valid = true;
+    m->last_max_diag = std::nan("");  // on the device; read when a host path needs it
+    const fx_status st = fx::batch_add_impl(B, m, n_rhs, m->adm_rhs.p, true, m->status.p, slot_out);
+    if (st != FX_OK)
+        return st;  // batch_add_impl set the message
+    FX_API_END(m->ctx)
+}
+
 fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_step_report *report) {
     if (!m)
         return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");

@@ -366,15 +366,17 @@ __global__ void k_penalty(int64_t n_unique, const int32_t *__restrict__ ud_dof, 
     }
     const int d = ud_dof[j];
     const int slot = diag_slot[d];
-    double k = kval[slot];
+    double k = kval ? kval[slot] : 0.0;
     double r = rhs ? rhs[d] : 0.0;
     const double ud = u[d];
     for (int c = ud_ptr[j]; c < ud_ptr[j + 1]; ++c) {
         k += penalty;
         r += penalty * (rec_target[rec_order[c]] - ud);
     }
-    kval[slot] = k;
-    keep[slot] = 1;
+    if (kval) {  // kval == nullptr: one more right-hand side of a K already penalised
+        kval[slot] = k;
+        keep[slot] = 1;
+    }
     if (rhs)
         rhs[d] = r;
 }
@@ -550,6 +552,8 @@ void launch_penalty(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const i
                     const double *rec_target, const int32_t *rec_order, double *rhs, const double *u) {
     if (n_unique <= 0)
         return;
+    if (std::isnan(m->last_max_diag))  // assembled by fx_batch_add_system: max_diag is on the device
+        m->last_max_diag = device_max_diag(m);
     double pen = m->last_max_diag * kPenaltyScale;
     hipLaunchKernelGGL(k_penalty, dim3(grid1(n_unique, 256)), dim3(256), 0, m->ctx->stream, n_unique, ud_dof,
                        ud_ptr, rec_order, rec_target, m->diag_slot.p, pen, m->kval.p, m->keep.p, rhs, u,
@@ -558,12 +562,13 @@ void launch_penalty(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const i
 }
 
 void launch_penalty_dev(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const int32_t *ud_ptr,
-                        const double *rec_target, const int32_t *rec_order, double *rhs, const double *u) {
+                        const double *rec_target, const int32_t *rec_order, double *rhs, const double *u,
+                        bool rhs_only) {
     if (n_unique <= 0)
         return;
     hipLaunchKernelGGL(k_penalty, dim3(grid1(n_unique, 256)), dim3(256), 0, m->ctx->stream, n_unique, ud_dof,
-                       ud_ptr, rec_order, rec_target, m->diag_slot.p, 0.0, m->kval.p, m->keep.p, rhs, u,
-                       (const unsigned long long *)m->max_diag_bits.p, m->k_skip);
+                       ud_ptr, rec_order, rec_target, m->diag_slot.p, 0.0, rhs_only ? nullptr : m->kval.p,
+                       m->keep.p, rhs, u, (const unsigned long long *)m->max_diag_bits.p, m->k_skip);
     FX_HIP(hipGetLastError());
 }
 

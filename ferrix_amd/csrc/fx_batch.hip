@@ -1336,6 +1336,8 @@ void advance_phases(fx_batch *B, int set) {
                 sl.err = FX_ERR_BREAKDOWN;
             else if (F.status[r] == 3 && sl.err == FX_OK)
                 sl.err = FX_ERR_NOT_CONVERGED;
+            else if (F.status[r] >= 16 && sl.err == FX_OK)
+                sl.err = (fx_status)(F.status[r] - 16);  // the mesh's element kernels failed
         }
         sl.phase = 2;
         sl.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - sl.t0).count();
@@ -1459,7 +1461,33 @@ void fx_batch_destroy(fx_batch *B) {
     delete B;
 }
 
-fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs, int32_t *slot_out) {
+}  // extern "C"
+
+namespace fx {
+namespace {
+// interleaved right-hand sides b[i R + r] from rhs-major ones (device)
+__global__ void k_interleave(int64_t n, int R, const double *__restrict__ src, double *__restrict__ dst) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n * R)
+        dst[q] = src[(q % R) * n + q / R];
+}
+
+// A slot whose mesh's element kernels failed (singular Jacobian, no
+// material values) stops before its first iteration with that failure,
+// reported as status 16 + fx_status.
+__global__ void k_slot_elem_check(BSys *__restrict__ S, const DevStatus *__restrict__ st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || st->code == 0)
+        return;
+    for (int r = 0; r < S->R; ++r)
+        S->rhs[r].status = 16 + st->code;
+}
+}  // namespace
+
+// fx_batch_add with the right-hand sides on the host (rhs_dev false) or in
+// device memory written on the mesh's stream (rhs_dev true), and optionally
+// the mesh's element status to check on the device.
+fx_status batch_add_impl(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs, bool rhs_dev,
+                         const DevStatus *elem_status, int32_t *slot_out) {
     if (!B || !m || !rhs || !slot_out)
         return FX_ERR_INVALID;
     fx_ctx *ctx = B->ctx;
@@ -1497,12 +1525,24 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         sl.mesh = m;
         sl.R = n_rhs;
         const int64_t n = m->n_dof, nv = n * n_rhs;
-        // interleave the right-hand sides: host layout is rhs-major
-        std::vector<double> inter(nv);
-        for (int r = 0; r < n_rhs; ++r)
-            for (int64_t i = 0; i < n; ++i)
-                inter[i * n_rhs + r] = rhs[(int64_t)r * n + i];
-        sl.b.upload(ctx, inter, s);
+        // interleave the right-hand sides: the caller's layout is rhs-major
+        if (rhs_dev) {
+            sl.b.alloc(ctx, nv);
+            hipLaunchKernelGGL(k_interleave, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, n, n_rhs, rhs,
+                               sl.b.p);
+            FX_HIP(hipGetLastError());
+            if (m->ctx != ctx) {  // the mesh's stream may overwrite rhs only after this read
+                if (!m->adm_rhs_ev)
+                    FX_HIP(hipEventCreateWithFlags(&m->adm_rhs_ev, hipEventDisableTiming));
+                FX_HIP(hipEventRecord(m->adm_rhs_ev, s));
+            }
+        } else {
+            std::vector<double> inter(nv);
+            for (int r = 0; r < n_rhs; ++r)
+                for (int64_t i = 0; i < n; ++i)
+                    inter[i * n_rhs + r] = rhs[(int64_t)r * n + i];
+            sl.b.upload(ctx, inter, s);
+        }
         for (auto *v : {&sl.x, &sl.r, &sl.z, &sl.p0, &sl.p1, &sl.ap})
             if (v != &sl.z || !is_poly(B))
                 v->alloc(ctx, is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed solve stores FP32 vectors
@@ -1552,6 +1592,10 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         reserve_sys(B, B->h_sys.size());
         std::memset(&B->fin[k], 0, sizeof(fx::BFin));
         FX_HIP(hipMemcpyAsync(B->d_sys.p + k, &S, sizeof(BSys), hipMemcpyHostToDevice, s));
+        if (elem_status) {
+            hipLaunchKernelGGL(k_slot_elem_check, dim3(1), dim3(64), 0, s, B->d_sys.p + k, elem_status);
+            FX_HIP(hipGetLastError());
+        }
         if (is_poly(B))
             sl.pending = true;  // started with the other pending slots by the next fx_batch_run
         else
@@ -1572,6 +1616,14 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
         return FX_ERR_OOM;
     }
     return FX_OK;
+}
+
+}  // namespace fx
+
+extern "C" {
+
+fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs, int32_t *slot_out) {
+    return fx::batch_add_impl(B, m, n_rhs, rhs, false, nullptr, slot_out);
 }
 
 fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {

@@ -276,6 +276,15 @@ struct fx_mesh {
     // reset, gather, device-side penalty) return at once: the converged
     // Newton iteration's decision (fx_newton.hip); nullptr outside it
     const int32_t *k_skip = nullptr;
+    // fx_batch_add_system: the penalty records of the last admission (kept
+    // while they repeat), the right-hand sides it penalised, and the event
+    // after which another context's batch has read them
+    fx::DBuf<int32_t> adm_ud_dof, adm_ud_ptr, adm_rec_order;
+    fx::DBuf<double> adm_target, adm_rhs;
+    std::vector<int64_t> adm_key_dof;
+    std::vector<double> adm_key_val;
+    int64_t adm_n_unique = 0;
+    hipEvent_t adm_rhs_ev = nullptr;
     bool k_valid = false;
     int spmv_group = 0;                  // lanes per node row from fx_mesh_autotune_spmv (0: heuristic)
     int64_t device_bytes = 0;
@@ -283,6 +292,8 @@ struct fx_mesh {
     ~fx_mesh() {
         for (auto &g : graphs)
             (void)hipGraphExecDestroy(g.exec);
+        if (adm_rhs_ev)
+            (void)hipEventDestroy(adm_rhs_ev);
     }
 };
 
@@ -345,6 +356,8 @@ __host__ __device__ inline bool at_rounding_floor(double crit, double crit_prev,
     return crit > 0.5 * crit_prev && rnorm <= 64.0 * DBL_EPSILON * absnorm;
 }
 void batch_free(fx_batch *b);                 // deletes a batch (its context's stream is idle)
+fx_status batch_add_impl(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs, bool rhs_dev,
+                         const DevStatus *elem_status, int32_t *slot_out);
 void batch_drop_mesh(fx_batch *b, fx_mesh *m);  // removes the slots that use m
 void batch_drop_mesh_foreign(fx_mesh *m);        // the same in live batches of other contexts
 void axpy_dev(fx_mesh *m, int64_t n, double a, const double *x, double *y);
@@ -356,7 +369,8 @@ void launch_penalty(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const i
 // the same with the factor 1e6 x max_diag read on the device from the last
 // assembly (no penalty when max_diag is not positive)
 void launch_penalty_dev(fx_mesh *m, int64_t n_unique, const int32_t *ud_dof, const int32_t *ud_ptr,
-                        const double *rec_target, const int32_t *rec_order, double *rhs, const double *u);
+                        const double *rec_target, const int32_t *rec_order, double *rhs, const double *u,
+                        bool rhs_only = false);  // rhs_only: another right-hand side, K untouched
 void launch_reset_status(fx_mesh *m);  // reset_status by a kernel (graph capture)
 void launch_scatter(fx_mesh *m, int64_t n, const int32_t *idx, const double *val, double *dst);
 void launch_fill(fx_mesh *m, double *dst, int64_t n, double v, hipStream_t s = nullptr);

@@ -357,6 +357,23 @@ class Batch:
         self._n[slot.value] = rhs.shape
         return slot.value
 
+    def add_system(self, mesh: DeviceMesh, loads, bc_dof, bc_target) -> int:
+        """Assembles mesh's linear K, applies the penalty BCs (records
+        bc_dof, bc_target in deck order) to K and to every load vector
+        (shape (n_dof,) or (n_rhs, n_dof), before the penalty), and admits
+        the system, with no host wait (fx_batch_add_system); returns the
+        slot.  The same results as DeviceMesh.assemble + apply_penalty + add."""
+        loads = _c(np.atleast_2d(loads), np.float64)
+        assert loads.shape[1] == mesh.n_dof
+        bd = _c(bc_dof, np.int64)
+        bv = _c(bc_target, np.float64)
+        assert bd.shape == bv.shape
+        slot = C.c_int32()
+        self.ctx.check(self.lib.fx_batch_add_system(self.h, mesh.h, loads.shape[0], _p(loads), len(bd), _p(bd),
+                                                    _p(bv), C.byref(slot)))
+        self._n[slot.value] = loads.shape
+        return slot.value
+
     def run(self, max_iters: int = 0) -> int:
         """Advances the running systems; returns how many still run."""
         n = C.c_int32()

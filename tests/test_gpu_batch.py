@@ -437,3 +437,59 @@ def test_mesh_of_another_context_destroyed_while_batched(ctx):
     finally:
         bt.close()
         mctx.close()
+
+
+@pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED, L.FX_PREC_MIXED_POLY])
+def test_device_admission_bitwise(ctx, precision):
+    """fx_batch_add_system (assembly, penalty and admission queued with no
+    host wait) against assemble + apply_penalty + fx_batch_add: the same
+    solutions bit for bit, and the same iteration counts."""
+    from ferrix_amd.batch import System, solve_batched, upload
+    specs = small_specs(6, seed=41)
+    opts = SolveOptions(precision=precision, stop_mode=L.FX_STOP_REL_R2, tol=1e-10,
+                        max_iter=400_000 if precision == L.FX_PREC_MIXED_POLY else 100_000)
+    out = []
+    for dev in (True, False):
+        systems = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
+        upload(ctx, systems)
+        out.append(solve_batched(ctx, systems, opts, chunk_iters=16, device_admission=dev))
+        for s in systems:
+            s.dm.close()
+    a, b = out
+    assert not a.failures and not b.failures
+    for k in b.x:
+        assert np.array_equal(a.x[k], b.x[k]), f"system {k}"
+        assert a.iterations[k] == b.iterations[k]
+
+
+def test_device_admission_element_failure(ctx):
+    """A mesh whose element K fails (an element collapsed onto a plane)
+    admitted without a host wait: its slot finishes with the singular
+    Jacobian status, and its neighbours in the batch solve as alone."""
+    from ferrix_amd.batch import System, admit
+    specs = small_specs(2, seed=43)
+    good = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
+    bad_job = meshgen.cfg1_c3d4_cantilever()
+    m = bad_job.mesh
+    xyz = m.xyz.copy()
+    nodes = m.conn[m.el_ptr[5]:m.el_ptr[6]]
+    for nd in nodes[1:]:
+        xyz[nd] = xyz[nodes[0]]
+    m.xyz = xyz
+    bad = System(99, [bad_job])
+    for s in good + [bad]:
+        s.dm = DeviceMesh(ctx, s.jobs[0].mesh, s.jobs[0].node_temp)
+    bt = Batch(ctx, REL)
+    slots = [admit(bt, good[0]), admit(bt, bad), admit(bt, good[1])]
+    bt.run()
+    st_bad, _ = bt.poll(slots[1])
+    say("bad slot state", st_bad)
+    assert st_bad == -L.FX_ERR_SINGULAR_JACOBIAN
+    for s, slot in ((good[0], slots[0]), (good[1], slots[2])):
+        assert bt.poll(slot)[0] == 1
+        alone = Batch(ctx, REL)
+        k = admit(alone, s)
+        alone.run()
+        assert np.array_equal(bt.result(slot), alone.result(k))
+        alone.close()
+    bt.close()
