@@ -11,10 +11,11 @@ NLGEOM static job (Newton-Raphson with the reference's increment control,
   requests whose memory is not allocated yet (FIFO; the head of the queue
   waits until memory frees up).  A fixed budget over the estimates can be
   given instead;
-* uploads its mesh (host symbolic phase + copies), assembles K and applies
-  the penalty BCs on an admission thread -- the host work overlaps with the
-  running batch -- and adds the system to the running :class:`Batch`: it
-  joins the next iteration chunk while the other systems keep going;
+* uploads its mesh (host symbolic phase + copies) on an admission thread --
+  the host work overlaps with the running batch -- then queues its
+  assembly, penalty BCs and admission to the running :class:`Batch` with no
+  host wait (``fx_batch_add_system``): it joins the next iteration chunk
+  while the other systems keep going;
 * runs NLGEOM requests on worker threads, each with its own context and
   stream (``fx_static_step``: increments, element and assembly kernels on
   the worker's stream); every Newton iteration's linear solve joins the
@@ -41,7 +42,7 @@ import numpy as np
 
 from . import _lib as L
 from . import meshgen
-from .batch import System, assemble
+from .batch import System, admit, assemble
 from .device import Batch, Context, DeviceMesh, SolveOptions, run_job
 from .inp import ConvergenceError, Project, deck_stem
 from .model import Job
@@ -229,9 +230,9 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
         return need <= free + cached - mem_reserve - inflight
 
     def prepare(r: Request) -> System:
-        s = System(r.rid, r.jobs, DeviceMesh(pctx, r.jobs[0].mesh, r.jobs[0].node_temp))
-        assemble(s)
-        return s
+        # the host symbolic phase and the copies; assembly and penalty are
+        # queued with the admission (fx_batch_add_system, no host wait)
+        return System(r.rid, r.jobs, DeviceMesh(pctx, r.jobs[0].mesh, r.jobs[0].node_temp))
 
     t0 = time.perf_counter()
     logged = 0
@@ -270,7 +271,7 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
                     res.latency[r.rid] = time.perf_counter() - t0 - r.arrival
                     used -= need
                     continue
-                running[bt.add(s.dm, s.rhs)] = (r, s, need)
+                running[admit(bt, s)] = (r, s, need)
             # Newton iterations' solves of the NLGEOM workers join the batch
             while not nl_queue.empty():
                 job = nl_queue.get_nowait()
