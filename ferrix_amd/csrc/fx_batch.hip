@@ -864,6 +864,9 @@ struct fx_batch {
     // orders the batch's stream after work queued on the stream of an
     // admitted mesh's own context (re-recorded at each such admission)
     hipEvent_t adm_ev = nullptr;
+    // serialises the batch's calls with a foreign mesh's removal from
+    // another host thread (batch_drop_mesh_foreign)
+    std::mutex mx;
     fx_batch();
     ~fx_batch();
 };
@@ -882,7 +885,11 @@ fx_batch::fx_batch() {
 
 fx_batch::~fx_batch() {
     {
+        // the slots go first, under the registry's lock: a foreign mesh being
+        // destroyed meanwhile either drops its slot here before, or finds
+        // its count of foreign slots already down and no batch to visit
         std::lock_guard<std::mutex> lk(g_batches_mx);
+        slots.clear();
         g_batches.erase(std::find(g_batches.begin(), g_batches.end(), this));
     }
     {
@@ -1365,6 +1372,7 @@ void batch_drop_mesh_foreign(fx_mesh *m) {
             uses |= sl && sl->used && sl->mesh == m;
         if (!uses)
             continue;
+        std::lock_guard<std::mutex> bl(B->mx);  // no call of the batch runs meanwhile
         (void)hipSetDevice(B->ctx->device);
         (void)hipStreamSynchronize(B->ctx->stream);  // the batch's stream may still read the mesh
         batch_drop_mesh(B, m);
@@ -1490,6 +1498,7 @@ fx_status batch_add_impl(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *r
                          const DevStatus *elem_status, int32_t *slot_out) {
     if (!B || !m || !rhs || !slot_out)
         return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);
     fx_ctx *ctx = B->ctx;
     int k = -1;  // the slot taken, released again if admission fails
     try {
@@ -1629,6 +1638,7 @@ fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs
 fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
     if (!B)
         return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);
     fx_ctx *ctx = B->ctx;
     try {
         FX_HIP(hipSetDevice(ctx->device));
@@ -1680,6 +1690,9 @@ fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
 }
 
 fx_status fx_batch_poll(fx_batch *B, int32_t slot, int32_t *state, fx_solve_report *rep) {
+    if (!B)
+        return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);  // the slot table may change under a foreign mesh's removal
     if (!B || slot < 0 || slot >= (int)B->slots.size() || !B->slots[slot] || !B->slots[slot]->used)
         return FX_ERR_INVALID;
     auto &sl = *B->slots[slot];
@@ -1701,6 +1714,9 @@ fx_status fx_batch_poll(fx_batch *B, int32_t slot, int32_t *state, fx_solve_repo
 }
 
 fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
+    if (!B)
+        return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);  // the slot table may change under a foreign mesh's removal
     if (!B || !x || slot < 0 || slot >= (int)B->slots.size() || !B->slots[slot] || !B->slots[slot]->used)
         return FX_ERR_INVALID;
     auto &sl = *B->slots[slot];
@@ -1724,6 +1740,7 @@ fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
 fx_status fx_batch_time_spmv(fx_batch *B, int32_t on) {
     if (!B)
         return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);
     B->timing = on != 0;
     return FX_OK;
 }
@@ -1732,6 +1749,7 @@ fx_status fx_batch_spmv_stats(fx_batch *B, int64_t *launches, int64_t *timed, do
                               double *bytes_total) {
     if (!B)
         return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);
     double by = 0.0;
     try {
         FX_HIP(hipMemcpyAsync(&by, B->d_bytes.p, sizeof(double), hipMemcpyDeviceToHost, B->ctx->stream));
@@ -1752,6 +1770,9 @@ fx_status fx_batch_spmv_stats(fx_batch *B, int64_t *launches, int64_t *timed, do
 }
 
 fx_status fx_batch_remove(fx_batch *B, int32_t slot) {
+    if (!B)
+        return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);  // the slot table may change under a foreign mesh's removal
     if (!B || slot < 0 || slot >= (int)B->slots.size() || !B->slots[slot] || !B->slots[slot]->used)
         return FX_ERR_INVALID;
     (void)hipStreamSynchronize(B->ctx->stream);

@@ -388,7 +388,11 @@ fx_status fx_mesh_autotune_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, 
  * All calls run on the context stream.  A mesh may belong to another context
  * on the same device: its assembly can then run on that context's stream
  * (from another host thread) while the batch runs, and fx_batch_add orders
- * the batch after the work queued on the mesh's stream so far.
+ * the batch after the work queued on the mesh's stream so far.  Threads: the
+ * calls on one batch are serialised by the batch, and destroying a mesh (or
+ * its context) from another host thread while the mesh has slots in a batch
+ * of another context waits for that batch's calls and drops the slots; a
+ * batch itself must not be destroyed while another thread calls it.
  */
 typedef struct fx_batch fx_batch;
 

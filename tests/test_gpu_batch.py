@@ -493,3 +493,57 @@ def test_device_admission_element_failure(ctx):
         assert np.array_equal(bt.result(slot), alone.result(k))
         alone.close()
     bt.close()
+
+
+def test_foreign_mesh_destroyed_while_batch_runs(ctx):
+    """A batch on one context runs meshes of another context while a second
+    host thread destroys one of those meshes and then the other context:
+    the destroyed mesh's slot leaves the batch (its poll reports an invalid
+    slot), the other systems finish with the results they get alone, and
+    destroying the batch and the context in either order is clean."""
+    import threading
+    import time
+    from ferrix_amd.device import Context
+    specs = small_specs(3, seed=47)
+    alone = []
+    for sp in specs:
+        dm, b = system(ctx, meshgen.cfg4_mesh(sp))
+        bt = Batch(ctx, REL)
+        k = bt.add(dm, b)
+        bt.run()
+        alone.append(bt.result(k))
+        bt.close()
+        dm.close()
+    for rnd in range(3):
+        other = Context(0)
+        sys_ = [system(other, meshgen.cfg4_mesh(sp)) for sp in specs]
+        bt = Batch(ctx, REL)
+        slots = [bt.add(dm, b) for dm, b in sys_]
+        dropped = threading.Event()
+
+        def destroy():
+            time.sleep(0.002 * rnd)
+            sys_[1][0].close()
+            dropped.set()
+
+        th = threading.Thread(target=destroy)
+        th.start()
+        done = {}
+        while len(done) < 2:
+            bt.run(max_iters=8)
+            for i in (0, 2):
+                if i not in done and bt.poll(slots[i])[0] != 0:
+                    done[i] = bt.result(slots[i])
+        th.join()
+        assert dropped.is_set()
+        with pytest.raises(Exception):
+            bt.poll(slots[1])  # the slot left with its mesh
+        for i in (0, 2):
+            assert np.array_equal(done[i], alone[i]), (rnd, i)
+        if rnd % 2 == 0:
+            bt.close()
+            other.close()
+        else:
+            other.close()  # drops the remaining foreign slots first
+            bt.close()
+        say("round", rnd, "ok")
