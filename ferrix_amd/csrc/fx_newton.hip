@@ -81,6 +81,10 @@ NewtonGraph::NewtonGraph(fx_mesh *mesh) : m(mesh) {
 }
 
 NewtonGraph::~NewtonGraph() {
+    // an error can leave the step with launches still queued: the front
+    // graph and k_nr_report (which writes the pinned report) must finish
+    // before either is released
+    (void)hipStreamSynchronize(m->ctx->stream);
     if (front)
         (void)hipGraphExecDestroy(front);
     if (report)
