@@ -473,8 +473,9 @@ def run_batch(a):
             step = lambda: solve_batched(ctx, admit, mixed, ws_bytes=ws, time_spmv=True,
                                          assemble_ahead=a.assemble_ahead)
         elif arm == "mixed_poly":
+            shape = a.inner_shape if a.inner_shape == "tune" else int(a.inner_shape)
             step = lambda: solve_batched(ctx, admit, poly, ws_bytes=ws, time_spmv=True,
-                                         assemble_ahead=a.assemble_ahead)
+                                         assemble_ahead=a.assemble_ahead, inner_shape=shape)
         elif arm == "fp64_batched":
             step = lambda: solve_batched(ctx, admit, fp64_b, ws_bytes=ws, time_spmv=True,
                                          assemble_ahead=a.assemble_ahead)
@@ -508,6 +509,12 @@ def run_batch(a):
             "failures": len(last.failures),
             "pcg_iterations_total": int(sum(last.iterations.values())),
         }
+        if last.inner_tune:
+            sh, t1, t2 = last.inner_tune
+            res["inner_shape"] = {"chosen": "per_load_case_count" if sh else "one_launch",
+                                  "us_one_launch": 1e3 * t1, "us_per_load_case_count": 1e3 * t2}
+        elif arm == "mixed_poly":
+            res["inner_shape"] = {"chosen": "per_load_case_count" if a.inner_shape == "1" else "one_launch"}
         if last.spmv_stats:
             launches, nt, ms, by = last.spmv_stats
             res["spmv"] = {"launches": launches, "timed": nt, "us_per_launch": 1e3 * ms / max(nt, 1),
@@ -678,6 +685,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--inner-shape", choices=["tune", "0", "1"], default="tune",
+                    help="polynomial batch: Chebyshev step as one launch (0), one per load-case count (1), "
+                         "or the faster of both timed on the running systems (tune)")
     ap.add_argument("--poly-degree", type=int, default=0,
                     help="mixed_poly: Chebyshev steps per outer iteration (0: the library default, 15)")
     ap.add_argument("--sweep-groups", action="store_true", help="print SpMV time per lane-group size")

@@ -100,6 +100,7 @@ SIGNATURES = {
     "fx_measure_fp64_peak": (i32, [P, i32, P, P]),
     "fx_profile_element": (i32, [P, i32, i32, P, P]),
     "fx_mesh_autotune_spmv": (i32, [P, i32, i32, P, P]),
+    "fx_mesh_autotune_elements": (i32, [P, i32, i32, P, P, P]),
     "fx_batch_create": (i32, [P, P, P]),
     "fx_batch_destroy": (None, [P]),
     "fx_batch_add": (i32, [P, P, i32, P, P]),
@@ -109,6 +110,8 @@ SIGNATURES = {
     "fx_batch_get": (i32, [P, i32, P]),
     "fx_batch_remove": (i32, [P, i32]),
     "fx_batch_time_spmv": (i32, [P, i32]),
+    "fx_batch_autotune": (i32, [P, i32, P, P, P]),
+    "fx_batch_set_inner_shape": (i32, [P, i32]),
     "fx_batch_spmv_stats": (i32, [P, P, P, P, P]),
     "fx_solve_csr_noalloc": (i32, [P, i64, i64, P, P, P, P, P, P, P]),
 }

@@ -100,6 +100,7 @@ class PopulationResult:
     iters_rhs: Dict[int, List[int]] = field(default_factory=dict)  # serial: key -> per load case
     failures: Dict[int, str] = field(default_factory=dict)
     spmv_stats: tuple = ()  # batched: (launches, timed launches, timed ms, algorithmic bytes of all launches)
+    inner_tune: tuple = ()  # polynomial batch: (shape, ms one launch, ms split) of fx_batch_autotune
 
     def dof_solves(self, systems: List[System]) -> int:
         return sum(s.n_dof * s.n_rhs for s in systems if s.key in self.x)
@@ -133,7 +134,7 @@ def operator_bytes(s: System, opts: SolveOptions) -> int:
 
 def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int = 0, chunk_iters: int = 256,
                   time_spmv: bool = False, assemble_ahead: bool = False,
-                  device_admission: bool = True) -> PopulationResult:
+                  device_admission: bool = True, inner_shape="tune") -> PopulationResult:
     """Continuous batching over `systems` in order.  ws_bytes > 0 caps the
     summed operator bytes of the running systems (at least one always runs);
     0 admits everything at once.  Each Batch.run call advances the running
@@ -151,13 +152,22 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
     penalty and admission are queued on the device with no host wait
     (fx_batch_add_system); False: the host reads max_diag and the
     penalised right-hand sides back per system first (assemble()).  The
-    results are the same bit for bit."""
+    results are the same bit for bit.
+
+    inner_shape (polynomial batch): "tune" times the Chebyshev step's two
+    shapes on the running systems once, after the first run call, and keeps
+    the faster (fx_batch_autotune); 0 / 1 sets one (one launch per step, one
+    per load-case count).  Every shape gives the same bits."""
     if assemble_ahead and any(s.dm.ctx is ctx for s in systems):
         raise ValueError("assemble_ahead: the meshes must belong to another context than the batch")
     res = PopulationResult()
     bt = Batch(ctx, opts)
     if time_spmv:
         bt.time_spmv(True)
+    poly = opts.precision == L.FX_PREC_MIXED_POLY
+    tune = poly and inner_shape == "tune"
+    if poly and not tune:
+        bt.set_inner_shape(int(inner_shape))
     running: Dict[int, System] = {}   # slot -> system
     ws = 0
     ready: "queue.Queue[Optional[System]]" = queue.Queue()
@@ -227,7 +237,11 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
                 if exhausted and nxt is None:
                     break
                 continue
-            if bt.run(max_iters=chunk_iters) == len(running):
+            n_run = bt.run(max_iters=chunk_iters)
+            if tune and n_run > 0:
+                res.inner_tune = bt.autotune()
+                tune = False
+            if n_run == len(running):
                 continue  # none finished: nothing to collect, no memory freed
             for slot in list(running):
                 st, rep = bt.poll(slot)

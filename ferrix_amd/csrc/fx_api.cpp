@@ -551,6 +551,13 @@ fx_status fx_mesh_upload(fx_ctx *ctx, int64_t n_nodes, const double *xyz, const 
     m->nod_cptr.upload(S.nod_cptr, s);
     m->nod_cbase.upload(S.nod_cbase, s);
     m->diag_slot.upload(S.diag_slot, s);
+    // C3D4 elements per workgroup: 128 until fx_mesh_autotune_elements
+    // times the others; FX_C3D4_BLOCK=64/128/256/512 sets it at upload
+    if (const char *e = std::getenv("FX_C3D4_BLOCK")) {
+        const int bs = std::atoi(e);
+        require(bs == 64 || bs == 128 || bs == 256 || bs == 512, "FX_C3D4_BLOCK must be 64, 128, 256 or 512");
+        m->c3d4_block = bs;
+    }
     m->ke.alloc(std::max<int64_t>(1, m->ke_total));
     m->fe.alloc(std::max<int64_t>(1, m->fe_total));
     m->kval.alloc(m->nnz_struct);
@@ -682,6 +689,23 @@ fx_status fx_profile_element(fx_mesh *m, int32_t reps, int32_t mfma, double *ms_
     require(reps > 0 && ms_per_launch && flop_per_launch, "fx_profile_element: bad argument");
     set_device(m->ctx);
     *ms_per_launch = fx::profile_element(m, reps, mfma != 0, flop_per_launch);
+    FX_API_END(m->ctx)
+}
+
+fx_status fx_mesh_autotune_elements(fx_mesh *m, int32_t reps, int32_t allow_mfma, int32_t *c3d4_block_out,
+                                    int32_t *c3d20_mfma_out, double *ms_per_launch) {
+    if (!m)
+        return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
+    FX_API_BEGIN
+    require(reps > 0, "fx_mesh_autotune_elements: reps must be positive");
+    set_device(m->ctx);
+    const double ms = fx::autotune_elements(m, reps, allow_mfma != 0);
+    if (c3d4_block_out)
+        *c3d4_block_out = m->c3d4_block;
+    if (c3d20_mfma_out)
+        *c3d20_mfma_out = m->blk20.n_el ? m->c20_mfma : -1;
+    if (ms_per_launch)
+        *ms_per_launch = ms;
     FX_API_END(m->ctx)
 }
 

@@ -99,9 +99,10 @@ void launch_poly_power_start(hipStream_t s, BSys *sys, const int2 *tab_vec, int 
                              int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int32_t *list,
                              int n_list, BFin *fin);
 void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, int n_vec, const int2 *tab_in_sell,
-                       int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int2 *tab_out_sell,
-                       int n_out_sell, const int2 *tab_out, int n_out, const int32_t *list, int n_list, BFin *fin,
-                       hipEvent_t ev0, hipEvent_t ev1);
+                       int n_in_sell, const int *n_in_sell_r, bool split, const int2 *const *tab_in_g,
+                       const int *n_in_g, const int2 *tab_out_sell, int n_out_sell, const int2 *tab_out, int n_out,
+                       const int32_t *list, int n_list, BFin *fin, hipEvent_t ev0, hipEvent_t ev1);
+double time_poly_inner(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int reps);
 
 inline int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
 

@@ -839,6 +839,7 @@ struct fx_batch {
         fx::DBuf<int32_t> run;
         int n1g[kGroups] = {0, 0, 0, 0, 0, 0};
         int n1s = 0, n2 = 0, n3s = 0, n3 = 0, nrun = 0;
+        int n1s_r[fx::kRmax] = {0, 0, 0, 0};  // polynomial batch: tab1s segments by load-case count
         int64_t version = -1;          // of the running-slot set these tables were built for
         int2 *h_tab = nullptr;         // pinned staging of the tables
         size_t h_tab_cap = 0;
@@ -847,6 +848,9 @@ struct fx_batch {
         int ev_used = 0;
     };
     ChunkSet cs[2];
+    // polynomial batch: the sliced-ELL inner step as one launch (false) or
+    // one launch per load-case count (fx_batch_autotune)
+    bool inner_split = false;
     int64_t run_version = 0;           // bumped when the tables must be rebuilt
     int64_t tab_blocks = 0, dead_blocks = 0;  // in the current tables, and of slots finished since
     fx::BFin *fin = nullptr;           // pinned host memory, one per slot capacity
@@ -1065,9 +1069,33 @@ void ensure_pinned(int2 *&p, size_t &cap, size_t n) {
 // running slots.  The tables are staged in the set's pinned buffer and
 // uploaded on the stream without waiting: the set's previous chunk has been
 // processed (its event waited), so neither buffer is still being read.
+//
+// Every table is ordered by the slots' load-case count, four first: a
+// block's work grows with the count, so the longest blocks lead a launch
+// and the shortest fill its tail, and neighbouring workgroups on a CU carry
+// equal work.  Each count's segment of an SpMV-shaped table is permuted on
+// its own (xcd_permute; over the whole table the XCD ranges are then only
+// rotated, and a launch over one segment starts them at block 0).  The
+// order changes no value: a block's partial sums go to its own slot.
+struct ByCount {
+    std::vector<int2> r[kRmax];
+    void add(int R, int k, int b) { r[R - 1].push_back(make_int2(k, b)); }
+    std::vector<int2> flat(bool permute, int *counts = nullptr) {
+        std::vector<int2> t;
+        for (int q = kRmax - 1; q >= 0; --q) {
+            if (counts)
+                counts[q] = (int)r[q].size();
+            if (permute)
+                xcd_permute(r[q]);
+            t.insert(t.end(), r[q].begin(), r[q].end());
+        }
+        return t;
+    }
+};
+
 void build_tables(fx_batch *B, int set) {
     fx_batch::ChunkSet &C = B->cs[set];
-    std::vector<int2> t1g[fx_batch::kGroups], t1s, t2, t3s, t3;
+    ByCount b1g[fx_batch::kGroups], b1s, b2, b3s, b3;
     std::vector<int> run;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         auto &sl = *B->slots[k];
@@ -1076,23 +1104,32 @@ void build_tables(fx_batch *B, int set) {
         run.push_back((int)k);
         const BSys &S = B->h_sys[k];
         for (int b = 0; b < S.nparts1; ++b)
-            (S.sell ? t1s : t1g[group_index(S.G)]).push_back(make_int2((int)k, b));
+            (S.sell ? b1s : b1g[group_index(S.G)]).add(S.R, (int)k, b);
         for (int b = 0; b < S.nparts2; ++b)
-            t2.push_back(make_int2((int)k, b));
+            b2.add(S.R, (int)k, b);
         if (S.poly)
             for (int b = 0; b < S.nparts3; ++b)
-                (S.sell ? t3s : t3).push_back(make_int2((int)k, b));
+                (S.sell ? b3s : b3).add(S.R, (int)k, b);
     }
+    std::vector<int2> t1g[fx_batch::kGroups];
+    for (int g = 0; g < fx_batch::kGroups; ++g)
+        t1g[g] = b1g[g].flat(true);
+    std::vector<int2> t1s = b1s.flat(true, C.n1s_r), t2 = b2.flat(false), t3s = b3s.flat(true), t3 = b3.flat(true);
+    // staged: every table below and the running-slot list
     size_t total = t1s.size() + t2.size() + t3s.size() + t3.size() + (run.size() + 1) / 2 + 1;
     for (auto &t : t1g)
         total += t.size();
     ensure_pinned(C.h_tab, C.h_tab_cap, total);
     hipStream_t s = B->ctx->stream;
     size_t off = 0;
+    auto room = [&](size_t n) {
+        if (off + n > C.h_tab_cap)
+            throw fx::Error{FX_ERR_INVALID, "batch tables: staging buffer too small (internal error)"};
+    };
     auto stage = [&](fx::DBuf<int2> &d, std::vector<int2> &t) {
         if (t.empty())
             return;
-        xcd_permute(t);
+        room(t.size());
         std::copy(t.begin(), t.end(), C.h_tab + off);
         d.alloc(std::max<size_t>(d.n, t.size()));
         FX_HIP(hipMemcpyAsync(d.p, C.h_tab + off, t.size() * sizeof(int2), hipMemcpyHostToDevice, s));
@@ -1109,6 +1146,7 @@ void build_tables(fx_batch *B, int set) {
     C.n3 = (int)t3.size();
     stage(C.tab3, t3);
     C.n2 = (int)t2.size();
+    room(t2.size() + (run.size() + 1) / 2);
     std::copy(t2.begin(), t2.end(), C.h_tab + off);  // vector blocks: no XCD permutation
     if (!t2.empty()) {
         C.tab2.alloc(std::max<size_t>(C.tab2.n, t2.size()));
@@ -1259,8 +1297,9 @@ void launch_iterations(fx_batch *B, int n, int set) {
         for (int g = 0; g < fx_batch::kGroups; ++g)
             tg[g] = C.tab1g[g].p;
         for (int k = 0; k < n; ++k)
-            fx::launch_poly_outer(s, B->d_sys.p, B->poly_deg, C.tab2.p, C.n2, C.tab1s.p, C.n1s, tg, C.n1g,
-                                  C.tab3s.p, C.n3s, C.tab3.p, C.n3, C.run.p, C.nrun, B->fin,
+            fx::launch_poly_outer(s, B->d_sys.p, B->poly_deg, C.tab2.p, C.n2, C.tab1s.p, C.n1s, C.n1s_r,
+                                  B->inner_split, tg, C.n1g, C.tab3s.p, C.n3s, C.tab3.p, C.n3, C.run.p, C.nrun,
+                                  B->fin,
                                   B->timing ? C.ev[2 * k] : nullptr, B->timing ? C.ev[2 * k + 1] : nullptr);
         C.ev_used = B->timing ? 2 * n : 0;
         B->spmv_launches += (int64_t)n * B->poly_deg;
@@ -1730,6 +1769,51 @@ fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
         for (int r = 0; r < sl.R; ++r)
             for (int64_t i = 0; i < n; ++i)
                 x[(int64_t)r * n + i] = inter[i * sl.R + r];
+    } catch (const fx::Error &e) {
+        fx::set_error(e.msg);
+        return e.code;
+    }
+    return FX_OK;
+}
+
+fx_status fx_batch_set_inner_shape(fx_batch *B, int32_t shape) {
+    if (!B)
+        return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);
+    if (shape != 0 && shape != 1) {
+        fx::set_error("fx_batch_set_inner_shape: shape must be 0 or 1");
+        return FX_ERR_INVALID;
+    }
+    B->inner_split = shape == 1;
+    return FX_OK;
+}
+
+fx_status fx_batch_autotune(fx_batch *B, int32_t reps, int32_t *shape_out, double *ms_one, double *ms_split) {
+    if (!B)
+        return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);
+    try {
+        if (!is_poly(B) || reps <= 0)
+            throw fx::Error{FX_ERR_INVALID, "fx_batch_autotune: needs a polynomial batch (FX_PREC_MIXED_POLY) and reps > 0"};
+        FX_HIP(hipSetDevice(B->ctx->device));
+        hipStream_t s = B->ctx->stream;
+        FX_HIP(hipStreamSynchronize(s));
+        // the running systems' tables (no chunk is in flight between calls)
+        fx_batch::ChunkSet &C = B->cs[0];
+        if (C.version != B->run_version)
+            build_tables(B, 0);
+        double t1 = 0.0, t2 = 0.0;
+        if (C.n1s > 0) {
+            t1 = fx::time_poly_inner(s, B->d_sys.p, C.tab1s.p, C.n1s, C.n1s_r, false, reps);
+            t2 = fx::time_poly_inner(s, B->d_sys.p, C.tab1s.p, C.n1s, C.n1s_r, true, reps);
+            B->inner_split = t2 < t1;
+        }
+        if (shape_out)
+            *shape_out = B->inner_split ? 1 : 0;
+        if (ms_one)
+            *ms_one = t1;
+        if (ms_split)
+            *ms_split = t2;
     } catch (const fx::Error &e) {
         fx::set_error(e.msg);
         return e.code;

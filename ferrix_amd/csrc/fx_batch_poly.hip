@@ -385,6 +385,43 @@ __global__ __launch_bounds__(kBB) void kp_cheb_sell(BSys *__restrict__ sys, cons
     }
 }
 
+// The same inner step for the systems with RR load cases only: registers
+// and LDS sized for RR (the kernel above carries those of four, and its
+// occupancy with them).  The arithmetic is cheb_sell<RR>'s.
+template <int RR>
+__global__ __launch_bounds__(kBB) void kp_cheb_sell_r(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
+    const int2 t = tab[blockIdx.x];
+    const BSys &S = sys[t.x];
+    const int go = S.go;
+    if (!go)
+        return;
+    __shared__ double tile[kBB * 3 * RR];
+    cheb_sell<RR>(S, t.y, j, go, tile);
+}
+
+// Inner step j over the sliced-ELL table: one launch (split = false) or one
+// per load-case count over its segment (n_r[r]: entries with r + 1 load
+// cases; the segments run from four load cases down to one).
+void launch_cheb_sell(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int j) {
+    if (!split) {
+        hipLaunchKernelGGL(kp_cheb_sell, dim3(n), dim3(kBB), 0, s, sys, tab, j);
+        return;
+    }
+    int off = 0;
+    for (int r = kRmax - 1; r >= 0; --r) {
+        const int c = n_r[r];
+        if (!c)
+            continue;
+        switch (r) {
+        case 0: hipLaunchKernelGGL(kp_cheb_sell_r<1>, dim3(c), dim3(kBB), 0, s, sys, tab + off, j); break;
+        case 1: hipLaunchKernelGGL(kp_cheb_sell_r<2>, dim3(c), dim3(kBB), 0, s, sys, tab + off, j); break;
+        case 2: hipLaunchKernelGGL(kp_cheb_sell_r<3>, dim3(c), dim3(kBB), 0, s, sys, tab + off, j); break;
+        default: hipLaunchKernelGGL(kp_cheb_sell_r<4>, dim3(c), dim3(kBB), 0, s, sys, tab + off, j); break;
+        }
+        off += c;
+    }
+}
+
 template <int G, int RR>
 __device__ __forceinline__ void cheb_lanes(const BSys &S, int lb, int j, int go) {
     const int64_t a = (int64_t)lb * (kBB / G) + threadIdx.x / G;
@@ -731,15 +768,15 @@ void launch_poly_power_start(hipStream_t s, BSys *sys, const int2 *tab_vec, int 
 }
 
 void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, int n_vec, const int2 *tab_in_sell,
-                       int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int2 *tab_out_sell,
-                       int n_out_sell, const int2 *tab_out, int n_out, const int32_t *list, int n_list, BFin *fin,
-                       hipEvent_t ev0, hipEvent_t ev1) {
+                       int n_in_sell, const int *n_in_sell_r, bool split, const int2 *const *tab_in_g,
+                       const int *n_in_g, const int2 *tab_out_sell, int n_out_sell, const int2 *tab_out, int n_out,
+                       const int32_t *list, int n_list, BFin *fin, hipEvent_t ev0, hipEvent_t ev1) {
     if (ev0)
         FX_HIP(hipEventRecord(ev0, s));
     for (int j = 1; j <= deg; ++j)
         launch_inner_tables(tab_in_sell, n_in_sell, tab_in_g, n_in_g, [&](int G, const int2 *tab, int n) {
             switch (G) {
-            case 0: hipLaunchKernelGGL(kp_cheb_sell, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
+            case 0: launch_cheb_sell(s, sys, tab, n, n_in_sell_r, split, j); break;
             case 2: hipLaunchKernelGGL(kp_cheb_lanes<2>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
             case 4: hipLaunchKernelGGL(kp_cheb_lanes<4>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
             case 8: hipLaunchKernelGGL(kp_cheb_lanes<8>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
@@ -757,6 +794,36 @@ void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, i
     hipLaunchKernelGGL(kp_coef, dim3(n_list), dim3(kBB), 0, s, sys, list);
     hipLaunchKernelGGL(kp_upd, dim3(n_vec), dim3(kBB), 0, s, sys, tab_vec);
     hipLaunchKernelGGL(kp_stop, dim3(n_list), dim3(kBB), 0, s, sys, list, fin, 1);
+}
+
+// `reps` launches of the first inner step over the sliced-ELL table, in one
+// shape, between two events after one untimed launch; ms per inner step.
+// Between outer iterations this has no effect on the solves: the step reads
+// the next start z_1 and rs and writes only z_2, which the first inner step
+// of the next outer iteration rewrites from the same inputs before anything
+// reads it.
+double time_poly_inner(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int reps) {
+    hipEvent_t e0, e1;
+    FX_HIP(hipEventCreate(&e0));
+    FX_HIP(hipEventCreate(&e1));
+    float ms = 0.f;
+    try {
+        launch_cheb_sell(s, sys, tab, n, n_r, split, 1);
+        FX_HIP(hipEventRecord(e0, s));
+        for (int k = 0; k < reps; ++k)
+            launch_cheb_sell(s, sys, tab, n, n_r, split, 1);
+        FX_HIP(hipEventRecord(e1, s));
+        FX_HIP(hipGetLastError());
+        FX_HIP(hipEventSynchronize(e1));
+        FX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / reps;
 }
 
 }  // namespace fx

@@ -136,10 +136,13 @@ __device__ __forceinline__ void kblock(double dxa, double dya, double dza, doubl
 }
 
 // ---------------------------------------------------------------------------
-// C3D4: one lane per element
+// C3D4: one lane per element; the workgroup size (elements per workgroup,
+// 64..kC4MaxBlock) is the mesh's c3d4_block, which fx_mesh_autotune_elements
+// picks by timing.  An element's arithmetic does not depend on it.
 // ---------------------------------------------------------------------------
+constexpr int kC4MaxBlock = 512;
 template <bool DO_K, bool DO_F>
-__global__ __launch_bounds__(128) void k_c3d4(int64_t n_el, const int32_t *__restrict__ conn,
+__global__ __launch_bounds__(kC4MaxBlock) void k_c3d4(int64_t n_el, const int32_t *__restrict__ conn,
                                               const int32_t *__restrict__ mat, const int64_t *__restrict__ gpos,
                                               const double *__restrict__ xyz, const double *__restrict__ temp,
                                               const double *__restrict__ u, int nlgeom,
@@ -611,20 +614,23 @@ bool c20_mfma_on() {
     return e && e[0] == '1';
 }
 
+// types: bit 0 the C3D4 block, bit 1 the C3D20 block
 template <bool K, bool F>
 void launch_types(fx_mesh *m, const double *u, int nlgeom, const double *const *tab4, const double *const *tab20,
-                  bool c20_mfma) {
+                  bool c20_mfma, int types = 3) {
     hipStream_t s = m->ctx->stream;
-    if (m->blk4.n_el) {
+    if (m->blk4.n_el && (types & 1)) {
         const ElemBlock &b = m->blk4;
-        int threads = 128;
+        const int threads = m->c3d4_block;
         int64_t blocks = (b.n_el + threads - 1) / threads;
         hipLaunchKernelGGL((k_c3d4<K, F>), dim3((unsigned)blocks), dim3(threads), 0, s, b.n_el, b.conn.p, b.mat.p,
                            b.gpos.p, m->xyz.p, m->temp.p, u, nlgeom, m->mats.p, tab4[0], tab4[1], tab4[2],
                            m->ke.p + b.ke_offset, m->fe.p + b.fe_offset, m->status.p, K ? m->k_skip : nullptr);
         FX_HIP(hipGetLastError());
     }
-    if (m->blk20.n_el && K && c20_mfma) {
+    if (!(types & 2)) {
+        return;
+    } else if (m->blk20.n_el && K && c20_mfma) {
         const ElemBlock &b = m->blk20;
         hipLaunchKernelGGL((k_c3d20_mfma<F>), dim3((unsigned)b.n_el), dim3(kC20MThreads), 0, s, b.n_el, b.conn.p,
                            b.mat.p, b.gpos.p, m->xyz.p, m->temp.p, u, nlgeom, m->mats.p, tab20[0], tab20[1],
@@ -750,7 +756,7 @@ void launch_element(fx_mesh *m, bool want_k, bool want_f, bool nlgeom) {
     // only under NLGEOM; for a linear K it is not read at all
     const double *u = (want_f || nlgeom) ? m->u.p : nullptr;
     int nl = nlgeom ? 1 : 0;
-    const bool mf = c20_mfma_on();
+    const bool mf = m->c20_mfma >= 0 ? m->c20_mfma != 0 : c20_mfma_on();
     if (want_k && want_f)
         launch_types<true, true>(m, u, nl, tab4, tab20, mf);
     else if (want_k)
@@ -761,19 +767,19 @@ void launch_element(fx_mesh *m, bool want_k, bool want_f, bool nlgeom) {
 
 // fx_profile_element: `reps` launches of the element-K kernels (reference
 // coordinates, K only) between two HIP events after one warm-up launch.
-double profile_element(fx_mesh *m, int reps, bool mfma, double *flop) {
+double profile_element(fx_mesh *m, int reps, bool mfma, double *flop, int types) {
     DevTables *t = &m->ctx->tables;
     const double *tab4[3] = {t->n4.p, t->dn4.p, t->w4.p};
     const double *tab20[3] = {t->n20.p, t->dn20.p, t->w20.p};
     hipStream_t s = m->ctx->stream;
     reset_status(m);
-    launch_types<true, false>(m, nullptr, 0, tab4, tab20, mfma);
+    launch_types<true, false>(m, nullptr, 0, tab4, tab20, mfma, types);
     hipEvent_t e0, e1;
     FX_HIP(hipEventCreate(&e0));
     FX_HIP(hipEventCreate(&e1));
     FX_HIP(hipEventRecord(e0, s));
     for (int k = 0; k < reps; ++k)
-        launch_types<true, false>(m, nullptr, 0, tab4, tab20, mfma);
+        launch_types<true, false>(m, nullptr, 0, tab4, tab20, mfma, types);
     FX_HIP(hipEventRecord(e1, s));
     FX_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -781,8 +787,41 @@ double profile_element(fx_mesh *m, int reps, bool mfma, double *flop) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     check_status(m, "fx_profile_element");
-    *flop = 2592.0 * (double)m->blk4.n_el + (mfma ? 860160.0 : 1283040.0) * (double)m->blk20.n_el;
+    *flop = ((types & 1) ? 2592.0 * (double)m->blk4.n_el : 0.0) +
+            ((types & 2) ? (mfma ? 860160.0 : 1283040.0) * (double)m->blk20.n_el : 0.0);
     return (double)ms / reps;
+}
+
+// fx_mesh_autotune_elements: the C3D4 workgroup size 64 / 128 / 256 / 512
+// and, when allowed, the C3D20 kernel (VALU or MFMA), each the fastest of
+// `reps` timed element-K launches of its own element block; kept in the
+// mesh.  Returns the chosen kernels' time per launch over both blocks.
+double autotune_elements(fx_mesh *m, int reps, bool allow_mfma) {
+    double flop = 0.0;
+    if (m->blk4.n_el) {
+        const int keep = m->c3d4_block;
+        double best_t = 0.0;
+        int best = keep;
+        try {
+            for (int bs : {64, 128, 256, kC4MaxBlock}) {
+                m->c3d4_block = bs;
+                const double t = profile_element(m, reps, false, &flop, 1);
+                if (best_t == 0.0 || t < best_t) {
+                    best_t = t;
+                    best = bs;
+                }
+            }
+        } catch (...) {
+            m->c3d4_block = keep;
+            throw;
+        }
+        m->c3d4_block = best;
+    }
+    if (m->blk20.n_el)
+        m->c20_mfma = allow_mfma && profile_element(m, reps, true, &flop, 2) < profile_element(m, reps, false, &flop, 2)
+                          ? 1
+                          : 0;
+    return profile_element(m, reps, m->c20_mfma == 1, &flop, 3);
 }
 
 }  // namespace fx

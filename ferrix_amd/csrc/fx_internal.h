@@ -287,6 +287,8 @@ struct fx_mesh {
     hipEvent_t adm_rhs_ev = nullptr;
     bool k_valid = false;
     int spmv_group = 0;                  // lanes per node row from fx_mesh_autotune_spmv (0: heuristic)
+    int c3d4_block = 128;                // threads per workgroup of k_c3d4 (fx_mesh_autotune_elements)
+    int c20_mfma = -1;                   // C3D20 element K on MFMA: 1 / 0, -1: FX_C3D20_MFMA decides
     int64_t device_bytes = 0;
     std::vector<fx::PcgGraph> graphs;    // captured iteration batches
     ~fx_mesh() {
@@ -300,7 +302,8 @@ struct fx_mesh {
 namespace fx {
 // launchers (defined in the .hip translation units)
 void launch_element(fx_mesh *m, bool want_k, bool want_f, bool nlgeom);
-double profile_element(fx_mesh *m, int reps, bool mfma, double *flop);  // ms per launch
+double profile_element(fx_mesh *m, int reps, bool mfma, double *flop, int types = 3);  // ms per launch
+double autotune_elements(fx_mesh *m, int reps, bool allow_mfma);                      // ms per launch
 void launch_recover(fx_mesh *m);         // per (element, node) strain and stress at the resident u
 void launch_gather_recover(fx_mesh *m, double *S, double *E);  // nodal averages (device arrays)
 void launch_assemble(fx_mesh *m);

@@ -218,6 +218,16 @@ class DeviceMesh:
         self.ctx.check(self.lib.fx_mesh_autotune_spmv(self.h, precision, reps, C.byref(g), C.byref(ms)))
         return g.value, ms.value
 
+    def autotune_elements(self, reps: int = 20, allow_mfma: bool = False):
+        """Times the element-K kernels (C3D4 at 64..512 elements per
+        workgroup; C3D20 on VALU and, if allowed, on MFMA) and keeps the
+        fastest for this mesh; returns (C3D4 workgroup size, C3D20 on MFMA
+        (-1: no C3D20 elements), ms per launch of the chosen kernels)."""
+        b, mf, ms = C.c_int32(), C.c_int32(), C.c_double()
+        self.ctx.check(self.lib.fx_mesh_autotune_elements(self.h, reps, 1 if allow_mfma else 0, C.byref(b),
+                                                          C.byref(mf), C.byref(ms)))
+        return b.value, mf.value, ms.value
+
     def get_u(self) -> np.ndarray:
         u = np.zeros(self.n_dof)
         self.ctx.check(self.lib.fx_state_get_u(self.h, _p(u)))
@@ -406,6 +416,18 @@ class Batch:
     def remove(self, slot: int):
         self.ctx.check(self.lib.fx_batch_remove(self.h, slot))
         self._n.pop(slot, None)
+
+    def autotune(self, reps: int = 10):
+        """Polynomial batch: times the Chebyshev step as one launch and as one
+        launch per load-case count on the running systems and keeps the
+        faster; returns (shape, ms one launch, ms split)."""
+        sh, a, b = C.c_int32(), C.c_double(), C.c_double()
+        self.ctx.check(self.lib.fx_batch_autotune(self.h, reps, C.byref(sh), C.byref(a), C.byref(b)))
+        return sh.value, a.value, b.value
+
+    def set_inner_shape(self, shape: int):
+        """Polynomial batch: 0 one launch per Chebyshev step, 1 one per load-case count."""
+        self.ctx.check(self.lib.fx_batch_set_inner_shape(self.h, int(shape)))
 
     def time_spmv(self, on: bool = True):
         """Bracket every later SpMV launch with HIP events."""

@@ -372,6 +372,19 @@ fx_status fx_profile_element(fx_mesh *mesh, int32_t reps, int32_t mfma, double *
 fx_status fx_mesh_autotune_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_t *spmv_group_out,
                                 double *ms_per_launch);
 
+/* Element-kernel autotuner (no counterpart in the reference): times `reps`
+ * element-K launches (reference coordinates) of the C3D4 kernel at 64, 128,
+ * 256 and 512 elements per workgroup and keeps the fastest for the mesh
+ * (default 128); with allow_mfma != 0 it also times the C3D20 kernel on
+ * VALU and on FP64 MFMA (the FX_C3D20_MFMA=1 kernel, within 1e-13 of the
+ * oracle rather than bit for bit) and keeps the faster, else the VALU
+ * kernel.  The workgroup size does not change any value.  *c3d4_block_out,
+ * *c3d20_mfma_out (-1 without C3D20 elements) and *ms_per_launch (the
+ * chosen kernels over all elements) may be NULL.  Overwrites the element
+ * buffers: call between steps or before assembling. */
+fx_status fx_mesh_autotune_elements(fx_mesh *mesh, int32_t reps, int32_t allow_mfma, int32_t *c3d4_block_out,
+                                    int32_t *c3d20_mfma_out, double *ms_per_launch);
+
 /* ---- batched solves ------------------------------------------------------
  * The reference runs one job at a time: one Solver::solve per step on one K
  * (solvers/mod.rs:18-44, static_step.rs:200-228).  A batch replaces N such
@@ -436,6 +449,23 @@ fx_status fx_batch_get(fx_batch *batch, int32_t slot, double *x);
 
 /* Frees the slot for reuse (a running system is dropped). */
 fx_status fx_batch_remove(fx_batch *batch, int32_t slot);
+
+/* Inner-step autotuner of the polynomial batch (FX_PREC_MIXED_POLY; no
+ * counterpart in the reference, whose solver has no batch).  The batch runs
+ * its sliced-ELL Chebyshev step in one of two shapes: 0, one launch over
+ * every running system, with the registers and LDS that four load cases
+ * need; 1, one launch per load-case count, each sized for its own count
+ * (more wavefronts in flight for systems with one or two load cases, at up
+ * to three more launches per step).  Both compute the same values bit for
+ * bit.  fx_batch_autotune times `reps` launches of each shape on the
+ * batch's running systems (call it between fx_batch_run calls while systems
+ * run; the timed steps do not change any solve) and keeps the faster;
+ * *shape_out and the two times in ms per inner step (0 with no sliced-ELL
+ * system running) may be NULL.  fx_batch_set_inner_shape sets the shape
+ * directly (default 0). */
+fx_status fx_batch_autotune(fx_batch *batch, int32_t reps, int32_t *shape_out, double *ms_one,
+                            double *ms_split);
+fx_status fx_batch_set_inner_shape(fx_batch *batch, int32_t shape);
 
 /* Instrumentation (no counterpart in the reference): on != 0 brackets every
  * SpMV launch of later fx_batch_run calls with two HIP events on the

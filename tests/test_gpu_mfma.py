@@ -10,6 +10,10 @@ Bars (its summation order is not the oracle's, so no bitwise bar):
   oracle's;
 * the NLGEOM beam (K rebuilt every Newton iteration): the oracle's number of
   linear solves and U within 1e-7, as the VALU kernel's test asks.
+
+The element autotuner (fx_mesh_autotune_elements) is tested here too: the
+C3D4 workgroup size changes no bit of K, and the C3D20 kernel it keeps is
+the VALU one unless MFMA is allowed.
 """
 import numpy as np
 import pytest
@@ -84,3 +88,46 @@ def test_nlgeom_beam(ctx, mfma):
     say("solves", reps[0].linear_solves, logs[0].linear_solves, "U rel err", e)
     assert reps[0].linear_solves == logs[0].linear_solves
     assert e < 1e-7
+
+
+@pytest.mark.parametrize("block", [64, 256, 512])
+def test_c3d4_workgroup_sizes_bitwise(ctx, monkeypatch, block):
+    """k_c3d4 at 64 / 256 / 512 elements per workgroup (FX_C3D4_BLOCK at
+    upload) assembles the default 128's K bit for bit; the element autotuner
+    picks one of the four and leaves K unchanged."""
+    job = meshgen.cfg4_mesh(meshgen.cfg4_batch_specs(n_meshes=4, seed=9, dof_lo=2e3, dof_hi=2e4)[1])[0]
+    ref = DeviceMesh(ctx, job.mesh, job.node_temp)
+    md0, _ = ref.assemble(False)
+    K0 = ref.matrix()
+    monkeypatch.setenv("FX_C3D4_BLOCK", str(block))
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    md1, _ = dm.assemble(False)
+    assert md1 == md0
+    for a, b in zip(dm.matrix(), K0):
+        assert np.array_equal(a, b)
+    bs, c20, ms = ref.autotune_elements(reps=3)
+    say("block", block, "tuned", bs, c20, f"{ms * 1e3:.1f} us")
+    assert bs in (64, 128, 256, 512) and c20 == -1 and ms > 0
+    ref.assemble(False)
+    for a, b in zip(ref.matrix(), K0):
+        assert np.array_equal(a, b)
+
+
+def test_element_autotune_c3d20(ctx):
+    """C3D20: without MFMA allowed the tuner keeps the VALU kernel (K bit for
+    bit); allowed, it keeps the faster one (K within 1e-13 of the VALU's)."""
+    job = meshgen.cfg2_c3d20_block(seed=5, nx=10, ny=4, nz=3, jitter=True)
+    n = job.n_dof
+    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
+    dm.assemble(False)
+    K0 = dense(*dm.matrix(), n)
+    bs, c20, ms = dm.autotune_elements(reps=3, allow_mfma=False)
+    assert c20 == 0
+    dm.assemble(False)
+    assert np.array_equal(dense(*dm.matrix(), n), K0)
+    bs, c20, ms = dm.autotune_elements(reps=3, allow_mfma=True)
+    say("C3D20 on MFMA:", c20, f"{ms * 1e3:.1f} us per launch")
+    assert c20 in (0, 1)
+    dm.assemble(False)
+    K1 = dense(*dm.matrix(), n)
+    assert np.max(np.abs(K1 - K0)) <= 1e-13 * np.max(np.abs(K0))

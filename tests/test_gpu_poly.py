@@ -20,7 +20,7 @@ import oracle as O
 from ferrix_amd import _lib as L
 from ferrix_amd import meshgen
 from ferrix_amd.device import Batch, DeviceMesh, SolveOptions
-from ferrix_amd.inp import ConvergenceError
+from ferrix_amd.inp import ConvergenceError, InvalidModelState
 
 from test_gpu_batch import population, small_specs, system
 
@@ -186,3 +186,42 @@ def test_poly_single_solve_is_rejected(ctx):
     from ferrix_amd.inp import InvalidModelState
     with pytest.raises(InvalidModelState, match="batch precision"):
         dm.solve(np.ones(job.n_dof), POLY)
+
+
+def test_poly_inner_shapes_bitwise(ctx):
+    """The Chebyshev step as one launch (shape 0), as one launch per
+    load-case count (shape 1), and the shape fx_batch_autotune picks after
+    timing both on the running systems mid-solve: the same bits."""
+    specs = small_specs(8, seed=41)
+    sys = population(ctx, specs)
+    counts = sorted({b.shape[0] for _, b in sys})
+    say("load-case counts", counts)
+    assert len(counts) >= 2
+    out, tuned = [], None
+    for shape in (0, 1, "tune"):
+        bt = Batch(ctx, POLY)
+        if shape != "tune":
+            bt.set_inner_shape(shape)
+        slots = [bt.add(dm, b) for dm, b in sys]
+        if shape == "tune":
+            assert bt.run(max_iters=45) > 0
+            tuned = bt.autotune(reps=5)
+            say("autotune: shape", tuned[0], "ms one launch", tuned[1], "ms split", tuned[2])
+            assert tuned[0] in (0, 1) and tuned[1] > 0 and tuned[2] > 0
+        assert bt.run() == 0
+        out.append([(bt.result(s), bt.poll(s)[1].iterations) for s in slots])
+        bt.close()
+    for k in range(len(sys)):
+        for other in out[1:]:
+            assert np.array_equal(out[0][k][0], other[k][0]), f"system {k}"
+            assert out[0][k][1] == other[k][1]
+    assert max(oracle_errors(meshgen.cfg4_mesh(specs[0]), out[1][0][0], sys[0][1])) < 1e-8
+
+
+def test_poly_autotune_needs_poly_batch(ctx):
+    bt = Batch(ctx, SolveOptions(precision=L.FX_PREC_FP64, stop_mode=L.FX_STOP_REL_R2, tol=1e-10))
+    with pytest.raises(InvalidModelState, match="polynomial batch"):
+        bt.autotune()
+    with pytest.raises(InvalidModelState, match="shape must be 0 or 1"):
+        bt.set_inner_shape(2)
+    bt.close()
