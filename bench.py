@@ -248,7 +248,9 @@ def run_elements(a):
     """C3D20 element stiffness on the cfg2 block (4,000 elements): the VALU
     kernel (the oracle's order, bit for bit) against the FP64-MFMA one
     (FX_C3D20_MFMA=1), each timed over --steps launches between HIP events,
-    beside the device's measured FP64 ceilings."""
+    beside the device's measured FP64 ceilings.  Then the C3D4 kernel on the
+    batch population's largest mesh at 64 / 128 / 256 / 512 elements per
+    workgroup and the element autotuner's choice."""
     from ferrix_amd import meshgen
     from ferrix_amd.device import Context, DeviceMesh
 
@@ -266,13 +268,33 @@ def run_elements(a):
                      "frac_of_datasheet": round(tf / FP64_PEAK_TFLOPS, 4),
                      "frac_of_measured_ceiling": round(tf / ceiling, 4)}
         print(f"# {name}: {json.dumps(out[name])}", flush=True)
+    # C3D4 element batch: elements per workgroup on the population's largest
+    # mesh, and the element autotuner's choice
+    j4 = meshgen.cfg4_mesh(max(meshgen.cfg4_batch_specs(), key=lambda sp: sp["n_dof"]))[0]
+    c3d4 = {"n_elements": j4.mesh.n_el, "us_per_launch": {}}
+    old = os.environ.get("FX_C3D4_BLOCK")
+    try:
+        for bs in (64, 128, 256, 512):
+            os.environ["FX_C3D4_BLOCK"] = str(bs)
+            d4 = DeviceMesh(ctx, j4.mesh, j4.node_temp)
+            c3d4["us_per_launch"][bs] = round(d4.profile_element(reps, False)[0] * 1e3, 2)
+            d4.close()
+    finally:
+        if old is None:
+            os.environ.pop("FX_C3D4_BLOCK", None)
+        else:
+            os.environ["FX_C3D4_BLOCK"] = old
+    d4 = DeviceMesh(ctx, j4.mesh, j4.node_temp)
+    bs, _, ms = d4.autotune_elements(reps)
+    c3d4["tuned_block"], c3d4["tuned_us"] = bs, round(ms * 1e3, 2)
+    print(f"# c3d4: {json.dumps(c3d4)}", flush=True)
     m = out["mfma"]
     print(json.dumps({
         "metric": "C3D20 element stiffness throughput (elements/s)", "value": round(job.mesh.n_el / (m["us_per_launch"] * 1e-6), 1),
         "unit": "elements/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": m["us_per_launch"] * 1e-3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "c3d20_element_k_40x10x10_block", "n_elements": job.mesh.n_el},
-        "arms": out, "fp64_ceilings_tflops": {"datasheet": FP64_PEAK_TFLOPS, "measured_valu": round(valu_peak, 2),
+        "arms": out, "c3d4_workgroup": c3d4, "fp64_ceilings_tflops": {"datasheet": FP64_PEAK_TFLOPS, "measured_valu": round(valu_peak, 2),
                                               "measured_mfma": round(mfma_peak, 2)},
         "roofline": {"bound": "mfma", "achieved": m["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": m["frac_of_datasheet"], "traffic": None, "kernel": "k_c3d20_mfma"}}), flush=True)
@@ -509,6 +531,8 @@ def run_batch(a):
             "failures": len(last.failures),
             "pcg_iterations_total": int(sum(last.iterations.values())),
         }
+        if arm != "fp64_serial":
+            res["first_run_at_ms"] = 1e3 * float(np.mean([r.first_run_at for r in runs]))
         if last.inner_tune:
             sh, t1, t2 = last.inner_tune
             res["inner_shape"] = {"chosen": "per_load_case_count" if sh else "one_launch",

@@ -101,6 +101,7 @@ class PopulationResult:
     failures: Dict[int, str] = field(default_factory=dict)
     spmv_stats: tuple = ()  # batched: (launches, timed launches, timed ms, algorithmic bytes of all launches)
     inner_tune: tuple = ()  # polynomial batch: (shape, ms one launch, ms split) of fx_batch_autotune
+    first_run_at: float = 0.0  # batched: s from the start to the first run call (the admissions before it)
 
     def dof_solves(self, systems: List[System]) -> int:
         return sum(s.n_dof * s.n_rhs for s in systems if s.key in self.x)
@@ -237,6 +238,8 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
                 if exhausted and nxt is None:
                     break
                 continue
+            if not res.first_run_at:
+                res.first_run_at = time.perf_counter() - t0
             n_run = bt.run(max_iters=chunk_iters)
             if tune and n_run > 0:
                 res.inner_tune = bt.autotune()

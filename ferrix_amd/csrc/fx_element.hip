@@ -794,33 +794,46 @@ double profile_element(fx_mesh *m, int reps, bool mfma, double *flop, int types)
 
 // fx_mesh_autotune_elements: the C3D4 workgroup size 64 / 128 / 256 / 512
 // and, when allowed, the C3D20 kernel (VALU or MFMA), each the fastest of
-// `reps` timed element-K launches of its own element block; kept in the
-// mesh.  Returns the chosen kernels' time per launch over both blocks.
+// `reps` timed element-K launches of its own element block, kept in the
+// mesh and stored for the block's bucket (element type, element count); a
+// bucket already timed on the context gives its choice untimed.  Returns the
+// chosen kernels' time per launch over both blocks.
 double autotune_elements(fx_mesh *m, int reps, bool allow_mfma) {
     double flop = 0.0;
     if (m->blk4.n_el) {
-        const int keep = m->c3d4_block;
-        double best_t = 0.0;
-        int best = keep;
-        try {
-            for (int bs : {64, 128, 256, kC4MaxBlock}) {
-                m->c3d4_block = bs;
-                const double t = profile_element(m, reps, false, &flop, 1);
-                if (best_t == 0.0 || t < best_t) {
-                    best_t = t;
-                    best = bs;
+        const auto key = tune_key(2, 4, m->blk4.n_el);
+        int best = tune_get(m->ctx, key);
+        if (best < 0) {
+            const int keep = m->c3d4_block;
+            double best_t = 0.0;
+            try {
+                for (int bs : {64, 128, 256, kC4MaxBlock}) {
+                    m->c3d4_block = bs;
+                    const double t = profile_element(m, reps, false, &flop, 1);
+                    if (best_t == 0.0 || t < best_t) {
+                        best_t = t;
+                        best = bs;
+                    }
                 }
+            } catch (...) {
+                m->c3d4_block = keep;
+                throw;
             }
-        } catch (...) {
-            m->c3d4_block = keep;
-            throw;
+            tune_put(m->ctx, key, best);
         }
         m->c3d4_block = best;
     }
-    if (m->blk20.n_el)
-        m->c20_mfma = allow_mfma && profile_element(m, reps, true, &flop, 2) < profile_element(m, reps, false, &flop, 2)
-                          ? 1
-                          : 0;
+    if (m->blk20.n_el) {
+        const auto key = tune_key(3, allow_mfma ? 20 : 0, m->blk20.n_el);
+        int mf = tune_get(m->ctx, key);
+        if (mf < 0) {
+            mf = allow_mfma && profile_element(m, reps, true, &flop, 2) < profile_element(m, reps, false, &flop, 2)
+                     ? 1
+                     : 0;
+            tune_put(m->ctx, key, mf);
+        }
+        m->c20_mfma = mf;
+    }
     return profile_element(m, reps, m->c20_mfma == 1, &flop, 3);
 }
 

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <array>
 #include <map>
 #include <atomic>
 #include <mutex>
@@ -171,9 +172,33 @@ struct fx_ctx {
     std::mutex pool_mx;
     std::multimap<size_t, void *> pool;
     size_t pool_bytes = 0;
+    // autotuner choices by size bucket (fx::tune_key): a mesh whose bucket
+    // was already timed on this context takes the stored choice untimed
+    std::mutex tune_mx;
+    std::map<std::array<int64_t, 3>, int> tune;
 };
 
 namespace fx {
+// Autotuner bucket: (what was tuned, a shape class, floor(log2(size))).
+inline std::array<int64_t, 3> tune_key(int64_t what, int64_t cls, int64_t size) {
+    int64_t lg = 0;
+    while (size > 1) {
+        size >>= 1;
+        ++lg;
+    }
+    return {what, cls, lg};
+}
+// The stored choice of a bucket, or -1.
+inline int tune_get(fx_ctx *ctx, const std::array<int64_t, 3> &k) {
+    std::lock_guard<std::mutex> lk(ctx->tune_mx);
+    auto it = ctx->tune.find(k);
+    return it == ctx->tune.end() ? -1 : it->second;
+}
+inline void tune_put(fx_ctx *ctx, const std::array<int64_t, 3> &k, int v) {
+    std::lock_guard<std::mutex> lk(ctx->tune_mx);
+    ctx->tune[k] = v;
+}
+
 // A device block of at least `bytes` from the context's pool (the smallest
 // that fits, if it is at most twice the request), else a new hipMalloc;
 // *cap: the block's size.

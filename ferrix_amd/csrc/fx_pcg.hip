@@ -387,6 +387,18 @@ double profile_spmv(fx_mesh *m, int precision, int reps, int G, double *bytes) {
 // the fastest as the mesh's group, which solves and batches then use when
 // their options leave spmv_group at 0.
 int autotune_spmv(fx_mesh *m, int precision, int reps, double *best_ms) {
+    // bucket: precision, block columns per node row, DOF count
+    const auto key = tune_key(1, 256 * precision + (m->nblk + m->n_nodes / 2) / std::max<int64_t>(1, m->n_nodes),
+                              m->n_dof);
+    const int known = tune_get(m->ctx, key);
+    if (known > 0) {
+        double bytes = 0.0;
+        const double t = profile_spmv(m, precision, reps, known, &bytes);
+        m->spmv_group = known;
+        if (best_ms)
+            *best_ms = t;
+        return known;
+    }
     int best = 0;
     double best_t = HUGE_VAL;
     for (int G = 2; G <= 64; G *= 2) {
@@ -398,6 +410,7 @@ int autotune_spmv(fx_mesh *m, int precision, int reps, double *best_ms) {
         }
     }
     m->spmv_group = best;
+    tune_put(m->ctx, key, best);
     if (best_ms)
         *best_ms = best_t;
     return best;

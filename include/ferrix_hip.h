@@ -368,7 +368,10 @@ fx_status fx_profile_element(fx_mesh *mesh, int32_t reps, int32_t mfma, double *
  * mesh's group; fx_solve and fx_batch_add use it whenever their options
  * leave spmv_group at 0 (until then a row-length heuristic picks it).
  * *spmv_group_out, *ms_per_launch (either may be NULL): the choice and its
- * time.  Overwrites the mesh's PCG work vectors; call between solves. */
+ * time.  The choice is stored on the context for the mesh's bucket
+ * (precision, block columns per node row, floor(log2(n_dof))): a later mesh
+ * of the same bucket takes it with one timed run of that group instead of
+ * the sweep.  Overwrites the mesh's PCG work vectors; call between solves. */
 fx_status fx_mesh_autotune_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, int32_t *spmv_group_out,
                                 double *ms_per_launch);
 
@@ -378,7 +381,10 @@ fx_status fx_mesh_autotune_spmv(fx_mesh *mesh, int32_t precision, int32_t reps, 
  * (default 128); with allow_mfma != 0 it also times the C3D20 kernel on
  * VALU and on FP64 MFMA (the FX_C3D20_MFMA=1 kernel, within 1e-13 of the
  * oracle rather than bit for bit) and keeps the faster, else the VALU
- * kernel.  The workgroup size does not change any value.  *c3d4_block_out,
+ * kernel.  Choices are stored on the context per bucket (element type,
+ * floor(log2(element count)), and allow_mfma for C3D20): a later mesh of a
+ * timed bucket takes them untimed.  The workgroup size does not change any
+ * value.  *c3d4_block_out,
  * *c3d20_mfma_out (-1 without C3D20 elements) and *ms_per_launch (the
  * chosen kernels over all elements) may be NULL.  Overwrites the element
  * buffers: call between steps or before assembling. */

@@ -131,3 +131,32 @@ def test_element_autotune_c3d20(ctx):
     dm.assemble(False)
     K1 = dense(*dm.matrix(), n)
     assert np.max(np.abs(K1 - K0)) <= 1e-13 * np.max(np.abs(K0))
+
+
+def test_element_tuning_cache():
+    """A second mesh of a timed bucket (the same C3D4 mesh again, on a fresh
+    context) takes the stored workgroup size; its K is bit for bit the
+    first mesh's."""
+    from ferrix_amd.device import Context
+    import time
+    job = meshgen.cfg4_mesh(meshgen.cfg4_batch_specs(n_meshes=4, seed=9, dof_lo=2e4, dof_hi=4e4)[2])[0]
+    c = Context(0)
+    try:
+        d1 = DeviceMesh(c, job.mesh, job.node_temp)
+        t = time.perf_counter()
+        b1, _, _ = d1.autotune_elements(reps=10)
+        t1 = time.perf_counter() - t
+        d2 = DeviceMesh(c, job.mesh, job.node_temp)
+        t = time.perf_counter()
+        b2, _, _ = d2.autotune_elements(reps=10)
+        t2 = time.perf_counter() - t
+        say("tuned", b1, f"{t1 * 1e3:.1f} ms; cached", b2, f"{t2 * 1e3:.1f} ms")
+        assert b1 == b2
+        d1.assemble(False)
+        d2.assemble(False)
+        for a, b in zip(d1.matrix(), d2.matrix()):
+            assert np.array_equal(a, b)
+        d1.close()
+        d2.close()
+    finally:
+        c.close()
