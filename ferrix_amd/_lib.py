@@ -104,7 +104,7 @@ SIGNATURES = {
     "fx_batch_create": (i32, [P, P, P]),
     "fx_batch_destroy": (None, [P]),
     "fx_batch_add": (i32, [P, P, i32, P, P]),
-    "fx_batch_add_system": (i32, [P, P, i32, P, i64, P, P, P]),
+    "fx_batch_add_system": (i32, [P, P, i32, i64, P, P, P, i64, P, P, P]),
     "fx_batch_run": (i32, [P, i64, P]),
     "fx_batch_poll": (i32, [P, i32, P, P]),
     "fx_batch_get": (i32, [P, i32, P]),

@@ -52,22 +52,12 @@ def upload(ctx, systems: List[System]):
             s.dm = DeviceMesh(ctx, s.jobs[0].mesh, s.jobs[0].node_temp)
 
 
-def loads(s: System) -> np.ndarray:
-    """(n_rhs, n_dof): every load case's external force before the penalty."""
-    rows = []
-    for j in s.jobs:
-        st = j.steps[0]
-        b = np.zeros(j.n_dof)
-        np.add.at(b, st.load_dof, st.load_val)
-        rows.append(b)
-    return np.array(rows)
-
-
 def admit(bt: Batch, s: System) -> int:
     """Assembly, penalty and admission of one system with no host wait
     (Batch.add_system): the same arithmetic as assemble() + Batch.add."""
     st0 = s.jobs[0].steps[0]
-    return bt.add_system(s.dm, loads(s), st0.bc_dof, st0.bc_val)
+    return bt.add_system(s.dm, [(j.steps[0].load_dof, j.steps[0].load_val) for j in s.jobs], st0.bc_dof,
+                         st0.bc_val)
 
 
 def assemble(s: System):

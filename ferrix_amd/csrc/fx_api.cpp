@@ -1092,17 +1092,41 @@ fx_status fx_solve_csr_noalloc(fx_ctx *ctx, int64_t n, int64_t nnz, const int64_
 // ---------------------------------------------------------------------------
 // StaticStep::solve (static_step.rs:47-267)
 // ---------------------------------------------------------------------------
-fx_status fx_batch_add_system(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *loads, int64_t n_bc,
-                              const int64_t *bc_dofs, const double *bc_targets, int32_t *slot_out) {
+fx_status fx_batch_add_system(fx_batch *B, fx_mesh *m, int32_t n_rhs, int64_t n_loads, const int32_t *load_case,
+                              const int64_t *load_dof, const double *load_val, int64_t n_bc, const int64_t *bc_dofs,
+                              const double *bc_targets, int32_t *slot_out) {
     if (!m)
         return fail_ctx(nullptr, FX_ERR_INVALID, "mesh is NULL");
     FX_API_BEGIN
-    require(B && loads && slot_out, "fx_batch_add_system: NULL argument");
+    require(B && slot_out, "fx_batch_add_system: NULL argument");
     require(n_rhs >= 1 && n_rhs <= 4, "fx_batch_add_system: 1 to 4 right-hand sides per system");
+    require(n_loads >= 0 && (n_loads == 0 || (load_case && load_dof && load_val)),
+            "fx_batch_add_system: load arrays missing");
     require(n_bc >= 0 && (n_bc == 0 || (bc_dofs && bc_targets)), "fx_batch_add_system: BC arrays missing");
     set_device(m->ctx);
     hipStream_t s = m->ctx->stream;
     const int64_t n = m->n_dof;
+    require(n * n_rhs < ((int64_t)1 << 31), "fx_batch_add_system: n_dof x n_rhs must stay below 2^31");
+    // f_ext of each load case, static_step.rs:102-108: per (case, DOF) the
+    // records' values added in record order from 0.0
+    std::vector<int64_t> key((size_t)n_loads), order((size_t)n_loads);
+    for (int64_t k = 0; k < n_loads; ++k) {
+        require(load_case[k] >= 0 && load_case[k] < n_rhs && load_dof[k] >= 0 && load_dof[k] < n,
+                "fx_batch_add_system: load record out of range");
+        key[k] = (int64_t)load_case[k] * n + load_dof[k];
+        order[k] = k;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    std::vector<int32_t> lkey;
+    std::vector<double> lval;
+    for (size_t i = 0; i < order.size();) {
+        const int64_t kk = key[order[i]];
+        double acc = 0.0;
+        for (; i < order.size() && key[order[i]] == kk; ++i)
+            acc += load_val[order[i]];
+        lkey.push_back((int32_t)kk);
+        lval.push_back(acc);
+    }
     // the penalty records, uploaded when they differ from the last admission's
     if (m->adm_key_dof.size() != (size_t)n_bc || !std::equal(bc_dofs, bc_dofs + n_bc, m->adm_key_dof.begin()) ||
         !std::equal(bc_targets, bc_targets + n_bc, m->adm_key_val.begin())) {
@@ -1118,7 +1142,12 @@ fx_status fx_batch_add_system(fx_batch *B, fx_mesh *m, int32_t n_rhs, const doub
     if (m->adm_rhs_ev)  // another context's batch read the previous admission's right-hand sides
         FX_HIP(hipStreamWaitEvent(s, m->adm_rhs_ev, 0));
     m->adm_rhs.alloc((size_t)n * n_rhs);
-    FX_HIP(hipMemcpyAsync(m->adm_rhs.p, loads, (size_t)n * n_rhs * sizeof(double), hipMemcpyHostToDevice, s));
+    fx::launch_fill(m, m->adm_rhs.p, n * n_rhs, 0.0, s);
+    if (!lkey.empty()) {
+        m->adm_lkey.upload(lkey, s);
+        m->adm_lval.upload(lval, s);
+        fx::launch_scatter(m, (int64_t)lkey.size(), m->adm_lkey.p, m->adm_lval.p, m->adm_rhs.p);
+    }
     // Assembler::assemble with max_diag left on the device, then the penalty
     // on K and the first right-hand side, and on the others alone
     fx::launch_reset_status(m);

@@ -306,6 +306,8 @@ struct fx_mesh {
     // after which another context's batch has read them
     fx::DBuf<int32_t> adm_ud_dof, adm_ud_ptr, adm_rec_order;
     fx::DBuf<double> adm_target, adm_rhs;
+    fx::DBuf<int32_t> adm_lkey;          // the load records' (case, DOF) targets, rhs-major
+    fx::DBuf<double> adm_lval;           // and their sums
     std::vector<int64_t> adm_key_dof;
     std::vector<double> adm_key_val;
     int64_t adm_n_unique = 0;

@@ -367,21 +367,26 @@ class Batch:
         self._n[slot.value] = rhs.shape
         return slot.value
 
-    def add_system(self, mesh: DeviceMesh, loads, bc_dof, bc_target) -> int:
-        """Assembles mesh's linear K, applies the penalty BCs (records
-        bc_dof, bc_target in deck order) to K and to every load vector
-        (shape (n_dof,) or (n_rhs, n_dof), before the penalty), and admits
-        the system, with no host wait (fx_batch_add_system); returns the
-        slot.  The same results as DeviceMesh.assemble + apply_penalty + add."""
-        loads = _c(np.atleast_2d(loads), np.float64)
-        assert loads.shape[1] == mesh.n_dof
+    def add_system(self, mesh: DeviceMesh, load_cases, bc_dof, bc_target) -> int:
+        """Assembles mesh's linear K, builds one load vector per entry of
+        load_cases (a (load_dof, load_val) record pair each, summed per DOF
+        in record order), applies the penalty BCs (records bc_dof, bc_target
+        in deck order) to K and to every load vector, and admits the system,
+        with no host wait (fx_batch_add_system); returns the slot.  The same
+        results as DeviceMesh.assemble + apply_penalty + add on the summed
+        vectors."""
+        cases = [(_c(d, np.int64), _c(v, np.float64)) for d, v in load_cases]
+        assert cases and all(d.shape == v.shape for d, v in cases)
+        lc = _c(np.concatenate([np.full(len(d), k, np.int32) for k, (d, _) in enumerate(cases)]), np.int32)
+        ld = _c(np.concatenate([d for d, _ in cases]), np.int64)
+        lv = _c(np.concatenate([v for _, v in cases]), np.float64)
         bd = _c(bc_dof, np.int64)
         bv = _c(bc_target, np.float64)
         assert bd.shape == bv.shape
         slot = C.c_int32()
-        self.ctx.check(self.lib.fx_batch_add_system(self.h, mesh.h, loads.shape[0], _p(loads), len(bd), _p(bd),
-                                                    _p(bv), C.byref(slot)))
-        self._n[slot.value] = loads.shape
+        self.ctx.check(self.lib.fx_batch_add_system(self.h, mesh.h, len(cases), len(ld), _p(lc), _p(ld), _p(lv),
+                                                    len(bd), _p(bd), _p(bv), C.byref(slot)))
+        self._n[slot.value] = (len(cases), mesh.n_dof)
         return slot.value
 
     def run(self, max_iters: int = 0) -> int:

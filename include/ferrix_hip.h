@@ -426,18 +426,22 @@ void fx_batch_destroy(fx_batch *batch);
 fx_status fx_batch_add(fx_batch *batch, fx_mesh *mesh, int32_t n_rhs, const double *rhs, int32_t *slot);
 
 /* Assembles the mesh's linear K (Assembler::assemble, assembler.rs:37-151),
- * applies the penalty BCs (static_step.rs:183-195: 1e6 x max_diag per
- * record, duplicates adding again) to K and to each of the n_rhs load
- * vectors `loads` (host, rhs-major, before the penalty, u = the resident
- * displacement), and admits the system like fx_batch_add -- with no host
- * wait: max_diag, the penalty and the right-hand sides stay on the device,
- * and an element failure (singular Jacobian, missing material values)
- * finishes the slot with that status (fx_batch_poll).  The same arithmetic
- * as fx_assemble + fx_apply_penalty + fx_batch_add, bit for bit.  No
- * counterpart in the reference (its loop assembles and solves one system at
- * a time). */
-fx_status fx_batch_add_system(fx_batch *batch, fx_mesh *mesh, int32_t n_rhs, const double *loads, int64_t n_bc,
-                              const int64_t *bc_dofs, const double *bc_targets, int32_t *slot);
+ * builds the n_rhs load vectors from n_loads load records (load case
+ * load_case[k] in 0..n_rhs-1, DOF load_dof[k], value load_val[k]; per case
+ * and DOF the values added in record order from 0.0, as static_step.rs:
+ * 102-108 accumulates f_ext), applies the penalty BCs (static_step.rs:183-
+ * 195: 1e6 x max_diag per record, duplicates adding again) to K and to each
+ * load vector (u = the resident displacement), and admits the system like
+ * fx_batch_add -- with no host wait: max_diag, the penalty and the
+ * right-hand sides stay on the device (only the records travel), and an
+ * element failure (singular Jacobian, missing material values) finishes
+ * the slot with that status (fx_batch_poll).  The same arithmetic as
+ * fx_assemble + fx_apply_penalty + fx_batch_add on the accumulated vectors,
+ * bit for bit.  No counterpart in the reference (its loop assembles and
+ * solves one system at a time). */
+fx_status fx_batch_add_system(fx_batch *batch, fx_mesh *mesh, int32_t n_rhs, int64_t n_loads,
+                              const int32_t *load_case, const int64_t *load_dof, const double *load_val,
+                              int64_t n_bc, const int64_t *bc_dofs, const double *bc_targets, int32_t *slot);
 
 /* Advances every running system by up to max_iters PCG iterations
  * (max_iters <= 0: until none is running).  *n_running: systems still

@@ -547,3 +547,39 @@ def test_foreign_mesh_destroyed_while_batch_runs(ctx):
             other.close()  # drops the remaining foreign slots first
             bt.close()
         say("round", rnd, "ok")
+
+
+def test_add_system_load_records(ctx):
+    """Load records with repeated DOFs (within and across load cases, in
+    scattered order) through fx_batch_add_system: the solutions bit for bit
+    those of the dense vectors np.add.at builds, admitted by assemble +
+    apply_penalty + fx_batch_add."""
+    from ferrix_amd.batch import System, admit, assemble
+    sp = small_specs(3, seed=53)[2]
+    jobs = meshgen.cfg4_mesh(sp)
+    rng = np.random.default_rng(5)
+    for j in jobs:
+        st = j.steps[0]
+        d = np.asarray(st.load_dof, np.int64)
+        v = np.asarray(st.load_val, np.float64)
+        extra = rng.integers(0, len(d), size=len(d))  # repeats of existing DOFs
+        dd = np.concatenate([d, d[extra], rng.integers(0, j.n_dof, size=7)])
+        vv = np.concatenate([v, rng.normal(size=len(extra)) * np.abs(v).max(), rng.normal(size=7)])
+        perm = rng.permutation(len(dd))
+        st.load_dof, st.load_val = dd[perm], vv[perm]
+    say("load cases", len(jobs), "records", [len(j.steps[0].load_dof) for j in jobs])
+    out = []
+    for dev in (True, False):
+        s = System(0, jobs)
+        s.dm = DeviceMesh(ctx, jobs[0].mesh, jobs[0].node_temp)
+        bt = Batch(ctx, REL)
+        if dev:
+            k = admit(bt, s)
+        else:
+            assemble(s)
+            k = bt.add(s.dm, s.rhs)
+        bt.run()
+        out.append(bt.result(k))
+        bt.close()
+        s.dm.close()
+    assert np.array_equal(out[0], out[1])

@@ -74,18 +74,24 @@ def test_serving_nlgeom_own_solver_vs_batch(ctx):
 def test_serving_admits_by_free_memory(ctx):
     """Admission against the device's free memory: a reserve that leaves
     room for about two requests' estimates holds the number admitted at
-    once down, and every result still matches the serial FP64 server."""
+    once down (the estimates are conservative, so a few small requests may
+    share the room), the memory in use at any admission decision stays
+    within the room plus one request, and every result still matches the
+    unlimited server's."""
+    import gc
+    gc.collect()  # earlier tests' device objects are released before the measurement
     reqs = stream()
     free, total, pool = ctx.memory()
     assert 0 < free <= total
-    room = 2 * max(r.est_bytes() for r in reqs)
+    big = max(r.est_bytes() for r in reqs)
+    room = 2 * big
     res = serve_batched(ctx, reqs, L.FX_PREC_FP64, chunk_iters=8, mem_reserve=free + pool - room)
     unlimited = serve_batched(ctx, reqs, L.FX_PREC_FP64, chunk_iters=8)
     say("free", free, "total", total, "admitted at once", res.max_admitted, "unlimited", unlimited.max_admitted,
-        "least free seen", res.min_free)
+        "least free seen", res.min_free, "room", room)
     assert not res.failures
     assert res.max_admitted < unlimited.max_admitted
-    assert res.max_admitted <= 3
+    assert free + pool - res.min_free <= room + big
     for r in reqs:
         assert np.array_equal(res.x[r.rid], unlimited.x[r.rid]), r.rid
 
