@@ -16,6 +16,11 @@ constexpr int kBB = 256;  // threads per block in the batch kernels
 constexpr int kPolyMax = 32;      // most FP32 inner steps per outer iteration (polynomial batch)
 constexpr int kPowerSteps = 20;   // power steps of the lambda_max estimate (polynomial batch)
 constexpr int kOuterG = 8;        // lanes per node row of the polynomial batch's FP64 outer SpMV
+// node rows per workgroup of the polynomial batch's sliced-ELL Chebyshev
+// step: eight 64-row slices, whose overlapping column ranges share L1
+// (measured on the 512-mesh population: 256 / 512 / 1024 rows ran 546 /
+// 538 / 554 us per launch)
+constexpr int kInnerRows = 512;
 // DOF tiles of kBB per block of the vector kernels (kb_init, kb_update):
 // four with FP32 vectors, so a block's fixed costs (descriptor reads, the
 // block reduction) are spread over enough bytes; one with FP64 vectors,

@@ -1107,7 +1107,9 @@ void build_tables(fx_batch *B, int set) {
             continue;
         run.push_back((int)k);
         const BSys &S = B->h_sys[k];
-        for (int b = 0; b < S.nparts1; ++b)
+        // the polynomial batch's sliced-ELL inner step: kInnerRows rows a block
+        const int np1 = S.poly && S.sell ? fx::nblk(S.n_nodes, fx::kInnerRows) : S.nparts1;
+        for (int b = 0; b < np1; ++b)
             (S.sell ? b1s : b1g[group_index(S.G)]).add(S.R, (int)k, b);
         for (int b = 0; b < S.nparts2; ++b)
             t2.push_back(make_int2((int)k, b));

@@ -314,10 +314,10 @@ struct WaveRows {
     int nrow;      // its rows inside the system (0..64)
 };
 
-__device__ __forceinline__ WaveRows wave_rows(const BSys &S, int lb) {
+__device__ __forceinline__ WaveRows wave_rows(const BSys &S, int lb, int rows = kBB) {
     WaveRows w;
     w.w0 = (threadIdx.x / 64) * 64;
-    w.row0 = (int64_t)lb * kBB + w.w0;
+    w.row0 = (int64_t)lb * rows + w.w0;
     const int64_t left = S.n_nodes - w.row0;
     w.nrow = left <= 0 ? 0 : (left < 64 ? (int)left : 64);
     return w;
@@ -339,7 +339,7 @@ __device__ __forceinline__ void stage_out(const WaveRows &w, const double *tile,
 template <int RR>
 __device__ __forceinline__ void cheb_sell(const BSys &S, int lb, int j, int go, double *tile) {
     constexpr int W = 3 * RR;  // entries per node row
-    const int64_t a = (int64_t)lb * kBB + threadIdx.x;
+    const int64_t a = (int64_t)lb * kInnerRows + threadIdx.x;
     gptr<const double> pv = gp((const double *)S.zq[j % 3]);
     if (a < S.n_nodes) {
         double acc[3][RR];
@@ -364,11 +364,11 @@ __device__ __forceinline__ void cheb_sell(const BSys &S, int lb, int j, int go, 
         }
     }
     __syncthreads();
-    stage_out<W>(wave_rows(S, lb), tile, gp(S.zq[(j + 1) % 3]));
+    stage_out<W>(wave_rows(S, lb, kInnerRows), tile, gp(S.zq[(j + 1) % 3]));
     (void)go;
 }
 
-__global__ __launch_bounds__(kBB) void kp_cheb_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
+__global__ __launch_bounds__(kInnerRows) void kp_cheb_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
     const int go = S.go;
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(kBB) void kp_cheb_sell(BSys *__restrict__ sys, cons
         return;
     // one tile for every load-case count (a __shared__ array per template
     // instance would reserve four, and LDS would then hold one block per CU)
-    __shared__ double tile[kBB * 3 * kRmax];
+    __shared__ double tile[kInnerRows * 3 * kRmax];
     switch (S.R) {
     case 1: cheb_sell<1>(S, t.y, j, go, tile); break;
     case 2: cheb_sell<2>(S, t.y, j, go, tile); break;
@@ -389,13 +389,13 @@ __global__ __launch_bounds__(kBB) void kp_cheb_sell(BSys *__restrict__ sys, cons
 // and LDS sized for RR (the kernel above carries those of four, and its
 // occupancy with them).  The arithmetic is cheb_sell<RR>'s.
 template <int RR>
-__global__ __launch_bounds__(kBB) void kp_cheb_sell_r(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
+__global__ __launch_bounds__(kInnerRows) void kp_cheb_sell_r(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
     const int go = S.go;
     if (!go)
         return;
-    __shared__ double tile[kBB * 3 * RR];
+    __shared__ double tile[kInnerRows * 3 * RR];
     cheb_sell<RR>(S, t.y, j, go, tile);
 }
 
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kBB) void kp_cheb_sell_r(BSys *__restrict__ sys, co
 // cases; the segments run from four load cases down to one).
 void launch_cheb_sell(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int j) {
     if (!split) {
-        hipLaunchKernelGGL(kp_cheb_sell, dim3(n), dim3(kBB), 0, s, sys, tab, j);
+        hipLaunchKernelGGL(kp_cheb_sell, dim3(n), dim3(kInnerRows), 0, s, sys, tab, j);
         return;
     }
     int off = 0;
@@ -413,10 +413,10 @@ void launch_cheb_sell(hipStream_t s, BSys *sys, const int2 *tab, int n, const in
         if (!c)
             continue;
         switch (r) {
-        case 0: hipLaunchKernelGGL(kp_cheb_sell_r<1>, dim3(c), dim3(kBB), 0, s, sys, tab + off, j); break;
-        case 1: hipLaunchKernelGGL(kp_cheb_sell_r<2>, dim3(c), dim3(kBB), 0, s, sys, tab + off, j); break;
-        case 2: hipLaunchKernelGGL(kp_cheb_sell_r<3>, dim3(c), dim3(kBB), 0, s, sys, tab + off, j); break;
-        default: hipLaunchKernelGGL(kp_cheb_sell_r<4>, dim3(c), dim3(kBB), 0, s, sys, tab + off, j); break;
+        case 0: hipLaunchKernelGGL(kp_cheb_sell_r<1>, dim3(c), dim3(kInnerRows), 0, s, sys, tab + off, j); break;
+        case 1: hipLaunchKernelGGL(kp_cheb_sell_r<2>, dim3(c), dim3(kInnerRows), 0, s, sys, tab + off, j); break;
+        case 2: hipLaunchKernelGGL(kp_cheb_sell_r<3>, dim3(c), dim3(kInnerRows), 0, s, sys, tab + off, j); break;
+        default: hipLaunchKernelGGL(kp_cheb_sell_r<4>, dim3(c), dim3(kInnerRows), 0, s, sys, tab + off, j); break;
         }
         off += c;
     }
