@@ -17,10 +17,14 @@ constexpr int kPolyMax = 32;      // most FP32 inner steps per outer iteration (
 constexpr int kPowerSteps = 20;   // power steps of the lambda_max estimate (polynomial batch)
 constexpr int kOuterG = 8;        // lanes per node row of the polynomial batch's FP64 outer SpMV
 // node rows per workgroup of the polynomial batch's sliced-ELL Chebyshev
-// step: eight 64-row slices, whose overlapping column ranges share L1
-// (measured on the 512-mesh population: 256 / 512 / 1024 rows ran 546 /
-// 538 / 554 us per launch)
-constexpr int kInnerRows = 512;
+// step: eight 64-row slices (their overlapping column ranges share L1) when
+// the running systems hold at least kInnerRowsLargeMin node rows, else kBB
+// (four slices: twice the workgroups for a small batch).  Measured on the
+// 512-mesh population (7.7 M node rows): 256 / 512 / 1024 rows ran 546 /
+// 538 / 554 us per launch; on the serving stream (up to about 0.6 M node
+// rows running) 256 rows kept the lower latencies.
+constexpr int kInnerRowsLarge = 512;
+constexpr int64_t kInnerRowsLargeMin = (int64_t)2 << 20;
 // DOF tiles of kBB per block of the vector kernels (kb_init, kb_update):
 // four with FP32 vectors, so a block's fixed costs (descriptor reads, the
 // block reduction) are spread over enough bytes; one with FP64 vectors,
@@ -104,10 +108,11 @@ void launch_poly_power_start(hipStream_t s, BSys *sys, const int2 *tab_vec, int 
                              int n_in_sell, const int2 *const *tab_in_g, const int *n_in_g, const int32_t *list,
                              int n_list, BFin *fin);
 void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, int n_vec, const int2 *tab_in_sell,
-                       int n_in_sell, const int *n_in_sell_r, bool split, const int2 *const *tab_in_g,
+                       int n_in_sell, const int *n_in_sell_r, bool split, int inner_rows, const int2 *const *tab_in_g,
                        const int *n_in_g, const int2 *tab_out_sell, int n_out_sell, const int2 *tab_out, int n_out,
                        const int32_t *list, int n_list, BFin *fin, hipEvent_t ev0, hipEvent_t ev1);
-double time_poly_inner(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int reps);
+double time_poly_inner(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int rows,
+                       int reps);
 
 inline int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
 

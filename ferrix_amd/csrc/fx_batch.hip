@@ -840,6 +840,7 @@ struct fx_batch {
         int n1g[kGroups] = {0, 0, 0, 0, 0, 0};
         int n1s = 0, n2 = 0, n3s = 0, n3 = 0, nrun = 0;
         int n1s_r[fx::kRmax] = {0, 0, 0, 0};  // polynomial batch: tab1s segments by load-case count
+        int inner_rows = fx::kBB;               // polynomial batch: node rows per tab1s entry
         int64_t version = -1;          // of the running-slot set these tables were built for
         int2 *h_tab = nullptr;         // pinned staging of the tables
         size_t h_tab_cap = 0;
@@ -1101,14 +1102,24 @@ void build_tables(fx_batch *B, int set) {
     ByCount b1g[fx_batch::kGroups], b1s, b3s, b3;
     std::vector<int2> t2;
     std::vector<int> run;
+    // the polynomial batch's inner step: kInnerRowsLarge rows per workgroup
+    // when the running sliced-ELL systems hold enough rows to fill the GPU
+    // many times over, kBB otherwise
+    int64_t poly_rows = 0;
+    for (size_t k = 0; k < B->slots.size(); ++k) {
+        auto &sl = *B->slots[k];
+        if (sl.used && sl.phase == 0 && !sl.pending && B->h_sys[k].poly && B->h_sys[k].sell)
+            poly_rows += B->h_sys[k].n_nodes;
+    }
+    C.inner_rows = poly_rows >= fx::kInnerRowsLargeMin ? fx::kInnerRowsLarge : kBB;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         auto &sl = *B->slots[k];
         if (!sl.used || sl.phase != 0 || sl.pending)
             continue;
         run.push_back((int)k);
         const BSys &S = B->h_sys[k];
-        // the polynomial batch's sliced-ELL inner step: kInnerRows rows a block
-        const int np1 = S.poly && S.sell ? fx::nblk(S.n_nodes, fx::kInnerRows) : S.nparts1;
+        // the polynomial batch's sliced-ELL inner step: C.inner_rows rows a block
+        const int np1 = S.poly && S.sell ? fx::nblk(S.n_nodes, C.inner_rows) : S.nparts1;
         for (int b = 0; b < np1; ++b)
             (S.sell ? b1s : b1g[group_index(S.G)]).add(S.R, (int)k, b);
         for (int b = 0; b < S.nparts2; ++b)
@@ -1304,8 +1315,8 @@ void launch_iterations(fx_batch *B, int n, int set) {
             tg[g] = C.tab1g[g].p;
         for (int k = 0; k < n; ++k)
             fx::launch_poly_outer(s, B->d_sys.p, B->poly_deg, C.tab2.p, C.n2, C.tab1s.p, C.n1s, C.n1s_r,
-                                  B->inner_split, tg, C.n1g, C.tab3s.p, C.n3s, C.tab3.p, C.n3, C.run.p, C.nrun,
-                                  B->fin,
+                                  B->inner_split, C.inner_rows, tg, C.n1g, C.tab3s.p, C.n3s, C.tab3.p, C.n3,
+                                  C.run.p, C.nrun, B->fin,
                                   B->timing ? C.ev[2 * k] : nullptr, B->timing ? C.ev[2 * k + 1] : nullptr);
         C.ev_used = B->timing ? 2 * n : 0;
         B->spmv_launches += (int64_t)n * B->poly_deg;
@@ -1810,8 +1821,8 @@ fx_status fx_batch_autotune(fx_batch *B, int32_t reps, int32_t *shape_out, doubl
             build_tables(B, 0);
         double t1 = 0.0, t2 = 0.0;
         if (C.n1s > 0) {
-            t1 = fx::time_poly_inner(s, B->d_sys.p, C.tab1s.p, C.n1s, C.n1s_r, false, reps);
-            t2 = fx::time_poly_inner(s, B->d_sys.p, C.tab1s.p, C.n1s, C.n1s_r, true, reps);
+            t1 = fx::time_poly_inner(s, B->d_sys.p, C.tab1s.p, C.n1s, C.n1s_r, false, C.inner_rows, reps);
+            t2 = fx::time_poly_inner(s, B->d_sys.p, C.tab1s.p, C.n1s, C.n1s_r, true, C.inner_rows, reps);
             B->inner_split = t2 < t1;
         }
         if (shape_out)

@@ -336,10 +336,10 @@ __device__ __forceinline__ void stage_out(const WaveRows &w, const double *tile,
 // was slower: those reads hit lines the wavefront's other loads bring in).
 // Entries of load cases that do not run are written as well (their values
 // are never read).
-template <int RR>
+template <int RR, int ROWS>
 __device__ __forceinline__ void cheb_sell(const BSys &S, int lb, int j, int go, double *tile) {
     constexpr int W = 3 * RR;  // entries per node row
-    const int64_t a = (int64_t)lb * kInnerRows + threadIdx.x;
+    const int64_t a = (int64_t)lb * ROWS + threadIdx.x;
     gptr<const double> pv = gp((const double *)S.zq[j % 3]);
     if (a < S.n_nodes) {
         double acc[3][RR];
@@ -364,11 +364,14 @@ __device__ __forceinline__ void cheb_sell(const BSys &S, int lb, int j, int go, 
         }
     }
     __syncthreads();
-    stage_out<W>(wave_rows(S, lb, kInnerRows), tile, gp(S.zq[(j + 1) % 3]));
+    stage_out<W>(wave_rows(S, lb, ROWS), tile, gp(S.zq[(j + 1) % 3]));
     (void)go;
 }
 
-__global__ __launch_bounds__(kInnerRows) void kp_cheb_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
+// ROWS node rows (ROWS / 64 slices) per workgroup; rows past the system's
+// end only take part in the barrier
+template <int ROWS>
+__global__ __launch_bounds__(ROWS) void kp_cheb_sell(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
     const int go = S.go;
@@ -376,35 +379,33 @@ __global__ __launch_bounds__(kInnerRows) void kp_cheb_sell(BSys *__restrict__ sy
         return;
     // one tile for every load-case count (a __shared__ array per template
     // instance would reserve four, and LDS would then hold one block per CU)
-    __shared__ double tile[kInnerRows * 3 * kRmax];
+    __shared__ double tile[ROWS * 3 * kRmax];
     switch (S.R) {
-    case 1: cheb_sell<1>(S, t.y, j, go, tile); break;
-    case 2: cheb_sell<2>(S, t.y, j, go, tile); break;
-    case 3: cheb_sell<3>(S, t.y, j, go, tile); break;
-    default: cheb_sell<4>(S, t.y, j, go, tile); break;
+    case 1: cheb_sell<1, ROWS>(S, t.y, j, go, tile); break;
+    case 2: cheb_sell<2, ROWS>(S, t.y, j, go, tile); break;
+    case 3: cheb_sell<3, ROWS>(S, t.y, j, go, tile); break;
+    default: cheb_sell<4, ROWS>(S, t.y, j, go, tile); break;
     }
 }
 
 // The same inner step for the systems with RR load cases only: registers
 // and LDS sized for RR (the kernel above carries those of four, and its
 // occupancy with them).  The arithmetic is cheb_sell<RR>'s.
-template <int RR>
-__global__ __launch_bounds__(kInnerRows) void kp_cheb_sell_r(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
+template <int RR, int ROWS>
+__global__ __launch_bounds__(ROWS) void kp_cheb_sell_r(BSys *__restrict__ sys, const int2 *__restrict__ tab, int j) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
     const int go = S.go;
     if (!go)
         return;
-    __shared__ double tile[kInnerRows * 3 * RR];
-    cheb_sell<RR>(S, t.y, j, go, tile);
+    __shared__ double tile[ROWS * 3 * RR];
+    cheb_sell<RR, ROWS>(S, t.y, j, go, tile);
 }
 
-// Inner step j over the sliced-ELL table: one launch (split = false) or one
-// per load-case count over its segment (n_r[r]: entries with r + 1 load
-// cases; the segments run from four load cases down to one).
-void launch_cheb_sell(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int j) {
+template <int ROWS>
+void launch_cheb_rows(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int j) {
     if (!split) {
-        hipLaunchKernelGGL(kp_cheb_sell, dim3(n), dim3(kInnerRows), 0, s, sys, tab, j);
+        hipLaunchKernelGGL(kp_cheb_sell<ROWS>, dim3(n), dim3(ROWS), 0, s, sys, tab, j);
         return;
     }
     int off = 0;
@@ -413,13 +414,25 @@ void launch_cheb_sell(hipStream_t s, BSys *sys, const int2 *tab, int n, const in
         if (!c)
             continue;
         switch (r) {
-        case 0: hipLaunchKernelGGL(kp_cheb_sell_r<1>, dim3(c), dim3(kInnerRows), 0, s, sys, tab + off, j); break;
-        case 1: hipLaunchKernelGGL(kp_cheb_sell_r<2>, dim3(c), dim3(kInnerRows), 0, s, sys, tab + off, j); break;
-        case 2: hipLaunchKernelGGL(kp_cheb_sell_r<3>, dim3(c), dim3(kInnerRows), 0, s, sys, tab + off, j); break;
-        default: hipLaunchKernelGGL(kp_cheb_sell_r<4>, dim3(c), dim3(kInnerRows), 0, s, sys, tab + off, j); break;
+        case 0: hipLaunchKernelGGL((kp_cheb_sell_r<1, ROWS>), dim3(c), dim3(ROWS), 0, s, sys, tab + off, j); break;
+        case 1: hipLaunchKernelGGL((kp_cheb_sell_r<2, ROWS>), dim3(c), dim3(ROWS), 0, s, sys, tab + off, j); break;
+        case 2: hipLaunchKernelGGL((kp_cheb_sell_r<3, ROWS>), dim3(c), dim3(ROWS), 0, s, sys, tab + off, j); break;
+        default: hipLaunchKernelGGL((kp_cheb_sell_r<4, ROWS>), dim3(c), dim3(ROWS), 0, s, sys, tab + off, j); break;
         }
         off += c;
     }
+}
+
+// Inner step j over the sliced-ELL table of `rows`-row groups (kBB or
+// kInnerRowsLarge): one launch (split = false) or one per load-case count
+// over its segment (n_r[r]: entries with r + 1 load cases; the segments run
+// from four load cases down to one).
+void launch_cheb_sell(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int rows,
+                      int j) {
+    if (rows == kInnerRowsLarge)
+        launch_cheb_rows<kInnerRowsLarge>(s, sys, tab, n, n_r, split, j);
+    else
+        launch_cheb_rows<kBB>(s, sys, tab, n, n_r, split, j);
 }
 
 template <int G, int RR>
@@ -768,7 +781,7 @@ void launch_poly_power_start(hipStream_t s, BSys *sys, const int2 *tab_vec, int 
 }
 
 void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, int n_vec, const int2 *tab_in_sell,
-                       int n_in_sell, const int *n_in_sell_r, bool split, const int2 *const *tab_in_g,
+                       int n_in_sell, const int *n_in_sell_r, bool split, int inner_rows, const int2 *const *tab_in_g,
                        const int *n_in_g, const int2 *tab_out_sell, int n_out_sell, const int2 *tab_out, int n_out,
                        const int32_t *list, int n_list, BFin *fin, hipEvent_t ev0, hipEvent_t ev1) {
     if (ev0)
@@ -776,7 +789,7 @@ void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, i
     for (int j = 1; j <= deg; ++j)
         launch_inner_tables(tab_in_sell, n_in_sell, tab_in_g, n_in_g, [&](int G, const int2 *tab, int n) {
             switch (G) {
-            case 0: launch_cheb_sell(s, sys, tab, n, n_in_sell_r, split, j); break;
+            case 0: launch_cheb_sell(s, sys, tab, n, n_in_sell_r, split, inner_rows, j); break;
             case 2: hipLaunchKernelGGL(kp_cheb_lanes<2>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
             case 4: hipLaunchKernelGGL(kp_cheb_lanes<4>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
             case 8: hipLaunchKernelGGL(kp_cheb_lanes<8>, dim3(n), dim3(kBB), 0, s, sys, tab, j); break;
@@ -802,16 +815,17 @@ void launch_poly_outer(hipStream_t s, BSys *sys, int deg, const int2 *tab_vec, i
 // the next start z_1 and rs and writes only z_2, which the first inner step
 // of the next outer iteration rewrites from the same inputs before anything
 // reads it.
-double time_poly_inner(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int reps) {
+double time_poly_inner(hipStream_t s, BSys *sys, const int2 *tab, int n, const int *n_r, bool split, int rows,
+                       int reps) {
     hipEvent_t e0, e1;
     FX_HIP(hipEventCreate(&e0));
     FX_HIP(hipEventCreate(&e1));
     float ms = 0.f;
     try {
-        launch_cheb_sell(s, sys, tab, n, n_r, split, 1);
+        launch_cheb_sell(s, sys, tab, n, n_r, split, rows, 1);
         FX_HIP(hipEventRecord(e0, s));
         for (int k = 0; k < reps; ++k)
-            launch_cheb_sell(s, sys, tab, n, n_r, split, 1);
+            launch_cheb_sell(s, sys, tab, n, n_r, split, rows, 1);
         FX_HIP(hipEventRecord(e1, s));
         FX_HIP(hipGetLastError());
         FX_HIP(hipEventSynchronize(e1));
