@@ -225,3 +225,28 @@ def test_poly_autotune_needs_poly_batch(ctx):
     with pytest.raises(InvalidModelState, match="shape must be 0 or 1"):
         bt.set_inner_shape(2)
     bt.close()
+
+
+def test_poly_rows_per_workgroup_bitwise(ctx):
+    """A batch whose running systems hold more than 2 M node rows runs the
+    Chebyshev step with 512 rows per workgroup; a system alone runs it with
+    256.  The system's solutions are the same bits either way."""
+    specs = meshgen.cfg4_batch_specs(n_meshes=40, seed=61, dof_lo=1.5e5, dof_hi=2.0e5)
+    sys = population(ctx, specs)
+    rows = sum(dm.n_dof // 3 for dm, _ in sys)
+    say("node rows in the batch", rows)
+    assert rows > 2 << 20
+    bt = Batch(ctx, POLY)
+    slots = [bt.add(dm, b) for dm, b in sys]
+    assert bt.run() == 0
+    mixed = [(bt.result(s), bt.poll(s)[1].iterations) for s in slots[:2]]
+    bt.close()
+    for (dm, b), (x_mix, it_mix) in zip(sys[:2], mixed):
+        alone = Batch(ctx, POLY)
+        k = alone.add(dm, b)
+        alone.run()
+        assert np.array_equal(alone.result(k), x_mix)
+        assert alone.poll(k)[1].iterations == it_mix
+        alone.close()
+    for dm, _ in sys:
+        dm.close()
