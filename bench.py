@@ -144,7 +144,7 @@ def run_single(a):
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     arms = {}
     for arm in a.arms.split(","):
-        prec = L.FX_PREC_MIXED if arm == "mixed" else L.FX_PREC_FP64
+        prec = {"mixed": L.FX_PREC_MIXED, "poly": L.FX_PREC_MIXED_POLY}.get(arm, L.FX_PREC_FP64)
         so = SolveOptions(precision=prec, stop_mode=L.FX_STOP_REL_R2, tol=st.tol, max_iter=st.max_iter,
                           inner_tol=a.inner_tol, max_refine=100)
         sd, keep = step_def(st, 1, so, so)
@@ -152,7 +152,9 @@ def run_single(a):
         # the SpMV autotuner on this arm's operator, before the timed steps
         # (a step reassembles K on the same pattern; the choice stays)
         dm.assemble(False)
-        group, _ = dm.autotune_spmv(prec) if a.spmv_group == 0 else (a.spmv_group, 0.0)
+        # (the polynomial solve's lane groups follow the FP64 operator's)
+        tune_prec = L.FX_PREC_FP64 if prec == L.FX_PREC_MIXED_POLY else prec
+        group, _ = dm.autotune_spmv(tune_prec) if a.spmv_group == 0 else (a.spmv_group, 0.0)
 
         def one():
             dm.zero_u()
@@ -190,10 +192,12 @@ def run_single(a):
         arms[arm] = arm_res
         print(f"# {arm}: {json.dumps(arm_res)}", flush=True)
 
-    head = "mixed" if "mixed" in arms else next(iter(arms))
+    head = next((h for h in ("poly", "mixed") if h in arms), next(iter(arms)))
     # roofline of the dominant kernel on the headline arm's operator
     dm.assemble(False)
-    dm.apply_penalty(st.bc_dof, st.bc_val, np.zeros(job.n_dof))
+    b = np.zeros(job.n_dof)
+    np.add.at(b, st.load_dof, st.load_val)
+    b, _ = dm.apply_penalty(st.bc_dof, st.bc_val, b)
     prec = L.FX_PREC_MIXED if head == "mixed" else L.FX_PREC_FP64
     if a.sweep_groups:
         for p_ in (L.FX_PREC_FP64, L.FX_PREC_MIXED):
@@ -209,6 +213,25 @@ def run_single(a):
             "bytes_per_launch": by, "us_per_launch": round(ms * 1e3, 2),
             "fp64_kernel": {"bytes_per_launch": by64, "us_per_launch": round(ms64 * 1e3, 2),
                             "achieved": round(by64 / (ms64 * 1e-3) / 1e9, 1)}}
+    if head == "poly":
+        # the polynomial solve's inner step (kp_cheb_lanes on the FP16
+        # node-block operator): the same system once more as the one slot of
+        # an instrumented batch, HIP events around every outer iteration's
+        # inner launches, the batch's algorithmic byte count
+        from ferrix_amd.device import Batch
+        bt = Batch(ctx, SolveOptions(precision=L.FX_PREC_MIXED_POLY, stop_mode=L.FX_STOP_REL_R2, tol=st.tol,
+                                     max_iter=st.max_iter))
+        bt.time_spmv(True)
+        bt.add(dm, b[None, :])
+        bt.run()
+        launches, nt, ms_t, by_t = bt.spmv_stats()
+        bt.close()
+        us = 1e3 * ms_t / max(nt, 1)
+        ach = (by_t / max(launches, 1)) / (us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "kp_cheb_lanes<G>",
+                "bytes_per_launch": by_t / max(launches, 1), "us_per_launch": round(us, 2), "launches": launches,
+                "fp64_kernel": roof["fp64_kernel"]}
     print(f"# roofline {json.dumps(roof)}", flush=True)
 
     cpu = None
@@ -229,7 +252,7 @@ def run_single(a):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-stored K / fp64" if head == "mixed" else "f64",
+        "dtype": DTYPE.get({"poly": "mixed_poly", "mixed": "mixed_batched"}.get(head, ""), "f64"),
         "data": "synthetic",
         "config": {"workload": job.name + "_linear_static_jacobi_pcg_rel1e-10", "n_dof": job.n_dof,
                    "n_elements": job.mesh.n_el, "element": "C3D20", "nnz": int(dm.nnz)},
@@ -238,6 +261,8 @@ def run_single(a):
         "arms": arms,
         "mixed_over_fp64": (round(arms["mixed"]["dof_solves_per_s"] / arms["fp64"]["dof_solves_per_s"], 3)
                             if "mixed" in arms and "fp64" in arms else None),
+        "poly_over_fp64": (round(arms["poly"]["dof_solves_per_s"] / arms["fp64"]["dof_solves_per_s"], 3)
+                           if "poly" in arms and "fp64" in arms else None),
         "roofline": roof,
         "cpu_baseline": {k: v for k, v in cpu.items() if k != "seconds_per_step"} if cpu else None,
     }
@@ -742,7 +767,7 @@ def main():
     elif a.workload == "single":
         a.steps = 10 if a.steps is None else a.steps
         a.warmup = 2 if a.warmup is None else a.warmup
-        a.arms = a.arms or "fp64,mixed"
+        a.arms = a.arms or "fp64,mixed,poly"
         run_single(a)
     else:
         a.steps = 1 if a.steps is None else a.steps

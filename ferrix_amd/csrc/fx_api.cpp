@@ -617,6 +617,7 @@ void fx_mesh_destroy(fx_mesh *m) {
     ctx->meshes.erase(it);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    fx::drop_poly_batch(m);
     for (fx_batch *b : ctx->batches)
         fx::batch_drop_mesh(b, m);
     fx::batch_drop_mesh_foreign(m);
@@ -856,9 +857,10 @@ struct DevBc {
 };
 
 void check_solve_opts(const fx_solve_opts &o) {
-    require(o.precision == FX_PREC_FP64 || o.precision == FX_PREC_MIXED,
-            o.precision == FX_PREC_MIXED_POLY ? "solve: FX_PREC_MIXED_POLY is a batch precision (fx_batch_create)"
-                                              : "solve: unknown precision");
+    require(o.precision == FX_PREC_FP64 || o.precision == FX_PREC_MIXED || o.precision == FX_PREC_MIXED_POLY,
+            "solve: unknown precision");
+    require(o.precision != FX_PREC_MIXED_POLY || (o.poly_degree >= 0 && o.poly_degree <= fx::kPolyMax),
+            "solve: poly_degree must be 0 (default 15) or 1.." + std::to_string(fx::kPolyMax));
     require(o.stop_mode == FX_STOP_ABS_SQRT_RZ || o.stop_mode == FX_STOP_REL_R2, "solve: unknown stop mode");
     require(o.max_iter >= 0, "solve: max_iter < 0");
     require(o.spmv_group == 0 || o.spmv_group == 2 || o.spmv_group == 4 || o.spmv_group == 8 ||
@@ -1227,8 +1229,9 @@ fx_status fx_static_step(fx_mesh *m, const fx_step_def *sd, fx_timer *timer, fx_
     // The Newton iteration on the device (fx_newton.hip) unless FX_NR_DEVICE=0;
     // its front as a replayed graph unless FX_NR_GRAPH=0 (plain launches)
     const bool dev_nr = env_on("FX_NR_DEVICE"), use_graph = env_on("FX_NR_GRAPH");
-    const bool mixed = opts.precision == FX_PREC_MIXED;
-    const bool host_solve = mixed || sd->solve != nullptr;  // the solve outside the device front
+    // the solve outside the device front: the mixed solves (host-driven
+    // refinement, or the polynomial batch) and a caller's solver
+    const bool host_solve = opts.precision != FX_PREC_FP64 || sd->solve != nullptr;
     fx::NewtonGraph ng(m);
     Events3 ev;
     bool nr_graph_ready = false;

@@ -13,7 +13,6 @@ namespace fx {
 
 constexpr int kRmax = 4;
 constexpr int kBB = 256;  // threads per block in the batch kernels
-constexpr int kPolyMax = 32;      // most FP32 inner steps per outer iteration (polynomial batch)
 constexpr int kPowerSteps = 20;   // power steps of the lambda_max estimate (polynomial batch)
 constexpr int kOuterG = 8;        // lanes per node row of the polynomial batch's FP64 outer SpMV
 // node rows per workgroup of the polynomial batch's sliced-ELL Chebyshev

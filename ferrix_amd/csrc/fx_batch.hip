@@ -869,6 +869,8 @@ struct fx_batch {
     // orders the batch's stream after work queued on the stream of an
     // admitted mesh's own context (re-recorded at each such admission)
     hipEvent_t adm_ev = nullptr;
+    // the mesh whose single solves this batch serves (fx::poly_solve), or null
+    fx_mesh *owner = nullptr;
     // serialises the batch's calls with a foreign mesh's removal from
     // another host thread (batch_drop_mesh_foreign)
     std::mutex mx;
@@ -889,6 +891,8 @@ fx_batch::fx_batch() {
 }
 
 fx_batch::~fx_batch() {
+    if (owner)
+        owner->poly_batch = nullptr;
     {
         // the slots go first, under the registry's lock: a foreign mesh being
         // destroyed meanwhile either drops its slot here before, or finds
@@ -1681,6 +1685,65 @@ fx_status batch_add_impl(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *r
         return FX_ERR_OOM;
     }
     return FX_OK;
+}
+
+// fx_solve and fx_static_step in FX_PREC_MIXED_POLY: the system as the one
+// slot of a polynomial batch kept with the mesh (created on the mesh's
+// context on first use, and again when the options change), b_dev in and
+// x_dev out on the mesh's stream; the report is fx_batch_poll's (iterations:
+// inner steps; outer_spmv: outer iterations).  The arithmetic is that of
+// the same system alone in any polynomial batch with these options.
+fx_status poly_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep) {
+    fx_batch *B = m->poly_batch;
+    if (B) {
+        const fx_solve_opts &k = B->opts;
+        if (k.stop_mode != o.stop_mode || k.tol != o.tol || k.max_iter != o.max_iter ||
+            k.poly_degree != o.poly_degree || k.spmv_group != o.spmv_group) {
+            fx_batch_destroy(B);  // its destructor clears m->poly_batch
+            B = nullptr;
+        }
+    }
+    if (!B) {
+        const fx_status st = fx_batch_create(m->ctx, &o, &B);
+        if (st != FX_OK)
+            return st;  // the message is set
+        B->owner = m;
+        m->poly_batch = B;
+    }
+    int32_t slot = -1;
+    fx_status st = batch_add_impl(B, m, 1, b_dev, true, nullptr, &slot);
+    if (st != FX_OK)
+        return st;
+    int32_t running = 0;
+    st = fx_batch_run(B, 0, &running);
+    if (st == FX_OK) {
+        int32_t state = 0;
+        fx_solve_report r{};
+        st = fx_batch_poll(B, slot, &state, &r);
+        if (st == FX_OK) {
+            if (rep)
+                *rep = r;
+            const fx_batch::Slot &sl = *B->slots[slot];
+            FX_HIP(hipMemcpyAsync(x_dev, sl.x.p, (size_t)m->n_dof * sizeof(double), hipMemcpyDeviceToDevice,
+                                  B->ctx->stream));
+            st = state == 1 ? FX_OK : (fx_status)(-state);
+        }
+    }
+    (void)fx_batch_remove(B, slot);  // waits for the stream: the copy has completed
+    return st;
+}
+
+void drop_poly_batch(fx_mesh *m) {
+    fx_batch *B = m->poly_batch;
+    if (!B)
+        return;
+    m->poly_batch = nullptr;
+    B->owner = nullptr;
+    auto &v = m->ctx->batches;
+    auto it = std::find(v.begin(), v.end(), B);
+    if (it != v.end())
+        v.erase(it);
+    delete B;
 }
 
 }  // namespace fx

@@ -22,6 +22,7 @@ struct fx_mesh;
 
 namespace fx {
 constexpr int kSliceRows = 64;  // node rows per slice of the sliced-ELL operator (one wavefront)
+constexpr int kPolyMax = 32;    // most Chebyshev inner steps per outer iteration (polynomial batch)
 // Host waits on the device (stream synchronisations) made by this thread;
 // fx_static_step reports the count of a step
 extern thread_local int64_t t_host_syncs;
@@ -318,6 +319,10 @@ struct fx_mesh {
     int c20_mfma = -1;                   // C3D20 element K on MFMA: 1 / 0, -1: FX_C3D20_MFMA decides
     int64_t device_bytes = 0;
     std::vector<fx::PcgGraph> graphs;    // captured iteration batches
+    // fx_solve / fx_static_step in FX_PREC_MIXED_POLY: a polynomial batch on
+    // the mesh's context with this mesh as its only system (fx::poly_solve),
+    // destroyed with the mesh
+    fx_batch *poly_batch = nullptr;
     ~fx_mesh() {
         for (auto &g : graphs)
             (void)hipGraphExecDestroy(g.exec);
@@ -346,6 +351,10 @@ void reset_status(fx_mesh *m);
 double device_max_diag(fx_mesh *m);
 // PCG (fx_pcg.hip)
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep);
+// pcg_solve in FX_PREC_MIXED_POLY (fx_batch.hip)
+fx_status poly_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep);
+// unlinks and frees the mesh's polynomial batch (the caller holds the context's registry lock)
+void drop_poly_batch(fx_mesh *m);
 // the FP64 solve in pieces (device-resident Newton iteration, fx_newton.hip):
 // Jacobi + start scalars written on the device (*skip != 0: stopped before
 // the first iteration) + x = 0, r = b; `iters` iterations queued with no

@@ -548,6 +548,8 @@ int pcg_continue_fp64(fx_mesh *m, double tol, int64_t max_iter, int G, PcgScalar
 int64_t pcg_iterations_done(const PcgScalars &h) { return iterations_done(h); }
 
 fx_status pcg_solve(fx_mesh *m, const double *b_dev, double *x_dev, const fx_solve_opts &o, fx_solve_report *rep) {
+    if (o.precision == FX_PREC_MIXED_POLY)
+        return poly_solve(m, b_dev, x_dev, o, rep);
     hipStream_t s = m->ctx->stream;
     auto t0 = std::chrono::steady_clock::now();
     const int64_t n = m->n_dof;

@@ -163,14 +163,18 @@ typedef enum fx_precision {
                           by inner_tol, keeping the search direction if the
                           two agree within 10%.  fx_solve keeps FP64 Krylov
                           vectors, a batch FP32 ones (dots in FP64) */
-    FX_PREC_MIXED_POLY = 2 /* batches only: flexible CG in FP64 (x, r, the
-                          directions, K64 z, every dot product, the stop
-                          rule on the FP64 residual) preconditioned by
-                          poly_degree Chebyshev steps on the Jacobi-scaled K
-                          stored in FP16, with FP64 iterates; the Chebyshev
-                          interval comes from a power iteration per system
-                          when it joins.  iterations in the report count the
-                          inner steps, outer_spmv the FP64 outer iterations */
+    FX_PREC_MIXED_POLY = 2 /* flexible CG in FP64 (x, r, the directions,
+                          K64 z, every dot product, the stop rule on the
+                          FP64 residual) preconditioned by poly_degree
+                          Chebyshev steps on the Jacobi-scaled K stored in
+                          FP16, with FP64 iterates; the Chebyshev interval
+                          comes from a power iteration per system when it
+                          joins.  A batch precision; fx_solve and
+                          fx_static_step run the system as the one slot of a
+                          batch kept with the mesh (same bits as alone in a
+                          batch).  iterations in the report count the inner
+                          steps (max_iter bounds them), outer_spmv the FP64
+                          outer iterations */
 } fx_precision;
 
 typedef enum fx_stop {

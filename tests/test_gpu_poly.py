@@ -179,13 +179,48 @@ def test_poly_not_converged(ctx):
             bt.result(s)
 
 
-def test_poly_single_solve_is_rejected(ctx):
-    job = meshgen.cfg1_c3d4_cantilever()
-    dm = DeviceMesh(ctx, job.mesh, job.node_temp)
-    dm.assemble(False)
-    from ferrix_amd.inp import InvalidModelState
-    with pytest.raises(InvalidModelState, match="batch precision"):
-        dm.solve(np.ones(job.n_dof), POLY)
+@pytest.mark.parametrize("kind", ["c3d4", "c3d20"])
+def test_poly_single_solve(ctx, kind):
+    """fx_solve in FX_PREC_MIXED_POLY: the system as the one slot of a batch
+    kept with the mesh.  Bit for bit the same system alone in a batch, within
+    1e-8 of the oracle's direct solve; a second solve reuses the batch, other
+    options make a new one, and destroying the mesh frees it."""
+    if kind == "c3d4":
+        jobs = meshgen.cfg4_mesh(small_specs(3, seed=71)[1])
+    else:
+        jobs = [meshgen.cfg2_c3d20_block(seed=5, nx=10, ny=4, nz=3, jitter=True)]
+    dm, b = system(ctx, jobs[:1])
+    x, rep = dm.solve(b[0], POLY)
+    say(kind, "inner steps", rep.iterations, "outer", rep.outer_spmv)
+    assert rep.iterations > 0 and rep.final_criterion <= 1e-10
+    bt = Batch(ctx, POLY)
+    k = bt.add(dm, b)
+    bt.run()
+    assert np.array_equal(bt.result(k)[0], x)
+    bt.close()
+    x2, _ = dm.solve(b[0], POLY)
+    assert np.array_equal(x2, x)
+    looser = SolveOptions(precision=L.FX_PREC_MIXED_POLY, stop_mode=L.FX_STOP_REL_R2, tol=1e-6, max_iter=100_000)
+    x3, rep3 = dm.solve(b[0], looser)
+    assert rep3.iterations < rep.iterations and np.linalg.norm(x3 - x) < 1e-4 * np.linalg.norm(x)
+    assert max(oracle_errors(jobs, x[None, :], b[:1])) < 1e-8
+    dm.close()
+
+
+def test_poly_static_steps(ctx):
+    """fx_static_step with its linear solves in FX_PREC_MIXED_POLY: the
+    linear cantilever and the NLGEOM beam take the FP64 steps' increments,
+    Newton iterations and linear solves, U within 1e-8 of theirs."""
+    from ferrix_amd.device import run_job
+    for job in (meshgen.cfg1_c3d4_cantilever(), meshgen.cfg3_c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)):
+        u64, r64 = run_job(ctx, job, precision=L.FX_PREC_FP64)
+        up, rp = run_job(ctx, job, precision=L.FX_PREC_MIXED_POLY)
+        e = np.linalg.norm(up - u64) / np.linalg.norm(u64)
+        say("n_dof", job.n_dof, "NR", r64[0].nr_iterations, rp[0].nr_iterations, "solves", r64[0].linear_solves,
+            rp[0].linear_solves, "U rel diff", e)
+        for f in ("increments", "cutbacks", "nr_iterations", "linear_solves"):
+            assert getattr(r64[0], f) == getattr(rp[0], f), f
+        assert e < 1e-8
 
 
 def test_poly_inner_shapes_bitwise(ctx):
