@@ -146,7 +146,7 @@ def run_single(a):
     for arm in a.arms.split(","):
         prec = {"mixed": L.FX_PREC_MIXED, "poly": L.FX_PREC_MIXED_POLY}.get(arm, L.FX_PREC_FP64)
         so = SolveOptions(precision=prec, stop_mode=L.FX_STOP_REL_R2, tol=st.tol, max_iter=st.max_iter,
-                          inner_tol=a.inner_tol, max_refine=100)
+                          inner_tol=a.inner_tol, max_refine=100, spmv_group=a.spmv_group)
         sd, keep = step_def(st, 1, so, so)
         lat, reps = [], []
         # the SpMV autotuner on this arm's operator, before the timed steps
@@ -220,7 +220,7 @@ def run_single(a):
         # inner launches, the batch's algorithmic byte count
         from ferrix_amd.device import Batch
         bt = Batch(ctx, SolveOptions(precision=L.FX_PREC_MIXED_POLY, stop_mode=L.FX_STOP_REL_R2, tol=st.tol,
-                                     max_iter=st.max_iter))
+                                     max_iter=st.max_iter, spmv_group=a.spmv_group))
         bt.time_spmv(True)
         bt.add(dm, b[None, :])
         bt.run()
