@@ -11,7 +11,11 @@ Bars:
 * both layouts of the FP16 operator: sliced ELL (C3D4 meshes) and node
   blocks with lane groups (C3D20 meshes, or C3D4 uploaded with FX_SELL=0);
 * the reference's stop rule (sqrt(r.z) < 1e-8 on the FP64 residual) and the
-  not-converged error of the reference's solver.
+  not-converged error of the reference's solver;
+* single systems through fx_solve / fx_static_step (a one-slot batch kept
+  with the mesh): bit for bit the system alone in a batch, and the FP64
+  steps' increments, Newton iterations and solves;
+* every inner-step shape and row width gives the same bits.
 """
 import numpy as np
 import pytest
