@@ -313,6 +313,28 @@ def test_spmv_autotune(ctx, precision):
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
 
 
+def test_spmv_tuning_cache():
+    """A second mesh of a timed bucket (the same mesh again, fresh context)
+    takes the stored lane group, per precision, with one timed run of it."""
+    from ferrix_amd.device import Context
+    job = meshgen.cfg2_c3d20_block(seed=3, nx=10, ny=4, nz=3)
+    c = Context(0)
+    try:
+        for precision in (L.FX_PREC_FP64, L.FX_PREC_MIXED):
+            out = []
+            for _ in range(2):
+                dm = DeviceMesh(c, job.mesh, job.node_temp)
+                dm.assemble(False)
+                g, ms = dm.autotune_spmv(precision, reps=10)
+                assert ms > 0.0
+                out.append(g)
+                dm.close()
+            say("precision", precision, "tuned / cached group", out)
+            assert out[0] == out[1]
+    finally:
+        c.close()
+
+
 def test_fp64_peak_microbenchmark(ctx):
     """The measured FP64 ceilings the element kernels' rates are stated
     against: both kernels run, and both land between a tenth of and just
