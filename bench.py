@@ -560,6 +560,7 @@ def run_batch(a):
         }
         if arm != "fp64_serial":
             res["first_run_at_ms"] = 1e3 * float(np.mean([r.first_run_at for r in runs]))
+            res["collect_ms"] = 1e3 * float(np.mean([r.collect_s for r in runs]))
         if last.inner_tune:
             sh, t1, t2 = last.inner_tune
             res["inner_shape"] = {"chosen": "per_load_case_count" if sh else "one_launch",

@@ -110,6 +110,7 @@ SIGNATURES = {
     "fx_batch_get": (i32, [P, i32, P]),
     "fx_batch_remove": (i32, [P, i32]),
     "fx_batch_time_spmv": (i32, [P, i32]),
+    "fx_batch_finished": (i32, [P, P, i32, P]),
     "fx_batch_autotune": (i32, [P, i32, P, P, P]),
     "fx_batch_set_inner_shape": (i32, [P, i32]),
     "fx_batch_spmv_stats": (i32, [P, P, P, P, P]),

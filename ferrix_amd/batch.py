@@ -92,6 +92,7 @@ class PopulationResult:
     spmv_stats: tuple = ()  # batched: (launches, timed launches, timed ms, algorithmic bytes of all launches)
     inner_tune: tuple = ()  # polynomial batch: (shape, ms one launch, ms split) of fx_batch_autotune
     first_run_at: float = 0.0  # batched: s from the start to the first run call (the admissions before it)
+    collect_s: float = 0.0     # batched: host s spent collecting finished systems (poll, get, remove)
 
     def dof_solves(self, systems: List[System]) -> int:
         return sum(s.n_dof * s.n_rhs for s in systems if s.key in self.x)
@@ -236,10 +237,9 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
                 tune = False
             if n_run == len(running):
                 continue  # none finished: nothing to collect, no memory freed
-            for slot in list(running):
+            tc = time.perf_counter()
+            for slot in bt.finished():
                 st, rep = bt.poll(slot)
-                if st == 0:
-                    continue
                 s = running.pop(slot)
                 ws -= operator_bytes(s, opts)
                 if st == 1:
@@ -249,6 +249,7 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
                 res.iterations[s.key] = rep.iterations
                 res.latency[s.key] = time.perf_counter() - t0
                 bt.remove(slot)
+            res.collect_s += time.perf_counter() - tc
         res.seconds = time.perf_counter() - t0
         res.spmv_stats = bt.spmv_stats()
     finally:

@@ -858,6 +858,7 @@ struct fx_batch {
     size_t fin_cap = 0;
     fx::DBuf<int2> d_tab4;
     fx::DBuf<int32_t> d_list2;
+    fx::DBuf<double> d_get;            // fx_batch_get: a slot's solutions, rhs-major
     std::vector<int2> h_tab_start;
     // SpMV timing (instrumentation): when on, every SpMV launch is bracketed
     // by two HIP events on the stream; the device counter sums the launches'
@@ -1540,6 +1541,13 @@ __global__ void k_interleave(int64_t n, int R, const double *__restrict__ src, d
         dst[q] = src[(q % R) * n + q / R];
 }
 
+// rhs-major from interleaved: dst[r*n + i] = src[i*R + r]
+__global__ void k_deinterleave(int64_t n, int R, const double *__restrict__ src, double *__restrict__ dst) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n * R)
+        dst[q] = src[(q % n) * R + q / n];
+}
+
 // A slot whose mesh's element kernels failed (singular Jacobian, no
 // material values) stops before its first iteration with that failure,
 // reported as status 16 + fx_status.
@@ -1832,6 +1840,23 @@ fx_status fx_batch_poll(fx_batch *B, int32_t slot, int32_t *state, fx_solve_repo
     return FX_OK;
 }
 
+fx_status fx_batch_finished(fx_batch *B, int32_t *slots, int32_t cap, int32_t *n) {
+    if (!B || !n || cap < 0 || (cap > 0 && !slots))
+        return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);
+    int32_t c = 0;
+    for (size_t k = 0; k < B->slots.size(); ++k) {
+        const auto &sl = B->slots[k];
+        if (sl && sl->used && sl->phase == 2) {
+            if (c < cap)
+                slots[c] = (int32_t)k;
+            ++c;
+        }
+    }
+    *n = c;
+    return FX_OK;
+}
+
 fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
     if (!B)
         return FX_ERR_INVALID;
@@ -1840,15 +1865,19 @@ fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
         return FX_ERR_INVALID;
     auto &sl = *B->slots[slot];
     try {
-        const int64_t n = sl.mesh->n_dof;
-        std::vector<double> inter(n * sl.R);
+        const int64_t n = sl.mesh->n_dof, nv = n * sl.R;
         const double *src = is_mixed(B) ? sl.xacc.p : sl.x.p;
-        FX_HIP(hipMemcpyAsync(inter.data(), src, inter.size() * sizeof(double), hipMemcpyDeviceToHost,
-                              B->ctx->stream));
-        FX_HIP(hipStreamSynchronize(B->ctx->stream));
-        for (int r = 0; r < sl.R; ++r)
-            for (int64_t i = 0; i < n; ++i)
-                x[(int64_t)r * n + i] = inter[i * sl.R + r];
+        hipStream_t s = B->ctx->stream;
+        if (sl.R > 1) {  // rhs-major on the device, then one copy
+            if ((int64_t)B->d_get.n < nv)
+                B->d_get.alloc((size_t)nv);
+            hipLaunchKernelGGL(fx::k_deinterleave, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, n, sl.R, src,
+                               B->d_get.p);
+            FX_HIP(hipGetLastError());
+            src = B->d_get.p;
+        }
+        FX_HIP(hipMemcpyAsync(x, src, (size_t)nv * sizeof(double), hipMemcpyDeviceToHost, s));
+        FX_HIP(hipStreamSynchronize(s));
     } catch (const fx::Error &e) {
         fx::set_error(e.msg);
         return e.code;

@@ -404,7 +404,7 @@ class Batch:
         return st.value, rep
 
     def get(self, slot: int) -> np.ndarray:
-        x = np.zeros(self._n[slot])
+        x = np.empty(self._n[slot])
         self.ctx.check(self.lib.fx_batch_get(self.h, slot, _p(x)))
         return x
 
@@ -433,6 +433,16 @@ class Batch:
     def set_inner_shape(self, shape: int):
         """Polynomial batch: 0 one launch per Chebyshev step, 1 one per load-case count."""
         self.ctx.check(self.lib.fx_batch_set_inner_shape(self.h, int(shape)))
+
+    def finished(self) -> List[int]:
+        """Slots whose solves have finished and are still in the batch."""
+        n = C.c_int32()
+        self.ctx.check(self.lib.fx_batch_finished(self.h, None, 0, C.byref(n)))
+        if n.value == 0:
+            return []
+        out = np.zeros(n.value, np.int32)
+        self.ctx.check(self.lib.fx_batch_finished(self.h, _p(out), n.value, C.byref(n)))
+        return [int(k) for k in out[:n.value]]
 
     def time_spmv(self, on: bool = True):
         """Bracket every later SpMV launch with HIP events."""

@@ -278,10 +278,9 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
                 nl_slots[bt.add(job.dm, job.rhs[None, :])] = job
             if running or nl_slots:
                 bt.run(max_iters=chunk_iters)
-                for slot in list(nl_slots):
+                done = bt.finished()
+                for slot in (k for k in done if k in nl_slots):
                     st, rep = bt.poll(slot)
-                    if st == 0:
-                        continue
                     job = nl_slots.pop(slot)
                     if st == 1:
                         job.x = bt.get(slot)[0]
@@ -289,10 +288,8 @@ def serve_batched(ctx: Context, reqs: List[Request], precision: int = L.FX_PREC_
                     res.batched_nl_solves += 1
                     bt.remove(slot)  # before the worker goes on with its mesh
                     job.done.set()
-                for slot in list(running):
+                for slot in (k for k in done if k in running):
                     st, rep = bt.poll(slot)
-                    if st == 0:
-                        continue
                     r, s, need = running.pop(slot)
                     if st == 1:
                         res.x[r.rid] = bt.get(slot)

@@ -458,6 +458,12 @@ fx_status fx_batch_run(fx_batch *batch, int64_t max_iters, int32_t *n_running);
  * fx_batch_add to completion). */
 fx_status fx_batch_poll(fx_batch *batch, int32_t slot, int32_t *state, fx_solve_report *report);
 
+/* The slots whose solves have finished (converged or stopped on an error)
+ * and are still in the batch, in slot order: the first `cap` of them into
+ * slots[], their number into *n (which may exceed cap).  Lets a driver
+ * collect results without polling every running slot. */
+fx_status fx_batch_finished(fx_batch *batch, int32_t *slots, int32_t cap, int32_t *n);
+
 /* Copies the solutions, x[r*n_dof + i]. */
 fx_status fx_batch_get(fx_batch *batch, int32_t slot, double *x);
 
