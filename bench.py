@@ -134,9 +134,9 @@ def run_single(a):
     from ferrix_amd import meshgen
     from ferrix_amd.device import Context, DeviceMesh, SolveOptions, step_def
 
-    job = meshgen.cfg2_c3d20_block()
+    job = meshgen.c3d20_block()
     st = job.steps[0]
-    golden = os.path.join(ROOT, "tests", "golden", "cfg2_oracle.npz")
+    golden = os.path.join(ROOT, "tests", "golden", "block_oracle.npz")
     u_ref = np.load(golden)["u"] if os.path.exists(golden) else None
     print(f"# workload {job.name}: {job.mesh.n_nodes} nodes, {job.mesh.n_el} C3D20, {job.n_dof} DOF", flush=True)
 
@@ -272,7 +272,7 @@ def run_single(a):
 
 
 def run_elements(a):
-    """C3D20 element stiffness on the cfg2 block (4,000 elements): the VALU
+    """C3D20 element stiffness on the 40x10x10 C3D20 block (4,000 elements): the VALU
     kernel (the oracle's order, bit for bit) against the FP64-MFMA one
     (FX_C3D20_MFMA=1), each timed over --steps launches between HIP events,
     beside the device's measured FP64 ceilings.  Then the C3D4 kernel on the
@@ -281,7 +281,7 @@ def run_elements(a):
     from ferrix_amd import meshgen
     from ferrix_amd.device import Context, DeviceMesh
 
-    job = meshgen.cfg2_c3d20_block()
+    job = meshgen.c3d20_block()
     ctx = Context(0)
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     valu_peak, mfma_peak = ctx.measure_fp64_peak(20)
@@ -297,7 +297,7 @@ def run_elements(a):
         print(f"# {name}: {json.dumps(out[name])}", flush=True)
     # C3D4 element batch: elements per workgroup on the population's largest
     # mesh, and the element autotuner's choice
-    j4 = meshgen.cfg4_mesh(max(meshgen.cfg4_batch_specs(), key=lambda sp: sp["n_dof"]))[0]
+    j4 = meshgen.population_mesh(max(meshgen.population_specs(), key=lambda sp: sp["n_dof"]))[0]
     c3d4 = {"n_elements": j4.mesh.n_el, "us_per_launch": {}}
     old = os.environ.get("FX_C3D4_BLOCK")
     try:
@@ -331,8 +331,8 @@ def population(n_meshes):
     from ferrix_amd import meshgen
     from ferrix_amd.batch import System
 
-    specs = meshgen.cfg4_batch_specs(n_meshes=n_meshes)
-    return specs, [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
+    specs = meshgen.population_specs(n_meshes=n_meshes)
+    return specs, [System(k, meshgen.population_mesh(sp)) for k, sp in enumerate(specs)]
 
 
 def host_info():
@@ -384,7 +384,7 @@ def cpu_baseline_batch(specs, stride=64, thread_legs=(4, None)):
     dofs, names, iters = 0, [], 0
     for k in range(0, len(specs), stride):
         sp = specs[k]
-        jobs = meshgen.cfg4_mesh(sp)
+        jobs = meshgen.population_mesh(sp)
         st = jobs[0].steps[0]
         t0 = time.perf_counter()
         r, c, v, md = O.assemble_triplets(jobs[0].mesh, jobs[0].node_temp)

@@ -184,7 +184,7 @@ def _clamp_and_load(grid: BlockGrid, load_dir, total: float, amp=None):
     return loads, bcs
 
 
-def cfg1_c3d4_cantilever() -> Job:
+def c3d4_cantilever() -> Job:
     """100x20x20 mm block, 20x4x4 hex cells of 5 mm cut into 1,920 C3D4
     (525 nodes, 1,575 DOF); E=210000, nu=0.3; x=0 clamped; -100 N in z over
     the 25 x=100 nodes."""
@@ -195,7 +195,7 @@ def cfg1_c3d4_cantilever() -> Job:
     return Job(mesh, np.zeros(g.n_nodes), [step], "c3d4_cantilever_20x4x4")
 
 
-def cfg2_c3d20_block(seed: int = 2, nx=40, ny=10, nz=10, jitter=True) -> Job:
+def c3d20_block(seed: int = 2, nx=40, ny=10, nz=10, jitter=True) -> Job:
     """400x100x100 mm block, 40x10x10 C3D20 (18,821 nodes, 56,463 DOF),
     coordinates jittered by U(-0.5, 0.5) mm per coordinate in node-ID order
     from SplitMix64(seed); E=70000, nu=0.33; x=0 clamped; -1000 N in z over
@@ -213,7 +213,7 @@ def cfg2_c3d20_block(seed: int = 2, nx=40, ny=10, nz=10, jitter=True) -> Job:
     return Job(mesh, np.zeros(g.n_nodes), [step], f"c3d20_block_{nx}x{ny}x{nz}")
 
 
-def cfg3_c3d20_nlgeom_beam(nx=80, ny=4, nz=4, tip_load=110000.0) -> Job:
+def c3d20_nlgeom_beam(nx=80, ny=4, nz=4, tip_load=110000.0) -> Job:
     """1000x50x50 mm beam, 80x4x4 C3D20 (7,265 nodes, 21,795 DOF),
     E=210000, nu=0.3, NLGEOM; tip load in -z ramped by *AMPLITUDE (0,0,1,1)
     with *STATIC 0.1, 1., 1e-5, 0.1."""
@@ -238,7 +238,7 @@ def c3d4_box(nx, ny, nz, h, E, nu, loads_dirs: List[Tuple[float, float, float]],
     return jobs
 
 
-def _cfg4_draw(rng: SplitMix64, index: int, dof_lo: float, dof_hi: float) -> dict:
+def _population_draw(rng: SplitMix64, index: int, dof_lo: float, dof_hi: float) -> dict:
     """One mesh of the batched-solve population (draw order: target DOF,
     aspect, E, nu, load-case count, directions)."""
     target = math.exp(math.log(dof_lo) + rng.uniform() * (math.log(dof_hi) - math.log(dof_lo)))
@@ -261,17 +261,17 @@ def _cfg4_draw(rng: SplitMix64, index: int, dof_lo: float, dof_hi: float) -> dic
                 n_dof=3 * (nx + 1) * (n + 1) * (n + 1))
 
 
-def cfg4_batch_specs(n_meshes=512, seed=4, dof_lo=2e3, dof_hi=2e5):
+def population_specs(n_meshes=512, seed=4, dof_lo=2e3, dof_hi=2e5):
     """Parameters of the batched-solve population: per mesh a target DOF
     (log-uniform), an aspect ratio U(1,10) of the x edge over the y/z edges,
     E log-uniform in [1e3, 3e5], nu U(0.2, 0.45) and 1-4 load cases, each a
     random unit direction times 100 N on the x=max face.  Returns a list of
-    dicts; :func:`cfg4_mesh` builds one."""
+    dicts; :func:`population_mesh` builds one."""
     rng = SplitMix64(seed)
-    return [_cfg4_draw(rng, m, dof_lo, dof_hi) for m in range(n_meshes)]
+    return [_population_draw(rng, m, dof_lo, dof_hi) for m in range(n_meshes)]
 
 
-def cfg4_mesh(spec) -> List[Job]:
+def population_mesh(spec) -> List[Job]:
     return c3d4_box(spec["nx"], spec["ny"], spec["nz"], spec["h"], spec["E"], spec["nu"], spec["dirs"])
 
 
@@ -299,7 +299,7 @@ def c3d20_block_cells(target_dof: float):
     return 4 * best[1], best[1]
 
 
-def cfg5_request_specs(n_requests=1000, seed=5, mix=(0.7, 0.2, 0.1)):
+def request_specs(n_requests=1000, seed=5, mix=(0.7, 0.2, 0.1)):
     """The serving stream's requests (arrival times are drawn separately):
     kind 'linear' (a population mesh with its 1-4 load cases), 'nlgeom' (a
     population mesh, NLGEOM, first load case) or 'c3d20' (a jittered 4:1:1
@@ -309,9 +309,9 @@ def cfg5_request_specs(n_requests=1000, seed=5, mix=(0.7, 0.2, 0.1)):
     for k in range(n_requests):
         u = rng.uniform()
         if u < mix[0]:
-            out.append(dict(kind="linear", mesh=_cfg4_draw(rng, k, 2e3, 2e5)))
+            out.append(dict(kind="linear", mesh=_population_draw(rng, k, 2e3, 2e5)))
         elif u < mix[0] + mix[1]:
-            out.append(dict(kind="nlgeom", mesh=_cfg4_draw(rng, k, 2e3, 2e5)))
+            out.append(dict(kind="nlgeom", mesh=_population_draw(rng, k, 2e3, 2e5)))
         else:
             target = math.exp(math.log(1e4) + rng.uniform() * (math.log(5e5) - math.log(1e4)))
             nx, n = c3d20_block_cells(target)
@@ -319,14 +319,14 @@ def cfg5_request_specs(n_requests=1000, seed=5, mix=(0.7, 0.2, 0.1)):
     return out
 
 
-def cfg5_jobs(req) -> List[Job]:
+def request_jobs(req) -> List[Job]:
     """Jobs of one request: the load cases (sharing mesh and BCs) of a linear
     request, or the single job of an NLGEOM / C3D20 request."""
     if req["kind"] == "linear":
-        return cfg4_mesh(req["mesh"])
+        return population_mesh(req["mesh"])
     if req["kind"] == "nlgeom":
         return [c3d4_box_nlgeom(req["mesh"])]
-    return [cfg2_c3d20_block(seed=req["seed"], nx=req["nx"], ny=req["n"], nz=req["n"])]
+    return [c3d20_block(seed=req["seed"], nx=req["nx"], ny=req["n"], nz=req["n"])]
 
 
 def to_inp(job: Job) -> str:

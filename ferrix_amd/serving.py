@@ -95,14 +95,14 @@ def request_from_deck(rid: int, arrival: float, deck_path: str,
 def make_requests(n: int, rate: float, seed: int = 5, arrival_seed: int = 55) -> List[Request]:
     """Requests of the serving mix with Poisson arrivals at `rate` per second
     (rate <= 0: everything arrives at t = 0)."""
-    specs = meshgen.cfg5_request_specs(n, seed)
+    specs = meshgen.request_specs(n, seed)
     rng = np.random.default_rng(arrival_seed)
     t = 0.0
     out = []
     for k, sp in enumerate(specs):
         if rate > 0:
             t += float(rng.exponential(1.0 / rate))
-        out.append(Request(k, t if rate > 0 else 0.0, sp["kind"], meshgen.cfg5_jobs(sp)))
+        out.append(Request(k, t if rate > 0 else 0.0, sp["kind"], meshgen.request_jobs(sp)))
     return out
 
 

@@ -1,14 +1,14 @@
 """Writes the golden fixtures from the CPU oracle (run from the repo root):
 
-    python tests/golden/make_golden.py [cfg1 cfg2 cfg3 ...]
+    python tests/golden/make_golden.py [cantilever block beam beam_converging]
 
 For every config: the oracle's filtered CSR pattern size and FNV-1a hash
 (row_ptr as int64 LE, then col as int32 LE), max_diag, and the converged
 displacement of the oracle's step driver with the DIRECT stand-in (SuperLU)
--- the FP64 CPU oracle the GPU results are compared against.  cfg3 is the
+-- the FP64 CPU oracle the GPU results are compared against.  beam is the
 NLGEOM beam (10 increments + the 1e-16 micro-increment); its per-increment
 NR iteration counts, relative residuals and the step's outcome (the error
-that stops it, if any) are stored too.  cfg3_conv is the same beam at a tip
+that stops it, if any) are stored too.  beam_converging is the same beam at a tip
 load the step completes.
 """
 import json
@@ -38,14 +38,14 @@ def fnv1a(*arrays) -> str:
 
 
 CONFIGS = {
-    "cfg1": meshgen.cfg1_c3d4_cantilever,
-    "cfg2": meshgen.cfg2_c3d20_block,
-    "cfg3": meshgen.cfg3_c3d20_nlgeom_beam,
+    "cantilever": meshgen.c3d4_cantilever,
+    "block": meshgen.c3d20_block,
+    "beam": meshgen.c3d20_nlgeom_beam,
     # the same 80x4x4 beam at a tip load the Picard NLGEOM iteration follows
-    # to the end of the step (cfg3's 110 kN stops at the minimum increment)
-    "cfg3_conv": lambda: meshgen.cfg3_c3d20_nlgeom_beam(tip_load=CFG3_CONV_LOAD),
+    # to the end of the step (the 110 kN of `beam` stops at the minimum increment)
+    "beam_converging": lambda: meshgen.c3d20_nlgeom_beam(tip_load=BEAM_CONVERGING_LOAD),
 }
-CFG3_CONV_LOAD = 20000.0
+BEAM_CONVERGING_LOAD = 20000.0
 
 
 def main(names):
@@ -56,8 +56,8 @@ def main(names):
         meta = dict(config=name, workload=job.name, n_nodes=job.mesh.n_nodes, n_el=job.mesh.n_el,
                     n_dof=job.n_dof, nnz=csr.nnz, max_diag=md,
                     pattern_fnv1a=O.pattern_fnv1a(csr.row_ptr, csr.col))
-        if name == "cfg3_conv":
-            meta["tip_load"] = CFG3_CONV_LOAD
+        if name == "beam_converging":
+            meta["tip_load"] = BEAM_CONVERGING_LOAD
         # the step driver with the DIRECT stand-in; an error that stops the
         # step (as the reference's would) is part of the fixture
         u = np.zeros(job.n_dof)

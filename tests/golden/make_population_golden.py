@@ -44,11 +44,11 @@ def rhs_of(jobs, pen):
 
 
 def main():
-    specs = meshgen.cfg4_batch_specs()
+    specs = meshgen.population_specs()
     out, meta = {}, []
     for k in MEMBERS:
         t0 = time.time()
-        jobs = meshgen.cfg4_mesh(specs[k])
+        jobs = meshgen.population_mesh(specs[k])
         csr, md, pen = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
         B = rhs_of(jobs, pen)
         U = np.array([O.direct_solve(csr, b) for b in B])

@@ -64,7 +64,7 @@ def population(ctx, specs, sell=True):
     old = os.environ.get("FX_SELL")
     os.environ["FX_SELL"] = "1" if sell else "0"
     try:
-        return [system(ctx, meshgen.cfg4_mesh(s)) for s in specs]
+        return [system(ctx, meshgen.population_mesh(s)) for s in specs]
     finally:
         if old is None:
             del os.environ["FX_SELL"]
@@ -73,7 +73,7 @@ def population(ctx, specs, sell=True):
 
 
 def small_specs(n, seed=11):
-    return meshgen.cfg4_batch_specs(n_meshes=n, seed=seed, dof_lo=300, dof_hi=6000)
+    return meshgen.population_specs(n_meshes=n, seed=seed, dof_lo=300, dof_hi=6000)
 
 
 REL = SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000, spmv_group=8)
@@ -184,7 +184,7 @@ def test_mixed_batch_alone_vs_mix(ctx, spmv_group):
         x = bt.get(s)
         assert np.array_equal(x, solo[k]), f"system {k}"
         # against the direct solve of the oracle's penalised matrix
-        jobs = meshgen.cfg4_mesh(spec)
+        jobs = meshgen.population_mesh(spec)
         csr, _, _ = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
         b = sys[k][1]
         for r in range(b.shape[0]):
@@ -237,7 +237,7 @@ def test_mixed_batch_symmetric_and_full_fp32_operators(ctx):
     say("algorithmic bytes per SpMV launch: full", per_launch["0"], "symmetric", per_launch["1"])
     assert per_launch["1"] < 0.8 * per_launch["0"]
     for k, spec in enumerate(specs):
-        jobs = meshgen.cfg4_mesh(spec)
+        jobs = meshgen.population_mesh(spec)
         csr, _, _ = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
         b = sys[k][1]
         for r in range(b.shape[0]):
@@ -252,7 +252,7 @@ def test_batch_with_unused_node(ctx):
     clamp.  Placed inside a slice, that row goes through the sliced-ELL
     operators (FP64, and the symmetric FP32 one of the mixed batch); both
     batches match the oracle's direct solve."""
-    job = meshgen.cfg4_mesh(small_specs(1, seed=47)[0])[0]
+    job = meshgen.population_mesh(small_specs(1, seed=47)[0])[0]
     m, st = job.mesh, job.steps[0]
     k = m.n_nodes // 2 + 5
     mesh = dataclasses.replace(m, node_ids=np.arange(1, m.n_nodes + 2, dtype=m.node_ids.dtype),
@@ -288,7 +288,7 @@ def test_mixed_batch_ill_conditioned_does_not_stall(ctx):
     Krylov direction once left load case 0 on a stalling FP32 Krylov space
     (23,000 iterations).  Each replacement now restarts the direction unless
     the new residual is within 10% of the iterated one."""
-    spec = meshgen.cfg4_batch_specs()[216]
+    spec = meshgen.population_specs()[216]
     assert (spec["nx"], spec["ny"], spec["nz"]) == (26, 4, 4)
     (dm, b), = population(ctx, [spec])
     bt = Batch(ctx, SolveOptions(precision=L.FX_PREC_MIXED, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=6000,
@@ -299,7 +299,7 @@ def test_mixed_batch_ill_conditioned_does_not_stall(ctx):
     say("iterations", rep.iterations, "replacements", rep.refinements, "criterion", rep.final_criterion)
     assert st == 1 and rep.iterations < 6000
     x = bt.result(s)
-    jobs = meshgen.cfg4_mesh(spec)
+    jobs = meshgen.population_mesh(spec)
     csr, _, _ = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
     for r in range(b.shape[0]):
         xd = O.direct_solve(csr, b[r])
@@ -364,7 +364,7 @@ def test_population_driver_budget(ctx, precision):
     same, since a slot's arithmetic does not depend on when it joins."""
     from ferrix_amd.batch import System, operator_bytes, solve_batched, upload
 
-    systems = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(small_specs(7, seed=53))]
+    systems = [System(k, meshgen.population_mesh(sp)) for k, sp in enumerate(small_specs(7, seed=53))]
     upload(ctx, systems)
     opts = SolveOptions(precision=precision, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000)
     a = solve_batched(ctx, systems, opts)
@@ -387,7 +387,7 @@ def test_population_driver_assembles_ahead(ctx, precision):
     from ferrix_amd.device import Context
 
     specs = small_specs(7, seed=53)
-    ref = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
+    ref = [System(k, meshgen.population_mesh(sp)) for k, sp in enumerate(specs)]
     upload(ctx, ref)
     opts = SolveOptions(precision=precision, stop_mode=L.FX_STOP_REL_R2, tol=1e-10, max_iter=100_000)
     a = solve_batched(ctx, ref, opts)
@@ -395,7 +395,7 @@ def test_population_driver_assembles_ahead(ctx, precision):
         solve_batched(ctx, ref, opts, assemble_ahead=True)  # meshes on the batch's own context
     mctx = Context(0)
     try:
-        systems = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
+        systems = [System(k, meshgen.population_mesh(sp)) for k, sp in enumerate(specs)]
         upload(mctx, systems)
         b = solve_batched(ctx, systems, opts, chunk_iters=16, assemble_ahead=True)
         budget = 2 * max(operator_bytes(s, opts) for s in systems)
@@ -450,7 +450,7 @@ def test_device_admission_bitwise(ctx, precision):
                         max_iter=400_000 if precision == L.FX_PREC_MIXED_POLY else 100_000)
     out = []
     for dev in (True, False):
-        systems = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
+        systems = [System(k, meshgen.population_mesh(sp)) for k, sp in enumerate(specs)]
         upload(ctx, systems)
         out.append(solve_batched(ctx, systems, opts, chunk_iters=16, device_admission=dev))
         for s in systems:
@@ -468,8 +468,8 @@ def test_device_admission_element_failure(ctx):
     Jacobian status, and its neighbours in the batch solve as alone."""
     from ferrix_amd.batch import System, admit
     specs = small_specs(2, seed=43)
-    good = [System(k, meshgen.cfg4_mesh(sp)) for k, sp in enumerate(specs)]
-    bad_job = meshgen.cfg1_c3d4_cantilever()
+    good = [System(k, meshgen.population_mesh(sp)) for k, sp in enumerate(specs)]
+    bad_job = meshgen.c3d4_cantilever()
     m = bad_job.mesh
     xyz = m.xyz.copy()
     nodes = m.conn[m.el_ptr[5]:m.el_ptr[6]]
@@ -507,7 +507,7 @@ def test_foreign_mesh_destroyed_while_batch_runs(ctx):
     specs = small_specs(3, seed=47)
     alone = []
     for sp in specs:
-        dm, b = system(ctx, meshgen.cfg4_mesh(sp))
+        dm, b = system(ctx, meshgen.population_mesh(sp))
         bt = Batch(ctx, REL)
         k = bt.add(dm, b)
         bt.run()
@@ -516,7 +516,7 @@ def test_foreign_mesh_destroyed_while_batch_runs(ctx):
         dm.close()
     for rnd in range(3):
         other = Context(0)
-        sys_ = [system(other, meshgen.cfg4_mesh(sp)) for sp in specs]
+        sys_ = [system(other, meshgen.population_mesh(sp)) for sp in specs]
         bt = Batch(ctx, REL)
         slots = [bt.add(dm, b) for dm, b in sys_]
         dropped = threading.Event()
@@ -556,7 +556,7 @@ def test_add_system_load_records(ctx):
     apply_penalty + fx_batch_add."""
     from ferrix_amd.batch import System, admit, assemble
     sp = small_specs(3, seed=53)[2]
-    jobs = meshgen.cfg4_mesh(sp)
+    jobs = meshgen.population_mesh(sp)
     rng = np.random.default_rng(5)
     for j in jobs:
         st = j.steps[0]

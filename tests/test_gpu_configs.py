@@ -45,9 +45,9 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_cfg2_full_size_pattern_and_max_diag(ctx):
-    meta, arr = golden("cfg2")
-    job = meshgen.cfg2_c3d20_block()
+def test_block_full_size_pattern_and_max_diag(ctx):
+    meta, arr = golden("block")
+    job = meshgen.c3d20_block()
     assert (job.mesh.n_nodes, job.mesh.n_el, job.n_dof) == (meta["n_nodes"], meta["n_el"], meta["n_dof"])
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     md, nnz = dm.assemble(False)
@@ -61,11 +61,11 @@ def test_cfg2_full_size_pattern_and_max_diag(ctx):
 
 
 @pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
-def test_cfg2_full_size_step(ctx, precision):
+def test_block_full_size_step(ctx, precision):
     """One linear step: Jacobi-PCG to ||r||/||b|| <= 1e-10 (the
     configuration's solver setting) against the oracle's direct solve."""
-    meta, arr = golden("cfg2")
-    job = meshgen.cfg2_c3d20_block()
+    meta, arr = golden("block")
+    job = meshgen.c3d20_block()
     u, reps = run_job(ctx, job, precision=precision)
     e = rel(u, arr["u"])
     say("precision", precision, "pcg", reps[0].pcg_iterations, "rel err", e)
@@ -93,16 +93,16 @@ def nlgeom_run(ctx, job, precision):
 
 
 @pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
-def test_cfg3_full_size_stops_like_the_oracle(ctx, precision):
+def test_beam_full_size_stops_like_the_oracle(ctx, precision):
     """80x4x4 C3D20 beam, 110 kN: the oracle's Picard iteration converges six
     increments (t = 0.1 ... 0.6), then fails 15 NR iterations at every
     halving of dt until dt < 1e-5 and stops with "Minimum time increment
     reached" (static_step.rs:251-258).  The GPU step stops with the same
     error after the same increments, cutbacks, NR iterations and linear
     solves, and leaves the displacement of the sixth increment."""
-    meta, arr = golden("cfg3")
+    meta, arr = golden("beam")
     assert meta["outcome"].startswith("Minimum time increment reached")
-    job = meshgen.cfg3_c3d20_nlgeom_beam()
+    job = meshgen.c3d20_nlgeom_beam()
     assert job.n_dof == meta["n_dof"] == 21_795
     u, rep, err, seen = nlgeom_run(ctx, job, precision)
     incs = meta["increments"]
@@ -123,12 +123,12 @@ def test_cfg3_full_size_stops_like_the_oracle(ctx, precision):
 
 
 @pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
-def test_cfg3_mesh_converging_load(ctx, precision):
+def test_beam_converging_load(ctx, precision):
     """The same 80x4x4 beam at a tip load the step completes: the same NR
     history as the oracle and the displacement within 1e-8 of its."""
-    meta, arr = golden("cfg3_conv")
+    meta, arr = golden("beam_converging")
     assert meta["outcome"] == "ok"
-    job = meshgen.cfg3_c3d20_nlgeom_beam(tip_load=meta["tip_load"])
+    job = meshgen.c3d20_nlgeom_beam(tip_load=meta["tip_load"])
     u, rep, err, seen = nlgeom_run(ctx, job, precision)
     assert err is None
     incs = meta["increments"]

@@ -41,10 +41,10 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_solve_csr_reference_rule_cfg1(ctx):
+def test_solve_csr_reference_rule_cantilever(ctx):
     """sqrt(r.z) < 1e-8, at most 1000 iterations (iterative.rs:26-121,
     static_step.rs:204) on the 20x4x4 C3D4 cantilever."""
-    csr, b = system(meshgen.cfg1_c3d4_cantilever())
+    csr, b = system(meshgen.c3d4_cantilever())
     x, rep = ctx.solve_csr(csr.row_ptr, csr.col, csr.val, b, SolveOptions())
     xo, it_o, _, ok = O.pcg(csr, b, 1e-8, 1000, 0)
     assert ok
@@ -60,7 +60,7 @@ def test_solve_csr_reference_rule_cfg1(ctx):
 def test_solve_csr_c3d20_block(ctx, group):
     """A jittered 10x4x3 C3D20 block (156 entries per row) to a relative
     residual of 1e-12, every lane-group size giving the direct solution."""
-    csr, b = system(meshgen.cfg2_c3d20_block(seed=7, nx=10, ny=4, nz=3))
+    csr, b = system(meshgen.c3d20_block(seed=7, nx=10, ny=4, nz=3))
     opts = SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-12, max_iter=100_000, spmv_group=group)
     x, rep = ctx.solve_csr(csr.row_ptr, csr.col, csr.val, b, opts)
     xo, it_o, _, ok = O.pcg(csr, b, 1e-12, 100_000, 1)
@@ -75,7 +75,7 @@ def test_solve_csr_c3d20_block(ctx, group):
 def test_solve_csr_uses_the_matrix_passed_in(ctx):
     """Two different matrices through the same context give their own
     solutions: nothing of an earlier call's matrix is kept."""
-    c1, b1 = system(meshgen.cfg1_c3d4_cantilever())
+    c1, b1 = system(meshgen.c3d4_cantilever())
     scaled = O.CSR(c1.n, c1.row_ptr, c1.col, 2.0 * c1.val)
     opts = SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-12, max_iter=100_000)
     x1, _ = ctx.solve_csr(c1.row_ptr, c1.col, c1.val, b1, opts)
@@ -102,7 +102,7 @@ def test_solve_csr_jacobi_guard(ctx):
 
 
 def test_solve_csr_not_converged(ctx):
-    csr, b = system(meshgen.cfg1_c3d4_cantilever())
+    csr, b = system(meshgen.c3d4_cantilever())
     with pytest.raises(ConvergenceError):
         ctx.solve_csr(csr.row_ptr, csr.col, csr.val, b, SolveOptions(max_iter=5))
 

@@ -44,7 +44,7 @@ def dense(rp, col, val, n):
 @pytest.mark.parametrize("seed", range(4))
 @pytest.mark.parametrize("nlgeom", [False, True])
 def test_single_element_k(ctx, mfma, seed, nlgeom):
-    job = meshgen.cfg2_c3d20_block(seed=seed + 11, nx=1, ny=1, nz=1, jitter=True)
+    job = meshgen.c3d20_block(seed=seed + 11, nx=1, ny=1, nz=1, jitter=True)
     n = job.n_dof
     u = np.random.default_rng(seed).normal(scale=2.0, size=n) if nlgeom else None
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
@@ -61,7 +61,7 @@ def test_single_element_k(ctx, mfma, seed, nlgeom):
 
 
 def test_block_assembly_and_linear_step(ctx, mfma):
-    job = meshgen.cfg2_c3d20_block(seed=7, nx=10, ny=4, nz=3, jitter=True)
+    job = meshgen.c3d20_block(seed=7, nx=10, ny=4, nz=3, jitter=True)
     n = job.n_dof
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     md, nnz = dm.assemble(False)
@@ -81,7 +81,7 @@ def test_block_assembly_and_linear_step(ctx, mfma):
 
 
 def test_nlgeom_beam(ctx, mfma):
-    job = meshgen.cfg3_c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)
+    job = meshgen.c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)
     u_o, logs = O.run_job(job)
     u, reps = run_job(ctx, job)
     e = np.linalg.norm(u - u_o) / np.linalg.norm(u_o)
@@ -95,7 +95,7 @@ def test_c3d4_workgroup_sizes_bitwise(ctx, monkeypatch, block):
     """k_c3d4 at 64 / 256 / 512 elements per workgroup (FX_C3D4_BLOCK at
     upload) assembles the default 128's K bit for bit; the element autotuner
     picks one of the four and leaves K unchanged."""
-    job = meshgen.cfg4_mesh(meshgen.cfg4_batch_specs(n_meshes=4, seed=9, dof_lo=2e3, dof_hi=2e4)[1])[0]
+    job = meshgen.population_mesh(meshgen.population_specs(n_meshes=4, seed=9, dof_lo=2e3, dof_hi=2e4)[1])[0]
     ref = DeviceMesh(ctx, job.mesh, job.node_temp)
     md0, _ = ref.assemble(False)
     K0 = ref.matrix()
@@ -116,7 +116,7 @@ def test_c3d4_workgroup_sizes_bitwise(ctx, monkeypatch, block):
 def test_element_autotune_c3d20(ctx):
     """C3D20: without MFMA allowed the tuner keeps the VALU kernel (K bit for
     bit); allowed, it keeps the faster one (K within 1e-13 of the VALU's)."""
-    job = meshgen.cfg2_c3d20_block(seed=5, nx=10, ny=4, nz=3, jitter=True)
+    job = meshgen.c3d20_block(seed=5, nx=10, ny=4, nz=3, jitter=True)
     n = job.n_dof
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     dm.assemble(False)
@@ -139,7 +139,7 @@ def test_element_tuning_cache():
     first mesh's."""
     from ferrix_amd.device import Context
     import time
-    job = meshgen.cfg4_mesh(meshgen.cfg4_batch_specs(n_meshes=4, seed=9, dof_lo=2e4, dof_hi=4e4)[2])[0]
+    job = meshgen.population_mesh(meshgen.population_specs(n_meshes=4, seed=9, dof_lo=2e4, dof_hi=4e4)[2])[0]
     c = Context(0)
     try:
         d1 = DeviceMesh(c, job.mesh, job.node_temp)

@@ -73,8 +73,8 @@ def compare(ctx, job, precision, label):
 
 @pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
 def test_linear_step(ctx, precision):
-    job = meshgen.cfg1_c3d4_cantilever()
-    r_h, r_d = compare(ctx, job, precision, f"cfg1 p{precision}")
+    job = meshgen.c3d4_cantilever()
+    r_h, r_d = compare(ctx, job, precision, f"cantilever p{precision}")
     if precision == L.FX_PREC_FP64:
         # iteration 1: front + queued solve (+ its chunked finish); iteration 2: the converged check
         assert r_d[0].host_syncs <= r_d[0].nr_iterations + r_d[0].linear_solves * 12
@@ -82,30 +82,30 @@ def test_linear_step(ctx, precision):
 
 @pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
 def test_nlgeom_beam(ctx, precision):
-    job = meshgen.cfg3_c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)
+    job = meshgen.c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)
     r_h, r_d = compare(ctx, job, precision, f"beam 8x2x2 p{precision}")
     if precision == L.FX_PREC_FP64:
         assert r_d[0].host_syncs < 5 * r_d[0].nr_iterations
 
 
 def test_c3d20_linear_block(ctx):
-    job = meshgen.cfg2_c3d20_block(seed=3, nx=10, ny=4, nz=3)
+    job = meshgen.c3d20_block(seed=3, nx=10, ny=4, nz=3)
     compare(ctx, job, L.FX_PREC_FP64, "c3d20 10x4x3")
 
 
 def test_failing_step_full_beam(ctx):
     """80x4x4 C3D20 beam at 110 kN: 'Minimum time increment reached' after
     the same history from both loops (FP64)."""
-    job = meshgen.cfg3_c3d20_nlgeom_beam()
+    job = meshgen.c3d20_nlgeom_beam()
     r_h, r_d = compare(ctx, job, L.FX_PREC_FP64, "beam 80x4x4 110 kN")
     assert r_h[0].cutbacks > 0
 
 
 @pytest.mark.parametrize("precision", [L.FX_PREC_FP64, L.FX_PREC_MIXED])
 def test_converging_full_beam(ctx, precision):
-    """The 80x4x4 beam at the tip load its step completes (the cfg3_conv
+    """The 80x4x4 beam at the tip load its step completes (the beam_converging
     golden's): both loops, timed."""
     import json
-    meta = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cfg3_conv_oracle.json")))
-    job = meshgen.cfg3_c3d20_nlgeom_beam(tip_load=meta["tip_load"])
+    meta = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "beam_converging_oracle.json")))
+    job = meshgen.c3d20_nlgeom_beam(tip_load=meta["tip_load"])
     compare(ctx, job, precision, f"beam 80x4x4 {meta['tip_load']:.0f} N p{precision}")

@@ -29,7 +29,7 @@ def say(*a):
 
 
 def small_c3d20(jitter=True, nx=6, ny=3, nz=2):
-    return meshgen.cfg2_c3d20_block(seed=7, nx=nx, ny=ny, nz=nz, jitter=jitter)
+    return meshgen.c3d20_block(seed=7, nx=nx, ny=ny, nz=nz, jitter=jitter)
 
 
 def oracle_csr(job, u=None, bc=None):
@@ -46,7 +46,7 @@ def assert_same_csr(dm, csr):
 
 
 @pytest.mark.parametrize("name,make", [
-    ("cfg1_c3d4", meshgen.cfg1_c3d4_cantilever),
+    ("cantilever_c3d4", meshgen.c3d4_cantilever),
     ("c3d20_jitter", lambda: small_c3d20(True)),
     ("c3d20_axis_aligned", lambda: small_c3d20(False)),
 ])
@@ -78,7 +78,7 @@ def test_assembly_nlgeom_bitwise(ctx):
 
 
 def test_internal_force_bitwise_c3d4(ctx):
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     u = np.random.default_rng(1).normal(size=job.n_dof)
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     dm.set_u(u)
@@ -89,7 +89,7 @@ def test_internal_force_bitwise_c3d4(ctx):
 
 
 def test_penalty_matches_oracle(ctx):
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     st = job.steps[0]
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     dm.assemble(False)
@@ -107,9 +107,9 @@ def test_penalty_matches_oracle(ctx):
     assert_same_csr(dm, csr)
 
 
-def test_pcg_reference_rule_cfg1(ctx):
+def test_pcg_reference_rule_cantilever(ctx):
     """sqrt(r.z) < 1e-8, max 1000 (iterative.rs / static_step.rs:204)."""
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     st = job.steps[0]
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     dm.assemble(False)
@@ -128,7 +128,7 @@ def test_pcg_reference_rule_cfg1(ctx):
 
 
 def test_pcg_not_converged(ctx):
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     st = job.steps[0]
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     dm.assemble(False)
@@ -169,8 +169,8 @@ def test_pcg_lane_groups(ctx, G):
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
 
 
-def test_step_cfg1_matches_oracle(ctx):
-    job = meshgen.cfg1_c3d4_cantilever()
+def test_step_cantilever_matches_oracle(ctx):
+    job = meshgen.c3d4_cantilever()
     u_o, logs = O.run_job(job)
     u, reps = run_job(ctx, job)
     say("nr", reps[0].nr_iterations, "pcg", reps[0].pcg_iterations)
@@ -199,7 +199,7 @@ def test_step_nlgeom_beam_matches_oracle(ctx, precision):
     """Picard NLGEOM (static_step.rs:101-258, coordinates X + u) on a small
     C3D20 beam, 10 ramped increments: same NR history as the oracle and the
     same displacements to 1e-7 (each linear solve to rel 1e-10)."""
-    job = meshgen.cfg3_c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)
+    job = meshgen.c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)
     u_o, logs = O.run_job(job)
     u, reps = run_job(ctx, job, precision=precision)
     say("nr", reps[0].nr_iterations, "solves", reps[0].linear_solves, "oracle solves", logs[0].linear_solves,
@@ -229,7 +229,7 @@ def test_reference_nlgeom_deck(ctx, precision):
 
 
 def test_singular_jacobian(ctx):
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     m = job.mesh
     xyz = m.xyz.copy()
     # collapse element 5 onto a plane: all four nodes at the first node
@@ -296,7 +296,7 @@ def test_spmv_autotune(ctx, precision):
     """fx_mesh_autotune_spmv times every lane-group size and keeps the
     fastest; a solve whose options leave spmv_group at 0 uses it, and its
     solution still matches the oracle's direct solve."""
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     st = job.steps[0]
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     dm.assemble(False)
@@ -317,7 +317,7 @@ def test_spmv_tuning_cache():
     """A second mesh of a timed bucket (the same mesh again, fresh context)
     takes the stored lane group, per precision, with one timed run of it."""
     from ferrix_amd.device import Context
-    job = meshgen.cfg2_c3d20_block(seed=3, nx=10, ny=4, nz=3)
+    job = meshgen.c3d20_block(seed=3, nx=10, ny=4, nz=3)
     c = Context(0)
     try:
         for precision in (L.FX_PREC_FP64, L.FX_PREC_MIXED):

@@ -72,7 +72,7 @@ def test_poly_batch_alone_vs_mix_and_oracle(ctx):
         x = bt.get(s)
         assert np.array_equal(x, solo[k][0]), f"system {k}: mix differs from alone"
         assert rep.iterations == solo[k][1].iterations
-        errs = oracle_errors(meshgen.cfg4_mesh(spec), x, sys[k][1])
+        errs = oracle_errors(meshgen.population_mesh(spec), x, sys[k][1])
         say("system", k, "n_dof", spec["n_dof"], "inner steps", rep.iterations, "outer", rep.outer_spmv,
             "rel err", max(errs))
         assert max(errs) < 1e-8
@@ -91,7 +91,7 @@ def test_poly_operator_layouts(ctx, sell):
     bt.run()
     for k, (s, spec) in enumerate(zip(slots, specs)):
         assert bt.poll(s)[0] == 1
-        errs = oracle_errors(meshgen.cfg4_mesh(spec), bt.result(s), sys[k][1])
+        errs = oracle_errors(meshgen.population_mesh(spec), bt.result(s), sys[k][1])
         say("sell", sell, "system", k, "rel err", max(errs))
         assert max(errs) < 1e-8
 
@@ -99,7 +99,7 @@ def test_poly_operator_layouts(ctx, sell):
 def test_poly_c3d20_block(ctx):
     """A jittered 10x4x3 C3D20 block (node-block operator, 27 blocks per node
     row) beside two C3D4 systems in one batch."""
-    job = meshgen.cfg2_c3d20_block(seed=7, nx=10, ny=4, nz=3)
+    job = meshgen.c3d20_block(seed=7, nx=10, ny=4, nz=3)
     dm20, b20 = system(ctx, [job])
     specs = small_specs(2, seed=8)
     sys = population(ctx, specs)
@@ -112,7 +112,7 @@ def test_poly_c3d20_block(ctx):
     say("c3d20 rel err", errs, "report", bt.poll(s20)[1].iterations)
     assert max(errs) < 1e-8
     for k, s in enumerate(slots):
-        assert max(oracle_errors(meshgen.cfg4_mesh(specs[k]), bt.result(s), sys[k][1])) < 1e-8
+        assert max(oracle_errors(meshgen.population_mesh(specs[k]), bt.result(s), sys[k][1])) < 1e-8
 
 
 @pytest.mark.parametrize("degree", [1, 8, 32])
@@ -127,7 +127,7 @@ def test_poly_degrees(ctx, degree):
     for k, s in enumerate(slots):
         st, rep = bt.poll(s)
         assert st == 1 and rep.iterations == degree * rep.outer_spmv
-        assert max(oracle_errors(meshgen.cfg4_mesh(specs[k]), bt.result(s), sys[k][1])) < 1e-8
+        assert max(oracle_errors(meshgen.population_mesh(specs[k]), bt.result(s), sys[k][1])) < 1e-8
 
 
 def test_poly_chunking_does_not_change_results(ctx):
@@ -161,7 +161,7 @@ def test_poly_reference_stop_rule(ctx):
         assert st == 1 and rep.final_criterion < 1e-8
         dm, b = sys[k]
         x = bt.result(s)
-        jobs = meshgen.cfg4_mesh(specs[k])
+        jobs = meshgen.population_mesh(specs[k])
         csr, _, _ = O.assemble_csr(jobs[0].mesh, jobs[0].node_temp, None, jobs[0].steps[0].bc_dof)
         inv = O.jacobi(csr)
         for r in range(b.shape[0]):
@@ -190,9 +190,9 @@ def test_poly_single_solve(ctx, kind):
     1e-8 of the oracle's direct solve; a second solve reuses the batch, other
     options make a new one, and destroying the mesh frees it."""
     if kind == "c3d4":
-        jobs = meshgen.cfg4_mesh(small_specs(3, seed=71)[1])
+        jobs = meshgen.population_mesh(small_specs(3, seed=71)[1])
     else:
-        jobs = [meshgen.cfg2_c3d20_block(seed=5, nx=10, ny=4, nz=3, jitter=True)]
+        jobs = [meshgen.c3d20_block(seed=5, nx=10, ny=4, nz=3, jitter=True)]
     dm, b = system(ctx, jobs[:1])
     x, rep = dm.solve(b[0], POLY)
     say(kind, "inner steps", rep.iterations, "outer", rep.outer_spmv)
@@ -216,7 +216,7 @@ def test_poly_static_steps(ctx):
     linear cantilever and the NLGEOM beam take the FP64 steps' increments,
     Newton iterations and linear solves, U within 1e-8 of theirs."""
     from ferrix_amd.device import run_job
-    for job in (meshgen.cfg1_c3d4_cantilever(), meshgen.cfg3_c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)):
+    for job in (meshgen.c3d4_cantilever(), meshgen.c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)):
         u64, r64 = run_job(ctx, job, precision=L.FX_PREC_FP64)
         up, rp = run_job(ctx, job, precision=L.FX_PREC_MIXED_POLY)
         e = np.linalg.norm(up - u64) / np.linalg.norm(u64)
@@ -254,7 +254,7 @@ def test_poly_inner_shapes_bitwise(ctx):
         for other in out[1:]:
             assert np.array_equal(out[0][k][0], other[k][0]), f"system {k}"
             assert out[0][k][1] == other[k][1]
-    assert max(oracle_errors(meshgen.cfg4_mesh(specs[0]), out[1][0][0], sys[0][1])) < 1e-8
+    assert max(oracle_errors(meshgen.population_mesh(specs[0]), out[1][0][0], sys[0][1])) < 1e-8
 
 
 def test_poly_autotune_needs_poly_batch(ctx):
@@ -270,7 +270,7 @@ def test_poly_rows_per_workgroup_bitwise(ctx):
     """A batch whose running systems hold more than 2 M node rows runs the
     Chebyshev step with 512 rows per workgroup; a system alone runs it with
     256.  The system's solutions are the same bits either way."""
-    specs = meshgen.cfg4_batch_specs(n_meshes=40, seed=61, dof_lo=1.5e5, dof_hi=2.0e5)
+    specs = meshgen.population_specs(n_meshes=40, seed=61, dof_lo=1.5e5, dof_hi=2.0e5)
     sys = population(ctx, specs)
     rows = sum(dm.n_dof // 3 for dm, _ in sys)
     say("node rows in the batch", rows)

@@ -44,11 +44,11 @@ OPTS = {
 def hard(ctx):
     meta = json.load(open(GOLD + ".json"))
     gold = np.load(GOLD + ".npz")
-    specs = meshgen.cfg4_batch_specs()
+    specs = meshgen.population_specs()
     out = []
     for m in meta:
         k = m["member"]
-        jobs = meshgen.cfg4_mesh(specs[k])
+        jobs = meshgen.population_mesh(specs[k])
         assert jobs[0].n_dof == m["n_dof"] and len(jobs) == m["n_rhs"]
         dm, b = system(ctx, jobs)
         out.append((k, dm, b, gold[f"m{k}_sub"], gold[f"m{k}_norm"], gold[f"m{k}_maxabs"]))

@@ -23,9 +23,9 @@ def check(ctx, job, u, T=None):
 
 
 @pytest.mark.parametrize("name,make", [
-    ("cfg1_c3d4", meshgen.cfg1_c3d4_cantilever),
-    ("c3d20_jitter", lambda: meshgen.cfg2_c3d20_block(seed=7, nx=6, ny=3, nz=2)),
-    ("c3d20_axis_aligned", lambda: meshgen.cfg2_c3d20_block(seed=7, nx=4, ny=2, nz=2, jitter=False)),
+    ("cantilever_c3d4", meshgen.c3d4_cantilever),
+    ("c3d20_jitter", lambda: meshgen.c3d20_block(seed=7, nx=6, ny=3, nz=2)),
+    ("c3d20_axis_aligned", lambda: meshgen.c3d20_block(seed=7, nx=4, ny=2, nz=2, jitter=False)),
 ])
 def test_recovery_bitwise(ctx, name, make):
     job = make()
@@ -34,7 +34,7 @@ def test_recovery_bitwise(ctx, name, make):
 
 
 def test_recovery_temperature_dependent(ctx):
-    job = meshgen.cfg2_c3d20_block(seed=6, nx=3, ny=2, nz=2)
+    job = meshgen.c3d20_block(seed=6, nx=3, ny=2, nz=2)
     m = job.mesh
     m.lut_n_e[0] = 2
     m.lut_t_e[0, :2] = [0.0, 100.0]
@@ -48,7 +48,7 @@ def test_recovery_after_step(ctx):
     """Stress of the solved cantilever: bitwise the oracle's recovery of the
     GPU displacement, and the bending stress has the expected sign pattern
     (tension on top near the clamp under a downward tip load)."""
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     u, _ = run_job(ctx, job)
     S, E = check(ctx, job, u)
     X = job.mesh.xyz.reshape(-1, 3)
@@ -62,7 +62,7 @@ def test_reaction_forces_balance_the_load(ctx):
     same u, and in equilibrium with the applied load: the rigid translations
     are in K's null space, so sum f_int = 0 per direction and the reactions
     carry exactly minus the applied total (penalty BCs, converged step)."""
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     st = job.steps[0]
     dm = DeviceMesh(ctx, job.mesh, job.node_temp)
     u, _ = run_job(ctx, job, mesh=dm)

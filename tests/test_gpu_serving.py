@@ -21,16 +21,16 @@ def say(*a):
 
 
 def stream():
-    specs = meshgen.cfg4_batch_specs(n_meshes=5, seed=17, dof_lo=500, dof_hi=8000)
+    specs = meshgen.population_specs(n_meshes=5, seed=17, dof_lo=500, dof_hi=8000)
     reqs = []
     t = 0.0
     for k, sp in enumerate(specs):
-        reqs.append(Request(len(reqs), t, "linear", meshgen.cfg4_mesh(sp)))
+        reqs.append(Request(len(reqs), t, "linear", meshgen.population_mesh(sp)))
         t += 0.004
         if k % 2 == 0:
             reqs.append(Request(len(reqs), t, "nlgeom", [meshgen.c3d4_box_nlgeom(sp)]))
             t += 0.004
-    reqs.append(Request(len(reqs), t, "c3d20", [meshgen.cfg2_c3d20_block(seed=3, nx=8, ny=2, nz=2)]))
+    reqs.append(Request(len(reqs), t, "c3d20", [meshgen.c3d20_block(seed=3, nx=8, ny=2, nz=2)]))
     return reqs
 
 
@@ -58,7 +58,7 @@ def test_serving_nlgeom_own_solver_vs_batch(ctx):
     """NLGEOM requests whose Newton solves join the batch against the same
     requests solving with the step's own FP64 PCG on the worker's stream:
     the same number of linear solves, U within 1e-8."""
-    specs = meshgen.cfg4_batch_specs(n_meshes=3, seed=29, dof_lo=500, dof_hi=6000)
+    specs = meshgen.population_specs(n_meshes=3, seed=29, dof_lo=500, dof_hi=6000)
     reqs = [Request(k, 0.0, "nlgeom", [meshgen.c3d4_box_nlgeom(sp)]) for k, sp in enumerate(specs)]
     inb = serve_batched(ctx, reqs, L.FX_PREC_FP64, chunk_iters=8)
     own = serve_batched(ctx, reqs, L.FX_PREC_FP64, chunk_iters=8, nl_in_batch=False)

@@ -69,7 +69,7 @@ def test_vtk_writer_nlgeom_increments(ctx, tmp_path):
     end of the step, the last file holding the final displacement."""
     from ferrix_amd import meshgen
 
-    job = meshgen.cfg3_c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)
+    job = meshgen.c3d20_nlgeom_beam(nx=8, ny=2, nz=2, tip_load=20000.0)
     w = VtkWriter(str(tmp_path / "beam.inp"), job.mesh)
     u, reps = run_job(ctx, job, writer=w)
     series = open(tmp_path / "beam" / "results.vtk.series").read()

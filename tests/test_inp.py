@@ -105,7 +105,7 @@ def test_nlgeom_deck_steps():
 
 
 def test_synthetic_deck_round_trip():
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     p = Project.from_string(meshgen.to_inp(job))
     j2 = Job.from_project(p)
     assert np.array_equal(j2.mesh.xyz, job.mesh.xyz)

@@ -118,7 +118,7 @@ def test_lut_reference_values():
 
 # ---------------------------------------------------------------- elements
 def _one_elem_mesh(nn):
-    job = meshgen.cfg2_c3d20_block(seed=11, nx=1, ny=1, nz=1) if nn == 20 else meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d20_block(seed=11, nx=1, ny=1, nz=1) if nn == 20 else meshgen.c3d4_cantilever()
     m = job.mesh
     nodes = m.conn[m.el_ptr[0]:m.el_ptr[1]]
     return m, m.xyz[nodes]
@@ -179,7 +179,7 @@ def test_internal_force_equals_k_u(nn):
 def test_patch_test_c3d20_jittered():
     """u = eps x (linear field): internal nodes carry no force and the
     stress is D eps -> f_int vanishes away from the boundary."""
-    job = meshgen.cfg2_c3d20_block(seed=3, nx=3, ny=3, nz=3)
+    job = meshgen.c3d20_block(seed=3, nx=3, ny=3, nz=3)
     m = job.mesh
     eps = np.array([[1e-3, 2e-4, -1e-4], [3e-4, -5e-4, 2e-4], [0.0, 1e-4, 7e-4]])
     u = (m.xyz @ eps.T).ravel()
@@ -230,7 +230,7 @@ def test_triplets_to_csr_matches_scipy():
 def test_assembly_matches_dense_element_sum():
     """Global K equals the dense sum of element matrices scattered by DOF
     (drop rule only removes |v| <= 1e-12)."""
-    job = meshgen.cfg2_c3d20_block(seed=5, nx=2, ny=2, nz=1)
+    job = meshgen.c3d20_block(seed=5, nx=2, ny=2, nz=1)
     m = job.mesh
     csr, md, _ = O.assemble_csr(m, job.node_temp)
     Kd = np.zeros((m.n_dof, m.n_dof))
@@ -244,7 +244,7 @@ def test_assembly_matches_dense_element_sum():
 
 
 def test_max_diag_is_per_element_maximum():
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     m = job.mesh
     _, _, _, md = O.assemble_triplets(m, job.node_temp)
     best = 0.0
@@ -256,8 +256,8 @@ def test_max_diag_is_per_element_maximum():
 
 
 # ---------------------------------------------------------------- solvers
-def test_pcg_against_dense_solve_cfg1():
-    job = meshgen.cfg1_c3d4_cantilever()
+def test_pcg_against_dense_solve_cantilever():
+    job = meshgen.c3d4_cantilever()
     st = job.steps[0]
     csr, md, pen = O.assemble_csr(job.mesh, job.node_temp, None, st.bc_dof)
     b = np.zeros(job.n_dof)
@@ -274,7 +274,7 @@ def test_pcg_against_dense_solve_cfg1():
 
 
 def test_pcg_reports_not_converged():
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     st = job.steps[0]
     csr, _, _ = O.assemble_csr(job.mesh, job.node_temp, None, st.bc_dof)
     b = np.zeros(job.n_dof)
@@ -288,7 +288,7 @@ def test_pcg_par_is_the_same_algorithm():
     in iterative.rs) on one thread is bit for bit fxo_pcg (same sums in the
     same order); on several threads only the dot products' association
     changes; both stop rules and the not-converged outcome agree."""
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     st = job.steps[0]
     csr, _, _ = O.assemble_csr(job.mesh, job.node_temp, None, st.bc_dof)
     b = np.zeros(job.n_dof)
@@ -349,7 +349,7 @@ def test_oracle_reference_decks(name, node, expected, tol):
 def test_linear_step_counts():
     """Linear step: one solve, converged at NR iteration 1 (Appendix-A
     behaviour of static_step.rs: f_int, K, solve, f_int)."""
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     u, logs = O.run_job(job)
     inc = logs[0].increments
     assert len(inc) == 1 and inc[0]["converged"] and inc[0]["iters"] == [0] and len(inc[0]["rel"]) == 2
@@ -359,7 +359,7 @@ def test_linear_step_counts():
 def test_micro_increment_quirk():
     """0.1 steps sum to 0.9999999999999999 < 1: an 11th increment of ~1e-16
     runs and converges at iteration 0 (static_step.rs:71-84, 213)."""
-    job = meshgen.cfg3_c3d20_nlgeom_beam(nx=4, ny=1, nz=1, tip_load=10.0)
+    job = meshgen.c3d20_nlgeom_beam(nx=4, ny=1, nz=1, tip_load=10.0)
     job.steps[0].solver = SOLVER_DIRECT
     u, logs = O.run_job(job)
     incs = logs[0].increments
@@ -367,16 +367,16 @@ def test_micro_increment_quirk():
     assert incs[-1]["dt"] < 1e-15 and incs[-1]["iters"] == []
 
 
-@pytest.mark.parametrize("name", ["cfg1", "cfg2"])
+@pytest.mark.parametrize("name", ["cantilever", "block"])
 def test_golden_fixture_metadata(name):
     meta = json.load(open(os.path.join(ROOT, "tests", "golden", f"{name}_oracle.json")))
-    gen = {"cfg1": meshgen.cfg1_c3d4_cantilever, "cfg2": meshgen.cfg2_c3d20_block}[name]()
+    gen = {"cantilever": meshgen.c3d4_cantilever, "block": meshgen.c3d20_block}[name]()
     assert (meta["n_nodes"], meta["n_el"], meta["n_dof"]) == (gen.mesh.n_nodes, gen.mesh.n_el, gen.n_dof)
-    if name == "cfg1":
+    if name == "cantilever":
         csr, md, _ = O.assemble_csr(gen.mesh, gen.node_temp)
         assert csr.nnz == meta["nnz"] and md == meta["max_diag"]
         u, _ = O.run_job(gen, solver_override=SOLVER_DIRECT)
-        g = np.load(os.path.join(ROOT, "tests", "golden", "cfg1_oracle.npz"))["u"]
+        g = np.load(os.path.join(ROOT, "tests", "golden", "cantilever_oracle.npz"))["u"]
         assert np.max(np.abs(u - g)) <= 1e-14 * np.max(np.abs(g))
 
 
@@ -391,7 +391,7 @@ def _voigt_field(eps):
 def test_stress_recovery_patch(nn):
     """A linear displacement field gives every node the constant strain and
     sigma = D eps (C3D4 and jittered C3D20 reproduce linear fields exactly)."""
-    job = meshgen.cfg1_c3d4_cantilever() if nn == 4 else meshgen.cfg2_c3d20_block(seed=9, nx=3, ny=2, nz=2)
+    job = meshgen.c3d4_cantilever() if nn == 4 else meshgen.c3d20_block(seed=9, nx=3, ny=2, nz=2)
     eps = np.array([1e-3, -2e-4, 3e-4, 5e-4, -1e-4, 2e-4])
     X = job.mesh.xyz.reshape(-1, 3)
     u = (X @ _voigt_field(eps).T).ravel()
@@ -406,7 +406,7 @@ def test_stress_recovery_single_element_node_by_differences():
     symmetric gradient of the element's isoparametric displacement at that
     corner, computed here by central differences of the shape functions N
     (not the derivative tables) through the geometric map."""
-    job = meshgen.cfg2_c3d20_block(seed=4, nx=2, ny=2, nz=2)
+    job = meshgen.c3d20_block(seed=4, nx=2, ny=2, nz=2)
     rng = np.random.default_rng(8)
     u = rng.normal(scale=1e-3, size=job.n_dof)
     S, E = O.stress_strain(job.mesh, job.node_temp, u)
@@ -438,7 +438,7 @@ def test_stress_recovery_uses_node_temperature():
     """update_state is called with the node's own temperature
     (static_step.rs:345-346), so with a temperature-dependent E every node's
     averaged stress is D(T_node) times its averaged strain."""
-    job = meshgen.cfg2_c3d20_block(seed=6, nx=2, ny=2, nz=1)
+    job = meshgen.c3d20_block(seed=6, nx=2, ny=2, nz=1)
     m = job.mesh
     m.lut_n_e[0] = 2
     m.lut_t_e[0, :2] = [0.0, 100.0]
@@ -456,7 +456,7 @@ def test_stress_recovery_uses_node_temperature():
 def test_pattern_fingerprint_is_fnv1a():
     """The golden fixtures' pattern hash (oracle C helper) is byte-wise
     FNV-1a over row_ptr (int64 LE) then col (int32 LE): equal to a plain
-    Python FNV-1a on a small pattern, and to the committed cfg1 hash."""
+    Python FNV-1a on a small pattern, and to the committed cantilever hash."""
     def fnv(data: bytes) -> str:
         h = 0xCBF29CE484222325
         for b in data:
@@ -468,7 +468,7 @@ def test_pattern_fingerprint_is_fnv1a():
     col = np.array([0, 2, 1, 0, 2], np.int32)
     assert O.pattern_fnv1a(rp, col) == fnv(rp.astype("<i8").tobytes() + col.astype("<i4").tobytes())
     assert O.pattern_fnv1a(rp, col) != O.pattern_fnv1a(rp, col[::-1].copy())
-    job = meshgen.cfg1_c3d4_cantilever()
+    job = meshgen.c3d4_cantilever()
     csr, _, _ = O.assemble_csr(job.mesh, job.node_temp)
-    meta = json.load(open(os.path.join(ROOT, "tests", "golden", "cfg1_oracle.json")))
+    meta = json.load(open(os.path.join(ROOT, "tests", "golden", "cantilever_oracle.json")))
     assert O.pattern_fnv1a(csr.row_ptr, csr.col) == meta["pattern_fnv1a"]

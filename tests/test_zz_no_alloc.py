@@ -92,10 +92,10 @@ def solve_both(ctx, csr, b, opts):
 
 def test_noalloc_matches_allocating_solve_twice(ctx):
     opts = SolveOptions(stop_mode=L.FX_STOP_REL_R2, tol=1e-12, max_iter=100_000)
-    big = system(meshgen.cfg2_c3d20_block(seed=7, nx=10, ny=4, nz=3))  # 2,475 nodes, 7,425 DOF
-    small = system(meshgen.cfg1_c3d4_cantilever())                     # 525 nodes, 1,575 DOF
+    big = system(meshgen.c3d20_block(seed=7, nx=10, ny=4, nz=3))  # 2,475 nodes, 7,425 DOF
+    small = system(meshgen.c3d4_cantilever())                     # 525 nodes, 1,575 DOF
     assert small[0].n < big[0].n
-    for name, (csr, b) in (("c3d20 10x4x3", big), ("cfg1 c3d4", small)):
+    for name, (csr, b) in (("c3d20 10x4x3", big), ("cantilever c3d4", small)):
         print("case", name, "n", csr.n, "nnz", csr.nnz, flush=True)
         x_h, rep_h, x_d, rep_d = solve_both(ctx, csr, b, opts)
         assert rep_d.iterations == rep_h.iterations
@@ -107,7 +107,7 @@ def test_noalloc_matches_allocating_solve_twice(ctx):
 
 def test_noalloc_reference_rule(ctx):
     """The reference's stop rule (sqrt(r.z) < 1e-8, 1000 iterations)."""
-    csr, b = system(meshgen.cfg1_c3d4_cantilever())
+    csr, b = system(meshgen.c3d4_cantilever())
     x_h, rep_h, x_d, rep_d = solve_both(ctx, csr, b, SolveOptions())
     assert rep_d.iterations == rep_h.iterations
     assert np.array_equal(x_d, x_h)
