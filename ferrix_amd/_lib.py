@@ -113,6 +113,7 @@ SIGNATURES = {
     "fx_batch_finished": (i32, [P, P, i32, P]),
     "fx_batch_autotune": (i32, [P, i32, P, P, P]),
     "fx_batch_set_inner_shape": (i32, [P, i32]),
+    "fx_batch_set_overlap": (i32, [P, i32]),
     "fx_batch_spmv_stats": (i32, [P, P, P, P, P]),
     "fx_solve_csr_noalloc": (i32, [P, i64, i64, P, P, P, P, P, P, P]),
 }

@@ -126,13 +126,16 @@ def operator_bytes(s: System, opts: SolveOptions) -> int:
 
 def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int = 0, chunk_iters: int = 256,
                   time_spmv: bool = False, assemble_ahead: bool = False,
-                  device_admission: bool = True, inner_shape="tune") -> PopulationResult:
+                  device_admission: bool = True, inner_shape="tune", overlap: bool = True) -> PopulationResult:
     """Continuous batching over `systems` in order.  ws_bytes > 0 caps the
     summed operator bytes of the running systems (at least one always runs);
     0 admits everything at once.  Each Batch.run call advances the running
     systems by up to chunk_iters iterations (the library overlaps the host's
     bookkeeping of one 16-iteration chunk with the next); finished systems
-    are collected and new ones admitted between calls.
+    are collected and new ones admitted between calls.  overlap: run()
+    returns with up to three chunks still running on the GPU, which keeps
+    iterating while this loop collects and admits (Batch.set_overlap); the
+    results are the same bit for bit.
 
     assemble_ahead: the systems' meshes belong to another context than the
     batch's (their own stream).  A host thread assembles them in order while
@@ -156,6 +159,8 @@ def solve_batched(ctx, systems: List[System], opts: SolveOptions, ws_bytes: int 
     bt = Batch(ctx, opts)
     if time_spmv:
         bt.time_spmv(True)
+    if overlap:
+        bt.set_overlap(True)
     poly = opts.precision == L.FX_PREC_MIXED_POLY
     tune = poly and inner_shape == "tune"
     if poly and not tune:

@@ -358,6 +358,47 @@ void pool_clear(fx_ctx *ctx) {
     ctx->pool.clear();
     ctx->pool_bytes = 0;
 }
+
+constexpr size_t kPinCap = (size_t)4 << 30;
+
+void *pin_take(fx_ctx *ctx, size_t bytes, size_t *cap) {
+    {
+        std::lock_guard<std::mutex> lk(ctx->pin_mx);
+        auto it = ctx->pin_pool.lower_bound(bytes);
+        if (it != ctx->pin_pool.end() && it->first <= 2 * bytes) {
+            void *p = it->second;
+            *cap = it->first;
+            ctx->pin_bytes -= it->first;
+            ctx->pin_pool.erase(it);
+            return p;
+        }
+    }
+    const size_t b = (bytes + 4095) / 4096 * 4096;
+    void *p = nullptr;
+    FX_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+    *cap = b;
+    return p;
+}
+
+void pin_give(fx_ctx *ctx, void *p, size_t cap) {
+    {
+        std::lock_guard<std::mutex> lk(ctx->pin_mx);
+        if (ctx->pin_bytes + cap <= kPinCap) {
+            ctx->pin_pool.emplace(cap, p);
+            ctx->pin_bytes += cap;
+            return;
+        }
+    }
+    (void)hipHostFree(p);
+}
+
+void pin_clear(fx_ctx *ctx) {
+    std::lock_guard<std::mutex> lk(ctx->pin_mx);
+    for (auto &kv : ctx->pin_pool)
+        (void)hipHostFree(kv.second);
+    ctx->pin_pool.clear();
+    ctx->pin_bytes = 0;
+}
 }  // namespace fx
 
 extern "C" {
@@ -421,6 +462,8 @@ fx_status fx_ctx_create(int device, fx_ctx **out) {
         if (ctx) {
             if (ctx->stream)
                 (void)hipStreamDestroy(ctx->stream);
+            if (ctx->copy_stream)
+                (void)hipStreamDestroy(ctx->copy_stream);
             delete ctx;
         }
         return fail_ctx(nullptr, e.code, e.msg);
@@ -447,10 +490,13 @@ void fx_ctx_destroy(fx_ctx *ctx) {
     }
     ctx->meshes.clear();
     fx::pool_clear(ctx);  // after the batches, whose slots return their blocks
+    fx::pin_clear(ctx);
     for (auto *b : {&ctx->tables.n4, &ctx->tables.dn4, &ctx->tables.w4, &ctx->tables.n20, &ctx->tables.dn20,
                     &ctx->tables.w20, &ctx->tables.dn20n})
         b->release();
     (void)hipStreamDestroy(ctx->stream);
+    if (ctx->copy_stream)
+        (void)hipStreamDestroy(ctx->copy_stream);
     delete ctx;
 }
 

@@ -94,6 +94,12 @@ struct BSys {
     int32_t nparts3;                    // row groups of the FP64 outer SpMV (sliced ELL: kBB node rows;
                                         // node blocks: kBB / kOuterG)
     int32_t pad3;
+    // result export (polynomial batch, when hx is set): the outer iteration
+    // in which the slot stops sets exp; the next one's kp_upd copies x into
+    // the pinned host array hx and its kp_stop writes the finish summary, so
+    // the host reads the solutions straight from hx once it sees the summary
+    double *hx;
+    int32_t exp, pad4;
     double lam;                         // Rayleigh-quotient estimate of lambda_max(D^-1 K32)
     double cheb[2 * kPolyMax + 2];      // [0] = 1/theta; per inner step j >= 1: c1_j at 2j, c2_j at 2j+1
     double cheb_bytes0, cheb_bytes_rhs; // algorithmic bytes of one inner step: operator part, per load case

@@ -821,6 +821,7 @@ struct fx_batch {
         // device vectors from the context's pool: a later slot reuses them
         fx::PoolBuf<double> b, rin, x, r, z, p0, p1, ap, xacc, part1, part2;
         fx::PoolBuf<double> zq0, zq1, zq2, rs;  // polynomial batch: Chebyshev iterates, scaled residual
+        fx::PinBuf<double> hx;                 // polynomial batch: the solutions exported at the finish
         bool pending = false;                  // polynomial batch: admitted, its start sequence not yet run
         std::chrono::steady_clock::time_point t0;
         double seconds = 0.0;
@@ -830,9 +831,13 @@ struct fx_batch {
     fx::DBuf<fx::BSys> d_sys;
     static constexpr int kGroups = 6;  // lane groups 2, 4, ..., 64
     // Iteration chunks are pipelined: while chunk k runs, the host processes
-    // the state of chunk k-1.  Each of the two chunks in flight has its own
-    // block tables (device, and their pinned host staging), its own pinned
-    // copy of the system table taken at its end, and its own events.
+    // the state of chunk k-1.  Each of the kSets chunks that may be in
+    // flight has its own block tables (device, and their pinned host
+    // staging) and its own events.  Three: a budgeted fx_batch_run with
+    // overlap leaves up to three queued, which at full load (8 ms each on
+    // the 512-mesh population) cover the caller's collection and admission
+    // between two calls.
+    static constexpr int kSets = 3;
     struct ChunkSet {
         // tab3s / tab3: the polynomial batch's FP64 outer SpMV (sliced-ELL and node-block operators)
         fx::DBuf<int2> tab1g[kGroups], tab1s, tab2, tab3s, tab3;
@@ -848,7 +853,13 @@ struct fx_batch {
         std::vector<hipEvent_t> ev;    // SpMV timing: 2 per iteration
         int ev_used = 0;
     };
-    ChunkSet cs[2];
+    ChunkSet cs[kSets];
+    // chunks launched and not yet processed (the oldest in cs[head]); with
+    // `overlap` (fx_batch_set_overlap) a budgeted fx_batch_run returns with
+    // up to three still running, so the GPU keeps iterating while the caller
+    // collects and admits systems
+    int inflight = 0, head = 0;
+    bool overlap = false;
     // polynomial batch: the sliced-ELL inner step as one launch (false) or
     // one launch per load-case count (fx_batch_autotune)
     bool inner_split = false;
@@ -858,7 +869,10 @@ struct fx_batch {
     size_t fin_cap = 0;
     fx::DBuf<int2> d_tab4;
     fx::DBuf<int32_t> d_list2;
-    fx::DBuf<double> d_get;            // fx_batch_get: a slot's solutions, rhs-major
+    // fx_batch_get: pinned host staging of a slot's solutions, rhs-major,
+    // written by the device and copied out by the host
+    double *h_get = nullptr;
+    size_t h_get_cap = 0;
     std::vector<int2> h_tab_start;
     // SpMV timing (instrumentation): when on, every SpMV launch is bracketed
     // by two HIP events on the stream; the device counter sums the launches'
@@ -907,6 +921,8 @@ fx_batch::~fx_batch() {
             (void)hipEventDestroy(adm_ev);
         if (fin)
             (void)hipHostFree(fin);
+        if (h_get)
+            (void)hipHostFree(h_get);
         for (auto &c : cs) {
             for (auto &e : c.ev)
                 if (e)
@@ -1026,6 +1042,7 @@ void fill_sys(fx_batch *B, fx_batch::Slot &sl, BSys &S) {
         S.zq[1] = sl.zq1.p;
         S.zq[2] = sl.zq2.p;
         S.rs = sl.rs.p;
+        S.hx = sl.hx.p;
         S.val16 = m->kval16.p;
         S.sval16 = m->sval16.p;
         S.sval16b = m->sval16.p ? m->sval16.p + (size_t)m->scol.n * 8 : nullptr;
@@ -1190,12 +1207,15 @@ void build_tables(fx_batch *B, int set) {
     B->dead_blocks = 0;
 }
 
+void drain(fx_batch *B);
+
 void reserve_sys(fx_batch *B, size_t n) {
     if (B->d_sys.p && B->d_sys.n >= n)
         return;
     const size_t cap = std::max<size_t>({n, 2 * B->d_sys.n, 16});
-    // the finish summaries: pinned host memory the kernels write (no chunk is
-    // in flight between fx_batch_run calls, when this runs)
+    // the finish summaries: pinned host memory the kernels write, replaced
+    // once no chunk is in flight
+    drain(B);
     fx::BFin *f = nullptr;
     FX_HIP(hipHostMalloc((void **)&f, cap * sizeof(fx::BFin), hipHostMallocDefault));
     std::memset(f, 0, cap * sizeof(fx::BFin));
@@ -1416,6 +1436,21 @@ void advance_phases(fx_batch *B, int set) {
         B->run_version++;
 }
 
+// Waits for and processes the chunks a budgeted fx_batch_run left in flight.
+// The calls that change what those chunks read or write (the finish
+// summaries' buffer, a running slot's vectors, the tables they were built
+// with) drain first.  A finished slot needs no drain: every kernel returns
+// at once for it (no load case runs, its summary is written), so its
+// vectors are final and nothing in flight reads or writes them.
+void drain(fx_batch *B) {
+    while (B->inflight > 0) {
+        FX_HIP(hipEventSynchronize(B->cs[B->head].done));
+        advance_phases(B, B->head);
+        B->head = (B->head + 1) % fx_batch::kSets;
+        --B->inflight;
+    }
+}
+
 }  // namespace
 
 namespace fx {
@@ -1541,13 +1576,6 @@ __global__ void k_interleave(int64_t n, int R, const double *__restrict__ src, d
         dst[q] = src[(q % R) * n + q / R];
 }
 
-// rhs-major from interleaved: dst[r*n + i] = src[i*R + r]
-__global__ void k_deinterleave(int64_t n, int R, const double *__restrict__ src, double *__restrict__ dst) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n * R)
-        dst[q] = src[(q % n) * R + q / n];
-}
-
 // A slot whose mesh's element kernels failed (singular Jacobian, no
 // material values) stops before its first iteration with that failure,
 // reported as status 16 + fx_status.
@@ -1623,9 +1651,14 @@ fx_status batch_add_impl(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *r
         for (auto *v : {&sl.x, &sl.r, &sl.z, &sl.p0, &sl.p1, &sl.ap})
             if (v != &sl.z || !is_poly(B))
                 v->alloc(ctx, is_mixed(B) ? (nv + 1) / 2 : nv);  // the mixed solve stores FP32 vectors
-        if (is_poly(B))
+        if (is_poly(B)) {
             for (auto *v : {&sl.zq0, &sl.zq1, &sl.zq2, &sl.rs})
                 v->alloc(ctx, nv);
+            // the solutions go to pinned host memory as the slot finishes
+            // (a single solve's batch copies them on the device instead)
+            if (!B->owner)
+                sl.hx.alloc(ctx, nv);
+        }
         if (is_mixed(B)) {
             sl.rin.alloc(ctx, nv);  // replaced residuals: written by each replacement pass before use
             sl.xacc.alloc(ctx, nv);
@@ -1756,6 +1789,19 @@ void drop_poly_batch(fx_mesh *m) {
 
 }  // namespace fx
 
+namespace {
+// x[r*n + i] = h[i*R + r]
+void deinterleave(const double *h, int64_t n, int R, double *x) {
+    if (R == 1) {
+        std::memcpy(x, h, (size_t)n * sizeof(double));
+        return;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        for (int r = 0; r < R; ++r)
+            x[(int64_t)r * n + i] = h[i * R + r];
+}
+}  // namespace
+
 extern "C" {
 
 fx_status fx_batch_add(fx_batch *B, fx_mesh *m, int32_t n_rhs, const double *rhs, int32_t *slot_out) {
@@ -1772,14 +1818,15 @@ fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
         if (is_poly(B))
             start_pending_poly(B);
         int64_t done_iters = 0;
-        int inflight = 0, head = 0;  // chunks launched and not yet processed; the oldest one's set
+        int processed = 0;
         for (;;) {
-            // keep two chunks in flight while slots run and the budget lasts
+            // keep kSets chunks in flight while slots run and the budget lasts
             int running = 0;
             for (auto &sl : B->slots)
                 running += sl && sl->used && sl->phase == 0;
-            if (inflight < 2 && running > 0 && (max_iters <= 0 || done_iters < max_iters)) {
-                const int set = (head + inflight) % 2;
+            const bool budget = max_iters <= 0 || done_iters < max_iters;
+            if (B->inflight < fx_batch::kSets && running > 0 && budget) {
+                const int set = (B->head + B->inflight) % fx_batch::kSets;
                 if (B->cs[set].version != B->run_version)
                     build_tables(B, set);
                 // a chunk: 16 iterations, or (polynomial batch) the outer
@@ -1790,15 +1837,19 @@ fx_status fx_batch_run(fx_batch *B, int64_t max_iters, int32_t *n_running_out) {
                     chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(1, (max_iters - done_iters) / per));
                 launch_iterations(B, chunk, set);
                 done_iters += (int64_t)chunk * per;
-                ++inflight;
+                ++B->inflight;
                 continue;
             }
-            if (inflight == 0)
+            // overlap: a budgeted call returns once the budget is launched
+            // and at least one chunk has been processed; the rest stays in
+            // flight
+            if (B->inflight == 0 || (B->overlap && !budget && processed > 0))
                 break;
-            FX_HIP(hipEventSynchronize(B->cs[head].done));
-            advance_phases(B, head);
-            head ^= 1;
-            --inflight;
+            FX_HIP(hipEventSynchronize(B->cs[B->head].done));
+            advance_phases(B, B->head);
+            B->head = (B->head + 1) % fx_batch::kSets;
+            --B->inflight;
+            ++processed;
         }
         if (n_running_out) {
             int c = 0;
@@ -1867,21 +1918,58 @@ fx_status fx_batch_get(fx_batch *B, int32_t slot, double *x) {
     try {
         const int64_t n = sl.mesh->n_dof, nv = n * sl.R;
         const double *src = is_mixed(B) ? sl.xacc.p : sl.x.p;
-        hipStream_t s = B->ctx->stream;
-        if (sl.R > 1) {  // rhs-major on the device, then one copy
-            if ((int64_t)B->d_get.n < nv)
-                B->d_get.alloc((size_t)nv);
-            hipLaunchKernelGGL(fx::k_deinterleave, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, n, sl.R, src,
-                               B->d_get.p);
-            FX_HIP(hipGetLastError());
-            src = B->d_get.p;
+        // a finished slot's solutions are final (the kernel that wrote its
+        // summary ran after the last update): copied on the context's copy
+        // stream, past the chunks in flight; a running slot's current
+        // iterate: after the work queued so far
+        const int R = sl.R;
+        if (sl.phase == 2 && sl.hx.p) {
+            // exported by the device before it wrote the finish summary
+            deinterleave(sl.hx.p, n, R, x);
+            return FX_OK;
         }
-        FX_HIP(hipMemcpyAsync(x, src, (size_t)nv * sizeof(double), hipMemcpyDeviceToHost, s));
+        fx_ctx *ctx = B->ctx;
+        FX_HIP(hipSetDevice(ctx->device));
+        std::call_once(ctx->copy_once, [ctx] {
+            if (hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess)
+                ctx->copy_stream = nullptr;
+        });
+        hipStream_t s = sl.phase == 2 && ctx->copy_stream ? ctx->copy_stream : ctx->stream;
+        // one copy command into pinned host memory (the DMA engine: no
+        // workgroups queued behind the chunks in flight; a pageable copy
+        // would be staged in pieces), then the host writes x rhs-major
+        if (B->h_get_cap < (size_t)nv) {
+            if (B->h_get)
+                FX_HIP(hipHostFree(B->h_get));
+            B->h_get = nullptr;
+            B->h_get_cap = 0;
+            FX_HIP(hipHostMalloc((void **)&B->h_get, (size_t)nv * sizeof(double), hipHostMallocDefault));
+            B->h_get_cap = (size_t)nv;
+        }
+        FX_HIP(hipMemcpyAsync(B->h_get, src, (size_t)nv * sizeof(double), hipMemcpyDeviceToHost, s));
         FX_HIP(hipStreamSynchronize(s));
+        deinterleave(B->h_get, n, R, x);
     } catch (const fx::Error &e) {
         fx::set_error(e.msg);
         return e.code;
     }
+    return FX_OK;
+}
+
+fx_status fx_batch_set_overlap(fx_batch *B, int32_t on) {
+    if (!B)
+        return FX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(B->mx);
+    if (!on) {
+        try {
+            FX_HIP(hipSetDevice(B->ctx->device));
+            drain(B);
+        } catch (const fx::Error &e) {
+            fx::set_error(e.msg);
+            return e.code;
+        }
+    }
+    B->overlap = on != 0;
     return FX_OK;
 }
 
@@ -1906,8 +1994,9 @@ fx_status fx_batch_autotune(fx_batch *B, int32_t reps, int32_t *shape_out, doubl
             throw fx::Error{FX_ERR_INVALID, "fx_batch_autotune: needs a polynomial batch (FX_PREC_MIXED_POLY) and reps > 0"};
         FX_HIP(hipSetDevice(B->ctx->device));
         hipStream_t s = B->ctx->stream;
+        drain(B);
         FX_HIP(hipStreamSynchronize(s));
-        // the running systems' tables (no chunk is in flight between calls)
+        // the running systems' tables (no chunk is in flight now)
         fx_batch::ChunkSet &C = B->cs[0];
         if (C.version != B->run_version)
             build_tables(B, 0);
@@ -1945,6 +2034,7 @@ fx_status fx_batch_spmv_stats(fx_batch *B, int64_t *launches, int64_t *timed, do
     std::lock_guard<std::mutex> lk(B->mx);
     double by = 0.0;
     try {
+        drain(B);  // the timing samples of the chunks in flight
         FX_HIP(hipMemcpyAsync(&by, B->d_bytes.p, sizeof(double), hipMemcpyDeviceToHost, B->ctx->stream));
         FX_HIP(hipStreamSynchronize(B->ctx->stream));
     } catch (const fx::Error &e) {
@@ -1968,7 +2058,19 @@ fx_status fx_batch_remove(fx_batch *B, int32_t slot) {
     std::lock_guard<std::mutex> lk(B->mx);  // the slot table may change under a foreign mesh's removal
     if (!B || slot < 0 || slot >= (int)B->slots.size() || !B->slots[slot] || !B->slots[slot]->used)
         return FX_ERR_INVALID;
-    (void)hipStreamSynchronize(B->ctx->stream);
+    // a running slot: the chunks in flight still iterate it (and may write
+    // its finish summary, which a later slot here starts from zero); a
+    // finished one: they do not touch it
+    if (B->slots[slot]->phase == 0) {
+        try {
+            drain(B);
+        } catch (const fx::Error &e) {
+            fx::set_error(e.msg);
+            return e.code;
+        }
+    }
+    if (B->inflight == 0)
+        (void)hipStreamSynchronize(B->ctx->stream);
     const bool was_running = B->slots[slot]->phase == 0;
     if (slot < (int)B->h_sys.size())
         B->h_sys[slot] = BSys{};

@@ -645,12 +645,28 @@ __device__ __forceinline__ void upd_tile(const BSys &S, int lb, int go, double (
     }
 }
 
+// x of a stopped slot into its pinned host array: the block's DOF tile is
+// one contiguous range of kBB * R interleaved entries; the stores are made
+// visible to the host before the kernel ends (the next kernel, kp_stop,
+// then writes the finish summary the host waits for)
+__device__ __forceinline__ void export_tile(const BSys &S, int lb) {
+    const int64_t nv = S.n_dof * S.R, q0 = (int64_t)lb * kBB * S.R;
+    const int64_t q1 = q0 + (int64_t)kBB * S.R < nv ? q0 + (int64_t)kBB * S.R : nv;
+    gptr<const double> x = gp((const double *)S.x);
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += kBB)
+        S.hx[q] = x[q];
+    __threadfence_system();
+}
+
 __global__ __launch_bounds__(kBB) void kp_upd(BSys *__restrict__ sys, const int2 *__restrict__ tab) {
     const int2 t = tab[blockIdx.x];
     const BSys &S = sys[t.x];
     const int go = S.go;
-    if (!go)
+    if (!go) {
+        if (S.exp && !S.reported)
+            export_tile(S, t.y);
         return;
+    }
     double v[3 * kRmax];
 #pragma unroll
     for (int k = 0; k < 3 * kRmax; ++k)
@@ -729,6 +745,10 @@ __global__ __launch_bounds__(kBB) void kp_stop(BSys *__restrict__ sys, const int
         return;
     }
     if (!S.reported) {  // thread 0, no load case runs any more: tell the host
+        if (S.hx && !S.exp) {  // after the next kp_upd has exported x
+            S.exp = 1;
+            return;
+        }
         BFin &F = fin[slot];
         int64_t it = 0;
         for (int r = 0; r < kRmax; ++r) {

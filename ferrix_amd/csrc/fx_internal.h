@@ -163,6 +163,12 @@ struct DevTables {
 struct fx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // result copies that must not wait behind the work queued on `stream`
+    // (fx_batch_get of finished slots); made on first use (copy_once), so
+    // contexts that never copy results (admission, Newton workers) hold
+    // no second stream
+    hipStream_t copy_stream = nullptr;
+    std::once_flag copy_once;
     int n_cu = 0;
     fx::DevTables tables;
     std::mutex reg;                   // guards the two registries below
@@ -173,6 +179,11 @@ struct fx_ctx {
     std::mutex pool_mx;
     std::multimap<size_t, void *> pool;
     size_t pool_bytes = 0;
+    // the same for pinned host blocks (the polynomial batch's result
+    // exports), freed with the context
+    std::mutex pin_mx;
+    std::multimap<size_t, void *> pin_pool;
+    size_t pin_bytes = 0;
     // autotuner choices by size bucket (fx::tune_key): a mesh whose bucket
     // was already timed on this context takes the stored choice untimed
     std::mutex tune_mx;
@@ -208,6 +219,11 @@ void *pool_take(fx_ctx *ctx, size_t bytes, size_t *cap);
 // work that used it must have completed on the context stream.
 void pool_give(fx_ctx *ctx, void *p, size_t cap);
 void pool_clear(fx_ctx *ctx);
+// Pinned host blocks (hipHostMalloc), pooled the same way (at most 4 GiB
+// kept); the device may write such a block only while its owner holds it.
+void *pin_take(fx_ctx *ctx, size_t bytes, size_t *cap);
+void pin_give(fx_ctx *ctx, void *p, size_t cap);
+void pin_clear(fx_ctx *ctx);
 
 // A device array owned through the context's pool.
 template <typename T>
@@ -237,6 +253,32 @@ struct PoolBuf {
         alloc(c, v.size());
         if (n)
             FX_HIP(hipMemcpyAsync(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+// A pinned host array from the context's pinned pool (returned on release).
+template <typename T>
+struct PinBuf {
+    fx_ctx *ctx = nullptr;
+    T *p = nullptr;
+    size_t n = 0, cap = 0;
+    PinBuf() = default;
+    PinBuf(const PinBuf &) = delete;
+    PinBuf &operator=(const PinBuf &) = delete;
+    ~PinBuf() { release(); }
+    void release() {
+        if (p)
+            pin_give(ctx, p, cap);
+        p = nullptr;
+        n = cap = 0;
+    }
+    void alloc(fx_ctx *c, size_t count) {
+        release();
+        ctx = c;
+        if (count == 0)
+            return;
+        p = (T *)pin_take(c, count * sizeof(T), &cap);
+        n = count;
     }
 };
 }  // namespace fx

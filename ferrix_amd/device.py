@@ -434,6 +434,12 @@ class Batch:
         """Polynomial batch: 0 one launch per Chebyshev step, 1 one per load-case count."""
         self.ctx.check(self.lib.fx_batch_set_inner_shape(self.h, int(shape)))
 
+    def set_overlap(self, on: bool):
+        """on: a budgeted run() returns with up to three iteration chunks still
+        running on the GPU, which keeps iterating while the caller collects
+        and admits systems (fx_batch_set_overlap)."""
+        self.ctx.check(self.lib.fx_batch_set_overlap(self.h, 1 if on else 0))
+
     def finished(self) -> List[int]:
         """Slots whose solves have finished and are still in the batch."""
         n = C.c_int32()

@@ -449,8 +449,24 @@ fx_status fx_batch_add_system(fx_batch *batch, fx_mesh *mesh, int32_t n_rhs, int
 
 /* Advances every running system by up to max_iters PCG iterations
  * (max_iters <= 0: until none is running).  *n_running: systems still
- * running on return. */
+ * running on return, as far as the host has seen (with overlap, see
+ * fx_batch_set_overlap, some may have finished on the GPU meanwhile). */
 fx_status fx_batch_run(fx_batch *batch, int64_t max_iters, int32_t *n_running);
+
+/* Overlap of the GPU's iterations with the caller's work between
+ * fx_batch_run calls (no counterpart in the reference, whose solver has no
+ * batch).  on != 0: a budgeted fx_batch_run returns once the budget is
+ * launched and the oldest chunk of it has completed, with up to three chunks
+ * (16 iterations each, or the outer iterations holding about 16 inner
+ * steps) still running on the GPU, so it keeps iterating while the caller
+ * collects and admits systems; the next fx_batch_run takes them up.
+ * fx_batch_poll, fx_batch_finished, and fx_batch_get and fx_batch_remove
+ * of finished slots do not wait for them; the calls that depend on them
+ * (removing a running slot, admitting past the slot capacity, the
+ * autotuner, the statistics) do.  Device memory freed meanwhile by another
+ * thread (hipFree waits for the whole device) then waits for those chunks.
+ * on = 0 (default): fx_batch_run returns with nothing in flight. */
+fx_status fx_batch_set_overlap(fx_batch *batch, int32_t on);
 
 /* *state: 0 running, 1 converged, -fx_status when it stopped on an error
  * (FX_ERR_NOT_CONVERGED, FX_ERR_BREAKDOWN).  report as for fx_solve
@@ -464,7 +480,8 @@ fx_status fx_batch_poll(fx_batch *batch, int32_t slot, int32_t *state, fx_solve_
  * collect results without polling every running slot. */
 fx_status fx_batch_finished(fx_batch *batch, int32_t *slots, int32_t cap, int32_t *n);
 
-/* Copies the solutions, x[r*n_dof + i]. */
+/* Copies the solutions, x[r*n_dof + i] (a running slot: its current
+ * iterate, after the work queued so far). */
 fx_status fx_batch_get(fx_batch *batch, int32_t slot, double *x);
 
 /* Frees the slot for reuse (a running system is dropped). */

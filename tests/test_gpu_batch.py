@@ -133,9 +133,11 @@ def test_fp64_batch_sell_matches_solve(ctx):
         assert abs(rep.iterations - max(its)) <= max(2, max(its) // 50)
 
 
-def test_continuous_admission_and_removal(ctx):
+@pytest.mark.parametrize("overlap", [False, True])
+def test_continuous_admission_and_removal(ctx, overlap):
     """Systems admitted while others run, one removed mid-solve and its slot
-    reused: every finished result equals the system solved alone."""
+    reused: every finished result equals the system solved alone (overlap:
+    the budgeted runs return with chunks still in flight)."""
     sys = population(ctx, small_specs(5, seed=23))
     solo = []
     for dm, b in sys:
@@ -145,6 +147,7 @@ def test_continuous_admission_and_removal(ctx):
         solo.append(bt.result(s))
         bt.close()
     bt = Batch(ctx, REL)
+    bt.set_overlap(overlap)
     s0 = bt.add(*sys[0])
     s1 = bt.add(*sys[1])
     bt.run(max_iters=20)

@@ -241,6 +241,8 @@ def test_poly_inner_shapes_bitwise(ctx):
         bt = Batch(ctx, POLY)
         if shape != "tune":
             bt.set_inner_shape(shape)
+        else:
+            bt.set_overlap(True)  # the tuner first takes up the chunks left in flight
         slots = [bt.add(dm, b) for dm, b in sys]
         if shape == "tune":
             assert bt.run(max_iters=45) > 0
@@ -255,6 +257,49 @@ def test_poly_inner_shapes_bitwise(ctx):
             assert np.array_equal(out[0][k][0], other[k][0]), f"system {k}"
             assert out[0][k][1] == other[k][1]
     assert max(oracle_errors(meshgen.population_mesh(specs[0]), out[1][0][0], sys[0][1])) < 1e-8
+
+
+@pytest.mark.parametrize("overlap", [False, True])
+def test_poly_overlap_driver_bitwise(ctx, overlap):
+    """A driver loop of budgeted runs that collects finished systems (copied
+    out while later chunks still run, with overlap), drops a running system
+    and admits it again later, and admits the others into freed slots:
+    every result and iteration count equals the system solved alone."""
+    specs = small_specs(6, seed=43)
+    sys = population(ctx, specs)
+    solo = []
+    for dm, b in sys:
+        bt = Batch(ctx, POLY)
+        s = bt.add(dm, b)
+        assert bt.run() == 0
+        solo.append((bt.result(s), bt.poll(s)[1].iterations))
+        bt.close()
+    bt = Batch(ctx, POLY)
+    bt.set_overlap(overlap)
+    todo, running, got = list(range(len(sys))), {}, {}
+    dropped, passes = None, 0
+    while running or todo:
+        while todo and len(running) < 3:
+            k = todo.pop(0)
+            running[bt.add(*sys[k])] = k
+        bt.run(max_iters=30)
+        passes += 1
+        if passes == 2:  # a system still running leaves, and joins again at the end
+            live = [s for s in running if bt.poll(s)[0] == 0]
+            if live:
+                dropped = running.pop(live[0])
+                bt.remove(live[0])
+                todo.append(dropped)
+        for s in bt.finished():
+            k = running.pop(s)
+            got[k] = (bt.result(s), bt.poll(s)[1].iterations)
+            bt.remove(s)
+    bt.close()
+    say("overlap", overlap, "passes", passes, "dropped", dropped)
+    assert dropped is not None and passes > 3
+    for k in range(len(sys)):
+        assert np.array_equal(got[k][0], solo[k][0]), f"system {k}"
+        assert got[k][1] == solo[k][1], f"system {k}"
 
 
 def test_poly_autotune_needs_poly_batch(ctx):
