@@ -480,8 +480,12 @@ fx_status fx_batch_poll(fx_batch *batch, int32_t slot, int32_t *state, fx_solve_
  * collect results without polling every running slot. */
 fx_status fx_batch_finished(fx_batch *batch, int32_t *slots, int32_t cap, int32_t *n);
 
-/* Copies the solutions, x[r*n_dof + i] (a running slot: its current
- * iterate, after the work queued so far). */
+/* Copies the solutions, x[r*n_dof + i].  A finished slot of a
+ * polynomial batch: from the pinned host array the device filled before it
+ * reported the finish (no GPU call; a batch serving fx_solve copies on the
+ * device instead); another finished slot: copied on the context's copy
+ * stream, past any chunks in flight; a running slot: its current iterate,
+ * after the work queued so far. */
 fx_status fx_batch_get(fx_batch *batch, int32_t slot, double *x);
 
 /* Frees the slot for reuse (a running system is dropped). */
