@@ -347,18 +347,26 @@ __device__ __forceinline__ void cheb_sell(const BSys &S, int lb, int j, int go, 
         sell_apply16<RR>(S, a, pv, acc);
         const double c1 = S.cheb[2 * j], c2 = S.cheb[2 * j + 1];
         const bool last = j == S.deg;
-        gptr<const double> zm = gp((const double *)S.zq[(j + 2) % 3]);  // z_{j-1}
-        gptr<const double> rs = gp((const double *)S.rs);
+        // the row's 3 RR entries of z_j, z_{j-1} and rs are contiguous: read
+        // each as one run (wide loads) before the arithmetic
+        double zj[3][RR], zmj[3][RR], rsv[3][RR];
+        gather_block<double, double, RR>(pv, (int)a, zj);
+        gather_block<double, double, RR>(gp((const double *)S.rs), (int)a, rsv);
+        if (j == 1) {  // z_0 = 0 (not stored)
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int r = 0; r < RR; ++r)
+                    zmj[p][r] = 0.0;
+        } else {
+            gather_block<double, double, RR>(gp((const double *)S.zq[(j + 2) % 3]), (int)a, zmj);
+        }
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-            const int64_t d = 3 * a + p;
-            const double sd = last ? jscale(S, d) : 1.0;
+            const double sd = last ? jscale(S, 3 * a + p) : 1.0;
 #pragma unroll
             for (int r = 0; r < RR; ++r) {
-                const int64_t qd = d * RR + r;
-                const double zj = pv[qd];
-                const double zmj = j == 1 ? 0.0 : zm[qd];  // z_0 = 0 (not stored)
-                const double zv = zj + c1 * (zj - zmj) + c2 * (rs[qd] - acc[p][r]);
+                const double zv = zj[p][r] + c1 * (zj[p][r] - zmj[p][r]) + c2 * (rsv[p][r] - acc[p][r]);
                 tile[threadIdx.x * W + p * RR + r] = last ? sd * zv : zv;
             }
         }
