@@ -474,10 +474,22 @@ __global__ __launch_bounds__(kBB) void kp_cheb_lanes(BSys *__restrict__ sys, con
 // node-block values with kOuterG lanes per row, or on the sliced-ELL FP64
 // operator one thread per row; partials per load case (z.s, z.r, p.r, z.w,
 // p.w) with p, s the previous direction and its image -> part1.
-template <int RR>
+// RUNS (the sliced-ELL kernel, every lane a row): the row's 3 RR entries of
+// each vector are read as contiguous runs (wide loads), every load case's,
+// before the running ones are used; else (the node-block kernel, the tail on
+// one lane of kOuterG) entry by entry, running load cases only, which keeps
+// that kernel's registers low.
+template <int RR, bool RUNS>
 __device__ __forceinline__ void outer_row_tail(const BSys &S, int64_t a, int go, gptr<const double> zv,
                                                const double (&acc)[3][RR], double (&v)[5 * kRmax],
                                                double *tile = nullptr) {
+    double zr[3][RR], sor[3][RR], por[3][RR], rrr[3][RR];
+    if constexpr (RUNS) {
+        gather_block<double, double, RR>(zv, (int)a, zr);
+        gather_block<double, double, RR>(gp((const double *)S.ps), (int)a, sor);
+        gather_block<double, double, RR>(gp((const double *)S.pp), (int)a, por);
+        gather_block<double, double, RR>(gp((const double *)S.r), (int)a, rrr);
+    }
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const int64_t d = 3 * a + p;
@@ -486,9 +498,19 @@ __device__ __forceinline__ void outer_row_tail(const BSys &S, int64_t a, int go,
             if (!((go >> r) & 1))
                 continue;
             const int64_t q = d * RR + r;
-            const double z = zv[q], w = acc[p][r];
-            const double so = gp((const double *)S.ps)[q], po = gp((const double *)S.pp)[q];
-            const double rr = gp((const double *)S.r)[q];
+            double z, so, po, rr;
+            if constexpr (RUNS) {
+                z = zr[p][r];
+                so = sor[p][r];
+                po = por[p][r];
+                rr = rrr[p][r];
+            } else {
+                z = zv[q];
+                so = gp((const double *)S.ps)[q];
+                po = gp((const double *)S.pp)[q];
+                rr = gp((const double *)S.r)[q];
+            }
+            const double w = acc[p][r];
             if (tile)
                 tile[threadIdx.x * 3 * RR + p * RR + r] = w;
             else
@@ -534,7 +556,7 @@ __device__ __forceinline__ void outer_row(const BSys &S, int64_t a, int lane, in
                 acc[p][r] += __shfl_xor(acc[p][r], off, G);
     if (lane != 0 || a >= S.n_nodes)
         return;
-    outer_row_tail<RR>(S, a, go, zv, acc, v);
+    outer_row_tail<RR, false>(S, a, go, zv, acc, v);
 }
 
 template <int RR, bool SELL>
@@ -550,7 +572,7 @@ __device__ __forceinline__ void outer_block(const BSys &S, int lb, int go, doubl
             double acc[3][RR];
             zero_acc<RR>(acc);
             sell_node_rows<double, double, RR>(gp(S.sl_ptr), gp(S.scol), gp(S.sval64), a, zv, acc);
-            outer_row_tail<RR>(S, a, go, zv, acc, v, tile);
+            outer_row_tail<RR, true>(S, a, go, zv, acc, v, tile);
         }
         __syncthreads();
         stage_out<3 * RR>(wave_rows(S, lb), tile, gp(S.pw));
@@ -622,23 +644,39 @@ __global__ __launch_bounds__(kBB) void kp_coef(BSys *__restrict__ sys, const int
 // p = z - beta p_old, s = w - beta s_old, x += alpha p, r -= alpha s, and
 // the next inner start (rs = S r, z_1 = rs / theta); partials (r.r,
 // r.(D^-1 r), 0) -> part2
+// RR consecutive doubles from v + i RR
+template <int RR>
+__device__ __forceinline__ void load_run(gptr<const double> v, int64_t i, double (&o)[RR]) {
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+        o[r] = v[i * RR + r];
+}
+
 template <int RR>
 __device__ __forceinline__ void upd_tile(const BSys &S, int lb, int go, double (&v)[3 * kRmax]) {
     const int64_t i = (int64_t)lb * kBB + threadIdx.x;
     if (i >= S.n_dof)
         return;
     const double inv = gp(S.inv)[i], si = jscale(S, i), ith = S.cheb[0];
-    gptr<const double> zv = gp((const double *)S.zq[(S.deg + 1) % 3]);
+    // the DOF's RR entries of each vector are contiguous: read as runs (wide
+    // loads), every load case's, then update the running ones
+    double zr[RR], ppr[RR], pwr[RR], psr[RR], xr[RR], rr0[RR];
+    load_run<RR>(gp((const double *)S.zq[(S.deg + 1) % 3]), i, zr);
+    load_run<RR>(gp((const double *)S.pp), i, ppr);
+    load_run<RR>(gp((const double *)S.pw), i, pwr);
+    load_run<RR>(gp((const double *)S.ps), i, psr);
+    load_run<RR>(gp((const double *)S.x), i, xr);
+    load_run<RR>(gp((const double *)S.r), i, rr0);
 #pragma unroll
     for (int r = 0; r < RR; ++r) {
         if (!((go >> r) & 1))
             continue;
         const double al = S.rhs[r].alpha, be = S.rhs[r].beta;
         const int64_t q = i * RR + r;
-        const double pn = zv[q] - be * gp((const double *)S.pp)[q];
-        const double sn = gp((const double *)S.pw)[q] - be * gp((const double *)S.ps)[q];
-        const double xn = gp((const double *)S.x)[q] + al * pn;
-        const double rn = gp((const double *)S.r)[q] - al * sn;
+        const double pn = zr[r] - be * ppr[r];
+        const double sn = pwr[r] - be * psr[r];
+        const double xn = xr[r] + al * pn;
+        const double rn = rr0[r] - al * sn;
         const double rsn = si * rn;
         gp(S.pp)[q] = pn;
         gp(S.ps)[q] = sn;
