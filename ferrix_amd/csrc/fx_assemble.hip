@@ -21,6 +21,7 @@
 // dropped hold 0.0 and keep = 0, so the exported CSR matches the filtered
 // one while SpMV sums are unchanged (0 * x adds nothing).
 #include "fx_reduce.h"
+#include "fx_sell.h"
 
 namespace fx {
 namespace {
@@ -263,7 +264,7 @@ __global__ void k_sell_convert(int64_t n_nodes, const int32_t *__restrict__ nb_p
                 VT v = (VT)0;
                 if (k < deg)
                     v = (VT)src[9 * b0 + (int64_t)p * 3 * deg + 3 * k + q];
-                dst[(kk * 9 + 3 * p + q) * kSliceRows + lane] = v;
+                dst[sell_index<VT>(kk, 3 * p + q, lane)] = v;
             }
     }
 }

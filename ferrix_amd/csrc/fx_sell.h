@@ -1,12 +1,15 @@
 // fx_sell.h -- sliced-ELL node-block operator for meshes with short node rows.
 //
 // Node rows are grouped in slices of 64 (one wavefront).  Within a slice the
-// 3x3 blocks of block column k of all 64 rows are stored together, entry
-// e = 3p+q of the block at sval[((kk*9 + e) * 64) + lane] with kk =
-// sl_ptr[slice] + k, and the block's column node at scol[kk*64 + lane].  One
-// thread owns one node row: every load of a block column is one coalesced
-// 64-lane access, and the gathered operand entries of a block are 3
-// consecutive DOFs.  Rows shorter than their slice's width are padded with
+// 3x3 blocks of block column k of all 64 rows are stored together (kk =
+// sl_ptr[slice] + k), 576 values per block column, and the block's column
+// node at scol[kk*64 + lane].  FP32 values: entry e = 3p+q of the block at
+// sval[(kk*9 + e) * 64 + lane].  FP64 values: entries 0..7 as four 16-byte
+// pairs per lane, pair m at sval[kk*576 + m*128 + lane*2], entry 8 at
+// sval[kk*576 + 512 + lane] (sell_index): five loads per block instead of
+// nine.  One thread owns one node row: every load of a block column is one
+// coalesced 64-lane access, and the gathered operand entries of a block are
+// 3 consecutive DOFs.  Rows shorter than their slice's width are padded with
 // zero blocks whose column is the row's own node (node 0 for the lanes past
 // the last row of the mesh).
 //
@@ -63,6 +66,36 @@ __device__ __forceinline__ void madd_block(const VT (&v)[9], const double (&x)[3
         }
 }
 
+// Position of entry e of block column kk for row `lane` of its slice.
+template <typename VT>
+__host__ __device__ __forceinline__ size_t sell_index(size_t kk, int e, int lane) {
+    if constexpr (sizeof(VT) == 8)
+        return e < 8 ? kk * 576 + (size_t)(e / 2) * 128 + (size_t)lane * 2 + (e & 1) : kk * 576 + 512 + lane;
+    else
+        return (kk * 9 + e) * kSlice + lane;
+}
+
+typedef double fx_d2 __attribute__((ext_vector_type(2)));
+
+// The nine values of block column kk of row `lane`.
+template <typename VT>
+__device__ __forceinline__ void load_sell_block(gptr<const VT> sval, size_t kk, int lane, VT (&v)[9]) {
+    if constexpr (sizeof(VT) == 8) {
+        gptr<const fx_d2> rec = (gptr<const fx_d2>)(sval + kk * 576);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const fx_d2 w = rec[m * kSlice + lane];
+            v[2 * m] = w.x;
+            v[2 * m + 1] = w.y;
+        }
+        v[8] = sval[kk * 576 + 512 + lane];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            v[e] = sval[(kk * 9 + e) * kSlice + lane];
+    }
+}
+
 template <typename VT, typename XT, int RR>
 __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> sl_ptr, gptr<const int32_t> scol,
                                                gptr<const VT> sval, int64_t a, gptr<const XT> pv,
@@ -80,14 +113,10 @@ __device__ __forceinline__ void sell_node_rows(gptr<const int32_t> sl_ptr, gptr<
         double x0[3][RR], x1[3][RR];
         VT v0[9], v1[9];
         gather_block<VT, XT, RR>(pv, c0, x0);
-#pragma unroll
-        for (int e = 0; e < 9; ++e)
-            v0[e] = sval[((size_t)k * 9 + e) * kSlice + lane];
+        load_sell_block<VT>(sval, (size_t)k, lane, v0);
         if (two) {
             gather_block<VT, XT, RR>(pv, c1, x1);
-#pragma unroll
-            for (int e = 0; e < 9; ++e)
-                v1[e] = sval[((size_t)(k + 1) * 9 + e) * kSlice + lane];
+            load_sell_block<VT>(sval, (size_t)k + 1, lane, v1);
         }
         if (k + 2 < kw)
             cn0 = scol[(size_t)(k + 2) * kSlice + lane];
